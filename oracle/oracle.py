@@ -1,0 +1,333 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of the CPU FP64 oracle (libwavecore_oracle.so).
+
+``OracleSolver`` has the same methods as the product's ``wavesim_amd.Solver`` (and the
+reference's ``Solver``), so the tests can run one scenario through both and compare. Only
+tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of bench.py import this module.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from wavesim_amd.config import (
+    CDispersion,
+    CDispersionStats,
+    CLimiting,
+    CLinearWave,
+    CSponge,
+    CWaveMaker,
+    Dispersion,
+    DispersionStats,
+    Limiting,
+    LinearWave,
+    Order,
+    Sponge,
+    WaveMaker,
+    c_dispersion,
+    c_sponge,
+    c_wavemaker,
+)
+from wavesim_amd.grid import Bathymetry, Grid, State
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwavecore_oracle.so")
+
+DOT_REFERENCE = 0
+DOT_CANONICAL = 1
+MATH_GLIBC = 0
+MATH_PORTABLE = 1
+
+_D = ctypes.c_double
+_P = ctypes.POINTER(ctypes.c_double)
+_U64 = ctypes.c_uint64
+_I32 = ctypes.c_int32
+_VP = ctypes.c_void_p
+
+
+def build() -> None:
+    """Compile the oracle with its Makefile (g++, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        sig = {
+            "wco_last_error": (ctypes.c_char_p, []),
+            "wco_solver_new": (_I32, [_U64, _U64, _D, _D, _P, ctypes.POINTER(_VP)]),
+            "wco_solver_free": (None, [_VP]),
+            "wco_set_cfl": (_I32, [_VP, _D]),
+            "wco_with_order": (_I32, [_VP, _I32]),
+            "wco_with_manning": (_I32, [_VP, _D]),
+            "wco_with_sponge": (_I32, [_VP, ctypes.POINTER(CSponge)]),
+            "wco_with_wavemaker": (_I32, [_VP, ctypes.POINTER(CWaveMaker)]),
+            "wco_with_dispersion": (_I32, [_VP, ctypes.POINTER(CDispersion)]),
+            "wco_with_periodic_y": (_I32, [_VP]),
+            "wco_with_shallow_cells": (_I32, [_VP, ctypes.POINTER(ctypes.c_uint8), _U64]),
+            "wco_bed": (_I32, [_VP, _P]),
+            "wco_set_dot_order": (_I32, [_VP, _I32]),
+            "wco_set_math": (_I32, [_VP, _I32]),
+            "wco_set_threads": (_I32, [_VP, _I32]),
+            "wco_state_set": (_I32, [_VP, _P, _P, _P, _P, _D]),
+            "wco_state_get": (_I32, [_VP, _P, _P, _P, _P, _P]),
+            "wco_stable_dt": (_I32, [_VP, _P]),
+            "wco_step": (_I32, [_VP, _D]),
+            "wco_limiting_cell": (_I32, [_VP, ctypes.POINTER(_I32), ctypes.POINTER(CLimiting)]),
+            "wco_breaking_indicator": (_I32, [_VP, _P]),
+            "wco_dispersion_stats": (_I32, [_VP, ctypes.POINTER(_I32),
+                                            ctypes.POINTER(CDispersionStats)]),
+            "wco_state_volume": (_I32, [_VP, _P]),
+            "wco_state_max_abs_momentum": (_I32, [_VP, _P]),
+            "wco_state_min_depth": (_I32, [_VP, _P]),
+            "wco_fill_boundaries": (_I32, [_VP]),
+            "wco_linear_wave": (_I32, [_D, _D, _I32, ctypes.POINTER(CLinearWave)]),
+            "wco_manning_factor": (_D, [_D, _D, _D, _D]),
+            "wco_face_values": (None, [_P, _D, _P, _P]),
+            "wco_smoothstep": (_D, [_D, _D, _D]),
+            "wco_hypot": (_D, [_D, _D, _I32]),
+            "wco_pow43": (_D, [_D, _I32]),
+            "wco_matvec": (_I32, [_U64, _U64, _D, _D, _I32, _P, _P, _P, _P, _P, _P, _P]),
+            "wco_dot": (_D, [_U64, _U64, _I32, _P, _P, _P, _P]),
+            "wco_dispersion_apply": (_I32, [_U64, _U64, _D, _D, _I32,
+                                            ctypes.POINTER(CDispersion), _P, _P, _P, _P, _P, _P,
+                                            _P, _I32]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_P)
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise ValueError(lib().wco_last_error().decode())
+
+
+class OracleSolver:
+    """The oracle behind the reference's ``Solver`` API (solver.rs:260-429)."""
+
+    def __init__(self, grid: Grid, bed: Bathymetry, *, dot_order: int = DOT_REFERENCE,
+                 math: int = MATH_GLIBC, threads: int = 1):
+        self.grid = grid
+        b = np.ascontiguousarray(bed.b, dtype=np.float64)
+        if b.size != grid.cells():
+            raise ValueError("bed must be a padded array of grid.cells() values")
+        h = _VP()
+        _check(lib().wco_solver_new(grid.nx, grid.ny, grid.dx, grid.dy, ptr(b), ctypes.byref(h)))
+        self._h = h
+        self._cfl = 0.4
+        self.order = Order.Second
+        self.manning = 0.0
+        self.periodic_y = False
+        self._dispersion = None
+        _check(lib().wco_set_dot_order(h, dot_order))
+        _check(lib().wco_set_math(h, math))
+        _check(lib().wco_set_threads(h, threads))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().wco_solver_free(self._h)
+            self._h = None
+
+    # -- builder -----------------------------------------------------------------------
+    @property
+    def cfl(self) -> float:
+        return self._cfl
+
+    @cfl.setter
+    def cfl(self, v: float) -> None:
+        self._cfl = float(v)
+        _check(lib().wco_set_cfl(self._h, self._cfl))
+
+    def with_order(self, order: Order) -> "OracleSolver":
+        _check(lib().wco_with_order(self._h, int(order)))
+        self.order = Order(order)
+        return self
+
+    def with_manning(self, n: float) -> "OracleSolver":
+        _check(lib().wco_with_manning(self._h, float(n)))
+        self.manning = float(n)
+        return self
+
+    def with_sponge(self, sponge: Sponge) -> "OracleSolver":
+        c = c_sponge(sponge)
+        _check(lib().wco_with_sponge(self._h, ctypes.byref(c)))
+        return self
+
+    def with_wavemaker(self, maker: WaveMaker) -> "OracleSolver":
+        c = c_wavemaker(maker)
+        _check(lib().wco_with_wavemaker(self._h, ctypes.byref(c)))
+        return self
+
+    def with_dispersion(self, d: Dispersion) -> "OracleSolver":
+        c = c_dispersion(d)
+        _check(lib().wco_with_dispersion(self._h, ctypes.byref(c)))
+        self._dispersion = d
+        return self
+
+    def with_periodic_y(self) -> "OracleSolver":
+        _check(lib().wco_with_periodic_y(self._h))
+        self.periodic_y = True
+        return self
+
+    def with_shallow_cells(self, cells) -> "OracleSolver":
+        a = np.ascontiguousarray(np.asarray(cells, dtype=bool).astype(np.uint8).ravel())
+        _check(lib().wco_with_shallow_cells(
+            self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), a.size))
+        return self
+
+    def dispersion(self):
+        return self._dispersion
+
+    @property
+    def bed(self) -> Bathymetry:
+        out = np.empty(self.grid.cells())
+        _check(lib().wco_bed(self._h, ptr(out)))
+        return Bathymetry(out)
+
+    # -- state traffic -------------------------------------------------------------------
+    def _upload(self, s: State) -> None:
+        h, hu, hv, br = s.arrays()
+        _check(lib().wco_state_set(self._h, ptr(np.ascontiguousarray(h)),
+                                   ptr(np.ascontiguousarray(hu)), ptr(np.ascontiguousarray(hv)),
+                                   ptr(np.ascontiguousarray(br)), s.time))
+
+    def _download_into(self, s: State) -> None:
+        n = self.grid.cells()
+        out = [np.empty(n) for _ in range(4)]
+        t = ctypes.c_double()
+        _check(lib().wco_state_get(self._h, *(ptr(a) for a in out), ctypes.byref(t)))
+        s._h, s._hu, s._hv, s._breaking = out
+        s._host_valid = True
+        s._dev_valid = False
+
+    # -- the API ---------------------------------------------------------------------------
+    def stable_dt(self, s: State) -> float:
+        self._upload(s)
+        out = ctypes.c_double()
+        _check(lib().wco_stable_dt(self._h, ctypes.byref(out)))
+        return out.value
+
+    def step(self, s: State, dt: float) -> None:
+        self._upload(s)
+        _check(lib().wco_step(self._h, float(dt)))
+        t = s.time
+        self._download_into(s)
+        s.time = t + float(dt)
+
+    def limiting_cell(self, s: State):
+        self._upload(s)
+        found = ctypes.c_int32()
+        out = CLimiting()
+        _check(lib().wco_limiting_cell(self._h, ctypes.byref(found), ctypes.byref(out)))
+        if not found.value:
+            return None
+        return Limiting(int(out.i), int(out.j), out.depth, out.speed)
+
+    def breaking_indicator(self, s: State) -> np.ndarray:
+        self._upload(s)
+        out = np.empty(self.grid.nx * self.grid.ny)
+        _check(lib().wco_breaking_indicator(self._h, ptr(out)))
+        return out
+
+    def dispersion_stats(self):
+        has = ctypes.c_int32()
+        out = CDispersionStats()
+        _check(lib().wco_dispersion_stats(self._h, ctypes.byref(has), ctypes.byref(out)))
+        if not has.value:
+            return None
+        return DispersionStats(int(out.solves), int(out.iterations), int(out.unconverged))
+
+    def volume(self, s: State) -> float:
+        """State::volume in this oracle's summation order."""
+        self._upload(s)
+        out = ctypes.c_double()
+        _check(lib().wco_state_volume(self._h, ctypes.byref(out)))
+        return out.value
+
+    def step_n(self, s: State, steps: int) -> None:
+        """``steps`` iterations of ``dt = stable_dt; step(dt)`` without host round trips."""
+        self._upload(s)
+        dt = ctypes.c_double()
+        t = s.time
+        for _ in range(steps):
+            _check(lib().wco_stable_dt(self._h, ctypes.byref(dt)))
+            if not (np.isfinite(dt.value) and dt.value > 0.0):
+                raise FloatingPointError(f"no valid time step: {dt.value}")
+            _check(lib().wco_step(self._h, dt.value))
+            t = t + dt.value
+        self._download_into(s)
+        s.time = t
+
+
+def linear_wave(omega: float, h: float, dispersive: bool) -> LinearWave:
+    out = CLinearWave()
+    _check(lib().wco_linear_wave(omega, h, int(dispersive), ctypes.byref(out)))
+    return LinearWave(out.k, out.phase_speed, out.group_speed)
+
+
+def manning_factor(h: float, speed: float, dt: float, n: float) -> float:
+    return lib().wco_manning_factor(h, speed, dt, n)
+
+
+def face_values(v, blend: float) -> tuple[float, float]:
+    a = np.ascontiguousarray(v, dtype=np.float64)
+    lo, hi = ctypes.c_double(), ctypes.c_double()
+    lib().wco_face_values(ptr(a), blend, ctypes.byref(lo), ctypes.byref(hi))
+    return lo.value, hi.value
+
+
+def smoothstep(lo: float, hi: float, x: float) -> float:
+    return lib().wco_smoothstep(lo, hi, x)
+
+
+def hypot(a: float, b: float, portable: bool) -> float:
+    return lib().wco_hypot(a, b, int(portable))
+
+
+def pow43(h: float, portable: bool) -> float:
+    return lib().wco_pow43(h, int(portable))
+
+
+def matvec(grid: Grid, periodic: bool, heff, mask, c, px, py):
+    n = grid.cells()
+    ax, ay = np.zeros(n), np.zeros(n)
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in (heff, mask, c, px, py)]
+    _check(lib().wco_matvec(grid.nx, grid.ny, grid.dx, grid.dy, int(periodic),
+                            *(ptr(a) for a in args), ptr(ax), ptr(ay)))
+    return ax, ay
+
+
+def dot(grid: Grid, ax, ay, bx, by, order: int = DOT_REFERENCE) -> float:
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in (ax, ay, bx, by)]
+    return lib().wco_dot(grid.nx, grid.ny, order, *(ptr(a) for a in args))
+
+
+def dispersion_apply(grid: Grid, periodic: bool, d: Dispersion, h, hu, hv, phi, lh, lhu, lhv,
+                     dot_order: int = DOT_REFERENCE):
+    """Dispersion::apply with fresh scratch; returns the new (lhu, lhv)."""
+    c = c_dispersion(d)
+    ins = [np.ascontiguousarray(a, dtype=np.float64) for a in (h, hu, hv, phi, lh)]
+    lhu = np.array(lhu, dtype=np.float64)
+    lhv = np.array(lhv, dtype=np.float64)
+    _check(lib().wco_dispersion_apply(grid.nx, grid.ny, grid.dx, grid.dy, int(periodic),
+                                      ctypes.byref(c), *(ptr(a) for a in ins), ptr(lhu),
+                                      ptr(lhv), dot_order))
+    return lhu, lhv

@@ -1,0 +1,430 @@
+"""The reference's known-answer tests, as checks over any solver with wavecore's API.
+
+Each ``check_*`` restates one test of crates/wavecore/tests/ (or an in-source unit test) with
+the same scenario and the same bounds, fixed by the reference before its first run. ``make``
+builds a solver from ``(grid, bed)``: the CPU oracle in test_oracle_known_answers.py, the GPU
+path in test_gpu_known_answers.py.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from helpers import Lcg, col, run, run_and_analyse, run_to
+from wavesim_amd.config import Dispersion, Order, Sponge, WaveMaker, linear_wave
+from wavesim_amd.grid import GHOST, G, Bathymetry, Grid, State
+
+ORDERS = (Order.First, Order.Second, Order.Fifth)
+
+
+# ---- well_balanced.rs -----------------------------------------------------------------------
+def check_lake_at_rest_over_rough_wet_bed(make, orders=ORDERS):
+    """well_balanced.rs:16-35."""
+    for order in orders:
+        grid = Grid(64, 48, 3.0, 3.0)
+        bed = Bathymetry.rough(grid, 8.0, 3.0)
+        s = State.lake_at_rest(grid, bed, 0.0)
+        solver = make(grid, bed).with_order(order)
+        v0 = s.volume(grid)
+        run(solver, s, 500)
+        assert s.max_abs_momentum(grid) < 1e-11, (order, s.max_abs_momentum(grid))
+        assert abs(s.volume(grid) - v0) < 1e-9 * v0, order
+        assert s.min_depth(grid) >= 0.0, order
+
+
+def check_lake_at_rest_with_dry_bumps(make, orders=ORDERS):
+    """well_balanced.rs:37-62."""
+    for order in orders:
+        grid = Grid(64, 48, 3.0, 3.0)
+        bed = Bathymetry.rough(grid, 0.5, 2.5)
+        s = State.lake_at_rest(grid, bed, 0.0)
+        solver = make(grid, bed).with_order(order)
+        v0 = s.volume(grid)
+        dry = int(np.count_nonzero(grid.inner(s.arrays()[0]) == 0.0))
+        assert dry > 0, "test bed has no dry cells"
+        run(solver, s, 500)
+        assert s.max_abs_momentum(grid) < 1e-11, (order, s.max_abs_momentum(grid))
+        assert abs(s.volume(grid) - v0) < 1e-9 * v0, order
+        assert s.min_depth(grid) >= 0.0, order
+
+
+def check_dam_break_conserves_volume_and_stays_positive(make, orders=ORDERS):
+    """well_balanced.rs:65-84."""
+    for order in orders:
+        grid = Grid(100, 4, 1.0, 1.0)
+        bed = Bathymetry.flat(grid, 0.0)
+        s = State.zeros(grid)
+        h = grid.inner(s.h)
+        h[:, :50] = 2.0
+        solver = make(grid, bed).with_order(order)
+        v0 = s.volume(grid)
+        run(solver, s, 300)
+        assert abs(s.volume(grid) - v0) < 1e-9 * v0, order
+        assert s.min_depth(grid) >= 0.0, order
+        assert s.arrays()[0][grid.idx(GHOST + 70, GHOST + 2)] > 0.0, order
+
+
+def check_walls_and_periodic_seam_keep_volume(make, cases=None):
+    """well_balanced.rs:87-121: Fifth with and without SGN, walls and a periodic seam."""
+    grid = Grid(40, 30, 1.0, 1.0)
+    bed = Bathymetry.flat(grid, 2.0)
+    cases = cases or [(p, d) for p in (False, True) for d in (False, True)]
+    for periodic, dispersive in cases:
+        solver = make(grid, bed.copy()).with_order(Order.Fifth)
+        if dispersive:
+            solver = solver.with_dispersion(Dispersion())
+        if periodic:
+            solver = solver.with_periodic_y()
+        s = State.lake_at_rest(grid, bed, 0.0)
+        cy = 0.5 if periodic else 2.5
+        x, y = grid.centres()
+        X, Y = np.meshgrid(x[GHOST:GHOST + grid.nx], y[GHOST:GHOST + grid.ny])
+        r2 = (X - 20.0) ** 2 + (Y - cy) ** 2
+        grid.inner(s.h)[:, :] += 1.3 * np.exp(-r2 / 32.0)
+        v0 = s.volume(grid)
+        run(solver, s, 300)
+        change = (s.volume(grid) - v0) / v0
+        assert abs(change) < 1e-12, (periodic, dispersive, change)
+
+
+# ---- time_step.rs ---------------------------------------------------------------------------
+def _lake(make):
+    grid = Grid(20, 10, 2.0, 2.0)
+    bed = Bathymetry.flat(grid, 2.0)
+    return make(grid, bed), State.lake_at_rest(grid, bed, 0.0), grid
+
+
+def check_nan_cell_is_seen_not_skipped(make):
+    """time_step.rs:12-31."""
+    solver, s, g = _lake(make)
+    still = solver.stable_dt(s)
+    s.hu[g.idx(GHOST + 7, GHOST + 3)] = math.nan
+    assert math.isnan(solver.stable_dt(s))
+    c = solver.limiting_cell(s)
+    assert (c.i, c.j) == (7, 3)
+    assert math.isnan(c.speed) and c.depth == 2.0
+    _, s2, _ = _lake(make)
+    s2.h[:] = math.nan
+    assert math.isnan(solver.stable_dt(s2))
+    assert math.isfinite(still) and still > 0.0
+
+
+def check_limiting_cell_is_fastest_water(make):
+    """time_step.rs:33-50."""
+    solver, s, g = _lake(make)
+    k = g.idx(GHOST + 12, GHOST + 4)
+    s.h[k] = 1e-3
+    s.hu[k] = 0.5
+    c = solver.limiting_cell(s)
+    assert (c.i, c.j) == (12, 4)
+    assert abs(c.speed - 500.0) < 1e-9 and c.depth == 1e-3
+    film = (500.0 + math.sqrt(9.81e-3)) / 2.0 + math.sqrt(9.81e-3) / 2.0
+    assert abs(solver.stable_dt(s) - solver.cfl / film) < 1e-9 * solver.cfl / film
+    s.hu[k] = math.inf
+    assert solver.stable_dt(s) == 0.0
+
+
+def check_still_water_limited_by_depth_dry_land_by_nothing(make):
+    """time_step.rs:52-62."""
+    solver, s, _ = _lake(make)
+    c = solver.limiting_cell(s)
+    assert (c.depth, c.speed) == (2.0, 0.0)
+    grid = Grid(6, 4, 2.0, 2.0)
+    bed = Bathymetry.flat(grid, -1.0)
+    dry = State.lake_at_rest(grid, bed, 0.0)
+    solver = make(grid, bed)
+    assert solver.limiting_cell(dry) is None
+    assert solver.stable_dt(dry) == math.inf
+
+
+# ---- dispersion tests (tests/dispersion.rs) -------------------------------------------------
+def sgn_omega(h, k):
+    return math.sqrt(G * h * k * k / (1.0 + (k * h) ** 2 / 3.0))
+
+
+def standing_wave_omega(make, dispersive: bool, order: Order) -> float:
+    """tests/dispersion.rs:54-89."""
+    depth, length, mode, amplitude = 5.0, 100.0, 8.0, 0.01
+    dx = 0.5
+    grid = Grid(200, 4, dx, dx)
+    solver = make(grid, Bathymetry.flat(grid, depth)).with_order(order)
+    if dispersive:
+        solver = solver.with_dispersion(Dispersion())
+    k = mode * math.pi / length
+    s = State.zeros(grid)
+    x, _ = grid.centres()
+    grid.inner(s.h)[:, :] = depth + amplitude * np.cos(k * x[GHOST:GHOST + grid.nx])[None, :]
+    probe = grid.idx(GHOST, GHOST + 2)
+    crossings = []
+    prev = (0.0, amplitude * math.cos(k * 0.25))
+    while s.time < 14.0:
+        dt = solver.stable_dt(s)
+        solver.step(s, dt)
+        eta = s.arrays()[0][probe] - depth
+        if prev[1] * eta < 0.0:
+            f = prev[1] / (prev[1] - eta)
+            crossings.append(prev[0] + f * (s.time - prev[0]))
+        prev = (s.time, eta)
+    assert len(crossings) >= 5, len(crossings)
+    return (len(crossings) - 1) * math.pi / (crossings[-1] - crossings[0])
+
+
+def check_standing_wave_sgn_frequency(make, orders=(Order.Second, Order.Fifth)):
+    """tests/dispersion.rs:91-117; returns the measured frequencies."""
+    depth, k = 5.0, 8.0 * math.pi / 100.0
+    sgn = sgn_omega(depth, k)
+    shallow = math.sqrt(G * depth) * k
+    exact = math.sqrt(G * k * math.tanh(k * depth))
+    measured_shallow = standing_wave_omega(make, False, Order.Second)
+    assert abs(measured_shallow / shallow - 1.0) < 0.01, (measured_shallow, shallow)
+    out = {"shallow": measured_shallow}
+    for order in orders:
+        m = standing_wave_omega(make, True, order)
+        out[order.name] = m
+        assert abs(m / sgn - 1.0) < 0.01, (order, m, sgn)
+        assert abs(m / exact - 1.0) < 0.03
+    assert abs(shallow / exact - 1.0) > 0.15
+    return out
+
+
+def check_linear_wave_inverts_dispersion_relation(lw=linear_wave):
+    """tests/dispersion.rs:119-152."""
+    for h, period in [(2.0, 10.0), (5.0, 8.0), (8.0, 14.0), (0.5, 4.0)]:
+        omega = 2.0 * math.pi / period
+        w = lw(omega, h, True)
+        assert abs(sgn_omega(h, w.k) / omega - 1.0) < 1e-12
+        assert abs(w.phase_speed - omega / w.k) < 1e-12
+        eps = 1e-6 * w.k
+        numerical = (sgn_omega(h, w.k + eps) - sgn_omega(h, w.k - eps)) / (2.0 * eps)
+        assert abs(w.group_speed / numerical - 1.0) < 1e-7
+        assert w.group_speed < w.phase_speed
+        sh = lw(omega, h, False)
+        c0 = math.sqrt(G * h)
+        assert abs(sh.phase_speed - c0) < 1e-12
+        assert abs(sh.group_speed - c0) < 1e-12
+        assert abs(sh.k * c0 - omega) < 1e-12
+        assert w.phase_speed <= c0
+
+
+def check_swell_crosses_shelf_on_coarse_cells(make):
+    """tests/dispersion.rs:456-492; returns the fraction of height kept."""
+    dx, depth, period, amplitude = 6.0, 12.0, 14.0, 0.1
+    grid = Grid(310, 4, dx, dx)
+    bed = Bathymetry.flat(grid, depth)
+    maker = WaveMaker(260.0, amplitude, period, 0.0, 2.0 * dx, 2.0)
+    solver = (make(grid, bed).with_order(Order.Fifth).with_dispersion(Dispersion())
+              .with_wavemaker(maker)
+              .with_sponge(Sponge(37, 1.5).edges(True, True, False, False)))
+    s = State.lake_at_rest(grid, solver.bed, 0.0)
+    j = GHOST + 2
+    probes = [(col(grid, 460.0), j), (col(grid, 1460.0), j)]
+    omega = 2.0 * math.pi / period
+    window = (160.0, 160.0 + 4.0 * period)
+    out = run_and_analyse(solver, s, window[1], omega, window, probes, grid, solver.bed.b)
+    kept = out[1][0] / out[0][0]
+    assert kept > 0.97, kept
+    return kept
+
+
+def check_still_water_with_dispersion(make):
+    """tests/dispersion.rs:629-652."""
+    for (mean_depth, amplitude) in [(8.0, 3.0), (0.5, 2.5)]:
+        for order in (Order.Second, Order.Fifth):
+            grid = Grid(64, 48, 3.0, 3.0)
+            bed = Bathymetry.rough(grid, mean_depth, amplitude)
+            s = State.lake_at_rest(grid, bed, 0.0)
+            solver = make(grid, bed).with_order(order).with_dispersion(Dispersion())
+            v0 = s.volume(grid)
+            run_to(solver, s, 30.0)
+            assert s.max_abs_momentum(grid) < 1e-9, (mean_depth, order, s.max_abs_momentum(grid))
+            assert abs(s.volume(grid) - v0) < 1e-9 * v0
+            assert s.min_depth(grid) >= 0.0
+
+
+# ---- stoker.rs --------------------------------------------------------------------------------
+class StokerExact:
+    """Exact 1-D dam break (stoker.rs:11-71)."""
+
+    def __init__(self, h_l, h_r):
+        c_l = math.sqrt(G * h_l)
+        self.h_l, self.h_r = h_l, h_r
+        if h_r <= 0.0:
+            self.h_m, self.u_m, self.shock = 0.0, 0.0, 2.0 * c_l
+            return
+
+        def f(h_m):
+            return 2.0 * (c_l - math.sqrt(G * h_m)) - (h_m - h_r) * math.sqrt(
+                0.5 * G * (h_m + h_r) / (h_m * h_r))
+
+        lo, hi = h_r, h_l
+        for _ in range(200):
+            mid = 0.5 * (lo + hi)
+            if f(mid) > 0.0:
+                lo = mid
+            else:
+                hi = mid
+        self.h_m = 0.5 * (lo + hi)
+        self.u_m = 2.0 * (c_l - math.sqrt(G * self.h_m))
+        self.shock = self.h_m * self.u_m / (self.h_m - h_r)
+
+    def depth(self, xi: np.ndarray) -> np.ndarray:
+        c_l = math.sqrt(G * self.h_l)
+        raref = (2.0 * c_l - xi) ** 2 / (9.0 * G)
+        if self.h_r <= 0.0:
+            return np.where(xi < -c_l, self.h_l, np.where(xi < 2.0 * c_l, raref, 0.0))
+        return np.where(
+            xi < -c_l, self.h_l,
+            np.where(xi <= self.u_m - math.sqrt(G * self.h_m), raref,
+                     np.where(xi < self.shock, self.h_m, self.h_r)))
+
+
+def dam_break_error(make, nx, order, h_l, h_r, t_end):
+    """stoker.rs:73-100."""
+    length, dam = 100.0, 50.0
+    dx = length / nx
+    grid = Grid(nx, 4, dx, dx)
+    bed = Bathymetry.flat(grid, 0.0)
+    s = State.zeros(grid)
+    x, _ = grid.centres()
+    xi = x[GHOST:GHOST + nx]
+    grid.inner(s.h)[:, :] = np.where(xi < dam, h_l, h_r)[None, :]
+    solver = make(grid, bed).with_order(order)
+    t = 0.0
+    while t < t_end:
+        dt = min(solver.stable_dt(s), t_end - t)
+        solver.step(s, dt)
+        t += dt
+    exact = StokerExact(h_l, h_r)
+    row = grid.view(s.arrays()[0])[GHOST + 2, GHOST:GHOST + nx]
+    return float(np.sum(np.abs(row - exact.depth((xi - dam) / t_end)) * dx))
+
+
+def check_dam_break(make, h_l, h_r):
+    """stoker.rs:114-141."""
+    t = 4.0
+    first = dam_break_error(make, 400, Order.First, h_l, h_r, t)
+    rates = {}
+    for order in (Order.Second, Order.Fifth):
+        coarse = dam_break_error(make, 200, order, h_l, h_r, t)
+        fine = dam_break_error(make, 400, order, h_l, h_r, t)
+        rate = math.log2(coarse / fine)
+        assert fine < 0.25 * first, (order, fine, first)
+        assert rate > 0.9, (order, rate)
+        rates[order.name] = rate
+    return rates
+
+
+# ---- waves.rs (fast ones) -----------------------------------------------------------------------
+def check_sponge_over_land_never_speeds_water_up(make):
+    """waves.rs:299-337."""
+    grid = Grid(20, 20, 2.0, 2.0)
+    bed = Bathymetry.flat(grid, -0.5)
+    s = State.zeros(grid)
+    grid.inner(s.h)[:, :] = 1.0
+    grid.inner(s.hu)[:, :] = 1.0
+    grid.inner(s.hv)[:, :] = 0.5
+    solver = make(grid, bed).with_sponge(Sponge(1_000_000, 1.5).edges(True, False, False, False))
+    for _ in range(3):
+        solver.step(s, 0.05)
+    h, hu, hv, _ = s.arrays()
+    for i in range(GHOST + 7, GHOST + 13):
+        for j in range(GHOST + 7, GHOST + 13):
+            k = grid.idx(i, j)
+            assert h[k] < 1.0
+            u, v = hu[k] / h[k], hv[k] / h[k]
+            assert abs(u - 1.0) < 1e-6 and abs(v - 0.5) < 1e-6, (i, j, u, v)
+
+
+# ---- shallow_cells.rs ---------------------------------------------------------------------------
+def _hump():
+    grid = Grid(120, 4, 1.0, 1.0)
+    bed = Bathymetry.flat(grid, 2.0)
+    s = State.lake_at_rest(grid, bed, 0.0)
+    x, _ = grid.centres()
+    grid.inner(s.h)[:, :] += 0.1 * np.exp(-(((x[GHOST:GHOST + grid.nx] - 40.0) / 5.0) ** 2))[None, :]
+    return grid, bed, s
+
+
+def _surface(solver, s, steps):
+    for _ in range(steps):
+        solver.step(s, 0.05)
+    return solver.grid.inner(s.arrays()[0]).ravel().copy()
+
+
+def check_held_cells_run_as_shallow_water(make):
+    """shallow_cells.rs:39-64."""
+    grid, bed, s = _hump()
+
+    def dispersive():
+        return make(grid, bed.copy()).with_order(Order.Fifth).with_dispersion(Dispersion())
+
+    held = _surface(dispersive().with_shallow_cells([True] * (grid.nx * grid.ny)), s.clone(), 200)
+    plain = _surface(make(grid, bed.copy()).with_order(Order.Second), s.clone(), 200)
+    free = _surface(dispersive(), s.clone(), 200)
+    assert np.max(np.abs(held - plain)) < 1e-9
+    assert np.max(np.abs(free - plain)) > 1e-4
+
+
+def check_only_held_cells_change(make):
+    """shallow_cells.rs:66-91."""
+    grid, bed, s = _hump()
+
+    def dispersive():
+        return make(grid, bed.copy()).with_order(Order.Fifth).with_dispersion(Dispersion())
+
+    far = [i >= 110 for _ in range(grid.ny) for i in range(grid.nx)]
+    held = _surface(dispersive().with_shallow_cells(far), s.clone(), 200)
+    everywhere = _surface(dispersive().with_shallow_cells([True] * (grid.nx * grid.ny)),
+                          s.clone(), 200)
+    free = _surface(dispersive(), s.clone(), 200)
+    near = np.max(np.abs(held - free))
+    whole = np.max(np.abs(everywhere - free))
+    assert near < 0.01 * whole, (near, whole)
+
+
+def check_breaking_indicator_reads_surface_not_held_cells(make):
+    """shallow_cells.rs:93-108."""
+    grid, bed, s = _hump()
+
+    def mk():
+        return make(grid, bed.copy()).with_order(Order.Fifth).with_dispersion(Dispersion())
+
+    held = mk().with_shallow_cells([True] * (grid.nx * grid.ny))
+    ind = held.breaking_indicator(s)
+    np.testing.assert_array_equal(ind, mk().breaking_indicator(s))
+    assert np.all(ind < 0.1)
+
+
+def check_wrong_size_mask_refused(make):
+    """shallow_cells.rs:110-114."""
+    grid, bed, _ = _hump()
+    try:
+        make(grid, bed).with_shallow_cells([True] * 10)
+    except ValueError as e:
+        assert "one entry per interior cell" in str(e)
+    else:
+        raise AssertionError("a mask of the wrong size was accepted")
+
+
+def random_operator_setup(periodic: bool):
+    """dispersion.rs:653-666 (+ the periodic/mirror preparation of 701-715)."""
+    grid = Grid(13, 9, 2.0, 3.0)
+    rng = Lcg(7)
+    n = grid.cells()
+    heff, mask, c = np.zeros(n), np.zeros(n), np.zeros(n)
+    for k in range(n):
+        heff[k] = 0.5 + 4.0 * (rng.next() + 1.0)
+        mask[k] = 1.0
+        c[k] = 3.0 * (rng.next() + 1.0)
+    return grid, heff, mask, c, rng
+
+
+def random_field(grid: Grid, rng: Lcg):
+    """dispersion.rs:668-676."""
+    x, y = np.zeros(grid.cells()), np.zeros(grid.cells())
+    for i, j in grid.interior():
+        x[grid.idx(i, j)] = rng.next()
+        y[grid.idx(i, j)] = rng.next()
+    return x, y

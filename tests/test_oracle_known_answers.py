@@ -1,0 +1,352 @@
+"""The CPU oracle against the reference's known-answer tests and unit tests.
+
+Pins the oracle without the reference binary (which needs a Rust toolchain this image lacks):
+each test below restates a reference test with its own bounds, and the white-box tests check
+the pieces against closed forms (quartic exactness, the exact drag step), brute force (a dense
+solve of the assembled SGN operator) and invariants (symmetry, positivity, masking).
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import known_answers as ka
+from helpers import Lcg
+from oracle import oracle as orc
+from wavesim_amd.config import Dispersion, Order
+from wavesim_amd.grid import GHOST, Grid, State, mirror_ghosts, wrap_ghosts_y
+
+
+def make(grid, bed):
+    return orc.OracleSolver(grid, bed)
+
+
+def make_canonical(grid, bed):
+    return orc.OracleSolver(grid, bed, dot_order=orc.DOT_CANONICAL, math=orc.MATH_PORTABLE)
+
+
+# ---- in-source unit tests --------------------------------------------------------------------
+def test_fifth_order_face_values_are_exact_for_quartics():
+    """solver.rs:740-754."""
+    p = lambda x: 1.0 + 2.0 * x - x * x + 0.5 * x ** 3 - 0.25 * x ** 4  # noqa: E731
+    integral = lambda x: x + x * x - x ** 3 / 3.0 + 0.125 * x ** 4 - 0.05 * x ** 5  # noqa: E731
+    v = [integral(m - 1.5) - integral(m - 2.5) for m in range(5)]
+    lo, hi = orc.face_values(v, 1.0)
+    assert abs(lo - p(-0.5)) < 1e-12
+    assert abs(hi - p(0.5)) < 1e-12
+    lo, hi = orc.face_values([0.0, 0.0, 1.0, 1.0, 1.0], 0.0)
+    assert 0.0 <= lo <= 1.0 and 0.0 <= hi <= 1.0
+
+
+def test_face_values_mc_limiter_closed_forms():
+    # A linear profile: MC keeps the exact slope; an extremum: the slope is zero.
+    lo, hi = orc.face_values([1.0, 2.0, 3.0, 4.0, 5.0], 0.0)
+    assert (lo, hi) == (2.5, 3.5)
+    lo, hi = orc.face_values([0.0, 1.0, 2.0, 1.0, 0.0], 0.0)
+    assert (lo, hi) == (2.0, 2.0)
+    # Blend 1 on a quartic-free cubic: the fifth-order weights sum to one.
+    lo, hi = orc.face_values([7.0] * 5, 0.7)
+    assert (lo, hi) == (7.0, 7.0)
+
+
+def test_smoothstep_is_a_clamped_cubic():
+    """dispersion.rs:829-834."""
+    assert orc.smoothstep(1.0, 3.0, 0.0) == 0.0
+    assert orc.smoothstep(1.0, 3.0, 4.0) == 1.0
+    assert abs(orc.smoothstep(1.0, 3.0, 2.0) - 0.5) < 1e-15
+
+
+def test_manning_factor_is_the_exact_quadratic_drag_step():
+    """waves.rs:224-236."""
+    h, u0, dt, n = 1.5, 0.8, 0.2, 0.03
+    k = 9.81 * n * n / h ** (4.0 / 3.0)
+    exact = u0 / (1.0 + k * u0 * dt)
+    assert abs(orc.manning_factor(h, u0, dt, n) * u0 - exact) < 1e-15
+    assert orc.manning_factor(h, u0, dt, 0.0) == 1.0
+    assert orc.manning_factor(h, 100.0, 100.0, 0.1) > 0.0
+
+
+def test_the_linear_wave_relation_inverts_the_dispersion_relation():
+    ka.check_linear_wave_inverts_dispersion_relation(orc.linear_wave)
+    with pytest.raises(ValueError):
+        orc.linear_wave(2.0 * math.pi / 1.0, 5.0, True)  # omega^2 h / g >= 3
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_the_operator_is_symmetric_and_positive_definite(periodic):
+    """dispersion.rs:701-733."""
+    grid, heff, mask, c, rng = ka.random_operator_setup(periodic)
+    if periodic:
+        wrap_ghosts_y(grid, c)
+        wrap_ghosts_y(grid, heff)
+    mirror_ghosts(grid, c)
+    if periodic:
+        wrap_ghosts_y(grid, c)
+    for _ in range(5):
+        x1, y1 = ka.random_field(grid, rng)
+        x2, y2 = ka.random_field(grid, rng)
+        ax1, ay1 = orc.matvec(grid, periodic, heff, mask, c, x1, y1)
+        ax2, ay2 = orc.matvec(grid, periodic, heff, mask, c, x2, y2)
+        left = orc.dot(grid, x2, y2, ax1, ay1)
+        right = orc.dot(grid, x1, y1, ax2, ay2)
+        assert abs(left - right) < 1e-10 * max(abs(left), 1.0)
+        assert orc.dot(grid, x1, y1, ax1, ay1) > 0.0
+
+
+def test_with_thin_cells_masked_out_the_operator_is_still_spd():
+    """dispersion.rs:735-781."""
+    grid, heff, mask, c, rng = ka.random_operator_setup(False)
+    mirror_ghosts(grid, c)
+    for k in range(grid.cells()):
+        mask[k] = 1.0 if rng.next() > -0.3 else 0.0
+    mirror_ghosts(grid, mask)
+    inner = np.array([grid.idx(i, j) for i, j in grid.interior()])
+    assert np.any(mask[inner] == 0.0)
+    for _ in range(5):
+        x1, y1 = ka.random_field(grid, rng)
+        x2, y2 = ka.random_field(grid, rng)
+        for v in (x1, y1):
+            v[inner] *= mask[inner]
+        for v in (x2, y2):
+            v[inner] *= mask[inner]
+        ax1, ay1 = orc.matvec(grid, False, heff, mask, c, x1, y1)
+        ax2, ay2 = orc.matvec(grid, False, heff, mask, c, x2, y2)
+        left = orc.dot(grid, x2, y2, ax1, ay1)
+        right = orc.dot(grid, x1, y1, ax2, ay2)
+        assert abs(left - right) < 1e-10 * max(abs(left), 1.0)
+        assert orc.dot(grid, x1, y1, ax1, ay1) > 0.0
+        masked = inner[mask[inner] == 0.0]
+        assert np.all(ax1[masked] == 0.0) and np.all(ay1[masked] == 0.0)
+
+
+def _film_run(film_speed):
+    grid = Grid(12, 8, 2.0, 2.0)
+    film = grid.idx(GHOST + 6, GHOST + 4)
+    s = State.zeros(grid)
+    h, hu, hv = s.h, s.hu, s.hv
+    for i, j in grid.interior():
+        k = grid.idx(i, j)
+        h[k] = 2.0
+        hu[k] = 2.0 * 0.3 * math.sin(0.7 * i)
+        hv[k] = 2.0 * 0.2 * math.cos(0.5 * j)
+    h[film] = 0.001
+    hu[film] = 0.001 * film_speed
+    hv[film] = 0.0
+    s.fill_boundaries(grid, False)
+    phi = np.ones(grid.cells())
+    phi[film] = 0.0
+    n = grid.cells()
+    lh = np.array([0.01 * (k % 7) for k in range(n)])
+    lhu = np.array([0.05 * (k % 5) for k in range(n)])
+    lhv = np.array([0.03 * (k % 3) for k in range(n)])
+    before = (lhu[film], lhv[film])
+    h, hu, hv, _ = s.arrays()
+    u, v = orc.dispersion_apply(grid, False, Dispersion(), h, hu, hv, phi, lh, lhu, lhv)
+    return grid, film, u, v, before
+
+
+def test_a_thin_film_neither_feels_nor_feeds_the_dispersive_system():
+    """dispersion.rs:783-827."""
+    grid, film, a_u, a_v, before = _film_run(1.0)
+    _, _, b_u, b_v, _ = _film_run(1000.0)
+    for i, j in grid.interior():
+        k = grid.idx(i, j)
+        if k == film:
+            assert (a_u[k], a_v[k]) == before
+            assert (b_u[k], b_v[k]) == before
+        else:
+            assert a_u[k] == b_u[k] and a_v[k] == b_v[k], (i, j)
+
+
+def test_pcg_solution_matches_a_dense_solve_of_the_assembled_operator():
+    """Brute force: on a tiny grid the SGN system solved by the oracle's PCG (tolerance 1e-13)
+    equals numpy's dense solve of M A M assembled column by column from the matvec. PCG stops at
+    a residual of 1e-10 absolute at the tightest (dispersion.rs:511), so w agrees to ~1e-10."""
+    grid = Grid(7, 5, 2.0, 3.0)
+    rng = Lcg(11)
+    n = grid.cells()
+    s = State.zeros(grid)
+    h, hu, hv = s.h, s.hu, s.hv
+    for i, j in grid.interior():
+        k = grid.idx(i, j)
+        h[k] = 3.0 + rng.next()
+        hu[k] = h[k] * 0.3 * rng.next()
+        hv[k] = h[k] * 0.2 * rng.next()
+    h[grid.idx(GHOST + 3, GHOST + 2)] = 0.01  # one masked cell
+    s.fill_boundaries(grid, False)
+    h, hu, hv, _ = s.arrays()
+    phi = np.zeros(n)
+    for k in range(n):
+        phi[k] = 0.6 + 0.4 * abs(rng.next())
+    mirror_ghosts(grid, phi)
+    lh = np.array([0.02 * rng.next() for _ in range(n)])
+    lhu = np.array([0.1 * rng.next() for _ in range(n)])
+    lhv = np.array([0.1 * rng.next() for _ in range(n)])
+    d = Dispersion(tolerance=1e-13, max_iterations=500, nonlinear=False)
+    out_u, out_v = orc.dispersion_apply(grid, False, d, h, hu, hv, phi, lh, lhu, lhv)
+
+    inner = [grid.idx(i, j) for i, j in grid.interior()]
+    mask = (h >= 0.05).astype(float)
+    c = np.where(h < 0.05, 0.0, phi * h * h * h / 3.0)
+    heff = np.maximum(h, 1e-8)
+    u = np.where(h > 1e-8, hu / h, 0.0)
+    v = np.where(h > 1e-8, hv / h, 0.0)
+    m = len(inner)
+    A = np.zeros((2 * m, 2 * m))
+    for col in range(2 * m):
+        px, py = np.zeros(n), np.zeros(n)
+        (px if col < m else py)[inner[col % m]] = 1.0
+        ax, ay = orc.matvec(grid, False, heff, mask, c, px, py)
+        A[:m, col] = ax[inner]
+        A[m:, col] = ay[inner]
+    bx = np.where(mask > 0, lhu - u * lh, 0.0)[inner]
+    by = np.where(mask > 0, lhv - v * lh, 0.0)[inner]
+    keep = np.concatenate([mask[inner], mask[inner]]) > 0
+    Ak = A[np.ix_(keep, keep)]
+    assert np.allclose(Ak, Ak.T, atol=1e-12)
+    w = np.zeros(2 * m)
+    w[keep] = np.linalg.solve(Ak, np.concatenate([bx, by])[keep])
+    hi = h[inner]
+    exp_u = np.where(mask[inner] > 0, hi * w[:m] + u[inner] * lh[inner], lhu[inner])
+    exp_v = np.where(mask[inner] > 0, hi * w[m:] + v[inner] * lh[inner], lhv[inner])
+    np.testing.assert_allclose(out_u[inner], exp_u, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(out_v[inner], exp_v, rtol=0, atol=1e-9)
+    # ...and not by accident: a wrong sign of the dispersive term breaks it by far more.
+    assert np.max(np.abs(out_u[inner] - np.where(mask[inner] > 0, lhu[inner], 0))) > 1e-4
+
+
+def test_canonical_dot_matches_a_correctly_rounded_sum():
+    """The canonical (GPU) summation order is a valid reordering: within a few ulps of the
+    exactly rounded sum (math.fsum) and of the reference's row order."""
+    rng = np.random.default_rng(5)
+    for nx, ny in [(1, 1), (13, 9), (64, 4), (65, 5), (300, 77), (1024, 33)]:
+        grid = Grid(nx, ny, 1.0, 1.0)
+        n = grid.cells()
+        a = [rng.standard_normal(n) for _ in range(4)]
+        inner = grid.inner(np.arange(n)).ravel()
+        terms = (a[0] * a[2] + a[1] * a[3])[inner]
+        exact = math.fsum(terms.tolist())
+        canon = orc.dot(grid, *a, order=orc.DOT_CANONICAL)
+        ref = orc.dot(grid, *a, order=orc.DOT_REFERENCE)
+        scale = float(np.sum(np.abs(terms)))
+        assert abs(canon - exact) <= 1e-14 * scale
+        assert abs(ref - exact) <= 1e-13 * scale
+
+
+def test_portable_math_is_within_one_ulp_of_glibc():
+    rng = np.random.default_rng(3)
+    worst_h = worst_p = 0
+    for _ in range(20000):
+        a = float(rng.standard_normal() * 10 ** rng.uniform(-4, 4))
+        b = float(rng.standard_normal() * 10 ** rng.uniform(-4, 4))
+        x, y = orc.hypot(a, b, True), orc.hypot(a, b, False)
+        worst_h = max(worst_h, abs(x - y) / math.ulp(y) if y else 0)
+        h = float(10 ** rng.uniform(-3.5, 3))
+        p, q = orc.pow43(h, True), orc.pow43(h, False)
+        worst_p = max(worst_p, abs(p - q) / math.ulp(q))
+    assert worst_h <= 1.0 and worst_p <= 1.0
+    assert orc.hypot(3.0, 4.0, True) == 5.0
+    assert orc.hypot(math.inf, math.nan, True) == math.inf
+    assert math.isnan(orc.hypot(1.0, math.nan, True))
+    assert orc.hypot(0.0, 0.0, True) == 0.0
+    assert orc.hypot(1e300, 1e300, True) == math.hypot(1e300, 1e300)
+    assert orc.hypot(1e-300, 1e-300, True) == math.hypot(1e-300, 1e-300)
+
+
+# ---- the reference's integration tests, on the oracle ----------------------------------------
+def test_lake_at_rest_over_rough_wet_bed():
+    ka.check_lake_at_rest_over_rough_wet_bed(make)
+
+
+def test_lake_at_rest_with_dry_bumps():
+    ka.check_lake_at_rest_with_dry_bumps(make)
+
+
+def test_dam_break_conserves_volume_and_stays_positive():
+    ka.check_dam_break_conserves_volume_and_stays_positive(make)
+
+
+def test_walls_and_a_periodic_seam_keep_the_water():
+    ka.check_walls_and_periodic_seam_keep_volume(make)
+
+
+def test_a_cell_that_is_no_longer_a_number_is_seen_not_skipped():
+    ka.check_nan_cell_is_seen_not_skipped(make)
+
+
+def test_the_limiting_cell_is_the_fastest_water():
+    ka.check_limiting_cell_is_fastest_water(make)
+
+
+def test_still_water_is_limited_by_its_depth_and_dry_land_by_nothing():
+    ka.check_still_water_limited_by_depth_dry_land_by_nothing(make)
+
+
+def test_a_standing_wave_oscillates_at_the_sgn_frequency():
+    out = ka.check_standing_wave_sgn_frequency(make)
+    # The reference's published numbers (README.md:336): Fifth 1.4261 (tolerance of the test's
+    # own discretisation), MC 1.4278, theory 1.4247.
+    assert abs(out["Fifth"] - 1.4261) < 2e-3, out
+    assert abs(out["Second"] - 1.4278) < 2e-3, out
+
+
+def test_still_water_stays_still_with_dispersion():
+    ka.check_still_water_with_dispersion(make)
+
+
+def test_a_swell_crosses_a_kilometre_of_shelf_on_coarse_cells():
+    kept = ka.check_swell_crosses_shelf_on_coarse_cells(make)
+    # README.md:338 publishes 100.7% for this case.
+    assert abs(kept - 1.007) < 0.01, kept
+
+
+def test_exact_wet_bed_solution_is_self_consistent():
+    """stoker.rs:102-112."""
+    e = ka.StokerExact(2.0, 0.5)
+    assert e.h_r < e.h_m < e.h_l
+    assert abs(e.shock * (e.h_m - e.h_r) - e.h_m * e.u_m) < 1e-12
+    lhs = e.shock * e.h_m * e.u_m
+    rhs = e.h_m * e.u_m ** 2 + 0.5 * 9.81 * (e.h_m ** 2 - e.h_r ** 2)
+    assert abs(lhs - rhs) < 1e-9
+
+
+def test_wet_bed_dam_break_matches_stoker():
+    rates = ka.check_dam_break(make, 2.0, 0.5)
+    # README.md:326 publishes rate 1.07.
+    assert abs(rates["Second"] - 1.07) < 0.05, rates
+
+
+def test_dry_bed_dam_break_matches_ritter():
+    rates = ka.check_dam_break(make, 2.0, 0.0)
+    # README.md:327 publishes rate 1.03.
+    assert abs(rates["Second"] - 1.03) < 0.05, rates
+
+
+def test_a_sponge_over_land_never_speeds_the_water_up():
+    ka.check_sponge_over_land_never_speeds_water_up(make)
+
+
+def test_held_cells_run_exactly_as_shallow_water():
+    ka.check_held_cells_run_as_shallow_water(make)
+
+
+def test_only_the_held_cells_change():
+    ka.check_only_held_cells_change(make)
+
+
+def test_the_breaking_indicator_reads_the_surface_not_the_held_cells():
+    ka.check_breaking_indicator_reads_surface_not_held_cells(make)
+
+
+def test_a_mask_of_the_wrong_size_is_refused():
+    ka.check_wrong_size_mask_refused(make)
+
+
+def test_canonical_mode_passes_the_same_checks():
+    """The GPU's summation order and libm pass the reference's checks just as well."""
+    ka.check_lake_at_rest_with_dry_bumps(make_canonical, orders=(Order.Fifth,))
+    ka.check_still_water_with_dispersion(make_canonical)
+    ka.check_walls_and_periodic_seam_keep_volume(make_canonical, cases=[(True, True)])
