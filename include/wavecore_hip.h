@@ -1,0 +1,166 @@
+/* wavecore_hip — wavecore's time step (FV + SGN) on MI355X (gfx950), behind a C ABI.
+ *
+ * Each entry point replaces one item of wavecore's public API (crates/wavecore/src/lib.rs:60-66)
+ * and keeps its argument meaning. The Rust crate wavecore-hip binds these
+ * one to one; the Python mirror wavesim_amd and the C++ header wavecore_hip.hpp wrap them too.
+ *
+ * Ownership and threading. A wc_solver owns, on one GPU, the bed, the settings, the current
+ * state, every scratch array and the dispersive solve's warm start (the reference keeps the
+ * warm start and the switch inside Solver through RefCell, solver.rs:272-279). The state stays
+ * on the device between steps; wc_state_set / wc_state_get are the only host<->device copies.
+ * Calls on one solver must not overlap (the reference's Solver is !Sync); different solvers
+ * are independent. Every call is synchronous unless its comment says otherwise.
+ *
+ * Arrays are padded like the reference's (grid.rs:6-58): (nx+6)*(ny+6) doubles, row-major with
+ * y outer, interior cells at [3, nx+3) x [3, ny+3). This build needs nx >= 3 and ny >= 3.
+ *
+ * Errors. Every int-returning call returns WC_OK or an error code and leaves a message for
+ * wc_last_error() (thread-local) where the reference would panic (grid.rs:21-27,
+ * forcing.rs:63-66,89-90, solver.rs:377-381,424-427). Numerical failure is not an error:
+ * wc_stable_dt reports NaN, 0 or +inf exactly as Solver::stable_dt does (solver.rs:431-450).
+ */
+#ifndef WAVECORE_HIP_H
+#define WAVECORE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WC_OK 0
+#define WC_EINVAL 1  /* invalid setup: where the reference panics */
+#define WC_EDEVICE 2 /* a HIP call failed, or no gfx950 device */
+#define WC_ENOMEM 3  /* device memory exhausted */
+
+/* Order (solver.rs:34-50). */
+#define WC_ORDER_FIRST 0
+#define WC_ORDER_SECOND 1
+#define WC_ORDER_FIFTH 2
+
+/* Arithmetic of the step. */
+#define WC_PRECISION_FP64 0 /* the plain path: FP64 throughout, matches the FP64 oracle */
+
+typedef struct wc_solver wc_solver;
+
+/* WaveMaker (forcing.rs:18-33). */
+typedef struct {
+  double x, amplitude, period, angle, sigma, ramp_periods;
+} wc_wavemaker_t;
+
+/* Sponge (forcing.rs:146-178); the edges are booleans. */
+typedef struct {
+  uint64_t width;
+  double strength, level;
+  int32_t west, east, south, north;
+} wc_sponge_t;
+
+/* Breaking (dispersion.rs:44-74), without the eddy viscosity, which is not built. */
+typedef struct {
+  double eta_ratio_lo, eta_ratio_hi, slope_lo, slope_hi;
+} wc_breaking_t;
+
+/* Dispersion (dispersion.rs:76-97). */
+typedef struct {
+  wc_breaking_t breaking;
+  int32_t nonlinear;
+  int32_t reserved; /* must be 0 */
+  double tolerance;
+  uint64_t max_iterations;
+} wc_dispersion_t;
+
+/* DispersionStats (dispersion.rs:99-106). */
+typedef struct {
+  uint64_t solves, iterations, unconverged;
+} wc_dispersion_stats_t;
+
+/* Limiting (solver.rs:248-258); i, j count interior cells from 0. */
+typedef struct {
+  uint64_t i, j;
+  double depth, speed;
+} wc_limiting_t;
+
+/* LinearWave (forcing.rs:35-44). */
+typedef struct {
+  double k, phase_speed, group_speed;
+} wc_linear_wave_t;
+
+/* The message of the last failed call on this thread. */
+const char* wc_last_error(void);
+
+/* Solver::new(Grid::new(nx, ny, dx, dy), Bathymetry { b }) (solver.rs:283-299, grid.rs:20-27).
+ * bed is a padded array, ghosts already filled as Bathymetry builds them. Defaults as the
+ * reference: cfl 0.4, Order::Second, no friction, walls. Runs on the current HIP device. */
+int wc_solver_new(uint64_t nx, uint64_t ny, double dx, double dy, const double* bed,
+                  wc_solver** out);
+void wc_solver_free(wc_solver* s);
+
+/* Builder (solver.rs:301-419). The `cfl` field (solver.rs:263) has a setter and a getter. */
+int wc_solver_set_cfl(wc_solver* s, double cfl);
+int wc_solver_cfl(const wc_solver* s, double* cfl);
+int wc_solver_with_order(wc_solver* s, int32_t order);                /* solver.rs:302 */
+int wc_solver_with_manning(wc_solver* s, double n);                   /* solver.rs:308 */
+int wc_solver_with_sponge(wc_solver* s, const wc_sponge_t* sponge);   /* solver.rs:314 */
+int wc_solver_with_wavemaker(wc_solver* s, const wc_wavemaker_t* m); /* solver.rs:321 */
+int wc_solver_with_dispersion(wc_solver* s, const wc_dispersion_t* d); /* solver.rs:330 */
+int wc_solver_with_periodic_y(wc_solver* s);                          /* solver.rs:414 */
+/* solver.rs:376: one byte per interior cell, row-major with y outer, nonzero = held. */
+int wc_solver_with_shallow_cells(wc_solver* s, const uint8_t* cells, uint64_t n);
+/* The bed as the solver holds it (with_periodic_y wraps its ghost rows; solver.rs:416). */
+int wc_solver_bed(const wc_solver* s, double* out);
+/* Selects the arithmetic (WC_PRECISION_*); FP64 by default. */
+int wc_solver_set_precision(wc_solver* s, int32_t precision);
+
+/* The state the solver steps (State, state.rs:7-17): padded h, hu, hv, breaking and the time.
+ * breaking may be NULL on set (zeros). Any output pointer may be NULL on get. */
+int wc_state_set(wc_solver* s, const double* h, const double* hu, const double* hv,
+                 const double* breaking, double time);
+int wc_state_get(wc_solver* s, double* h, double* hu, double* hv, double* breaking,
+                 double* time);
+/* Sets only the time of the state held by the solver. */
+int wc_state_set_time(wc_solver* s, double time);
+
+/* Solver::stable_dt (solver.rs:434-450): cfl / max rate, +inf if nothing moves, NaN if any
+ * cell is NaN, 0 if any moves infinitely fast. */
+int wc_stable_dt(wc_solver* s, double* dt);
+/* Solver::step (solver.rs:481-507): one SSP-RK2 step (Euler for Order::First) of size dt,
+ * with sponges and friction, advancing the time by dt. */
+int wc_step(wc_solver* s, double dt);
+/* Solver::limiting_cell (solver.rs:455-477); *found = 0 where the reference returns None. */
+int wc_limiting_cell(wc_solver* s, int32_t* found, wc_limiting_t* out);
+/* Solver::breaking_indicator (solver.rs:341-349): nx*ny values, row-major with y outer. */
+int wc_breaking_indicator(wc_solver* s, double* out);
+/* Solver::dispersion_stats (solver.rs:392-396); *has = 0 for shallow water (None). */
+int wc_dispersion_stats(wc_solver* s, int32_t* has, wc_dispersion_stats_t* out);
+/* State::volume / max_abs_momentum / min_depth (state.rs:79-102) of the held state. The
+ * volume sums the interior in the GPU's canonical order (64x4 tiles, wavefront butterflies),
+ * not serially: it agrees with the reference's serial sum to rounding. */
+int wc_state_volume(wc_solver* s, double* out);
+int wc_state_max_abs_momentum(wc_solver* s, double* out);
+int wc_state_min_depth(wc_solver* s, double* out);
+
+/* Free functions (forcing.rs:53-75, 294-297). wc_linear_wave returns WC_EINVAL where the
+ * reference panics (a wave too short for the dispersive model). */
+int wc_linear_wave(double omega, double h, int32_t dispersive, wc_linear_wave_t* out);
+double wc_manning_factor(double h, double speed, double dt, double n);
+
+/* Runs `steps` iterations of { dt = stable_dt; step(dt) } on the device without returning to
+ * the caller in between (the loop of crates/wavecore/tests/common/mod.rs:53-66 and
+ * wavesim's Stage::step, lib.rs:1385-1434). Stops early, with WC_OK, if dt is not finite and
+ * positive; *done_steps reports how many steps ran and *last_dt the last dt. */
+int wc_run_steps(wc_solver* s, uint64_t steps, uint64_t* done_steps, double* last_dt);
+
+/* Measurement hooks for benchmarks: average device time in milliseconds of each kernel family
+ * over the steps since the last reset, measured with HIP events on the solver's stream, and the
+ * kernel launch counts. names receives a ';'-separated list. Timing adds events around every
+ * launch; leave it off (the default) outside measurements. */
+int wc_profile_enable(wc_solver* s, int32_t on);
+int wc_profile_reset(wc_solver* s);
+int wc_profile_read(wc_solver* s, char* names, uint64_t names_len, double* total_ms,
+                    uint64_t* launches, uint64_t max_kernels, uint64_t* n_kernels);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,1005 @@
+// Host side of libwavecore_hip: the C ABI of include/wavecore_hip.h over the gfx950 kernels.
+//
+// A wc_solver is the reference's Solver (crates/wavecore/src/solver.rs:260-280) plus the one
+// State it steps, resident on one GPU. Everything a step needs lives in device memory; the host
+// only evaluates what depends on nothing but the time and dt (the wave-maker's ramp and phase,
+// the sponges' exp(-rate dt)), with the same libm calls the reference makes, and ships those few
+// numbers each step.
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/wavecore_hip.h"
+#include "wc_kernels.h"
+
+using namespace wc;
+
+namespace {
+
+constexpr double TAU = 6.283185307179586476925286766559;
+constexpr double PI = 3.14159265358979323846264338327950;
+constexpr int PRM_SLOTS = 3;
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_OK(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(e_ == hipErrorOutOfMemory ? WC_ENOMEM : WC_EDEVICE,                     \
+                  std::string(#expr) + " failed: " + hipGetErrorString(e_));              \
+  } while (0)
+
+enum Family {
+  F_FILL,
+  F_SWITCH,
+  F_FLUX,
+  F_EULER,
+  F_SGN_SETUP,
+  F_CG_INIT,
+  F_CG_MATVEC,
+  F_CG_UPDATE,
+  F_CG_PUPDATE,
+  F_CG_STATS,
+  F_SGN_FINISH,
+  F_REDUCE,
+  F_COUNT
+};
+const char* kFamilyNames[F_COUNT] = {"fill_state", "switch",    "flux",      "finish_euler",
+                                     "sgn_setup",  "cg_init",   "cg_matvec", "cg_update",
+                                     "cg_pupdate", "cg_stats",  "sgn_finish", "reduce"};
+
+struct HostSponge {
+  uint64_t width;
+  double strength, level;
+  bool west, east, south, north;
+  long long offset, len;
+};
+
+// linear_wave (forcing.rs:53-75).
+bool linear_wave(double omega, double h, bool dispersive, wc_linear_wave_t* out) {
+  if (!dispersive) {
+    const double c = sqrt(WC_G * h);
+    out->k = omega / c;
+    out->phase_speed = c;
+    out->group_speed = c;
+    return true;
+  }
+  const double nu = omega * omega * h / WC_G;
+  if (!(nu < 3.0)) return false;
+  const double kh = sqrt(nu / (1.0 - nu / 3.0));
+  const double k = kh / h;
+  const double phase = omega / k;
+  out->k = k;
+  out->phase_speed = phase;
+  out->group_speed = phase / (1.0 + kh * kh / 3.0);
+  return true;
+}
+
+}  // namespace
+
+struct wc_solver {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Geo geo{};
+  size_t cells = 0;
+  std::vector<double> bed;
+
+  double cfl = 0.4;
+  int order = WC_ORDER_SECOND;
+  double manning = 0.0;
+  bool periodic = false;
+  int precision = WC_PRECISION_FP64;
+
+  // wave-maker and its Drive (forcing.rs:77-139)
+  bool has_maker = false;
+  wc_wavemaker_t maker{};
+  bool has_drive = false;
+  double omega = 0.0, ky = 0.0, strength = 0.0;
+  int src_i0 = 0, src_i1 = 0;
+
+  std::vector<HostSponge> sponges;
+  std::vector<double> sponge_rates;  // concatenated tables, static
+
+  bool dispersive = false;
+  wc_dispersion_t disp{};
+  bool has_held = false;
+  double time = 0.0;
+  unsigned long long prev_it[2] = {0, 0};
+
+  // device arrays (padded, FP64)
+  double *h = nullptr, *hu = nullptr, *hv = nullptr, *br = nullptr;
+  double *h1 = nullptr, *hu1 = nullptr, *hv1 = nullptr;
+  double *b = nullptr, *phi = nullptr, *lh = nullptr, *lhu = nullptr, *lhv = nullptr;
+  double *c = nullptr, *dgx = nullptr, *dgy = nullptr, *bx = nullptr, *by = nullptr;
+  double *wx = nullptr, *wy = nullptr, *rx = nullptr, *ry = nullptr, *px = nullptr,
+         *py = nullptr, *ax = nullptr, *ay = nullptr;
+  uint8_t* held = nullptr;
+  double* part = nullptr;
+  CgScalars* sc = nullptr;
+  double* src_col = nullptr;
+  double* sponge_rate_d = nullptr;
+  double* prm = nullptr;
+  size_t prm_len = 0;
+  unsigned long long* red64 = nullptr;  // [0] max rate bits, [1] max momentum, [2] min-h key
+  unsigned int* red32 = nullptr;        // [0] ticket, [1] first NaN, [2] first max
+  double* redf = nullptr;               // [0] volume sum
+  std::vector<void*> owned;
+
+  // pinned host staging
+  double* prm_host[PRM_SLOTS] = {nullptr, nullptr, nullptr};
+  hipEvent_t prm_ev[PRM_SLOTS] = {nullptr, nullptr, nullptr};
+  bool prm_used[PRM_SLOTS] = {false, false, false};
+  int prm_slot = 0;
+  size_t prm_cap = 0;
+  CgScalars* sc_host = nullptr;
+  unsigned long long* red64_host = nullptr;
+  unsigned int* red32_host = nullptr;
+  double* redf_host = nullptr;
+
+  // profiling
+  bool prof = false;
+  double prof_ms[F_COUNT] = {};
+  unsigned long long prof_n[F_COUNT] = {};
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+};
+
+namespace {
+
+int set_device(wc_solver* s) {
+  HIP_OK(hipSetDevice(s->device));
+  return WC_OK;
+}
+
+template <class T>
+int dalloc(wc_solver* s, T** p, size_t n) {
+  void* v = nullptr;
+  HIP_OK(hipMalloc(&v, std::max<size_t>(n, 1) * sizeof(T)));
+  HIP_OK(hipMemsetAsync(v, 0, std::max<size_t>(n, 1) * sizeof(T), s->stream));
+  s->owned.push_back(v);
+  *p = static_cast<T*>(v);
+  return WC_OK;
+}
+
+hipEvent_t take_event(wc_solver* s) {
+  if (!s->ev_pool.empty()) {
+    hipEvent_t e = s->ev_pool.back();
+    s->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+template <typename... KArgs, typename... Args>
+int launch(wc_solver* s, int fam, void (*k)(KArgs...), dim3 grid, dim3 block, Args... args) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (s->prof) {
+    e0 = take_event(s);
+    e1 = take_event(s);
+    if (e0) HIP_OK(hipEventRecord(e0, s->stream));
+  }
+  hipLaunchKernelGGL(k, grid, block, 0, s->stream, args...);
+  HIP_OK(hipGetLastError());
+  if (s->prof && e0 && e1) {
+    HIP_OK(hipEventRecord(e1, s->stream));
+    s->ev_pending.push_back({fam, {e0, e1}});
+  }
+  return WC_OK;
+}
+
+#define TRY(expr)               \
+  do {                          \
+    int rc_ = (expr);           \
+    if (rc_ != WC_OK) return rc_; \
+  } while (0)
+
+int prof_collect(wc_solver* s) {
+  if (s->ev_pending.empty()) return WC_OK;
+  HIP_OK(hipStreamSynchronize(s->stream));
+  for (auto& p : s->ev_pending) {
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+    s->prof_ms[p.first] += ms;
+    s->prof_n[p.first] += 1;
+    s->ev_pool.push_back(p.second.first);
+    s->ev_pool.push_back(p.second.second);
+  }
+  s->ev_pending.clear();
+  return WC_OK;
+}
+
+dim3 tiles(const wc_solver* s) { return dim3((unsigned)(s->geo.ntx * s->geo.nty)); }
+dim3 over(long long n) { return dim3((unsigned)((n + NT - 1) / NT)); }
+
+Breaking breaking_of(const wc_solver* s) {
+  // Solver::breaking_config (solver.rs:619-623): the dispersive model's, or the defaults.
+  if (s->dispersive)
+    return Breaking{s->disp.breaking.eta_ratio_lo, s->disp.breaking.eta_ratio_hi,
+                    s->disp.breaking.slope_lo, s->disp.breaking.slope_hi};
+  return Breaking{0.30, 0.55, 0.40, 0.65};
+}
+
+// Drive::new (forcing.rs:88-117) and the static per-column part of Drive::source.
+int build_drive(wc_solver* s, double still_depth) {
+  const wc_wavemaker_t& m = s->maker;
+  if (!(still_depth > 0.0)) return fail(WC_EINVAL, "wave-maker must sit in water");
+  if (!(m.period > 0.0 && m.sigma > 0.0 && m.amplitude >= 0.0))
+    return fail(WC_EINVAL, "wave-maker needs period > 0, sigma > 0 and amplitude >= 0");
+  const double omega = TAU / m.period;
+  wc_linear_wave_t lw;
+  if (!linear_wave(omega, still_depth, s->dispersive, &lw))
+    return fail(WC_EINVAL, "a wave of this angular frequency is too short for the dispersive "
+                           "model at the wave-maker's depth");
+  const double k = lw.k, c = lw.group_speed;
+  const double ky = k * sin(m.angle);
+  const double kx = k * cos(m.angle);
+  const double rolloff = exp(0.5 * kx * kx * m.sigma * m.sigma);
+  const double flux = 2.0 * c * m.amplitude * cos(m.angle) * rolloff;
+  const size_t nx = (size_t)s->geo.nx;
+  double norm = -0.0;
+  for (size_t i = 0; i < nx; ++i) {
+    const double x = ((double)i + 0.5) * s->geo.dx;
+    const double d = x - m.x;
+    norm = norm + exp(-(d * d) / (2.0 * m.sigma * m.sigma)) * s->geo.dx;
+  }
+  const double strength = flux / norm;
+  // strength * profile per column where the profile is at least 1e-12 (a Gaussian: one run).
+  std::vector<double> col(nx, 0.0);
+  int i0 = -1, i1 = -1;
+  for (size_t i = 0; i < nx; ++i) {
+    const double x = ((double)i + 0.5) * s->geo.dx;  // Grid::centre (grid.rs:52-57)
+    const double d = x - m.x;
+    const double profile = exp(-(d * d) / (2.0 * m.sigma * m.sigma));
+    if (!(profile < 1e-12)) {
+      if (i0 < 0) i0 = (int)i;
+      i1 = (int)i + 1;
+      col[i] = strength * profile;
+    }
+  }
+  for (int i = std::max(i0, 0); i < i1; ++i) {
+    const double x = ((double)i + 0.5) * s->geo.dx;
+    const double d = x - m.x;
+    if (exp(-(d * d) / (2.0 * m.sigma * m.sigma)) < 1e-12)
+      return fail(WC_EINVAL, "internal: the wave-maker's columns are not one run");
+  }
+  if (!s->src_col) TRY(dalloc(s, &s->src_col, nx));
+  HIP_OK(hipMemcpyAsync(s->src_col, col.data(), nx * sizeof(double), hipMemcpyHostToDevice,
+                        s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  s->omega = omega;
+  s->ky = ky;
+  s->strength = strength;
+  s->src_i0 = i0 < 0 ? 0 : i0;
+  s->src_i1 = i0 < 0 ? 0 : i1;
+  s->has_drive = true;
+  return WC_OK;
+}
+
+// solver.rs:421-429
+int check_periodic(wc_solver* s) {
+  if (s->periodic && s->has_drive) {
+    const double turns = s->ky * (double)s->geo.ny * s->geo.dy / TAU;
+    if (!(fabs(turns - round(turns)) < 1e-6)) {
+      char buf[200];
+      snprintf(buf, sizeof buf,
+               "wave-maker does not fit the periodic domain: k_y * Ly / 2pi = %.17g, must be a "
+               "whole number",
+               turns);
+      return fail(WC_EINVAL, buf);
+    }
+  }
+  return WC_OK;
+}
+
+// solver.rs:400-409
+int rebuild_drive(wc_solver* s) {
+  if (!s->has_maker) return WC_OK;
+  const Geo& g = s->geo;
+  double fi = round(s->maker.x / g.dx - 0.5);
+  fi = wc_max(fi, 0.0);
+  size_t i = fi >= 1.8e19 ? (size_t)-1 : (size_t)fi;
+  if (i > (size_t)g.nx - 1) i = (size_t)g.nx - 1;
+  i += GH;
+  const double depth = -s->bed[(size_t)(GH + g.ny / 2) * g.w + i];
+  TRY(build_drive(s, depth));
+  return check_periodic(s);
+}
+
+// Per-step values the host evaluates (glibc exp, sin, cos, as the reference calls them): the
+// ramp and phase of the wave-maker at t and t + dt, and each sponge's exp(-rate dt).
+int upload_step_params(wc_solver* s, double t, double dt) {
+  const int ny = s->geo.ny;
+  const size_t need = s->prm_len;
+  if (need == 0) return WC_OK;
+  const int slot = s->prm_slot;
+  s->prm_slot = (s->prm_slot + 1) % PRM_SLOTS;
+  if (s->prm_used[slot]) HIP_OK(hipEventSynchronize(s->prm_ev[slot]));
+  double* p = s->prm_host[slot];
+  if (s->has_drive) {
+    const wc_wavemaker_t& m = s->maker;
+    for (int st = 0; st < 2; ++st) {
+      const double tt = st == 0 ? t : t + dt;
+      const double ramp_time = m.ramp_periods * m.period;
+      p[st] = tt >= ramp_time ? 1.0 : 0.5 * (1.0 - cos(PI * tt / ramp_time));
+      double* row = p + 2 + (size_t)st * ny;
+      for (int j = 0; j < ny; ++j) {
+        const double y = ((double)j + 0.5) * s->geo.dy;
+        row[j] = sin(s->omega * tt - s->ky * y);
+      }
+    }
+  }
+  double* f = p + 2 + 2 * (size_t)ny;
+  for (size_t q = 0; q < s->sponge_rates.size(); ++q) {
+    const double rate = s->sponge_rates[q];
+    f[q] = rate > 0.0 ? exp(-rate * dt) : 1.0;
+  }
+  HIP_OK(hipMemcpyAsync(s->prm, p, need * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  HIP_OK(hipEventRecord(s->prm_ev[slot], s->stream));
+  s->prm_used[slot] = true;
+  return WC_OK;
+}
+
+int resize_params(wc_solver* s) {
+  const size_t need = 2 + 2 * (size_t)s->geo.ny + s->sponge_rates.size();
+  s->prm_len = need;
+  if (need <= s->prm_cap) return WC_OK;
+  HIP_OK(hipStreamSynchronize(s->stream));
+  for (int q = 0; q < PRM_SLOTS; ++q) {
+    if (s->prm_host[q]) HIP_OK(hipHostFree(s->prm_host[q]));
+    HIP_OK(hipHostMalloc((void**)&s->prm_host[q], need * sizeof(double), hipHostMallocDefault));
+    memset(s->prm_host[q], 0, need * sizeof(double));
+    s->prm_used[q] = false;
+  }
+  void* d = nullptr;
+  HIP_OK(hipMalloc(&d, need * sizeof(double)));
+  HIP_OK(hipMemsetAsync(d, 0, need * sizeof(double), s->stream));
+  if (s->prm) {
+    auto it = std::find(s->owned.begin(), s->owned.end(), (void*)s->prm);
+    if (it != s->owned.end()) s->owned.erase(it);
+    HIP_OK(hipFree(s->prm));
+  }
+  s->prm = static_cast<double*>(d);
+  s->owned.push_back(d);
+  s->prm_cap = need;
+  return WC_OK;
+}
+
+StepArgs step_args(wc_solver* s, double dt, int stage) {
+  StepArgs a{};
+  a.g = s->geo;
+  a.order = s->order;
+  a.dt = dt;
+  a.b = s->b;
+  a.phi = s->phi;
+  a.s_h = s->h;
+  a.s_hu = s->hu;
+  a.s_hv = s->hv;
+  a.s1_h = s->h1;
+  a.s1_hu = s->hu1;
+  a.s1_hv = s->hv1;
+  a.in_h = stage == 0 ? s->h : s->h1;
+  a.in_hu = stage == 0 ? s->hu : s->hu1;
+  a.in_hv = stage == 0 ? s->hv : s->hv1;
+  a.lh = s->lh;
+  a.lhu = s->lhu;
+  a.lhv = s->lhv;
+  a.has_drive = s->has_drive ? 1 : 0;
+  a.src_i0 = s->src_i0;
+  a.src_i1 = s->src_i1;
+  a.src_col = s->src_col;
+  a.prm = s->prm;
+  a.ramp_prm = stage;
+  a.sin_prm = 2 + stage * s->geo.ny;
+  a.sponge_prm = 2 + 2 * s->geo.ny;
+  a.n_sponges = (int)s->sponges.size();
+  for (size_t m = 0; m < s->sponges.size(); ++m) {
+    const HostSponge& hs = s->sponges[m];
+    a.sponges[m] = SpongeDev{hs.offset, hs.len, hs.level, hs.west ? 1 : 0, hs.east ? 1 : 0,
+                             hs.south ? 1 : 0, hs.north ? 1 : 0};
+  }
+  a.sponge_rate = s->sponge_rate_d;
+  a.manning = s->manning;
+  return a;
+}
+
+SgnArgs sgn_args(wc_solver* s, int stage) {
+  SgnArgs a{};
+  a.g = s->geo;
+  a.h = stage == 0 ? s->h : s->h1;
+  a.hu = stage == 0 ? s->hu : s->hu1;
+  a.hv = stage == 0 ? s->hv : s->hv1;
+  a.phi = s->phi;
+  a.lh = s->lh;
+  a.lhu = s->lhu;
+  a.lhv = s->lhv;
+  a.c = s->c;
+  a.dgx = s->dgx;
+  a.dgy = s->dgy;
+  a.bx = s->bx;
+  a.by = s->by;
+  a.wx = s->wx;
+  a.wy = s->wy;
+  a.rx = s->rx;
+  a.ry = s->ry;
+  a.px = s->px;
+  a.py = s->py;
+  a.ax = s->ax;
+  a.ay = s->ay;
+  a.part = s->part;
+  a.sc = s->sc;
+  a.nonlinear = s->disp.nonlinear ? 1 : 0;
+  a.tolerance = s->disp.tolerance;
+  a.max_iterations = s->disp.max_iterations;
+  return a;
+}
+
+// The PCG loop (dispersion.rs:493-545). Iterations run in batches; every kernel of an
+// iteration returns at once when the device-side flag says the solve is over, so the host
+// only looks at the flag between batches. The first batch is as long as the last solve.
+int pcg(wc_solver* s, int stage) {
+  const SgnArgs a = sgn_args(s, stage);
+  TRY(launch(s, F_CG_INIT, k_cg_init, tiles(s), dim3(NT), a));
+  const unsigned long long maxit = s->disp.max_iterations;
+  unsigned long long launched = 0;
+  unsigned long long batch = std::max<unsigned long long>(s->prev_it[stage], 1);
+  bool seen = false;
+  while (launched < maxit) {
+    const unsigned long long n = std::min(batch, maxit - launched);
+    for (unsigned long long q = 0; q < n; ++q) {
+      TRY(launch(s, F_CG_MATVEC, k_cg_matvec, tiles(s), dim3(NT), a));
+      TRY(launch(s, F_CG_UPDATE, k_cg_update, tiles(s), dim3(NT), a));
+      TRY(launch(s, F_CG_PUPDATE, k_cg_pupdate, tiles(s), dim3(NT), a));
+    }
+    launched += n;
+    HIP_OK(hipMemcpyAsync(s->sc_host, s->sc, sizeof(CgScalars), hipMemcpyDeviceToHost,
+                          s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    seen = true;
+    if (s->sc_host->done) break;
+    batch = 4;
+  }
+  if (!seen) {
+    HIP_OK(hipMemcpyAsync(s->sc_host, s->sc, sizeof(CgScalars), hipMemcpyDeviceToHost,
+                          s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+  }
+  s->prev_it[stage] = s->sc_host->it;
+  TRY(launch(s, F_CG_STATS, k_cg_stats, dim3(1), dim3(64), s->sc));
+  return WC_OK;
+}
+
+// One evaluation of the rates at stage `stage` and the update they drive (Solver::rhs,
+// solver.rs:625-733, then advance or the RK2 average).
+int stage(wc_solver* s, double dt, int stage, int mode) {
+  const Geo& g = s->geo;
+  const double* in_h = stage == 0 ? s->h : s->h1;
+  if (s->order == WC_ORDER_FIFTH || s->dispersive)
+    TRY(launch(s, F_SWITCH, k_switch, over((long long)g.w * g.rows), dim3(NT), g, in_h,
+               (const double*)s->b, breaking_of(s), (const uint8_t*)(s->has_held ? s->held : nullptr),
+               s->phi));
+  const StepArgs a = step_args(s, dt, stage);
+  if (!s->dispersive) {
+    TRY(launch(s, F_FLUX, k_flux, tiles(s), dim3(NT), a, mode));
+    return WC_OK;
+  }
+  TRY(launch(s, F_FLUX, k_flux, tiles(s), dim3(NT), a, (int)MODE_RATES));
+  TRY(launch(s, F_SGN_SETUP, k_sgn_setup, tiles(s), dim3(NT), sgn_args(s, stage)));
+  TRY(pcg(s, stage));
+  TRY(launch(s, F_SGN_FINISH, k_sgn_finish, tiles(s), dim3(NT), a, (const double*)s->wx,
+             (const double*)s->wy, mode));
+  return WC_OK;
+}
+
+int fill(wc_solver* s, double* h, double* hu, double* hv, double* br, double* ah, double* ahu,
+         double* ahv) {
+  const Geo& g = s->geo;
+  return launch(s, F_FILL, k_fill_state, over(6LL * g.w + 6LL * g.ny), dim3(NT), g, h, hu, hv,
+                br, ah, ahu, ahv);
+}
+
+// Solver::step (solver.rs:481-507).
+int do_step(wc_solver* s, double dt) {
+  const double t = s->time;
+  TRY(upload_step_params(s, t, dt));
+  TRY(fill(s, s->h, s->hu, s->hv, s->br, nullptr, nullptr, nullptr));
+  if (s->order == WC_ORDER_FIRST) {
+    TRY(stage(s, dt, 0, MODE_S1));
+    TRY(launch(s, F_EULER, k_finish_euler, tiles(s), dim3(NT), step_args(s, dt, 0)));
+  } else {
+    TRY(stage(s, dt, 0, MODE_S1));
+    TRY(fill(s, s->h1, s->hu1, s->hv1, nullptr, s->h, s->hu, s->hv));
+    TRY(stage(s, dt, 1, MODE_FINAL));
+  }
+  s->time = t + dt;
+  return WC_OK;
+}
+
+int stable_dt(wc_solver* s, double* dt) {
+  const Geo& g = s->geo;
+  HIP_OK(hipMemsetAsync(s->red64, 0, sizeof(unsigned long long), s->stream));
+  TRY(launch(s, F_REDUCE, k_max_rate, over((long long)g.nx * g.ny), dim3(NT), g,
+             (const double*)s->h, (const double*)s->hu, (const double*)s->hv, s->red64));
+  HIP_OK(hipMemcpyAsync(s->red64_host, s->red64, sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  double rate;
+  const unsigned long long bits = s->red64_host[0];
+  memcpy(&rate, &bits, sizeof rate);
+  if (isnan(rate))
+    *dt = NAN;
+  else if (rate > 0.0)
+    *dt = s->cfl / rate;
+  else
+    *dt = INFINITY;
+  return WC_OK;
+}
+
+int alloc_dispersion(wc_solver* s) {
+  if (s->c) return WC_OK;
+  const size_t n = s->cells;
+  for (double** p : {&s->c, &s->dgx, &s->dgy, &s->bx, &s->by, &s->wx, &s->wy, &s->rx, &s->ry,
+                     &s->px, &s->py, &s->ax, &s->ay})
+    TRY(dalloc(s, p, n));
+  return WC_OK;
+}
+
+}  // namespace
+
+// ================================================================== the C ABI
+extern "C" {
+
+const char* wc_last_error(void) { return g_err.c_str(); }
+
+int wc_solver_new(uint64_t nx, uint64_t ny, double dx, double dy, const double* bed,
+                  wc_solver** out) {
+  if (!out) return fail(WC_EINVAL, "null output pointer");
+  *out = nullptr;
+  if (!(nx > 0 && ny > 0)) return fail(WC_EINVAL, "grid must have at least one interior cell");
+  if (!(dx > 0.0 && dy > 0.0)) return fail(WC_EINVAL, "cell size must be positive");
+  if (nx < 3 || ny < 3)
+    return fail(WC_EINVAL, "this build needs at least 3 interior cells along x and along y");
+  if ((nx + 6) * (ny + 6) > (uint64_t)0x7fffffff / 2)
+    return fail(WC_EINVAL, "grid too large for 32-bit cell indices");
+  if (!bed) return fail(WC_EINVAL, "null bed");
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIP_OK(hipGetDeviceProperties(&prop, dev));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(WC_EDEVICE, std::string("libwavecore_hip is built for gfx950, device is ") +
+                                prop.gcnArchName);
+  auto* s = new wc_solver();
+  s->device = dev;
+  Geo& g = s->geo;
+  g.nx = (int)nx;
+  g.ny = (int)ny;
+  g.w = (int)nx + 2 * GH;
+  g.rows = (int)ny + 2 * GH;
+  g.dx = dx;
+  g.dy = dy;
+  g.periodic = 0;
+  g.ntx = (g.nx + TX - 1) / TX;
+  g.nty = (g.ny + TY - 1) / TY;
+  s->cells = (size_t)g.w * g.rows;
+  s->bed.assign(bed, bed + s->cells);
+  auto cleanup = [&](int rc) {
+    wc_solver_free(s);
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(fail(WC_EDEVICE, "hipStreamCreate failed"));
+  const size_t n = s->cells;
+  int rc = WC_OK;
+  for (double** p : {&s->h, &s->hu, &s->hv, &s->br, &s->h1, &s->hu1, &s->hv1, &s->b, &s->phi,
+                     &s->lh, &s->lhu, &s->lhv})
+    if ((rc = dalloc(s, p, n)) != WC_OK) return cleanup(rc);
+  const size_t ntiles = (size_t)g.ntx * g.nty;
+  if ((rc = dalloc(s, &s->part, 4 * ntiles)) != WC_OK) return cleanup(rc);
+  if ((rc = dalloc(s, &s->sc, 1)) != WC_OK) return cleanup(rc);
+  if ((rc = dalloc(s, &s->red64, 4)) != WC_OK) return cleanup(rc);
+  if ((rc = dalloc(s, &s->red32, 4)) != WC_OK) return cleanup(rc);
+  if ((rc = dalloc(s, &s->redf, 4)) != WC_OK) return cleanup(rc);
+  if (hipMemcpyAsync(s->b, s->bed.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                     s->stream) != hipSuccess)
+    return cleanup(fail(WC_EDEVICE, "bed upload failed"));
+  if (hipHostMalloc((void**)&s->sc_host, sizeof(CgScalars), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&s->red64_host, 4 * sizeof(unsigned long long),
+                    hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&s->red32_host, 4 * sizeof(unsigned int), hipHostMallocDefault) !=
+          hipSuccess ||
+      hipHostMalloc((void**)&s->redf_host, 4 * sizeof(double), hipHostMallocDefault) !=
+          hipSuccess)
+    return cleanup(fail(WC_EDEVICE, "pinned allocation failed"));
+  for (int q = 0; q < PRM_SLOTS; ++q)
+    if (hipEventCreateWithFlags(&s->prm_ev[q], hipEventDisableTiming) != hipSuccess)
+      return cleanup(fail(WC_EDEVICE, "hipEventCreate failed"));
+  if ((rc = resize_params(s)) != WC_OK) return cleanup(rc);
+  if (hipStreamSynchronize(s->stream) != hipSuccess)
+    return cleanup(fail(WC_EDEVICE, "initial synchronisation failed"));
+  *out = s;
+  return WC_OK;
+}
+
+void wc_solver_free(wc_solver* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (void* p : s->owned) (void)hipFree(p);
+  for (int q = 0; q < PRM_SLOTS; ++q) {
+    if (s->prm_host[q]) (void)hipHostFree(s->prm_host[q]);
+    if (s->prm_ev[q]) (void)hipEventDestroy(s->prm_ev[q]);
+  }
+  if (s->sc_host) (void)hipHostFree(s->sc_host);
+  if (s->red64_host) (void)hipHostFree(s->red64_host);
+  if (s->red32_host) (void)hipHostFree(s->red32_host);
+  if (s->redf_host) (void)hipHostFree(s->redf_host);
+  for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
+  for (auto& p : s->ev_pending) {
+    (void)hipEventDestroy(p.second.first);
+    (void)hipEventDestroy(p.second.second);
+  }
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+int wc_solver_set_cfl(wc_solver* s, double cfl) {
+  s->cfl = cfl;
+  return WC_OK;
+}
+
+int wc_solver_cfl(const wc_solver* s, double* cfl) {
+  *cfl = s->cfl;
+  return WC_OK;
+}
+
+int wc_solver_with_order(wc_solver* s, int32_t order) {
+  if (order < WC_ORDER_FIRST || order > WC_ORDER_FIFTH)
+    return fail(WC_EINVAL, "order must be 0 (first), 1 (second) or 2 (fifth)");
+  s->order = order;
+  return WC_OK;
+}
+
+int wc_solver_with_manning(wc_solver* s, double n) {
+  s->manning = n;
+  return WC_OK;
+}
+
+int wc_solver_with_sponge(wc_solver* s, const wc_sponge_t* sp) {
+  TRY(set_device(s));
+  if ((int)s->sponges.size() >= MAX_SPONGES)
+    return fail(WC_EINVAL, "at most 16 sponges in this build");
+  HostSponge hs{};
+  hs.width = sp->width;
+  hs.strength = sp->strength;
+  hs.level = sp->level;
+  hs.west = sp->west != 0;
+  hs.east = sp->east != 0;
+  hs.south = sp->south != 0;
+  hs.north = sp->north != 0;
+  // Sponge::rate (forcing.rs:181-206) only ever sees distances below max(nx, ny).
+  const uint64_t span = (uint64_t)std::max(s->geo.nx, s->geo.ny);
+  hs.len = (long long)std::min<uint64_t>(hs.width, span);
+  hs.offset = (long long)s->sponge_rates.size();
+  const double w = (double)hs.width;
+  for (long long d = 0; d < hs.len; ++d) {
+    const double f = (w - (double)d) / w;
+    s->sponge_rates.push_back(hs.strength * (f * f));
+  }
+  s->sponges.push_back(hs);
+  HIP_OK(hipStreamSynchronize(s->stream));
+  if (s->sponge_rate_d) {
+    auto it = std::find(s->owned.begin(), s->owned.end(), (void*)s->sponge_rate_d);
+    if (it != s->owned.end()) s->owned.erase(it);
+    HIP_OK(hipFree(s->sponge_rate_d));
+    s->sponge_rate_d = nullptr;
+  }
+  TRY(dalloc(s, &s->sponge_rate_d, s->sponge_rates.size()));
+  HIP_OK(hipMemcpyAsync(s->sponge_rate_d, s->sponge_rates.data(),
+                        s->sponge_rates.size() * sizeof(double), hipMemcpyHostToDevice,
+                        s->stream));
+  TRY(resize_params(s));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  return WC_OK;
+}
+
+int wc_solver_with_wavemaker(wc_solver* s, const wc_wavemaker_t* m) {
+  TRY(set_device(s));
+  s->maker = *m;
+  s->has_maker = true;
+  return rebuild_drive(s);
+}
+
+int wc_solver_with_dispersion(wc_solver* s, const wc_dispersion_t* d) {
+  TRY(set_device(s));
+  if (d->reserved != 0) return fail(WC_EINVAL, "wc_dispersion_t.reserved must be 0");
+  s->disp = *d;
+  s->dispersive = true;
+  TRY(alloc_dispersion(s));
+  // Work::new (dispersion.rs:237-266): fresh scratch and statistics.
+  for (double* p : {s->c, s->dgx, s->dgy, s->bx, s->by, s->wx, s->wy, s->rx, s->ry, s->px, s->py,
+                    s->ax, s->ay})
+    HIP_OK(hipMemsetAsync(p, 0, s->cells * sizeof(double), s->stream));
+  HIP_OK(hipMemsetAsync(s->sc, 0, sizeof(CgScalars), s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  s->prev_it[0] = s->prev_it[1] = 0;
+  return rebuild_drive(s);
+}
+
+int wc_solver_with_periodic_y(wc_solver* s) {
+  TRY(set_device(s));
+  s->periodic = true;
+  s->geo.periodic = 1;
+  // wrap_ghosts_y on the bed (grid.rs:79-87)
+  const int w = s->geo.w, ny = s->geo.ny;
+  for (int i = 0; i < w; ++i)
+    for (int k = 0; k < GH; ++k) {
+      s->bed[(size_t)(GH - 1 - k) * w + i] = s->bed[(size_t)(GH + ny - 1 - k) * w + i];
+      s->bed[(size_t)(GH + ny + k) * w + i] = s->bed[(size_t)(GH + k) * w + i];
+    }
+  HIP_OK(hipMemcpyAsync(s->b, s->bed.data(), s->cells * sizeof(double), hipMemcpyHostToDevice,
+                        s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  return check_periodic(s);
+}
+
+int wc_solver_with_shallow_cells(wc_solver* s, const uint8_t* cells, uint64_t n) {
+  TRY(set_device(s));
+  if (n != (uint64_t)s->geo.nx * (uint64_t)s->geo.ny)
+    return fail(WC_EINVAL, "one entry per interior cell");
+  if (!s->held) TRY(dalloc(s, &s->held, n));
+  std::vector<uint8_t> v(n);
+  for (uint64_t q = 0; q < n; ++q) v[q] = cells[q] ? 1 : 0;
+  HIP_OK(hipMemcpyAsync(s->held, v.data(), n, hipMemcpyHostToDevice, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  s->has_held = true;
+  return WC_OK;
+}
+
+int wc_solver_bed(const wc_solver* s, double* out) {
+  memcpy(out, s->bed.data(), s->cells * sizeof(double));
+  return WC_OK;
+}
+
+int wc_solver_set_precision(wc_solver* s, int32_t precision) {
+  if (precision != WC_PRECISION_FP64) return fail(WC_EINVAL, "unknown precision");
+  s->precision = precision;
+  return WC_OK;
+}
+
+int wc_state_set(wc_solver* s, const double* h, const double* hu, const double* hv,
+                 const double* breaking, double time) {
+  TRY(set_device(s));
+  const size_t bytes = s->cells * sizeof(double);
+  HIP_OK(hipMemcpyAsync(s->h, h, bytes, hipMemcpyHostToDevice, s->stream));
+  HIP_OK(hipMemcpyAsync(s->hu, hu, bytes, hipMemcpyHostToDevice, s->stream));
+  HIP_OK(hipMemcpyAsync(s->hv, hv, bytes, hipMemcpyHostToDevice, s->stream));
+  if (breaking)
+    HIP_OK(hipMemcpyAsync(s->br, breaking, bytes, hipMemcpyHostToDevice, s->stream));
+  else
+    HIP_OK(hipMemsetAsync(s->br, 0, bytes, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  s->time = time;
+  return WC_OK;
+}
+
+int wc_state_get(wc_solver* s, double* h, double* hu, double* hv, double* breaking,
+                 double* time) {
+  TRY(set_device(s));
+  const size_t bytes = s->cells * sizeof(double);
+  if (h) HIP_OK(hipMemcpyAsync(h, s->h, bytes, hipMemcpyDeviceToHost, s->stream));
+  if (hu) HIP_OK(hipMemcpyAsync(hu, s->hu, bytes, hipMemcpyDeviceToHost, s->stream));
+  if (hv) HIP_OK(hipMemcpyAsync(hv, s->hv, bytes, hipMemcpyDeviceToHost, s->stream));
+  if (breaking) HIP_OK(hipMemcpyAsync(breaking, s->br, bytes, hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  if (time) *time = s->time;
+  return WC_OK;
+}
+
+int wc_state_set_time(wc_solver* s, double time) {
+  s->time = time;
+  return WC_OK;
+}
+
+int wc_stable_dt(wc_solver* s, double* dt) {
+  TRY(set_device(s));
+  return stable_dt(s, dt);
+}
+
+int wc_step(wc_solver* s, double dt) {
+  TRY(set_device(s));
+  TRY(do_step(s, dt));
+  if (!s->prof) HIP_OK(hipStreamSynchronize(s->stream));
+  return WC_OK;
+}
+
+int wc_run_steps(wc_solver* s, uint64_t steps, uint64_t* done_steps, double* last_dt) {
+  TRY(set_device(s));
+  uint64_t n = 0;
+  double dt = 0.0;
+  for (; n < steps; ++n) {
+    TRY(stable_dt(s, &dt));
+    if (!(isfinite(dt) && dt > 0.0)) break;
+    TRY(do_step(s, dt));
+  }
+  HIP_OK(hipStreamSynchronize(s->stream));
+  if (done_steps) *done_steps = n;
+  if (last_dt) *last_dt = dt;
+  return WC_OK;
+}
+
+int wc_limiting_cell(wc_solver* s, int32_t* found, wc_limiting_t* out) {
+  TRY(set_device(s));
+  const Geo& g = s->geo;
+  const long long n = (long long)g.nx * g.ny;
+  HIP_OK(hipMemsetAsync(s->red64, 0, sizeof(unsigned long long), s->stream));
+  HIP_OK(hipMemsetAsync(s->red32 + 1, 0xff, 2 * sizeof(unsigned int), s->stream));
+  TRY(launch(s, F_REDUCE, k_limit_scan, over(n), dim3(NT), g, (const double*)s->h,
+             (const double*)s->hu, (const double*)s->hv, s->red64, s->red32 + 1));
+  HIP_OK(hipMemcpyAsync(s->red64_host, s->red64, sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipMemcpyAsync(s->red32_host, s->red32, 4 * sizeof(unsigned int), hipMemcpyDeviceToHost,
+                        s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  unsigned int cell = s->red32_host[1];
+  if (cell == 0xffffffffu) {
+    if (s->red64_host[0] == 0ull) {
+      *found = 0;
+      return WC_OK;
+    }
+    TRY(launch(s, F_REDUCE, k_limit_first, over(n), dim3(NT), g, (const double*)s->h,
+               (const double*)s->hu, (const double*)s->hv,
+               (const unsigned long long*)s->red64, s->red32 + 2));
+    HIP_OK(hipMemcpyAsync(s->red32_host, s->red32, 4 * sizeof(unsigned int),
+                          hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    cell = s->red32_host[2];
+  }
+  const size_t i = GH + cell % (unsigned)g.nx, j = GH + cell / (unsigned)g.nx;
+  const size_t k = j * (size_t)g.w + i;
+  double v[3];
+  HIP_OK(hipMemcpyAsync(&v[0], s->h + k, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipMemcpyAsync(&v[1], s->hu + k, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipMemcpyAsync(&v[2], s->hv + k, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  const double u = wc_velocity(v[0], v[1]);
+  const double vv = wc_velocity(v[0], v[2]);
+  *found = 1;
+  out->i = i - GH;
+  out->j = j - GH;
+  out->depth = v[0];
+  out->speed = wc_hypot(u, vv);
+  return WC_OK;
+}
+
+int wc_breaking_indicator(wc_solver* s, double* out) {
+  TRY(set_device(s));
+  const Geo& g = s->geo;
+  const size_t n = (size_t)g.nx * g.ny;
+  TRY(launch(s, F_REDUCE, k_breaking, over((long long)n), dim3(NT), g, (const double*)s->h,
+             (const double*)s->b, breaking_of(s), s->lh));
+  HIP_OK(hipMemcpyAsync(out, s->lh, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  return WC_OK;
+}
+
+int wc_dispersion_stats(wc_solver* s, int32_t* has, wc_dispersion_stats_t* out) {
+  TRY(set_device(s));
+  *has = s->dispersive ? 1 : 0;
+  if (!s->dispersive) return WC_OK;
+  HIP_OK(hipMemcpyAsync(s->sc_host, s->sc, sizeof(CgScalars), hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  out->solves = s->sc_host->solves;
+  out->iterations = s->sc_host->iterations;
+  out->unconverged = s->sc_host->unconverged;
+  return WC_OK;
+}
+
+static int state_sums(wc_solver* s) {
+  const Geo& g = s->geo;
+  HIP_OK(hipMemsetAsync(s->red64 + 1, 0, sizeof(unsigned long long), s->stream));
+  HIP_OK(hipMemsetAsync(s->red64 + 2, 0xff, sizeof(unsigned long long), s->stream));
+  HIP_OK(hipMemsetAsync(s->red32, 0, sizeof(unsigned int), s->stream));
+  TRY(launch(s, F_REDUCE, k_state_sums, tiles(s), dim3(NT), g, (const double*)s->h,
+             (const double*)s->hu, (const double*)s->hv, s->part, s->red32, s->redf,
+             s->red64 + 1, s->red64 + 2));
+  HIP_OK(hipMemcpyAsync(s->red64_host, s->red64, 4 * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipMemcpyAsync(s->redf_host, s->redf, sizeof(double), hipMemcpyDeviceToHost,
+                        s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  return WC_OK;
+}
+
+int wc_state_volume(wc_solver* s, double* out) {
+  TRY(set_device(s));
+  TRY(state_sums(s));
+  *out = s->redf_host[0] * s->geo.dx * s->geo.dy;
+  return WC_OK;
+}
+
+int wc_state_max_abs_momentum(wc_solver* s, double* out) {
+  TRY(set_device(s));
+  TRY(state_sums(s));
+  const unsigned long long bits = s->red64_host[1];
+  memcpy(out, &bits, sizeof(double));
+  return WC_OK;
+}
+
+int wc_state_min_depth(wc_solver* s, double* out) {
+  TRY(set_device(s));
+  TRY(state_sums(s));
+  unsigned long long key = s->red64_host[2];
+  if (key == 0xffffffffffffffffull) {
+    *out = INFINITY;
+    return WC_OK;
+  }
+  const unsigned long long bits = (key >> 63) ? (key & 0x7fffffffffffffffull) : ~key;
+  memcpy(out, &bits, sizeof(double));
+  return WC_OK;
+}
+
+int wc_linear_wave(double omega, double h, int32_t dispersive, wc_linear_wave_t* out) {
+  if (!linear_wave(omega, h, dispersive != 0, out))
+    return fail(WC_EINVAL, "a wave of this angular frequency is too short for the dispersive "
+                           "model in this depth");
+  return WC_OK;
+}
+
+double wc_manning_factor(double h, double speed, double dt, double n) {
+  const double k = WC_G * n * n / pow(h, 4.0 / 3.0);
+  return 1.0 / (1.0 + dt * k * speed);
+}
+
+int wc_profile_enable(wc_solver* s, int32_t on) {
+  TRY(set_device(s));
+  TRY(prof_collect(s));
+  s->prof = on != 0;
+  return WC_OK;
+}
+
+int wc_profile_reset(wc_solver* s) {
+  TRY(set_device(s));
+  TRY(prof_collect(s));
+  for (int f = 0; f < F_COUNT; ++f) {
+    s->prof_ms[f] = 0.0;
+    s->prof_n[f] = 0;
+  }
+  return WC_OK;
+}
+
+int wc_profile_read(wc_solver* s, char* names, uint64_t names_len, double* total_ms,
+                    uint64_t* launches, uint64_t max_kernels, uint64_t* n_kernels) {
+  TRY(set_device(s));
+  TRY(prof_collect(s));
+  std::string all;
+  for (int f = 0; f < F_COUNT; ++f) {
+    if (f) all += ";";
+    all += kFamilyNames[f];
+    if ((uint64_t)f < max_kernels) {
+      total_ms[f] = s->prof_ms[f];
+      launches[f] = s->prof_n[f];
+    }
+  }
+  if (names && names_len) {
+    strncpy(names, all.c_str(), names_len - 1);
+    names[names_len - 1] = '\0';
+  }
+  *n_kernels = F_COUNT;
+  return WC_OK;
+}
+
+}  // extern "C"

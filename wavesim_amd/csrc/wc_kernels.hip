@@ -1,0 +1,624 @@
+// gfx950 kernels of wavecore's FP64 step. Launched by wc_api.hip; declared in wc_kernels.h.
+//
+// One thread per cell on 64x4-cell tiles (a wavefront per tile row, so every global access of
+// a row is coalesced along x), blocks mapped to tiles XCD by XCD (wc::tile_of_block). The
+// arithmetic of each kernel is that of the reference function it cites, in the same order.
+
+#include "wc_kernels.h"
+
+namespace wc {
+
+// ---- ghost layers ---------------------------------------------------------------------------
+// state.rs:68-76: h, hu, hv and breaking ghosts from the interior. With `avg`, also the ghost
+// part of the RK average of solver.rs:493-500: avg = (avg + this) / 2 for h, hu, hv, because
+// the second stage's state carries the first stage's ghosts (solver.rs:490-491, 594-595).
+__global__ void __launch_bounds__(NT) k_fill_state(Geo g, double* h, double* hu, double* hv,
+                                                   double* br, double* ah, double* ahu,
+                                                   double* ahv) {
+  const int n = 6 * g.w + 6 * g.ny;
+  const int t = blockIdx.x * NT + threadIdx.x;
+  if (t >= n) return;
+  int i, j;
+  if (t < 3 * g.w) {
+    j = t / g.w;
+    i = t % g.w;
+  } else if (t < 6 * g.w) {
+    const int u = t - 3 * g.w;
+    j = g.ny + GH + u / g.w;
+    i = u % g.w;
+  } else {
+    const int u = t - 6 * g.w;
+    j = GH + u / 6;
+    const int c = u % 6;
+    i = c < 3 ? c : g.nx + GH + (c - 3);
+  }
+  const int k = j * g.w + i;
+  const double vh = ghosted(h, g, i, j);
+  const double vu = ghosted_vx(hu, g, i, j);
+  const double vv = ghosted_vy(hv, g, i, j);
+  h[k] = vh;
+  hu[k] = vu;
+  hv[k] = vv;
+  if (br) br[k] = ghosted(br, g, i, j);
+  if (ah) {
+    ah[k] = 0.5 * (ah[k] + vh);
+    ahu[k] = 0.5 * (ahu[k] + vu);
+    ahv[k] = 0.5 * (ahv[k] + vv);
+  }
+}
+
+// ---- the breaking switch (dispersion.rs:120-160) ----------------------------------------------
+// phi on every padded cell: a ghost takes the value of the interior cell it mirrors or wraps,
+// as the reference's mirror_ghosts/wrap_ghosts_y leave it. `held` (may be null) forces 0 on
+// the cells held to shallow water (solver.rs:640-651). h must have its ghosts filled.
+__device__ __forceinline__ double switch_at(const Geo& g, const double* h, const double* b,
+                                           const Breaking& br, int i, int j) {
+  const int k = j * g.w + i;
+  if (h[k] < WC_H_DISPERSION_MIN) return 0.0;
+  const double ix = 0.5 / g.dx, iy = 0.5 / g.dy;
+  const double ex = ((h[k + 1] + b[k + 1]) - (h[k - 1] + b[k - 1])) * ix;
+  const double ey = ((h[k + g.w] + b[k + g.w]) - (h[k - g.w] + b[k - g.w])) * iy;
+  const double ratio = wc_max((h[k] + b[k]) / h[k], 0.0);
+  const double off = wc_max(wc_smoothstep(br.er_lo, br.er_hi, ratio),
+                            wc_smoothstep(br.sl_lo, br.sl_hi, wc_hypot(ex, ey)));
+  return 1.0 - off;
+}
+
+__global__ void __launch_bounds__(NT) k_switch(Geo g, const double* h, const double* b,
+                                               Breaking br, const uint8_t* held, double* phi) {
+  const int t = blockIdx.x * NT + threadIdx.x;
+  if (t >= g.w * g.rows) return;
+  const int i = t % g.w, j = t / g.w;
+  const int mi = map_x(i, g.nx), mj = map_y(j, g.ny, g.periodic);
+  double v;
+  if (held && held[(mj - GH) * g.nx + (mi - GH)]) {
+    v = 0.0;
+  } else {
+    v = switch_at(g, h, b, br, mi, mj);
+  }
+  phi[t] = v;
+}
+
+// breaking_into (dispersion.rs:183-201) on a copy of the state whose ghosts are filled: the
+// ghost reads of h go through the ghost map, the state's own ghosts are left alone.
+__global__ void __launch_bounds__(NT) k_breaking(Geo g, const double* h, const double* b,
+                                                 Breaking br, double* out) {
+  const int t = blockIdx.x * NT + threadIdx.x;
+  if (t >= g.nx * g.ny) return;
+  const int i = GH + t % g.nx, j = GH + t / g.nx;
+  const int k = j * g.w + i;
+  const double hc = h[k];
+  if (hc < WC_H_DISPERSION_MIN) {
+    out[t] = 0.0;
+    return;
+  }
+  const double ix = 0.5 / g.dx, iy = 0.5 / g.dy;
+  const double ex = ((ghosted(h, g, i + 1, j) + b[k + 1]) - (ghosted(h, g, i - 1, j) + b[k - 1])) * ix;
+  const double ey =
+      ((ghosted(h, g, i, j + 1) + b[k + g.w]) - (ghosted(h, g, i, j - 1) + b[k - g.w])) * iy;
+  const double ratio = wc_max((hc + b[k]) / hc, 0.0);
+  const double off = wc_max(wc_smoothstep(br.er_lo, br.er_hi, ratio),
+                            wc_smoothstep(br.sl_lo, br.sl_hi, wc_hypot(ex, ey)));
+  out[t] = 1.0 - (1.0 - off);
+}
+
+// ---- the stage update ----------------------------------------------------------------------------
+// clean_dry (solver.rs:577-591)
+__device__ __forceinline__ void clean_dry(double& h, double& hu, double& hv) {
+  if (h < WC_H_DRY) {
+    h = wc_max(h, 0.0);
+    hu = 0.0;
+    hv = 0.0;
+  }
+}
+
+// damp (solver.rs:535-574): the sponges in order, then Manning friction, then clean_dry.
+__device__ __forceinline__ void damp(const StepArgs& a, int i, int j, double b, double& h,
+                                     double& hu, double& hv) {
+  const int ii = i - GH, jj = j - GH;
+  for (int m = 0; m < a.n_sponges; ++m) {
+    const SpongeDev& sp = a.sponges[m];
+    long long d = 0x7fffffffffffffffLL;
+    if (sp.west) d = d < ii ? d : ii;
+    if (sp.east) d = d < (a.g.nx - 1 - ii) ? d : (a.g.nx - 1 - ii);
+    if (sp.south) d = d < jj ? d : jj;
+    if (sp.north) d = d < (a.g.ny - 1 - jj) ? d : (a.g.ny - 1 - jj);
+    if (d < sp.len) {
+      const double rate = a.sponge_rate[sp.offset + d];
+      if (rate > 0.0) {
+        const double f = a.prm[a.sponge_prm + sp.offset + d];
+        const double ht = wc_max(sp.level - b, 0.0);
+        h = f * h + (1.0 - f) * ht;
+        hu = f * hu + (1.0 - f) * ht * 0.0;
+        hv = f * hv + (1.0 - f) * ht * 0.0;
+      }
+    }
+  }
+  if (a.manning > 0.0 && h > WC_H_FRICTION_MIN) {
+    const double speed = wc_hypot(hu, hv) / h;
+    const double kk = WC_G * a.manning * a.manning / wc_pow43(h);
+    const double fac = 1.0 / (1.0 + a.dt * kk * speed);
+    hu *= fac;
+    hv *= fac;
+  }
+  clean_dry(h, hu, hv);
+}
+
+// Where a stage's rates go (solver.rs:481-507, 594-616):
+//   MODE_S1:    s1 = clean_dry(s + dt L)               (stage 1, and the Euler step)
+//   MODE_FINAL: s = damp(clean_dry((s + clean_dry(s1 + dt L)) / 2))  (stage 2)
+//   MODE_RATES: store L (the dispersive correction still has to be added)
+__device__ __forceinline__ void stage_out(const StepArgs& a, int mode, int i, int j, int k,
+                                          double rh, double rhu, double rhv) {
+  if (mode == MODE_RATES) {
+    a.lh[k] = rh;
+    a.lhu[k] = rhu;
+    a.lhv[k] = rhv;
+    return;
+  }
+  double h = a.in_h[k], hu = a.in_hu[k], hv = a.in_hv[k];
+  h += a.dt * rh;
+  hu += a.dt * rhu;
+  hv += a.dt * rhv;
+  clean_dry(h, hu, hv);
+  if (mode == MODE_S1) {
+    a.s1_h[k] = h;
+    a.s1_hu[k] = hu;
+    a.s1_hv[k] = hv;
+    return;
+  }
+  double sh = 0.5 * (a.s_h[k] + h);
+  double shu = 0.5 * (a.s_hu[k] + hu);
+  double shv = 0.5 * (a.s_hv[k] + hv);
+  clean_dry(sh, shu, shv);
+  damp(a, i, j, a.b[k], sh, shu, shv);
+  a.s_h[k] = sh;
+  a.s_hu[k] = shu;
+  a.s_hv[k] = shv;
+}
+
+// ---- the finite-volume rates (solver.rs:625-718) ----------------------------------------------
+__device__ __forceinline__ Cell cell_at(const StepArgs& a, int k, bool normal_is_x) {
+  const double h = a.in_h[k];
+  const double un = normal_is_x ? a.in_hu[k] : a.in_hv[k];
+  const double ut = normal_is_x ? a.in_hv[k] : a.in_hu[k];
+  return Cell{h, wc_velocity(h, un), wc_velocity(h, ut), a.b[k], a.phi[k]};
+}
+
+__global__ void __launch_bounds__(NT) k_flux(StepArgs a, int mode) {
+  const Geo& g = a.g;
+  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  const int tx = tile % g.ntx, ty = tile / g.ntx;
+  const int i = GH + tx * TX + (threadIdx.x & 63);
+  const int j = GH + ty * TY + (threadIdx.x >> 6);
+  if (i >= g.nx + GH || j >= g.ny + GH) return;
+  Cell sx[7], sy[7];
+#pragma unroll
+  for (int m = 0; m < 7; ++m) {
+    sx[m] = cell_at(a, j * g.w + (i + m - 3), true);
+    sy[m] = cell_at(a, (j + m - 3) * g.w + i, false);
+  }
+  Face xl, xh, yl, yh;
+  double xs, ys;
+  axis_terms(sx, g.dx, a.order, xl, xh, xs);
+  axis_terms(sy, g.dy, a.order, yl, yh, ys);
+  double rh = -(xh.f0 - xl.f0) / g.dx - (yh.f0 - yl.f0) / g.dy;
+  const double rhu = -((xh.f1 + xh.cl) - (xl.f1 + xl.cr)) / g.dx + xs - (yh.f2 - yl.f2) / g.dy;
+  const double rhv = -((yh.f1 + yh.cl) - (yl.f1 + yl.cr)) / g.dy + ys - (xh.f2 - xl.f2) / g.dx;
+  if (a.has_drive) {
+    // Drive::source (forcing.rs:125-138) with the host-evaluated exp, cos and sin.
+    const int ii = i - GH;
+    double src = 0.0;
+    if (ii >= a.src_i0 && ii < a.src_i1) src = a.src_col[ii] * a.prm[a.ramp_prm] * a.prm[a.sin_prm + (j - GH)];
+    rh = rh + src;
+  }
+  stage_out(a, mode, i, j, j * g.w + i, rh, rhu, rhv);
+}
+
+// The Euler step of Order::First (solver.rs:485-487): s1 holds clean_dry(s + dt L); copy it to
+// the interior of s and damp (the ghosts of s keep their filled values, as advance's clone did).
+__global__ void __launch_bounds__(NT) k_finish_euler(StepArgs a) {
+  const Geo& g = a.g;
+  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  const int tx = tile % g.ntx, ty = tile / g.ntx;
+  const int i = GH + tx * TX + (threadIdx.x & 63);
+  const int j = GH + ty * TY + (threadIdx.x >> 6);
+  if (i >= g.nx + GH || j >= g.ny + GH) return;
+  const int k = j * g.w + i;
+  double h = a.s1_h[k], hu = a.s1_hu[k], hv = a.s1_hv[k];
+  damp(a, i, j, a.b[k], h, hu, hv);
+  a.s_h[k] = h;
+  a.s_hu[k] = hu;
+  a.s_hv[k] = hv;
+}
+
+// ---- SGN dispersion (dispersion.rs:335-563) -----------------------------------------------------
+// Setup: u, v, the mask, the masked velocities and their divergence, c = phi h^3/3 and c N,
+// the Jacobi diagonal, the right-hand side b, and the warm start zeroed where masked. The tile
+// is staged in LDS with three ghost rings: um, vm on the tile + 3, div on + 2, c and f on + 1.
+constexpr int SX3 = TX + 6, SY3 = TY + 6;  // tile + 3
+constexpr int SX2 = TX + 4, SY2 = TY + 4;  // tile + 2
+constexpr int SX1 = TX + 2, SY1 = TY + 2;  // tile + 1
+
+__global__ void __launch_bounds__(NT) k_sgn_setup(SgnArgs a) {
+  const Geo& g = a.g;
+  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  const int tx = tile % g.ntx, ty = tile / g.ntx;
+  const int i0 = GH + tx * TX, j0 = GH + ty * TY;  // padded corner of the tile
+  __shared__ double s_h[SY3][SX3], s_um[SY3][SX3], s_vm[SY3][SX3];
+  __shared__ double s_div[SY2][SX2];
+  __shared__ double s_c[SY1][SX1], s_f[SY1][SX1];
+  const double ix = 0.5 / g.dx, iy = 0.5 / g.dy;
+  for (int t = threadIdx.x; t < SX3 * SY3; t += NT) {
+    const int lx = t % SX3, ly = t / SX3;
+    const int i = i0 - 3 + lx, j = j0 - 3 + ly;
+    double h = 0.0, um = 0.0, vm = 0.0;
+    if (i < g.w && j < g.rows) {
+      const int k = j * g.w + i;
+      h = a.h[k];
+      const double m = h >= WC_H_DISPERSION_MIN ? 1.0 : 0.0;
+      um = m * wc_velocity(h, a.hu[k]);
+      vm = m * wc_velocity(h, a.hv[k]);
+    }
+    s_h[ly][lx] = h;
+    s_um[ly][lx] = um;
+    s_vm[ly][lx] = vm;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < SX2 * SY2; t += NT) {
+    const int lx = t % SX2, ly = t / SX2;
+    const int x = lx + 1, y = ly + 1;  // in the +3 frame
+    s_div[ly][lx] = (s_um[y][x + 1] - s_um[y][x - 1]) * ix + (s_vm[y + 1][x] - s_vm[y - 1][x]) * iy;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < SX1 * SY1; t += NT) {
+    const int lx = t % SX1, ly = t / SX1;
+    const int i = i0 - 1 + lx, j = j0 - 1 + ly;
+    const int x3 = lx + 2, y3 = ly + 2;  // in the +3 frame
+    const int x2 = lx + 1, y2 = ly + 1;  // in the +2 frame
+    double coeff = 0.0, f = 0.0;
+    if (i <= g.nx + GH && j <= g.ny + GH) {
+      const double depth = s_h[y3][x3];
+      coeff = depth < WC_H_DISPERSION_MIN ? 0.0
+                                          : a.phi[j * g.w + i] * depth * depth * depth / 3.0;
+      if (a.nonlinear && coeff > 0.0) {
+        const double n = s_um[y3][x3] * (s_div[y2][x2 + 1] - s_div[y2][x2 - 1]) * ix +
+                         s_vm[y3][x3] * (s_div[y2 + 1][x2] - s_div[y2 - 1][x2]) * iy -
+                         s_div[y2][x2] * s_div[y2][x2];
+        f = coeff * n;
+      }
+      // c is read by the matrix product on the interior and its first ring.
+      const bool inside = i >= GH && i < g.nx + GH && j >= GH && j < g.ny + GH;
+      const bool ring1 = i >= GH - 1 && j >= GH - 1;
+      const bool mine = (lx >= 1 && lx <= TX && ly >= 1 && ly <= TY);
+      if ((mine && inside) || (!inside && ring1)) a.c[j * g.w + i] = coeff;
+    }
+    s_c[ly][lx] = coeff;
+    s_f[ly][lx] = f;
+  }
+  __syncthreads();
+  const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+  const int i = i0 + lx, j = j0 + ly;
+  if (i >= g.nx + GH || j >= g.ny + GH) return;
+  const int k = j * g.w + i;
+  const int x1 = lx + 1, y1 = ly + 1;
+  const double h = s_h[ly + 3][lx + 3];
+  const double heff = wc_max(h, WC_H_DRY);
+  const bool masked = !(h >= WC_H_DISPERSION_MIN);
+  double dgx, dgy, bx, by;
+  if (masked) {
+    dgx = 1.0;
+    dgy = 1.0;
+    bx = 0.0;
+    by = 0.0;
+    a.wx[k] = 0.0;
+    a.wy[k] = 0.0;
+  } else {
+    dgx = heff + (s_c[y1][x1 + 1] + s_c[y1][x1 - 1]) * 0.25 / (g.dx * g.dx);
+    dgy = heff + (s_c[y1 + 1][x1] + s_c[y1 - 1][x1]) * 0.25 / (g.dy * g.dy);
+    const double u = wc_velocity(h, a.hu[k]);
+    const double v = wc_velocity(h, a.hv[k]);
+    bx = a.lhu[k] - u * a.lh[k] + (s_f[y1][x1 + 1] - s_f[y1][x1 - 1]) * ix;
+    by = a.lhv[k] - v * a.lh[k] + (s_f[y1 + 1][x1] - s_f[y1 - 1][x1]) * iy;
+  }
+  a.dgx[k] = dgx;
+  a.dgy[k] = dgy;
+  a.bx[k] = bx;
+  a.by[k] = by;
+}
+
+// A p on the tile (matvec, dispersion.rs:274-317): p on the tile + 2 (ghosts by the vector
+// ghost map, as fill_vector_ghosts leaves them), c div p on the tile + 1, then
+// A p = mask (heff p - grad(c div p)). Returns this thread's (Apx, Apy, px, py).
+struct ApOut {
+  double ax, ay, px, py;
+  bool valid;
+};
+
+__device__ __forceinline__ ApOut tile_matvec(const SgnArgs& a, const double* px, const double* py,
+                                             int i0, int j0, double (*s_px)[SX2],
+                                             double (*s_py)[SX2], double (*s_f)[SX1]) {
+  const Geo& g = a.g;
+  const double ix = 0.5 / g.dx, iy = 0.5 / g.dy;
+  for (int t = threadIdx.x; t < SX2 * SY2; t += NT) {
+    const int lx = t % SX2, ly = t / SX2;
+    const int i = i0 - 2 + lx, j = j0 - 2 + ly;
+    double vx = 0.0, vy = 0.0;
+    if (i <= g.nx + GH + 1 && j <= g.ny + GH + 1) {
+      vx = ghosted_vx(px, g, i, j);
+      vy = ghosted_vy(py, g, i, j);
+    }
+    s_px[ly][lx] = vx;
+    s_py[ly][lx] = vy;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < SX1 * SY1; t += NT) {
+    const int lx = t % SX1, ly = t / SX1;
+    const int i = i0 - 1 + lx, j = j0 - 1 + ly;
+    double f = 0.0;
+    if (i <= g.nx + GH && j <= g.ny + GH) {
+      const int x = lx + 1, y = ly + 1;
+      f = a.c[j * g.w + i] *
+          ((s_px[y][x + 1] - s_px[y][x - 1]) * ix + (s_py[y + 1][x] - s_py[y - 1][x]) * iy);
+    }
+    s_f[ly][lx] = f;
+  }
+  __syncthreads();
+  ApOut o;
+  const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+  const int i = i0 + lx, j = j0 + ly;
+  o.valid = i < g.nx + GH && j < g.ny + GH;
+  o.ax = o.ay = o.px = o.py = 0.0;
+  if (o.valid) {
+    const int k = j * g.w + i;
+    const double h = a.h[k];
+    const double mask = h >= WC_H_DISPERSION_MIN ? 1.0 : 0.0;
+    const double heff = wc_max(h, WC_H_DRY);
+    const int x1 = lx + 1, y1 = ly + 1;
+    o.px = s_px[ly + 2][lx + 2];
+    o.py = s_py[ly + 2][lx + 2];
+    o.ax = mask * (heff * o.px - (s_f[y1][x1 + 1] - s_f[y1][x1 - 1]) * ix);
+    o.ay = mask * (heff * o.py - (s_f[y1 + 1][x1] - s_f[y1 - 1][x1]) * iy);
+  }
+  return o;
+}
+
+// PCG start (dispersion.rs:493-514): r = b - A w, z = r / diag, p = z, and the norms.
+__global__ void __launch_bounds__(NT) k_cg_init(SgnArgs a) {
+  const Geo& g = a.g;
+  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  const int tx = tile % g.ntx, ty = tile / g.ntx;
+  const int i0 = GH + tx * TX, j0 = GH + ty * TY;
+  __shared__ double s_px[SY2][SX2], s_py[SY2][SX2], s_f[SY1][SX1];
+  const ApOut o = tile_matvec(a, a.wx, a.wy, i0, j0, s_px, s_py, s_f);
+  double v[3] = {0.0, 0.0, 0.0};
+  if (o.valid) {
+    const int k = (j0 + (threadIdx.x >> 6)) * g.w + i0 + (threadIdx.x & 63);
+    const double bx = a.bx[k], by = a.by[k];
+    const double rx = bx - o.ax, ry = by - o.ay;
+    const double zx = rx / a.dgx[k], zy = ry / a.dgy[k];
+    a.rx[k] = rx;
+    a.ry[k] = ry;
+    a.px[k] = zx;
+    a.py[k] = zy;
+    v[0] = bx * bx + by * by;
+    v[1] = rx * zx + ry * zy;
+    v[2] = rx * rx + ry * ry;
+  }
+  if (publish_tile<3>(v, a.part, gridDim.x, tile, &a.sc->ticket[0])) {
+    double s[3];
+    final_sums<3>(a.part, gridDim.x, s);
+    if (threadIdx.x == 0) {
+      CgScalars* sc = a.sc;
+      const double bnorm = sqrt(s[0]);
+      sc->target = a.tolerance * bnorm + 1e-10;
+      sc->rz = s[1];
+      sc->it = 0;
+      sc->converged = sqrt(s[2]) <= sc->target ? 1 : 0;
+      sc->done = (sc->converged || a.max_iterations == 0) ? 1 : 0;
+    }
+  }
+}
+
+// One PCG iteration, part 1 (dispersion.rs:516-518): Ap, p.Ap and alpha = rz / p.Ap.
+__global__ void __launch_bounds__(NT) k_cg_matvec(SgnArgs a) {
+  if (a.sc->done) return;
+  const Geo& g = a.g;
+  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  const int tx = tile % g.ntx, ty = tile / g.ntx;
+  const int i0 = GH + tx * TX, j0 = GH + ty * TY;
+  __shared__ double s_px[SY2][SX2], s_py[SY2][SX2], s_f[SY1][SX1];
+  const ApOut o = tile_matvec(a, a.px, a.py, i0, j0, s_px, s_py, s_f);
+  double v[1] = {0.0};
+  if (o.valid) {
+    const int k = (j0 + (threadIdx.x >> 6)) * g.w + i0 + (threadIdx.x & 63);
+    a.ax[k] = o.ax;
+    a.ay[k] = o.ay;
+    v[0] = o.px * o.ax + o.py * o.ay;
+  }
+  if (publish_tile<1>(v, a.part, gridDim.x, tile, &a.sc->ticket[1])) {
+    double s[1];
+    final_sums<1>(a.part, gridDim.x, s);
+    if (threadIdx.x == 0) a.sc->alpha = a.sc->rz / s[0];
+  }
+}
+
+// Part 2 (dispersion.rs:519-526): w += alpha p, r -= alpha Ap, z = r / diag, the convergence
+// test on |r| and beta = rz' / rz.
+__global__ void __launch_bounds__(NT) k_cg_update(SgnArgs a) {
+  CgScalars* sc = a.sc;
+  if (sc->done) return;
+  const Geo& g = a.g;
+  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  const int tx = tile % g.ntx, ty = tile / g.ntx;
+  const int i = GH + tx * TX + (threadIdx.x & 63);
+  const int j = GH + ty * TY + (threadIdx.x >> 6);
+  const double alpha = sc->alpha;
+  const double na = -alpha;
+  double v[2] = {0.0, 0.0};
+  if (i < g.nx + GH && j < g.ny + GH) {
+    const int k = j * g.w + i;
+    const double px = a.px[k], py = a.py[k];
+    a.wx[k] = a.wx[k] + alpha * px;
+    a.wy[k] = a.wy[k] + alpha * py;
+    const double rx = a.rx[k] + na * a.ax[k];
+    const double ry = a.ry[k] + na * a.ay[k];
+    a.rx[k] = rx;
+    a.ry[k] = ry;
+    const double zx = rx / a.dgx[k], zy = ry / a.dgy[k];
+    v[0] = rx * rx + ry * ry;
+    v[1] = rx * zx + ry * zy;
+  }
+  if (publish_tile<2>(v, a.part, gridDim.x, tile, &sc->ticket[2])) {
+    double s[2];
+    final_sums<2>(a.part, gridDim.x, s);
+    if (threadIdx.x == 0) {
+      sc->it += 1;
+      sc->converged = sqrt(s[0]) <= sc->target ? 1 : 0;
+      sc->beta = s[1] / sc->rz;
+      sc->rz = s[1];
+      sc->done = (sc->converged || sc->it >= a.max_iterations) ? 1 : 0;
+    }
+  }
+}
+
+// Part 3 (dispersion.rs:527-539): p = z + beta p. Skipped once the solve is done: p is not
+// read again.
+__global__ void __launch_bounds__(NT) k_cg_pupdate(SgnArgs a) {
+  if (a.sc->done) return;
+  const Geo& g = a.g;
+  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  const int tx = tile % g.ntx, ty = tile / g.ntx;
+  const int i = GH + tx * TX + (threadIdx.x & 63);
+  const int j = GH + ty * TY + (threadIdx.x >> 6);
+  if (i >= g.nx + GH || j >= g.ny + GH) return;
+  const int k = j * g.w + i;
+  const double beta = a.sc->beta;
+  a.px[k] = a.rx[k] / a.dgx[k] + beta * a.px[k];
+  a.py[k] = a.ry[k] / a.dgy[k] + beta * a.py[k];
+}
+
+// The end of a solve: the counters of DispersionStats (dispersion.rs:541-545).
+__global__ void k_cg_stats(CgScalars* sc) {
+  if (threadIdx.x == 0) {
+    sc->solves += 1;
+    sc->iterations += sc->it;
+    if (!sc->converged) sc->unconverged += 1;
+  }
+}
+
+// d(hu)/dt = h w + u dh/dt where the cell takes part (dispersion.rs:547-562), then the stage
+// update of k_flux.
+__global__ void __launch_bounds__(NT) k_sgn_finish(StepArgs a, const double* wx, const double* wy,
+                                                   int mode) {
+  const Geo& g = a.g;
+  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  const int tx = tile % g.ntx, ty = tile / g.ntx;
+  const int i = GH + tx * TX + (threadIdx.x & 63);
+  const int j = GH + ty * TY + (threadIdx.x >> 6);
+  if (i >= g.nx + GH || j >= g.ny + GH) return;
+  const int k = j * g.w + i;
+  const double h = a.in_h[k];
+  const double lh = a.lh[k];
+  double lhu = a.lhu[k], lhv = a.lhv[k];
+  if ((h >= WC_H_DISPERSION_MIN ? 1.0 : 0.0) > 0.0) {
+    lhu = h * wx[k] + wc_velocity(h, a.in_hu[k]) * lh;
+    lhv = h * wy[k] + wc_velocity(h, a.in_hv[k]) * lh;
+  }
+  stage_out(a, mode, i, j, k, lh, lhu, lhv);
+}
+
+// ---- reductions over the state ---------------------------------------------------------------
+// stable_dt (solver.rs:434-450): the largest rate, NaN if any is NaN. Rates are +0, positive,
+// +inf or NaN, so their bit patterns order like their values and a NaN of either sign is above
+// every number: an unsigned max of the bits is max_or_nan.
+__global__ void __launch_bounds__(NT) k_max_rate(Geo g, const double* h, const double* hu,
+                                                 const double* hv, unsigned long long* out) {
+  const int t = blockIdx.x * NT + threadIdx.x;
+  unsigned long long bits = 0ull;
+  if (t < g.nx * g.ny) {
+    const int k = (GH + t / g.nx) * g.w + GH + t % g.nx;
+    const double hc = h[k];
+    const double c = sqrt(WC_G * hc);
+    const double u = fabs(wc_velocity(hc, hu[k]));
+    const double v = fabs(wc_velocity(hc, hv[k]));
+    const double r = (u + c) / g.dx + (v + c) / g.dy;
+    bits = (unsigned long long)__double_as_longlong(r);
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const unsigned long long o = __shfl_xor(bits, off, 64);
+    bits = o > bits ? o : bits;
+  }
+  if ((threadIdx.x & 63) == 0 && bits != 0ull) atomicMax(out, bits);
+}
+
+// limiting_cell (solver.rs:455-477), pass 1: the first NaN cell (row-major), and the bits of
+// the largest rate.
+__global__ void __launch_bounds__(NT) k_limit_scan(Geo g, const double* h, const double* hu,
+                                                   const double* hv, unsigned long long* maxbits,
+                                                   unsigned int* first_nan) {
+  const int t = blockIdx.x * NT + threadIdx.x;
+  if (t >= g.nx * g.ny) return;
+  const int k = (GH + t / g.nx) * g.w + GH + t % g.nx;
+  const double hc = h[k];
+  const double c = sqrt(WC_G * hc);
+  const double u = fabs(wc_velocity(hc, hu[k]));
+  const double v = fabs(wc_velocity(hc, hv[k]));
+  const double r = (u + c) / g.dx + (v + c) / g.dy;
+  if (isnan(r)) {
+    atomicMin(first_nan, (unsigned int)t);
+  } else if (r > 0.0) {
+    atomicMax(maxbits, (unsigned long long)__double_as_longlong(r));
+  }
+}
+
+// Pass 2: the first cell whose rate equals the largest.
+__global__ void __launch_bounds__(NT) k_limit_first(Geo g, const double* h, const double* hu,
+                                                    const double* hv,
+                                                    const unsigned long long* maxbits,
+                                                    unsigned int* first) {
+  const int t = blockIdx.x * NT + threadIdx.x;
+  if (t >= g.nx * g.ny) return;
+  const int k = (GH + t / g.nx) * g.w + GH + t % g.nx;
+  const double hc = h[k];
+  const double c = sqrt(WC_G * hc);
+  const double u = fabs(wc_velocity(hc, hu[k]));
+  const double v = fabs(wc_velocity(hc, hv[k]));
+  const double r = (u + c) / g.dx + (v + c) / g.dy;
+  if ((unsigned long long)__double_as_longlong(r) == *maxbits) atomicMin(first, (unsigned int)t);
+}
+
+// State::volume in the canonical order (k_sum_h then its last block), State::max_abs_momentum
+// and State::min_depth (NaN cells drop out of both, as in the reference's folds).
+__global__ void __launch_bounds__(NT) k_state_sums(Geo g, const double* h, const double* hu,
+                                                   const double* hv, double* part,
+                                                   unsigned int* ticket, double* out,
+                                                   unsigned long long* maxmom,
+                                                   unsigned long long* minh_key) {
+  const int tile = blockIdx.x;
+  const int tx = tile % g.ntx, ty = tile / g.ntx;
+  const int i = GH + tx * TX + (threadIdx.x & 63);
+  const int j = GH + ty * TY + (threadIdx.x >> 6);
+  double v[1] = {0.0};
+  if (i < g.nx + GH && j < g.ny + GH) {
+    const int k = j * g.w + i;
+    const double hc = h[k];
+    v[0] = hc;
+    const double m = wc_max(fabs(hu[k]), fabs(hv[k]));
+    if (!isnan(m)) atomicMax(maxmom, (unsigned long long)__double_as_longlong(m));
+    if (!isnan(hc)) {
+      // order-preserving key of a double
+      unsigned long long b = (unsigned long long)__double_as_longlong(hc);
+      b = (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+      atomicMin(minh_key, b);
+    }
+  }
+  if (publish_tile<1>(v, part, gridDim.x, tile, ticket)) {
+    double s[1];
+    final_sums<1>(part, gridDim.x, s);
+    if (threadIdx.x == 0) *out = s[0];
+  }
+}
+
+}  // namespace wc

@@ -1,0 +1,341 @@
+"""``Solver``: wavecore's Solver (crates/wavecore/src/solver.rs:260-507) on the GPU.
+
+Same builder methods, same ``stable_dt`` / ``step`` / ``limiting_cell`` / ``breaking_indicator``
+/ ``dispersion_stats``, same errors (``ValueError`` where the reference panics). Every call goes
+through the C ABI of ``libwavecore_hip.so`` (include/wavecore_hip.h); the state stays in device
+memory between steps (see ``State``'s residency protocol). There is no CPU fallback: without the
+built library, or without a gfx950 GPU, constructing a Solver raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import weakref
+
+import numpy as np
+
+from .config import (
+    CDispersion,
+    CDispersionStats,
+    CLimiting,
+    CLinearWave,
+    CSponge,
+    CWaveMaker,
+    Dispersion,
+    DispersionStats,
+    Limiting,
+    LinearWave,
+    Order,
+    Sponge,
+    WaveMaker,
+    c_dispersion,
+    c_sponge,
+    c_wavemaker,
+)
+from .grid import Bathymetry, Grid, State
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwavecore_hip.so")
+
+WC_OK, WC_EINVAL, WC_EDEVICE, WC_ENOMEM = 0, 1, 2, 3
+PRECISION_FP64 = 0
+
+_D = ctypes.c_double
+_P = ctypes.POINTER(ctypes.c_double)
+_U64 = ctypes.c_uint64
+_I32 = ctypes.c_int32
+_VP = ctypes.c_void_p
+
+# Every entry point of include/wavecore_hip.h with its ctypes signature.
+SIGNATURES = {
+    "wc_last_error": (ctypes.c_char_p, []),
+    "wc_solver_new": (_I32, [_U64, _U64, _D, _D, _P, ctypes.POINTER(_VP)]),
+    "wc_solver_free": (None, [_VP]),
+    "wc_solver_set_cfl": (_I32, [_VP, _D]),
+    "wc_solver_cfl": (_I32, [_VP, _P]),
+    "wc_solver_with_order": (_I32, [_VP, _I32]),
+    "wc_solver_with_manning": (_I32, [_VP, _D]),
+    "wc_solver_with_sponge": (_I32, [_VP, ctypes.POINTER(CSponge)]),
+    "wc_solver_with_wavemaker": (_I32, [_VP, ctypes.POINTER(CWaveMaker)]),
+    "wc_solver_with_dispersion": (_I32, [_VP, ctypes.POINTER(CDispersion)]),
+    "wc_solver_with_periodic_y": (_I32, [_VP]),
+    "wc_solver_with_shallow_cells": (_I32, [_VP, ctypes.POINTER(ctypes.c_uint8), _U64]),
+    "wc_solver_bed": (_I32, [_VP, _P]),
+    "wc_solver_set_precision": (_I32, [_VP, _I32]),
+    "wc_state_set": (_I32, [_VP, _P, _P, _P, _P, _D]),
+    "wc_state_get": (_I32, [_VP, _P, _P, _P, _P, _P]),
+    "wc_state_set_time": (_I32, [_VP, _D]),
+    "wc_stable_dt": (_I32, [_VP, _P]),
+    "wc_step": (_I32, [_VP, _D]),
+    "wc_limiting_cell": (_I32, [_VP, ctypes.POINTER(_I32), ctypes.POINTER(CLimiting)]),
+    "wc_breaking_indicator": (_I32, [_VP, _P]),
+    "wc_dispersion_stats": (_I32, [_VP, ctypes.POINTER(_I32), ctypes.POINTER(CDispersionStats)]),
+    "wc_state_volume": (_I32, [_VP, _P]),
+    "wc_state_max_abs_momentum": (_I32, [_VP, _P]),
+    "wc_state_min_depth": (_I32, [_VP, _P]),
+    "wc_linear_wave": (_I32, [_D, _D, _I32, ctypes.POINTER(CLinearWave)]),
+    "wc_manning_factor": (_D, [_D, _D, _D, _D]),
+    "wc_run_steps": (_I32, [_VP, _U64, ctypes.POINTER(_U64), _P]),
+    "wc_profile_enable": (_I32, [_VP, _I32]),
+    "wc_profile_reset": (_I32, [_VP]),
+    "wc_profile_read": (_I32, [_VP, ctypes.c_char_p, _U64, _P, ctypes.POINTER(_U64), _U64,
+                               ctypes.POINTER(_U64)]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """The built library; raises if it is missing (there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built: run `python -m wavesim_amd.build` (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class DeviceError(RuntimeError):
+    pass
+
+
+def _check(rc: int) -> None:
+    if rc == WC_OK:
+        return
+    msg = lib().wc_last_error().decode()
+    if rc == WC_EINVAL:
+        raise ValueError(msg)
+    if rc == WC_ENOMEM:
+        raise MemoryError(msg)
+    raise DeviceError(msg)
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_P)
+
+
+class Solver:
+    """wavecore's ``Solver`` on one MI355X GPU (the current HIP device)."""
+
+    def __init__(self, grid: Grid, bed: Bathymetry):
+        b = np.ascontiguousarray(bed.b, dtype=np.float64)
+        if b.size != grid.cells():
+            raise ValueError("bed must be a padded array of grid.cells() values")
+        self.grid = grid
+        h = _VP()
+        _check(lib().wc_solver_new(grid.nx, grid.ny, grid.dx, grid.dy, _ptr(b), ctypes.byref(h)))
+        self._h = h
+        self.order = Order.Second
+        self.manning = 0.0
+        self.periodic_y = False
+        self._dispersion = None
+        self._holding = None  # weak reference to the State on the device
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            held = self._holding() if self._holding else None
+            if held is not None and not held._host_valid:
+                try:
+                    self._download(held)
+                except Exception:
+                    pass
+            lib().wc_solver_free(h)
+            self._h = None
+
+    # -- builder (solver.rs:301-419) ---------------------------------------------------------
+    @property
+    def cfl(self) -> float:
+        out = ctypes.c_double()
+        _check(lib().wc_solver_cfl(self._h, ctypes.byref(out)))
+        return out.value
+
+    @cfl.setter
+    def cfl(self, v: float) -> None:
+        _check(lib().wc_solver_set_cfl(self._h, float(v)))
+
+    def with_order(self, order: Order) -> "Solver":
+        _check(lib().wc_solver_with_order(self._h, int(order)))
+        self.order = Order(order)
+        return self
+
+    def with_manning(self, n: float) -> "Solver":
+        _check(lib().wc_solver_with_manning(self._h, float(n)))
+        self.manning = float(n)
+        return self
+
+    def with_sponge(self, sponge: Sponge) -> "Solver":
+        c = c_sponge(sponge)
+        _check(lib().wc_solver_with_sponge(self._h, ctypes.byref(c)))
+        return self
+
+    def with_wavemaker(self, maker: WaveMaker) -> "Solver":
+        c = c_wavemaker(maker)
+        _check(lib().wc_solver_with_wavemaker(self._h, ctypes.byref(c)))
+        return self
+
+    def with_dispersion(self, d: Dispersion) -> "Solver":
+        c = c_dispersion(d)
+        _check(lib().wc_solver_with_dispersion(self._h, ctypes.byref(c)))
+        self._dispersion = d
+        return self
+
+    def with_periodic_y(self) -> "Solver":
+        _check(lib().wc_solver_with_periodic_y(self._h))
+        self.periodic_y = True
+        return self
+
+    def with_shallow_cells(self, cells) -> "Solver":
+        a = np.ascontiguousarray(np.asarray(cells, dtype=bool).astype(np.uint8).ravel())
+        _check(lib().wc_solver_with_shallow_cells(
+            self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), a.size))
+        return self
+
+    def with_precision(self, precision: int) -> "Solver":
+        _check(lib().wc_solver_set_precision(self._h, int(precision)))
+        return self
+
+    def dispersion(self):
+        return self._dispersion
+
+    @property
+    def bed(self) -> Bathymetry:
+        out = np.empty(self.grid.cells())
+        _check(lib().wc_solver_bed(self._h, _ptr(out)))
+        return Bathymetry(out)
+
+    # -- residency ---------------------------------------------------------------------------
+    def _download(self, s: State) -> None:
+        n = self.grid.cells()
+        out = [np.empty(n) for _ in range(4)]
+        _check(lib().wc_state_get(self._h, *(_ptr(a) for a in out), None))
+        s._h, s._hu, s._hv, s._breaking = out
+
+    def _hold(self, s: State) -> None:
+        """Make the device hold ``s`` (uploading it if the device copy is not current)."""
+        if s._holder is self and s._dev_valid:
+            _check(lib().wc_state_set_time(self._h, s.time))
+            return
+        prev = self._holding() if self._holding else None
+        if prev is not None and prev is not s and prev._holder is self:
+            prev._sync_host()
+            prev._holder = None
+            prev._dev_valid = False
+        if s._holder is not None and s._holder is not self:
+            s._sync_host()
+        h, hu, hv, br = (np.ascontiguousarray(a) for a in (s._h, s._hu, s._hv, s._breaking))
+        if h.size != self.grid.cells():
+            raise ValueError("state does not match the grid")
+        _check(lib().wc_state_set(self._h, _ptr(h), _ptr(hu), _ptr(hv), _ptr(br), s.time))
+        s._holder = self
+        s._dev_valid = True
+        s._host_valid = True
+        self._holding = weakref.ref(s)
+
+    # -- the API -----------------------------------------------------------------------------
+    def stable_dt(self, s: State) -> float:
+        """solver.rs:434-450."""
+        self._hold(s)
+        out = ctypes.c_double()
+        _check(lib().wc_stable_dt(self._h, ctypes.byref(out)))
+        return out.value
+
+    def step(self, s: State, dt: float) -> None:
+        """solver.rs:481-507: advances ``s`` (on the device) and its time by ``dt``."""
+        self._hold(s)
+        t = s.time
+        _check(lib().wc_step(self._h, float(dt)))
+        s.time = t + float(dt)
+        s._host_valid = False
+
+    def run_steps(self, s: State, steps: int) -> tuple[int, float]:
+        """``steps`` iterations of ``dt = stable_dt(s); step(s, dt)`` in one call; stops early
+        when dt is not finite and positive. Returns (steps taken, last dt)."""
+        self._hold(s)
+        done = ctypes.c_uint64()
+        last = ctypes.c_double()
+        _check(lib().wc_run_steps(self._h, int(steps), ctypes.byref(done), ctypes.byref(last)))
+        tt = ctypes.c_double()
+        _check(lib().wc_state_get(self._h, None, None, None, None, ctypes.byref(tt)))
+        s.time = tt.value
+        s._host_valid = False
+        return int(done.value), last.value
+
+    def limiting_cell(self, s: State):
+        """solver.rs:455-477."""
+        self._hold(s)
+        found = ctypes.c_int32()
+        out = CLimiting()
+        _check(lib().wc_limiting_cell(self._h, ctypes.byref(found), ctypes.byref(out)))
+        if not found.value:
+            return None
+        return Limiting(int(out.i), int(out.j), out.depth, out.speed)
+
+    def breaking_indicator(self, s: State) -> np.ndarray:
+        """solver.rs:341-349."""
+        self._hold(s)
+        out = np.empty(self.grid.nx * self.grid.ny)
+        _check(lib().wc_breaking_indicator(self._h, _ptr(out)))
+        return out
+
+    def dispersion_stats(self):
+        """solver.rs:392-396."""
+        has = ctypes.c_int32()
+        out = CDispersionStats()
+        _check(lib().wc_dispersion_stats(self._h, ctypes.byref(has), ctypes.byref(out)))
+        if not has.value:
+            return None
+        return DispersionStats(int(out.solves), int(out.iterations), int(out.unconverged))
+
+    def volume(self, s: State) -> float:
+        """State::volume on the device (canonical summation order)."""
+        self._hold(s)
+        out = ctypes.c_double()
+        _check(lib().wc_state_volume(self._h, ctypes.byref(out)))
+        return out.value
+
+    def max_abs_momentum(self, s: State) -> float:
+        self._hold(s)
+        out = ctypes.c_double()
+        _check(lib().wc_state_max_abs_momentum(self._h, ctypes.byref(out)))
+        return out.value
+
+    def min_depth(self, s: State) -> float:
+        self._hold(s)
+        out = ctypes.c_double()
+        _check(lib().wc_state_min_depth(self._h, ctypes.byref(out)))
+        return out.value
+
+    # -- measurement ---------------------------------------------------------------------------
+    def profile(self, on: bool) -> None:
+        _check(lib().wc_profile_enable(self._h, int(on)))
+
+    def profile_reset(self) -> None:
+        _check(lib().wc_profile_reset(self._h))
+
+    def profile_read(self) -> dict:
+        names = ctypes.create_string_buffer(1024)
+        ms = (ctypes.c_double * 64)()
+        n = (ctypes.c_uint64 * 64)()
+        cnt = ctypes.c_uint64()
+        _check(lib().wc_profile_read(self._h, names, 1024, ms, n, 64, ctypes.byref(cnt)))
+        keys = names.value.decode().split(";")
+        return {k: (ms[q], int(n[q])) for q, k in enumerate(keys[: cnt.value])}
+
+
+def linear_wave(omega: float, h: float, dispersive: bool) -> LinearWave:
+    out = CLinearWave()
+    _check(lib().wc_linear_wave(omega, h, int(dispersive), ctypes.byref(out)))
+    return LinearWave(out.k, out.phase_speed, out.group_speed)
+
+
+def manning_factor(h: float, speed: float, dt: float, n: float) -> float:
+    return lib().wc_manning_factor(h, speed, dt, n)
