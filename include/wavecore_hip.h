@@ -108,6 +108,10 @@ int wc_solver_with_periodic_y(wc_solver* s);                          /* solver.
 int wc_solver_with_shallow_cells(wc_solver* s, const uint8_t* cells, uint64_t n);
 /* The bed as the solver holds it (with_periodic_y wraps its ghost rows; solver.rs:416). */
 int wc_solver_bed(const wc_solver* s, double* out);
+/* Implementation choices that change speed, never results (no counterpart in the reference):
+ *   "pcg_kernels" 1 = three kernels per PCG iteration, 2 = two fused kernels (default);
+ *   "pcg_batch"   PCG iterations queued between looks at the convergence flag (default 4). */
+int wc_solver_set_option(wc_solver* s, const char* key, int64_t value);
 /* Selects the arithmetic (WC_PRECISION_*); FP64 by default. */
 int wc_solver_set_precision(wc_solver* s, int32_t precision);
 

@@ -48,8 +48,12 @@ def assert_states_equal(sg: State, so: State, label: str):
     assert sg.time == so.time, (label, sg.time, so.time)
 
 
-def lockstep(grid, bed, configure, state: State, steps: int, label: str, fixed_dt=None):
-    g = configure(Solver(grid, bed.copy()))
+def lockstep(grid, bed, configure, state: State, steps: int, label: str, fixed_dt=None,
+             options=None):
+    g = Solver(grid, bed.copy())
+    for k, v in (options or {}).items():
+        g.set_option(k, v)
+    g = configure(g)
     o = configure(oracle(grid, bed.copy()))
     sg, so = state.clone(), state.clone()
     say(f"[{label}] {grid.nx}x{grid.ny}, {steps} steps")
@@ -94,8 +98,9 @@ def test_each_order_over_a_rough_bed_with_dry_land(order):
              150, f"rough-dry {order.name}")
 
 
+@pytest.mark.parametrize("pcg", [1, 2])
 @pytest.mark.parametrize("periodic", [False, True])
-def test_sgn_fifth_walls_and_periodic_seam(periodic):
+def test_sgn_fifth_walls_and_periodic_seam(periodic, pcg):
     grid = Grid(40, 30, 1.0, 1.0)
     bed = Bathymetry.flat(grid, 2.0)
 
@@ -108,10 +113,12 @@ def test_sgn_fifth_walls_and_periodic_seam(periodic):
     X, Y = np.meshgrid(x[GHOST:GHOST + grid.nx], y[GHOST:GHOST + grid.ny])
     cy = 0.5 if periodic else 2.5
     grid.inner(s.h)[:, :] += 1.3 * np.exp(-((X - 20.0) ** 2 + (Y - cy) ** 2) / 32.0)
-    lockstep(grid, bed, configure, s, 120, f"sgn seam periodic={periodic}")
+    lockstep(grid, bed, configure, s, 120, f"sgn seam periodic={periodic} pcg={pcg}",
+             options={"pcg_kernels": pcg})
 
 
-def test_sgn_rough_bed_tight_tolerance_with_maker_sponges_and_friction():
+@pytest.mark.parametrize("pcg", [1, 2])
+def test_sgn_rough_bed_tight_tolerance_with_maker_sponges_and_friction(pcg):
     w = configs.c1_rough_bed(nx=160, ny=64, steps=200)
 
     def configure(s):
@@ -122,7 +129,8 @@ def test_sgn_rough_bed_tight_tolerance_with_maker_sponges_and_friction():
     # start mid-ramp so the maker and the dispersion have something to do
     s = w.still_water()
     s.time = 20.0
-    lockstep(w.grid, w.bed, configure, s, w.steps, "sgn rough bed")
+    lockstep(w.grid, w.bed, configure, s, w.steps, f"sgn rough bed pcg={pcg}",
+             options={"pcg_kernels": pcg, "pcg_batch": 3})
 
 
 def test_sgn_second_order_linear_dispersion_and_held_cells():

@@ -141,3 +141,109 @@ def c1_rough_bed(nx: int = 1024, ny: int = 512, seed: int = 1, dx: float = 3.0,
     notes = {"angle_deg": math.degrees(angle), "alongshore_waves": n_fit,
              "maker_depth_m": depth, "sponge_cells": sponge_cells}
     return Workload(f"rough_bed_{nx}x{ny}_fifth_sgn", grid, bed, configure, steps, notes)
+
+
+def _sponge_cells(period: float, depth: float, dx: float, nx: int) -> int:
+    """wavesim's offshore sponge: 1.5 wavelengths, clamped to [4, nx/3] (lib.rs:644-645)."""
+    wl = 2.0 * math.pi / linear_wave(2.0 * math.pi / period, depth, True).k
+    return int(min(max(math.ceil(1.5 * wl / dx), 4), max(nx // 3, 4)))
+
+
+def gaussian_lobes(rng: Lcg, n: int, xr, yr, height, width):
+    lobes = [(rng.uniform(*xr), rng.uniform(*yr), rng.uniform(*height), rng.uniform(*width))
+             for _ in range(n)]
+
+    def f(x, y):
+        out = np.zeros_like(x)
+        for x0, y0, hgt, wd in lobes:
+            out += hgt * np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / (2.0 * wd * wd))
+        return out
+
+    return f
+
+
+def c3_reef_instance(seed: int, dx: float = 3.0) -> Workload:
+    """One instance of the batched spot-sheet sweep: a size drawn from {256x128, 512x256,
+    768x384, 1024x512, 1536x768}, a 1:30 plane plus 1-4 Gaussian reef lobes (4-10 m high,
+    40-200 m wide) plus a 3 m channel, swell angle -20..20 degrees, tide -0.5..0.5 m (the bed
+    shifted down by the tide, as waveio's Bed::bathymetry does, crates/waveio/src/bed.rs:239-246),
+    H 1-3 m, T 10-18 s; fifth order + SGN, 2000 steps."""
+    rng = Lcg(seed)
+    sizes = [(256, 128), (512, 256), (768, 384), (1024, 512), (1536, 768)]
+    nx, ny = sizes[min(int(rng.uniform() * len(sizes)), len(sizes) - 1)]
+    grid = Grid(nx, ny, dx, dx)
+    lx, ly = nx * dx, ny * dx
+    angle = math.radians(rng.uniform(-20.0, 20.0))
+    tide = rng.uniform(-0.5, 0.5)
+    height = rng.uniform(1.0, 3.0)
+    period = rng.uniform(10.0, 18.0)
+    n_lobes = 1 + min(int(rng.uniform() * 4), 3)
+    # the plane meets still water 85% of the way across; the offshore depth follows
+    shore = 0.85 * lx
+    lobes = gaussian_lobes(rng, n_lobes, (0.45 * lx, 0.8 * lx), (0.0, ly), (4.0, 10.0),
+                           (40.0, 200.0))
+    ch_y = rng.uniform(0.2 * ly, 0.8 * ly)
+
+    def bed_fn(x, y):
+        plane = (x - shore) / 30.0
+        channel = -3.0 * np.exp(-((y - ch_y) / 30.0) ** 2) * (x > 0.4 * lx)
+        return np.minimum(plane + lobes(x, y), plane + 2.0) + channel - tide
+
+    bed = Bathymetry.from_fn(grid, bed_fn)
+    depth0 = float(-np.max(grid.inner(bed.b)[:, 0]))
+    sponge = _sponge_cells(period, max(depth0, 1.0), dx, nx)
+    maker_x = (sponge + 4.0 * SIGMA_CELLS) * dx
+    maker = WaveMaker(maker_x, height / 2.0, period, angle, SIGMA_CELLS * dx, RAMP_PERIODS)
+
+    def configure(s):
+        s = (s.with_order(Order.Fifth)
+             .with_dispersion(Dispersion(tolerance=1e-8, max_iterations=200))
+             .with_wavemaker(maker)
+             .with_sponge(Sponge(sponge, SPONGE_STRENGTH).edges(True, False, True, True)))
+        s.cfl = 0.45
+        return s
+
+    notes = {"seed": seed, "nx": nx, "ny": ny, "angle_deg": math.degrees(angle), "tide_m": tide,
+             "H_m": height, "T_s": period, "lobes": n_lobes, "sponge_cells": sponge}
+    return Workload(f"reef_sheet_{seed}_{nx}x{ny}", grid, bed, configure, 2000, notes)
+
+
+def c4_large_reef(nx: int = 4096, ny: int = 2048, dx: float = 3.0, seed: int = 4000,
+                  steps: int = 6000) -> Workload:
+    """4096x2048 cells at 3 m: a shelf 40 m deep shoaling at 1:50 onto a reef flat 1 m deep,
+    8 Gaussian reef lobes, the rough-bed Fourier overlay (16 modes, 30-300 m, 0-0.4 m), a dry
+    beach strip; wave-maker H 3 m, T 16 s at 15 degrees, sponges on the west, south and north,
+    Manning 0.025, fifth order + SGN by PCG to 1e-8, breaking switch on, 6000 steps."""
+    grid = Grid(nx, ny, dx, dx)
+    lx, ly = nx * dx, ny * dx
+    rng = Lcg(seed)
+    overlay = fourier_overlay(rng, 16, (30.0, 300.0), (0.0, 0.4))
+    x_edge = 0.55 * lx  # shelf edge
+    x_flat = x_edge + 39.0 * 50.0  # 40 m -> 1 m at 1:50
+    x_beach = 0.95 * lx  # dry beach strip beyond
+    lobes = gaussian_lobes(rng, 8, (x_edge, x_beach), (0.0, ly), (4.0, 10.0), (40.0, 200.0))
+
+    def bed_fn(x, y):
+        base = np.where(x < x_edge, -40.0,
+                        np.where(x < x_flat, -40.0 + (x - x_edge) / 50.0, -1.0))
+        base = np.where(x > x_beach, -1.0 + (x - x_beach) / 20.0, base)
+        return np.minimum(base + lobes(x, y), 3.0) + overlay(x, y)
+
+    bed = Bathymetry.from_fn(grid, bed_fn)
+    period, height = 16.0, 3.0
+    sponge = _sponge_cells(period, 40.0, dx, nx)
+    maker_x = (sponge + 4.0 * SIGMA_CELLS) * dx
+    maker = WaveMaker(maker_x, height / 2.0, period, math.radians(15.0), SIGMA_CELLS * dx,
+                      RAMP_PERIODS)
+
+    def configure(s):
+        s = (s.with_order(Order.Fifth)
+             .with_dispersion(Dispersion(tolerance=1e-8, max_iterations=200))
+             .with_wavemaker(maker)
+             .with_sponge(Sponge(sponge, SPONGE_STRENGTH).edges(True, False, True, True))
+             .with_manning(0.025))
+        s.cfl = 0.45
+        return s
+
+    notes = {"seed": seed, "sponge_cells": sponge, "maker_x_m": maker_x}
+    return Workload(f"large_reef_{nx}x{ny}_fifth_sgn", grid, bed, configure, steps, notes)

@@ -125,6 +125,11 @@ struct wc_solver {
   double *c = nullptr, *dgx = nullptr, *dgy = nullptr, *bx = nullptr, *by = nullptr;
   double *wx = nullptr, *wy = nullptr, *rx = nullptr, *ry = nullptr, *px = nullptr,
          *py = nullptr, *ax = nullptr, *ay = nullptr;
+  double *zx = nullptr, *zy = nullptr, *qx = nullptr, *qy = nullptr;  // fused PCG: z, 2nd p
+
+  // implementation options (wc_solver_set_option)
+  int pcg_kernels = 2;
+  unsigned long long pcg_batch = 4;
   uint8_t* held = nullptr;
   double* part = nullptr;
   CgScalars* sc = nullptr;
@@ -439,6 +444,8 @@ SgnArgs sgn_args(wc_solver* s, int stage) {
   a.py = s->py;
   a.ax = s->ax;
   a.ay = s->ay;
+  a.zx = s->zx;
+  a.zy = s->zy;
   a.part = s->part;
   a.sc = s->sc;
   a.nonlinear = s->disp.nonlinear ? 1 : 0;
@@ -460,9 +467,28 @@ int pcg(wc_solver* s, int stage) {
   while (launched < maxit) {
     const unsigned long long n = std::min(batch, maxit - launched);
     for (unsigned long long q = 0; q < n; ++q) {
-      TRY(launch(s, F_CG_MATVEC, k_cg_matvec, tiles(s), dim3(NT), a));
-      TRY(launch(s, F_CG_UPDATE, k_cg_update, tiles(s), dim3(NT), a));
-      TRY(launch(s, F_CG_PUPDATE, k_cg_pupdate, tiles(s), dim3(NT), a));
+      if (s->pcg_kernels == 1) {
+        TRY(launch(s, F_CG_MATVEC, k_cg_matvec, tiles(s), dim3(NT), a));
+        TRY(launch(s, F_CG_UPDATE, k_cg_update, tiles(s), dim3(NT), a));
+        TRY(launch(s, F_CG_PUPDATE, k_cg_pupdate, tiles(s), dim3(NT), a));
+        continue;
+      }
+      // p of iteration 1 is what cg_init stored in (px, py); afterwards the new p alternates
+      // between (qx, qy) and (px, py).
+      const unsigned long long it = launched + q + 1;
+      const int first = it == 1 ? 1 : 0;
+      double* cur_x = (it % 2 == 1) ? s->px : s->qx;
+      double* cur_y = (it % 2 == 1) ? s->py : s->qy;
+      const double* old_x = (it % 2 == 1) ? s->qx : s->px;
+      const double* old_y = (it % 2 == 1) ? s->qy : s->py;
+      if (first) {
+        old_x = s->px;
+        old_y = s->py;
+      }
+      TRY(launch(s, F_CG_MATVEC, k_cg_matvec2, tiles(s), dim3(NT), a, first, old_x, old_y,
+                 cur_x, cur_y));
+      TRY(launch(s, F_CG_UPDATE, k_cg_update2, tiles(s), dim3(NT), a, (const double*)cur_x,
+                 (const double*)cur_y));
     }
     launched += n;
     HIP_OK(hipMemcpyAsync(s->sc_host, s->sc, sizeof(CgScalars), hipMemcpyDeviceToHost,
@@ -470,7 +496,7 @@ int pcg(wc_solver* s, int stage) {
     HIP_OK(hipStreamSynchronize(s->stream));
     seen = true;
     if (s->sc_host->done) break;
-    batch = 4;
+    batch = s->pcg_batch;
   }
   if (!seen) {
     HIP_OK(hipMemcpyAsync(s->sc_host, s->sc, sizeof(CgScalars), hipMemcpyDeviceToHost,
@@ -552,7 +578,7 @@ int alloc_dispersion(wc_solver* s) {
   if (s->c) return WC_OK;
   const size_t n = s->cells;
   for (double** p : {&s->c, &s->dgx, &s->dgy, &s->bx, &s->by, &s->wx, &s->wy, &s->rx, &s->ry,
-                     &s->px, &s->py, &s->ax, &s->ay})
+                     &s->px, &s->py, &s->ax, &s->ay, &s->zx, &s->zy, &s->qx, &s->qy})
     TRY(dalloc(s, p, n));
   return WC_OK;
 }
@@ -772,6 +798,21 @@ int wc_solver_with_shallow_cells(wc_solver* s, const uint8_t* cells, uint64_t n)
 int wc_solver_bed(const wc_solver* s, double* out) {
   memcpy(out, s->bed.data(), s->cells * sizeof(double));
   return WC_OK;
+}
+
+int wc_solver_set_option(wc_solver* s, const char* key, int64_t value) {
+  if (!key) return fail(WC_EINVAL, "null option name");
+  if (strcmp(key, "pcg_kernels") == 0) {
+    if (value != 1 && value != 2) return fail(WC_EINVAL, "pcg_kernels must be 1 or 2");
+    s->pcg_kernels = (int)value;
+    return WC_OK;
+  }
+  if (strcmp(key, "pcg_batch") == 0) {
+    if (value < 1 || value > 1000) return fail(WC_EINVAL, "pcg_batch must be in [1, 1000]");
+    s->pcg_batch = (unsigned long long)value;
+    return WC_OK;
+  }
+  return fail(WC_EINVAL, std::string("unknown option ") + key);
 }
 
 int wc_solver_set_precision(wc_solver* s, int32_t precision) {

@@ -59,6 +59,7 @@ struct SgnArgs {
   const double *h, *hu, *hv, *phi;
   const double *lh, *lhu, *lhv;
   double *c, *dgx, *dgy, *bx, *by, *wx, *wy, *rx, *ry, *px, *py, *ax, *ay;
+  double *zx, *zy;  // z = r / diag, kept between the two kernels of a fused iteration
   double* part;
   CgScalars* sc;
   int nonlinear;
@@ -79,6 +80,9 @@ __global__ void k_cg_matvec(SgnArgs a);
 __global__ void k_cg_update(SgnArgs a);
 __global__ void k_cg_pupdate(SgnArgs a);
 __global__ void k_cg_stats(CgScalars* sc);
+__global__ void k_cg_matvec2(SgnArgs a, int first, const double* psx, const double* psy,
+                             double* pdx, double* pdy);
+__global__ void k_cg_update2(SgnArgs a, const double* pcx, const double* pcy);
 __global__ void k_sgn_finish(StepArgs a, const double* wx, const double* wy, int mode);
 __global__ void k_max_rate(Geo g, const double* h, const double* hu, const double* hv,
                            unsigned long long* out);

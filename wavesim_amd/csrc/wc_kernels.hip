@@ -498,6 +498,138 @@ __global__ void __launch_bounds__(NT) k_cg_pupdate(SgnArgs a) {
   a.py[k] = a.ry[k] / a.dgy[k] + beta * a.py[k];
 }
 
+// ---- fused PCG iteration: two kernels, no stored diagonal ----------------------------------
+// The Jacobi diagonal (dispersion.rs:442-457) at an interior cell, from c and h as the setup
+// computes it.
+__device__ __forceinline__ void diag_at(const SgnArgs& a, int k, double h, double& dgx,
+                                        double& dgy) {
+  if (!(h >= WC_H_DISPERSION_MIN)) {
+    dgx = 1.0;
+    dgy = 1.0;
+    return;
+  }
+  const double heff = wc_max(h, WC_H_DRY);
+  const int w = a.g.w;
+  dgx = heff + (a.c[k + 1] + a.c[k - 1]) * 0.25 / (a.g.dx * a.g.dx);
+  dgy = heff + (a.c[k + w] + a.c[k - w]) * 0.25 / (a.g.dy * a.g.dy);
+}
+
+// Part 1 (dispersion.rs:527-539 of the previous iteration, then 516-518): p = z + beta p on the
+// tile + 2 (the first iteration takes p as cg_init stored it), A p, p.Ap and alpha. The new p of
+// the tile's own cells goes to pdst for part 2 and the next iteration.
+__global__ void __launch_bounds__(NT) k_cg_matvec2(SgnArgs a, int first, const double* psx,
+                                                   const double* psy, double* pdx, double* pdy) {
+  CgScalars* sc = a.sc;
+  if (sc->done) return;
+  const Geo& g = a.g;
+  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  const int tx = tile % g.ntx, ty = tile / g.ntx;
+  const int i0 = GH + tx * TX, j0 = GH + ty * TY;
+  __shared__ double s_px[SY2][SX2], s_py[SY2][SX2], s_f[SY1][SX1];
+  const double beta = sc->beta;
+  const double ix = 0.5 / g.dx, iy = 0.5 / g.dy;
+  for (int t = threadIdx.x; t < SX2 * SY2; t += NT) {
+    const int lx = t % SX2, ly = t / SX2;
+    const int i = i0 - 2 + lx, j = j0 - 2 + ly;
+    double vx = 0.0, vy = 0.0;
+    if (i <= g.nx + GH + 1 && j <= g.ny + GH + 1) {
+      const int m = map_y(j, g.ny, g.periodic) * g.w + map_x(i, g.nx);
+      if (first) {
+        vx = psx[m];
+        vy = psy[m];
+      } else {
+        vx = a.zx[m] + beta * psx[m];
+        vy = a.zy[m] + beta * psy[m];
+      }
+      if (is_ghost_x(i, g.nx)) vx = -vx;
+      if (!g.periodic && is_ghost_y(j, g.ny)) vy = -vy;
+    }
+    s_px[ly][lx] = vx;
+    s_py[ly][lx] = vy;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < SX1 * SY1; t += NT) {
+    const int lx = t % SX1, ly = t / SX1;
+    const int i = i0 - 1 + lx, j = j0 - 1 + ly;
+    double f = 0.0;
+    if (i <= g.nx + GH && j <= g.ny + GH) {
+      const int x = lx + 1, y = ly + 1;
+      f = a.c[j * g.w + i] *
+          ((s_px[y][x + 1] - s_px[y][x - 1]) * ix + (s_py[y + 1][x] - s_py[y - 1][x]) * iy);
+    }
+    s_f[ly][lx] = f;
+  }
+  __syncthreads();
+  const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+  const int i = i0 + lx, j = j0 + ly;
+  double v[1] = {0.0};
+  if (i < g.nx + GH && j < g.ny + GH) {
+    const int k = j * g.w + i;
+    const double h = a.h[k];
+    const double mask = h >= WC_H_DISPERSION_MIN ? 1.0 : 0.0;
+    const double heff = wc_max(h, WC_H_DRY);
+    const int x1 = lx + 1, y1 = ly + 1;
+    const double px = s_px[ly + 2][lx + 2], py = s_py[ly + 2][lx + 2];
+    const double ax = mask * (heff * px - (s_f[y1][x1 + 1] - s_f[y1][x1 - 1]) * ix);
+    const double ay = mask * (heff * py - (s_f[y1 + 1][x1] - s_f[y1 - 1][x1]) * iy);
+    if (!first) {
+      pdx[k] = px;
+      pdy[k] = py;
+    }
+    a.ax[k] = ax;
+    a.ay[k] = ay;
+    v[0] = px * ax + py * ay;
+  }
+  if (publish_tile<1>(v, a.part, gridDim.x, tile, &sc->ticket[1])) {
+    double s[1];
+    final_sums<1>(a.part, gridDim.x, s);
+    if (threadIdx.x == 0) sc->alpha = sc->rz / s[0];
+  }
+}
+
+// Part 2 (dispersion.rs:519-526): w += alpha p, r -= alpha Ap, z = r / diag (stored for the
+// next iteration's p), |r|, r.z, the convergence test and beta.
+__global__ void __launch_bounds__(NT) k_cg_update2(SgnArgs a, const double* pcx,
+                                                   const double* pcy) {
+  CgScalars* sc = a.sc;
+  if (sc->done) return;
+  const Geo& g = a.g;
+  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  const int tx = tile % g.ntx, ty = tile / g.ntx;
+  const int i = GH + tx * TX + (threadIdx.x & 63);
+  const int j = GH + ty * TY + (threadIdx.x >> 6);
+  const double alpha = sc->alpha;
+  const double na = -alpha;
+  double v[2] = {0.0, 0.0};
+  if (i < g.nx + GH && j < g.ny + GH) {
+    const int k = j * g.w + i;
+    a.wx[k] = a.wx[k] + alpha * pcx[k];
+    a.wy[k] = a.wy[k] + alpha * pcy[k];
+    const double rx = a.rx[k] + na * a.ax[k];
+    const double ry = a.ry[k] + na * a.ay[k];
+    a.rx[k] = rx;
+    a.ry[k] = ry;
+    double dgx, dgy;
+    diag_at(a, k, a.h[k], dgx, dgy);
+    const double zx = rx / dgx, zy = ry / dgy;
+    a.zx[k] = zx;
+    a.zy[k] = zy;
+    v[0] = rx * rx + ry * ry;
+    v[1] = rx * zx + ry * zy;
+  }
+  if (publish_tile<2>(v, a.part, gridDim.x, tile, &sc->ticket[2])) {
+    double s[2];
+    final_sums<2>(a.part, gridDim.x, s);
+    if (threadIdx.x == 0) {
+      sc->it += 1;
+      sc->converged = sqrt(s[0]) <= sc->target ? 1 : 0;
+      sc->beta = s[1] / sc->rz;
+      sc->rz = s[1];
+      sc->done = (sc->converged || sc->it >= a.max_iterations) ? 1 : 0;
+    }
+  }
+}
+
 // The end of a solve: the counters of DispersionStats (dispersion.rs:541-545).
 __global__ void k_cg_stats(CgScalars* sc) {
   if (threadIdx.x == 0) {

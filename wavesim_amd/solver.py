@@ -63,6 +63,7 @@ SIGNATURES = {
     "wc_solver_with_shallow_cells": (_I32, [_VP, ctypes.POINTER(ctypes.c_uint8), _U64]),
     "wc_solver_bed": (_I32, [_VP, _P]),
     "wc_solver_set_precision": (_I32, [_VP, _I32]),
+    "wc_solver_set_option": (_I32, [_VP, ctypes.c_char_p, ctypes.c_int64]),
     "wc_state_set": (_I32, [_VP, _P, _P, _P, _P, _D]),
     "wc_state_get": (_I32, [_VP, _P, _P, _P, _P, _P]),
     "wc_state_set_time": (_I32, [_VP, _D]),
@@ -197,6 +198,11 @@ class Solver:
         a = np.ascontiguousarray(np.asarray(cells, dtype=bool).astype(np.uint8).ravel())
         _check(lib().wc_solver_with_shallow_cells(
             self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), a.size))
+        return self
+
+    def set_option(self, key: str, value: int) -> "Solver":
+        """Implementation choices that change speed, never results (wc_solver_set_option)."""
+        _check(lib().wc_solver_set_option(self._h, key.encode(), int(value)))
         return self
 
     def with_precision(self, precision: int) -> "Solver":
