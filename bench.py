@@ -27,20 +27,28 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-updates/s (FV+SGN step) at equal accuracy; PCG iters/solve; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes per interior cell of one launch of each kernel family (FP64: each
-# array read or written once, halo re-reads excluded), for the roofline.
+# Algorithmic HBM bytes per interior cell of one launch of each kernel family (each array read
+# or written once, halo re-reads excluded), for the roofline.
 BYTES_PER_CELL = {
-    "cg_matvec": 8 * (2 + 1 + 1 + 2),      # read p(2), c, h; write Ap(2)
-    "cg_update": 8 * (2 + 2 + 2 + 2 + 2 + 2 + 2),  # read w, p, r, Ap, diag; write w, r
-    "cg_pupdate": 8 * (2 + 2 + 2 + 2),     # read r, diag, p; write p
-    "cg_init": 8 * (2 + 1 + 1 + 2 + 2 + 2 + 2),  # read w, c, h, b, diag; write r, p
-    "sgn_setup": 8 * (7 + 5),              # read h, hu, hv, phi, lh, lhu, lhv; write c, diag, b
-    "flux": 8 * (5 + 3),                   # read h, hu, hv, b, phi; write lh, lhu, lhv
-    "sgn_finish": 8 * (3 + 3 + 2 + 3 + 3 + 1) // 2 * 2,  # stage in + rates + w + (s, b) + out
-    "switch": 8 * 3,                       # read h, b; write phi (per padded cell ~ interior)
-    "fill_state": 0,
+    "fp64": {
+        "cg_matvec": 8 * (2 + 2 + 1 + 1 + 2 + 2),  # read p_old, z, c, h; write p, Ap
+        "cg_update": 8 * (2 + 2 + 2 + 2 + 1 + 1 + 2 + 2 + 2),  # read w, p, r, Ap, c, h; write w, r, z
+        "cg_init": 8 * (2 + 1 + 1 + 2 + 2 + 2 + 2),  # read w, c, h, b, diag; write r, p
+        "sgn_setup": 8 * (7 + 5),  # read h, hu, hv, phi, lh, lhu, lhv; write c, diag, b
+        "flux": 8 * (5 + 3),  # read h, hu, hv, b, phi; write lh, lhu, lhv
+        "sgn_finish": 8 * (3 + 3 + 2 + 3 + 1 + 3),  # read s_in, rates, w, s, b; write s
+        "switch": 8 * 3,  # read h, b; write phi
+    },
+    "fp32": {
+        "cg_matvec": 4 * (2 + 2 + 1 + 1 + 2 + 2),
+        "cg_update": 4 * (2 + 2 + 2 + 2 + 1 + 1 + 2 + 2 + 2),
+        "cg_init": 4 * (2 + 1 + 1 + 2 + 2 + 2),  # read w, c, h, b; write r, p
+        "sgn_setup": 4 * (3 + 1 + 2 + 3) + 8,  # read h, hu, hv, phi, lhu, lhv, c; lh FP64; write c, b
+        "flux": 4 * (2 + 2 + 1 + 2) + 8 + 8,  # read h pair, hu, hv, phi; b FP64; write lhu, lhv; lh FP64
+        "sgn_finish": 4 * (6 + 3 + 2 + 2 + 6) + 8 * 3,  # pair, s1, w, rates; l1h, lh, b; write pair
+        "switch": 4 * 3 + 8,
+    },
 }
-
 
 def parse():
     p = argparse.ArgumentParser(description=__doc__)
@@ -50,14 +58,104 @@ def parse():
     p.add_argument("--spinup", type=int, default=1500)
     p.add_argument("--workload", default="rough_bed", choices=["rough_bed", "plane_beach"])
     p.add_argument("--profile-steps", type=int, default=20)
-    p.add_argument("--cpu-steps", type=int, default=4)
+    p.add_argument("--cpu-seconds", type=float, default=10.0,
+                   help="CPU work per thread count for the baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--precision", default="both", choices=["fp64", "fp32", "both"])
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="only run a device copy of known size (to scale HBM counters) and exit")
     return p.parse_args()
 
 
-def cpu_baseline(w, state_arrays, t0, steps):
-    """The CPU FP64 restatement (reference row order, glibc) on the host's cores, on `steps`
-    steps from the same mid-run state; cell-updates/s."""
+def calibrate_copy():
+    """A plain device copy of 2 x 256 MiB, ten times, for scaling FETCH_SIZE / WRITE_SIZE."""
+    import torch
+
+    n = 256 * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(10):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read_per_launch": 4 * n, "copy_bytes_written_per_launch": 4 * n,
+                      "launches": 10}))
+
+
+def measure(w, precision, arrays, t0, warmup, steps, profile_steps):
+    """Time `steps` steps of one precision from the given state; per-kernel times with HIP
+    events on the solver's stream."""
+    import torch
+
+    from wavesim_amd.grid import State
+    from wavesim_amd.solver import PRECISION_FP32, Solver
+
+    g = w.solver(Solver)
+    if precision == "fp32":
+        g.with_precision(PRECISION_FP32)
+    h, hu, hv, br = arrays
+    s = State(h.copy(), hu.copy(), hv.copy(), br.copy(), t0)
+    g.run_steps(s, warmup)
+    st0 = g.dispersion_stats()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    done, last_dt = g.run_steps(s, steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t
+    st1 = g.dispersion_stats()
+    if done != steps:
+        raise SystemExit(f"{precision}: the run stopped after {done} steps (dt = {last_dt})")
+    solves = st1.solves - st0.solves if st1 else 0
+    iters = st1.iterations - st0.iterations if st1 else 0
+    n_cells = w.grid.nx * w.grid.ny
+    g.profile(True)
+    g.profile_reset()
+    g.run_steps(s, profile_steps)
+    prof = g.profile_read()
+    g.profile(False)
+    fam = {k: v for k, v in prof.items() if v[1] > 0}
+    table = BYTES_PER_CELL[precision]
+    data = {k: v for k, v in fam.items() if table.get(k, 0) > 0}
+    dominant = max(data, key=lambda k: data[k][0])
+    tot_ms, launches = data[dominant]
+    avg_ms = tot_ms / launches
+    achieved = table[dominant] * n_cells / (avg_ms * 1e-3) / 1e9
+    kernels = {k: {"total_ms": round(v[0], 4), "launches": v[1],
+                   "avg_us": round(1e3 * v[0] / v[1], 3),
+                   "bytes_per_cell": table.get(k, 0)} for k, v in sorted(fam.items())}
+    print(f"[bench] {precision}: {steps} steps in {elapsed:.3f} s = "
+          f"{n_cells * steps / elapsed / 1e6:.1f} M cell-updates/s, "
+          f"{iters / max(solves, 1):.1f} PCG iterations per solve; {dominant} "
+          f"{1e3 * avg_ms:.1f} us = {achieved:.0f} GB/s", file=sys.stderr, flush=True)
+    return {
+        "value": n_cells * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+        "pcg_iters_per_solve": iters / max(solves, 1), "dominant": dominant,
+        "avg_ms": avg_ms, "achieved": achieved, "bytes_per_cell": table[dominant],
+        "kernels": kernels, "solver": g, "state": s,
+    }
+
+
+def accuracy(w, r64, r32):
+    """The FP32 form against FP64 after the same window from the same state."""
+    grid = w.grid
+    b = grid.inner(w.bed.b)
+    s64, s32 = r64["state"], r32["state"]
+    h64, h32 = grid.inner(s64.arrays()[0]), grid.inner(s32.arrays()[0])
+    wet = (h64 > 0.05) | (h32 > 0.05)
+    d = np.where(wet, (h32 + b) - (h64 + b), 0.0)
+    m64 = r64["solver"].breaking_indicator(s64) >= 0.8
+    m32 = r32["solver"].breaking_indicator(s32) >= 0.8
+    v64, v32 = r64["solver"].volume(s64), r32["solver"].volume(s32)
+    return {"eta_max_err_m": float(np.max(np.abs(d))),
+            "eta_rms_err_m": float(np.sqrt(np.mean(d[wet] ** 2))),
+            "breaking_mask_agreement": float(np.mean(m64 == m32)),
+            "breaking_cells_fp64": int(m64.sum()),
+            "volume_rel_diff": abs(v32 - v64) / v64,
+            "time_diff_s": abs(s32.time - s64.time)}
+
+
+def cpu_baseline(w, state_arrays, t0, seconds):
+    """The CPU FP64 restatement (reference row order, glibc) on the host's cores, from the same
+    mid-run state, for about `seconds` of CPU work per thread count; cell-updates/s."""
     from oracle import oracle as orc
     from wavesim_amd.grid import State
 
@@ -67,11 +165,15 @@ def cpu_baseline(w, state_arrays, t0, steps):
         o = w.solver(lambda g, b: orc.OracleSolver(g, b, threads=threads))
         h, hu, hv, br = state_arrays
         s = State(h.copy(), hu.copy(), hv.copy(), br.copy(), t0)
-        n = steps if threads > 1 else max(1, steps // 4)
+        n, el = 0, 0.0
         t = time.perf_counter()
-        o.step_n(s, n)
-        el = time.perf_counter() - t
+        while el < seconds and n < 10000:
+            o.step_n(s, 1)
+            n += 1
+            el = time.perf_counter() - t
         out[threads] = (w.grid.nx * w.grid.ny * n / el, n, el)
+        print(f"[bench] cpu baseline, {threads} threads: {n} steps in {el:.1f} s",
+              file=sys.stderr, flush=True)
     return cores, out
 
 
@@ -90,52 +192,36 @@ def main():
         print(json.dumps({"error": "no GPU"}))
         sys.exit(1)
     torch.cuda.init()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     w = configs.c1_rough_bed() if args.workload == "rough_bed" else configs.c0_plane_beach()
     g = w.solver(Solver)
     s = w.still_water()
-    n_cells = w.grid.nx * w.grid.ny
     print(f"[bench] {w.name}: spin-up {args.spinup} steps", file=sys.stderr, flush=True)
     t = time.perf_counter()
     done, _ = g.run_steps(s, args.spinup)
     print(f"[bench] spin-up {done} steps in {time.perf_counter() - t:.1f} s, t = {s.time:.1f} s",
           file=sys.stderr, flush=True)
-    g.run_steps(s, args.warmup)
-    st0 = g.dispersion_stats()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    done, last_dt = g.run_steps(s, args.steps)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    st1 = g.dispersion_stats()
-    if done != args.steps:
-        raise SystemExit(f"the run stopped after {done} steps (dt = {last_dt})")
-    solves = st1.solves - st0.solves if st1 else 0
-    iters = st1.iterations - st0.iterations if st1 else 0
-    value = n_cells * args.steps / elapsed
-    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s: {value / 1e6:.1f} M cell-updates/s,"
-          f" {iters / max(solves, 1):.1f} PCG iterations per solve", file=sys.stderr, flush=True)
-
-    # Per-kernel device time, with HIP events on the solver's stream.
-    g.profile(True)
-    g.profile_reset()
-    g.run_steps(s, args.profile_steps)
-    prof = g.profile_read()
-    g.profile(False)
-    fam = {k: v for k, v in prof.items() if v[1] > 0}
-    dominant = max(fam, key=lambda k: fam[k][0])
-    tot_ms, launches = fam[dominant]
-    avg_ms = tot_ms / launches
-    bpc = BYTES_PER_CELL.get(dominant, 0)
-    achieved = bpc * n_cells / (avg_ms * 1e-3) / 1e9
-    kernels = {k: {"total_ms": round(v[0], 4), "launches": v[1],
-                   "avg_us": round(1e3 * v[0] / v[1], 3)} for k, v in sorted(fam.items())}
+    arrays = [a.copy() for a in s.arrays()]
+    t0 = s.time
+    del g
+    precisions = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
+    res = {p: measure(w, p, arrays, t0, args.warmup, args.steps, args.profile_steps)
+           for p in precisions}
+    acc = accuracy(w, res["fp64"], res["fp32"]) if len(res) == 2 else None
+    chosen = "fp64"
+    if acc is not None and acc["eta_max_err_m"] < 1e-3 and acc["breaking_mask_agreement"] >= 0.999:
+        chosen = "fp32"
+    elif args.precision == "fp32":
+        chosen = "fp32"
+    r = res[chosen]
 
     cpu = None
     if not args.no_cpu:
-        arrays = [a.copy() for a in s.arrays()]
-        cores, res = cpu_baseline(w, arrays, s.time, args.cpu_steps)
-        v, n, el = res[cores]
-        v1, n1, el1 = res[1]
+        cores, cres = cpu_baseline(w, arrays, t0, args.cpu_seconds)
+        v, n, el = cres[cores]
+        v1, n1, el1 = cres[1]
         cpu = {"value": v, "unit": "cell-updates/s", "cores": cores, "kind": "port",
                "sample": f"{n} steps of the same workload from the same mid-run state "
                          f"({el:.1f} s), C++ FP64 restatement, reference row order and glibc, "
@@ -144,25 +230,32 @@ def main():
 
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": r["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if chosen == "fp32" else "f64",
         "data": "synthetic",
         "config": {"workload": w.name, "nx": w.grid.nx, "ny": w.grid.ny, "dx_m": w.grid.dx,
-                   "precision": "fp64 plain path", "pcg_tolerance": 1e-8,
-                   "spinup_steps": args.spinup, "t_start_s": round(s.time, 3), **w.notes},
-        "pcg_iters_per_solve": iters / max(solves, 1),
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_cell": bpc, "avg_launch_us": 1e3 * avg_ms},
-        "kernels": kernels,
+                   "path": ("FP32 chosen form (FP32 storage, FP64 inner products, norms, PCG "
+                            "scalars, surface-minus-bed differences)" if chosen == "fp32"
+                            else "FP64 plain path"),
+                   "preconditioner": "Jacobi (the reference's)", "pcg_tolerance": 1e-8,
+                   "spinup_steps": args.spinup, "t_start_s": round(t0, 3), **w.notes},
+        "pcg_iters_per_solve": r["pcg_iters_per_solve"],
+        "roofline": {"bound": "hbm", "kernel": r["dominant"], "achieved": r["achieved"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["achieved"] / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_cell": r["bytes_per_cell"],
+                     "avg_launch_us": 1e3 * r["avg_ms"]},
+        "paths": {p: {"value": v["value"], "ms_per_step": v["ms_per_step"],
+                      "pcg_iters_per_solve": v["pcg_iters_per_solve"], "kernels": v["kernels"]}
+                  for p, v in res.items()},
+        "fp32_vs_fp64": acc,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line, default=float))

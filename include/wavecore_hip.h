@@ -40,6 +40,11 @@ extern "C" {
 
 /* Arithmetic of the step. */
 #define WC_PRECISION_FP64 0 /* the plain path: FP64 throughout, matches the FP64 oracle */
+/* The chosen form: FP32 storage and reconstruction, the state as FP32 (hi, lo) pairs with a
+ * compensated update, FP64 inner products, norms, PCG scalars and surface-minus-bed differences.
+ * Differs from FP64 by rounding; in this mode the ghosts returned by wc_state_get are refilled
+ * from the final interior. */
+#define WC_PRECISION_FP32 1
 
 typedef struct wc_solver wc_solver;
 
@@ -110,9 +115,11 @@ int wc_solver_with_shallow_cells(wc_solver* s, const uint8_t* cells, uint64_t n)
 int wc_solver_bed(const wc_solver* s, double* out);
 /* Implementation choices that change speed, never results (no counterpart in the reference):
  *   "pcg_kernels" 1 = three kernels per PCG iteration, 2 = two fused kernels (default);
- *   "pcg_batch"   PCG iterations queued between looks at the convergence flag (default 4). */
+ *   "pcg_batch"   PCG iterations queued between looks at the convergence flag (default 4);
+ *   "flux_kernel" 1 = one thread per cell, as the reference's loop; 2 = LDS tiles, each cell
+ *                 reconstructed and each face fluxed once (default). */
 int wc_solver_set_option(wc_solver* s, const char* key, int64_t value);
-/* Selects the arithmetic (WC_PRECISION_*); FP64 by default. */
+/* Selects the arithmetic (WC_PRECISION_*); FP64 by default. Switching converts the held state. */
 int wc_solver_set_precision(wc_solver* s, int32_t precision);
 
 /* The state the solver steps (State, state.rs:7-17): padded h, hu, hv, breaking and the time.

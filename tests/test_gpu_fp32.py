@@ -1,0 +1,113 @@
+"""The FP32 chosen form against the FP64 path, on the GPU.
+
+The FP32 form stores the bulk in FP32 and keeps FP64 where accuracy lives (the state as FP32
+pairs with a compensated update, inner products, norms, PCG scalars, surface-minus-bed
+differences). It is held to the bounds of its specification: surface elevation within 1e-3 m
+of FP64, the breaking mask (indicator >= 0.8, crates/wavesim/src/lib.rs:1222) identical on at
+least 99.9% of cells, and the reference's own still-water and conservation checks.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import known_answers as ka
+from wavesim_amd import configs
+from wavesim_amd.config import Dispersion, Order
+from wavesim_amd.grid import GHOST, Bathymetry, Grid, State
+from wavesim_amd.solver import PRECISION_FP32, Solver
+
+pytestmark = pytest.mark.gpu
+
+
+def fp32(grid, bed):
+    return Solver(grid, bed).with_precision(PRECISION_FP32)
+
+
+def compare(w, s64: State, s32: State, g64: Solver, g32: Solver):
+    grid = w.grid
+    b = grid.inner(w.bed.b)
+    h64, h32 = grid.inner(s64.arrays()[0]), grid.inner(s32.arrays()[0])
+    wet = (h64 > 0.05) | (h32 > 0.05)
+    eta64, eta32 = h64 + b, h32 + b
+    d = np.where(wet, eta32 - eta64, 0.0)
+    m64 = g64.breaking_indicator(s64) >= 0.8
+    m32 = g32.breaking_indicator(s32) >= 0.8
+    return {
+        "eta_max_err_m": float(np.max(np.abs(d))),
+        "eta_rms_err_m": float(np.sqrt(np.mean(d[wet] ** 2))) if wet.any() else 0.0,
+        "mask_agreement": float(np.mean(m64 == m32)),
+        "breaking_cells": int(m64.sum()),
+        "volume_rel_diff": abs(g32.volume(s32) - g64.volume(s64)) / g64.volume(s64),
+    }
+
+
+def run_pair(w, steps, t0=0.0, state=None):
+    g64 = w.solver(Solver)
+    g32 = w.solver(fp32)
+    s64 = state.clone() if state is not None else w.still_water()
+    s64.time = t0
+    s32 = s64.clone()
+    n64, _ = g64.run_steps(s64, steps)
+    n32, _ = g32.run_steps(s32, steps)
+    assert n64 == steps and n32 == steps
+    return g64, g32, s64, s32
+
+
+def test_fp32_rough_bed_matches_fp64_within_its_bounds():
+    w = configs.c1_rough_bed(nx=256, ny=128)
+    g64, g32, s64, s32 = run_pair(w, 600, t0=40.0)
+    r = compare(w, s64, s32, g64, g32)
+    print("fp32 vs fp64:", r, g64.dispersion_stats(), g32.dispersion_stats(), flush=True)
+    assert r["eta_max_err_m"] < 1e-3, r
+    assert r["mask_agreement"] >= 0.999, r
+    st = g32.dispersion_stats()
+    assert st.unconverged == 0
+
+
+def test_fp32_plane_beach_fv_matches_fp64():
+    w = configs.c0_plane_beach()
+    g64, g32, s64, s32 = run_pair(w, 2000)
+    r = compare(w, s64, s32, g64, g32)
+    print("fp32 vs fp64 (FV):", r, flush=True)
+    assert r["eta_max_err_m"] < 1e-3, r
+
+
+def test_fp32_lake_at_rest_and_closed_basin_volume():
+    # well_balanced.rs:16-62 and 87-121 on the FP32 form
+    for mean_depth, amplitude in [(8.0, 3.0), (0.5, 2.5)]:
+        grid = Grid(64, 48, 3.0, 3.0)
+        bed = Bathymetry.rough(grid, mean_depth, amplitude)
+        s = State.lake_at_rest(grid, bed, 0.0)
+        g = fp32(grid, bed).with_order(Order.Fifth).with_dispersion(Dispersion())
+        v0 = s.volume(grid)
+        g.run_steps(s, 300)
+        assert s.max_abs_momentum(grid) < 1e-9, s.max_abs_momentum(grid)
+        assert abs(s.volume(grid) - v0) < 1e-9 * v0
+        assert s.min_depth(grid) >= 0.0
+    ka.check_walls_and_periodic_seam_keep_volume(fp32, cases=[(False, True), (True, True)])
+
+
+def test_fp32_standing_wave_frequency():
+    out = ka.check_standing_wave_sgn_frequency(fp32, orders=(Order.Fifth,))
+    assert abs(out["Fifth"] - 1.4261) < 3e-3, out
+
+
+def test_switching_precision_converts_the_state():
+    w = configs.c1_rough_bed(nx=96, ny=48)
+    g = w.solver(Solver)
+    s = w.still_water()
+    s.time = 30.0
+    g.run_steps(s, 50)
+    before = [a.copy() for a in s.arrays()[:3]]
+    g.with_precision(PRECISION_FP32)
+    s._host_valid = False  # fetch the converted state back from the device
+    after = s.arrays()[:3]
+    for a, b in zip(after, before):
+        inner = w.grid.inner(a) - w.grid.inner(b)
+        assert np.max(np.abs(inner)) <= 1e-12 * max(1.0, np.max(np.abs(b)))
+    n, dt = g.run_steps(s, 20)
+    assert n == 20 and math.isfinite(dt)

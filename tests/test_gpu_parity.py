@@ -18,7 +18,7 @@ import pytest
 import known_answers as ka
 from oracle import oracle as orc
 from wavesim_amd import configs
-from wavesim_amd.config import Dispersion, Order, Sponge, WaveMaker
+from wavesim_amd.config import Dispersion, Order, Sponge, WaveMaker, linear_wave
 from wavesim_amd.grid import GHOST, Bathymetry, Grid, State
 from wavesim_amd.solver import Solver
 
@@ -85,17 +85,20 @@ def hump_state(grid: Grid, bed: Bathymetry, amp=0.6, x0=None, y0=None, r=6.0):
     return s
 
 
-def test_plane_beach_fv_2000_steps_is_bitwise_the_oracle():
+@pytest.mark.parametrize("flux", [1, 2])
+def test_plane_beach_fv_2000_steps_is_bitwise_the_oracle(flux):
     w = configs.c0_plane_beach()
-    lockstep(w.grid, w.bed, w.configure, w.still_water(), w.steps, w.name)
+    lockstep(w.grid, w.bed, w.configure, w.still_water(), w.steps, f"{w.name} flux={flux}",
+             options={"flux_kernel": flux})
 
 
+@pytest.mark.parametrize("flux", [1, 2])
 @pytest.mark.parametrize("order", [Order.First, Order.Second, Order.Fifth])
-def test_each_order_over_a_rough_bed_with_dry_land(order):
+def test_each_order_over_a_rough_bed_with_dry_land(order, flux):
     grid = Grid(96, 40, 3.0, 3.0)
     bed = Bathymetry.rough(grid, 1.5, 2.5)
     lockstep(grid, bed, lambda s: s.with_order(order), hump_state(grid, bed, amp=1.0),
-             150, f"rough-dry {order.name}")
+             150, f"rough-dry {order.name} flux={flux}", options={"flux_kernel": flux})
 
 
 @pytest.mark.parametrize("pcg", [1, 2])
@@ -130,7 +133,7 @@ def test_sgn_rough_bed_tight_tolerance_with_maker_sponges_and_friction(pcg):
     s = w.still_water()
     s.time = 20.0
     lockstep(w.grid, w.bed, configure, s, w.steps, f"sgn rough bed pcg={pcg}",
-             options={"pcg_kernels": pcg, "pcg_batch": 3})
+             options={"pcg_kernels": pcg, "pcg_batch": 3, "flux_kernel": pcg})
 
 
 def test_sgn_second_order_linear_dispersion_and_held_cells():
@@ -152,22 +155,20 @@ def test_oblique_maker_in_a_periodic_channel():
     grid = Grid(200, 24, 2.0, 2.5)
     bed = Bathymetry.from_fn(grid, lambda x, y: -6.0 + x / 100.0)
     ly = grid.ny * grid.dy
-    k = 2.0 * math.pi / 60.0
-    # choose the angle so exactly one alongshore wave fits (waves.rs:114-116)
-    omega = k * math.sqrt(9.81 * 5.7)
-    period = 2.0 * math.pi / omega
+    period = 8.0
+    maker_x = 40.0
+    # the depth the wave-maker reads (solver.rs:405-406) sets k; one alongshore wave must fit
+    i = round(maker_x / grid.dx - 0.5) + GHOST
+    depth = -bed.b[grid.idx(i, GHOST + grid.ny // 2)]
+    k = linear_wave(2.0 * math.pi / period, depth, False).k
     angle = math.asin(2.0 * math.pi / (k * ly))
-    maker = WaveMaker(40.0, 0.1, period, angle, 4.0, 1.0)
+    maker = WaveMaker(maker_x, 0.1, period, angle, 4.0, 1.0)
 
     def configure(sv):
         return (sv.with_periodic_y().with_wavemaker(maker)
                 .with_sponge(Sponge(12, 1.0).edges(True, True, False, False))
                 .with_sponge(Sponge(5, 0.3).edges(False, True, False, False)))
 
-    try:
-        configure(oracle(grid, bed.copy()))
-    except ValueError:
-        pytest.skip("angle does not fit in this discretisation")
     lockstep(grid, bed, configure, State.lake_at_rest(grid, bed, 0.0), 300, "oblique periodic")
 
 

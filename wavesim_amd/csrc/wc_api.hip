@@ -54,11 +54,13 @@ enum Family {
   F_CG_STATS,
   F_SGN_FINISH,
   F_REDUCE,
+  F_CG_REDUCE,
   F_COUNT
 };
 const char* kFamilyNames[F_COUNT] = {"fill_state", "switch",    "flux",      "finish_euler",
                                      "sgn_setup",  "cg_init",   "cg_matvec", "cg_update",
-                                     "cg_pupdate", "cg_stats",  "sgn_finish", "reduce"};
+                                     "cg_pupdate", "cg_stats",  "sgn_finish", "reduce",
+                                     "cg_reduce"};
 
 struct HostSponge {
   uint64_t width;
@@ -127,8 +129,20 @@ struct wc_solver {
          *py = nullptr, *ax = nullptr, *ay = nullptr;
   double *zx = nullptr, *zy = nullptr, *qx = nullptr, *qy = nullptr;  // fused PCG: z, 2nd p
 
+  // the FP32 chosen form
+  float *f_sh = nullptr, *f_shu = nullptr, *f_shv = nullptr, *f_slh = nullptr, *f_slhu = nullptr,
+        *f_slhv = nullptr;
+  float *f_s1h = nullptr, *f_s1hu = nullptr, *f_s1hv = nullptr;
+  double *f_l1h = nullptr, *f_lh = nullptr;  // mass rates stay FP64
+  float *f_l1hu = nullptr, *f_l1hv = nullptr;
+  float *f_lhu = nullptr, *f_lhv = nullptr, *f_phi = nullptr;
+  float *f_c = nullptr, *f_bx = nullptr, *f_by = nullptr, *f_wx = nullptr, *f_wy = nullptr,
+        *f_rx = nullptr, *f_ry = nullptr, *f_px = nullptr, *f_py = nullptr, *f_qx = nullptr,
+        *f_qy = nullptr, *f_zx = nullptr, *f_zy = nullptr, *f_ax = nullptr, *f_ay = nullptr;
+
   // implementation options (wc_solver_set_option)
   int pcg_kernels = 2;
+  int flux_kernel = 2;
   unsigned long long pcg_batch = 4;
   uint8_t* held = nullptr;
   double* part = nullptr;
@@ -138,7 +152,7 @@ struct wc_solver {
   double* prm = nullptr;
   size_t prm_len = 0;
   unsigned long long* red64 = nullptr;  // [0] max rate bits, [1] max momentum, [2] min-h key
-  unsigned int* red32 = nullptr;        // [0] ticket, [1] first NaN, [2] first max
+  unsigned int* red32 = nullptr;        // [1] first NaN, [2] first max
   double* redf = nullptr;               // [0] volume sum
   std::vector<void*> owned;
 
@@ -460,6 +474,7 @@ SgnArgs sgn_args(wc_solver* s, int stage) {
 int pcg(wc_solver* s, int stage) {
   const SgnArgs a = sgn_args(s, stage);
   TRY(launch(s, F_CG_INIT, k_cg_init, tiles(s), dim3(NT), a));
+  TRY(launch(s, F_CG_REDUCE, k_cg_reduce, dim3(1), dim3(NT), a, (int)REDUCE_INIT));
   const unsigned long long maxit = s->disp.max_iterations;
   unsigned long long launched = 0;
   unsigned long long batch = std::max<unsigned long long>(s->prev_it[stage], 1);
@@ -469,7 +484,9 @@ int pcg(wc_solver* s, int stage) {
     for (unsigned long long q = 0; q < n; ++q) {
       if (s->pcg_kernels == 1) {
         TRY(launch(s, F_CG_MATVEC, k_cg_matvec, tiles(s), dim3(NT), a));
+        TRY(launch(s, F_CG_REDUCE, k_cg_reduce, dim3(1), dim3(NT), a, (int)REDUCE_ALPHA));
         TRY(launch(s, F_CG_UPDATE, k_cg_update, tiles(s), dim3(NT), a));
+        TRY(launch(s, F_CG_REDUCE, k_cg_reduce, dim3(1), dim3(NT), a, (int)REDUCE_BETA));
         TRY(launch(s, F_CG_PUPDATE, k_cg_pupdate, tiles(s), dim3(NT), a));
         continue;
       }
@@ -487,8 +504,10 @@ int pcg(wc_solver* s, int stage) {
       }
       TRY(launch(s, F_CG_MATVEC, k_cg_matvec2, tiles(s), dim3(NT), a, first, old_x, old_y,
                  cur_x, cur_y));
+      TRY(launch(s, F_CG_REDUCE, k_cg_reduce, dim3(1), dim3(NT), a, (int)REDUCE_ALPHA));
       TRY(launch(s, F_CG_UPDATE, k_cg_update2, tiles(s), dim3(NT), a, (const double*)cur_x,
                  (const double*)cur_y));
+      TRY(launch(s, F_CG_REDUCE, k_cg_reduce, dim3(1), dim3(NT), a, (int)REDUCE_BETA));
     }
     launched += n;
     HIP_OK(hipMemcpyAsync(s->sc_host, s->sc, sizeof(CgScalars), hipMemcpyDeviceToHost,
@@ -518,11 +537,29 @@ int stage(wc_solver* s, double dt, int stage, int mode) {
                (const double*)s->b, breaking_of(s), (const uint8_t*)(s->has_held ? s->held : nullptr),
                s->phi));
   const StepArgs a = step_args(s, dt, stage);
-  if (!s->dispersive) {
-    TRY(launch(s, F_FLUX, k_flux, tiles(s), dim3(NT), a, mode));
-    return WC_OK;
+  const int flux_mode = s->dispersive ? (int)MODE_RATES : mode;
+  if (s->flux_kernel == 2) {
+    FluxTileArgs<double> f{};
+    f.g = a.g;
+    f.order = a.order;
+    f.h = a.in_h;
+    f.h_lo = nullptr;
+    f.hu = a.in_hu;
+    f.hv = a.in_hv;
+    f.b = a.b;
+    f.phi = a.phi;
+    f.has_drive = a.has_drive;
+    f.src_i0 = a.src_i0;
+    f.src_i1 = a.src_i1;
+    f.src_col = a.src_col;
+    f.prm = a.prm;
+    f.ramp_prm = a.ramp_prm;
+    f.sin_prm = a.sin_prm;
+    TRY(launch(s, F_FLUX, k_flux_tile64, tiles(s), dim3(NT), f, a, flux_mode));
+  } else {
+    TRY(launch(s, F_FLUX, k_flux, tiles(s), dim3(NT), a, flux_mode));
   }
-  TRY(launch(s, F_FLUX, k_flux, tiles(s), dim3(NT), a, (int)MODE_RATES));
+  if (!s->dispersive) return WC_OK;
   TRY(launch(s, F_SGN_SETUP, k_sgn_setup, tiles(s), dim3(NT), sgn_args(s, stage)));
   TRY(pcg(s, stage));
   TRY(launch(s, F_SGN_FINISH, k_sgn_finish, tiles(s), dim3(NT), a, (const double*)s->wx,
@@ -580,6 +617,218 @@ int alloc_dispersion(wc_solver* s) {
   for (double** p : {&s->c, &s->dgx, &s->dgy, &s->bx, &s->by, &s->wx, &s->wy, &s->rx, &s->ry,
                      &s->px, &s->py, &s->ax, &s->ay, &s->zx, &s->zy, &s->qx, &s->qy})
     TRY(dalloc(s, p, n));
+  return WC_OK;
+}
+
+int alloc_fp32(wc_solver* s) {
+  if (s->f_sh) return WC_OK;
+  const size_t n = s->cells;
+  TRY(dalloc(s, &s->f_l1h, n));
+  TRY(dalloc(s, &s->f_lh, n));
+  for (float** p : {&s->f_sh, &s->f_shu, &s->f_shv, &s->f_slh, &s->f_slhu, &s->f_slhv, &s->f_s1h,
+                    &s->f_s1hu, &s->f_s1hv, &s->f_l1hu, &s->f_l1hv,
+                    &s->f_lhu, &s->f_lhv, &s->f_phi, &s->f_c, &s->f_bx, &s->f_by, &s->f_wx,
+                    &s->f_wy, &s->f_rx, &s->f_ry, &s->f_px, &s->f_py, &s->f_qx, &s->f_qy,
+                    &s->f_zx, &s->f_zy, &s->f_ax, &s->f_ay})
+    TRY(dalloc(s, p, n));
+  return WC_OK;
+}
+
+// FP64 state arrays -> FP32 pairs (after an upload, or when switching to FP32).
+int split_state(wc_solver* s) {
+  const long long n = (long long)s->cells;
+  TRY(launch(s, F_REDUCE, k_split32, over(n), dim3(NT), n, (const double*)s->h, s->f_sh,
+             s->f_slh));
+  TRY(launch(s, F_REDUCE, k_split32, over(n), dim3(NT), n, (const double*)s->hu, s->f_shu,
+             s->f_slhu));
+  TRY(launch(s, F_REDUCE, k_split32, over(n), dim3(NT), n, (const double*)s->hv, s->f_shv,
+             s->f_slhv));
+  return WC_OK;
+}
+
+// FP32 pairs -> the FP64 state arrays, ghosts refilled from the interior (before a download
+// or a diagnostic, or when switching back to FP64).
+int join_state(wc_solver* s) {
+  if (s->precision != WC_PRECISION_FP32) return WC_OK;
+  const long long n = (long long)s->cells;
+  TRY(launch(s, F_REDUCE, k_join32, over(n), dim3(NT), n, (const float*)s->f_sh,
+             (const float*)s->f_slh, s->h));
+  TRY(launch(s, F_REDUCE, k_join32, over(n), dim3(NT), n, (const float*)s->f_shu,
+             (const float*)s->f_slhu, s->hu));
+  TRY(launch(s, F_REDUCE, k_join32, over(n), dim3(NT), n, (const float*)s->f_shv,
+             (const float*)s->f_slhv, s->hv));
+  TRY(fill(s, s->h, s->hu, s->hv, nullptr, nullptr, nullptr, nullptr));
+  return WC_OK;
+}
+
+Sgn32 sgn32_args(wc_solver* s, int stage) {
+  Sgn32 a{};
+  a.g = s->geo;
+  a.h = stage == 0 ? s->f_sh : s->f_s1h;
+  a.hu = stage == 0 ? s->f_shu : s->f_s1hu;
+  a.hv = stage == 0 ? s->f_shv : s->f_s1hv;
+  a.phi = s->f_phi;
+  a.lh = s->f_lh;
+  a.lhu = s->f_lhu;
+  a.lhv = s->f_lhv;
+  a.c = s->f_c;
+  a.bx = s->f_bx;
+  a.by = s->f_by;
+  a.wx = s->f_wx;
+  a.wy = s->f_wy;
+  a.rx = s->f_rx;
+  a.ry = s->f_ry;
+  a.px = s->f_px;
+  a.py = s->f_py;
+  a.ax = s->f_ax;
+  a.ay = s->f_ay;
+  a.zx = s->f_zx;
+  a.zy = s->f_zy;
+  a.part = s->part;
+  a.sc = s->sc;
+  a.nonlinear = s->disp.nonlinear ? 1 : 0;
+  a.tolerance = s->disp.tolerance;
+  a.max_iterations = s->disp.max_iterations;
+  return a;
+}
+
+int pcg32(wc_solver* s, int stage) {
+  const Sgn32 a = sgn32_args(s, stage);
+  TRY(launch(s, F_CG_INIT, k_cg_init32, tiles(s), dim3(NT), a));
+  TRY(launch(s, F_CG_REDUCE, k_cg_reduce32, dim3(1), dim3(NT), a, (int)REDUCE_INIT));
+  const unsigned long long maxit = s->disp.max_iterations;
+  unsigned long long launched = 0;
+  unsigned long long batch = std::max<unsigned long long>(s->prev_it[stage], 1);
+  bool seen = false;
+  while (launched < maxit) {
+    const unsigned long long n = std::min(batch, maxit - launched);
+    for (unsigned long long q = 0; q < n; ++q) {
+      const unsigned long long it = launched + q + 1;
+      const int first = it == 1 ? 1 : 0;
+      float* cur_x = (it % 2 == 1) ? s->f_px : s->f_qx;
+      float* cur_y = (it % 2 == 1) ? s->f_py : s->f_qy;
+      const float* old_x = first ? s->f_px : ((it % 2 == 1) ? s->f_qx : s->f_px);
+      const float* old_y = first ? s->f_py : ((it % 2 == 1) ? s->f_qy : s->f_py);
+      TRY(launch(s, F_CG_MATVEC, k_cg_matvec32, tiles(s), dim3(NT), a, first, old_x, old_y,
+                 cur_x, cur_y));
+      TRY(launch(s, F_CG_REDUCE, k_cg_reduce32, dim3(1), dim3(NT), a, (int)REDUCE_ALPHA));
+      TRY(launch(s, F_CG_UPDATE, k_cg_update32, tiles(s), dim3(NT), a, (const float*)cur_x,
+                 (const float*)cur_y));
+      TRY(launch(s, F_CG_REDUCE, k_cg_reduce32, dim3(1), dim3(NT), a, (int)REDUCE_BETA));
+    }
+    launched += n;
+    HIP_OK(hipMemcpyAsync(s->sc_host, s->sc, sizeof(CgScalars), hipMemcpyDeviceToHost,
+                          s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    seen = true;
+    if (s->sc_host->done) break;
+    batch = s->pcg_batch;
+  }
+  if (!seen) {
+    HIP_OK(hipMemcpyAsync(s->sc_host, s->sc, sizeof(CgScalars), hipMemcpyDeviceToHost,
+                          s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+  }
+  s->prev_it[stage] = s->sc_host->it;
+  TRY(launch(s, F_CG_STATS, k_cg_stats, dim3(1), dim3(64), s->sc));
+  return WC_OK;
+}
+
+int stage32(wc_solver* s, double dt, int stage, int mode) {
+  const Geo& g = s->geo;
+  float* in_h = stage == 0 ? s->f_sh : s->f_s1h;
+  const float* in_lo = stage == 0 ? s->f_slh : nullptr;
+  float* in_hu = stage == 0 ? s->f_shu : s->f_s1hu;
+  float* in_hv = stage == 0 ? s->f_shv : s->f_s1hv;
+  if (s->order == WC_ORDER_FIFTH || s->dispersive)
+    TRY(launch(s, F_SWITCH, k_switch32, over((long long)g.w * g.rows), dim3(NT), g,
+               (const float*)in_h, in_lo, (const double*)s->b, breaking_of(s),
+               (const uint8_t*)(s->has_held ? s->held : nullptr), s->f_phi));
+  FluxTileArgs<float> f{};
+  f.g = g;
+  f.order = s->order;
+  f.h = in_h;
+  f.h_lo = in_lo;
+  f.hu = in_hu;
+  f.hv = in_hv;
+  f.b = s->b;
+  f.phi = s->f_phi;
+  f.has_drive = s->has_drive ? 1 : 0;
+  f.src_i0 = s->src_i0;
+  f.src_i1 = s->src_i1;
+  f.src_col = s->src_col;
+  f.prm = s->prm;
+  f.ramp_prm = stage;
+  f.sin_prm = 2 + stage * g.ny;
+  TRY(launch(s, F_FLUX, k_flux_tile32, tiles(s), dim3(NT), f, s->f_lh, s->f_lhu, s->f_lhv));
+  if (s->dispersive) {
+    TRY(launch(s, F_SGN_SETUP, k_sgn_setup32, tiles(s), dim3(NT), sgn32_args(s, stage)));
+    TRY(pcg32(s, stage));
+  }
+  Stage32 a{};
+  a.g = g;
+  a.dt = dt;
+  a.b = s->b;
+  a.sh = s->f_sh;
+  a.shu = s->f_shu;
+  a.shv = s->f_shv;
+  a.sl_h = s->f_slh;
+  a.sl_hu = s->f_slhu;
+  a.sl_hv = s->f_slhv;
+  a.s1h = s->f_s1h;
+  a.s1hu = s->f_s1hu;
+  a.s1hv = s->f_s1hv;
+  a.l1h = s->f_l1h;
+  a.l1hu = s->f_l1hu;
+  a.l1hv = s->f_l1hv;
+  a.lh = s->f_lh;
+  a.lhu = s->f_lhu;
+  a.lhv = s->f_lhv;
+  a.wx = s->f_wx;
+  a.wy = s->f_wy;
+  a.dispersive = s->dispersive ? 1 : 0;
+  a.damp = step_args(s, dt, stage);
+  TRY(launch(s, F_SGN_FINISH, k_stage32, tiles(s), dim3(NT), a, mode));
+  return WC_OK;
+}
+
+int do_step32(wc_solver* s, double dt) {
+  const double t = s->time;
+  const Geo& g = s->geo;
+  TRY(upload_step_params(s, t, dt));
+  TRY(launch(s, F_FILL, k_fill32, over(6LL * g.w + 6LL * g.ny), dim3(NT), g, s->f_sh, s->f_slh,
+             s->f_shu, s->f_shv));
+  if (s->order == WC_ORDER_FIRST) {
+    TRY(stage32(s, dt, 0, MODE_EULER));
+  } else {
+    TRY(stage32(s, dt, 0, MODE_S1));
+    TRY(launch(s, F_FILL, k_fill32, over(6LL * g.w + 6LL * g.ny), dim3(NT), g, s->f_s1h,
+               (float*)nullptr, s->f_s1hu, s->f_s1hv));
+    TRY(stage32(s, dt, 1, MODE_FINAL));
+  }
+  s->time = t + dt;
+  return WC_OK;
+}
+
+int stable_dt32(wc_solver* s, double* dt) {
+  const Geo& g = s->geo;
+  HIP_OK(hipMemsetAsync(s->red64, 0, sizeof(unsigned long long), s->stream));
+  TRY(launch(s, F_REDUCE, k_max_rate32, over((long long)g.nx * g.ny), dim3(NT), g,
+             (const float*)s->f_sh, (const float*)s->f_slh, (const float*)s->f_shu,
+             (const float*)s->f_slhu, (const float*)s->f_shv, (const float*)s->f_slhv,
+             s->red64));
+  HIP_OK(hipMemcpyAsync(s->red64_host, s->red64, sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  double rate;
+  const unsigned long long bits = s->red64_host[0];
+  memcpy(&rate, &bits, sizeof rate);
+  if (isnan(rate))
+    *dt = NAN;
+  else if (rate > 0.0)
+    *dt = s->cfl / rate;
+  else
+    *dt = INFINITY;
   return WC_OK;
 }
 
@@ -807,6 +1056,11 @@ int wc_solver_set_option(wc_solver* s, const char* key, int64_t value) {
     s->pcg_kernels = (int)value;
     return WC_OK;
   }
+  if (strcmp(key, "flux_kernel") == 0) {
+    if (value != 1 && value != 2) return fail(WC_EINVAL, "flux_kernel must be 1 or 2");
+    s->flux_kernel = (int)value;
+    return WC_OK;
+  }
   if (strcmp(key, "pcg_batch") == 0) {
     if (value < 1 || value > 1000) return fail(WC_EINVAL, "pcg_batch must be in [1, 1000]");
     s->pcg_batch = (unsigned long long)value;
@@ -816,8 +1070,19 @@ int wc_solver_set_option(wc_solver* s, const char* key, int64_t value) {
 }
 
 int wc_solver_set_precision(wc_solver* s, int32_t precision) {
-  if (precision != WC_PRECISION_FP64) return fail(WC_EINVAL, "unknown precision");
-  s->precision = precision;
+  TRY(set_device(s));
+  if (precision != WC_PRECISION_FP64 && precision != WC_PRECISION_FP32)
+    return fail(WC_EINVAL, "precision must be WC_PRECISION_FP64 or WC_PRECISION_FP32");
+  if (precision == s->precision) return WC_OK;
+  if (precision == WC_PRECISION_FP32) {
+    TRY(alloc_fp32(s));
+    s->precision = precision;
+    TRY(split_state(s));
+  } else {
+    TRY(join_state(s));
+    s->precision = precision;
+  }
+  HIP_OK(hipStreamSynchronize(s->stream));
   return WC_OK;
 }
 
@@ -832,6 +1097,7 @@ int wc_state_set(wc_solver* s, const double* h, const double* hu, const double* 
     HIP_OK(hipMemcpyAsync(s->br, breaking, bytes, hipMemcpyHostToDevice, s->stream));
   else
     HIP_OK(hipMemsetAsync(s->br, 0, bytes, s->stream));
+  if (s->precision == WC_PRECISION_FP32) TRY(split_state(s));
   HIP_OK(hipStreamSynchronize(s->stream));
   s->time = time;
   return WC_OK;
@@ -841,6 +1107,7 @@ int wc_state_get(wc_solver* s, double* h, double* hu, double* hv, double* breaki
                  double* time) {
   TRY(set_device(s));
   const size_t bytes = s->cells * sizeof(double);
+  TRY(join_state(s));
   if (h) HIP_OK(hipMemcpyAsync(h, s->h, bytes, hipMemcpyDeviceToHost, s->stream));
   if (hu) HIP_OK(hipMemcpyAsync(hu, s->hu, bytes, hipMemcpyDeviceToHost, s->stream));
   if (hv) HIP_OK(hipMemcpyAsync(hv, s->hv, bytes, hipMemcpyDeviceToHost, s->stream));
@@ -857,12 +1124,12 @@ int wc_state_set_time(wc_solver* s, double time) {
 
 int wc_stable_dt(wc_solver* s, double* dt) {
   TRY(set_device(s));
-  return stable_dt(s, dt);
+  return s->precision == WC_PRECISION_FP32 ? stable_dt32(s, dt) : stable_dt(s, dt);
 }
 
 int wc_step(wc_solver* s, double dt) {
   TRY(set_device(s));
-  TRY(do_step(s, dt));
+  TRY(s->precision == WC_PRECISION_FP32 ? do_step32(s, dt) : do_step(s, dt));
   if (!s->prof) HIP_OK(hipStreamSynchronize(s->stream));
   return WC_OK;
 }
@@ -871,10 +1138,11 @@ int wc_run_steps(wc_solver* s, uint64_t steps, uint64_t* done_steps, double* las
   TRY(set_device(s));
   uint64_t n = 0;
   double dt = 0.0;
+  const bool f32 = s->precision == WC_PRECISION_FP32;
   for (; n < steps; ++n) {
-    TRY(stable_dt(s, &dt));
+    TRY(f32 ? stable_dt32(s, &dt) : stable_dt(s, &dt));
     if (!(isfinite(dt) && dt > 0.0)) break;
-    TRY(do_step(s, dt));
+    TRY(f32 ? do_step32(s, dt) : do_step(s, dt));
   }
   HIP_OK(hipStreamSynchronize(s->stream));
   if (done_steps) *done_steps = n;
@@ -884,6 +1152,7 @@ int wc_run_steps(wc_solver* s, uint64_t steps, uint64_t* done_steps, double* las
 
 int wc_limiting_cell(wc_solver* s, int32_t* found, wc_limiting_t* out) {
   TRY(set_device(s));
+  TRY(join_state(s));
   const Geo& g = s->geo;
   const long long n = (long long)g.nx * g.ny;
   HIP_OK(hipMemsetAsync(s->red64, 0, sizeof(unsigned long long), s->stream));
@@ -928,6 +1197,7 @@ int wc_limiting_cell(wc_solver* s, int32_t* found, wc_limiting_t* out) {
 
 int wc_breaking_indicator(wc_solver* s, double* out) {
   TRY(set_device(s));
+  TRY(join_state(s));
   const Geo& g = s->geo;
   const size_t n = (size_t)g.nx * g.ny;
   TRY(launch(s, F_REDUCE, k_breaking, over((long long)n), dim3(NT), g, (const double*)s->h,
@@ -950,13 +1220,14 @@ int wc_dispersion_stats(wc_solver* s, int32_t* has, wc_dispersion_stats_t* out) 
 }
 
 static int state_sums(wc_solver* s) {
+  TRY(join_state(s));
   const Geo& g = s->geo;
   HIP_OK(hipMemsetAsync(s->red64 + 1, 0, sizeof(unsigned long long), s->stream));
   HIP_OK(hipMemsetAsync(s->red64 + 2, 0xff, sizeof(unsigned long long), s->stream));
-  HIP_OK(hipMemsetAsync(s->red32, 0, sizeof(unsigned int), s->stream));
   TRY(launch(s, F_REDUCE, k_state_sums, tiles(s), dim3(NT), g, (const double*)s->h,
-             (const double*)s->hu, (const double*)s->hv, s->part, s->red32, s->redf,
-             s->red64 + 1, s->red64 + 2));
+             (const double*)s->hu, (const double*)s->hv, s->part, s->red64 + 1, s->red64 + 2));
+  TRY(launch(s, F_REDUCE, k_sum_tiles, dim3(1), dim3(NT), (const double*)s->part,
+             g.ntx * g.nty, s->redf));
   HIP_OK(hipMemcpyAsync(s->red64_host, s->red64, 4 * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, s->stream));
   HIP_OK(hipMemcpyAsync(s->redf_host, s->redf, sizeof(double), hipMemcpyDeviceToHost,

@@ -119,27 +119,17 @@ __device__ __forceinline__ void final_sums(const double* part, int np, double (&
   __syncthreads();
 }
 
-// Writes this tile's N sums and reports whether this block is the last to finish: that block
-// then sees every tile's sums. The ticket counter is reset by the last block.
+// Writes this tile's N sums to part[n * np + tile] with plain stores; a separate one-block
+// kernel adds them up once every tile has finished (the kernel boundary makes them visible on
+// every XCD, which a device-scope fence per block would do at the price of an L2 write-back).
 template <int N>
-__device__ __forceinline__ bool publish_tile(double (&v)[N], double* part, int np, int tile,
-                                             unsigned int* ticket) {
-  __shared__ int last;
+__device__ __forceinline__ void store_tile(double (&v)[N], double* part, int np, int tile) {
   double sums[N];
   tile_sums<N>(v, sums);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int n = 0; n < N; ++n) part[n * np + tile] = sums[n];
-    __threadfence();
-    const unsigned int t = atomicAdd(ticket, 1u);
-    last = (t == (unsigned int)(np - 1));
   }
-  __syncthreads();
-  if (last) {
-    __threadfence();
-    if (threadIdx.x == 0) *ticket = 0u;
-  }
-  return last != 0;
 }
 
 // ---- finite volumes (solver.rs:102-239) -------------------------------------------------------

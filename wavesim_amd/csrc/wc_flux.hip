@@ -1,0 +1,242 @@
+// The finite-volume rates on a 64x4 tile staged in LDS, each cell reconstructed once and each
+// face fluxed once (solver.rs:625-718 reconstructs every cell three times per axis and fluxes
+// every face twice; both are pure functions of their inputs, so doing each once gives the same
+// numbers).
+//
+// Template parameter R is the type the reconstruction runs in (face_values on the surface and
+// the velocities, solver.rs:125-138). The surface-minus-bed differences (face depths, the
+// hydrostatic reconstruction, solver.rs:164-190), the HLL flux and the rates stay in FP64 for
+// both: with R = double this is the reference's arithmetic bit for bit; with R = float it is
+// the FP32 chosen form, whose lake at rest stays at rest because eta - b is never formed in
+// FP32.
+
+#include "wc_kernels.h"
+#include "wc_stage.h"
+
+namespace wc {
+
+namespace {
+
+constexpr int BX = TX + 6, BY = TY + 6;  // the tile and three ghost rings
+constexpr int RXN = TX + 2;              // x-reconstructed cells per row: the tile +- 1
+constexpr int RYN = TY + 2;              // y-reconstructed rows: the tile +- 1
+
+template <typename R>
+struct SideT {
+  double h, z;
+  R un, ut;
+};
+
+template <typename R>
+__device__ __forceinline__ R minmod3r(R a, R b, R c) {
+  const R zero = R(0);
+  if (a > zero && b > zero && c > zero) return (R)wc_min(wc_min((double)a, (double)b), (double)c);
+  if (a < zero && b < zero && c < zero) return (R)wc_max(wc_max((double)a, (double)b), (double)c);
+  return zero;
+}
+
+// solver.rs:125-138 in R arithmetic.
+template <typename R>
+__device__ __forceinline__ void face_values_r(R low2, R low, R c, R high, R high2, R blend, R& lo,
+                                              R& hi) {
+  const R s = minmod3r<R>(R(2) * (c - low), R(0.5) * ((c - low) + (high - c)), R(2) * (high - c));
+  const R mc_lo = c - R(0.5) * s, mc_hi = c + R(0.5) * s;
+  if (blend <= R(0)) {
+    lo = mc_lo;
+    hi = mc_hi;
+    return;
+  }
+  const R fifth_hi = (R(2) * low2 - R(13) * low + R(47) * c + R(27) * high - R(3) * high2) / R(60);
+  const R fifth_lo = (R(2) * high2 - R(13) * high + R(47) * c + R(27) * low - R(3) * low2) / R(60);
+  lo = mc_lo + blend * (fifth_lo - mc_lo);
+  hi = mc_hi + blend * (fifth_hi - mc_hi);
+}
+
+// solver.rs:146-183 for the cell in the middle of five, reading the staged tile along a line:
+// e[], d[] (depth, FP64), n[] (normal velocity), t[] (tangential), b[] (bed, FP64), p[] (phi).
+template <typename R>
+__device__ __forceinline__ void reconstruct_r(const R* e, const double* d, const R* n, const R* t,
+                                              const double* b, const R* p, int stride, int order,
+                                              SideT<R>& lo_out, SideT<R>& hi_out) {
+  const double h1 = d[stride], h2 = d[2 * stride], h3 = d[3 * stride];
+  const SideT<R> first{h2, b[2 * stride], n[2 * stride], t[2 * stride]};
+  if (order == 0 || h1 < WC_H_SECOND || h2 < WC_H_SECOND || h3 < WC_H_SECOND) {
+    lo_out = first;
+    hi_out = first;
+    return;
+  }
+  R blend = R(0);
+  if (order == 2 && d[0] >= WC_H_SECOND && d[4 * stride] >= WC_H_SECOND) {
+    double bl = 1.0;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) bl = wc_min(bl, (double)p[m * stride]);
+    blend = (R)bl;
+  }
+  R eta_lo, eta_hi, un_lo, un_hi, ut_lo, ut_hi;
+  face_values_r<R>(e[0], e[stride], e[2 * stride], e[3 * stride], e[4 * stride], blend, eta_lo,
+                   eta_hi);
+  face_values_r<R>(n[0], n[stride], n[2 * stride], n[3 * stride], n[4 * stride], blend, un_lo,
+                   un_hi);
+  face_values_r<R>(t[0], t[stride], t[2 * stride], t[3 * stride], t[4 * stride], blend, ut_lo,
+                   ut_hi);
+  const double cb = b[2 * stride];
+  const double s_z = 0.25 * (b[3 * stride] - b[stride]);
+  SideT<R> l, r;
+  l.z = cb + -1.0 * s_z;
+  l.h = (double)eta_lo - l.z;
+  l.un = un_lo;
+  l.ut = ut_lo;
+  r.z = cb + 1.0 * s_z;
+  r.h = (double)eta_hi - r.z;
+  r.un = un_hi;
+  r.ut = ut_hi;
+  if (l.h < WC_H_FACE_MIN || r.h < WC_H_FACE_MIN) {
+    lo_out = first;
+    hi_out = first;
+    return;
+  }
+  lo_out = l;
+  hi_out = r;
+}
+
+template <typename R>
+__device__ __forceinline__ Face face_r(const SideT<R>& a, const SideT<R>& b) {
+  const Side l{a.h, (double)a.un, (double)a.ut, a.z};
+  const Side r{b.h, (double)b.un, (double)b.ut, b.z};
+  return face(l, r);
+}
+
+template <typename R>
+struct TileLds {
+  union {
+    struct {
+      R eta[BY][BX], un_x[BY][BX], un_y[BY][BX], phi[BY][BX];
+      double h[BY][BX], b[BY][BX];
+    } box;
+    struct {
+      Face fx[TY][TX + 1];
+      Face fy[TY + 1][TX];
+    } faces;
+  } u;
+  SideT<R> xlo[TY][RXN], xhi[TY][RXN];
+  SideT<R> ylo[RYN][TX], yhi[RYN][TX];
+};
+
+}  // namespace
+
+// Rates of every tile cell, then the stage epilogue of `mode` (see stage_out).
+// in_lo: the low part of the depth in the FP32 chosen form (null in FP64).
+template <typename R>
+__device__ __forceinline__ void flux_tile(const FluxTileArgs<R>& a, int mode, double& rh,
+                                          double& rhu, double& rhv, int& i_out, int& j_out,
+                                          bool& valid) {
+  const Geo& g = a.g;
+  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  const int tx = tile % g.ntx, ty = tile / g.ntx;
+  const int i0 = GH + tx * TX, j0 = GH + ty * TY;
+  __shared__ TileLds<R> L;
+  // stage the tile and three rings: eta = h + b, h, u, v, b, phi
+  for (int t = threadIdx.x; t < BX * BY; t += NT) {
+    const int lx = t % BX, ly = t / BX;
+    const int i = i0 - 3 + lx, j = j0 - 3 + ly;
+    double hd = 0.0, bb = 0.0, u = 0.0, v = 0.0;
+    R ph = R(0);
+    if (i < g.w && j < g.rows) {
+      const int k = j * g.w + i;
+      hd = (double)a.h[k];
+      if (a.h_lo) hd = hd + (double)a.h_lo[k];
+      bb = a.b[k];
+      u = wc_velocity(hd, (double)a.hu[k]);
+      v = wc_velocity(hd, (double)a.hv[k]);
+      ph = a.phi[k];
+    }
+    L.u.box.eta[ly][lx] = (R)(hd + bb);
+    L.u.box.h[ly][lx] = hd;
+    L.u.box.un_x[ly][lx] = (R)u;
+    L.u.box.un_y[ly][lx] = (R)v;
+    L.u.box.b[ly][lx] = bb;
+    L.u.box.phi[ly][lx] = ph;
+  }
+  __syncthreads();
+  // x: cells of tile columns -1..64 of every tile row; stencil along the row.
+  for (int q = threadIdx.x; q < TY * RXN; q += NT) {
+    const int cx = q % RXN, row = q / RXN;
+    const int y = row + 3, x0 = cx;  // stencil columns cx..cx+4 in the box = tile cols cx-3..cx+1
+    reconstruct_r<R>(&L.u.box.eta[y][x0], &L.u.box.h[y][x0], &L.u.box.un_x[y][x0],
+                     &L.u.box.un_y[y][x0], &L.u.box.b[y][x0], &L.u.box.phi[y][x0], 1, a.order,
+                     L.xlo[row][cx], L.xhi[row][cx]);
+  }
+  // y: tile columns 0..63 of tile rows -1..4; stencil along the column.
+  for (int q = threadIdx.x; q < RYN * TX; q += NT) {
+    const int col = q % TX, ry = q / TX;
+    const int x = col + 3, y0 = ry;
+    reconstruct_r<R>(&L.u.box.eta[y0][x], &L.u.box.h[y0][x], &L.u.box.un_y[y0][x],
+                     &L.u.box.un_x[y0][x], &L.u.box.b[y0][x], &L.u.box.phi[y0][x], BX, a.order,
+                     L.ylo[ry][col], L.yhi[ry][col]);
+  }
+  __syncthreads();
+  // faces: x-face f of a row sits between tile columns f-1 and f.
+  for (int q = threadIdx.x; q < TY * (TX + 1); q += NT) {
+    const int f = q % (TX + 1), row = q / (TX + 1);
+    L.u.faces.fx[row][f] = face_r<R>(L.xhi[row][f], L.xlo[row][f + 1]);
+  }
+  for (int q = threadIdx.x; q < (TY + 1) * TX; q += NT) {
+    const int col = q % TX, f = q / TX;
+    L.u.faces.fy[f][col] = face_r<R>(L.yhi[f][col], L.ylo[f + 1][col]);
+  }
+  __syncthreads();
+  const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+  const int i = i0 + lx, j = j0 + ly;
+  i_out = i;
+  j_out = j;
+  valid = i < g.nx + GH && j < g.ny + GH;
+  if (!valid) return;
+  const Face& xl = L.u.faces.fx[ly][lx];
+  const Face& xh = L.u.faces.fx[ly][lx + 1];
+  const Face& yl = L.u.faces.fy[ly][lx];
+  const Face& yh = L.u.faces.fy[ly + 1][lx];
+  const SideT<R>& xs_lo = L.xlo[ly][lx + 1];
+  const SideT<R>& xs_hi = L.xhi[ly][lx + 1];
+  const SideT<R>& ys_lo = L.ylo[ly + 1][lx];
+  const SideT<R>& ys_hi = L.yhi[ly + 1][lx];
+  const double xs = -WC_G * 0.5 * (xs_lo.h + xs_hi.h) * (xs_hi.z - xs_lo.z) / g.dx;
+  const double ys = -WC_G * 0.5 * (ys_lo.h + ys_hi.h) * (ys_hi.z - ys_lo.z) / g.dy;
+  rh = -(xh.f0 - xl.f0) / g.dx - (yh.f0 - yl.f0) / g.dy;
+  rhu = -((xh.f1 + xh.cl) - (xl.f1 + xl.cr)) / g.dx + xs - (yh.f2 - yl.f2) / g.dy;
+  rhv = -((yh.f1 + yh.cl) - (yl.f1 + yl.cr)) / g.dy + ys - (xh.f2 - xl.f2) / g.dx;
+  if (a.has_drive) {
+    const int ii = i - GH;
+    double src = 0.0;
+    if (ii >= a.src_i0 && ii < a.src_i1)
+      src = a.src_col[ii] * a.prm[a.ramp_prm] * a.prm[a.sin_prm + (j - GH)];
+    rh = rh + src;
+  }
+}
+
+// FP64: the tile rates with the stage epilogue of k_flux.
+__global__ void __launch_bounds__(NT) k_flux_tile64(FluxTileArgs<double> f, StepArgs a,
+                                                    int mode) {
+  double rh, rhu, rhv;
+  int i, j;
+  bool valid;
+  flux_tile<double>(f, mode, rh, rhu, rhv, i, j, valid);
+  if (!valid) return;
+  stage_out(a, mode, i, j, j * a.g.w + i, rh, rhu, rhv);
+}
+
+// FP32 chosen form: the rates for the stage update that follows; the mass rate stays FP64 so
+// the volume is conserved to FP64 rounding, the momentum rates are rounded to FP32.
+__global__ void __launch_bounds__(NT) k_flux_tile32(FluxTileArgs<float> f, double* lh, float* lhu,
+                                                    float* lhv) {
+  double rh, rhu, rhv;
+  int i, j;
+  bool valid;
+  flux_tile<float>(f, MODE_RATES, rh, rhu, rhv, i, j, valid);
+  if (!valid) return;
+  const int k = j * f.g.w + i;
+  lh[k] = rh;
+  lhu[k] = (float)rhu;
+  lhv[k] = (float)rhv;
+}
+
+}  // namespace wc

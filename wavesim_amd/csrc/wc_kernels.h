@@ -6,7 +6,8 @@
 
 namespace wc {
 
-enum { MODE_S1 = 0, MODE_FINAL = 1, MODE_RATES = 2 };
+enum { MODE_S1 = 0, MODE_FINAL = 1, MODE_RATES = 2, MODE_EULER = 3 };
+enum { REDUCE_INIT = 0, REDUCE_ALPHA = 1, REDUCE_BETA = 2 };
 
 struct Breaking {
   double er_lo, er_hi, sl_lo, sl_hi;
@@ -27,7 +28,6 @@ struct CgScalars {
   double rz, target, alpha, beta;
   unsigned long long it;
   int converged, done;
-  unsigned int ticket[4];
   unsigned long long solves, iterations, unconverged;
 };
 
@@ -54,6 +54,21 @@ struct StepArgs {
   double manning;
 };
 
+// Inputs of the tiled flux kernels (wc_flux.hip); R is the storage type of the stage state.
+template <typename R>
+struct FluxTileArgs {
+  Geo g;
+  int order;
+  const R *h, *h_lo, *hu, *hv;  // h_lo: low part of the FP32 depth pair, null in FP64
+  const double* b;
+  const R* phi;
+  int has_drive;
+  int src_i0, src_i1;
+  const double* src_col;
+  const double* prm;
+  int ramp_prm, sin_prm;
+};
+
 struct SgnArgs {
   Geo g;
   const double *h, *hu, *hv, *phi;
@@ -66,6 +81,51 @@ struct SgnArgs {
   double tolerance;
   unsigned long long max_iterations;
 };
+
+// The FP32 chosen form (wc_fp32.hip).
+struct Sgn32 {
+  Geo g;
+  const float *h, *hu, *hv, *phi;  // stage state (high parts) and the switch
+  const double* lh;                // mass rate (FP64: the faces' fluxes cancel exactly)
+  const float *lhu, *lhv;          // momentum rates
+  float *c, *bx, *by, *wx, *wy, *rx, *ry, *px, *py, *ax, *ay, *zx, *zy;
+  double* part;
+  CgScalars* sc;
+  int nonlinear;
+  double tolerance;
+  unsigned long long max_iterations;
+};
+
+struct Stage32 {
+  Geo g;
+  double dt;
+  const double* b;
+  float *sh, *shu, *shv, *sl_h, *sl_hu, *sl_hv;  // the state as FP32 pairs (hi, lo)
+  float *s1h, *s1hu, *s1hv;                      // the stage-1 state
+  double* l1h;                                   // the stage-1 rates (mass in FP64)
+  float *l1hu, *l1hv;
+  const double* lh;                              // this stage's shallow-water rates
+  const float *lhu, *lhv;
+  const float *wx, *wy;                          // the SGN solution
+  int dispersive;
+  StepArgs damp;                                 // sponges and friction (dt, tables, manning)
+};
+
+__global__ void k_split32(long long n, const double* a, float* hi, float* lo);
+__global__ void k_join32(long long n, const float* hi, const float* lo, double* a);
+__global__ void k_fill32(Geo g, float* h, float* hl, float* hu, float* hv);
+__global__ void k_switch32(Geo g, const float* h, const float* hl, const double* b, Breaking br,
+                           const uint8_t* held, float* phi);
+__global__ void k_sgn_setup32(Sgn32 a);
+__global__ void k_cg_init32(Sgn32 a);
+__global__ void k_cg_matvec32(Sgn32 a, int first, const float* psx, const float* psy,
+                              float* pdx, float* pdy);
+__global__ void k_cg_update32(Sgn32 a, const float* pcx, const float* pcy);
+__global__ void k_cg_reduce32(Sgn32 a, int kind);
+__global__ void k_stage32(Stage32 a, int mode);
+__global__ void k_max_rate32(Geo g, const float* h, const float* hl, const float* hu,
+                             const float* hul, const float* hv, const float* hvl,
+                             unsigned long long* out);
 
 __global__ void k_fill_state(Geo g, double* h, double* hu, double* hv, double* br, double* ah,
                              double* ahu, double* ahv);
@@ -84,6 +144,8 @@ __global__ void k_cg_matvec2(SgnArgs a, int first, const double* psx, const doub
                              double* pdx, double* pdy);
 __global__ void k_cg_update2(SgnArgs a, const double* pcx, const double* pcy);
 __global__ void k_sgn_finish(StepArgs a, const double* wx, const double* wy, int mode);
+__global__ void k_flux_tile64(FluxTileArgs<double> f, StepArgs a, int mode);
+__global__ void k_flux_tile32(FluxTileArgs<float> f, double* lh, float* lhu, float* lhv);
 __global__ void k_max_rate(Geo g, const double* h, const double* hu, const double* hv,
                            unsigned long long* out);
 __global__ void k_limit_scan(Geo g, const double* h, const double* hu, const double* hv,
@@ -91,7 +153,9 @@ __global__ void k_limit_scan(Geo g, const double* h, const double* hu, const dou
 __global__ void k_limit_first(Geo g, const double* h, const double* hu, const double* hv,
                               const unsigned long long* maxbits, unsigned int* first);
 __global__ void k_state_sums(Geo g, const double* h, const double* hu, const double* hv,
-                             double* part, unsigned int* ticket, double* out,
-                             unsigned long long* maxmom, unsigned long long* minh_key);
+                             double* part, unsigned long long* maxmom,
+                             unsigned long long* minh_key);
+__global__ void k_sum_tiles(const double* part, int np, double* out);
+__global__ void k_cg_reduce(SgnArgs a, int kind);
 
 }  // namespace wc

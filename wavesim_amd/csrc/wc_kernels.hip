@@ -5,6 +5,7 @@
 // arithmetic of each kernel is that of the reference function it cites, in the same order.
 
 #include "wc_kernels.h"
+#include "wc_stage.h"
 
 namespace wc {
 
@@ -100,81 +101,6 @@ __global__ void __launch_bounds__(NT) k_breaking(Geo g, const double* h, const d
   const double off = wc_max(wc_smoothstep(br.er_lo, br.er_hi, ratio),
                             wc_smoothstep(br.sl_lo, br.sl_hi, wc_hypot(ex, ey)));
   out[t] = 1.0 - (1.0 - off);
-}
-
-// ---- the stage update ----------------------------------------------------------------------------
-// clean_dry (solver.rs:577-591)
-__device__ __forceinline__ void clean_dry(double& h, double& hu, double& hv) {
-  if (h < WC_H_DRY) {
-    h = wc_max(h, 0.0);
-    hu = 0.0;
-    hv = 0.0;
-  }
-}
-
-// damp (solver.rs:535-574): the sponges in order, then Manning friction, then clean_dry.
-__device__ __forceinline__ void damp(const StepArgs& a, int i, int j, double b, double& h,
-                                     double& hu, double& hv) {
-  const int ii = i - GH, jj = j - GH;
-  for (int m = 0; m < a.n_sponges; ++m) {
-    const SpongeDev& sp = a.sponges[m];
-    long long d = 0x7fffffffffffffffLL;
-    if (sp.west) d = d < ii ? d : ii;
-    if (sp.east) d = d < (a.g.nx - 1 - ii) ? d : (a.g.nx - 1 - ii);
-    if (sp.south) d = d < jj ? d : jj;
-    if (sp.north) d = d < (a.g.ny - 1 - jj) ? d : (a.g.ny - 1 - jj);
-    if (d < sp.len) {
-      const double rate = a.sponge_rate[sp.offset + d];
-      if (rate > 0.0) {
-        const double f = a.prm[a.sponge_prm + sp.offset + d];
-        const double ht = wc_max(sp.level - b, 0.0);
-        h = f * h + (1.0 - f) * ht;
-        hu = f * hu + (1.0 - f) * ht * 0.0;
-        hv = f * hv + (1.0 - f) * ht * 0.0;
-      }
-    }
-  }
-  if (a.manning > 0.0 && h > WC_H_FRICTION_MIN) {
-    const double speed = wc_hypot(hu, hv) / h;
-    const double kk = WC_G * a.manning * a.manning / wc_pow43(h);
-    const double fac = 1.0 / (1.0 + a.dt * kk * speed);
-    hu *= fac;
-    hv *= fac;
-  }
-  clean_dry(h, hu, hv);
-}
-
-// Where a stage's rates go (solver.rs:481-507, 594-616):
-//   MODE_S1:    s1 = clean_dry(s + dt L)               (stage 1, and the Euler step)
-//   MODE_FINAL: s = damp(clean_dry((s + clean_dry(s1 + dt L)) / 2))  (stage 2)
-//   MODE_RATES: store L (the dispersive correction still has to be added)
-__device__ __forceinline__ void stage_out(const StepArgs& a, int mode, int i, int j, int k,
-                                          double rh, double rhu, double rhv) {
-  if (mode == MODE_RATES) {
-    a.lh[k] = rh;
-    a.lhu[k] = rhu;
-    a.lhv[k] = rhv;
-    return;
-  }
-  double h = a.in_h[k], hu = a.in_hu[k], hv = a.in_hv[k];
-  h += a.dt * rh;
-  hu += a.dt * rhu;
-  hv += a.dt * rhv;
-  clean_dry(h, hu, hv);
-  if (mode == MODE_S1) {
-    a.s1_h[k] = h;
-    a.s1_hu[k] = hu;
-    a.s1_hv[k] = hv;
-    return;
-  }
-  double sh = 0.5 * (a.s_h[k] + h);
-  double shu = 0.5 * (a.s_hu[k] + hu);
-  double shv = 0.5 * (a.s_hv[k] + hv);
-  clean_dry(sh, shu, shv);
-  damp(a, i, j, a.b[k], sh, shu, shv);
-  a.s_h[k] = sh;
-  a.s_hu[k] = shu;
-  a.s_hv[k] = shv;
 }
 
 // ---- the finite-volume rates (solver.rs:625-718) ----------------------------------------------
@@ -405,19 +331,7 @@ __global__ void __launch_bounds__(NT) k_cg_init(SgnArgs a) {
     v[1] = rx * zx + ry * zy;
     v[2] = rx * rx + ry * ry;
   }
-  if (publish_tile<3>(v, a.part, gridDim.x, tile, &a.sc->ticket[0])) {
-    double s[3];
-    final_sums<3>(a.part, gridDim.x, s);
-    if (threadIdx.x == 0) {
-      CgScalars* sc = a.sc;
-      const double bnorm = sqrt(s[0]);
-      sc->target = a.tolerance * bnorm + 1e-10;
-      sc->rz = s[1];
-      sc->it = 0;
-      sc->converged = sqrt(s[2]) <= sc->target ? 1 : 0;
-      sc->done = (sc->converged || a.max_iterations == 0) ? 1 : 0;
-    }
-  }
+  store_tile<3>(v, a.part, gridDim.x, tile);
 }
 
 // One PCG iteration, part 1 (dispersion.rs:516-518): Ap, p.Ap and alpha = rz / p.Ap.
@@ -436,11 +350,7 @@ __global__ void __launch_bounds__(NT) k_cg_matvec(SgnArgs a) {
     a.ay[k] = o.ay;
     v[0] = o.px * o.ax + o.py * o.ay;
   }
-  if (publish_tile<1>(v, a.part, gridDim.x, tile, &a.sc->ticket[1])) {
-    double s[1];
-    final_sums<1>(a.part, gridDim.x, s);
-    if (threadIdx.x == 0) a.sc->alpha = a.sc->rz / s[0];
-  }
+  store_tile<1>(v, a.part, gridDim.x, tile);
 }
 
 // Part 2 (dispersion.rs:519-526): w += alpha p, r -= alpha Ap, z = r / diag, the convergence
@@ -469,17 +379,7 @@ __global__ void __launch_bounds__(NT) k_cg_update(SgnArgs a) {
     v[0] = rx * rx + ry * ry;
     v[1] = rx * zx + ry * zy;
   }
-  if (publish_tile<2>(v, a.part, gridDim.x, tile, &sc->ticket[2])) {
-    double s[2];
-    final_sums<2>(a.part, gridDim.x, s);
-    if (threadIdx.x == 0) {
-      sc->it += 1;
-      sc->converged = sqrt(s[0]) <= sc->target ? 1 : 0;
-      sc->beta = s[1] / sc->rz;
-      sc->rz = s[1];
-      sc->done = (sc->converged || sc->it >= a.max_iterations) ? 1 : 0;
-    }
-  }
+  store_tile<2>(v, a.part, gridDim.x, tile);
 }
 
 // Part 3 (dispersion.rs:527-539): p = z + beta p. Skipped once the solve is done: p is not
@@ -580,11 +480,7 @@ __global__ void __launch_bounds__(NT) k_cg_matvec2(SgnArgs a, int first, const d
     a.ay[k] = ay;
     v[0] = px * ax + py * ay;
   }
-  if (publish_tile<1>(v, a.part, gridDim.x, tile, &sc->ticket[1])) {
-    double s[1];
-    final_sums<1>(a.part, gridDim.x, s);
-    if (threadIdx.x == 0) sc->alpha = sc->rz / s[0];
-  }
+  store_tile<1>(v, a.part, gridDim.x, tile);
 }
 
 // Part 2 (dispersion.rs:519-526): w += alpha p, r -= alpha Ap, z = r / diag (stored for the
@@ -617,16 +513,44 @@ __global__ void __launch_bounds__(NT) k_cg_update2(SgnArgs a, const double* pcx,
     v[0] = rx * rx + ry * ry;
     v[1] = rx * zx + ry * zy;
   }
-  if (publish_tile<2>(v, a.part, gridDim.x, tile, &sc->ticket[2])) {
-    double s[2];
-    final_sums<2>(a.part, gridDim.x, s);
+  store_tile<2>(v, a.part, gridDim.x, tile);
+}
+
+// The second stage of the PCG reductions, one block, and the scalar recurrences they feed:
+//   REDUCE_INIT    b.b, r.z, r.r -> target = tol |b| + 1e-10, rz, converged (dispersion.rs:510-514)
+//   REDUCE_ALPHA   p.Ap -> alpha = rz / p.Ap (dispersion.rs:517-518)
+//   REDUCE_BETA    r.r, r.z -> it + 1, converged, beta = rz' / rz (dispersion.rs:522-526)
+__global__ void __launch_bounds__(NT) k_cg_reduce(SgnArgs a, int kind) {
+  CgScalars* sc = a.sc;
+  const int np = a.g.ntx * a.g.nty;
+  if (kind == REDUCE_INIT) {
+    double s[3];
+    final_sums<3>(a.part, np, s);
     if (threadIdx.x == 0) {
-      sc->it += 1;
-      sc->converged = sqrt(s[0]) <= sc->target ? 1 : 0;
-      sc->beta = s[1] / sc->rz;
+      const double bnorm = sqrt(s[0]);
+      sc->target = a.tolerance * bnorm + 1e-10;
       sc->rz = s[1];
-      sc->done = (sc->converged || sc->it >= a.max_iterations) ? 1 : 0;
+      sc->it = 0;
+      sc->converged = sqrt(s[2]) <= sc->target ? 1 : 0;
+      sc->done = (sc->converged || a.max_iterations == 0) ? 1 : 0;
     }
+    return;
+  }
+  if (sc->done) return;
+  if (kind == REDUCE_ALPHA) {
+    double s[1];
+    final_sums<1>(a.part, np, s);
+    if (threadIdx.x == 0) sc->alpha = sc->rz / s[0];
+    return;
+  }
+  double s[2];
+  final_sums<2>(a.part, np, s);
+  if (threadIdx.x == 0) {
+    sc->it += 1;
+    sc->converged = sqrt(s[0]) <= sc->target ? 1 : 0;
+    sc->beta = s[1] / sc->rz;
+    sc->rz = s[1];
+    sc->done = (sc->converged || sc->it >= a.max_iterations) ? 1 : 0;
   }
 }
 
@@ -725,7 +649,6 @@ __global__ void __launch_bounds__(NT) k_limit_first(Geo g, const double* h, cons
 // and State::min_depth (NaN cells drop out of both, as in the reference's folds).
 __global__ void __launch_bounds__(NT) k_state_sums(Geo g, const double* h, const double* hu,
                                                    const double* hv, double* part,
-                                                   unsigned int* ticket, double* out,
                                                    unsigned long long* maxmom,
                                                    unsigned long long* minh_key) {
   const int tile = blockIdx.x;
@@ -746,11 +669,14 @@ __global__ void __launch_bounds__(NT) k_state_sums(Geo g, const double* h, const
       atomicMin(minh_key, b);
     }
   }
-  if (publish_tile<1>(v, part, gridDim.x, tile, ticket)) {
-    double s[1];
-    final_sums<1>(part, gridDim.x, s);
-    if (threadIdx.x == 0) *out = s[0];
-  }
+  store_tile<1>(v, part, gridDim.x, tile);
+}
+
+// The second stage of the canonical sum of k_state_sums.
+__global__ void __launch_bounds__(NT) k_sum_tiles(const double* part, int np, double* out) {
+  double s[1];
+  final_sums<1>(part, np, s);
+  if (threadIdx.x == 0) *out = s[0];
 }
 
 }  // namespace wc

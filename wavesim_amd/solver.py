@@ -40,6 +40,7 @@ LIB_PATH = os.path.join(HERE, "libwavecore_hip.so")
 
 WC_OK, WC_EINVAL, WC_EDEVICE, WC_ENOMEM = 0, 1, 2, 3
 PRECISION_FP64 = 0
+PRECISION_FP32 = 1
 
 _D = ctypes.c_double
 _P = ctypes.POINTER(ctypes.c_double)
