@@ -114,7 +114,8 @@ int wc_solver_with_shallow_cells(wc_solver* s, const uint8_t* cells, uint64_t n)
 /* The bed as the solver holds it (with_periodic_y wraps its ghost rows; solver.rs:416). */
 int wc_solver_bed(const wc_solver* s, double* out);
 /* Implementation choices that change speed, never results (no counterpart in the reference):
- *   "pcg_kernels" 1 = three kernels per PCG iteration, 2 = two fused kernels (default);
+ *   "pcg_kernels" 1 = three kernels and two reductions per PCG iteration; 2 = two kernels and
+ *                 two reductions; 3 = two kernels, reductions folded into them (default);
  *   "pcg_batch"   PCG iterations queued between looks at the convergence flag (default 4);
  *   "flux_kernel" 1 = one thread per cell, as the reference's loop; 2 = LDS tiles, each cell
  *                 reconstructed and each face fluxed once (default). */

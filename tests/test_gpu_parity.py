@@ -101,7 +101,7 @@ def test_each_order_over_a_rough_bed_with_dry_land(order, flux):
              150, f"rough-dry {order.name} flux={flux}", options={"flux_kernel": flux})
 
 
-@pytest.mark.parametrize("pcg", [1, 2])
+@pytest.mark.parametrize("pcg", [1, 2, 3])
 @pytest.mark.parametrize("periodic", [False, True])
 def test_sgn_fifth_walls_and_periodic_seam(periodic, pcg):
     grid = Grid(40, 30, 1.0, 1.0)
@@ -120,7 +120,7 @@ def test_sgn_fifth_walls_and_periodic_seam(periodic, pcg):
              options={"pcg_kernels": pcg})
 
 
-@pytest.mark.parametrize("pcg", [1, 2])
+@pytest.mark.parametrize("pcg", [1, 2, 3])
 def test_sgn_rough_bed_tight_tolerance_with_maker_sponges_and_friction(pcg):
     w = configs.c1_rough_bed(nx=160, ny=64, steps=200)
 
@@ -133,7 +133,7 @@ def test_sgn_rough_bed_tight_tolerance_with_maker_sponges_and_friction(pcg):
     s = w.still_water()
     s.time = 20.0
     lockstep(w.grid, w.bed, configure, s, w.steps, f"sgn rough bed pcg={pcg}",
-             options={"pcg_kernels": pcg, "pcg_batch": 3, "flux_kernel": pcg})
+             options={"pcg_kernels": pcg, "pcg_batch": 3, "flux_kernel": 1 + pcg % 2})
 
 
 def test_sgn_second_order_linear_dispersion_and_held_cells():

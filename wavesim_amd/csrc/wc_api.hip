@@ -132,7 +132,7 @@ struct wc_solver {
   // the FP32 chosen form
   float *f_sh = nullptr, *f_shu = nullptr, *f_shv = nullptr, *f_slh = nullptr, *f_slhu = nullptr,
         *f_slhv = nullptr;
-  float *f_s1h = nullptr, *f_s1hu = nullptr, *f_s1hv = nullptr;
+  float *f_s1h = nullptr, *f_s1hl = nullptr, *f_s1hu = nullptr, *f_s1hv = nullptr;
   double *f_l1h = nullptr, *f_lh = nullptr;  // mass rates stay FP64
   float *f_l1hu = nullptr, *f_l1hv = nullptr;
   float *f_lhu = nullptr, *f_lhv = nullptr, *f_phi = nullptr;
@@ -141,7 +141,8 @@ struct wc_solver {
         *f_qy = nullptr, *f_zx = nullptr, *f_zy = nullptr, *f_ax = nullptr, *f_ay = nullptr;
 
   // implementation options (wc_solver_set_option)
-  int pcg_kernels = 2;
+  int pcg_kernels = 3;
+  double* lanes = nullptr;  // 6 x 256 lane sums of the 256-lane PCG
   int flux_kernel = 2;
   unsigned long long pcg_batch = 4;
   uint8_t* held = nullptr;
@@ -468,11 +469,70 @@ SgnArgs sgn_args(wc_solver* s, int stage) {
   return a;
 }
 
+// The 256-lane PCG loop (wc_pcg.hip): two kernels per iteration, batches of iterations
+// between looks at the device-side flag, a one-block check closing every batch.
+template <typename V>
+int pcg3(wc_solver* s, int stage, CgV<V> a, V* pA_x, V* pA_y, V* pB_x, V* pB_y) {
+  a.acc_init = s->lanes;
+  a.acc_pap = s->lanes + 3 * 256;
+  a.acc_update = s->lanes + 4 * 256;
+  a.sc = s->sc;
+  a.tolerance = s->disp.tolerance;
+  a.max_iterations = s->disp.max_iterations;
+  const dim3 grid(256), block(1024);
+  TRY(launch(s, F_CG_INIT, k_cg3_init<V>, grid, block, a));
+  const unsigned long long maxit = s->disp.max_iterations;
+  unsigned long long n = 1;
+  unsigned long long batch = std::max<unsigned long long>(s->prev_it[stage], 1);
+  while (true) {
+    const unsigned long long stop = std::min(n + batch, maxit + 1);
+    for (; n < stop; ++n) {
+      // p of iteration 1 is what init stored in A; then p_n goes to B (n even) or A (n odd)
+      V* cur_x = (n % 2 == 1) ? pA_x : pB_x;
+      V* cur_y = (n % 2 == 1) ? pA_y : pB_y;
+      const V* old_x = (n == 1) ? pA_x : ((n % 2 == 1) ? pB_x : pA_x);
+      const V* old_y = (n == 1) ? pA_y : ((n % 2 == 1) ? pB_y : pA_y);
+      TRY(launch(s, F_CG_MATVEC, k_cg3_matvec<V>, grid, block, a, n, old_x, old_y, cur_x,
+                 cur_y));
+      TRY(launch(s, F_CG_UPDATE, k_cg3_update<V>, grid, block, a, n, (const V*)cur_x,
+                 (const V*)cur_y));
+    }
+    TRY(launch(s, F_CG_REDUCE, k_cg3_check<V>, dim3(1), dim3(256), a, n));
+    HIP_OK(hipMemcpyAsync(s->sc_host, s->sc, sizeof(CgScalars), hipMemcpyDeviceToHost,
+                          s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    if (s->sc_host->done || n > maxit) break;
+    batch = s->pcg_batch;
+  }
+  s->prev_it[stage] = s->sc_host->it;
+  TRY(launch(s, F_CG_STATS, k_cg_stats, dim3(1), dim3(64), s->sc));
+  return WC_OK;
+}
+
 // The PCG loop (dispersion.rs:493-545). Iterations run in batches; every kernel of an
 // iteration returns at once when the device-side flag says the solve is over, so the host
 // only looks at the flag between batches. The first batch is as long as the last solve.
 int pcg(wc_solver* s, int stage) {
   const SgnArgs a = sgn_args(s, stage);
+  if (s->pcg_kernels == 3) {
+    CgV<double> v{};
+    v.g = s->geo;
+    v.h = a.h;
+    v.c = s->c;
+    v.bx = s->bx;
+    v.by = s->by;
+    v.wx = s->wx;
+    v.wy = s->wy;
+    v.rx = s->rx;
+    v.ry = s->ry;
+    v.px = s->px;
+    v.py = s->py;
+    v.zx = s->zx;
+    v.zy = s->zy;
+    v.ax = s->ax;
+    v.ay = s->ay;
+    return pcg3<double>(s, stage, v, s->px, s->py, s->qx, s->qy);
+  }
   TRY(launch(s, F_CG_INIT, k_cg_init, tiles(s), dim3(NT), a));
   TRY(launch(s, F_CG_REDUCE, k_cg_reduce, dim3(1), dim3(NT), a, (int)REDUCE_INIT));
   const unsigned long long maxit = s->disp.max_iterations;
@@ -626,7 +686,7 @@ int alloc_fp32(wc_solver* s) {
   TRY(dalloc(s, &s->f_l1h, n));
   TRY(dalloc(s, &s->f_lh, n));
   for (float** p : {&s->f_sh, &s->f_shu, &s->f_shv, &s->f_slh, &s->f_slhu, &s->f_slhv, &s->f_s1h,
-                    &s->f_s1hu, &s->f_s1hv, &s->f_l1hu, &s->f_l1hv,
+                    &s->f_s1hl, &s->f_s1hu, &s->f_s1hv, &s->f_l1hu, &s->f_l1hv,
                     &s->f_lhu, &s->f_lhv, &s->f_phi, &s->f_c, &s->f_bx, &s->f_by, &s->f_wx,
                     &s->f_wy, &s->f_rx, &s->f_ry, &s->f_px, &s->f_py, &s->f_qx, &s->f_qy,
                     &s->f_zx, &s->f_zy, &s->f_ax, &s->f_ay})
@@ -694,6 +754,25 @@ Sgn32 sgn32_args(wc_solver* s, int stage) {
 
 int pcg32(wc_solver* s, int stage) {
   const Sgn32 a = sgn32_args(s, stage);
+  if (s->pcg_kernels == 3) {
+    CgV<float> v{};
+    v.g = s->geo;
+    v.h = a.h;
+    v.c = s->f_c;
+    v.bx = s->f_bx;
+    v.by = s->f_by;
+    v.wx = s->f_wx;
+    v.wy = s->f_wy;
+    v.rx = s->f_rx;
+    v.ry = s->f_ry;
+    v.px = s->f_px;
+    v.py = s->f_py;
+    v.zx = s->f_zx;
+    v.zy = s->f_zy;
+    v.ax = s->f_ax;
+    v.ay = s->f_ay;
+    return pcg3<float>(s, stage, v, s->f_px, s->f_py, s->f_qx, s->f_qy);
+  }
   TRY(launch(s, F_CG_INIT, k_cg_init32, tiles(s), dim3(NT), a));
   TRY(launch(s, F_CG_REDUCE, k_cg_reduce32, dim3(1), dim3(NT), a, (int)REDUCE_INIT));
   const unsigned long long maxit = s->disp.max_iterations;
@@ -737,7 +816,7 @@ int pcg32(wc_solver* s, int stage) {
 int stage32(wc_solver* s, double dt, int stage, int mode) {
   const Geo& g = s->geo;
   float* in_h = stage == 0 ? s->f_sh : s->f_s1h;
-  const float* in_lo = stage == 0 ? s->f_slh : nullptr;
+  const float* in_lo = stage == 0 ? s->f_slh : s->f_s1hl;
   float* in_hu = stage == 0 ? s->f_shu : s->f_s1hu;
   float* in_hv = stage == 0 ? s->f_shv : s->f_s1hv;
   if (s->order == WC_ORDER_FIFTH || s->dispersive)
@@ -776,6 +855,7 @@ int stage32(wc_solver* s, double dt, int stage, int mode) {
   a.sl_hu = s->f_slhu;
   a.sl_hv = s->f_slhv;
   a.s1h = s->f_s1h;
+  a.s1hl = s->f_s1hl;
   a.s1hu = s->f_s1hu;
   a.s1hv = s->f_s1hv;
   a.l1h = s->f_l1h;
@@ -803,7 +883,7 @@ int do_step32(wc_solver* s, double dt) {
   } else {
     TRY(stage32(s, dt, 0, MODE_S1));
     TRY(launch(s, F_FILL, k_fill32, over(6LL * g.w + 6LL * g.ny), dim3(NT), g, s->f_s1h,
-               (float*)nullptr, s->f_s1hu, s->f_s1hv));
+               s->f_s1hl, s->f_s1hu, s->f_s1hv));
     TRY(stage32(s, dt, 1, MODE_FINAL));
   }
   s->time = t + dt;
@@ -885,6 +965,7 @@ int wc_solver_new(uint64_t nx, uint64_t ny, double dx, double dy, const double* 
   const size_t ntiles = (size_t)g.ntx * g.nty;
   if ((rc = dalloc(s, &s->part, 4 * ntiles)) != WC_OK) return cleanup(rc);
   if ((rc = dalloc(s, &s->sc, 1)) != WC_OK) return cleanup(rc);
+  if ((rc = dalloc(s, &s->lanes, 6 * 256)) != WC_OK) return cleanup(rc);
   if ((rc = dalloc(s, &s->red64, 4)) != WC_OK) return cleanup(rc);
   if ((rc = dalloc(s, &s->red32, 4)) != WC_OK) return cleanup(rc);
   if ((rc = dalloc(s, &s->redf, 4)) != WC_OK) return cleanup(rc);
@@ -1052,7 +1133,7 @@ int wc_solver_bed(const wc_solver* s, double* out) {
 int wc_solver_set_option(wc_solver* s, const char* key, int64_t value) {
   if (!key) return fail(WC_EINVAL, "null option name");
   if (strcmp(key, "pcg_kernels") == 0) {
-    if (value != 1 && value != 2) return fail(WC_EINVAL, "pcg_kernels must be 1 or 2");
+    if (value < 1 || value > 3) return fail(WC_EINVAL, "pcg_kernels must be 1, 2 or 3");
     s->pcg_kernels = (int)value;
     return WC_OK;
   }

@@ -363,7 +363,8 @@ __global__ void __launch_bounds__(NT) k_stage32(Stage32 a, int mode) {
   if (i >= g.nx + GH || j >= g.ny + GH) return;
   const int k = j * g.w + i;
   // the stage input, as the rates saw it
-  const double ih = mode == MODE_FINAL ? (double)a.s1h[k] : (double)a.sh[k] + (double)a.sl_h[k];
+  const double ih = mode == MODE_FINAL ? (double)a.s1h[k] + (double)a.s1hl[k]
+                                       : (double)a.sh[k] + (double)a.sl_h[k];
   const double ihu = mode == MODE_FINAL ? (double)a.s1hu[k] : (double)a.shu[k];
   const double ihv = mode == MODE_FINAL ? (double)a.s1hv[k] : (double)a.shv[k];
   const double lh = a.lh[k];
@@ -379,7 +380,10 @@ __global__ void __launch_bounds__(NT) k_stage32(Stage32 a, int mode) {
   if (mode == MODE_S1) {
     double h = sh + a.dt * lh, hu = shu + a.dt * lhu, hv = shv + a.dt * lhv;
     clean_dry(h, hu, hv);
-    a.s1h[k] = (float)h;
+    float hi, lo;
+    split(h, hi, lo);
+    a.s1h[k] = hi;
+    a.s1hl[k] = lo;
     a.s1hu[k] = (float)hu;
     a.s1hv[k] = (float)hv;
     a.l1h[k] = lh;
@@ -394,7 +398,8 @@ __global__ void __launch_bounds__(NT) k_stage32(Stage32 a, int mode) {
     hv = shv + a.dt * lhv;
     clean_dry(h, hu, hv);
   } else {
-    const double s1h = (double)a.s1h[k], s1hu = (double)a.s1hu[k], s1hv = (double)a.s1hv[k];
+    const double s1h = (double)a.s1h[k] + (double)a.s1hl[k];
+    const double s1hu = (double)a.s1hu[k], s1hv = (double)a.s1hv[k];
     double h2 = s1h + a.dt * lh, hu2 = s1hu + a.dt * lhu, hv2 = s1hv + a.dt * lhv;
     const bool cleaned = !(sh + a.dt * a.l1h[k] >= WC_H_DRY) || !(h2 >= WC_H_DRY);
     clean_dry(h2, hu2, hv2);

@@ -26,6 +26,7 @@ constexpr int MAX_SPONGES = 16;
 // Scalars of one PCG solve and the DispersionStats counters, in device memory.
 struct CgScalars {
   double rz, target, alpha, beta;
+  double rzs[2];  // r.z of the previous iteration, by iteration parity (wc_pcg.hip)
   unsigned long long it;
   int converged, done;
   unsigned long long solves, iterations, unconverged;
@@ -82,6 +83,28 @@ struct SgnArgs {
   unsigned long long max_iterations;
 };
 
+// The 256-lane PCG (wc_pcg.hip), on FP64 (V = double) or FP32 (V = float) storage.
+template <typename V>
+struct CgV {
+  Geo g;
+  const V *h, *c, *bx, *by;
+  V *wx, *wy, *rx, *ry, *px, *py, *zx, *zy, *ax, *ay;
+  double *acc_init, *acc_pap, *acc_update;  // 3 x 256, 256, 2 x 256 lane sums
+  CgScalars* sc;
+  double tolerance;
+  unsigned long long max_iterations;
+};
+
+template <typename V>
+__global__ void k_cg3_init(CgV<V> a);
+template <typename V>
+__global__ void k_cg3_matvec(CgV<V> a, unsigned long long n, const V* psx, const V* psy, V* pdx,
+                             V* pdy);
+template <typename V>
+__global__ void k_cg3_update(CgV<V> a, unsigned long long n, const V* pcx, const V* pcy);
+template <typename V>
+__global__ void k_cg3_check(CgV<V> a, unsigned long long n);
+
 // The FP32 chosen form (wc_fp32.hip).
 struct Sgn32 {
   Geo g;
@@ -101,7 +124,7 @@ struct Stage32 {
   double dt;
   const double* b;
   float *sh, *shu, *shv, *sl_h, *sl_hu, *sl_hv;  // the state as FP32 pairs (hi, lo)
-  float *s1h, *s1hu, *s1hv;                      // the stage-1 state
+  float *s1h, *s1hl, *s1hu, *s1hv;               // the stage-1 state (depth as a pair)
   double* l1h;                                   // the stage-1 rates (mass in FP64)
   float *l1hu, *l1hv;
   const double* lh;                              // this stage's shallow-water rates
