@@ -42,6 +42,8 @@ BYTES_PER_CELL = {
     "fp32": {
         "cg_matvec": 4 * (2 + 2 + 1 + 1 + 2 + 2),
         "cg_update": 4 * (2 + 2 + 2 + 2 + 1 + 1 + 2 + 2 + 2),
+        # one kernel per iteration: read r, Au, Ap, 1/diag, c, h, p, w; write p, Ap, w, r, Au
+        "cg_iter": 4 * (2 + 2 + 2 + 2 + 1 + 1 + 2 + 2 + 2 + 2 + 2 + 2 + 2),
         "cg_init": 4 * (2 + 1 + 1 + 2 + 2 + 2),  # read w, c, h, b; write r, p
         "sgn_setup": 4 * (3 + 1 + 2 + 3) + 8,  # read h, hu, hv, phi, lhu, lhv, c; lh FP64; write c, b
         "flux": 4 * (2 + 2 + 1 + 2) + 8 + 8,  # read h pair, hu, hv, phi; b FP64; write lhu, lhv; lh FP64
@@ -62,6 +64,8 @@ def parse():
                    help="CPU work per thread count for the baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--precision", default="both", choices=["fp64", "fp32", "both"])
+    p.add_argument("--pcg32", type=int, default=None,
+                   help="PCG kernels of the FP32 form (3: two per iteration, 4: one)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run a device copy of known size (to scale HBM counters) and exit")
     return p.parse_args()
@@ -81,7 +85,7 @@ def calibrate_copy():
                       "launches": 10}))
 
 
-def measure(w, precision, arrays, t0, warmup, steps, profile_steps):
+def measure(w, precision, arrays, t0, warmup, steps, profile_steps, pcg32=None):
     """Time `steps` steps of one precision from the given state; per-kernel times with HIP
     events on the solver's stream."""
     import torch
@@ -92,6 +96,8 @@ def measure(w, precision, arrays, t0, warmup, steps, profile_steps):
     g = w.solver(Solver)
     if precision == "fp32":
         g.with_precision(PRECISION_FP32)
+        if pcg32:
+            g.set_option("pcg_kernels", pcg32)
     h, hu, hv, br = arrays
     s = State(h.copy(), hu.copy(), hv.copy(), br.copy(), t0)
     g.run_steps(s, warmup)
@@ -207,7 +213,8 @@ def main():
     t0 = s.time
     del g
     precisions = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
-    res = {p: measure(w, p, arrays, t0, args.warmup, args.steps, args.profile_steps)
+    res = {p: measure(w, p, arrays, t0, args.warmup, args.steps, args.profile_steps,
+                      args.pcg32)
            for p in precisions}
     acc = accuracy(w, res["fp64"], res["fp32"]) if len(res) == 2 else None
     chosen = "fp64"

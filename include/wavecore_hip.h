@@ -162,6 +162,29 @@ double wc_manning_factor(double h, double speed, double dt, double n);
  * positive; *done_steps reports how many steps ran and *last_dt the last dt. */
 int wc_run_steps(wc_solver* s, uint64_t steps, uint64_t* done_steps, double* last_dt);
 
+/* Relaxation zones (forcing.rs:227-284): the water in the zones is pulled towards targets the
+ * caller supplies at every step (one-way nesting, crates/wavesim/src/lib.rs:1894-1897, and
+ * the lab's wave paddle, crates/wavecore/tests/lab.rs:253-271). A relaxation belongs to the
+ * solver it was made for and acts on that solver's resident state.
+ *   wc_relaxation_new   Relaxation::new (forcing.rs:239-248): every interior cell where one of
+ *                       the zones has a rate, at the largest of their rates, in the reference's
+ *                       row order. Zones are Sponges (rate only; their level is not used).
+ *   wc_relaxation_len / wc_relaxation_cells
+ *                       Relaxation::cells (forcing.rs:252-254): padded (i, j) of each cell, in
+ *                       the order apply takes the targets.
+ *   wc_relaxation_apply Relaxation::apply (forcing.rs:259-283) over dt: targets holds
+ *                       (eta, u, v) per cell, interleaved (3 * len doubles), in host memory;
+ *                       wc_relaxation_apply_device takes them in device memory of the
+ *                       solver's GPU (a coarse run's output, say) and does not copy them. */
+typedef struct wc_relaxation wc_relaxation;
+int wc_relaxation_new(wc_solver* s, const wc_sponge_t* zones, uint64_t n_zones,
+                      wc_relaxation** out);
+void wc_relaxation_free(wc_relaxation* r);
+uint64_t wc_relaxation_len(const wc_relaxation* r);
+int wc_relaxation_cells(const wc_relaxation* r, uint64_t* i, uint64_t* j);
+int wc_relaxation_apply(wc_relaxation* r, double dt, const double* targets);
+int wc_relaxation_apply_device(wc_relaxation* r, double dt, const double* targets);
+
 /* Measurement hooks for benchmarks: average device time in milliseconds of each kernel family
  * over the steps since the last reset, measured with HIP events on the solver's stream, and the
  * kernel launch counts. names receives a ';'-separated list. Timing adds events around every

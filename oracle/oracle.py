@@ -90,6 +90,12 @@ def lib():
             "wco_state_max_abs_momentum": (_I32, [_VP, _P]),
             "wco_state_min_depth": (_I32, [_VP, _P]),
             "wco_fill_boundaries": (_I32, [_VP]),
+            "wco_relaxation_new": (_I32, [_VP, ctypes.POINTER(CSponge), _U64,
+                                          ctypes.POINTER(_VP)]),
+            "wco_relaxation_free": (None, [_VP]),
+            "wco_relaxation_len": (_U64, [_VP]),
+            "wco_relaxation_cells": (_I32, [_VP, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
+            "wco_relaxation_apply": (_I32, [_VP, _D, _P]),
             "wco_linear_wave": (_I32, [_D, _D, _I32, ctypes.POINTER(CLinearWave)]),
             "wco_manning_factor": (_D, [_D, _D, _D, _D]),
             "wco_face_values": (None, [_P, _D, _P, _P]),
@@ -275,6 +281,52 @@ class OracleSolver:
             t = t + dt.value
         self._download_into(s)
         s.time = t
+
+
+def relaxation_targets(target, n: int) -> np.ndarray:
+    """The (eta, u, v) of each of n zone cells, interleaved, from a callable m -> (eta, u, v)
+    (the reference's closure) or an array of shape (n, 3)."""
+    if callable(target):
+        out = np.empty((n, 3))
+        for m in range(n):
+            out[m] = target(m)
+    else:
+        out = np.array(target, dtype=np.float64).reshape(n, 3)
+    return np.ascontiguousarray(out.reshape(-1))
+
+
+class OracleRelaxation:
+    """Relaxation (forcing.rs:227-284) on the oracle: ``new``, ``cells``, ``apply``."""
+
+    def __init__(self, solver: OracleSolver, zones):
+        self.solver = solver
+        arr = (CSponge * max(len(zones), 1))(*[c_sponge(z) for z in zones])
+        h = _VP()
+        _check(lib().wco_relaxation_new(solver._h, arr, len(zones), ctypes.byref(h)))
+        self._r = h
+
+    def __del__(self):
+        if getattr(self, "_r", None):
+            lib().wco_relaxation_free(self._r)
+            self._r = None
+
+    def __len__(self) -> int:
+        return int(lib().wco_relaxation_len(self._r))
+
+    def cells(self) -> list[tuple[int, int]]:
+        n = len(self)
+        i = (_U64 * max(n, 1))()
+        j = (_U64 * max(n, 1))()
+        _check(lib().wco_relaxation_cells(self._r, i, j))
+        return [(int(i[m]), int(j[m])) for m in range(n)]
+
+    def apply(self, s: State, dt: float, target) -> None:
+        t = relaxation_targets(target, len(self))
+        self.solver._upload(s)
+        _check(lib().wco_relaxation_apply(self._r, float(dt), ptr(t)))
+        time = s.time
+        self.solver._download_into(s)
+        s.time = time
 
 
 def linear_wave(omega: float, h: float, dispersive: bool) -> LinearWave:

@@ -379,6 +379,12 @@ Axis axis_terms(const Cell* st, double spacing, int order) {
 }
 
 // ---------------------------------------------------------------- reductions
+// Lane 0's value of a 64-lane xor butterfly that adds lane l^1 first, then l^2, ..., l^32.
+void butterfly(double* v) {
+  for (size_t off = 1; off < 64; off *= 2)
+    for (size_t l = 0; l < 64; l += 2 * off) v[l] = v[l] + v[l + off];
+}
+
 // The GPU's canonical summation order for an interior reduction: see the header.
 template <class Term>
 double canonical_sum(const Grid& g, const Par& par, Term term) {
@@ -395,8 +401,7 @@ double canonical_sum(const Grid& g, const Par& par, Term term) {
         const size_t i = tx * 64 + l, j = ty * 4 + r;
         v[l] = (i < nx && j < ny) ? term(GH + i, GH + j) : 0.0;
       }
-      for (size_t off = 32; off >= 1; off /= 2)
-        for (size_t l = 0; l < off; ++l) v[l] = v[l] + v[l + off];
+      butterfly(v);
       rs[r] = v[0];
     }
     part[p] = (rs[0] + rs[1]) + (rs[2] + rs[3]);
@@ -409,8 +414,7 @@ double canonical_sum(const Grid& g, const Par& par, Term term) {
   double ws[4];
   for (size_t wv = 0; wv < 4; ++wv) {
     double* v = acc + 64 * wv;
-    for (size_t off = 32; off >= 1; off /= 2)
-      for (size_t l = 0; l < off; ++l) v[l] = v[l] + v[l + off];
+    butterfly(v);
     ws[wv] = v[0];
   }
   return (ws[0] + ws[1]) + (ws[2] + ws[3]);
@@ -947,6 +951,79 @@ int wco_with_sponge(wco_solver* s, const wco_sponge_t* sp) {
   x.south = sp->south != 0;
   x.north = sp->north != 0;
   s->sponges.push_back(x);
+  return 0;
+}
+
+// forcing.rs:227-284
+struct wco_relaxation {
+  wco_solver* s = nullptr;
+  std::vector<uint64_t> i, j;
+  std::vector<double> rate;
+};
+
+// Relaxation::new (forcing.rs:239-248)
+int wco_relaxation_new(wco_solver* s, const wco_sponge_t* zones, uint64_t n_zones,
+                       wco_relaxation** out) {
+  auto* r = new wco_relaxation;
+  r->s = s;
+  std::vector<Sponge> zs;
+  for (uint64_t z = 0; z < n_zones; ++z) {
+    Sponge x;
+    x.width = zones[z].width;
+    x.strength = zones[z].strength;
+    x.level = zones[z].level;
+    x.west = zones[z].west != 0;
+    x.east = zones[z].east != 0;
+    x.south = zones[z].south != 0;
+    x.north = zones[z].north != 0;
+    zs.push_back(x);
+  }
+  const Grid& g = s->g;
+  for (size_t j = GH; j < GH + g.ny; ++j)
+    for (size_t i = GH; i < GH + g.nx; ++i) {
+      double rate = 0.0;
+      for (const Sponge& z : zs) rate = wc_max(rate, z.rate(g, i, j));
+      if (rate > 0.0) {
+        r->i.push_back(i);
+        r->j.push_back(j);
+        r->rate.push_back(rate);
+      }
+    }
+  *out = r;
+  return 0;
+}
+
+void wco_relaxation_free(wco_relaxation* r) { delete r; }
+
+uint64_t wco_relaxation_len(const wco_relaxation* r) { return r->i.size(); }
+
+int wco_relaxation_cells(const wco_relaxation* r, uint64_t* i, uint64_t* j) {
+  for (size_t m = 0; m < r->i.size(); ++m) {
+    if (i) i[m] = r->i[m];
+    if (j) j[m] = r->j[m];
+  }
+  return 0;
+}
+
+// Relaxation::apply (forcing.rs:259-283) with relax_cell (213-225)
+int wco_relaxation_apply(wco_relaxation* r, double dt, const double* targets) {
+  State& st = r->s->s;
+  const Grid& g = r->s->g;
+  const double* b = r->s->bed.data();
+  for (size_t m = 0; m < r->i.size(); ++m) {
+    const size_t k = g.idx(r->i[m], r->j[m]);
+    const double f = exp(-r->rate[m] * dt);
+    const double eta = targets[3 * m], u = targets[3 * m + 1], v = targets[3 * m + 2];
+    const double ht = wc_max(eta - b[k], 0.0);
+    st.h[k] = f * st.h[k] + (1.0 - f) * ht;
+    st.hu[k] = f * st.hu[k] + (1.0 - f) * ht * u;
+    st.hv[k] = f * st.hv[k] + (1.0 - f) * ht * v;
+    if (st.h[k] < H_DRY) {
+      st.h[k] = wc_max(st.h[k], 0.0);
+      st.hu[k] = 0.0;
+      st.hv[k] = 0.0;
+    }
+  }
   return 0;
 }
 

@@ -10,8 +10,9 @@
  * the reference's text alone:
  *   - dot order (wco_set_dot_order): 0 = the reference's row order (each row summed serially,
  *     then the rows in order from 0.0, par.rs:90-98); 1 = the GPU's canonical order (64x4-cell
- *     tiles, a 64-lane butterfly per tile row, rows paired, tiles summed by 256 strided lanes
- *     and a final butterfly), which the GPU kernels follow bit for bit;
+ *     tiles, a 64-lane xor butterfly per tile row adding lane l^1 first and l^32 last, rows
+ *     paired, tiles summed by 256 strided lanes and a final butterfly), which the GPU kernels
+ *     follow bit for bit;
  *   - libm (wco_set_math): 0 = glibc's hypot and pow (what the Rust reference calls on Linux);
  *     1 = the portable IEEE-only hypot and pow(h, 4/3) that the GPU evaluates.
  * exp, sin and cos only ever see host-known arguments and always come from glibc, in the
@@ -99,6 +100,15 @@ int wco_state_volume(wco_solver* s, double* out);
 int wco_state_max_abs_momentum(wco_solver* s, double* out);
 int wco_state_min_depth(wco_solver* s, double* out);
 int wco_fill_boundaries(wco_solver* s);
+
+/* Relaxation (forcing.rs:227-284), acting on the oracle solver's state. */
+typedef struct wco_relaxation wco_relaxation;
+int wco_relaxation_new(wco_solver* s, const wco_sponge_t* zones, uint64_t n_zones,
+                       wco_relaxation** out);
+void wco_relaxation_free(wco_relaxation* r);
+uint64_t wco_relaxation_len(const wco_relaxation* r);
+int wco_relaxation_cells(const wco_relaxation* r, uint64_t* i, uint64_t* j);
+int wco_relaxation_apply(wco_relaxation* r, double dt, const double* targets);
 
 int wco_linear_wave(double omega, double h, int32_t dispersive, wco_linear_wave_t* out);
 double wco_manning_factor(double h, double speed, double dt, double n);

@@ -337,6 +337,96 @@ def check_sponge_over_land_never_speeds_water_up(make):
             assert abs(u - 1.0) < 1e-6 and abs(v - 0.5) < 1e-6, (i, j, u, v)
 
 
+def check_relaxing_towards_still_water_keeps_a_lake_at_rest(make, relaxation):
+    """waves.rs:339-352. ``relaxation(solver, zones)`` builds the zone object."""
+    grid = Grid(12, 9, 3.0, 3.0)
+    bed = Bathymetry.rough(grid, 1.0, 1.5)
+    s = State.lake_at_rest(grid, bed, 0.0)
+    before = s.clone()
+    solver = make(grid, bed)
+    zone = relaxation(solver, [Sponge(12, 2.0).edges(True, True, True, True)])
+    zone.apply(s, 0.3, lambda m: (0.0, 0.0, 0.0))
+    h, hu, hv, _ = s.arrays()
+    h0 = before.arrays()[0]
+    for i, j in grid.interior():
+        k = grid.idx(i, j)
+        assert abs(h[k] - h0[k]) < 1e-12, (i, j)
+        assert (hu[k], hv[k]) == (0.0, 0.0)
+
+
+def check_relaxation_moves_depth_and_momentum_together(make, relaxation):
+    """waves.rs:354-375: 1 m of water at 2 m/s pulled towards a surface 0.5 m up at rest; by
+    f = exp(-rate dt) the depth goes to f + 1.5 (1 - f), the momentum to 2 f."""
+    grid = Grid(4, 4, 1.0, 1.0)
+    bed = Bathymetry.flat(grid, 1.0)
+    s = State.lake_at_rest(grid, bed, 0.0)
+    s.hu = 2.0 * s.h
+    solver = make(grid, bed)
+    zone = relaxation(solver, [Sponge(1_000_000, 1.5).edges(True, False, False, False)])
+    zone.apply(s, 0.2, lambda m: (0.5, 0.0, 0.0))
+    i, j = zone.cells()[0]
+    k = grid.idx(i, j)
+    f = math.exp(-1.5 * 0.2)
+    h, hu, _, _ = s.arrays()
+    assert abs(h[k] - (f + 1.5 * (1.0 - f))) < 1e-5
+    assert abs(hu[k] - 2.0 * f) < 1e-5
+    assert hu[k] / h[k] < 2.0
+
+
+def check_relaxation_zone_cells_and_rates(make, relaxation):
+    """Relaxation::new (forcing.rs:239-248): the zone is every interior cell, in row order,
+    where the largest of the zones' rates is positive; each relaxes by exp(-rate dt) towards
+    its own target. Pinned cell by cell against the Sponge::rate formula (forcing.rs:181-206)
+    with targets that differ per cell."""
+    grid = Grid(11, 7, 2.0, 2.0)
+    bed = Bathymetry.flat(grid, 3.0)
+    zones = [Sponge(3, 0.8).edges(True, False, False, True),
+             Sponge(5, 0.3).edges(False, True, False, False)]
+
+    def rate(i, j):
+        a, b = i - GHOST, j - GHOST
+
+        def ramp(d, w):
+            return ((w - d) / w) ** 2 if d < w else 0.0
+
+        best = 0.0
+        for z in zones:
+            r = 0.0
+            if z.west:
+                r = max(r, ramp(a, z.width))
+            if z.east:
+                r = max(r, ramp(grid.nx - 1 - a, z.width))
+            if z.south:
+                r = max(r, ramp(b, z.width))
+            if z.north:
+                r = max(r, ramp(grid.ny - 1 - b, z.width))
+            best = max(best, z.strength * r)
+        return best
+
+    want = [(i, j) for i, j in grid.interior() if rate(i, j) > 0.0]
+    solver = make(grid, bed)
+    zone = relaxation(solver, zones)
+    assert zone.cells() == want
+    s = State.lake_at_rest(grid, bed, 0.0)
+    h0 = s.arrays()[0].copy()
+    targets = np.array([(0.1 * m, 0.01 * m, -0.02 * m) for m in range(len(want))])
+    dt = 0.7
+    zone.apply(s, dt, targets)
+    h, hu, hv, _ = s.arrays()
+    inside = set(want)
+    for m, (i, j) in enumerate(want):
+        k = grid.idx(i, j)
+        f = math.exp(-rate(i, j) * dt)
+        ht = max(targets[m, 0] + 3.0, 0.0)
+        assert abs(h[k] - (f * 3.0 + (1.0 - f) * ht)) < 1e-12
+        assert abs(hu[k] - (1.0 - f) * ht * targets[m, 1]) < 1e-12
+        assert abs(hv[k] - (1.0 - f) * ht * targets[m, 2]) < 1e-12
+    for i, j in grid.interior():
+        if (i, j) not in inside:
+            k = grid.idx(i, j)
+            assert h[k] == h0[k] and hu[k] == 0.0 and hv[k] == 0.0
+
+
 # ---- shallow_cells.rs ---------------------------------------------------------------------------
 def _hump():
     grid = Grid(120, 4, 1.0, 1.0)

@@ -23,8 +23,13 @@ from wavesim_amd.solver import PRECISION_FP32, Solver
 pytestmark = pytest.mark.gpu
 
 
-def fp32(grid, bed):
-    return Solver(grid, bed).with_precision(PRECISION_FP32)
+def fp32(grid, bed, pcg=None):
+    s = Solver(grid, bed).with_precision(PRECISION_FP32)
+    return s.set_option("pcg_kernels", pcg) if pcg else s
+
+
+def fp32_with(pcg):
+    return lambda grid, bed: fp32(grid, bed, pcg)
 
 
 def compare(w, s64: State, s32: State, g64: Solver, g32: Solver):
@@ -45,9 +50,9 @@ def compare(w, s64: State, s32: State, g64: Solver, g32: Solver):
     }
 
 
-def run_pair(w, steps, t0=0.0, state=None):
+def run_pair(w, steps, t0=0.0, state=None, pcg=None):
     g64 = w.solver(Solver)
-    g32 = w.solver(fp32)
+    g32 = w.solver(fp32_with(pcg))
     s64 = state.clone() if state is not None else w.still_water()
     s64.time = t0
     s32 = s64.clone()
@@ -57,11 +62,13 @@ def run_pair(w, steps, t0=0.0, state=None):
     return g64, g32, s64, s32
 
 
-def test_fp32_rough_bed_matches_fp64_within_its_bounds():
+@pytest.mark.parametrize("pcg", [3, 4])
+def test_fp32_rough_bed_matches_fp64_within_its_bounds(pcg):
     w = configs.c1_rough_bed(nx=256, ny=128)
-    g64, g32, s64, s32 = run_pair(w, 600, t0=40.0)
+    g64, g32, s64, s32 = run_pair(w, 600, t0=40.0, pcg=pcg)
     r = compare(w, s64, s32, g64, g32)
-    print("fp32 vs fp64:", r, g64.dispersion_stats(), g32.dispersion_stats(), flush=True)
+    print(f"fp32 (pcg {pcg}) vs fp64:", r, g64.dispersion_stats(), g32.dispersion_stats(),
+          flush=True)
     assert r["eta_max_err_m"] < 1e-3, r
     assert r["mask_agreement"] >= 0.999, r
     st = g32.dispersion_stats()
@@ -76,23 +83,26 @@ def test_fp32_plane_beach_fv_matches_fp64():
     assert r["eta_max_err_m"] < 1e-3, r
 
 
-def test_fp32_lake_at_rest_and_closed_basin_volume():
+@pytest.mark.parametrize("pcg", [3, 4])
+def test_fp32_lake_at_rest_and_closed_basin_volume(pcg):
     # well_balanced.rs:16-62 and 87-121 on the FP32 form
     for mean_depth, amplitude in [(8.0, 3.0), (0.5, 2.5)]:
         grid = Grid(64, 48, 3.0, 3.0)
         bed = Bathymetry.rough(grid, mean_depth, amplitude)
         s = State.lake_at_rest(grid, bed, 0.0)
-        g = fp32(grid, bed).with_order(Order.Fifth).with_dispersion(Dispersion())
+        g = fp32(grid, bed, pcg).with_order(Order.Fifth).with_dispersion(Dispersion())
         v0 = s.volume(grid)
         g.run_steps(s, 300)
         assert s.max_abs_momentum(grid) < 1e-9, s.max_abs_momentum(grid)
         assert abs(s.volume(grid) - v0) < 1e-9 * v0
         assert s.min_depth(grid) >= 0.0
-    ka.check_walls_and_periodic_seam_keep_volume(fp32, cases=[(False, True), (True, True)])
+    ka.check_walls_and_periodic_seam_keep_volume(fp32_with(pcg),
+                                                 cases=[(False, True), (True, True)])
 
 
-def test_fp32_standing_wave_frequency():
-    out = ka.check_standing_wave_sgn_frequency(fp32, orders=(Order.Fifth,))
+@pytest.mark.parametrize("pcg", [3, 4])
+def test_fp32_standing_wave_frequency(pcg):
+    out = ka.check_standing_wave_sgn_frequency(fp32_with(pcg), orders=(Order.Fifth,))
     assert abs(out["Fifth"] - 1.4261) < 3e-3, out
 
 

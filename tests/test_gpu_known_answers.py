@@ -6,7 +6,7 @@ from __future__ import annotations
 import pytest
 
 import known_answers as ka
-from wavesim_amd.solver import Solver, linear_wave
+from wavesim_amd.solver import PRECISION_FP32, PRECISION_FP64, Relaxation, Solver, linear_wave
 
 pytestmark = pytest.mark.gpu
 
@@ -71,6 +71,16 @@ def test_dry_bed_dam_break_matches_ritter():
 
 def test_a_sponge_over_land_never_speeds_the_water_up():
     ka.check_sponge_over_land_never_speeds_water_up(make)
+
+
+@pytest.mark.parametrize("precision", [PRECISION_FP64, PRECISION_FP32])
+def test_relaxation_known_answers(precision):
+    def mk(grid, bed):
+        return Solver(grid, bed).with_precision(precision)
+
+    ka.check_relaxing_towards_still_water_keeps_a_lake_at_rest(mk, Relaxation)
+    ka.check_relaxation_moves_depth_and_momentum_together(mk, Relaxation)
+    ka.check_relaxation_zone_cells_and_rates(mk, Relaxation)
 
 
 def test_held_cells_run_exactly_as_shallow_water():

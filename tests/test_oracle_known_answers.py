@@ -329,6 +329,18 @@ def test_a_sponge_over_land_never_speeds_the_water_up():
     ka.check_sponge_over_land_never_speeds_water_up(make)
 
 
+def test_relaxing_towards_still_water_keeps_a_lake_at_rest():
+    ka.check_relaxing_towards_still_water_keeps_a_lake_at_rest(make, orc.OracleRelaxation)
+
+
+def test_relaxation_moves_depth_and_momentum_towards_the_target_together():
+    ka.check_relaxation_moves_depth_and_momentum_together(make, orc.OracleRelaxation)
+
+
+def test_relaxation_zone_cells_rates_and_targets():
+    ka.check_relaxation_zone_cells_and_rates(make, orc.OracleRelaxation)
+
+
 def test_held_cells_run_exactly_as_shallow_water():
     ka.check_held_cells_run_as_shallow_water(make)
 

@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -55,12 +56,14 @@ enum Family {
   F_SGN_FINISH,
   F_REDUCE,
   F_CG_REDUCE,
+  F_RELAX,
+  F_CG_ITER,
   F_COUNT
 };
 const char* kFamilyNames[F_COUNT] = {"fill_state", "switch",    "flux",      "finish_euler",
                                      "sgn_setup",  "cg_init",   "cg_matvec", "cg_update",
                                      "cg_pupdate", "cg_stats",  "sgn_finish", "reduce",
-                                     "cg_reduce"};
+                                     "cg_reduce",  "relax",     "cg_iter"};
 
 struct HostSponge {
   uint64_t width;
@@ -139,10 +142,13 @@ struct wc_solver {
   float *f_c = nullptr, *f_bx = nullptr, *f_by = nullptr, *f_wx = nullptr, *f_wy = nullptr,
         *f_rx = nullptr, *f_ry = nullptr, *f_px = nullptr, *f_py = nullptr, *f_qx = nullptr,
         *f_qy = nullptr, *f_zx = nullptr, *f_zy = nullptr, *f_ax = nullptr, *f_ay = nullptr;
+  // second buffers of r, A u and A p for the one-kernel PCG (wc_pcg1.hip)
+  float *f_r2x = nullptr, *f_r2y = nullptr, *f_q2x = nullptr, *f_q2y = nullptr,
+        *f_s2x = nullptr, *f_s2y = nullptr;
 
   // implementation options (wc_solver_set_option)
   int pcg_kernels = 3;
-  double* lanes = nullptr;  // 6 x 256 lane sums of the 256-lane PCG
+  double* lanes = nullptr;  // 8 x 256 lane sums of the 256-lane PCGs
   int flux_kernel = 2;
   unsigned long long pcg_batch = 4;
   uint8_t* held = nullptr;
@@ -497,7 +503,7 @@ int pcg3(wc_solver* s, int stage, CgV<V> a, V* pA_x, V* pA_y, V* pB_x, V* pB_y) 
       TRY(launch(s, F_CG_UPDATE, k_cg3_update<V>, grid, block, a, n, (const V*)cur_x,
                  (const V*)cur_y));
     }
-    TRY(launch(s, F_CG_REDUCE, k_cg3_check<V>, dim3(1), dim3(256), a, n));
+    TRY(launch(s, F_CG_REDUCE, k_cg3_check<V>, dim3(1), dim3(64), a, n));
     HIP_OK(hipMemcpyAsync(s->sc_host, s->sc, sizeof(CgScalars), hipMemcpyDeviceToHost,
                           s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
@@ -514,7 +520,7 @@ int pcg3(wc_solver* s, int stage, CgV<V> a, V* pA_x, V* pA_y, V* pB_x, V* pB_y) 
 // only looks at the flag between batches. The first batch is as long as the last solve.
 int pcg(wc_solver* s, int stage) {
   const SgnArgs a = sgn_args(s, stage);
-  if (s->pcg_kernels == 3) {
+  if (s->pcg_kernels >= 3) {  // 4 (one kernel per iteration) is for the FP32 form only
     CgV<double> v{};
     v.g = s->geo;
     v.h = a.h;
@@ -689,7 +695,8 @@ int alloc_fp32(wc_solver* s) {
                     &s->f_s1hl, &s->f_s1hu, &s->f_s1hv, &s->f_l1hu, &s->f_l1hv,
                     &s->f_lhu, &s->f_lhv, &s->f_phi, &s->f_c, &s->f_bx, &s->f_by, &s->f_wx,
                     &s->f_wy, &s->f_rx, &s->f_ry, &s->f_px, &s->f_py, &s->f_qx, &s->f_qy,
-                    &s->f_zx, &s->f_zy, &s->f_ax, &s->f_ay})
+                    &s->f_zx, &s->f_zy, &s->f_ax, &s->f_ay, &s->f_r2x, &s->f_r2y, &s->f_q2x,
+                    &s->f_q2y, &s->f_s2x, &s->f_s2y})
     TRY(dalloc(s, p, n));
   return WC_OK;
 }
@@ -752,7 +759,62 @@ Sgn32 sgn32_args(wc_solver* s, int stage) {
   return a;
 }
 
+// The one-kernel PCG (wc_pcg1.hip), FP32 form: start kernels, then one kernel per iteration in
+// batches, a one-wave check closing every batch.
+int pcg1_32(wc_solver* s, int stage) {
+  const Sgn32 sa = sgn32_args(s, stage);
+  CgG<float> a{};
+  a.g = s->geo;
+  a.h = sa.h;
+  a.c = s->f_c;
+  a.bx = s->f_bx;
+  a.by = s->f_by;
+  a.wx = s->f_wx;
+  a.wy = s->f_wy;
+  a.px = s->f_px;
+  a.py = s->f_py;
+  a.dix = s->f_zx;
+  a.diy = s->f_zy;
+  a.r[0][0] = s->f_rx;
+  a.r[0][1] = s->f_ry;
+  a.r[1][0] = s->f_r2x;
+  a.r[1][1] = s->f_r2y;
+  a.q[0][0] = s->f_ax;
+  a.q[0][1] = s->f_ay;
+  a.q[1][0] = s->f_q2x;
+  a.q[1][1] = s->f_q2y;
+  a.s[0][0] = s->f_qx;
+  a.s[0][1] = s->f_qy;
+  a.s[1][0] = s->f_s2x;
+  a.s[1][1] = s->f_s2y;
+  a.lanes_b = s->lanes;
+  a.lanes = s->lanes + 256;
+  a.sc = s->sc;
+  a.tolerance = s->disp.tolerance;
+  a.max_iterations = s->disp.max_iterations;
+  const dim3 grid(256), block(1024);
+  TRY(launch(s, F_CG_INIT, k_cg1_init<float>, grid, block, a));
+  TRY(launch(s, F_CG_INIT, k_cg1_start<float>, grid, block, a));
+  const unsigned long long maxit = s->disp.max_iterations;
+  unsigned long long n = 1;
+  unsigned long long batch = std::max<unsigned long long>(s->prev_it[stage], 1);
+  while (true) {
+    const unsigned long long stop = std::min(n + batch, maxit + 1);
+    for (; n < stop; ++n) TRY(launch(s, F_CG_ITER, k_cg1_iter<float>, grid, block, a, n));
+    TRY(launch(s, F_CG_REDUCE, k_cg1_check<float>, dim3(1), dim3(64), a, n));
+    HIP_OK(hipMemcpyAsync(s->sc_host, s->sc, sizeof(CgScalars), hipMemcpyDeviceToHost,
+                          s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    if (s->sc_host->done || n > maxit) break;
+    batch = s->pcg_batch;
+  }
+  s->prev_it[stage] = s->sc_host->it;
+  TRY(launch(s, F_CG_STATS, k_cg_stats, dim3(1), dim3(64), s->sc));
+  return WC_OK;
+}
+
 int pcg32(wc_solver* s, int stage) {
+  if (s->pcg_kernels == 4) return pcg1_32(s, stage);
   const Sgn32 a = sgn32_args(s, stage);
   if (s->pcg_kernels == 3) {
     CgV<float> v{};
@@ -965,7 +1027,7 @@ int wc_solver_new(uint64_t nx, uint64_t ny, double dx, double dy, const double* 
   const size_t ntiles = (size_t)g.ntx * g.nty;
   if ((rc = dalloc(s, &s->part, 4 * ntiles)) != WC_OK) return cleanup(rc);
   if ((rc = dalloc(s, &s->sc, 1)) != WC_OK) return cleanup(rc);
-  if ((rc = dalloc(s, &s->lanes, 6 * 256)) != WC_OK) return cleanup(rc);
+  if ((rc = dalloc(s, &s->lanes, 8 * 256)) != WC_OK) return cleanup(rc);
   if ((rc = dalloc(s, &s->red64, 4)) != WC_OK) return cleanup(rc);
   if ((rc = dalloc(s, &s->red32, 4)) != WC_OK) return cleanup(rc);
   if ((rc = dalloc(s, &s->redf, 4)) != WC_OK) return cleanup(rc);
@@ -1133,7 +1195,7 @@ int wc_solver_bed(const wc_solver* s, double* out) {
 int wc_solver_set_option(wc_solver* s, const char* key, int64_t value) {
   if (!key) return fail(WC_EINVAL, "null option name");
   if (strcmp(key, "pcg_kernels") == 0) {
-    if (value < 1 || value > 3) return fail(WC_EINVAL, "pcg_kernels must be 1, 2 or 3");
+    if (value < 1 || value > 4) return fail(WC_EINVAL, "pcg_kernels must be 1, 2, 3 or 4");
     s->pcg_kernels = (int)value;
     return WC_OK;
   }
@@ -1212,6 +1274,150 @@ int wc_step(wc_solver* s, double dt) {
   TRY(set_device(s));
   TRY(s->precision == WC_PRECISION_FP32 ? do_step32(s, dt) : do_step(s, dt));
   if (!s->prof) HIP_OK(hipStreamSynchronize(s->stream));
+  return WC_OK;
+}
+
+// ---- relaxation zones (forcing.rs:227-284) ---------------------------------------------------
+
+struct wc_relaxation {
+  wc_solver* s = nullptr;
+  std::vector<uint64_t> ci, cj;       // padded (i, j), the reference's order
+  std::vector<double> rates;          // the distinct rates, in order of first appearance
+  int* cell = nullptr;                // device: padded index of each cell
+  int* rate = nullptr;                // device: index into rates
+  double* factor = nullptr;           // device: exp(-rate dt) per distinct rate
+  double* target = nullptr;           // device: staging for host targets
+  double* factor_host = nullptr;      // pinned
+  hipEvent_t factor_ev = nullptr;     // the last copy out of factor_host
+  bool factor_pending = false;
+};
+
+int wc_relaxation_new(wc_solver* s, const wc_sponge_t* zones, uint64_t n_zones,
+                      wc_relaxation** out) {
+  if (!s || !out || (n_zones > 0 && !zones)) return fail(WC_EINVAL, "null argument");
+  *out = nullptr;
+  TRY(set_device(s));
+  const Geo& g = s->geo;
+  auto r = std::make_unique<wc_relaxation>();
+  r->s = s;
+  std::vector<int> cell, rate;
+  for (uint64_t j = GH; j < (uint64_t)g.ny + GH; ++j)
+    for (uint64_t i = GH; i < (uint64_t)g.nx + GH; ++i) {
+      // Sponge::rate (forcing.rs:181-206) of each zone, folded with max from 0.0
+      double best = 0.0;
+      for (uint64_t z = 0; z < n_zones; ++z) {
+        const wc_sponge_t& sp = zones[z];
+        const uint64_t a = i - GH, b = j - GH;
+        const double w = (double)sp.width;
+        auto ramp = [&](uint64_t d) {
+          if (d < sp.width) {
+            const double f = (w - (double)d) / w;
+            return f * f;
+          }
+          return 0.0;
+        };
+        double m = 0.0;
+        if (sp.west) m = std::max(m, ramp(a));
+        if (sp.east) m = std::max(m, ramp((uint64_t)g.nx - 1 - a));
+        if (sp.south) m = std::max(m, ramp(b));
+        if (sp.north) m = std::max(m, ramp((uint64_t)g.ny - 1 - b));
+        best = std::max(best, sp.strength * m);
+      }
+      if (!(best > 0.0)) continue;
+      size_t q = 0;
+      while (q < r->rates.size() && r->rates[q] != best) ++q;
+      if (q == r->rates.size()) r->rates.push_back(best);
+      r->ci.push_back(i);
+      r->cj.push_back(j);
+      cell.push_back((int)(j * g.w + i));
+      rate.push_back((int)q);
+    }
+  const size_t n = cell.size();
+  HIP_OK(hipMalloc(&r->cell, std::max<size_t>(n, 1) * sizeof(int)));
+  HIP_OK(hipMalloc(&r->rate, std::max<size_t>(n, 1) * sizeof(int)));
+  HIP_OK(hipMalloc(&r->target, std::max<size_t>(3 * n, 1) * sizeof(double)));
+  HIP_OK(hipMalloc(&r->factor, std::max<size_t>(r->rates.size(), 1) * sizeof(double)));
+  HIP_OK(hipHostMalloc(&r->factor_host, std::max<size_t>(r->rates.size(), 1) * sizeof(double),
+                       hipHostMallocDefault));
+  HIP_OK(hipEventCreateWithFlags(&r->factor_ev, hipEventDisableTiming));
+  if (n) {
+    HIP_OK(hipMemcpy(r->cell, cell.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(r->rate, rate.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  }
+  *out = r.release();
+  return WC_OK;
+}
+
+void wc_relaxation_free(wc_relaxation* r) {
+  if (!r) return;
+  (void)hipSetDevice(r->s->device);
+  if (r->factor_ev) {
+    (void)hipEventSynchronize(r->factor_ev);
+    (void)hipEventDestroy(r->factor_ev);
+  }
+  (void)hipStreamSynchronize(r->s->stream);
+  (void)hipFree(r->cell);
+  (void)hipFree(r->rate);
+  (void)hipFree(r->target);
+  (void)hipFree(r->factor);
+  (void)hipHostFree(r->factor_host);
+  delete r;
+}
+
+uint64_t wc_relaxation_len(const wc_relaxation* r) { return r ? (uint64_t)r->ci.size() : 0; }
+
+int wc_relaxation_cells(const wc_relaxation* r, uint64_t* i, uint64_t* j) {
+  if (!r) return fail(WC_EINVAL, "null relaxation");
+  for (size_t m = 0; m < r->ci.size(); ++m) {
+    if (i) i[m] = r->ci[m];
+    if (j) j[m] = r->cj[m];
+  }
+  return WC_OK;
+}
+
+namespace {
+
+int relax_apply(wc_relaxation* r, double dt, const double* targets_device) {
+  wc_solver* s = r->s;
+  const size_t n = r->ci.size();
+  if (n == 0) return WC_OK;
+  // exp(-rate dt) with the reference's expression, once per distinct rate; the pinned buffer
+  // is rewritten only after its previous copy has left it.
+  if (r->factor_pending) HIP_OK(hipEventSynchronize(r->factor_ev));
+  for (size_t q = 0; q < r->rates.size(); ++q) r->factor_host[q] = exp(-r->rates[q] * dt);
+  HIP_OK(hipMemcpyAsync(r->factor, r->factor_host, r->rates.size() * sizeof(double),
+                        hipMemcpyHostToDevice, s->stream));
+  HIP_OK(hipEventRecord(r->factor_ev, s->stream));
+  r->factor_pending = true;
+  const RelaxArgs a{(long long)n, r->cell, r->rate, r->factor, targets_device, s->b};
+  const dim3 grid((unsigned)((n + NT - 1) / NT));
+  if (s->precision == WC_PRECISION_FP32)
+    TRY(launch(s, F_RELAX, k_relax32, grid, dim3(NT), a, s->f_sh, s->f_slh, s->f_shu, s->f_slhu,
+               s->f_shv, s->f_slhv));
+  else
+    TRY(launch(s, F_RELAX, k_relax64, grid, dim3(NT), a, s->h, s->hu, s->hv));
+  return WC_OK;
+}
+
+}  // namespace
+
+int wc_relaxation_apply(wc_relaxation* r, double dt, const double* targets) {
+  if (!r || (!targets && !r->ci.empty())) return fail(WC_EINVAL, "null argument");
+  TRY(set_device(r->s));
+  const size_t n = r->ci.size();
+  if (n)
+    HIP_OK(hipMemcpyAsync(r->target, targets, 3 * n * sizeof(double), hipMemcpyHostToDevice,
+                          r->s->stream));
+  TRY(relax_apply(r, dt, r->target));
+  if (!r->s->prof) HIP_OK(hipStreamSynchronize(r->s->stream));
+  return WC_OK;
+}
+
+int wc_relaxation_apply_device(wc_relaxation* r, double dt, const double* targets) {
+  if (!r || (!targets && !r->ci.empty())) return fail(WC_EINVAL, "null argument");
+  TRY(set_device(r->s));
+  TRY(relax_apply(r, dt, targets));
+  if (!r->s->prof) HIP_OK(hipStreamSynchronize(r->s->stream));
   return WC_OK;
 }
 

@@ -71,13 +71,37 @@ __device__ __forceinline__ int tile_of_block(int b, int nb) {
 
 // ---- the canonical reduction --------------------------------------------------------------
 // Interior reductions sum in one fixed order, which the oracle's canonical mode repeats:
-// within a 64x4 tile, a 64-lane butterfly per row (lane l adds lane l^32, then l^16, ...),
-// rows paired as (r0 + r1) + (r2 + r3); then the tile sums, by 256 lanes that each add tiles
-// lane, lane+256, ... from 0.0, and a final butterfly per wave with the waves paired.
+// within a 64x4 tile, a 64-lane butterfly per row (lane l adds lane l^1, then l^2, l^4, ...,
+// l^32), rows paired as (r0 + r1) + (r2 + r3); then the tile sums, by 256 lanes that each add
+// tiles lane, lane+256, ... from 0.0, and a final butterfly per wave with the waves paired.
+//
+// The butterfly runs without the LDS crossbar: the first two steps are quad permutations, and
+// once each quad (then each half row) holds one value, the l^4 and l^8 partners are the same
+// values the half-row and row mirrors bring; the l^16 and l^32 steps add the four row sums,
+// read from lanes 0, 16, 32, 48, as (R0 + R1) + (R2 + R3). Every lane computes its own sum
+// and its partner's with the same two operands, so all 64 lanes end with the same bits, which
+// are those of the plain butterfly. All 64 lanes of the wave must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
-  return v;
+  v = v + dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]: lane l^1
+  v = v + dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]: lane l^2
+  v = v + dpp_move<0x141>(v);  // row_half_mirror: the other quad of the half row, l^4
+  v = v + dpp_move<0x140>(v);  // row_mirror: the other half row, l^8
+  const double r0 = lane_value(v, 0), r1 = lane_value(v, 16);
+  const double r2 = lane_value(v, 32), r3 = lane_value(v, 48);
+  return (r0 + r1) + (r2 + r3);
 }
 
 template <int N>

@@ -27,6 +27,7 @@ constexpr int MAX_SPONGES = 16;
 struct CgScalars {
   double rz, target, alpha, beta;
   double rzs[2];  // r.z of the previous iteration, by iteration parity (wc_pcg.hip)
+  double alphas[2];  // alpha of the previous iteration, by parity (wc_pcg1.hip)
   unsigned long long it;
   int converged, done;
   unsigned long long solves, iterations, unconverged;
@@ -94,6 +95,32 @@ struct CgV {
   double tolerance;
   unsigned long long max_iterations;
 };
+
+// The one-kernel-per-iteration PCG of wc_pcg1.hip: r, q (= A u) and s (= A p) alternate
+// between two buffers by iteration parity, p and w are updated in place, dix/diy hold the
+// inverse Jacobi diagonal.
+template <typename V>
+struct CgG {
+  Geo g;
+  const V *h, *c, *bx, *by;
+  V *wx, *wy, *px, *py, *dix, *diy;
+  V* r[2][2];
+  V* q[2][2];
+  V* s[2][2];
+  double* lanes_b;  // 256 lanes of b.b
+  double* lanes;    // 2 x 3 x 256 lanes of r.u, q.u, r.r, by parity
+  CgScalars* sc;
+  double tolerance;
+  unsigned long long max_iterations;
+};
+template <typename V>
+__global__ void k_cg1_init(CgG<V> a);
+template <typename V>
+__global__ void k_cg1_start(CgG<V> a);
+template <typename V>
+__global__ void k_cg1_iter(CgG<V> a, unsigned long long n);
+template <typename V>
+__global__ void k_cg1_check(CgG<V> a, unsigned long long n);
 
 template <typename V>
 __global__ void k_cg3_init(CgV<V> a);
@@ -180,5 +207,20 @@ __global__ void k_state_sums(Geo g, const double* h, const double* hu, const dou
                              unsigned long long* minh_key);
 __global__ void k_sum_tiles(const double* part, int np, double* out);
 __global__ void k_cg_reduce(SgnArgs a, int kind);
+
+// Relaxation::apply (forcing.rs:259-283): cell m of the zone list relaxes by the factor
+// factor[rate[m]] (exp(-rate dt), evaluated on the host per distinct rate) towards
+// target[3m .. 3m+2] = (eta, u, v).
+struct RelaxArgs {
+  long long n;
+  const int* cell;
+  const int* rate;
+  const double* factor;
+  const double* target;
+  const double* b;
+};
+__global__ void k_relax64(RelaxArgs a, double* h, double* hu, double* hv);
+__global__ void k_relax32(RelaxArgs a, float* h, float* hl, float* hu, float* hul, float* hv,
+                          float* hvl);
 
 }  // namespace wc

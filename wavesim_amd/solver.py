@@ -79,6 +79,12 @@ SIGNATURES = {
     "wc_linear_wave": (_I32, [_D, _D, _I32, ctypes.POINTER(CLinearWave)]),
     "wc_manning_factor": (_D, [_D, _D, _D, _D]),
     "wc_run_steps": (_I32, [_VP, _U64, ctypes.POINTER(_U64), _P]),
+    "wc_relaxation_new": (_I32, [_VP, ctypes.POINTER(CSponge), _U64, ctypes.POINTER(_VP)]),
+    "wc_relaxation_free": (None, [_VP]),
+    "wc_relaxation_len": (_U64, [_VP]),
+    "wc_relaxation_cells": (_I32, [_VP, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
+    "wc_relaxation_apply": (_I32, [_VP, _D, _P]),
+    "wc_relaxation_apply_device": (_I32, [_VP, _D, _VP]),
     "wc_profile_enable": (_I32, [_VP, _I32]),
     "wc_profile_reset": (_I32, [_VP]),
     "wc_profile_read": (_I32, [_VP, ctypes.c_char_p, _U64, _P, ctypes.POINTER(_U64), _U64,
@@ -336,6 +342,62 @@ class Solver:
         _check(lib().wc_profile_read(self._h, names, 1024, ms, n, 64, ctypes.byref(cnt)))
         keys = names.value.decode().split(";")
         return {k: (ms[q], int(n[q])) for q, k in enumerate(keys[: cnt.value])}
+
+
+def relaxation_targets(target, n: int) -> np.ndarray:
+    """(eta, u, v) of each of the n zone cells, interleaved, from the reference's closure
+    ``m -> (eta, u, v)`` or from an array of shape (n, 3)."""
+    if callable(target):
+        out = np.empty((n, 3))
+        for m in range(n):
+            out[m] = target(m)
+    else:
+        out = np.array(target, dtype=np.float64).reshape(n, 3)
+    return np.ascontiguousarray(out.reshape(-1))
+
+
+class Relaxation:
+    """wavecore's ``Relaxation`` (forcing.rs:227-284): zones where the water is pulled towards
+    targets the caller supplies at every step.
+
+    ``Relaxation(solver, zones)`` is ``Relaxation::new(grid, zones)`` for the solver's grid;
+    ``apply(state, dt, target)`` is ``apply(grid, bed, state, dt, target)`` with the solver's
+    bed, on the device copy of ``state``. ``target`` is the reference's closure
+    ``m -> (eta, u, v)``, an (n, 3) array, or a device pointer (an int) to 3n doubles.
+    """
+
+    def __init__(self, solver: "Solver", zones):
+        self.solver = solver
+        arr = (CSponge * max(len(zones), 1))(*[c_sponge(z) for z in zones])
+        h = _VP()
+        _check(lib().wc_relaxation_new(solver._h, arr, len(zones), ctypes.byref(h)))
+        self._r = h
+
+    def __del__(self):
+        if getattr(self, "_r", None):
+            lib().wc_relaxation_free(self._r)
+            self._r = None
+
+    def __len__(self) -> int:
+        return int(lib().wc_relaxation_len(self._r))
+
+    def cells(self) -> list[tuple[int, int]]:
+        """forcing.rs:252-254: padded (i, j) of each cell, in the order ``apply`` takes."""
+        n = len(self)
+        i = (_U64 * max(n, 1))()
+        j = (_U64 * max(n, 1))()
+        _check(lib().wc_relaxation_cells(self._r, i, j))
+        return [(int(i[m]), int(j[m])) for m in range(n)]
+
+    def apply(self, s: State, dt: float, target) -> None:
+        """forcing.rs:259-283."""
+        self.solver._hold(s)
+        if isinstance(target, int):
+            _check(lib().wc_relaxation_apply_device(self._r, float(dt), _VP(target)))
+        else:
+            t = relaxation_targets(target, len(self))
+            _check(lib().wc_relaxation_apply(self._r, float(dt), _ptr(t)))
+        s._host_valid = False
 
 
 def linear_wave(omega: float, h: float, dispersive: bool) -> LinearWave:
