@@ -113,9 +113,12 @@ int wc_solver_with_periodic_y(wc_solver* s);                          /* solver.
 int wc_solver_with_shallow_cells(wc_solver* s, const uint8_t* cells, uint64_t n);
 /* The bed as the solver holds it (with_periodic_y wraps its ghost rows; solver.rs:416). */
 int wc_solver_bed(const wc_solver* s, double* out);
-/* Implementation choices that change speed, never results (no counterpart in the reference):
+/* Implementation choices (no counterpart in the reference). In FP64 they change speed, never
+ * results; in the FP32 form "pcg_kernels" 4 also changes the rounding:
  *   "pcg_kernels" 1 = three kernels and two reductions per PCG iteration; 2 = two kernels and
- *                 two reductions; 3 = two kernels, reductions folded into them (default);
+ *                 two reductions; 3 = two kernels, reductions folded into them (FP64 default);
+ *                 4 = one kernel per iteration, Chronopoulos-Gear recurrences (FP32 default;
+ *                 FP64 runs 3 instead);
  *   "pcg_batch"   PCG iterations queued between looks at the convergence flag (default 4);
  *   "flux_kernel" 1 = one thread per cell, as the reference's loop; 2 = LDS tiles, each cell
  *                 reconstructed and each face fluxed once (default). */

@@ -142,12 +142,12 @@ struct wc_solver {
   float *f_c = nullptr, *f_bx = nullptr, *f_by = nullptr, *f_wx = nullptr, *f_wy = nullptr,
         *f_rx = nullptr, *f_ry = nullptr, *f_px = nullptr, *f_py = nullptr, *f_qx = nullptr,
         *f_qy = nullptr, *f_zx = nullptr, *f_zy = nullptr, *f_ax = nullptr, *f_ay = nullptr;
-  // second buffers of r, A u and A p for the one-kernel PCG (wc_pcg1.hip)
-  float *f_r2x = nullptr, *f_r2y = nullptr, *f_q2x = nullptr, *f_q2y = nullptr,
-        *f_s2x = nullptr, *f_s2y = nullptr;
+  // the one-kernel PCG's own vectors (wc_pcg1.hip), interleaved (x, y)
+  float2 *g_r[2] = {nullptr, nullptr}, *g_q[2] = {nullptr, nullptr},
+         *g_s[2] = {nullptr, nullptr}, *g_p = nullptr, *g_di = nullptr;
 
   // implementation options (wc_solver_set_option)
-  int pcg_kernels = 3;
+  int pcg_kernels = 0;  // 0: the default of each precision (FP64 3, FP32 4)
   double* lanes = nullptr;  // 8 x 256 lane sums of the 256-lane PCGs
   int flux_kernel = 2;
   unsigned long long pcg_batch = 4;
@@ -520,7 +520,8 @@ int pcg3(wc_solver* s, int stage, CgV<V> a, V* pA_x, V* pA_y, V* pB_x, V* pB_y) 
 // only looks at the flag between batches. The first batch is as long as the last solve.
 int pcg(wc_solver* s, int stage) {
   const SgnArgs a = sgn_args(s, stage);
-  if (s->pcg_kernels >= 3) {  // 4 (one kernel per iteration) is for the FP32 form only
+  // 0 (default) and 4 (one kernel per iteration, the FP32 form's) run the 256-lane form here
+  if (s->pcg_kernels == 0 || s->pcg_kernels >= 3) {
     CgV<double> v{};
     v.g = s->geo;
     v.h = a.h;
@@ -695,8 +696,10 @@ int alloc_fp32(wc_solver* s) {
                     &s->f_s1hl, &s->f_s1hu, &s->f_s1hv, &s->f_l1hu, &s->f_l1hv,
                     &s->f_lhu, &s->f_lhv, &s->f_phi, &s->f_c, &s->f_bx, &s->f_by, &s->f_wx,
                     &s->f_wy, &s->f_rx, &s->f_ry, &s->f_px, &s->f_py, &s->f_qx, &s->f_qy,
-                    &s->f_zx, &s->f_zy, &s->f_ax, &s->f_ay, &s->f_r2x, &s->f_r2y, &s->f_q2x,
-                    &s->f_q2y, &s->f_s2x, &s->f_s2y})
+                    &s->f_zx, &s->f_zy, &s->f_ax, &s->f_ay})
+    TRY(dalloc(s, p, n));
+  for (float2** p : {&s->g_r[0], &s->g_r[1], &s->g_q[0], &s->g_q[1], &s->g_s[0], &s->g_s[1],
+                     &s->g_p, &s->g_di})
     TRY(dalloc(s, p, n));
   return WC_OK;
 }
@@ -763,7 +766,7 @@ Sgn32 sgn32_args(wc_solver* s, int stage) {
 // batches, a one-wave check closing every batch.
 int pcg1_32(wc_solver* s, int stage) {
   const Sgn32 sa = sgn32_args(s, stage);
-  CgG<float> a{};
+  CgG a{};
   a.g = s->geo;
   a.h = sa.h;
   a.c = s->f_c;
@@ -771,37 +774,28 @@ int pcg1_32(wc_solver* s, int stage) {
   a.by = s->f_by;
   a.wx = s->f_wx;
   a.wy = s->f_wy;
-  a.px = s->f_px;
-  a.py = s->f_py;
-  a.dix = s->f_zx;
-  a.diy = s->f_zy;
-  a.r[0][0] = s->f_rx;
-  a.r[0][1] = s->f_ry;
-  a.r[1][0] = s->f_r2x;
-  a.r[1][1] = s->f_r2y;
-  a.q[0][0] = s->f_ax;
-  a.q[0][1] = s->f_ay;
-  a.q[1][0] = s->f_q2x;
-  a.q[1][1] = s->f_q2y;
-  a.s[0][0] = s->f_qx;
-  a.s[0][1] = s->f_qy;
-  a.s[1][0] = s->f_s2x;
-  a.s[1][1] = s->f_s2y;
+  a.p = s->g_p;
+  a.di = s->g_di;
+  for (int m = 0; m < 2; ++m) {
+    a.r[m] = s->g_r[m];
+    a.q[m] = s->g_q[m];
+    a.s[m] = s->g_s[m];
+  }
   a.lanes_b = s->lanes;
   a.lanes = s->lanes + 256;
   a.sc = s->sc;
   a.tolerance = s->disp.tolerance;
   a.max_iterations = s->disp.max_iterations;
   const dim3 grid(256), block(1024);
-  TRY(launch(s, F_CG_INIT, k_cg1_init<float>, grid, block, a));
-  TRY(launch(s, F_CG_INIT, k_cg1_start<float>, grid, block, a));
+  TRY(launch(s, F_CG_INIT, k_cg1_init, grid, block, a));
+  TRY(launch(s, F_CG_INIT, k_cg1_start, grid, block, a));
   const unsigned long long maxit = s->disp.max_iterations;
   unsigned long long n = 1;
   unsigned long long batch = std::max<unsigned long long>(s->prev_it[stage], 1);
   while (true) {
     const unsigned long long stop = std::min(n + batch, maxit + 1);
-    for (; n < stop; ++n) TRY(launch(s, F_CG_ITER, k_cg1_iter<float>, grid, block, a, n));
-    TRY(launch(s, F_CG_REDUCE, k_cg1_check<float>, dim3(1), dim3(64), a, n));
+    for (; n < stop; ++n) TRY(launch(s, F_CG_ITER, k_cg1_iter, grid, block, a, n));
+    TRY(launch(s, F_CG_REDUCE, k_cg1_check, dim3(1), dim3(64), a, n));
     HIP_OK(hipMemcpyAsync(s->sc_host, s->sc, sizeof(CgScalars), hipMemcpyDeviceToHost,
                           s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
@@ -814,7 +808,7 @@ int pcg1_32(wc_solver* s, int stage) {
 }
 
 int pcg32(wc_solver* s, int stage) {
-  if (s->pcg_kernels == 4) return pcg1_32(s, stage);
+  if (s->pcg_kernels == 0 || s->pcg_kernels == 4) return pcg1_32(s, stage);
   const Sgn32 a = sgn32_args(s, stage);
   if (s->pcg_kernels == 3) {
     CgV<float> v{};
@@ -1276,6 +1270,11 @@ int wc_step(wc_solver* s, double dt) {
   if (!s->prof) HIP_OK(hipStreamSynchronize(s->stream));
   return WC_OK;
 }
+
+namespace wc {
+// The error setter for the other sources of the library (wc_unet.hip).
+int api_fail(int code, const char* msg) { return fail(code, msg); }
+}  // namespace wc
 
 // ---- relaxation zones (forcing.rs:227-284) ---------------------------------------------------
 

@@ -96,31 +96,27 @@ struct CgV {
   unsigned long long max_iterations;
 };
 
-// The one-kernel-per-iteration PCG of wc_pcg1.hip: r, q (= A u) and s (= A p) alternate
-// between two buffers by iteration parity, p and w are updated in place, dix/diy hold the
-// inverse Jacobi diagonal.
-template <typename V>
+// The one-kernel-per-iteration PCG of wc_pcg1.hip (FP32 form): r, q (= A u) and s (= A p)
+// alternate between two buffers by iteration parity, p and w are updated in place, di holds the
+// inverse Jacobi diagonal; r, q, s, p and di are interleaved (x, y) pairs.
 struct CgG {
   Geo g;
-  const V *h, *c, *bx, *by;
-  V *wx, *wy, *px, *py, *dix, *diy;
-  V* r[2][2];
-  V* q[2][2];
-  V* s[2][2];
+  const float *h, *c, *bx, *by;
+  float *wx, *wy;
+  float2 *p, *di;
+  float2* r[2];
+  float2* q[2];
+  float2* s[2];
   double* lanes_b;  // 256 lanes of b.b
   double* lanes;    // 2 x 3 x 256 lanes of r.u, q.u, r.r, by parity
   CgScalars* sc;
   double tolerance;
   unsigned long long max_iterations;
 };
-template <typename V>
-__global__ void k_cg1_init(CgG<V> a);
-template <typename V>
-__global__ void k_cg1_start(CgG<V> a);
-template <typename V>
-__global__ void k_cg1_iter(CgG<V> a, unsigned long long n);
-template <typename V>
-__global__ void k_cg1_check(CgG<V> a, unsigned long long n);
+__global__ void k_cg1_init(CgG a);
+__global__ void k_cg1_start(CgG a);
+__global__ void k_cg1_iter(CgG a, unsigned long long n);
+__global__ void k_cg1_check(CgG a, unsigned long long n);
 
 template <typename V>
 __global__ void k_cg3_init(CgV<V> a);

@@ -16,9 +16,14 @@
 // Kernel K_n does iteration i = n - 1. A u_{i+1} needs u on the tile + 2, so the tile recomputes
 // s_i, r_{i+1} and u_{i+1} there from r_i, q_i and s_{i-1}, which K_{n-1} wrote: r, q and s
 // alternate between two buffers by iteration parity (neighbouring tiles read the old values
-// while this one writes the new), p and w are updated in place. The lane sums follow the
-// canonical order of wc_pcg.hip (256 blocks, block b owns tiles b, b+256, ...), in two buffers
-// by parity; every wave of K_{n+1} reduces them itself.
+// while this one writes the new), p and w are updated in place.
+//
+// Nothing here has to repeat the oracle's summation order, so the layout is this kernel's own:
+// tiles of 64x16 cells (one 1024-thread block, a cell per thread; the halo is a third of the
+// tile rather than the 64x4 tile's 113%), and r, q, s, p and the inverse diagonal stored as
+// interleaved (x, y) pairs, one 8-byte load per cell and vector. The 256 blocks own tiles b,
+// b+256, ...; each adds its tiles' sums (rows in order) into one lane, and every wave of the
+// next kernel reduces the 256 lanes itself, so no separate reduction kernel runs.
 
 #include "wc_kernels.h"
 
@@ -27,10 +32,10 @@ namespace wc {
 namespace {
 
 constexpr int LANES = 256;
-constexpr int PG = 4;
-constexpr int PT = NT * PG;
-constexpr int S2X = TX + 4, S2Y = TY + 4;
-constexpr int S1X = TX + 2, S1Y = TY + 2;
+constexpr int BX = 64, BY = 16;  // tile
+constexpr int PT = BX * BY;      // threads per block, a cell each
+constexpr int S2X = BX + 4, S2Y = BY + 4;
+constexpr int S1X = BX + 2, S1Y = BY + 2;
 constexpr int H2 = S2X * S2Y, H1 = S1X * S1Y;
 
 template <int N>
@@ -46,87 +51,85 @@ __device__ __forceinline__ void lane_sums(const double* acc, double (&out)[N]) {
     out[n] = (wave_sum(v[n][0]) + wave_sum(v[n][1])) + (wave_sum(v[n][2]) + wave_sum(v[n][3]));
 }
 
-// Tile sums of the round's four tiles (one per 256-thread group), added in tile order.
+__device__ __forceinline__ int tiles_x(const Geo& g) { return (g.nx + BX - 1) / BX; }
+__device__ __forceinline__ int tiles_all(const Geo& g) {
+  return tiles_x(g) * ((g.ny + BY - 1) / BY);
+}
+
+// The round's row sums (one wave per tile row), then thread 0 adds the rows in order.
 template <int N>
-__device__ __forceinline__ void add_round(double (&v)[N], double (*sums)[PG][N][TY], int r,
-                                          int np, double (&acc)[N]) {
-  const int q = threadIdx.x / NT, local = threadIdx.x % NT;
-  const int lane = local & 63, row = local >> 6;
+__device__ __forceinline__ void add_rows(double (&v)[N], double (*sums)[BY][N], int r,
+                                         double (&acc)[N]) {
+  const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
 #pragma unroll
   for (int n = 0; n < N; ++n) {
     const double t = wave_sum(v[n]);
-    if (lane == 0) sums[r & 1][q][n][row] = t;
+    if (lane == 0) sums[r & 1][row][n] = t;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int g2 = 0; g2 < PG; ++g2) {
-      if ((int)blockIdx.x + LANES * (PG * r + g2) >= np) continue;
-      const double* t = &sums[r & 1][g2][0][0];
-      for (int n = 0; n < N; ++n)
-        acc[n] = acc[n] + ((t[n * TY + 0] + t[n * TY + 1]) + (t[n * TY + 2] + t[n * TY + 3]));
-    }
-  }
+  if (threadIdx.x == 0)
+    for (int y = 0; y < BY; ++y)
+      for (int n = 0; n < N; ++n) acc[n] = acc[n] + sums[r & 1][y][n];
 }
 
 // Stages the vector u (given per halo cell by fn(m, vx, vy), m the mapped interior index; the
 // wall flips are applied here) on the tile + 2 and c (grad . u) on the tile + 1; returns A u
 // at the thread's cell (and u there).
-template <typename V, class Fn>
-__device__ __forceinline__ void stencil(const Geo& g, const V* c, const V* hh, int tile,
-                                        double (*s_ux)[S2Y][S2X], double (*s_uy)[S2Y][S2X],
-                                        double (*s_f)[S1Y][S1X], Fn fn, bool& own, int& k,
-                                        double& ux, double& uy, double& ax, double& ay) {
-  const int q = threadIdx.x / NT, local = threadIdx.x % NT;
-  const bool have = tile >= 0;
-  const int i0 = GH + (have ? tile % g.ntx : 0) * TX, j0 = GH + (have ? tile / g.ntx : 0) * TY;
+template <class Fn>
+__device__ __forceinline__ void stencil(const Geo& g, const float* c, const float* hh, int tile,
+                                        double (*s_u)[S2Y][S2X], double (*s_f)[S1X], Fn fn,
+                                        bool& own, int& k, double& ux, double& uy, double& ax,
+                                        double& ay) {
+  const int t0 = threadIdx.x;
+  const int tx = tiles_x(g);
+  const int i0 = GH + (tile % tx) * BX, j0 = GH + (tile / tx) * BY;
   const double ix = 0.5 / g.dx, iy = 0.5 / g.dy;
-  for (int t = local; t < H2; t += NT) {
+  for (int t = t0; t < H2; t += PT) {
     const int lx = t % S2X, ly = t / S2X;
     const int i = i0 - 2 + lx, j = j0 - 2 + ly;
     double vx = 0.0, vy = 0.0;
-    if (have && i <= g.nx + GH + 1 && j <= g.ny + GH + 1) {
+    if (i <= g.nx + GH + 1 && j <= g.ny + GH + 1) {
       const int m = map_y(j, g.ny, g.periodic) * g.w + map_x(i, g.nx);
       fn(m, vx, vy);
       if (is_ghost_x(i, g.nx)) vx = -vx;
       if (!g.periodic && is_ghost_y(j, g.ny)) vy = -vy;
     }
-    s_ux[q][ly][lx] = vx;
-    s_uy[q][ly][lx] = vy;
+    s_u[0][ly][lx] = vx;
+    s_u[1][ly][lx] = vy;
   }
   __syncthreads();
-  for (int t = local; t < H1; t += NT) {
+  for (int t = t0; t < H1; t += PT) {
     const int lx = t % S1X, ly = t / S1X;
     const int i = i0 - 1 + lx, j = j0 - 1 + ly;
     double f = 0.0;
-    if (have && i <= g.nx + GH && j <= g.ny + GH) {
+    if (i <= g.nx + GH && j <= g.ny + GH) {
       const int x = lx + 1, y = ly + 1;
-      f = (double)c[j * g.w + i] * ((s_ux[q][y][x + 1] - s_ux[q][y][x - 1]) * ix +
-                                    (s_uy[q][y + 1][x] - s_uy[q][y - 1][x]) * iy);
+      f = (double)c[j * g.w + i] * ((s_u[0][y][x + 1] - s_u[0][y][x - 1]) * ix +
+                                    (s_u[1][y + 1][x] - s_u[1][y - 1][x]) * iy);
     }
-    s_f[q][ly][lx] = f;
+    s_f[ly][lx] = f;
   }
   __syncthreads();
-  const int lx = local & 63, ly = local >> 6;
+  const int lx = t0 & 63, ly = t0 >> 6;
   const int i = i0 + lx, j = j0 + ly;
-  own = have && i < g.nx + GH && j < g.ny + GH;
+  own = i < g.nx + GH && j < g.ny + GH;
   k = j * g.w + i;
-  ux = s_ux[q][ly + 2][lx + 2];
-  uy = s_uy[q][ly + 2][lx + 2];
+  ux = s_u[0][ly + 2][lx + 2];
+  uy = s_u[1][ly + 2][lx + 2];
   ax = ay = 0.0;
   if (own) {
     const double h = (double)hh[k];
     const double mask = h >= WC_H_DISPERSION_MIN ? 1.0 : 0.0;
     const double heff = wc_max(h, WC_H_DRY);
     const int x1 = lx + 1, y1 = ly + 1;
-    ax = mask * (heff * ux - (s_f[q][y1][x1 + 1] - s_f[q][y1][x1 - 1]) * ix);
-    ay = mask * (heff * uy - (s_f[q][y1 + 1][x1] - s_f[q][y1 - 1][x1]) * iy);
+    ax = mask * (heff * ux - (s_f[y1][x1 + 1] - s_f[y1][x1 - 1]) * ix);
+    ay = mask * (heff * uy - (s_f[y1 + 1][x1] - s_f[y1 - 1][x1]) * iy);
   }
 }
 
 // The scalars of iteration i = n - 1, from K_{n-1}'s lanes (or the start's); false when the
 // solve is over. Every wave computes the same numbers; block 0 records them.
-template <typename V>
-__device__ __forceinline__ bool start(const CgG<V>& a, unsigned long long n, double& alpha,
+__device__ __forceinline__ bool start(const CgG& a, unsigned long long n, double& alpha,
                                       double& beta) {
   CgScalars* sc = a.sc;
   const unsigned long long i = n - 1;
@@ -162,30 +165,29 @@ __device__ __forceinline__ bool start(const CgG<V>& a, unsigned long long n, dou
   return !done;
 }
 
+__device__ __forceinline__ float2 f2(double x, double y) { return make_float2((float)x, (float)y); }
+
 }  // namespace
 
 // r_0 = b - A w (w zero where masked, as set up), the inverse Jacobi diagonal
-// (dispersion.rs:466-480), lanes of b.b.
-template <typename V>
-__global__ void __launch_bounds__(PT) k_cg1_init(CgG<V> a) {
+// (dispersion.rs:466-480), p = s_{-1} = 0, lanes of b.b.
+__global__ void __launch_bounds__(PT) k_cg1_init(CgG a) {
   const Geo& g = a.g;
-  const int np = g.ntx * g.nty;
-  const int q = threadIdx.x / NT;
-  __shared__ double s_ux[PG][S2Y][S2X], s_uy[PG][S2Y][S2X], s_f[PG][S1Y][S1X];
-  __shared__ double sums[2][PG][1][TY];
+  const int nt = tiles_all(g);
+  __shared__ double s_u[2][S2Y][S2X], s_f[S1Y][S1X];
+  __shared__ double sums[2][BY][1];
   double acc[1] = {0.0};
-  for (int r = 0; (int)blockIdx.x + LANES * PG * r < np; ++r) {
-    const int tile0 = blockIdx.x + LANES * (PG * r + q);
-    const int tile = tile0 < np ? tile0 : -1;
+  for (int r = 0; (int)blockIdx.x + LANES * r < nt; ++r) {
+    const int tile = blockIdx.x + LANES * r;
     bool own;
     int k;
     double ux, uy, ax, ay;
-    stencil<V>(g, a.c, a.h, tile, s_ux, s_uy, s_f,
-               [&](int m, double& vx, double& vy) {
-                 vx = (double)a.wx[m];
-                 vy = (double)a.wy[m];
-               },
-               own, k, ux, uy, ax, ay);
+    stencil(g, a.c, a.h, tile, s_u, s_f,
+            [&](int m, double& vx, double& vy) {
+              vx = (double)a.wx[m];
+              vy = (double)a.wy[m];
+            },
+            own, k, ux, uy, ax, ay);
     double v[1] = {0.0};
     if (own) {
       const double h = (double)a.h[k];
@@ -196,129 +198,111 @@ __global__ void __launch_bounds__(PT) k_cg1_init(CgG<V> a) {
         dgy = heff + ((double)a.c[k + g.w] + (double)a.c[k - g.w]) * 0.25 / (g.dy * g.dy);
       }
       const double bx = (double)a.bx[k], by = (double)a.by[k];
-      a.r[0][0][k] = (V)(bx - ax);
-      a.r[0][1][k] = (V)(by - ay);
-      a.dix[k] = (V)(1.0 / dgx);
-      a.diy[k] = (V)(1.0 / dgy);
-      a.px[k] = (V)0;
-      a.py[k] = (V)0;
-      a.s[1][0][k] = (V)0;
-      a.s[1][1][k] = (V)0;
+      a.r[0][k] = f2(bx - ax, by - ay);
+      a.di[k] = f2(1.0 / dgx, 1.0 / dgy);
+      a.p[k] = make_float2(0.f, 0.f);
+      a.s[1][k] = make_float2(0.f, 0.f);
       v[0] = bx * bx + by * by;
     }
-    add_round<1>(v, sums, r, np, acc);
+    add_rows<1>(v, sums, r, acc);
   }
   if (threadIdx.x == 0) a.lanes_b[blockIdx.x] = acc[0];
 }
 
 // u_0 = D^-1 r_0, q_0 = A u_0; lanes of gamma_0 = r.u, delta_0 = q.u, r.r (slot 0).
-template <typename V>
-__global__ void __launch_bounds__(PT) k_cg1_start(CgG<V> a) {
+__global__ void __launch_bounds__(PT) k_cg1_start(CgG a) {
   const Geo& g = a.g;
-  const int np = g.ntx * g.nty;
-  const int q = threadIdx.x / NT;
-  __shared__ double s_ux[PG][S2Y][S2X], s_uy[PG][S2Y][S2X], s_f[PG][S1Y][S1X];
-  __shared__ double sums[2][PG][3][TY];
+  const int nt = tiles_all(g);
+  __shared__ double s_u[2][S2Y][S2X], s_f[S1Y][S1X];
+  __shared__ double sums[2][BY][3];
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int r = 0; (int)blockIdx.x + LANES * PG * r < np; ++r) {
-    const int tile0 = blockIdx.x + LANES * (PG * r + q);
-    const int tile = tile0 < np ? tile0 : -1;
+  for (int r = 0; (int)blockIdx.x + LANES * r < nt; ++r) {
+    const int tile = blockIdx.x + LANES * r;
     bool own;
     int k;
     double ux, uy, ax, ay;
-    stencil<V>(g, a.c, a.h, tile, s_ux, s_uy, s_f,
-               [&](int m, double& vx, double& vy) {
-                 vx = (double)a.r[0][0][m] * (double)a.dix[m];
-                 vy = (double)a.r[0][1][m] * (double)a.diy[m];
-               },
-               own, k, ux, uy, ax, ay);
+    stencil(g, a.c, a.h, tile, s_u, s_f,
+            [&](int m, double& vx, double& vy) {
+              const float2 rr = a.r[0][m], d = a.di[m];
+              vx = (double)rr.x * (double)d.x;
+              vy = (double)rr.y * (double)d.y;
+            },
+            own, k, ux, uy, ax, ay);
     double v[3] = {0.0, 0.0, 0.0};
     if (own) {
-      const double qx = (double)(V)ax, qy = (double)(V)ay;
-      a.q[0][0][k] = (V)qx;
-      a.q[0][1][k] = (V)qy;
-      const double rx = (double)a.r[0][0][k], ry = (double)a.r[0][1][k];
+      const double qx = (double)(float)ax, qy = (double)(float)ay;
+      a.q[0][k] = f2(qx, qy);
+      const float2 rr = a.r[0][k];
+      const double rx = rr.x, ry = rr.y;
       v[0] = rx * ux + ry * uy;
       v[1] = qx * ux + qy * uy;
       v[2] = rx * rx + ry * ry;
     }
-    add_round<3>(v, sums, r, np, acc);
+    add_rows<3>(v, sums, r, acc);
   }
   if (threadIdx.x == 0)
     for (int m = 0; m < 3; ++m) a.lanes[m * LANES + blockIdx.x] = acc[m];
 }
 
 // Iteration i = n - 1 (see the top of the file).
-template <typename V>
-__global__ void __launch_bounds__(PT) k_cg1_iter(CgG<V> a, unsigned long long n) {
+__global__ void __launch_bounds__(PT) k_cg1_iter(CgG a, unsigned long long n) {
   const Geo& g = a.g;
-  const int np = g.ntx * g.nty;
-  const int q = threadIdx.x / NT;
+  const int nt = tiles_all(g);
   double alpha = 0.0, beta = 0.0;
-  if ((int)blockIdx.x < np && !start<V>(a, n, alpha, beta)) return;
+  if ((int)blockIdx.x < nt && !start(a, n, alpha, beta)) return;
   const unsigned long long i = n - 1;
   const int cur = (int)(i & 1), nxt = cur ^ 1;
-  const V *rx0 = a.r[cur][0], *ry0 = a.r[cur][1], *qx0 = a.q[cur][0], *qy0 = a.q[cur][1];
-  const V *sx0 = a.s[nxt][0], *sy0 = a.s[nxt][1];  // s_{i-1}: written by K_{n-1} into nxt
+  const float2 *r0 = a.r[cur], *q0 = a.q[cur];
+  const float2* s0 = a.s[nxt];  // s_{i-1}: written by K_{n-1} into nxt
   const double na = -alpha;
-  __shared__ double s_ux[PG][S2Y][S2X], s_uy[PG][S2Y][S2X], s_f[PG][S1Y][S1X];
-  __shared__ double sums[2][PG][3][TY];
+  __shared__ double s_u[2][S2Y][S2X], s_f[S1Y][S1X];
+  __shared__ double sums[2][BY][3];
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int r = 0; (int)blockIdx.x + LANES * PG * r < np; ++r) {
-    const int tile0 = blockIdx.x + LANES * (PG * r + q);
-    const int tile = tile0 < np ? tile0 : -1;
+  for (int r = 0; (int)blockIdx.x + LANES * r < nt; ++r) {
+    const int tile = blockIdx.x + LANES * r;
     bool own;
     int k;
     double ux, uy, ax, ay;
     // u_{i+1} = D^-1 (r_i - alpha (q_i + beta s_{i-1})) on the tile + 2
-    stencil<V>(g, a.c, a.h, tile, s_ux, s_uy, s_f,
-               [&](int m, double& vx, double& vy) {
-                 const double sx = (double)(V)((double)qx0[m] + beta * (double)sx0[m]);
-                 const double sy = (double)(V)((double)qy0[m] + beta * (double)sy0[m]);
-                 vx = (double)(V)((double)rx0[m] + na * sx) * (double)a.dix[m];
-                 vy = (double)(V)((double)ry0[m] + na * sy) * (double)a.diy[m];
-               },
-               own, k, ux, uy, ax, ay);
+    stencil(g, a.c, a.h, tile, s_u, s_f,
+            [&](int m, double& vx, double& vy) {
+              const float2 qq = q0[m], ss = s0[m], rr = r0[m], d = a.di[m];
+              const double sx = (double)(float)((double)qq.x + beta * (double)ss.x);
+              const double sy = (double)(float)((double)qq.y + beta * (double)ss.y);
+              vx = (double)(float)((double)rr.x + na * sx) * (double)d.x;
+              vy = (double)(float)((double)rr.y + na * sy) * (double)d.y;
+            },
+            own, k, ux, uy, ax, ay);
     double v[3] = {0.0, 0.0, 0.0};
     if (own) {
-      const double dix = (double)a.dix[k], diy = (double)a.diy[k];
-      const double rx = (double)rx0[k], ry = (double)ry0[k];
-      const double sx = (double)(V)((double)qx0[k] + beta * (double)sx0[k]);
-      const double sy = (double)(V)((double)qy0[k] + beta * (double)sy0[k]);
-      const double px = (double)(V)(rx * dix + beta * (double)a.px[k]);
-      const double py = (double)(V)(ry * diy + beta * (double)a.py[k]);
-      a.px[k] = (V)px;
-      a.py[k] = (V)py;
-      a.wx[k] = (V)((double)a.wx[k] + alpha * px);
-      a.wy[k] = (V)((double)a.wy[k] + alpha * py);
-      a.s[cur][0][k] = (V)sx;
-      a.s[cur][1][k] = (V)sy;
-      const double nrx = (double)(V)(rx + na * sx), nry = (double)(V)(ry + na * sy);
-      a.r[nxt][0][k] = (V)nrx;
-      a.r[nxt][1][k] = (V)nry;
-      const double qx = (double)(V)ax, qy = (double)(V)ay;
-      a.q[nxt][0][k] = (V)qx;
-      a.q[nxt][1][k] = (V)qy;
+      const float2 qq = q0[k], ss = s0[k], rr = r0[k], d = a.di[k], pp = a.p[k];
+      const double sx = (double)(float)((double)qq.x + beta * (double)ss.x);
+      const double sy = (double)(float)((double)qq.y + beta * (double)ss.y);
+      const double px = (double)(float)((double)rr.x * (double)d.x + beta * (double)pp.x);
+      const double py = (double)(float)((double)rr.y * (double)d.y + beta * (double)pp.y);
+      a.p[k] = f2(px, py);
+      a.wx[k] = (float)((double)a.wx[k] + alpha * px);
+      a.wy[k] = (float)((double)a.wy[k] + alpha * py);
+      a.s[cur][k] = f2(sx, sy);
+      const double nrx = (double)(float)((double)rr.x + na * sx);
+      const double nry = (double)(float)((double)rr.y + na * sy);
+      a.r[nxt][k] = f2(nrx, nry);
+      const double qx = (double)(float)ax, qy = (double)(float)ay;
+      a.q[nxt][k] = f2(qx, qy);
       v[0] = nrx * ux + nry * uy;
       v[1] = qx * ux + qy * uy;
       v[2] = nrx * nrx + nry * nry;
     }
-    add_round<3>(v, sums, r, np, acc);
+    add_rows<3>(v, sums, r, acc);
   }
   if (threadIdx.x == 0)
     for (int m = 0; m < 3; ++m) a.lanes[3 * LANES * (n & 1) + m * LANES + blockIdx.x] = acc[m];
 }
 
 // K_n's prologue alone, one wave: whether the solve is over before iteration n - 1.
-template <typename V>
-__global__ void __launch_bounds__(64) k_cg1_check(CgG<V> a, unsigned long long n) {
+__global__ void __launch_bounds__(64) k_cg1_check(CgG a, unsigned long long n) {
   double alpha, beta;
-  start<V>(a, n, alpha, beta);
+  start(a, n, alpha, beta);
 }
-
-template __global__ void k_cg1_init<float>(CgG<float>);
-template __global__ void k_cg1_start<float>(CgG<float>);
-template __global__ void k_cg1_iter<float>(CgG<float>, unsigned long long);
-template __global__ void k_cg1_check<float>(CgG<float>, unsigned long long);
 
 }  // namespace wc
