@@ -971,6 +971,11 @@ int stable_dt32(wc_solver* s, double* dt) {
 }  // namespace
 
 // ================================================================== the C ABI
+namespace wc {
+// The error setter for the other sources of the library (wc_unet.hip).
+int api_fail(int code, const char* msg) { return fail(code, msg); }
+}  // namespace wc
+
 extern "C" {
 
 const char* wc_last_error(void) { return g_err.c_str(); }
@@ -1270,11 +1275,6 @@ int wc_step(wc_solver* s, double dt) {
   if (!s->prof) HIP_OK(hipStreamSynchronize(s->stream));
   return WC_OK;
 }
-
-namespace wc {
-// The error setter for the other sources of the library (wc_unet.hip).
-int api_fail(int code, const char* msg) { return fail(code, msg); }
-}  // namespace wc
 
 // ---- relaxation zones (forcing.rs:227-284) ---------------------------------------------------
 
