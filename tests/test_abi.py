@@ -54,3 +54,32 @@ def test_free_functions_work_without_a_gpu():
     with pytest.raises(ValueError):
         gpu.linear_wave(2 * 3.141592653589793 / 1.0, 5.0, True)
     assert gpu.manning_factor(1.5, 0.8, 0.2, 0.0) == 1.0
+
+
+UNET_HEADER = os.path.join(ROOT, "include", "wavecore_unet.h")
+
+
+def declared_unet() -> list[str]:
+    text = open(UNET_HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(wcu_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.mark.skipif(not os.path.exists(gpu.LIB_PATH), reason="library not built")
+def test_library_exports_the_network_api():
+    from wavesim_amd import unet
+
+    out = subprocess.run(["nm", "-D", "--defined-only", gpu.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (wcu_[a-z0-9_]+)$", out, flags=re.M))
+    names = declared_unet()
+    assert "wcu_net_forward" in names and "wcu_net_backward" in names
+    assert [n for n in names if n not in exported] == []
+    assert sorted(unet.SIGNATURES) == names
+    # host-only entry points: the parameter layout
+    assert unet.param_count() == sum(d["weights"] + d["biases"] for d in unet.layers())
+    from oracle import unet_oracle as uo
+
+    assert unet.param_count() == uo.param_count()
+    for d, (name, taps, ci, co, _) in zip(unet.layers(), uo.LAYERS):
+        assert (d["name"], d["taps"], d["cin"], d["cout"]) == (name, taps, ci, co)
