@@ -165,6 +165,31 @@ double wc_manning_factor(double h, double speed, double dt, double n);
  * positive; *done_steps reports how many steps ran and *last_dt the last dt. */
 int wc_run_steps(wc_solver* s, uint64_t steps, uint64_t* done_steps, double* last_dt);
 
+/* The dispersive solve's preconditioner (no counterpart in the reference, whose PCG uses the
+ * Jacobi diagonal, dispersion.rs:568-589). WC_PRECOND_NETWORK: the learned network of
+ * include/wavecore_unet.h (params: wcu_param_count() FP32 values, host memory) inside a
+ * flexible PCG with the same tolerance, falling back to Jacobi for the rest of a solve whose
+ * residual stops decreasing; used by the FP32 form (the FP64 path keeps Jacobi, to stay equal
+ * to the oracle). Features: depth / (4 dx), WC_BED_FEATURE_SCALE times the bed's centred
+ * gradient, the residual over its RMS; the output times RMS / (4 dx) is the correction.
+ * wc_solver_precond_fallbacks: solves that fell back, since the solver was made. */
+#define WC_PRECOND_JACOBI 0
+#define WC_PRECOND_NETWORK 1
+#define WC_BED_FEATURE_SCALE 10.0
+int wc_solver_set_preconditioner(wc_solver* s, int32_t kind, const float* params, uint64_t n);
+int wc_solver_precond_fallbacks(wc_solver* s, uint64_t* out);
+/* Training pairs from the solver's own solves (FP32 form, Jacobi PCG): with the option
+ * "capture" set to 1, each solve records its depth and, before its iterations 0, 2, 5 and 10
+ * (slots 0..3), the residual r_n = b - A w_n and the iterate w_n. wc_solver_capture copies the
+ * last solve's slot into host arrays of (nx+6)*(ny+6) floats (any may be NULL), the error as
+ * w* - w_n with w* the solve's solution; *valid = 0 if the solve ended before that iteration.
+ * Option "precond_stall": iterations without a new best residual before the network falls back
+ * to Jacobi (default 4, 0 = never). wc_solver_residual_history: r.r before each iteration of
+ * the last network-preconditioned solve (*n values, at most max). */
+int wc_solver_capture(wc_solver* s, int32_t slot, float* h, float* rx, float* ry, float* ex,
+                      float* ey, int32_t* valid);
+int wc_solver_residual_history(wc_solver* s, double* out, uint64_t max, uint64_t* n);
+
 /* Relaxation zones (forcing.rs:227-284): the water in the zones is pulled towards targets the
  * caller supplies at every step (one-way nesting, crates/wavesim/src/lib.rs:1894-1897, and
  * the lab's wave paddle, crates/wavecore/tests/lab.rs:253-271). A relaxation belongs to the

@@ -1,0 +1,87 @@
+"""The learned preconditioner inside the solver (flexible PCG, FP32 form) and the capture of
+training pairs, on the GPU.
+
+The method stays the judge: with any network (here an untrained one) every solve must reach the
+reference's tolerance, falling back to Jacobi when the network stalls, so the state after a
+run matches the Jacobi run's to the tolerance's effect. The trained network must cut the PCG
+iterations on a bed it was not trained on.
+"""
+
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from wavesim_amd import configs
+from wavesim_amd.grid import State
+from wavesim_amd.solver import PRECISION_FP32, Solver
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TRAINED = os.path.join(os.path.dirname(HERE), "wavesim_amd", "data", "precond_rough_bed.npy")
+
+
+def mid_run(nx, ny, seed, spinup):
+    w = configs.c1_rough_bed(nx=nx, ny=ny, seed=seed)
+    g = w.solver(Solver).with_precision(PRECISION_FP32)
+    s = w.still_water()
+    g.run_steps(s, spinup)
+    return w, [a.copy() for a in s.arrays()], s.time
+
+
+def run_from(w, arrays, t0, steps, params=None):
+    g = w.solver(Solver).with_precision(PRECISION_FP32)
+    if params is not None:
+        g.with_preconditioner(params)
+    s = State(*(a.copy() for a in arrays), t0)
+    n, _ = g.run_steps(s, steps)
+    assert n == steps
+    return g, s
+
+
+def test_untrained_network_still_reaches_the_tolerance():
+    from wavesim_amd.unet import init_params
+
+    w, arrays, t0 = mid_run(256, 128, 7, 500)
+    gj, sj = run_from(w, arrays, t0, 30)
+    gn, sn = run_from(w, arrays, t0, 30, init_params(3))
+    stj, stn = gj.dispersion_stats(), gn.dispersion_stats()
+    print("jacobi", stj, "network", stn, "fallbacks", gn.precond_fallbacks(), flush=True)
+    assert stn.unconverged == 0 and stj.unconverged == 0
+    hj, hn = sj.arrays()[0], sn.arrays()[0]
+    assert np.max(np.abs(hj - hn)) < 1e-4
+
+
+def test_capture_records_the_solves_pairs():
+    w, arrays, t0 = mid_run(128, 64, 3, 300)
+    g = w.solver(Solver).with_precision(PRECISION_FP32).set_option("capture", 1)
+    s = State(*(a.copy() for a in arrays), t0)
+    g.run_steps(s, 3)
+    cap = g.capture(0)
+    assert cap is not None
+    inner = w.grid.inner
+    for k in ("h", "rx", "ry", "ex", "ey"):
+        assert np.all(np.isfinite(cap[k]))
+    h = inner(cap["h"])
+    wet = h >= 0.05
+    assert np.max(np.abs(inner(cap["rx"]))) > 0
+    assert np.all(inner(cap["ex"])[~wet] == 0) and np.all(inner(cap["ey"])[~wet] == 0)
+    assert np.allclose(h, inner(s.arrays()[0]), atol=1.0)  # the stage depth is near the state
+
+
+@pytest.mark.skipif(not os.path.exists(TRAINED), reason="no trained parameters")
+def test_trained_network_cuts_the_iterations_on_a_held_out_bed():
+    params = np.load(TRAINED)
+    w, arrays, t0 = mid_run(512, 256, 9001, 900)
+    gj, sj = run_from(w, arrays, t0, 20)
+    gn, sn = run_from(w, arrays, t0, 20, params)
+    stj, stn = gj.dispersion_stats(), gn.dispersion_stats()
+    ij = stj.iterations / stj.solves
+    i_n = stn.iterations / stn.solves
+    print(f"iterations per solve: jacobi {ij:.1f}, network {i_n:.1f}, fallbacks "
+          f"{gn.precond_fallbacks()}", flush=True)
+    assert stn.unconverged == 0
+    assert i_n < 0.8 * ij

@@ -17,7 +17,13 @@
 #include <vector>
 
 #include "../../include/wavecore_hip.h"
+#include "../../include/wavecore_unet.h"
 #include "wc_kernels.h"
+
+namespace wc {
+int unet_body(wcu_net* n, hipStream_t st, const int* skip);
+void unet_views(wcu_net* n, void** x0, const void** y0, int* w0, int* h0, const float** wo);
+}  // namespace wc
 
 using namespace wc;
 
@@ -58,12 +64,13 @@ enum Family {
   F_CG_REDUCE,
   F_RELAX,
   F_CG_ITER,
+  F_NET,
   F_COUNT
 };
 const char* kFamilyNames[F_COUNT] = {"fill_state", "switch",    "flux",      "finish_euler",
                                      "sgn_setup",  "cg_init",   "cg_matvec", "cg_update",
                                      "cg_pupdate", "cg_stats",  "sgn_finish", "reduce",
-                                     "cg_reduce",  "relax",     "cg_iter"};
+                                     "cg_reduce",  "relax",     "cg_iter",   "net"};
 
 struct HostSponge {
   uint64_t width;
@@ -145,6 +152,18 @@ struct wc_solver {
   // the one-kernel PCG's own vectors (wc_pcg1.hip), interleaved (x, y)
   float2 *g_r[2] = {nullptr, nullptr}, *g_q[2] = {nullptr, nullptr},
          *g_s[2] = {nullptr, nullptr}, *g_p = nullptr, *g_di = nullptr;
+
+  // the learned preconditioner (wc_fpcg.hip) and the capture of solves for its training
+  int precond = WC_PRECOND_JACOBI;
+  wcu_net* net = nullptr;
+  float *bgx = nullptr, *bgy = nullptr;
+  int capture = 0;
+  static constexpr int CAP_SLOTS = 4;  // the solve's state before iterations 0, 2, 5, 10
+  float *cap_wx[CAP_SLOTS] = {}, *cap_wy[CAP_SLOTS] = {}, *cap_h = nullptr;
+  float2* cap_r[CAP_SLOTS] = {};
+  unsigned long long cap_last_it = 0;
+  int precond_stall = 4;
+  double* history = nullptr;  // r.r per iteration of the last network-preconditioned solve
 
   // implementation options (wc_solver_set_option)
   int pcg_kernels = 0;  // 0: the default of each precision (FP64 3, FP32 4)
@@ -787,15 +806,106 @@ int pcg1_32(wc_solver* s, int stage) {
   a.tolerance = s->disp.tolerance;
   a.max_iterations = s->disp.max_iterations;
   const dim3 grid(256), block(1024);
+  const size_t fbytes = s->cells * sizeof(float);
+  static constexpr unsigned long long cap_it[wc_solver::CAP_SLOTS] = {0, 2, 5, 10};
+  // training pairs: the depth, and (r_n, w_n) before iteration n of the solve
+  auto capture_at = [&](int slot, const float2* r) -> int {
+    HIP_OK(hipMemcpyAsync(s->cap_wx[slot], s->f_wx, fbytes, hipMemcpyDeviceToDevice, s->stream));
+    HIP_OK(hipMemcpyAsync(s->cap_wy[slot], s->f_wy, fbytes, hipMemcpyDeviceToDevice, s->stream));
+    HIP_OK(hipMemcpyAsync(s->cap_r[slot], r, 2 * fbytes, hipMemcpyDeviceToDevice, s->stream));
+    return WC_OK;
+  };
+  if (s->capture) HIP_OK(hipMemcpyAsync(s->cap_h, sa.h, fbytes, hipMemcpyDeviceToDevice, s->stream));
   TRY(launch(s, F_CG_INIT, k_cg1_init, grid, block, a));
+  if (s->capture) TRY(capture_at(0, s->g_r[0]));
   TRY(launch(s, F_CG_INIT, k_cg1_start, grid, block, a));
   const unsigned long long maxit = s->disp.max_iterations;
   unsigned long long n = 1;
   unsigned long long batch = std::max<unsigned long long>(s->prev_it[stage], 1);
   while (true) {
     const unsigned long long stop = std::min(n + batch, maxit + 1);
-    for (; n < stop; ++n) TRY(launch(s, F_CG_ITER, k_cg1_iter, grid, block, a, n));
+    for (; n < stop; ++n) {
+      TRY(launch(s, F_CG_ITER, k_cg1_iter, grid, block, a, n));
+      if (s->capture)  // K_n leaves r_n in r[n & 1] and w_n in w
+        for (int slot = 1; slot < wc_solver::CAP_SLOTS; ++slot)
+          if (n == cap_it[slot]) TRY(capture_at(slot, s->g_r[n & 1]));
+    }
     TRY(launch(s, F_CG_REDUCE, k_cg1_check, dim3(1), dim3(64), a, n));
+    HIP_OK(hipMemcpyAsync(s->sc_host, s->sc, sizeof(CgScalars), hipMemcpyDeviceToHost,
+                          s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    if (s->sc_host->done || n > maxit) break;
+    batch = s->pcg_batch;
+  }
+  s->prev_it[stage] = s->sc_host->it;
+  s->cap_last_it = s->sc_host->it;
+  TRY(launch(s, F_CG_STATS, k_cg_stats, dim3(1), dim3(64), s->sc));
+  return WC_OK;
+}
+
+// Flexible PCG with the network (wc_fpcg.hip), FP32 form: batches of iterations, each the input
+// kernel, the network, the output kernel, the matvec and the update; a one-wave check ends a
+// batch.
+int pcg_net32(wc_solver* s, int stage) {
+  const Sgn32 sa = sgn32_args(s, stage);
+  FcgArgs a{};
+  a.g = s->geo;
+  a.h = sa.h;
+  a.c = s->f_c;
+  a.bx = s->f_bx;
+  a.by = s->f_by;
+  a.wx = s->f_wx;
+  a.wy = s->f_wy;
+  a.r = s->g_r[0];
+  a.q = s->g_q[0];
+  a.z = s->g_s[0];
+  a.di = s->g_di;
+  a.p[0] = s->g_p;
+  a.p[1] = s->g_r[1];
+  a.bgx = s->bgx;
+  a.bgy = s->bgy;
+  void* x0;
+  const void* y0;
+  int w0, h0;
+  const float* wo;
+  unet_views(s->net, &x0, &y0, &w0, &h0, &wo);
+  a.x0 = (__bf16*)x0;
+  a.y0 = (const __bf16*)y0;
+  a.nw = w0;
+  a.wo = wo;
+  a.hs = 1.0 / (4.0 * s->geo.dx);
+  a.lanes = s->lanes;
+  a.sc = s->sc;
+  a.tolerance = s->disp.tolerance;
+  a.max_iterations = s->disp.max_iterations;
+  a.stall_limit = s->precond_stall;
+  a.history = s->history;
+  a.history_len = 1024;
+  const dim3 grid(256), block(1024);
+  TRY(launch(s, F_CG_INIT, k_fcg_init, grid, block, a));
+  const unsigned long long maxit = s->disp.max_iterations;
+  unsigned long long n = 1;
+  unsigned long long batch = std::max<unsigned long long>(s->prev_it[stage], 1);
+  while (true) {
+    const unsigned long long stop = std::min(n + batch, maxit + 1);
+    for (; n < stop; ++n) {
+      TRY(launch(s, F_NET, k_fcg_in, grid, block, a, n));
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if (s->prof) {
+        e0 = take_event(s);
+        e1 = take_event(s);
+        if (e0) HIP_OK(hipEventRecord(e0, s->stream));
+      }
+      TRY(unet_body(s->net, s->stream, &s->sc->net_off));
+      if (s->prof && e0 && e1) {
+        HIP_OK(hipEventRecord(e1, s->stream));
+        s->ev_pending.push_back({F_NET, {e0, e1}});
+      }
+      TRY(launch(s, F_NET, k_fcg_out, grid, block, a, n));
+      TRY(launch(s, F_CG_MATVEC, k_fcg_mv, grid, block, a, n));
+      TRY(launch(s, F_CG_UPDATE, k_fcg_up, grid, block, a, n));
+    }
+    TRY(launch(s, F_CG_REDUCE, k_fcg_check, dim3(1), dim3(64), a, n));
     HIP_OK(hipMemcpyAsync(s->sc_host, s->sc, sizeof(CgScalars), hipMemcpyDeviceToHost,
                           s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
@@ -808,6 +918,7 @@ int pcg1_32(wc_solver* s, int stage) {
 }
 
 int pcg32(wc_solver* s, int stage) {
+  if (s->precond == WC_PRECOND_NETWORK) return pcg_net32(s, stage);
   if (s->pcg_kernels == 0 || s->pcg_kernels == 4) return pcg1_32(s, stage);
   const Sgn32 a = sgn32_args(s, stage);
   if (s->pcg_kernels == 3) {
@@ -1055,6 +1166,7 @@ void wc_solver_free(wc_solver* s) {
   if (!s) return;
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->net) wcu_net_free(s->net);
   for (void* p : s->owned) (void)hipFree(p);
   for (int q = 0; q < PRM_SLOTS; ++q) {
     if (s->prm_host[q]) (void)hipHostFree(s->prm_host[q]);
@@ -1198,6 +1310,25 @@ int wc_solver_set_option(wc_solver* s, const char* key, int64_t value) {
     s->pcg_kernels = (int)value;
     return WC_OK;
   }
+  if (strcmp(key, "capture") == 0) {
+    if (value != 0 && value != 1) return fail(WC_EINVAL, "capture must be 0 or 1");
+    if (value && !s->cap_h) {
+      TRY(set_device(s));
+      TRY(dalloc(s, &s->cap_h, s->cells));
+      for (int slot = 0; slot < wc_solver::CAP_SLOTS; ++slot) {
+        TRY(dalloc(s, &s->cap_wx[slot], s->cells));
+        TRY(dalloc(s, &s->cap_wy[slot], s->cells));
+        TRY(dalloc(s, &s->cap_r[slot], s->cells));
+      }
+    }
+    s->capture = (int)value;
+    return WC_OK;
+  }
+  if (strcmp(key, "precond_stall") == 0) {
+    if (value < 0 || value > 1000) return fail(WC_EINVAL, "precond_stall must be in [0, 1000]");
+    s->precond_stall = (int)value;
+    return WC_OK;
+  }
   if (strcmp(key, "flux_kernel") == 0) {
     if (value != 1 && value != 2) return fail(WC_EINVAL, "flux_kernel must be 1 or 2");
     s->flux_kernel = (int)value;
@@ -1273,6 +1404,85 @@ int wc_step(wc_solver* s, double dt) {
   TRY(set_device(s));
   TRY(s->precision == WC_PRECISION_FP32 ? do_step32(s, dt) : do_step(s, dt));
   if (!s->prof) HIP_OK(hipStreamSynchronize(s->stream));
+  return WC_OK;
+}
+
+// ---- the learned preconditioner ----------------------------------------------------------------
+
+int wc_solver_set_preconditioner(wc_solver* s, int32_t kind, const float* params, uint64_t n) {
+  if (!s) return fail(WC_EINVAL, "null solver");
+  if (kind == WC_PRECOND_JACOBI) {
+    s->precond = kind;
+    return WC_OK;
+  }
+  if (kind != WC_PRECOND_NETWORK) return fail(WC_EINVAL, "unknown preconditioner");
+  if (!params || n != wcu_param_count())
+    return fail(WC_EINVAL, "the network needs wcu_param_count() parameters");
+  TRY(set_device(s));
+  if (!s->net) {
+    int rc = wcu_net_new((uint64_t)s->geo.nx, (uint64_t)s->geo.ny, &s->net);
+    if (rc != WC_OK) return rc;
+    TRY(dalloc(s, &s->bgx, s->cells));
+    TRY(dalloc(s, &s->bgy, s->cells));
+    TRY(dalloc(s, &s->history, 1024));
+  }
+  float* d = nullptr;
+  HIP_OK(hipMalloc(&d, n * sizeof(float)));
+  HIP_OK(hipMemcpyAsync(d, params, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
+  int rc = wcu_net_set_params(s->net, d, s->stream);
+  HIP_OK(hipStreamSynchronize(s->stream));
+  HIP_OK(hipFree(d));
+  if (rc != WC_OK) return rc;
+  TRY(launch(s, F_FILL, k_bed_features, over((long long)s->cells), dim3(NT), s->geo,
+             (const double*)s->b, WC_BED_FEATURE_SCALE, s->bgx, s->bgy));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  s->precond = kind;
+  return WC_OK;
+}
+
+int wc_solver_precond_fallbacks(wc_solver* s, uint64_t* out) {
+  if (!s || !out) return fail(WC_EINVAL, "null argument");
+  TRY(set_device(s));
+  HIP_OK(hipMemcpy(s->sc_host, s->sc, sizeof(CgScalars), hipMemcpyDeviceToHost));
+  *out = s->sc_host->fallbacks;
+  return WC_OK;
+}
+
+int wc_solver_capture(wc_solver* s, int32_t slot, float* h, float* rx, float* ry, float* ex,
+                      float* ey, int32_t* valid) {
+  if (!s || !s->cap_h) return fail(WC_EINVAL, "capture is not on");
+  if (slot < 0 || slot >= wc_solver::CAP_SLOTS) return fail(WC_EINVAL, "capture slot out of range");
+  static constexpr unsigned long long cap_it[wc_solver::CAP_SLOTS] = {0, 2, 5, 10};
+  TRY(set_device(s));
+  const size_t n = s->cells;
+  std::vector<float> w0x(n), w0y(n), wx(n), wy(n), r(2 * n);
+  HIP_OK(hipStreamSynchronize(s->stream));
+  if (h) HIP_OK(hipMemcpy(h, s->cap_h, n * sizeof(float), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(r.data(), s->cap_r[slot], 2 * n * sizeof(float), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(w0x.data(), s->cap_wx[slot], n * sizeof(float), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(w0y.data(), s->cap_wy[slot], n * sizeof(float), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(wx.data(), s->f_wx, n * sizeof(float), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(wy.data(), s->f_wy, n * sizeof(float), hipMemcpyDeviceToHost));
+  for (size_t k = 0; k < n; ++k) {
+    if (rx) rx[k] = r[2 * k];
+    if (ry) ry[k] = r[2 * k + 1];
+    if (ex) ex[k] = wx[k] - w0x[k];
+    if (ey) ey[k] = wy[k] - w0y[k];
+  }
+  if (valid) *valid = s->cap_last_it > cap_it[slot] ? 1 : 0;
+  return WC_OK;
+}
+
+int wc_solver_residual_history(wc_solver* s, double* out, uint64_t max, uint64_t* n) {
+  if (!s || !n) return fail(WC_EINVAL, "null argument");
+  *n = 0;
+  if (!s->history) return WC_OK;
+  TRY(set_device(s));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  HIP_OK(hipMemcpy(s->sc_host, s->sc, sizeof(CgScalars), hipMemcpyDeviceToHost));
+  const uint64_t count = std::min<uint64_t>(std::min<uint64_t>(s->sc_host->it + 1, 1024), max);
+  if (out && count) HIP_OK(hipMemcpy(out, s->history, count * sizeof(double), hipMemcpyDeviceToHost));
+  *n = count;
   return WC_OK;
 }
 

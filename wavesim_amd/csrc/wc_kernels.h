@@ -28,6 +28,10 @@ struct CgScalars {
   double rz, target, alpha, beta;
   double rzs[2];  // r.z of the previous iteration, by iteration parity (wc_pcg.hip)
   double alphas[2];  // alpha of the previous iteration, by parity (wc_pcg1.hip)
+  // the flexible PCG with the network (wc_fpcg.hip)
+  double rr_prev, sigma;
+  int stall, fallback, restart, net_off;
+  unsigned long long fallbacks;
   unsigned long long it;
   int converged, done;
   unsigned long long solves, iterations, unconverged;
@@ -113,6 +117,37 @@ struct CgG {
   double tolerance;
   unsigned long long max_iterations;
 };
+// The flexible PCG with the learned preconditioner (wc_fpcg.hip), FP32 form. Vectors are
+// interleaved (x, y) pairs in grid layout; lanes: [0] b.b, [1] r.r, [2] r.z, [3] r.z_old,
+// [4] p.Ap, 256 each.
+struct FcgArgs {
+  Geo g;
+  const float *h, *c, *bx, *by;
+  float *wx, *wy;
+  float2 *r, *q, *z, *di;
+  float2* p[2];
+  const float *bgx, *bgy;  // bed-gradient features
+  __bf16* x0;              // network input, NHWC 8 channels, one-pixel border, width nw
+  const __bf16* y0;        // network's last hidden layer, 16 channels
+  int nw;
+  const float* wo;         // output layer: 16 x 2 weights, 2 biases
+  double hs;               // 1 / (4 dx): depth feature and output scale
+  double* lanes;
+  CgScalars* sc;
+  double tolerance;
+  unsigned long long max_iterations;
+  int stall_limit;            // 0: never fall back
+  double* history;            // r.r before each iteration of the solve (may be null)
+  unsigned long long history_len;
+};
+__global__ void k_fcg_init(FcgArgs a);
+__global__ void k_fcg_in(FcgArgs a, unsigned long long n);
+__global__ void k_fcg_out(FcgArgs a, unsigned long long n);
+__global__ void k_fcg_mv(FcgArgs a, unsigned long long n);
+__global__ void k_fcg_up(FcgArgs a, unsigned long long n);
+__global__ void k_fcg_check(FcgArgs a, unsigned long long n);
+__global__ void k_bed_features(Geo g, const double* b, double scale, float* bgx, float* bgy);
+
 __global__ void k_cg1_init(CgG a);
 __global__ void k_cg1_start(CgG a);
 __global__ void k_cg1_iter(CgG a, unsigned long long n);

@@ -25,107 +25,13 @@
 // b+256, ...; each adds its tiles' sums (rows in order) into one lane, and every wave of the
 // next kernel reduces the 256 lanes itself, so no separate reduction kernel runs.
 
-#include "wc_kernels.h"
+#include "wc_tile16.h"
 
 namespace wc {
 
 namespace {
 
-constexpr int LANES = 256;
-constexpr int BX = 64, BY = 16;  // tile
-constexpr int PT = BX * BY;      // threads per block, a cell each
-constexpr int S2X = BX + 4, S2Y = BY + 4;
-constexpr int S1X = BX + 2, S1Y = BY + 2;
-constexpr int H2 = S2X * S2Y, H1 = S1X * S1Y;
-
-template <int N>
-__device__ __forceinline__ void lane_sums(const double* acc, double (&out)[N]) {
-  const int lane = threadIdx.x & 63;
-  double v[N][4];
-#pragma unroll
-  for (int n = 0; n < N; ++n)
-#pragma unroll
-    for (int w = 0; w < 4; ++w) v[n][w] = acc[n * LANES + 64 * w + lane];
-#pragma unroll
-  for (int n = 0; n < N; ++n)
-    out[n] = (wave_sum(v[n][0]) + wave_sum(v[n][1])) + (wave_sum(v[n][2]) + wave_sum(v[n][3]));
-}
-
-__device__ __forceinline__ int tiles_x(const Geo& g) { return (g.nx + BX - 1) / BX; }
-__device__ __forceinline__ int tiles_all(const Geo& g) {
-  return tiles_x(g) * ((g.ny + BY - 1) / BY);
-}
-
-// The round's row sums (one wave per tile row), then thread 0 adds the rows in order.
-template <int N>
-__device__ __forceinline__ void add_rows(double (&v)[N], double (*sums)[BY][N], int r,
-                                         double (&acc)[N]) {
-  const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
-#pragma unroll
-  for (int n = 0; n < N; ++n) {
-    const double t = wave_sum(v[n]);
-    if (lane == 0) sums[r & 1][row][n] = t;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0)
-    for (int y = 0; y < BY; ++y)
-      for (int n = 0; n < N; ++n) acc[n] = acc[n] + sums[r & 1][y][n];
-}
-
-// Stages the vector u (given per halo cell by fn(m, vx, vy), m the mapped interior index; the
-// wall flips are applied here) on the tile + 2 and c (grad . u) on the tile + 1; returns A u
-// at the thread's cell (and u there).
-template <class Fn>
-__device__ __forceinline__ void stencil(const Geo& g, const float* c, const float* hh, int tile,
-                                        double (*s_u)[S2Y][S2X], double (*s_f)[S1X], Fn fn,
-                                        bool& own, int& k, double& ux, double& uy, double& ax,
-                                        double& ay) {
-  const int t0 = threadIdx.x;
-  const int tx = tiles_x(g);
-  const int i0 = GH + (tile % tx) * BX, j0 = GH + (tile / tx) * BY;
-  const double ix = 0.5 / g.dx, iy = 0.5 / g.dy;
-  for (int t = t0; t < H2; t += PT) {
-    const int lx = t % S2X, ly = t / S2X;
-    const int i = i0 - 2 + lx, j = j0 - 2 + ly;
-    double vx = 0.0, vy = 0.0;
-    if (i <= g.nx + GH + 1 && j <= g.ny + GH + 1) {
-      const int m = map_y(j, g.ny, g.periodic) * g.w + map_x(i, g.nx);
-      fn(m, vx, vy);
-      if (is_ghost_x(i, g.nx)) vx = -vx;
-      if (!g.periodic && is_ghost_y(j, g.ny)) vy = -vy;
-    }
-    s_u[0][ly][lx] = vx;
-    s_u[1][ly][lx] = vy;
-  }
-  __syncthreads();
-  for (int t = t0; t < H1; t += PT) {
-    const int lx = t % S1X, ly = t / S1X;
-    const int i = i0 - 1 + lx, j = j0 - 1 + ly;
-    double f = 0.0;
-    if (i <= g.nx + GH && j <= g.ny + GH) {
-      const int x = lx + 1, y = ly + 1;
-      f = (double)c[j * g.w + i] * ((s_u[0][y][x + 1] - s_u[0][y][x - 1]) * ix +
-                                    (s_u[1][y + 1][x] - s_u[1][y - 1][x]) * iy);
-    }
-    s_f[ly][lx] = f;
-  }
-  __syncthreads();
-  const int lx = t0 & 63, ly = t0 >> 6;
-  const int i = i0 + lx, j = j0 + ly;
-  own = i < g.nx + GH && j < g.ny + GH;
-  k = j * g.w + i;
-  ux = s_u[0][ly + 2][lx + 2];
-  uy = s_u[1][ly + 2][lx + 2];
-  ax = ay = 0.0;
-  if (own) {
-    const double h = (double)hh[k];
-    const double mask = h >= WC_H_DISPERSION_MIN ? 1.0 : 0.0;
-    const double heff = wc_max(h, WC_H_DRY);
-    const int x1 = lx + 1, y1 = ly + 1;
-    ax = mask * (heff * ux - (s_f[y1][x1 + 1] - s_f[y1][x1 - 1]) * ix);
-    ay = mask * (heff * uy - (s_f[y1 + 1][x1] - s_f[y1 - 1][x1]) * iy);
-  }
-}
+using namespace t16;
 
 // The scalars of iteration i = n - 1, from K_{n-1}'s lanes (or the start's); false when the
 // solve is over. Every wave computes the same numbers; block 0 records them.

@@ -105,12 +105,14 @@ struct ConvArgs {
   const float* bias;  // may be null
   bf16* out;
   const bf16* mask;   // may be null: the output is kept only where mask > 0 (ReLU backward)
+  const int* skip;    // may be null: the kernel does nothing while *skip != 0
 };
 
 template <int TAPS, int CIN, int COUT, int PB, bool RELU>
 __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
   constexpr int NB = COUT / 16;
   constexpr int STEPS = (CIN >= 32) ? TAPS * (CIN / 32) : (TAPS + (32 / CIN) - 1) / (32 / CIN);
+  if (a.skip && *a.skip) return;
   const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
   const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int bpr = a.W / (16 * PB);
@@ -205,7 +207,8 @@ __device__ inline size_t pix(int y, int x, int W, int C) {
 }
 
 // 2x2 average: out (H/2 x W/2) from in (H x W).
-__global__ void k_pool(const bf16* in, int H, int W, int C, bf16* out) {
+__global__ void k_pool(const bf16* in, int H, int W, int C, bf16* out, const int* skip) {
+  if (skip && *skip) return;
   const int c8 = C / 8;
   const long long n = (long long)(H / 2) * (W / 2) * c8;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -223,7 +226,9 @@ __global__ void k_pool(const bf16* in, int H, int W, int C, bf16* out) {
 }
 
 // out = up2(t) + e at level size H x W (t at H/2 x W/2).
-__global__ void k_upadd(const bf16* t, const bf16* e, int H, int W, int C, bf16* out) {
+__global__ void k_upadd(const bf16* t, const bf16* e, int H, int W, int C, bf16* out,
+                        const int* skip) {
+  if (skip && *skip) return;
   const int c8 = C / 8;
   const long long n = (long long)H * W * c8;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -605,11 +610,12 @@ int nalloc(wcu_net* n, T** p, size_t count) {
   return WC_OK;
 }
 
-int conv_layer(wcu_net* n, int l, Tensor in, Tensor out, hipStream_t st) {
+int conv_layer(wcu_net* n, int l, Tensor in, Tensor out, hipStream_t st,
+               const int* skip = nullptr) {
   const LayerDef& d = LAYERS[l];
   ConvArgs a{n->act[in], n->H[d.level], n->W[d.level], n->wpk[l],
              n->params + layer_offset(l) + (size_t)d.taps * d.cin_real * d.cout, n->act[out],
-             nullptr};
+             nullptr, skip};
   return launch_conv(d.taps, d.cin, d.cout, d.relu, a, st);
 }
 
@@ -617,7 +623,7 @@ int conv_layer(wcu_net* n, int l, Tensor in, Tensor out, hipStream_t st) {
 // kept where mask > 0.
 int conv_back(wcu_net* n, int l, const bf16* dy, bf16* dx, const bf16* mask, hipStream_t st) {
   const LayerDef& d = LAYERS[l];
-  ConvArgs a{dy, n->H[d.level], n->W[d.level], n->wpkT[l], nullptr, dx, mask};
+  ConvArgs a{dy, n->H[d.level], n->W[d.level], n->wpkT[l], nullptr, dx, mask, nullptr};
   return launch_conv(d.taps, d.cout, d.cin, 0, a, st);
 }
 
@@ -644,20 +650,23 @@ int wgrad_layer(wcu_net* n, int l, const bf16* x, const bf16* dy, float* grads,
   return WC_OK;
 }
 
-int elementwise_pool(wcu_net* n, Tensor in, Tensor out, hipStream_t st) {
+int elementwise_pool(wcu_net* n, Tensor in, Tensor out, hipStream_t st,
+                     const int* skip = nullptr) {
   const TensorDef& d = TENSORS[in];
   const int H = n->H[d.level], W = n->W[d.level];
   hipLaunchKernelGGL(k_pool, blocks_for((long long)(H / 2) * (W / 2) * (d.c / 8)), dim3(256), 0,
-                     st, (const bf16*)n->act[in], H, W, d.c, n->act[out]);
+                     st, (const bf16*)n->act[in], H, W, d.c, n->act[out], skip);
   HIPC(hipGetLastError());
   return WC_OK;
 }
 
-int elementwise_upadd(wcu_net* n, Tensor t, Tensor e, Tensor out, hipStream_t st) {
+int elementwise_upadd(wcu_net* n, Tensor t, Tensor e, Tensor out, hipStream_t st,
+                      const int* skip = nullptr) {
   const TensorDef& d = TENSORS[e];
   const int H = n->H[d.level], W = n->W[d.level];
   hipLaunchKernelGGL(k_upadd, blocks_for((long long)H * W * (d.c / 8)), dim3(256), 0, st,
-                     (const bf16*)n->act[t], (const bf16*)n->act[e], H, W, d.c, n->act[out]);
+                     (const bf16*)n->act[t], (const bf16*)n->act[e], H, W, d.c, n->act[out],
+                     skip);
   HIPC(hipGetLastError());
   return WC_OK;
 }
@@ -915,25 +924,40 @@ int wcu_net_tensor(wcu_net* n, int32_t id, float* host, uint64_t* h, uint64_t* w
 
 namespace wc {
 
+// E0 .. D0 (everything but the output layer); every kernel does nothing while *skip != 0.
+int unet_body(wcu_net* n, hipStream_t st, const int* skip) {
+  int rc;
+  if ((rc = conv_layer(n, L_E0, X0, E0, st, skip))) return rc;
+  if ((rc = elementwise_pool(n, E0, P1, st, skip))) return rc;
+  if ((rc = conv_layer(n, L_E1, P1, E1, st, skip))) return rc;
+  if ((rc = elementwise_pool(n, E1, P2, st, skip))) return rc;
+  if ((rc = conv_layer(n, L_E2, P2, E2, st, skip))) return rc;
+  if ((rc = elementwise_pool(n, E2, P3, st, skip))) return rc;
+  if ((rc = conv_layer(n, L_E3, P3, E3, st, skip))) return rc;
+  if ((rc = conv_layer(n, L_B, E3, Y3, st, skip))) return rc;
+  if ((rc = conv_layer(n, L_T3, Y3, T3, st, skip))) return rc;
+  if ((rc = elementwise_upadd(n, T3, E2, U2, st, skip))) return rc;
+  if ((rc = conv_layer(n, L_D2, U2, Y2, st, skip))) return rc;
+  if ((rc = conv_layer(n, L_T2, Y2, T2, st, skip))) return rc;
+  if ((rc = elementwise_upadd(n, T2, E1, U1, st, skip))) return rc;
+  if ((rc = conv_layer(n, L_D1, U1, Y1, st, skip))) return rc;
+  if ((rc = conv_layer(n, L_T1, Y1, T1, st, skip))) return rc;
+  if ((rc = elementwise_upadd(n, T1, E0, U0, st, skip))) return rc;
+  return conv_layer(n, L_D0, U0, Y0, st, skip);
+}
+
+// Where the solver's preconditioner writes the input and reads the last hidden layer.
+void unet_views(wcu_net* n, void** x0, const void** y0, int* w0, int* h0, const float** wo) {
+  *x0 = n->act[X0];
+  *y0 = n->act[Y0];
+  *w0 = n->W[0];
+  *h0 = n->H[0];
+  *wo = n->params + layer_offset(L_O);
+}
+
 int unet_forward(wcu_net* n, float* out_xy, hipStream_t st) {
   int rc;
-  if ((rc = conv_layer(n, L_E0, X0, E0, st))) return rc;
-  if ((rc = elementwise_pool(n, E0, P1, st))) return rc;
-  if ((rc = conv_layer(n, L_E1, P1, E1, st))) return rc;
-  if ((rc = elementwise_pool(n, E1, P2, st))) return rc;
-  if ((rc = conv_layer(n, L_E2, P2, E2, st))) return rc;
-  if ((rc = elementwise_pool(n, E2, P3, st))) return rc;
-  if ((rc = conv_layer(n, L_E3, P3, E3, st))) return rc;
-  if ((rc = conv_layer(n, L_B, E3, Y3, st))) return rc;
-  if ((rc = conv_layer(n, L_T3, Y3, T3, st))) return rc;
-  if ((rc = elementwise_upadd(n, T3, E2, U2, st))) return rc;
-  if ((rc = conv_layer(n, L_D2, U2, Y2, st))) return rc;
-  if ((rc = conv_layer(n, L_T2, Y2, T2, st))) return rc;
-  if ((rc = elementwise_upadd(n, T2, E1, U1, st))) return rc;
-  if ((rc = conv_layer(n, L_D1, U1, Y1, st))) return rc;
-  if ((rc = conv_layer(n, L_T1, Y1, T1, st))) return rc;
-  if ((rc = elementwise_upadd(n, T1, E0, U0, st))) return rc;
-  if ((rc = conv_layer(n, L_D0, U0, Y0, st))) return rc;
+  if ((rc = unet_body(n, st, nullptr))) return rc;
   hipLaunchKernelGGL(k_out, blocks_for((long long)n->nx * n->ny), dim3(256), 0, st,
                      (const bf16*)n->act[Y0], n->W[0], n->nx, n->ny,
                      (const float*)(n->params + layer_offset(L_O)), n->out);

@@ -1,0 +1,211 @@
+"""Trains the learned preconditioner on SGN systems the solver itself generates, on the GPU.
+
+    python -m wavesim_amd.precond_train --beds 256 --out wavesim_amd/data/precond_rough_bed.npy
+
+Data. Seeded rough beds built like the 1024x512 rough-bed workload (wavesim_amd.configs
+.c1_rough_bed), at --nx x --ny cells. The FP32 solver with the reference's Jacobi PCG runs each
+bed with the option "capture" on; after --spinup steps, every --every steps the last solve's
+depth h and, before its iterations 0, 2, 5 and 10, the residual r_n = b - A w_n and the error
+e_n = w* - w_n (w* the solution to 1e-8) become samples: the network should map r_n to e_n,
+for the first residual as for the later ones a PCG meets.
+
+Features, as the solver's preconditioner builds them (wc_fpcg.hip): h / (4 dx), the bed's
+centred gradient times BED_FEATURE_SCALE, r / sigma (sigma the RMS of r over the cells);
+target e * (4 dx) / sigma; loss: mean squared error over the cells the dispersive solve
+touches (h >= 0.05). Training: the library's forward, backward and Adam kernels
+(wavesim_amd.unet), one sample per step.
+"""
+
+from __future__ import annotations
+
+import argparse
+import sys
+import time
+
+import numpy as np
+
+from .grid import GHOST, Grid
+from .solver import BED_FEATURE_SCALE, PRECISION_FP32, Solver
+
+H_DISPERSION_MIN = 0.05
+
+
+def bed_features(grid: Grid, b: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """BED_FEATURE_SCALE * the centred bed gradient on the padded grid (0 on the outer ring)."""
+    w, rows = grid.width(), grid.ny + 2 * GHOST
+    B = np.asarray(b, dtype=np.float64).reshape(rows, w)
+    gx = np.zeros_like(B)
+    gy = np.zeros_like(B)
+    gx[1:-1, 1:-1] = BED_FEATURE_SCALE * (B[1:-1, 2:] - B[1:-1, :-2]) / (2.0 * grid.dx)
+    gy[1:-1, 1:-1] = BED_FEATURE_SCALE * (B[2:, 1:-1] - B[:-2, 1:-1]) / (2.0 * grid.dy)
+    return gx.astype(np.float32).ravel(), gy.astype(np.float32).ravel()
+
+
+def sample_from_capture(grid: Grid, cap: dict, bgx, bgy) -> dict | None:
+    """One training sample (grid-layout float32 arrays plus the interior target and weight)."""
+    inner = grid.inner
+    rx, ry = inner(cap["rx"]).astype(np.float64), inner(cap["ry"]).astype(np.float64)
+    sigma = float(np.sqrt(np.mean(rx * rx + ry * ry)))
+    if not (sigma > 0.0 and np.isfinite(sigma)):
+        return None
+    scale = 4.0 * grid.dx / sigma
+    ex, ey = inner(cap["ex"]).astype(np.float64), inner(cap["ey"]).astype(np.float64)
+    target = np.stack([ex * scale, ey * scale], -1).astype(np.float32).ravel()
+    weight = (inner(cap["h"]) >= H_DISPERSION_MIN).astype(np.float32).ravel()
+    return {"h": cap["h"], "bgx": bgx, "bgy": bgy, "rx": cap["rx"], "ry": cap["ry"],
+            "r_scale": 1.0 / sigma, "target": target, "weight": weight}
+
+
+def generate(seeds, nx: int, ny: int, spinup: int, every: int, count: int, log=print) -> list:
+    """Samples from the solver's own solves on each seeded bed."""
+    from . import configs
+
+    out = []
+    for n, seed in enumerate(seeds):
+        w = configs.c1_rough_bed(nx=nx, ny=ny, seed=int(seed))
+        g = w.solver(Solver).with_precision(PRECISION_FP32)
+        s = w.still_water()
+        bgx, bgy = bed_features(w.grid, w.bed.b)
+        g.run_steps(s, spinup)
+        g.set_option("capture", 1)
+        for _ in range(count):
+            g.run_steps(s, every)
+            for slot in range(4):
+                cap = g.capture(slot)
+                smp = sample_from_capture(w.grid, cap, bgx, bgy) if cap is not None else None
+                if smp is not None:
+                    out.append(smp)
+        g.set_option("capture", 0)
+        if log and (n + 1) % 16 == 0:
+            log(f"[data] {n + 1}/{len(seeds)} beds, {len(out)} samples, t = {s.time:.0f} s")
+    return out
+
+
+def train(samples, grid: Grid, steps: int, lr: float = 1e-3, seed: int = 0, log=print):
+    """Trains from scratch; returns the FP32 parameters (numpy) and the loss history."""
+    import torch
+
+    from .unet import Adam, UNet, init_params, param_count
+
+    dev = [{k: (torch.tensor(v, device="cuda") if isinstance(v, np.ndarray) else v)
+            for k, v in smp.items()} for smp in samples]
+    net = UNet(grid.nx, grid.ny)
+    params = torch.tensor(init_params(seed), device="cuda")
+    net.set_params(params)
+    grads = torch.zeros(param_count(), device="cuda")
+    opt = Adam(params, lr=lr)
+    rng = np.random.default_rng(seed)
+    hs = 1.0 / (4.0 * grid.dx)
+    history = []
+    t0 = time.perf_counter()
+    order = rng.permutation(len(dev))
+    for step in range(steps):
+        if step and step % len(dev) == 0:
+            order = rng.permutation(len(dev))
+        d = dev[order[step % len(dev)]]
+        opt.lr = lr * (0.1 + 0.9 * 0.5 * (1.0 + np.cos(np.pi * step / steps)))
+        net.input(d["h"], d["bgx"], d["bgy"], d["rx"], d["ry"], pitch=grid.width(), ghost=GHOST,
+                  h_scale=hs, r_scale=d["r_scale"])
+        net.forward()
+        history.append(net.loss(d["target"], d["weight"]))
+        net.backward(grads)
+        opt.step(grads)
+        net.set_params(params)
+        if log and (step + 1) % max(steps // 20, 1) == 0:
+            recent = float(np.mean(history[-max(steps // 20, 1):]))
+            log(f"[train] step {step + 1}/{steps}: loss {recent:.4f} "
+                f"({time.perf_counter() - t0:.1f} s)")
+    torch.cuda.synchronize()
+    return params.cpu().numpy(), history
+
+
+def evaluate(params, seeds, nx: int, ny: int, spinup: int, steps: int, log=print) -> list:
+    """PCG iterations per solve and time per step, Jacobi against the network, on each bed from
+    the same mid-run state (FP32 form)."""
+    import torch
+
+    from . import configs
+    from .grid import State
+
+    rows = []
+    for seed in seeds:
+        w = configs.c1_rough_bed(nx=nx, ny=ny, seed=int(seed))
+        g = w.solver(Solver).with_precision(PRECISION_FP32)
+        s = w.still_water()
+        g.run_steps(s, spinup)
+        arrays = [a.copy() for a in s.arrays()]
+        t0 = s.time
+        row = {"seed": int(seed), "t": t0}
+        for name, p in (("jacobi", None), ("network", params)):
+            gg = w.solver(Solver).with_precision(PRECISION_FP32)
+            if p is not None:
+                gg.with_preconditioner(p)
+            ss = State(*(a.copy() for a in arrays), t0)
+            gg.run_steps(ss, 2)  # warm starts
+            st0 = gg.dispersion_stats()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            gg.run_steps(ss, steps)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t
+            st1 = gg.dispersion_stats()
+            row[name] = {"iters_per_solve": (st1.iterations - st0.iterations) /
+                         max(st1.solves - st0.solves, 1),
+                         "ms_per_step": 1e3 * el / steps,
+                         "unconverged": st1.unconverged - st0.unconverged,
+                         "fallbacks": gg.precond_fallbacks() if p is not None else 0}
+            if p is not None:
+                row["eta_diff_m"] = float(np.max(np.abs(ss.arrays()[0] - ref_h)))
+                hist = gg.residual_history()
+                row["history"] = (np.sqrt(hist / hist[0]) if hist.size else hist).tolist()
+            else:
+                ref_h = ss.arrays()[0].copy()
+        rows.append(row)
+        if log:
+            j, nn = row["jacobi"], row["network"]
+            log(f"[eval] seed {seed}: jacobi {j['iters_per_solve']:.1f} it/solve "
+                f"{j['ms_per_step']:.2f} ms/step; network {nn['iters_per_solve']:.1f} it/solve "
+                f"{nn['ms_per_step']:.2f} ms/step, {nn['fallbacks']} fallbacks, "
+                f"max |dh| {row['eta_diff_m']:.2e} m")
+            log("[eval]   network |r|/|r0| of the last solve: " +
+                " ".join(f"{v:.1e}" for v in row["history"][:40]))
+    return rows
+
+
+def main(argv=None) -> int:
+    p = argparse.ArgumentParser(description=__doc__)
+    p.add_argument("--beds", type=int, default=256)
+    p.add_argument("--nx", type=int, default=512)
+    p.add_argument("--ny", type=int, default=256)
+    p.add_argument("--spinup", type=int, default=900)
+    p.add_argument("--every", type=int, default=60)
+    p.add_argument("--count", type=int, default=4)
+    p.add_argument("--steps", type=int, default=6000)
+    p.add_argument("--lr", type=float, default=1e-3)
+    p.add_argument("--eval-beds", type=int, default=4)
+    p.add_argument("--eval-nx", type=int, default=1024)
+    p.add_argument("--eval-ny", type=int, default=512)
+    p.add_argument("--eval-spinup", type=int, default=1500)
+    p.add_argument("--eval-steps", type=int, default=40)
+    p.add_argument("--out", default=None)
+    a = p.parse_args(argv)
+
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
+
+    grid = Grid(a.nx, a.ny, 3.0, 3.0)
+    t = time.perf_counter()
+    samples = generate(range(1000, 1000 + a.beds), a.nx, a.ny, a.spinup, a.every, a.count, log)
+    log(f"[data] {len(samples)} samples in {time.perf_counter() - t:.1f} s")
+    params, hist = train(samples, grid, a.steps, a.lr, log=log)
+    if a.out:
+        np.save(a.out, params.astype(np.float32))
+        log(f"[train] saved {a.out}")
+    if a.eval_beds:
+        evaluate(params, range(5000, 5000 + a.eval_beds), a.eval_nx, a.eval_ny, a.eval_spinup,
+                 a.eval_steps, log)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -41,6 +41,9 @@ LIB_PATH = os.path.join(HERE, "libwavecore_hip.so")
 WC_OK, WC_EINVAL, WC_EDEVICE, WC_ENOMEM = 0, 1, 2, 3
 PRECISION_FP64 = 0
 PRECISION_FP32 = 1
+PRECOND_JACOBI = 0
+PRECOND_NETWORK = 1
+BED_FEATURE_SCALE = 10.0
 
 _D = ctypes.c_double
 _P = ctypes.POINTER(ctypes.c_double)
@@ -79,6 +82,11 @@ SIGNATURES = {
     "wc_linear_wave": (_I32, [_D, _D, _I32, ctypes.POINTER(CLinearWave)]),
     "wc_manning_factor": (_D, [_D, _D, _D, _D]),
     "wc_run_steps": (_I32, [_VP, _U64, ctypes.POINTER(_U64), _P]),
+    "wc_solver_set_preconditioner": (_I32, [_VP, _I32, ctypes.POINTER(ctypes.c_float), _U64]),
+    "wc_solver_precond_fallbacks": (_I32, [_VP, ctypes.POINTER(_U64)]),
+    "wc_solver_capture": (_I32, [_VP, _I32] + [ctypes.POINTER(ctypes.c_float)] * 5
+                          + [ctypes.POINTER(_I32)]),
+    "wc_solver_residual_history": (_I32, [_VP, _P, _U64, ctypes.POINTER(_U64)]),
     "wc_relaxation_new": (_I32, [_VP, ctypes.POINTER(CSponge), _U64, ctypes.POINTER(_VP)]),
     "wc_relaxation_free": (None, [_VP]),
     "wc_relaxation_len": (_U64, [_VP]),
@@ -101,6 +109,13 @@ def lib() -> ctypes.CDLL:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 f"{LIB_PATH} is not built: run `python -m wavesim_amd.build` (hipcc, gfx950)")
+        # One HIP runtime per process: torch ships its own libamdhip64.so.7, and the library's
+        # dependency of the same name resolves to it if torch is loaded first; the other way
+        # round two runtimes would meet and torch would see no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:  # pragma: no cover - the library then uses ROCm's runtime alone
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)
@@ -211,6 +226,45 @@ class Solver:
         """Implementation choices that change speed, never results (wc_solver_set_option)."""
         _check(lib().wc_solver_set_option(self._h, key.encode(), int(value)))
         return self
+
+    def with_preconditioner(self, params=None) -> "Solver":
+        """The dispersive solve's preconditioner: None for the reference's Jacobi diagonal, or
+        the learned network's FP32 parameters (wavesim_amd.unet) inside a flexible PCG with the
+        same tolerance (the FP32 form; the FP64 path keeps Jacobi)."""
+        if params is None:
+            _check(lib().wc_solver_set_preconditioner(self._h, PRECOND_JACOBI, None, 0))
+            return self
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        _check(lib().wc_solver_set_preconditioner(
+            self._h, PRECOND_NETWORK, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), p.size))
+        return self
+
+    def precond_fallbacks(self) -> int:
+        """Solves that fell back from the network to Jacobi since the solver was made."""
+        out = ctypes.c_uint64()
+        _check(lib().wc_solver_precond_fallbacks(self._h, ctypes.byref(out)))
+        return int(out.value)
+
+    def capture(self, slot: int = 0) -> dict | None:
+        """The last solve's depth, residual r_n and error w* - w_n before iteration n = 0, 2, 5,
+        10 (slot 0..3; option "capture" on), as padded float32 grid arrays; None if the solve
+        ended before that iteration."""
+        n = self.grid.cells()
+        out = {k: np.empty(n, dtype=np.float32) for k in ("h", "rx", "ry", "ex", "ey")}
+        fp = ctypes.POINTER(ctypes.c_float)
+        valid = _I32()
+        _check(lib().wc_solver_capture(self._h, slot,
+                                       *(out[k].ctypes.data_as(fp)
+                                         for k in ("h", "rx", "ry", "ex", "ey")),
+                                       ctypes.byref(valid)))
+        return out if valid.value else None
+
+    def residual_history(self) -> np.ndarray:
+        """r.r before each iteration of the last network-preconditioned solve."""
+        out = np.empty(1024)
+        n = _U64()
+        _check(lib().wc_solver_residual_history(self._h, _ptr(out), 1024, ctypes.byref(n)))
+        return out[:n.value].copy()
 
     def with_precision(self, precision: int) -> "Solver":
         _check(lib().wc_solver_set_precision(self._h, int(precision)))
