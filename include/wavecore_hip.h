@@ -183,12 +183,53 @@ int wc_solver_precond_fallbacks(wc_solver* s, uint64_t* out);
  * (slots 0..3), the residual r_n = b - A w_n and the iterate w_n. wc_solver_capture copies the
  * last solve's slot into host arrays of (nx+6)*(ny+6) floats (any may be NULL), the error as
  * w* - w_n with w* the solve's solution; *valid = 0 if the solve ended before that iteration.
- * Option "precond_stall": iterations without a new best residual before the network falls back
- * to Jacobi (default 4, 0 = never). wc_solver_residual_history: r.r before each iteration of
+ * Options "precond_stall" and "precond_ratio": the network falls back to Jacobi after
+ * precond_stall iterations in a row (default 2, 0 = never) that each keep more than
+ * precond_ratio percent (default 70) of the residual norm. wc_solver_residual_history: r.r before each iteration of
  * the last network-preconditioned solve (*n values, at most max). */
 int wc_solver_capture(wc_solver* s, int32_t slot, float* h, float* rx, float* ry, float* ex,
                       float* ey, int32_t* valid);
 int wc_solver_residual_history(wc_solver* s, double* out, uint64_t max, uint64_t* n);
+
+/* The run's per-step statistics (wavesim's Stage::record and Stats, crates/wavesim/src/
+ * lib.rs:849-937, 1099-1130, 1276-1300), accumulated on the GPU for the solver's state:
+ *   wc_stats_new     recording from time `from`, waves of `period` s (the first ends at
+ *                    from + period); elevation: the nx*ny bed elevation (f32, wavesim's
+ *                    Bed::elevation) that eta = h + elevation uses, or NULL for the solver's bed.
+ *   wc_stats_record  Stage::record after a step of dt has ended (the solver's time is its end).
+ *   wc_stats_read    seconds recorded, waves closed, and per cell (nx*ny, any NULL): breaking
+ *                    seconds (indicator >= 0.8), eta max and min, the mean, first and last wave
+ *                    heights (crest to trough per period).
+ *   wc_stats_harmonic FirstHarmonic::amplitudes: (a, -b) per cell, *ok = 0 for None.
+ * StepWatch (lib.rs:849-896): wc_watch_init with the still sea's step; wc_watch_collapsed counts
+ * a stable step and sets *ratio > 0 at the end of a block of WC_WATCH_STEPS whose mean step was
+ * more than WC_COLLAPSE times shorter. wc_stage_step is Stage::next_dt + advance + record
+ * (lib.rs:1385-1434) without frames: *status WC_STAGE_OK (stepped by *dt), _UNSTABLE (stable
+ * dt NaN or <= 0), _NO_WATER (+inf), _COLLAPSED (*ratio set); st and w may be NULL. */
+typedef struct wc_stats wc_stats;
+typedef struct {
+  double reference;
+  uint32_t steps;
+  double seconds;
+} wc_watch_t;
+#define WC_WATCH_STEPS 1000
+#define WC_COLLAPSE 5.0
+#define WC_STAGE_OK 0
+#define WC_STAGE_UNSTABLE 1
+#define WC_STAGE_NO_WATER 2
+#define WC_STAGE_COLLAPSED 3
+int wc_stats_new(wc_solver* s, double from, double period, const float* elevation,
+                 wc_stats** out);
+void wc_stats_free(wc_stats* st);
+int wc_stats_record(wc_stats* st, double dt);
+int wc_stats_read(wc_stats* st, double* seconds, uint64_t* waves, double* breaking_seconds,
+                  double* eta_max, double* eta_min, double* wave_mean, double* wave_first,
+                  double* wave_last);
+int wc_stats_harmonic(wc_stats* st, int32_t* ok, double* a, double* b);
+int wc_watch_init(wc_watch_t* w, double reference);
+int wc_watch_collapsed(wc_watch_t* w, double stable, double* ratio);
+int wc_stage_step(wc_solver* s, wc_stats* st, wc_watch_t* w, double until, double* dt,
+                  int32_t* status, double* ratio);
 
 /* Relaxation zones (forcing.rs:227-284): the water in the zones is pulled towards targets the
  * caller supplies at every step (one-way nesting, crates/wavesim/src/lib.rs:1894-1897, and

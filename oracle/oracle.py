@@ -96,6 +96,12 @@ def lib():
             "wco_relaxation_len": (_U64, [_VP]),
             "wco_relaxation_cells": (_I32, [_VP, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
             "wco_relaxation_apply": (_I32, [_VP, _D, _P]),
+            "wco_stats_new": (_I32, [_VP, _D, _D, ctypes.POINTER(ctypes.c_float),
+                                     ctypes.POINTER(_VP)]),
+            "wco_stats_free": (None, [_VP]),
+            "wco_stats_record": (_I32, [_VP, _D]),
+            "wco_stats_read": (_I32, [_VP, _P, ctypes.POINTER(_U64), _P, _P, _P, _P, _P, _P]),
+            "wco_stats_harmonic": (_I32, [_VP, ctypes.POINTER(_I32), _P, _P]),
             "wco_linear_wave": (_I32, [_D, _D, _I32, ctypes.POINTER(CLinearWave)]),
             "wco_manning_factor": (_D, [_D, _D, _D, _D]),
             "wco_face_values": (None, [_P, _D, _P, _P]),
@@ -327,6 +333,48 @@ class OracleRelaxation:
         time = s.time
         self.solver._download_into(s)
         s.time = time
+
+
+class OracleStats:
+    """wavesim's Stats and Stage::record (crates/wavesim/src/lib.rs:899-932, 1276-1300) on the
+    oracle solver's state (record after the oracle has stepped the State it holds)."""
+
+    def __init__(self, solver: OracleSolver, from_: float, period: float, elevation=None):
+        self.solver = solver
+        self.n = solver.grid.nx * solver.grid.ny
+        el = None
+        if elevation is not None:
+            self._el = np.ascontiguousarray(elevation, dtype=np.float32)
+            el = self._el.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        h = _VP()
+        _check(lib().wco_stats_new(solver._h, float(from_), float(period), el, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().wco_stats_free(self._h)
+            self._h = None
+
+    def record(self, s: State, dt: float) -> None:
+        self.solver._upload(s)
+        _check(lib().wco_stats_record(self._h, float(dt)))
+
+    def read(self) -> dict:
+        keys = ("breaking_seconds", "eta_max", "eta_min", "wave_mean", "wave_first", "wave_last")
+        out = {k: np.empty(self.n) for k in keys}
+        sec = ctypes.c_double()
+        waves = _U64()
+        _check(lib().wco_stats_read(self._h, ctypes.byref(sec), ctypes.byref(waves),
+                                    *(ptr(out[k]) for k in keys)))
+        out["seconds"] = sec.value
+        out["waves"] = int(waves.value)
+        return out
+
+    def harmonic(self):
+        ok = _I32()
+        a, b = np.empty(self.n), np.empty(self.n)
+        _check(lib().wco_stats_harmonic(self._h, ctypes.byref(ok), ptr(a), ptr(b)))
+        return (a, b) if ok.value else None
 
 
 def linear_wave(omega: float, h: float, dispersive: bool) -> LinearWave:

@@ -1172,6 +1172,123 @@ int wco_breaking_indicator(wco_solver* s, double* out) {
   return 0;
 }
 
+// wavesim's Stats and Stage::record (crates/wavesim/src/lib.rs:899-932, 1099-1130, 1276-1300)
+struct wco_stats {
+  wco_solver* s = nullptr;
+  double from = 0, period = 0, next_wave = 0, seconds = 0, omega = 0;
+  double moments[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<double> breaking_seconds, eta_max, eta_min, wave_max, wave_min;
+  std::vector<std::vector<double>> waves;
+  std::vector<double> sums;  // 3 per cell
+  std::vector<float> elevation;
+};
+
+int wco_stats_new(wco_solver* s, double from, double period, const float* elevation,
+                  wco_stats** out) {
+  auto* st = new wco_stats;
+  const size_t n = s->g.nx * s->g.ny;
+  st->s = s;
+  st->from = from;
+  st->period = period;
+  st->next_wave = from + period;
+  st->omega = TAU / period;
+  st->breaking_seconds.assign(n, 0.0);
+  st->eta_max.assign(n, -INFINITY);
+  st->eta_min.assign(n, INFINITY);
+  st->wave_max.assign(n, -INFINITY);
+  st->wave_min.assign(n, INFINITY);
+  st->sums.assign(3 * n, 0.0);
+  if (elevation) st->elevation.assign(elevation, elevation + n);
+  *out = st;
+  return 0;
+}
+
+void wco_stats_free(wco_stats* st) { delete st; }
+
+int wco_stats_record(wco_stats* st, double dt) {
+  wco_solver* s = st->s;
+  const Grid& g = s->g;
+  const double inside = std::min(dt, s->s.time - st->from);
+  if (inside <= 0.0) return 0;
+  std::vector<double> now(g.nx * g.ny);
+  wco_breaking_indicator(s, now.data());
+  if (s->s.time > st->next_wave + 1e-9) {  // close_wave
+    std::vector<double> heights(now.size());
+    for (size_t n = 0; n < now.size(); ++n)
+      heights[n] = std::isfinite(st->wave_max[n]) ? st->wave_max[n] - st->wave_min[n] : 0.0;
+    st->waves.push_back(heights);
+    std::fill(st->wave_max.begin(), st->wave_max.end(), -INFINITY);
+    std::fill(st->wave_min.begin(), st->wave_min.end(), INFINITY);
+    st->next_wave += st->period;
+  }
+  const double sn = sin(st->omega * s->s.time), cs = cos(st->omega * s->s.time);
+  const double v[6] = {1.0, cs, sn, cs * cs, cs * sn, sn * sn};
+  for (int m = 0; m < 6; ++m) st->moments[m] += inside * v[m];
+  const double w[3] = {inside, inside * cs, inside * sn};
+  size_t n = 0;
+  for (size_t j = GH; j < GH + g.ny; ++j)
+    for (size_t i = GH; i < GH + g.nx; ++i, ++n) {
+      const size_t k = g.idx(i, j);
+      const double eta =
+          s->s.h[k] + (st->elevation.empty() ? s->bed[k] : (double)st->elevation[n]);
+      for (int m = 0; m < 3; ++m) st->sums[3 * n + m] += w[m] * eta;
+      st->eta_max[n] = wc_max(st->eta_max[n], eta);
+      st->eta_min[n] = wc_min(st->eta_min[n], eta);
+      st->wave_max[n] = wc_max(st->wave_max[n], eta);
+      st->wave_min[n] = wc_min(st->wave_min[n], eta);
+      if (now[n] >= 0.8) st->breaking_seconds[n] += inside;
+    }
+  st->seconds += inside;
+  return 0;
+}
+
+int wco_stats_read(wco_stats* st, double* seconds, uint64_t* waves, double* breaking_seconds,
+                   double* eta_max, double* eta_min, double* wave_mean, double* wave_first,
+                   double* wave_last) {
+  const size_t n = st->eta_max.size();
+  if (seconds) *seconds = st->seconds;
+  if (waves) *waves = st->waves.size();
+  for (size_t q = 0; q < n; ++q) {
+    if (breaking_seconds) breaking_seconds[q] = st->breaking_seconds[q];
+    if (eta_max) eta_max[q] = st->eta_max[q];
+    if (eta_min) eta_min[q] = st->eta_min[q];
+    double sum = 0.0;
+    for (const auto& wv : st->waves) sum += wv[q];
+    if (wave_mean) wave_mean[q] = st->waves.empty() ? 0.0 : sum / (double)st->waves.size();
+    if (wave_first) wave_first[q] = st->waves.empty() ? 0.0 : st->waves.front()[q];
+    if (wave_last) wave_last[q] = st->waves.empty() ? 0.0 : st->waves.back()[q];
+  }
+  return 0;
+}
+
+// FirstHarmonic::amplitudes (lib.rs:1136-1160)
+int wco_stats_harmonic(wco_stats* st, int32_t* ok, double* a, double* b) {
+  const double* M = st->moments;
+  const double m[3][3] = {{M[0], M[1], M[2]}, {M[1], M[3], M[4]}, {M[2], M[4], M[5]}};
+  auto cof = [&](int r, int k) {
+    const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+    return m[r1][k1] * m[r2][k2] - m[r1][k2] * m[r2][k1];
+  };
+  double det = 0.0;
+  for (int k = 0; k < 3; ++k) det += m[0][k] * cof(0, k);
+  if (std::isnan(det) || det <= 1e-3 * M[0] * M[0] * M[0] / 4.0) {
+    *ok = 0;
+    return 0;
+  }
+  *ok = 1;
+  const size_t n = st->eta_max.size();
+  for (size_t q = 0; q < n; ++q) {
+    auto row = [&](int r) {
+      double t = 0.0;
+      for (int k = 0; k < 3; ++k) t += cof(k, r) / det * st->sums[3 * q + k];
+      return t;
+    };
+    if (a) a[q] = row(1);
+    if (b) b[q] = -row(2);
+  }
+  return 0;
+}
+
 int wco_dispersion_stats(const wco_solver* s, int32_t* has, wco_dispersion_stats_t* out) {
   *has = s->dispersive ? 1 : 0;
   if (s->dispersive) *out = s->work.stats;

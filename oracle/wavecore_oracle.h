@@ -110,6 +110,18 @@ uint64_t wco_relaxation_len(const wco_relaxation* r);
 int wco_relaxation_cells(const wco_relaxation* r, uint64_t* i, uint64_t* j);
 int wco_relaxation_apply(wco_relaxation* r, double dt, const double* targets);
 
+/* wavesim's per-step statistics (crates/wavesim/src/lib.rs:899-932, 1099-1160, 1276-1300) on
+ * the oracle solver's state; the product's wc_stats_* with the prefix wco_. */
+typedef struct wco_stats wco_stats;
+int wco_stats_new(wco_solver* s, double from, double period, const float* elevation,
+                  wco_stats** out);
+void wco_stats_free(wco_stats* st);
+int wco_stats_record(wco_stats* st, double dt);
+int wco_stats_read(wco_stats* st, double* seconds, uint64_t* waves, double* breaking_seconds,
+                   double* eta_max, double* eta_min, double* wave_mean, double* wave_first,
+                   double* wave_last);
+int wco_stats_harmonic(wco_stats* st, int32_t* ok, double* a, double* b);
+
 int wco_linear_wave(double omega, double h, int32_t dispersive, wco_linear_wave_t* out);
 double wco_manning_factor(double h, double speed, double dt, double n);
 

@@ -17,4 +17,13 @@ from .config import (  # noqa: F401
     manning_factor,
 )
 from .grid import GHOST, G, H_DRY, THREADED_CELLS, Bathymetry, Grid, State  # noqa: F401
-from .solver import PRECISION_FP32, PRECISION_FP64, DeviceError, Relaxation, Solver  # noqa: F401,E402
+from .solver import (  # noqa: F401,E402
+    PRECISION_FP32,
+    PRECISION_FP64,
+    DeviceError,
+    Relaxation,
+    Solver,
+    Stats,
+    StepWatch,
+    stage_step,
+)

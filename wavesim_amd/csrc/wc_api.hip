@@ -162,7 +162,8 @@ struct wc_solver {
   float *cap_wx[CAP_SLOTS] = {}, *cap_wy[CAP_SLOTS] = {}, *cap_h = nullptr;
   float2* cap_r[CAP_SLOTS] = {};
   unsigned long long cap_last_it = 0;
-  int precond_stall = 4;
+  int precond_stall = 2;
+  int precond_ratio = 70;  // percent
   double* history = nullptr;  // r.r per iteration of the last network-preconditioned solve
 
   // implementation options (wc_solver_set_option)
@@ -530,7 +531,7 @@ int pcg3(wc_solver* s, int stage, CgV<V> a, V* pA_x, V* pA_y, V* pB_x, V* pB_y) 
     batch = s->pcg_batch;
   }
   s->prev_it[stage] = s->sc_host->it;
-  TRY(launch(s, F_CG_STATS, k_cg_stats, dim3(1), dim3(64), s->sc));
+  TRY(launch(s, F_CG_STATS, k_cg_stats, dim3(1), dim3(64), s->sc, 0ull));
   return WC_OK;
 }
 
@@ -609,7 +610,7 @@ int pcg(wc_solver* s, int stage) {
     HIP_OK(hipStreamSynchronize(s->stream));
   }
   s->prev_it[stage] = s->sc_host->it;
-  TRY(launch(s, F_CG_STATS, k_cg_stats, dim3(1), dim3(64), s->sc));
+  TRY(launch(s, F_CG_STATS, k_cg_stats, dim3(1), dim3(64), s->sc, 0ull));
   return WC_OK;
 }
 
@@ -783,7 +784,7 @@ Sgn32 sgn32_args(wc_solver* s, int stage) {
 
 // The one-kernel PCG (wc_pcg1.hip), FP32 form: start kernels, then one kernel per iteration in
 // batches, a one-wave check closing every batch.
-int pcg1_32(wc_solver* s, int stage) {
+int pcg1_32(wc_solver* s, int stage, unsigned long long earlier = 0) {
   const Sgn32 sa = sgn32_args(s, stage);
   CgG a{};
   a.g = s->geo;
@@ -839,7 +840,7 @@ int pcg1_32(wc_solver* s, int stage) {
   }
   s->prev_it[stage] = s->sc_host->it;
   s->cap_last_it = s->sc_host->it;
-  TRY(launch(s, F_CG_STATS, k_cg_stats, dim3(1), dim3(64), s->sc));
+  TRY(launch(s, F_CG_STATS, k_cg_stats, dim3(1), dim3(64), s->sc, earlier));
   return WC_OK;
 }
 
@@ -879,13 +880,14 @@ int pcg_net32(wc_solver* s, int stage) {
   a.tolerance = s->disp.tolerance;
   a.max_iterations = s->disp.max_iterations;
   a.stall_limit = s->precond_stall;
+  a.min_ratio = 0.01 * s->precond_ratio;
   a.history = s->history;
   a.history_len = 1024;
   const dim3 grid(256), block(1024);
   TRY(launch(s, F_CG_INIT, k_fcg_init, grid, block, a));
   const unsigned long long maxit = s->disp.max_iterations;
   unsigned long long n = 1;
-  unsigned long long batch = std::max<unsigned long long>(s->prev_it[stage], 1);
+  unsigned long long batch = 4;
   while (true) {
     const unsigned long long stop = std::min(n + batch, maxit + 1);
     for (; n < stop; ++n) {
@@ -910,10 +912,12 @@ int pcg_net32(wc_solver* s, int stage) {
                           s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
     if (s->sc_host->done || n > maxit) break;
-    batch = s->pcg_batch;
+    // Fallen back: the rest of the solve is the reference's Jacobi PCG from the current w (the
+    // fallback restarts the directions anyway), in the one-kernel form, with the network's
+    // iterations counted in.
+    if (s->sc_host->fallback) return pcg1_32(s, stage, s->sc_host->it);
   }
-  s->prev_it[stage] = s->sc_host->it;
-  TRY(launch(s, F_CG_STATS, k_cg_stats, dim3(1), dim3(64), s->sc));
+  TRY(launch(s, F_CG_STATS, k_cg_stats, dim3(1), dim3(64), s->sc, 0ull));
   return WC_OK;
 }
 
@@ -976,7 +980,7 @@ int pcg32(wc_solver* s, int stage) {
     HIP_OK(hipStreamSynchronize(s->stream));
   }
   s->prev_it[stage] = s->sc_host->it;
-  TRY(launch(s, F_CG_STATS, k_cg_stats, dim3(1), dim3(64), s->sc));
+  TRY(launch(s, F_CG_STATS, k_cg_stats, dim3(1), dim3(64), s->sc, 0ull));
   return WC_OK;
 }
 
@@ -1324,6 +1328,11 @@ int wc_solver_set_option(wc_solver* s, const char* key, int64_t value) {
     s->capture = (int)value;
     return WC_OK;
   }
+  if (strcmp(key, "precond_ratio") == 0) {
+    if (value < 1 || value > 100) return fail(WC_EINVAL, "precond_ratio must be in [1, 100]");
+    s->precond_ratio = (int)value;
+    return WC_OK;
+  }
   if (strcmp(key, "precond_stall") == 0) {
     if (value < 0 || value > 1000) return fail(WC_EINVAL, "precond_stall must be in [0, 1000]");
     s->precond_stall = (int)value;
@@ -1483,6 +1492,239 @@ int wc_solver_residual_history(wc_solver* s, double* out, uint64_t max, uint64_t
   const uint64_t count = std::min<uint64_t>(std::min<uint64_t>(s->sc_host->it + 1, 1024), max);
   if (out && count) HIP_OK(hipMemcpy(out, s->history, count * sizeof(double), hipMemcpyDeviceToHost));
   *n = count;
+  return WC_OK;
+}
+
+// ---- the run's statistics and the driver step (crates/wavesim/src/lib.rs:849-937,1099-1434) --
+
+struct wc_stats {
+  wc_solver* s = nullptr;
+  double from = 0.0, period = 0.0, next_wave = 0.0, seconds = 0.0, omega = 0.0;
+  double moments[6] = {0, 0, 0, 0, 0, 0};
+  uint64_t waves = 0;
+  double *breaking_seconds = nullptr, *eta_max = nullptr, *eta_min = nullptr,
+         *wave_max = nullptr, *wave_min = nullptr, *harm = nullptr, *wave_sum = nullptr,
+         *wave_first = nullptr, *wave_last = nullptr;
+  float* elevation = nullptr;
+  std::vector<void*> owned;
+};
+
+namespace {
+
+__global__ void k_fill_value(double* a, long long n, double v) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = v;
+}
+
+}  // namespace
+
+int wc_stats_new(wc_solver* s, double from, double period, const float* elevation,
+                 wc_stats** out) {
+  if (!s || !out) return fail(WC_EINVAL, "null argument");
+  if (!(period > 0.0)) return fail(WC_EINVAL, "the wave period must be positive");
+  *out = nullptr;
+  TRY(set_device(s));
+  auto st = std::make_unique<wc_stats>();
+  st->s = s;
+  st->from = from;
+  st->period = period;
+  st->next_wave = from + period;
+  st->omega = 2.0 * M_PI / period;
+  const long long n = (long long)s->geo.nx * s->geo.ny;
+  auto alloc = [&](double** p, long long count, double v) -> int {
+    HIP_OK(hipMalloc(p, std::max<long long>(count, 1) * sizeof(double)));
+    st->owned.push_back(*p);
+    hipLaunchKernelGGL(k_fill_value, over(count), dim3(NT), 0, s->stream, *p, count, v);
+    HIP_OK(hipGetLastError());
+    return WC_OK;
+  };
+  TRY(alloc(&st->breaking_seconds, n, 0.0));
+  TRY(alloc(&st->eta_max, n, -INFINITY));
+  TRY(alloc(&st->eta_min, n, INFINITY));
+  TRY(alloc(&st->wave_max, n, -INFINITY));
+  TRY(alloc(&st->wave_min, n, INFINITY));
+  TRY(alloc(&st->harm, 3 * n, 0.0));
+  TRY(alloc(&st->wave_sum, n, 0.0));
+  TRY(alloc(&st->wave_first, n, 0.0));
+  TRY(alloc(&st->wave_last, n, 0.0));
+  if (elevation) {
+    HIP_OK(hipMalloc(&st->elevation, n * sizeof(float)));
+    st->owned.push_back(st->elevation);
+    HIP_OK(hipMemcpyAsync(st->elevation, elevation, n * sizeof(float), hipMemcpyHostToDevice,
+                          s->stream));
+  }
+  HIP_OK(hipStreamSynchronize(s->stream));
+  *out = st.release();
+  return WC_OK;
+}
+
+void wc_stats_free(wc_stats* st) {
+  if (!st) return;
+  (void)hipSetDevice(st->s->device);
+  (void)hipStreamSynchronize(st->s->stream);
+  for (void* p : st->owned) (void)hipFree(p);
+  delete st;
+}
+
+// Stage::record (lib.rs:1276-1300) after a step of dt that has just ended.
+int wc_stats_record(wc_stats* st, double dt) {
+  if (!st) return fail(WC_EINVAL, "null statistics");
+  wc_solver* s = st->s;
+  const double inside = std::min(dt, s->time - st->from);
+  if (inside <= 0.0) return WC_OK;
+  TRY(set_device(s));
+  TRY(join_state(s));
+  RecordArgs a{};
+  a.g = s->geo;
+  a.h = s->h;
+  a.b = s->b;
+  a.br = breaking_of(s);
+  a.elevation = st->elevation;
+  if (s->time > st->next_wave + 1e-9) {  // Stats::close_wave (lib.rs:918-932)
+    a.close_wave = 1;
+    st->waves += 1;
+    a.first_wave = st->waves == 1 ? 1 : 0;
+    st->next_wave += st->period;
+  }
+  // FirstHarmonic::weights (lib.rs:1118-1128)
+  const double sn = sin(st->omega * s->time), cs = cos(st->omega * s->time);
+  const double v[6] = {1.0, cs, sn, cs * cs, cs * sn, sn * sn};
+  for (int m = 0; m < 6; ++m) st->moments[m] += inside * v[m];
+  a.inside = inside;
+  a.w0 = inside;
+  a.w1 = inside * cs;
+  a.w2 = inside * sn;
+  a.breaking_seconds = st->breaking_seconds;
+  a.eta_max = st->eta_max;
+  a.eta_min = st->eta_min;
+  a.wave_max = st->wave_max;
+  a.wave_min = st->wave_min;
+  a.harm = st->harm;
+  a.wave_sum = st->wave_sum;
+  a.wave_first = st->wave_first;
+  a.wave_last = st->wave_last;
+  TRY(launch(s, F_REDUCE, k_record, over((long long)s->geo.nx * s->geo.ny), dim3(NT), a));
+  st->seconds += inside;
+  if (!s->prof) HIP_OK(hipStreamSynchronize(s->stream));
+  return WC_OK;
+}
+
+int wc_stats_read(wc_stats* st, double* seconds, uint64_t* waves, double* breaking_seconds,
+                  double* eta_max, double* eta_min, double* wave_mean, double* wave_first,
+                  double* wave_last) {
+  if (!st) return fail(WC_EINVAL, "null statistics");
+  wc_solver* s = st->s;
+  TRY(set_device(s));
+  const size_t n = (size_t)s->geo.nx * s->geo.ny;
+  HIP_OK(hipStreamSynchronize(s->stream));
+  auto get = [&](double* dst, const double* src) -> int {
+    if (dst) HIP_OK(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyDeviceToHost));
+    return WC_OK;
+  };
+  if (seconds) *seconds = st->seconds;
+  if (waves) *waves = st->waves;
+  TRY(get(breaking_seconds, st->breaking_seconds));
+  TRY(get(eta_max, st->eta_max));
+  TRY(get(eta_min, st->eta_min));
+  TRY(get(wave_first, st->wave_first));
+  TRY(get(wave_last, st->wave_last));
+  if (wave_mean) {
+    TRY(get(wave_mean, st->wave_sum));
+    for (size_t k = 0; k < n; ++k)
+      wave_mean[k] = st->waves ? wave_mean[k] / (double)st->waves : 0.0;
+  }
+  return WC_OK;
+}
+
+// FirstHarmonic::amplitudes (lib.rs:1136-1160): (a, -b) per cell, *ok = 0 where the reference
+// returns None.
+int wc_stats_harmonic(wc_stats* st, int32_t* ok, double* a_out, double* b_out) {
+  if (!st || !ok) return fail(WC_EINVAL, "null argument");
+  wc_solver* s = st->s;
+  TRY(set_device(s));
+  const size_t n = (size_t)s->geo.nx * s->geo.ny;
+  const double s0 = st->moments[0], c = st->moments[1], sn = st->moments[2];
+  const double cc = st->moments[3], cs = st->moments[4], ss = st->moments[5];
+  const double m[3][3] = {{s0, c, sn}, {c, cc, cs}, {sn, cs, ss}};
+  auto cof = [&](int r, int k) {
+    const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+    return m[r1][k1] * m[r2][k2] - m[r1][k2] * m[r2][k1];
+  };
+  double det = 0.0;
+  for (int k = 0; k < 3; ++k) det += m[0][k] * cof(0, k);
+  if (std::isnan(det) || det <= 1e-3 * s0 * s0 * s0 / 4.0) {
+    *ok = 0;
+    return WC_OK;
+  }
+  *ok = 1;
+  std::vector<double> e(3 * n);
+  HIP_OK(hipStreamSynchronize(s->stream));
+  HIP_OK(hipMemcpy(e.data(), st->harm, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+  auto inv = [&](int r, int k) { return cof(k, r) / det; };
+  for (size_t q = 0; q < n; ++q) {
+    auto row = [&](int r) {
+      double t = 0.0;
+      for (int k = 0; k < 3; ++k) t += inv(r, k) * e[3 * q + k];
+      return t;
+    };
+    if (a_out) a_out[q] = row(1);
+    if (b_out) b_out[q] = -row(2);
+  }
+  return WC_OK;
+}
+
+// StepWatch (lib.rs:861-896)
+int wc_watch_init(wc_watch_t* w, double reference) {
+  if (!w) return fail(WC_EINVAL, "null watch");
+  w->reference = std::isfinite(reference) ? reference : 0.0;
+  w->steps = 0;
+  w->seconds = 0.0;
+  return WC_OK;
+}
+
+int wc_watch_collapsed(wc_watch_t* w, double stable, double* ratio) {
+  if (!w || !ratio) return fail(WC_EINVAL, "null argument");
+  *ratio = 0.0;
+  w->steps += 1;
+  w->seconds += stable;
+  if (w->steps < WC_WATCH_STEPS) return WC_OK;
+  const double mean = w->seconds / (double)w->steps;
+  w->steps = 0;
+  w->seconds = 0.0;
+  if (mean * WC_COLLAPSE < w->reference) *ratio = w->reference / mean;
+  return WC_OK;
+}
+
+// Stage::next_dt + advance + record (lib.rs:1385-1434), without frames: the next step as long
+// as is stable and ending no later than `until`.
+int wc_stage_step(wc_solver* s, wc_stats* st, wc_watch_t* w, double until, double* dt,
+                  int32_t* status, double* ratio) {
+  if (!s || !dt || !status) return fail(WC_EINVAL, "null argument");
+  *status = WC_STAGE_OK;
+  if (ratio) *ratio = 0.0;
+  double stable = 0.0;
+  TRY(wc_stable_dt(s, &stable));
+  *dt = stable;
+  if (std::isnan(stable) || stable <= 0.0) {
+    *status = WC_STAGE_UNSTABLE;
+    return WC_OK;
+  }
+  if (std::isinf(stable)) {
+    *status = WC_STAGE_NO_WATER;
+    return WC_OK;
+  }
+  if (w) {
+    double r = 0.0;
+    TRY(wc_watch_collapsed(w, stable, &r));
+    if (r > 0.0) {
+      *status = WC_STAGE_COLLAPSED;
+      if (ratio) *ratio = r;
+      return WC_OK;
+    }
+  }
+  *dt = std::min(stable, until - s->time);
+  TRY(wc_step(s, *dt));
+  if (st) TRY(wc_stats_record(st, *dt));
   return WC_OK;
 }
 

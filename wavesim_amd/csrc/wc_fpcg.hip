@@ -8,7 +8,7 @@
 // reference's (target = tol ||b|| + 1e-10). When the residual stops decreasing the solve falls
 // back to the reference's Jacobi
 // preconditioner for the rest of its iterations, restarting the directions. "Stops decreasing":
-// the best r.r of the solve has not improved for stall_limit iterations in a row.
+// stall_limit iterations in a row each cut the residual norm by less than the factor min_ratio.
 //
 // Per iteration n: F_in (convergence and stall test on r_{n-1}; the network input from r / sigma,
 // sigma its RMS), the network (E0 .. D0, skipped while done or fallen back), F_out (z_n: the
@@ -54,13 +54,11 @@ __device__ __forceinline__ bool fcg_start(const FcgArgs& a, unsigned long long n
     sc->it = it;
     sc->done = done ? 1 : 0;
     if (update) {
-      // rr_prev holds the best r.r of the solve so far; stall counts the iterations since
-      // it last improved (CG's residual norm need not fall every iteration, its best must)
+      // stall counts the iterations in a row whose residual norm fell by less than the factor
+      // a.min_ratio (r.r by less than its square): the network has stopped earning its cost
       int stall = 0, fallback = 0;
-      double best = rr;
       if (n > 1) {
-        best = wc_min(rr, sc->rr_prev);
-        stall = rr < sc->rr_prev ? 0 : sc->stall + 1;
+        stall = rr > a.min_ratio * a.min_ratio * sc->rr_prev ? sc->stall + 1 : 0;
         fallback = sc->fallback;
       }
       const int fall_now = !fallback && a.stall_limit > 0 && stall >= a.stall_limit;
@@ -69,7 +67,7 @@ __device__ __forceinline__ bool fcg_start(const FcgArgs& a, unsigned long long n
       sc->stall = stall;
       sc->fallback = fallback;
       sc->restart = fall_now;
-      sc->rr_prev = best;
+      sc->rr_prev = rr;
       sc->sigma = sigma;
       sc->net_off = (done || fallback) ? 1 : 0;
       if (a.history && it < a.history_len) a.history[it] = rr;
@@ -145,9 +143,10 @@ __global__ void __launch_bounds__(PT) k_fcg_init(FcgArgs a) {
 __global__ void __launch_bounds__(PT) k_fcg_in(FcgArgs a, unsigned long long n) {
   double sigma;
   if (!fcg_start(a, n, true, sigma)) return;
-  // After a fallback the input is not used; a block that still reads the flag as it was before
-  // this kernel's block 0 updated it only writes an unused input.
-  if (a.sc->fallback) return;
+  // After a fallback the input is not used. A solve starts without one (at n = 1 the flag in
+  // memory may still be the previous solve's, which block 0 is resetting); later, a block that
+  // reads the flag as it was before block 0 set it only writes an unused input.
+  if (n > 1 && a.sc->fallback) return;
   const Geo& g = a.g;
   const double rs = sigma > 0.0 ? 1.0 / sigma : 0.0;
   const long long cells = (long long)g.nx * g.ny;

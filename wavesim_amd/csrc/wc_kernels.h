@@ -137,6 +137,7 @@ struct FcgArgs {
   double tolerance;
   unsigned long long max_iterations;
   int stall_limit;            // 0: never fall back
+  double min_ratio;           // an iteration that keeps more than this of |r| stalls
   double* history;            // r.r before each iteration of the solve (may be null)
   unsigned long long history_len;
 };
@@ -220,7 +221,8 @@ __global__ void k_cg_init(SgnArgs a);
 __global__ void k_cg_matvec(SgnArgs a);
 __global__ void k_cg_update(SgnArgs a);
 __global__ void k_cg_pupdate(SgnArgs a);
-__global__ void k_cg_stats(CgScalars* sc);
+// Counts a finished solve; `earlier` iterations of it ran in an earlier phase (the network's).
+__global__ void k_cg_stats(CgScalars* sc, unsigned long long earlier);
 __global__ void k_cg_matvec2(SgnArgs a, int first, const double* psx, const double* psy,
                              double* pdx, double* pdy);
 __global__ void k_cg_update2(SgnArgs a, const double* pcx, const double* pcy);
@@ -238,6 +240,23 @@ __global__ void k_state_sums(Geo g, const double* h, const double* hu, const dou
                              unsigned long long* minh_key);
 __global__ void k_sum_tiles(const double* part, int np, double* out);
 __global__ void k_cg_reduce(SgnArgs a, int kind);
+
+// Stage::record (crates/wavesim/src/lib.rs:1276-1300) for one step: per interior cell n, the
+// surface eta = h + elevation joins the extremes, the wave in progress and the first-harmonic
+// sums, and the cell's breaking seconds grow by `inside` where the indicator is >= 0.8. With
+// close_wave the wave in progress is closed first (lib.rs:918-932): its crest-to-trough height
+// (0 if it saw no sample) is added to wave_sum and kept as the last wave (and the first).
+struct RecordArgs {
+  Geo g;
+  const double *h, *b;
+  Breaking br;
+  const float* elevation;  // nx*ny, the caller's bed elevation; null: the solver's bed
+  double inside, w0, w1, w2;
+  int close_wave, first_wave;
+  double *breaking_seconds, *eta_max, *eta_min, *wave_max, *wave_min, *harm;
+  double *wave_sum, *wave_first, *wave_last;
+};
+__global__ void k_record(RecordArgs a);
 
 // Relaxation::apply (forcing.rs:259-283): cell m of the zone list relaxes by the factor
 // factor[rate[m]] (exp(-rate dt), evaluated on the host per distinct rate) towards

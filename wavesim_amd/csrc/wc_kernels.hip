@@ -555,10 +555,10 @@ __global__ void __launch_bounds__(NT) k_cg_reduce(SgnArgs a, int kind) {
 }
 
 // The end of a solve: the counters of DispersionStats (dispersion.rs:541-545).
-__global__ void k_cg_stats(CgScalars* sc) {
+__global__ void k_cg_stats(CgScalars* sc, unsigned long long earlier) {
   if (threadIdx.x == 0) {
     sc->solves += 1;
-    sc->iterations += sc->it;
+    sc->iterations += earlier + sc->it;
     if (!sc->converged) sc->unconverged += 1;
   }
 }

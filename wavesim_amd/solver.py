@@ -87,6 +87,14 @@ SIGNATURES = {
     "wc_solver_capture": (_I32, [_VP, _I32] + [ctypes.POINTER(ctypes.c_float)] * 5
                           + [ctypes.POINTER(_I32)]),
     "wc_solver_residual_history": (_I32, [_VP, _P, _U64, ctypes.POINTER(_U64)]),
+    "wc_stats_new": (_I32, [_VP, _D, _D, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(_VP)]),
+    "wc_stats_free": (None, [_VP]),
+    "wc_stats_record": (_I32, [_VP, _D]),
+    "wc_stats_read": (_I32, [_VP, _P, ctypes.POINTER(_U64), _P, _P, _P, _P, _P, _P]),
+    "wc_stats_harmonic": (_I32, [_VP, ctypes.POINTER(_I32), _P, _P]),
+    "wc_watch_init": (_I32, [ctypes.c_void_p, _D]),
+    "wc_watch_collapsed": (_I32, [ctypes.c_void_p, _D, _P]),
+    "wc_stage_step": (_I32, [_VP, _VP, ctypes.c_void_p, _D, _P, ctypes.POINTER(_I32), _P]),
     "wc_relaxation_new": (_I32, [_VP, ctypes.POINTER(CSponge), _U64, ctypes.POINTER(_VP)]),
     "wc_relaxation_free": (None, [_VP]),
     "wc_relaxation_len": (_U64, [_VP]),
@@ -396,6 +404,96 @@ class Solver:
         _check(lib().wc_profile_read(self._h, names, 1024, ms, n, 64, ctypes.byref(cnt)))
         keys = names.value.decode().split(";")
         return {k: (ms[q], int(n[q])) for q, k in enumerate(keys[: cnt.value])}
+
+
+STAGE_OK, STAGE_UNSTABLE, STAGE_NO_WATER, STAGE_COLLAPSED = 0, 1, 2, 3
+WATCH_STEPS = 1000
+COLLAPSE = 5.0
+
+
+class CWatch(ctypes.Structure):
+    _fields_ = [("reference", ctypes.c_double), ("steps", ctypes.c_uint32),
+                ("seconds", ctypes.c_double)]
+
+
+class StepWatch:
+    """wavesim's StepWatch (crates/wavesim/src/lib.rs:849-896): the time step averaged over
+    WATCH_STEPS steps, checked for a collapse to a COLLAPSE-th of the still sea's."""
+
+    def __init__(self, reference: float):
+        self._c = CWatch()
+        _check(lib().wc_watch_init(ctypes.byref(self._c), float(reference)))
+
+    def collapsed(self, stable: float):
+        """Counts a step; the ratio at the end of a collapsed block, else None."""
+        r = ctypes.c_double()
+        _check(lib().wc_watch_collapsed(ctypes.byref(self._c), float(stable), ctypes.byref(r)))
+        return r.value if r.value > 0.0 else None
+
+
+class Stats:
+    """wavesim's per-step statistics (Stats and Stage::record, crates/wavesim/src/lib.rs:899-932,
+    1276-1300) accumulated on the GPU for the solver's state: recording from time ``from_``,
+    waves of ``period`` s; ``elevation`` (nx*ny float32, the caller's bed elevation) or None for
+    the solver's bed."""
+
+    def __init__(self, solver: "Solver", from_: float, period: float, elevation=None):
+        self.solver = solver
+        self.n = solver.grid.nx * solver.grid.ny
+        el = None
+        if elevation is not None:
+            self._el = np.ascontiguousarray(elevation, dtype=np.float32)
+            el = self._el.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        h = _VP()
+        _check(lib().wc_stats_new(solver._h, float(from_), float(period), el, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().wc_stats_free(self._h)
+            self._h = None
+
+    def record(self, dt: float) -> None:
+        _check(lib().wc_stats_record(self._h, float(dt)))
+
+    def read(self) -> dict:
+        out = {k: np.empty(self.n) for k in ("breaking_seconds", "eta_max", "eta_min",
+                                              "wave_mean", "wave_first", "wave_last")}
+        sec = ctypes.c_double()
+        waves = _U64()
+        _check(lib().wc_stats_read(self._h, ctypes.byref(sec), ctypes.byref(waves),
+                                   *(_ptr(out[k]) for k in ("breaking_seconds", "eta_max",
+                                                            "eta_min", "wave_mean", "wave_first",
+                                                            "wave_last"))))
+        out["seconds"] = sec.value
+        out["waves"] = int(waves.value)
+        out["breaking_fraction"] = out["breaking_seconds"] / max(sec.value, 5e-324)
+        return out
+
+    def harmonic(self):
+        """FirstHarmonic::amplitudes: (a, -b) per cell, or None."""
+        ok = _I32()
+        a, b = np.empty(self.n), np.empty(self.n)
+        _check(lib().wc_stats_harmonic(self._h, ctypes.byref(ok), _ptr(a), _ptr(b)))
+        return (a, b) if ok.value else None
+
+
+def stage_step(solver: "Solver", s: State, stats: "Stats | None", watch: "StepWatch | None",
+               until: float):
+    """Stage::next_dt + advance + record (crates/wavesim/src/lib.rs:1385-1434), no frames:
+    (dt, status, ratio) with status STAGE_OK when a step of dt was taken."""
+    solver._hold(s)
+    dt, ratio = ctypes.c_double(), ctypes.c_double()
+    status = _I32()
+    _check(lib().wc_stage_step(solver._h, stats._h if stats else None,
+                               ctypes.byref(watch._c) if watch else None, float(until),
+                               ctypes.byref(dt), ctypes.byref(status), ctypes.byref(ratio)))
+    if status.value == STAGE_OK:
+        tt = ctypes.c_double()
+        _check(lib().wc_state_get(solver._h, None, None, None, None, ctypes.byref(tt)))
+        s.time = tt.value
+        s._host_valid = False
+    return dt.value, status.value, ratio.value
 
 
 def relaxation_targets(target, n: int) -> np.ndarray:
