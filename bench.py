@@ -52,6 +52,22 @@ BYTES_PER_CELL = {
     },
 }
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "cg_iter_rough_bed_1024x512_pmc.md")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary (FETCH_SIZE
+    and WRITE_SIZE, each scaled by what the counter reads for a copy of known size), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith(f"traffic_bytes_per_launch {kernel} "):
+                    return float(line.split()[2])
+    except OSError:
+        pass
+    return None
+
+
 def parse():
     p = argparse.ArgumentParser(description=__doc__)
     p.add_argument("--gpus", type=int, default=1)
@@ -90,7 +106,9 @@ def calibrate_copy():
 
 def measure(w, precision, arrays, t0, warmup, steps, profile_steps, pcg32=None, network=None):
     """Time `steps` steps of one precision from the given state; per-kernel times with HIP
-    events on the solver's stream."""
+    events on the solver's stream: for the one-kernel PCG iteration of the FP32 form, one pair
+    of events around each batch of its launches inside the timed window itself; for the kernel
+    table, a pair around every launch over `profile_steps` further steps."""
     import torch
 
     from wavesim_amd.grid import State
@@ -108,11 +126,18 @@ def measure(w, precision, arrays, t0, warmup, steps, profile_steps, pcg32=None, 
     s = State(h.copy(), hu.copy(), hv.copy(), br.copy(), t0)
     g.run_steps(s, warmup)
     st0 = g.dispersion_stats()
+    one_kernel = precision == "fp32" and network is None and pcg32 in (None, 4)
+    if one_kernel:
+        g.profile(2)
+        g.profile_reset()
     torch.cuda.synchronize()
     t = time.perf_counter()
     done, last_dt = g.run_steps(s, steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t
+    timed = g.profile_read() if one_kernel else {}
+    if one_kernel:
+        g.profile(0)
     st1 = g.dispersion_stats()
     if done != steps:
         raise SystemExit(f"{precision}: the run stopped after {done} steps (dt = {last_dt})")
@@ -128,9 +153,16 @@ def measure(w, precision, arrays, t0, warmup, steps, profile_steps, pcg32=None, 
     table = BYTES_PER_CELL[precision]
     data = {k: v for k, v in fam.items() if table.get(k, 0) > 0}
     dominant = max(data, key=lambda k: data[k][0])
-    tot_ms, launches = data[dominant]
+    window = "profile"
+    if one_kernel and timed.get("cg_iter", (0, 0))[1] > 0:
+        dominant, window = "cg_iter", "timed"
+        tot_ms, launches = timed["cg_iter"]
+    else:
+        tot_ms, launches = data[dominant]
     avg_ms = tot_ms / launches
-    achieved = table[dominant] * n_cells / (avg_ms * 1e-3) / 1e9
+    # a launch after its solve has converged returns at once: count only the iterations done
+    useful = min(iters, launches) if window == "timed" and iters > 0 else launches
+    achieved = table[dominant] * n_cells * useful / (tot_ms * 1e-3) / 1e9
     kernels = {k: {"total_ms": round(v[0], 4), "launches": v[1],
                    "avg_us": round(1e3 * v[0] / v[1], 3),
                    "bytes_per_cell": table.get(k, 0)} for k, v in sorted(fam.items())}
@@ -143,6 +175,7 @@ def measure(w, precision, arrays, t0, warmup, steps, profile_steps, pcg32=None, 
         "value": n_cells * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
         "pcg_iters_per_solve": iters / max(solves, 1), "dominant": dominant,
         "avg_ms": avg_ms, "achieved": achieved, "bytes_per_cell": table[dominant],
+        "launches": launches, "iterations": iters, "window": window,
         "kernels": kernels, "solver": g, "state": s, "precond_fallbacks": fallbacks,
     }
 
@@ -275,8 +308,11 @@ def main():
         "pcg_iters_per_solve": r["pcg_iters_per_solve"],
         "roofline": {"bound": "hbm", "kernel": r["dominant"], "achieved": r["achieved"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["achieved"] / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_cell": r["bytes_per_cell"],
-                     "avg_launch_us": 1e3 * r["avg_ms"]},
+                     "traffic": measured_traffic(r["dominant"]),
+                     "algorithmic_bytes_per_launch": r["bytes_per_cell"] * w.grid.nx * w.grid.ny,
+                     "bytes_per_cell": r["bytes_per_cell"],
+                     "avg_launch_us": 1e3 * r["avg_ms"], "launches": r["launches"],
+                     "pcg_iterations": r["iterations"], "window": r["window"]},
         "paths": {p: {"value": v["value"], "ms_per_step": v["ms_per_step"],
                       "pcg_iters_per_solve": v["pcg_iters_per_solve"],
                       "precond_fallbacks": v["precond_fallbacks"], "kernels": v["kernels"],

@@ -201,6 +201,10 @@ int wc_solver_residual_history(wc_solver* s, double* out, uint64_t max, uint64_t
  *                    seconds (indicator >= 0.8), eta max and min, the mean, first and last wave
  *                    heights (crest to trough per period).
  *   wc_stats_harmonic FirstHarmonic::amplitudes: (a, -b) per cell, *ok = 0 for None.
+ *   wc_stats_finish  the end of a run (Stage::finish, lib.rs:1451-1453): closes the wave in
+ *                    progress if its period has ended (to within 1e-9 s).
+ * Freeing the solver first leaves its statistics detached: their calls then fail with
+ * WC_EINVAL and wc_stats_free stays safe.
  * StepWatch (lib.rs:849-896): wc_watch_init with the still sea's step; wc_watch_collapsed counts
  * a stable step and sets *ratio > 0 at the end of a block of WC_WATCH_STEPS whose mean step was
  * more than WC_COLLAPSE times shorter. wc_stage_step is Stage::next_dt + advance + record
@@ -226,6 +230,7 @@ int wc_stats_read(wc_stats* st, double* seconds, uint64_t* waves, double* breaki
                   double* eta_max, double* eta_min, double* wave_mean, double* wave_first,
                   double* wave_last);
 int wc_stats_harmonic(wc_stats* st, int32_t* ok, double* a, double* b);
+int wc_stats_finish(wc_stats* st);
 int wc_watch_init(wc_watch_t* w, double reference);
 int wc_watch_collapsed(wc_watch_t* w, double stable, double* ratio);
 int wc_stage_step(wc_solver* s, wc_stats* st, wc_watch_t* w, double until, double* dt,
@@ -254,10 +259,12 @@ int wc_relaxation_cells(const wc_relaxation* r, uint64_t* i, uint64_t* j);
 int wc_relaxation_apply(wc_relaxation* r, double dt, const double* targets);
 int wc_relaxation_apply_device(wc_relaxation* r, double dt, const double* targets);
 
-/* Measurement hooks for benchmarks: average device time in milliseconds of each kernel family
- * over the steps since the last reset, measured with HIP events on the solver's stream, and the
- * kernel launch counts. names receives a ';'-separated list. Timing adds events around every
- * launch; leave it off (the default) outside measurements. */
+/* Measurement hooks for benchmarks: device time in milliseconds of each kernel family over the
+ * steps since the last reset, measured with HIP events on the solver's stream, and the kernel
+ * launch counts. names receives a ';'-separated list. on = 1 puts a pair of events around every
+ * launch; on = 2 times only the one-kernel PCG iteration of the FP32 form ("cg_iter"), one pair
+ * of events around each batch of its back-to-back launches, which leaves a timed run nearly
+ * undisturbed; 0 (the default) times nothing. */
 int wc_profile_enable(wc_solver* s, int32_t on);
 int wc_profile_reset(wc_solver* s);
 int wc_profile_read(wc_solver* s, char* names, uint64_t names_len, double* total_ms,

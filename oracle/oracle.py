@@ -102,6 +102,7 @@ def lib():
             "wco_stats_record": (_I32, [_VP, _D]),
             "wco_stats_read": (_I32, [_VP, _P, ctypes.POINTER(_U64), _P, _P, _P, _P, _P, _P]),
             "wco_stats_harmonic": (_I32, [_VP, ctypes.POINTER(_I32), _P, _P]),
+            "wco_stats_finish": (_I32, [_VP]),
             "wco_linear_wave": (_I32, [_D, _D, _I32, ctypes.POINTER(CLinearWave)]),
             "wco_manning_factor": (_D, [_D, _D, _D, _D]),
             "wco_face_values": (None, [_P, _D, _P, _P]),
@@ -358,6 +359,10 @@ class OracleStats:
     def record(self, s: State, dt: float) -> None:
         self.solver._upload(s)
         _check(lib().wco_stats_record(self._h, float(dt)))
+
+    def finish(self) -> None:
+        """Stage::finish (lib.rs:1451-1453), after the last record."""
+        _check(lib().wco_stats_finish(self._h))
 
     def read(self) -> dict:
         keys = ("breaking_seconds", "eta_max", "eta_min", "wave_mean", "wave_first", "wave_last")

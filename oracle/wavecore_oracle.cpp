@@ -228,10 +228,14 @@ struct Drive {
     if (!linear_wave(omega, still_depth, dispersive, &lw))
       return fail("the wave is too short for the dispersive model at the wave-maker") == 0;
     const double k = lw.k, c = lw.group_speed;
-    ky = k * sin(s.angle);
-    const double kx = k * cos(s.angle);
+    // angle.sin() and angle.cos(): one sincos call, as LLVM lowers the pair for glibc (its
+    // last bit can differ from separate sin and cos calls)
+    double sa, ca;
+    sincos(s.angle, &sa, &ca);
+    ky = k * sa;
+    const double kx = k * ca;
     const double rolloff = exp(0.5 * kx * kx * s.sigma * s.sigma);
-    const double flux = 2.0 * c * s.amplitude * cos(s.angle) * rolloff;
+    const double flux = 2.0 * c * s.amplitude * ca * rolloff;
     double norm = -0.0;  // Rust's float Sum folds from -0.0
     for (size_t i = 0; i < g.nx; ++i) {
       const double x = ((double)i + 0.5) * g.dx;
@@ -1205,6 +1209,24 @@ int wco_stats_new(wco_solver* s, double from, double period, const float* elevat
 
 void wco_stats_free(wco_stats* st) { delete st; }
 
+namespace {
+// Stats::close_wave (lib.rs:918-932)
+void close_wave(wco_stats* st) {
+  std::vector<double> heights(st->wave_max.size());
+  for (size_t n = 0; n < heights.size(); ++n)
+    heights[n] = std::isfinite(st->wave_max[n]) ? st->wave_max[n] - st->wave_min[n] : 0.0;
+  st->waves.push_back(heights);
+  std::fill(st->wave_max.begin(), st->wave_max.end(), -INFINITY);
+  std::fill(st->wave_min.begin(), st->wave_min.end(), INFINITY);
+  st->next_wave += st->period;
+}
+}  // namespace
+
+int wco_stats_finish(wco_stats* st) {
+  if (st->s->s.time >= st->next_wave - 1e-9) close_wave(st);
+  return 0;
+}
+
 int wco_stats_record(wco_stats* st, double dt) {
   wco_solver* s = st->s;
   const Grid& g = s->g;
@@ -1212,16 +1234,9 @@ int wco_stats_record(wco_stats* st, double dt) {
   if (inside <= 0.0) return 0;
   std::vector<double> now(g.nx * g.ny);
   wco_breaking_indicator(s, now.data());
-  if (s->s.time > st->next_wave + 1e-9) {  // close_wave
-    std::vector<double> heights(now.size());
-    for (size_t n = 0; n < now.size(); ++n)
-      heights[n] = std::isfinite(st->wave_max[n]) ? st->wave_max[n] - st->wave_min[n] : 0.0;
-    st->waves.push_back(heights);
-    std::fill(st->wave_max.begin(), st->wave_max.end(), -INFINITY);
-    std::fill(st->wave_min.begin(), st->wave_min.end(), INFINITY);
-    st->next_wave += st->period;
-  }
-  const double sn = sin(st->omega * s->s.time), cs = cos(st->omega * s->s.time);
+  if (s->s.time > st->next_wave + 1e-9) close_wave(st);
+  double sn, cs;  // (omega t).sin_cos(), one sincos call as above
+  sincos(st->omega * s->s.time, &sn, &cs);
   const double v[6] = {1.0, cs, sn, cs * cs, cs * sn, sn * sn};
   for (int m = 0; m < 6; ++m) st->moments[m] += inside * v[m];
   const double w[3] = {inside, inside * cs, inside * sn};

@@ -121,6 +121,8 @@ int wco_stats_read(wco_stats* st, double* seconds, uint64_t* waves, double* brea
                    double* eta_max, double* eta_min, double* wave_mean, double* wave_first,
                    double* wave_last);
 int wco_stats_harmonic(wco_stats* st, int32_t* ok, double* a, double* b);
+// Stage::finish (lib.rs:1451-1453): the wave in progress closes if its period has ended.
+int wco_stats_finish(wco_stats* st);
 
 int wco_linear_wave(double omega, double h, int32_t dispersive, wco_linear_wave_t* out);
 double wco_manning_factor(double h, double speed, double dt, double n);

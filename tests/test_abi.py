@@ -83,3 +83,20 @@ def test_library_exports_the_network_api():
     assert unet.param_count() == uo.param_count()
     for d, (name, taps, ci, co, _) in zip(unet.layers(), uo.LAYERS):
         assert (d["name"], d["taps"], d["cin"], d["cout"]) == (name, taps, ci, co)
+
+
+RUN_HEADER = os.path.join(ROOT, "include", "wavesim_run.h")
+
+
+def test_library_exports_the_run_writer():
+    from wavesim_amd import run
+
+    text = re.sub(r"/\*.*?\*/", "", open(RUN_HEADER).read(), flags=re.S)
+    names = sorted(set(re.findall(r"\b(wc_[a-z0-9_]+)\s*\(", text)))
+    assert "wc_run_create" in names and "wc_run_frame_from" in names
+    assert sorted(run.SIGNATURES) == names
+    if os.path.exists(gpu.LIB_PATH):
+        out = subprocess.run(["nm", "-D", "--defined-only", gpu.LIB_PATH], capture_output=True,
+                             text=True, check=True).stdout
+        exported = set(re.findall(r"\bT (wc_[a-z0-9_]+)$", out, flags=re.M))
+        assert [n for n in names if n not in exported] == []

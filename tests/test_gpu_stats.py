@@ -65,3 +65,30 @@ def test_stage_step_reports_what_the_reference_treats_as_errors():
     s = State.lake_at_rest(grid, bed, 0.0)
     dt, status, _ = stage_step(g, s, None, None, 10.0)
     assert status == STAGE_NO_WATER and math.isinf(dt)
+
+
+def test_statistics_outliving_their_solver_fail_cleanly():
+    """Freeing the solver first (as a garbage collector may) leaves the statistics and the
+    relaxation zones detached: their calls raise, their own free is safe."""
+    import gc
+
+    from wavesim_amd.config import Sponge
+    from wavesim_amd.grid import Bathymetry, Grid
+    from wavesim_amd.solver import Relaxation, _check, _ptr, lib
+
+    grid = Grid(16, 8, 1.0, 1.0)
+    bed = Bathymetry.flat(grid, 2.0)
+    g = Solver(grid, bed)
+    st = Stats(g, 0.0, 5.0)
+    rl = Relaxation(g, [Sponge(3, 0.5).edges(True, False, False, False)])
+    assert len(rl) == 3 * 8
+    Solver.__del__(g)  # the solver's own free, while the Python objects still exist
+    with pytest.raises(ValueError, match="freed"):
+        st.record(0.1)
+    with pytest.raises(ValueError, match="freed"):
+        st.harmonic()
+    targets = np.zeros(3 * len(rl))
+    with pytest.raises(ValueError, match="freed"):
+        _check(lib().wc_relaxation_apply(rl._r, 0.1, _ptr(targets)))
+    del st, rl
+    gc.collect()

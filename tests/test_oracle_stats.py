@@ -44,6 +44,13 @@ def test_first_harmonic_and_wave_heights_of_a_pure_oscillation():
     assert np.allclose(r["eta_max"], mean + crest, atol=1e-3)
     assert np.allclose(r["eta_min"], mean - crest, atol=1e-3)
     assert np.all(r["breaking_seconds"] == 0.0)
+    # Stage::finish closes the third wave, which ended with the last step (t = 36 s)
+    st.finish()
+    r3 = st.read()
+    assert r3["waves"] == 3 and np.array_equal(r3["wave_first"], r["wave_first"])
+    assert np.allclose(r3["wave_last"], r["wave_last"], rtol=1e-3)
+    st.finish()  # nothing more is due
+    assert st.read()["waves"] == 3
 
 
 def test_harmonic_is_none_before_half_a_period():

@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <limits>
 #include <memory>
 #include <string>
 #include <vector>
@@ -19,6 +20,7 @@
 #include "../../include/wavecore_hip.h"
 #include "../../include/wavecore_unet.h"
 #include "wc_kernels.h"
+#include "wc_run.h"
 
 namespace wc {
 int unet_body(wcu_net* n, hipStream_t st, const int* skip);
@@ -65,12 +67,14 @@ enum Family {
   F_RELAX,
   F_CG_ITER,
   F_NET,
+  F_FRAME,
   F_COUNT
 };
 const char* kFamilyNames[F_COUNT] = {"fill_state", "switch",    "flux",      "finish_euler",
                                      "sgn_setup",  "cg_init",   "cg_matvec", "cg_update",
                                      "cg_pupdate", "cg_stats",  "sgn_finish", "reduce",
-                                     "cg_reduce",  "relax",     "cg_iter",   "net"};
+                                     "cg_reduce",  "relax",     "cg_iter",   "net",
+                                     "frame"};
 
 struct HostSponge {
   uint64_t width;
@@ -182,6 +186,10 @@ struct wc_solver {
   unsigned int* red32 = nullptr;        // [1] first NaN, [2] first max
   double* redf = nullptr;               // [0] volume sum
   std::vector<void*> owned;
+  // statistics and relaxation zones built on this solver: freeing the solver releases their
+  // device memory and leaves them detached, whichever order a caller frees things in
+  std::vector<wc_stats*> stats;
+  std::vector<wc_relaxation*> relaxations;
 
   // pinned host staging
   double* prm_host[PRM_SLOTS] = {nullptr, nullptr, nullptr};
@@ -194,12 +202,18 @@ struct wc_solver {
   unsigned int* red32_host = nullptr;
   double* redf_host = nullptr;
 
-  // profiling
-  bool prof = false;
+  // profiling: prof times every launch with its own pair of events; prof_batch times only the
+  // one-kernel PCG iterations, one pair of events around each batch of back-to-back launches
+  bool prof = false, prof_batch = false;
   double prof_ms[F_COUNT] = {};
   unsigned long long prof_n[F_COUNT] = {};
+  struct PendingTime {
+    int fam;
+    hipEvent_t e0, e1;
+    unsigned long long launches;
+  };
   std::vector<hipEvent_t> ev_pool;
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+  std::vector<PendingTime> ev_pending;
 };
 
 namespace {
@@ -242,7 +256,7 @@ int launch(wc_solver* s, int fam, void (*k)(KArgs...), dim3 grid, dim3 block, Ar
   HIP_OK(hipGetLastError());
   if (s->prof && e0 && e1) {
     HIP_OK(hipEventRecord(e1, s->stream));
-    s->ev_pending.push_back({fam, {e0, e1}});
+    s->ev_pending.push_back({fam, e0, e1, 1});
   }
   return WC_OK;
 }
@@ -258,11 +272,11 @@ int prof_collect(wc_solver* s) {
   HIP_OK(hipStreamSynchronize(s->stream));
   for (auto& p : s->ev_pending) {
     float ms = 0.f;
-    HIP_OK(hipEventElapsedTime(&ms, p.second.first, p.second.second));
-    s->prof_ms[p.first] += ms;
-    s->prof_n[p.first] += 1;
-    s->ev_pool.push_back(p.second.first);
-    s->ev_pool.push_back(p.second.second);
+    HIP_OK(hipEventElapsedTime(&ms, p.e0, p.e1));
+    s->prof_ms[p.fam] += ms;
+    s->prof_n[p.fam] += p.launches;
+    s->ev_pool.push_back(p.e0);
+    s->ev_pool.push_back(p.e1);
   }
   s->ev_pending.clear();
   return WC_OK;
@@ -291,10 +305,14 @@ int build_drive(wc_solver* s, double still_depth) {
     return fail(WC_EINVAL, "a wave of this angular frequency is too short for the dispersive "
                            "model at the wave-maker's depth");
   const double k = lw.k, c = lw.group_speed;
-  const double ky = k * sin(m.angle);
-  const double kx = k * cos(m.angle);
+  // angle.sin() and angle.cos() of one argument: LLVM lowers the pair to glibc's sincos on
+  // x86_64-linux-gnu, whose last bit can differ from sin's and cos's, so call it by name.
+  double sa, ca;
+  sincos(m.angle, &sa, &ca);
+  const double ky = k * sa;
+  const double kx = k * ca;
   const double rolloff = exp(0.5 * kx * kx * m.sigma * m.sigma);
-  const double flux = 2.0 * c * m.amplitude * cos(m.angle) * rolloff;
+  const double flux = 2.0 * c * m.amplitude * ca * rolloff;
   const size_t nx = (size_t)s->geo.nx;
   double norm = -0.0;
   for (size_t i = 0; i < nx; ++i) {
@@ -825,11 +843,22 @@ int pcg1_32(wc_solver* s, int stage, unsigned long long earlier = 0) {
   unsigned long long batch = std::max<unsigned long long>(s->prev_it[stage], 1);
   while (true) {
     const unsigned long long stop = std::min(n + batch, maxit + 1);
+    hipEvent_t b0 = nullptr, b1 = nullptr;
+    if (s->prof_batch) {
+      b0 = take_event(s);
+      b1 = take_event(s);
+      if (b0) HIP_OK(hipEventRecord(b0, s->stream));
+    }
+    const unsigned long long first = n;
     for (; n < stop; ++n) {
       TRY(launch(s, F_CG_ITER, k_cg1_iter, grid, block, a, n));
       if (s->capture)  // K_n leaves r_n in r[n & 1] and w_n in w
         for (int slot = 1; slot < wc_solver::CAP_SLOTS; ++slot)
           if (n == cap_it[slot]) TRY(capture_at(slot, s->g_r[n & 1]));
+    }
+    if (s->prof_batch && b0 && b1) {
+      HIP_OK(hipEventRecord(b1, s->stream));
+      s->ev_pending.push_back({F_CG_ITER, b0, b1, n - first});
     }
     TRY(launch(s, F_CG_REDUCE, k_cg1_check, dim3(1), dim3(64), a, n));
     HIP_OK(hipMemcpyAsync(s->sc_host, s->sc, sizeof(CgScalars), hipMemcpyDeviceToHost,
@@ -901,7 +930,7 @@ int pcg_net32(wc_solver* s, int stage) {
       TRY(unet_body(s->net, s->stream, &s->sc->net_off));
       if (s->prof && e0 && e1) {
         HIP_OK(hipEventRecord(e1, s->stream));
-        s->ev_pending.push_back({F_NET, {e0, e1}});
+        s->ev_pending.push_back({F_NET, e0, e1, 1});
       }
       TRY(launch(s, F_NET, k_fcg_out, grid, block, a, n));
       TRY(launch(s, F_CG_MATVEC, k_fcg_mv, grid, block, a, n));
@@ -984,23 +1013,30 @@ int pcg32(wc_solver* s, int stage) {
   return WC_OK;
 }
 
-int stage32(wc_solver* s, double dt, int stage, int mode) {
+// The argument blocks of one FP32 stage, shared by the single launches below and the batched
+// ones (wc_batch_*).
+BSwitch switch_args32(wc_solver* s, int stage) {
+  BSwitch a{};
+  a.g = s->geo;
+  a.h = stage == 0 ? s->f_sh : s->f_s1h;
+  a.hl = stage == 0 ? s->f_slh : s->f_s1hl;
+  a.b = s->b;
+  a.br = breaking_of(s);
+  a.held = s->has_held ? s->held : nullptr;
+  a.phi = s->f_phi;
+  return a;
+}
+
+BFlux flux_args32(wc_solver* s, int stage) {
   const Geo& g = s->geo;
-  float* in_h = stage == 0 ? s->f_sh : s->f_s1h;
-  const float* in_lo = stage == 0 ? s->f_slh : s->f_s1hl;
-  float* in_hu = stage == 0 ? s->f_shu : s->f_s1hu;
-  float* in_hv = stage == 0 ? s->f_shv : s->f_s1hv;
-  if (s->order == WC_ORDER_FIFTH || s->dispersive)
-    TRY(launch(s, F_SWITCH, k_switch32, over((long long)g.w * g.rows), dim3(NT), g,
-               (const float*)in_h, in_lo, (const double*)s->b, breaking_of(s),
-               (const uint8_t*)(s->has_held ? s->held : nullptr), s->f_phi));
-  FluxTileArgs<float> f{};
+  BFlux a{};
+  FluxTileArgs<float>& f = a.f;
   f.g = g;
   f.order = s->order;
-  f.h = in_h;
-  f.h_lo = in_lo;
-  f.hu = in_hu;
-  f.hv = in_hv;
+  f.h = stage == 0 ? s->f_sh : s->f_s1h;
+  f.h_lo = stage == 0 ? s->f_slh : s->f_s1hl;
+  f.hu = stage == 0 ? s->f_shu : s->f_s1hu;
+  f.hv = stage == 0 ? s->f_shv : s->f_s1hv;
   f.b = s->b;
   f.phi = s->f_phi;
   f.has_drive = s->has_drive ? 1 : 0;
@@ -1010,13 +1046,15 @@ int stage32(wc_solver* s, double dt, int stage, int mode) {
   f.prm = s->prm;
   f.ramp_prm = stage;
   f.sin_prm = 2 + stage * g.ny;
-  TRY(launch(s, F_FLUX, k_flux_tile32, tiles(s), dim3(NT), f, s->f_lh, s->f_lhu, s->f_lhv));
-  if (s->dispersive) {
-    TRY(launch(s, F_SGN_SETUP, k_sgn_setup32, tiles(s), dim3(NT), sgn32_args(s, stage)));
-    TRY(pcg32(s, stage));
-  }
+  a.lh = s->f_lh;
+  a.lhu = s->f_lhu;
+  a.lhv = s->f_lhv;
+  return a;
+}
+
+Stage32 stage_args32(wc_solver* s, double dt, int stage) {
   Stage32 a{};
-  a.g = g;
+  a.g = s->geo;
   a.dt = dt;
   a.b = s->b;
   a.sh = s->f_sh;
@@ -1039,7 +1077,25 @@ int stage32(wc_solver* s, double dt, int stage, int mode) {
   a.wy = s->f_wy;
   a.dispersive = s->dispersive ? 1 : 0;
   a.damp = step_args(s, dt, stage);
-  TRY(launch(s, F_SGN_FINISH, k_stage32, tiles(s), dim3(NT), a, mode));
+  return a;
+}
+
+bool switch_on(const wc_solver* s) { return s->order == WC_ORDER_FIFTH || s->dispersive; }
+
+int stage32(wc_solver* s, double dt, int stage, int mode) {
+  const Geo& g = s->geo;
+  if (switch_on(s)) {
+    const BSwitch w = switch_args32(s, stage);
+    TRY(launch(s, F_SWITCH, k_switch32, over((long long)g.w * g.rows), dim3(NT), g, w.h, w.hl,
+               w.b, w.br, w.held, w.phi));
+  }
+  const BFlux f = flux_args32(s, stage);
+  TRY(launch(s, F_FLUX, k_flux_tile32, tiles(s), dim3(NT), f.f, f.lh, f.lhu, f.lhv));
+  if (s->dispersive) {
+    TRY(launch(s, F_SGN_SETUP, k_sgn_setup32, tiles(s), dim3(NT), sgn32_args(s, stage)));
+    TRY(pcg32(s, stage));
+  }
+  TRY(launch(s, F_SGN_FINISH, k_stage32, tiles(s), dim3(NT), stage_args32(s, dt, stage), mode));
   return WC_OK;
 }
 
@@ -1089,6 +1145,11 @@ int stable_dt32(wc_solver* s, double* dt) {
 namespace wc {
 // The error setter for the other sources of the library (wc_unet.hip).
 int api_fail(int code, const char* msg) { return fail(code, msg); }
+}  // namespace wc
+
+namespace wc {
+void detach(wc_stats* st);       // releases the device memory; the object stays, inert
+void detach(wc_relaxation* r);
 }  // namespace wc
 
 extern "C" {
@@ -1170,6 +1231,8 @@ void wc_solver_free(wc_solver* s) {
   if (!s) return;
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (wc_stats* st : std::vector<wc_stats*>(s->stats)) wc::detach(st);
+  for (wc_relaxation* r : std::vector<wc_relaxation*>(s->relaxations)) wc::detach(r);
   if (s->net) wcu_net_free(s->net);
   for (void* p : s->owned) (void)hipFree(p);
   for (int q = 0; q < PRM_SLOTS; ++q) {
@@ -1182,8 +1245,8 @@ void wc_solver_free(wc_solver* s) {
   if (s->redf_host) (void)hipHostFree(s->redf_host);
   for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
   for (auto& p : s->ev_pending) {
-    (void)hipEventDestroy(p.second.first);
-    (void)hipEventDestroy(p.second.second);
+    (void)hipEventDestroy(p.e0);
+    (void)hipEventDestroy(p.e1);
   }
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -1526,6 +1589,13 @@ int wc_stats_new(wc_solver* s, double from, double period, const float* elevatio
   TRY(set_device(s));
   auto st = std::make_unique<wc_stats>();
   st->s = s;
+  s->stats.push_back(st.get());
+  struct Undo {  // a failed construction leaves nothing behind
+    std::unique_ptr<wc_stats>& st;
+    ~Undo() {
+      if (st) wc::detach(st.get());
+    }
+  } undo{st};
   st->from = from;
   st->period = period;
   st->next_wave = from + period;
@@ -1560,15 +1630,14 @@ int wc_stats_new(wc_solver* s, double from, double period, const float* elevatio
 
 void wc_stats_free(wc_stats* st) {
   if (!st) return;
-  (void)hipSetDevice(st->s->device);
-  (void)hipStreamSynchronize(st->s->stream);
-  for (void* p : st->owned) (void)hipFree(p);
+  wc::detach(st);
   delete st;
 }
 
 // Stage::record (lib.rs:1276-1300) after a step of dt that has just ended.
 int wc_stats_record(wc_stats* st, double dt) {
   if (!st) return fail(WC_EINVAL, "null statistics");
+  if (!st->s) return fail(WC_EINVAL, "the statistics' solver has been freed");
   wc_solver* s = st->s;
   const double inside = std::min(dt, s->time - st->from);
   if (inside <= 0.0) return WC_OK;
@@ -1587,7 +1656,9 @@ int wc_stats_record(wc_stats* st, double dt) {
     st->next_wave += st->period;
   }
   // FirstHarmonic::weights (lib.rs:1118-1128)
-  const double sn = sin(st->omega * s->time), cs = cos(st->omega * s->time);
+  // (omega t).sin_cos(): glibc's sincos, as the wave-maker's angle
+  double sn, cs;
+  sincos(st->omega * s->time, &sn, &cs);
   const double v[6] = {1.0, cs, sn, cs * cs, cs * sn, sn * sn};
   for (int m = 0; m < 6; ++m) st->moments[m] += inside * v[m];
   a.inside = inside;
@@ -1613,6 +1684,7 @@ int wc_stats_read(wc_stats* st, double* seconds, uint64_t* waves, double* breaki
                   double* eta_max, double* eta_min, double* wave_mean, double* wave_first,
                   double* wave_last) {
   if (!st) return fail(WC_EINVAL, "null statistics");
+  if (!st->s) return fail(WC_EINVAL, "the statistics' solver has been freed");
   wc_solver* s = st->s;
   TRY(set_device(s));
   const size_t n = (size_t)s->geo.nx * s->geo.ny;
@@ -1640,6 +1712,7 @@ int wc_stats_read(wc_stats* st, double* seconds, uint64_t* waves, double* breaki
 // returns None.
 int wc_stats_harmonic(wc_stats* st, int32_t* ok, double* a_out, double* b_out) {
   if (!st || !ok) return fail(WC_EINVAL, "null argument");
+  if (!st->s) return fail(WC_EINVAL, "the statistics' solver has been freed");
   wc_solver* s = st->s;
   TRY(set_device(s));
   const size_t n = (size_t)s->geo.nx * s->geo.ny;
@@ -1673,6 +1746,137 @@ int wc_stats_harmonic(wc_stats* st, int32_t* ok, double* a_out, double* b_out) {
   return WC_OK;
 }
 
+// The end of a run (lib.rs:1451-1453): the wave in progress is closed if it is due.
+int wc_stats_finish(wc_stats* st) {
+  if (!st) return fail(WC_EINVAL, "null statistics");
+  if (!st->s) return fail(WC_EINVAL, "the statistics' solver has been freed");
+  wc_solver* s = st->s;
+  if (!(s->time >= st->next_wave - 1e-9)) return WC_OK;
+  TRY(set_device(s));
+  RecordArgs a{};
+  a.g = s->geo;
+  st->waves += 1;
+  a.first_wave = st->waves == 1 ? 1 : 0;
+  st->next_wave += st->period;
+  a.wave_max = st->wave_max;
+  a.wave_min = st->wave_min;
+  a.wave_sum = st->wave_sum;
+  a.wave_first = st->wave_first;
+  a.wave_last = st->wave_last;
+  TRY(launch(s, F_REDUCE, k_close_wave, over((long long)s->geo.nx * s->geo.ny), dim3(NT), a));
+  if (!s->prof) HIP_OK(hipStreamSynchronize(s->stream));
+  return WC_OK;
+}
+
+// Stage::frame_if_due's frame (lib.rs:1348-1380) from the solver's state, through a pinned slot
+// of the run writer; the disk write is the writer thread's.
+int wc_run_frame_from(wc_run_writer* w, wc_solver* s, double tide, double* rise,
+                      int32_t* status) {
+  if (!w || !s || !rise || !status) return fail(WC_EINVAL, "null argument");
+  TRY(set_device(s));
+  TRY(join_state(s));
+  wc::run::DeviceFrame f;
+  TRY(wc::run::begin_device_frame(w, s->device, s->geo.nx, s->geo.ny, &f));
+  const size_t n = f.cells;
+  const int rc = [&]() -> int {
+    HIP_OK(hipMemsetAsync(f.scalars, 0, 2 * sizeof(unsigned long long), s->stream));
+    FrameArgs a{};
+    a.g = s->geo;
+    a.h = s->h;
+    a.b = s->b;
+    a.br = breaking_of(s);
+    a.elevation = f.elevation;
+    a.tide = tide;
+    a.eta = f.device + (size_t)f.eta * n;
+    a.breaking = f.device + (size_t)f.breaking * n;
+    a.rise_bits = f.scalars;
+    a.nonfinite = (unsigned int*)(f.scalars + 1);
+    TRY(launch(s, F_FRAME, k_frame, over((long long)n), dim3(NT), a));
+    HIP_OK(hipMemcpyAsync(f.pinned, f.device, 2 * n * sizeof(float), hipMemcpyDeviceToHost,
+                          s->stream));
+    HIP_OK(hipMemcpyAsync(f.scalars_host, f.scalars, 2 * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    return WC_OK;
+  }();
+  if (rc) {
+    wc::run::abandon_frame(w, f);
+    return rc;
+  }
+  double r;
+  memcpy(&r, &f.scalars_host[0], sizeof r);
+  const bool nonfinite = (unsigned int)f.scalars_host[1] != 0;
+  *rise = r;
+  if (!std::isfinite(r) || r > 100.0 || nonfinite) {
+    wc::run::abandon_frame(w, f);
+    *status = WC_STAGE_UNSTABLE;
+    return WC_OK;
+  }
+  *status = WC_STAGE_OK;
+  return wc::run::commit_frame(w, f, s->time);
+}
+
+// Stage::finish's statistics (lib.rs:1436-1509) without the break line.
+int wc_run_stats_from(wc_run_writer* w, wc_stats* st) {
+  if (!w || !st) return fail(WC_EINVAL, "null argument");
+  if (!st->s) return fail(WC_EINVAL, "the statistics' solver has been freed");
+  wc_solver* s = st->s;
+  TRY(wc_stats_finish(st));
+  if (!(st->seconds > 0.0)) return WC_OK;
+  const long long nx = s->geo.nx, ny = s->geo.ny;
+  const size_t n = (size_t)nx * ny;
+  std::vector<double> bs(n), hi(n), lo(n), mean(n), first(n), last(n), ha(n), hb(n);
+  double seconds = 0.0;
+  uint64_t waves = 0;
+  TRY(wc_stats_read(st, &seconds, &waves, bs.data(), hi.data(), lo.data(), mean.data(),
+                    first.data(), last.data()));
+  int32_t ok = 0;
+  TRY(wc_stats_harmonic(st, &ok, ha.data(), hb.data()));
+  auto f32s = [&](const std::vector<double>& v) {
+    std::vector<float> o(v.size());
+    for (size_t q = 0; q < v.size(); ++q) o[q] = (float)v[q];
+    return o;
+  };
+  // Stage::breaking_fraction (lib.rs:1337-1341)
+  const double t = std::max(seconds, std::numeric_limits<double>::min());
+  std::vector<float> fraction(n);
+  for (size_t q = 0; q < n; ++q) fraction[q] = (float)(bs[q] / t);
+  std::vector<float> high = f32s(hi), low = f32s(lo), height = f32s(mean), re = f32s(ha),
+                     im = f32s(hb);
+  std::vector<const char*> names = {"break_fraction", "eta_max", "eta_min"};
+  std::vector<const float*> blocks = {fraction.data(), high.data(), low.data()};
+  if (waves > 0) {
+    wc::run::note(w, "waves_recorded", wc::json::Value::uint(waves));
+    // Stage::unsettled (lib.rs:1315-1335): the median over the middle half of the width of
+    // |last / first - 1| per period, over the cells whose first wave exceeded 0.3 m
+    if (waves >= 2) {
+      std::vector<double> change;
+      for (long long j = ny / 4; j < ny - ny / 4; ++j)
+        for (long long i = 0; i < nx; ++i) {
+          const size_t m = (size_t)(j * nx + i);
+          if (first[m] > 0.3)
+            change.push_back(std::fabs(last[m] / first[m] - 1.0) / (double)(waves - 1));
+        }
+      if (!change.empty()) {
+        std::sort(change.begin(), change.end(), [](double a, double b) {
+          return wc::json::total_order(a) < wc::json::total_order(b);
+        });
+        wc::run::note(w, "unsettled_per_period", wc::json::Value::number(change[change.size() / 2]));
+      }
+    }
+    names.push_back("wave_height");
+    blocks.push_back(height.data());
+    if (ok) {
+      names.push_back("harmonic_re");
+      names.push_back("harmonic_im");
+      blocks.push_back(re.data());
+      blocks.push_back(im.data());
+    }
+  }
+  return wc_run_write_stats(w, names.data(), blocks.data(), names.size(), nullptr, st->from,
+                            s->time);
+}
+
 // StepWatch (lib.rs:861-896)
 int wc_watch_init(wc_watch_t* w, double reference) {
   if (!w) return fail(WC_EINVAL, "null watch");
@@ -1700,6 +1904,7 @@ int wc_watch_collapsed(wc_watch_t* w, double stable, double* ratio) {
 int wc_stage_step(wc_solver* s, wc_stats* st, wc_watch_t* w, double until, double* dt,
                   int32_t* status, double* ratio) {
   if (!s || !dt || !status) return fail(WC_EINVAL, "null argument");
+  if (st && st->s != s) return fail(WC_EINVAL, "the statistics belong to another solver");
   *status = WC_STAGE_OK;
   if (ratio) *ratio = 0.0;
   double stable = 0.0;
@@ -1784,6 +1989,13 @@ int wc_relaxation_new(wc_solver* s, const wc_sponge_t* zones, uint64_t n_zones,
       rate.push_back((int)q);
     }
   const size_t n = cell.size();
+  s->relaxations.push_back(r.get());
+  struct Undo {  // a failed construction leaves nothing behind
+    std::unique_ptr<wc_relaxation>& r;
+    ~Undo() {
+      if (r) wc::detach(r.get());
+    }
+  } undo{r};
   HIP_OK(hipMalloc(&r->cell, std::max<size_t>(n, 1) * sizeof(int)));
   HIP_OK(hipMalloc(&r->rate, std::max<size_t>(n, 1) * sizeof(int)));
   HIP_OK(hipMalloc(&r->target, std::max<size_t>(3 * n, 1) * sizeof(double)));
@@ -1801,19 +2013,50 @@ int wc_relaxation_new(wc_solver* s, const wc_sponge_t* zones, uint64_t n_zones,
 
 void wc_relaxation_free(wc_relaxation* r) {
   if (!r) return;
-  (void)hipSetDevice(r->s->device);
+  wc::detach(r);
+  delete r;
+}
+
+}  // extern "C"
+
+namespace wc {
+
+void detach(wc_stats* st) {
+  wc_solver* s = st->s;
+  if (!s) return;
+  (void)hipSetDevice(s->device);
+  (void)hipStreamSynchronize(s->stream);
+  for (void* p : st->owned) (void)hipFree(p);
+  st->owned.clear();
+  s->stats.erase(std::remove(s->stats.begin(), s->stats.end(), st), s->stats.end());
+  st->s = nullptr;
+}
+
+void detach(wc_relaxation* r) {
+  wc_solver* s = r->s;
+  if (!s) return;
+  (void)hipSetDevice(s->device);
   if (r->factor_ev) {
     (void)hipEventSynchronize(r->factor_ev);
     (void)hipEventDestroy(r->factor_ev);
+    r->factor_ev = nullptr;
   }
-  (void)hipStreamSynchronize(r->s->stream);
+  (void)hipStreamSynchronize(s->stream);
   (void)hipFree(r->cell);
   (void)hipFree(r->rate);
   (void)hipFree(r->target);
   (void)hipFree(r->factor);
   (void)hipHostFree(r->factor_host);
-  delete r;
+  r->cell = r->rate = nullptr;
+  r->target = r->factor = r->factor_host = nullptr;
+  s->relaxations.erase(std::remove(s->relaxations.begin(), s->relaxations.end(), r),
+                       s->relaxations.end());
+  r->s = nullptr;
 }
+
+}  // namespace wc
+
+extern "C" {
 
 uint64_t wc_relaxation_len(const wc_relaxation* r) { return r ? (uint64_t)r->ci.size() : 0; }
 
@@ -1854,6 +2097,7 @@ int relax_apply(wc_relaxation* r, double dt, const double* targets_device) {
 
 int wc_relaxation_apply(wc_relaxation* r, double dt, const double* targets) {
   if (!r || (!targets && !r->ci.empty())) return fail(WC_EINVAL, "null argument");
+  if (!r->s) return fail(WC_EINVAL, "the relaxation's solver has been freed");
   TRY(set_device(r->s));
   const size_t n = r->ci.size();
   if (n)
@@ -1866,6 +2110,7 @@ int wc_relaxation_apply(wc_relaxation* r, double dt, const double* targets) {
 
 int wc_relaxation_apply_device(wc_relaxation* r, double dt, const double* targets) {
   if (!r || (!targets && !r->ci.empty())) return fail(WC_EINVAL, "null argument");
+  if (!r->s) return fail(WC_EINVAL, "the relaxation's solver has been freed");
   TRY(set_device(r->s));
   TRY(relax_apply(r, dt, targets));
   if (!r->s->prof) HIP_OK(hipStreamSynchronize(r->s->stream));
@@ -2017,7 +2262,8 @@ double wc_manning_factor(double h, double speed, double dt, double n) {
 int wc_profile_enable(wc_solver* s, int32_t on) {
   TRY(set_device(s));
   TRY(prof_collect(s));
-  s->prof = on != 0;
+  s->prof = on == 1;
+  s->prof_batch = on == 2;
   return WC_OK;
 }
 

@@ -127,11 +127,11 @@ struct TileLds {
 // Rates of every tile cell, then the stage epilogue of `mode` (see stage_out).
 // in_lo: the low part of the depth in the FP32 chosen form (null in FP64).
 template <typename R>
-__device__ __forceinline__ void flux_tile(const FluxTileArgs<R>& a, int mode, double& rh,
-                                          double& rhu, double& rhv, int& i_out, int& j_out,
-                                          bool& valid) {
+__device__ __forceinline__ void flux_tile(const FluxTileArgs<R>& a, int mode, int blk, int nblk,
+                                          double& rh, double& rhu, double& rhv, int& i_out,
+                                          int& j_out, bool& valid) {
   const Geo& g = a.g;
-  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  const int tile = tile_of_block(blk, nblk);
   const int tx = tile % g.ntx, ty = tile / g.ntx;
   const int i0 = GH + tx * TX, j0 = GH + ty * TY;
   __shared__ TileLds<R> L;
@@ -219,24 +219,39 @@ __global__ void __launch_bounds__(NT) k_flux_tile64(FluxTileArgs<double> f, Step
   double rh, rhu, rhv;
   int i, j;
   bool valid;
-  flux_tile<double>(f, mode, rh, rhu, rhv, i, j, valid);
+  flux_tile<double>(f, mode, blockIdx.x, gridDim.x, rh, rhu, rhv, i, j, valid);
   if (!valid) return;
   stage_out(a, mode, i, j, j * a.g.w + i, rh, rhu, rhv);
 }
 
 // FP32 chosen form: the rates for the stage update that follows; the mass rate stays FP64 so
 // the volume is conserved to FP64 rounding, the momentum rates are rounded to FP32.
-__global__ void __launch_bounds__(NT) k_flux_tile32(FluxTileArgs<float> f, double* lh, float* lhu,
-                                                    float* lhv) {
+__device__ __forceinline__ void flux32_body(const FluxTileArgs<float>& f, double* lh, float* lhu,
+                                            float* lhv, int blk, int nblk) {
   double rh, rhu, rhv;
   int i, j;
   bool valid;
-  flux_tile<float>(f, MODE_RATES, rh, rhu, rhv, i, j, valid);
+  flux_tile<float>(f, MODE_RATES, blk, nblk, rh, rhu, rhv, i, j, valid);
   if (!valid) return;
   const int k = j * f.g.w + i;
   lh[k] = rh;
   lhu[k] = (float)rhu;
   lhv[k] = (float)rhv;
+}
+
+__global__ void __launch_bounds__(NT) k_flux_tile32(FluxTileArgs<float> f, double* lh, float* lhu,
+                                                    float* lhv) {
+  flux32_body(f, lh, lhu, lhv, blockIdx.x, gridDim.x);
+}
+
+// Many instances in one launch (see kb_fill32).
+__global__ void __launch_bounds__(NT) kb_flux32(const BFlux* args, BMap m) {
+  for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
+    const int q = batch_instance(m, vb);
+    const BFlux& a = args[q];
+    flux32_body(a.f, a.lh, a.lhu, a.lhv, vb - m.first[q], m.first[q + 1] - m.first[q]);
+    __syncthreads();  // the next virtual block reuses the LDS
+  }
 }
 
 }  // namespace wc

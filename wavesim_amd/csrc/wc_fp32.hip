@@ -55,9 +55,10 @@ __global__ void __launch_bounds__(NT) k_join32(long long n, const float* hi, con
 
 // Ghost layers of the FP32 state (both parts of h; the high parts of hu, hv, which is all the
 // rates read of them).
-__global__ void __launch_bounds__(NT) k_fill32(Geo g, float* h, float* hl, float* hu, float* hv) {
+__device__ __forceinline__ void fill32_body(const Geo& g, float* h, float* hl, float* hu,
+                                            float* hv, int blk) {
   const int n = 6 * g.w + 6 * g.ny;
-  const int t = blockIdx.x * NT + threadIdx.x;
+  const int t = blk * NT + threadIdx.x;
   if (t >= n) return;
   int i, j;
   if (t < 3 * g.w) {
@@ -81,11 +82,15 @@ __global__ void __launch_bounds__(NT) k_fill32(Geo g, float* h, float* hl, float
   hv[k] = (!g.periodic && is_ghost_y(j, g.ny)) ? -hv[m] : hv[m];
 }
 
+__global__ void __launch_bounds__(NT) k_fill32(Geo g, float* h, float* hl, float* hu, float* hv) {
+  fill32_body(g, h, hl, hu, hv, blockIdx.x);
+}
+
 // The switch (dispersion.rs:120-160) in FP64 from the FP32 pair, stored in FP32.
-__global__ void __launch_bounds__(NT) k_switch32(Geo g, const float* h, const float* hl,
-                                                 const double* b, Breaking br,
-                                                 const uint8_t* held, float* phi) {
-  const int t = blockIdx.x * NT + threadIdx.x;
+__device__ __forceinline__ void switch32_body(const Geo& g, const float* h, const float* hl,
+                                              const double* b, const Breaking& br,
+                                              const uint8_t* held, float* phi, int blk) {
+  const int t = blk * NT + threadIdx.x;
   if (t >= g.w * g.rows) return;
   const int i = t % g.w, j = t / g.w;
   const int mi = map_x(i, g.nx), mj = map_y(j, g.ny, g.periodic);
@@ -107,10 +112,16 @@ __global__ void __launch_bounds__(NT) k_switch32(Geo g, const float* h, const fl
   phi[t] = (float)v;
 }
 
+__global__ void __launch_bounds__(NT) k_switch32(Geo g, const float* h, const float* hl,
+                                                 const double* b, Breaking br,
+                                                 const uint8_t* held, float* phi) {
+  switch32_body(g, h, hl, b, br, held, phi, blockIdx.x);
+}
+
 // SGN setup (dispersion.rs:385-490) on FP32 storage; arithmetic in FP64.
-__global__ void __launch_bounds__(NT) k_sgn_setup32(Sgn32 a) {
+__device__ __forceinline__ void sgn_setup32_body(const Sgn32& a, int blk, int nblk) {
   const Geo& g = a.g;
-  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  const int tile = tile_of_block(blk, nblk);
   const int i0 = tile_i(g, tile), j0 = tile_j(g, tile);
   __shared__ double s_h[SY3f][SX3f], s_um[SY3f][SX3f], s_vm[SY3f][SX3f];
   __shared__ double s_div[SY2f][SX2f];
@@ -229,6 +240,10 @@ __device__ __forceinline__ void matvec32_tail(const Sgn32& a, int i0, int j0,
   py = s_py[ly + 2][lx + 2];
   ax = mask * (heff * px - (s_f[y1][x1 + 1] - s_f[y1][x1 - 1]) * ix);
   ay = mask * (heff * py - (s_f[y1 + 1][x1] - s_f[y1 - 1][x1]) * iy);
+}
+
+__global__ void __launch_bounds__(NT) k_sgn_setup32(Sgn32 a) {
+  sgn_setup32_body(a, blockIdx.x, gridDim.x);
 }
 
 // r = b - A w, p = z = r / diag, and b.b, r.z, r.r (dispersion.rs:493-514).
@@ -355,9 +370,10 @@ __global__ void __launch_bounds__(NT) k_cg_update32(Sgn32 a, const float* pcx, c
 //   MODE_FINAL  s2 = clean(s1 + dt L); s = s + dt (L1 + L) / 2 where nothing was cleaned, else
 //               clean((s + s2) / 2); then damp, on the FP32 pair
 //   MODE_EULER  s = damp(clean(s + dt L))
-__global__ void __launch_bounds__(NT) k_stage32(Stage32 a, int mode) {
+__device__ __forceinline__ void stage32_body(const Stage32& a, double dt, int mode, int blk,
+                                             int nblk) {
   const Geo& g = a.g;
-  const int tile = tile_of_block(blockIdx.x, gridDim.x);
+  const int tile = tile_of_block(blk, nblk);
   const int i = tile_i(g, tile) + (threadIdx.x & 63);
   const int j = tile_j(g, tile) + (threadIdx.x >> 6);
   if (i >= g.nx + GH || j >= g.ny + GH) return;
@@ -378,7 +394,7 @@ __global__ void __launch_bounds__(NT) k_stage32(Stage32 a, int mode) {
   const double shu = (double)a.shu[k] + (double)a.sl_hu[k];
   const double shv = (double)a.shv[k] + (double)a.sl_hv[k];
   if (mode == MODE_S1) {
-    double h = sh + a.dt * lh, hu = shu + a.dt * lhu, hv = shv + a.dt * lhv;
+    double h = sh + dt * lh, hu = shu + dt * lhu, hv = shv + dt * lhv;
     clean_dry(h, hu, hv);
     float hi, lo;
     split(h, hi, lo);
@@ -393,20 +409,20 @@ __global__ void __launch_bounds__(NT) k_stage32(Stage32 a, int mode) {
   }
   double h, hu, hv;
   if (mode == MODE_EULER) {
-    h = sh + a.dt * lh;
-    hu = shu + a.dt * lhu;
-    hv = shv + a.dt * lhv;
+    h = sh + dt * lh;
+    hu = shu + dt * lhu;
+    hv = shv + dt * lhv;
     clean_dry(h, hu, hv);
   } else {
     const double s1h = (double)a.s1h[k] + (double)a.s1hl[k];
     const double s1hu = (double)a.s1hu[k], s1hv = (double)a.s1hv[k];
-    double h2 = s1h + a.dt * lh, hu2 = s1hu + a.dt * lhu, hv2 = s1hv + a.dt * lhv;
-    const bool cleaned = !(sh + a.dt * a.l1h[k] >= WC_H_DRY) || !(h2 >= WC_H_DRY);
+    double h2 = s1h + dt * lh, hu2 = s1hu + dt * lhu, hv2 = s1hv + dt * lhv;
+    const bool cleaned = !(sh + dt * a.l1h[k] >= WC_H_DRY) || !(h2 >= WC_H_DRY);
     clean_dry(h2, hu2, hv2);
     if (!cleaned) {
-      h = sh + 0.5 * a.dt * (a.l1h[k] + lh);
-      hu = shu + 0.5 * a.dt * ((double)a.l1hu[k] + lhu);
-      hv = shv + 0.5 * a.dt * ((double)a.l1hv[k] + lhv);
+      h = sh + 0.5 * dt * (a.l1h[k] + lh);
+      hu = shu + 0.5 * dt * ((double)a.l1hu[k] + lhu);
+      hv = shv + 0.5 * dt * ((double)a.l1hv[k] + lhv);
     } else {
       h = 0.5 * (sh + h2);
       hu = 0.5 * (shu + hu2);
@@ -414,7 +430,7 @@ __global__ void __launch_bounds__(NT) k_stage32(Stage32 a, int mode) {
     }
     clean_dry(h, hu, hv);
   }
-  damp(a.damp, i, j, a.b[k], h, hu, hv);
+  damp(a.damp, dt, i, j, a.b[k], h, hu, hv);
   float hi, lo;
   split(h, hi, lo);
   a.sh[k] = hi;
@@ -427,12 +443,16 @@ __global__ void __launch_bounds__(NT) k_stage32(Stage32 a, int mode) {
   a.sl_hv[k] = lo;
 }
 
+__global__ void __launch_bounds__(NT) k_stage32(Stage32 a, int mode) {
+  stage32_body(a, a.dt, mode, blockIdx.x, gridDim.x);
+}
+
 // The largest rate (stable_dt) of the FP32 pair state.
-__global__ void __launch_bounds__(NT) k_max_rate32(Geo g, const float* h, const float* hl,
-                                                   const float* hu, const float* hul,
-                                                   const float* hv, const float* hvl,
-                                                   unsigned long long* out) {
-  const int t = blockIdx.x * NT + threadIdx.x;
+__device__ __forceinline__ void max_rate32_body(const Geo& g, const float* h, const float* hl,
+                                                const float* hu, const float* hul,
+                                                const float* hv, const float* hvl,
+                                                unsigned long long* out, int blk) {
+  const int t = blk * NT + threadIdx.x;
   unsigned long long bits = 0ull;
   if (t < g.nx * g.ny) {
     const int k = (GH + t / g.nx) * g.w + GH + t % g.nx;
@@ -449,6 +469,13 @@ __global__ void __launch_bounds__(NT) k_max_rate32(Geo g, const float* h, const 
     bits = o > bits ? o : bits;
   }
   if ((threadIdx.x & 63) == 0 && bits != 0ull) atomicMax(out, bits);
+}
+
+__global__ void __launch_bounds__(NT) k_max_rate32(Geo g, const float* h, const float* hl,
+                                                   const float* hu, const float* hul,
+                                                   const float* hv, const float* hvl,
+                                                   unsigned long long* out) {
+  max_rate32_body(g, h, hl, hu, hul, hv, hvl, out, blockIdx.x);
 }
 
 // The second stage of the PCG reductions for the FP32 form (same recurrences as k_cg_reduce).
@@ -482,6 +509,62 @@ __global__ void __launch_bounds__(NT) k_cg_reduce32(Sgn32 a, int kind) {
     sc->beta = s[1] / sc->rz;
     sc->rz = s[1];
     sc->done = (sc->converged || sc->it >= a.max_iterations) ? 1 : 0;
+  }
+}
+
+// ---- many instances in one launch ----------------------------------------------------------
+// The grid walks every instance's virtual blocks, gridDim.x at a time; a virtual block runs the
+// single-instance body with its instance's arguments, so each instance computes exactly what
+// its own launch would.
+__global__ void __launch_bounds__(NT) kb_fill32(const BFill* args, BMap m) {
+  for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
+    const int q = batch_instance(m, vb);
+    const BFill& a = args[q];
+    fill32_body(a.g, a.h, a.hl, a.hu, a.hv, vb - m.first[q]);
+  }
+}
+
+__global__ void __launch_bounds__(NT) kb_switch32(const BSwitch* args, BMap m) {
+  for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
+    const int q = batch_instance(m, vb);
+    const BSwitch& a = args[q];
+    switch32_body(a.g, a.h, a.hl, a.b, a.br, a.held, a.phi, vb - m.first[q]);
+  }
+}
+
+__global__ void __launch_bounds__(NT) kb_sgn_setup32(const Sgn32* args, BMap m) {
+  for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
+    const int q = batch_instance(m, vb);
+    sgn_setup32_body(args[q], vb - m.first[q], m.first[q + 1] - m.first[q]);
+    __syncthreads();  // the next virtual block reuses the LDS
+  }
+}
+
+__global__ void __launch_bounds__(NT) kb_stage32(const Stage32* args, const double* dt, int mode,
+                                                 BMap m) {
+  for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
+    const int q = batch_instance(m, vb);
+    stage32_body(args[q], dt[q], mode, vb - m.first[q], m.first[q + 1] - m.first[q]);
+  }
+}
+
+__global__ void __launch_bounds__(NT) kb_max_rate32(const BRate* args, BMap m) {
+  for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
+    const int q = batch_instance(m, vb);
+    const BRate& a = args[q];
+    max_rate32_body(a.g, a.h, a.hl, a.hu, a.hul, a.hv, a.hvl, a.out, vb - m.first[q]);
+  }
+}
+
+// Per-step host values of every instance, staged in one buffer: instance q's segment
+// [off[q], off[q+1]) goes to dst[q].
+__global__ void __launch_bounds__(NT) kb_scatter(const double* src, const long long* off,
+                                                 double* const* dst, int n) {
+  for (int q = 0; q < n; ++q) {
+    const long long len = off[q + 1] - off[q];
+    for (long long t = (long long)blockIdx.x * NT + threadIdx.x; t < len;
+         t += (long long)gridDim.x * NT)
+      dst[q][t] = src[off[q] + t];
   }
 }
 

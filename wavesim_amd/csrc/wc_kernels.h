@@ -13,6 +13,23 @@ struct Breaking {
   double er_lo, er_hi, sl_lo, sl_hi;
 };
 
+// breaking_into's indicator at interior cell (i, j), padded index k (dispersion.rs:183-201):
+// 1 - phi with phi the switch of the thresholds br, 0 where the water is shallower than
+// H_DISPERSION_MIN.
+__device__ __forceinline__ double breaking_at(const double* h, const double* b, const Geo& g,
+                                              const Breaking& br, int i, int j, int k) {
+  const double hc = h[k];
+  if (hc < WC_H_DISPERSION_MIN) return 0.0;
+  const double ix = 0.5 / g.dx, iy = 0.5 / g.dy;
+  const double ex = ((ghosted(h, g, i + 1, j) + b[k + 1]) - (ghosted(h, g, i - 1, j) + b[k - 1])) * ix;
+  const double ey =
+      ((ghosted(h, g, i, j + 1) + b[k + g.w]) - (ghosted(h, g, i, j - 1) + b[k - g.w])) * iy;
+  const double ratio = wc_max((hc + b[k]) / hc, 0.0);
+  const double off = wc_max(wc_smoothstep(br.er_lo, br.er_hi, ratio),
+                            wc_smoothstep(br.sl_lo, br.sl_hi, wc_hypot(ex, ey)));
+  return 1.0 - (1.0 - off);
+}
+
 // One sponge as the kernels use it: its table of rates and per-step factors, indexed by the
 // distance in cells to the nearest active edge (forcing.rs:181-206).
 struct SpongeDev {
@@ -257,6 +274,24 @@ struct RecordArgs {
   double *wave_sum, *wave_first, *wave_last;
 };
 __global__ void k_record(RecordArgs a);
+// Stats::close_wave alone (lib.rs:918-932), for the end of a run (lib.rs:1451-1453).
+__global__ void k_close_wave(RecordArgs a);
+
+// Stage::frame_if_due's frame (crates/wavesim/src/lib.rs:1348-1380) for interior cell n:
+// eta[n] = (h + elevation[n]) as f32, breaking[n] = the indicator as f32; rise_bits gets the
+// largest (eta as f64) - tide over the cells deeper than 1e-3 m (as the bits of a non-negative
+// double, from 0), nonfinite is set where an eta is not finite.
+struct FrameArgs {
+  Geo g;
+  const double *h, *b;
+  Breaking br;
+  const float* elevation;
+  double tide;
+  float *eta, *breaking;
+  unsigned long long* rise_bits;
+  unsigned int* nonfinite;
+};
+__global__ void k_frame(FrameArgs a);
 
 // Relaxation::apply (forcing.rs:259-283): cell m of the zone list relaxes by the factor
 // factor[rate[m]] (exp(-rate dt), evaluated on the host per distinct rate) towards
@@ -272,5 +307,58 @@ struct RelaxArgs {
 __global__ void k_relax64(RelaxArgs a, double* h, double* hu, double* hv);
 __global__ void k_relax32(RelaxArgs a, float* h, float* hl, float* hu, float* hul, float* hv,
                           float* hvl);
+
+// ---- many instances in one launch (wc_fp32.hip, wc_flux.hip, wc_pcg1.hip: kb_*) -------------
+// Instance q owns virtual blocks [first[q], first[q+1]) of `total`; first lives in device
+// memory. Argument blocks are arrays in device memory, one per instance.
+struct BMap {
+  const int* first;
+  int n, total;
+};
+__device__ __forceinline__ int batch_instance(const BMap& m, int vb) {
+  int lo = 0, hi = m.n;  // first[lo] <= vb < first[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (m.first[mid] <= vb)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+struct BFill {
+  Geo g;
+  float *h, *hl, *hu, *hv;
+};
+struct BSwitch {
+  Geo g;
+  const float *h, *hl;
+  const double* b;
+  Breaking br;
+  const uint8_t* held;
+  float* phi;
+};
+struct BFlux {
+  FluxTileArgs<float> f;
+  double* lh;
+  float *lhu, *lhv;
+};
+struct BRate {
+  Geo g;
+  const float *h, *hl, *hu, *hul, *hv, *hvl;
+  unsigned long long* out;
+};
+__global__ void kb_fill32(const BFill* args, BMap m);
+__global__ void kb_switch32(const BSwitch* args, BMap m);
+__global__ void kb_flux32(const BFlux* args, BMap m);
+__global__ void kb_sgn_setup32(const Sgn32* args, BMap m);
+__global__ void kb_stage32(const Stage32* args, const double* dt, int mode, BMap m);
+__global__ void kb_max_rate32(const BRate* args, BMap m);
+__global__ void kb_scatter(const double* src, const long long* off, double* const* dst, int n);
+__global__ void kb_cg1_init(const CgG* args, BMap m);
+__global__ void kb_cg1_start(const CgG* args, BMap m);
+__global__ void kb_cg1_iter(const CgG* args, unsigned long long n, BMap m);
+__global__ void kb_cg1_check(const CgG* args, unsigned long long n, int stats, int* done,
+                             unsigned long long* it);
 
 }  // namespace wc

@@ -87,20 +87,7 @@ __global__ void __launch_bounds__(NT) k_breaking(Geo g, const double* h, const d
   const int t = blockIdx.x * NT + threadIdx.x;
   if (t >= g.nx * g.ny) return;
   const int i = GH + t % g.nx, j = GH + t / g.nx;
-  const int k = j * g.w + i;
-  const double hc = h[k];
-  if (hc < WC_H_DISPERSION_MIN) {
-    out[t] = 0.0;
-    return;
-  }
-  const double ix = 0.5 / g.dx, iy = 0.5 / g.dy;
-  const double ex = ((ghosted(h, g, i + 1, j) + b[k + 1]) - (ghosted(h, g, i - 1, j) + b[k - 1])) * ix;
-  const double ey =
-      ((ghosted(h, g, i, j + 1) + b[k + g.w]) - (ghosted(h, g, i, j - 1) + b[k - g.w])) * iy;
-  const double ratio = wc_max((hc + b[k]) / hc, 0.0);
-  const double off = wc_max(wc_smoothstep(br.er_lo, br.er_hi, ratio),
-                            wc_smoothstep(br.sl_lo, br.sl_hi, wc_hypot(ex, ey)));
-  out[t] = 1.0 - (1.0 - off);
+  out[t] = breaking_at(h, b, g, br, i, j, j * g.w + i);
 }
 
 // ---- the finite-volume rates (solver.rs:625-718) ----------------------------------------------
@@ -152,7 +139,7 @@ __global__ void __launch_bounds__(NT) k_finish_euler(StepArgs a) {
   if (i >= g.nx + GH || j >= g.ny + GH) return;
   const int k = j * g.w + i;
   double h = a.s1_h[k], hu = a.s1_hu[k], hv = a.s1_hv[k];
-  damp(a, i, j, a.b[k], h, hu, hv);
+  damp(a, a.dt, i, j, a.b[k], h, hu, hv);
   a.s_h[k] = h;
   a.s_hu[k] = hu;
   a.s_hv[k] = hv;

@@ -35,7 +35,7 @@ using namespace t16;
 
 // The scalars of iteration i = n - 1, from K_{n-1}'s lanes (or the start's); false when the
 // solve is over. Every wave computes the same numbers; block 0 records them.
-__device__ __forceinline__ bool start(const CgG& a, unsigned long long n, double& alpha,
+__device__ __forceinline__ bool start(const CgG& a, unsigned long long n, int blk, double& alpha,
                                       double& beta) {
   CgScalars* sc = a.sc;
   const unsigned long long i = n - 1;
@@ -60,7 +60,7 @@ __device__ __forceinline__ bool start(const CgG& a, unsigned long long n, double
     beta = gamma / sc->rzs[(i - 1) & 1];
     alpha = gamma / (delta - beta * gamma / sc->alphas[(i - 1) & 1]);
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blk == 0 && threadIdx.x == 0) {
     if (n == 1) sc->target = target;
     sc->converged = conv;
     sc->it = i;
@@ -77,14 +77,14 @@ __device__ __forceinline__ float2 f2(double x, double y) { return make_float2((f
 
 // r_0 = b - A w (w zero where masked, as set up), the inverse Jacobi diagonal
 // (dispersion.rs:466-480), p = s_{-1} = 0, lanes of b.b.
-__global__ void __launch_bounds__(PT) k_cg1_init(CgG a) {
+__device__ __forceinline__ void cg1_init_body(const CgG& a, int blk) {
   const Geo& g = a.g;
   const int nt = tiles_all(g);
   __shared__ double s_u[2][S2Y][S2X], s_f[S1Y][S1X];
   __shared__ double sums[2][BY][1];
   double acc[1] = {0.0};
-  for (int r = 0; (int)blockIdx.x + LANES * r < nt; ++r) {
-    const int tile = blockIdx.x + LANES * r;
+  for (int r = 0; blk + LANES * r < nt; ++r) {
+    const int tile = blk + LANES * r;
     bool own;
     int k;
     double ux, uy, ax, ay;
@@ -112,18 +112,20 @@ __global__ void __launch_bounds__(PT) k_cg1_init(CgG a) {
     }
     add_rows<1>(v, sums, r, acc);
   }
-  if (threadIdx.x == 0) a.lanes_b[blockIdx.x] = acc[0];
+  if (threadIdx.x == 0) a.lanes_b[blk] = acc[0];
 }
 
+__global__ void __launch_bounds__(PT) k_cg1_init(CgG a) { cg1_init_body(a, blockIdx.x); }
+
 // u_0 = D^-1 r_0, q_0 = A u_0; lanes of gamma_0 = r.u, delta_0 = q.u, r.r (slot 0).
-__global__ void __launch_bounds__(PT) k_cg1_start(CgG a) {
+__device__ __forceinline__ void cg1_start_body(const CgG& a, int blk) {
   const Geo& g = a.g;
   const int nt = tiles_all(g);
   __shared__ double s_u[2][S2Y][S2X], s_f[S1Y][S1X];
   __shared__ double sums[2][BY][3];
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int r = 0; (int)blockIdx.x + LANES * r < nt; ++r) {
-    const int tile = blockIdx.x + LANES * r;
+  for (int r = 0; blk + LANES * r < nt; ++r) {
+    const int tile = blk + LANES * r;
     bool own;
     int k;
     double ux, uy, ax, ay;
@@ -147,15 +149,17 @@ __global__ void __launch_bounds__(PT) k_cg1_start(CgG a) {
     add_rows<3>(v, sums, r, acc);
   }
   if (threadIdx.x == 0)
-    for (int m = 0; m < 3; ++m) a.lanes[m * LANES + blockIdx.x] = acc[m];
+    for (int m = 0; m < 3; ++m) a.lanes[m * LANES + blk] = acc[m];
 }
 
+__global__ void __launch_bounds__(PT) k_cg1_start(CgG a) { cg1_start_body(a, blockIdx.x); }
+
 // Iteration i = n - 1 (see the top of the file).
-__global__ void __launch_bounds__(PT) k_cg1_iter(CgG a, unsigned long long n) {
+__device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n, int blk) {
   const Geo& g = a.g;
   const int nt = tiles_all(g);
   double alpha = 0.0, beta = 0.0;
-  if ((int)blockIdx.x < nt && !start(a, n, alpha, beta)) return;
+  if (blk < nt && !start(a, n, blk, alpha, beta)) return;
   const unsigned long long i = n - 1;
   const int cur = (int)(i & 1), nxt = cur ^ 1;
   const float2 *r0 = a.r[cur], *q0 = a.q[cur];
@@ -164,8 +168,8 @@ __global__ void __launch_bounds__(PT) k_cg1_iter(CgG a, unsigned long long n) {
   __shared__ double s_u[2][S2Y][S2X], s_f[S1Y][S1X];
   __shared__ double sums[2][BY][3];
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int r = 0; (int)blockIdx.x + LANES * r < nt; ++r) {
-    const int tile = blockIdx.x + LANES * r;
+  for (int r = 0; blk + LANES * r < nt; ++r) {
+    const int tile = blk + LANES * r;
     bool own;
     int k;
     double ux, uy, ax, ay;
@@ -202,13 +206,64 @@ __global__ void __launch_bounds__(PT) k_cg1_iter(CgG a, unsigned long long n) {
     add_rows<3>(v, sums, r, acc);
   }
   if (threadIdx.x == 0)
-    for (int m = 0; m < 3; ++m) a.lanes[3 * LANES * (n & 1) + m * LANES + blockIdx.x] = acc[m];
+    for (int m = 0; m < 3; ++m) a.lanes[3 * LANES * (n & 1) + m * LANES + blk] = acc[m];
+}
+
+__global__ void __launch_bounds__(PT) k_cg1_iter(CgG a, unsigned long long n) {
+  cg1_iter_body(a, n, blockIdx.x);
 }
 
 // K_n's prologue alone, one wave: whether the solve is over before iteration n - 1.
 __global__ void __launch_bounds__(64) k_cg1_check(CgG a, unsigned long long n) {
   double alpha, beta;
-  start(a, n, alpha, beta);
+  start(a, n, 0, alpha, beta);
+}
+
+// ---- many instances in one launch (see kb_fill32) -------------------------------------------
+// Instance q runs min(256, its tiles) virtual blocks; its lanes past those stay zero, as the
+// single launch's empty blocks write them, so the sums and every iterate are the same.
+__global__ void __launch_bounds__(PT) kb_cg1_init(const CgG* args, BMap m) {
+  for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
+    const int q = batch_instance(m, vb);
+    cg1_init_body(args[q], vb - m.first[q]);
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(PT) kb_cg1_start(const CgG* args, BMap m) {
+  for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
+    const int q = batch_instance(m, vb);
+    cg1_start_body(args[q], vb - m.first[q]);
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(PT) kb_cg1_iter(const CgG* args, unsigned long long n, BMap m) {
+  for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
+    const int q = batch_instance(m, vb);
+    cg1_iter_body(args[q], n, vb - m.first[q]);
+    __syncthreads();
+  }
+}
+
+// One wave per instance: k_cg1_check, then the instance's flag and iteration into done[q],
+// it[q]; with `stats` (the solve is over) DispersionStats counts the solve as k_cg_stats does.
+__global__ void __launch_bounds__(64) kb_cg1_check(const CgG* args, unsigned long long n,
+                                                   int stats, int* done, unsigned long long* it) {
+  const int q = blockIdx.x;
+  const CgG& a = args[q];
+  double alpha, beta;
+  start(a, n, 0, alpha, beta);
+  if (threadIdx.x == 0) {
+    CgScalars* sc = a.sc;
+    done[q] = sc->done;
+    it[q] = sc->it;
+    if (stats) {
+      sc->solves += 1;
+      sc->iterations += sc->it;
+      if (!sc->converged) sc->unconverged += 1;
+    }
+  }
 }
 
 }  // namespace wc

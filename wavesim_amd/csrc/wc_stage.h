@@ -16,8 +16,8 @@ __device__ __forceinline__ void clean_dry(double& h, double& hu, double& hv) {
 }
 
 // damp (solver.rs:535-574): the sponges in order, then Manning friction, then clean_dry.
-__device__ __forceinline__ void damp(const StepArgs& a, int i, int j, double b, double& h,
-                                     double& hu, double& hv) {
+__device__ __forceinline__ void damp(const StepArgs& a, double dt, int i, int j, double b,
+                                     double& h, double& hu, double& hv) {
   const int ii = i - GH, jj = j - GH;
   for (int m = 0; m < a.n_sponges; ++m) {
     const SpongeDev& sp = a.sponges[m];
@@ -40,7 +40,7 @@ __device__ __forceinline__ void damp(const StepArgs& a, int i, int j, double b, 
   if (a.manning > 0.0 && h > WC_H_FRICTION_MIN) {
     const double speed = wc_hypot(hu, hv) / h;
     const double kk = WC_G * a.manning * a.manning / wc_pow43(h);
-    const double fac = 1.0 / (1.0 + a.dt * kk * speed);
+    const double fac = 1.0 / (1.0 + dt * kk * speed);
     hu *= fac;
     hv *= fac;
   }
@@ -74,7 +74,7 @@ __device__ __forceinline__ void stage_out(const StepArgs& a, int mode, int i, in
   double shu = 0.5 * (a.s_hu[k] + hu);
   double shv = 0.5 * (a.s_hv[k] + hv);
   clean_dry(sh, shu, shv);
-  damp(a, i, j, a.b[k], sh, shu, shv);
+  damp(a, a.dt, i, j, a.b[k], sh, shu, shv);
   a.s_h[k] = sh;
   a.s_hu[k] = shu;
   a.s_hv[k] = shv;

@@ -92,6 +92,7 @@ SIGNATURES = {
     "wc_stats_record": (_I32, [_VP, _D]),
     "wc_stats_read": (_I32, [_VP, _P, ctypes.POINTER(_U64), _P, _P, _P, _P, _P, _P]),
     "wc_stats_harmonic": (_I32, [_VP, ctypes.POINTER(_I32), _P, _P]),
+    "wc_stats_finish": (_I32, [_VP]),
     "wc_watch_init": (_I32, [ctypes.c_void_p, _D]),
     "wc_watch_collapsed": (_I32, [ctypes.c_void_p, _D, _P]),
     "wc_stage_step": (_I32, [_VP, _VP, ctypes.c_void_p, _D, _P, ctypes.POINTER(_I32), _P]),
@@ -390,8 +391,10 @@ class Solver:
         return out.value
 
     # -- measurement ---------------------------------------------------------------------------
-    def profile(self, on: bool) -> None:
-        _check(lib().wc_profile_enable(self._h, int(on)))
+    def profile(self, mode: int) -> None:
+        """0: off; 1 (True): events around every launch; 2: events around each batch of the
+        FP32 form's one-kernel PCG iterations only."""
+        _check(lib().wc_profile_enable(self._h, int(mode)))
 
     def profile_reset(self) -> None:
         _check(lib().wc_profile_reset(self._h))
@@ -469,6 +472,11 @@ class Stats:
         out["waves"] = int(waves.value)
         out["breaking_fraction"] = out["breaking_seconds"] / max(sec.value, 5e-324)
         return out
+
+    def finish(self) -> None:
+        """The end of a run (crates/wavesim/src/lib.rs:1451-1453): closes the wave in progress
+        if its period has ended."""
+        _check(lib().wc_stats_finish(self._h))
 
     def harmonic(self):
         """FirstHarmonic::amplitudes: (a, -b) per cell, or None."""
