@@ -75,7 +75,9 @@ def parse():
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--spinup", type=int, default=1500)
     p.add_argument("--workload", default="rough_bed",
-                   choices=["rough_bed", "plane_beach", "large_reef"])
+                   choices=["rough_bed", "plane_beach", "large_reef", "sweep"])
+    p.add_argument("--instances", type=int, default=96,
+                   help="sweep: the number of reef-sheet instances (seeds 3000, 3001, ...)")
     p.add_argument("--profile-steps", type=int, default=20)
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="CPU work per thread count for the baseline sample")
@@ -223,6 +225,69 @@ def cpu_baseline(w, state_arrays, t0, seconds):
     return cores, out
 
 
+def sweep(args):
+    """The batched spot-sheet sweep: reef-sheet instances of uneven size (wavesim_amd.configs
+    .c3_reef_instance, seeds 3000...), FP32 form, all stepped together by wc_batch_run_steps;
+    then the same solvers one at a time for comparison. cell-updates/s counts every instance's
+    interior cells."""
+    import torch
+
+    from wavesim_amd import configs
+    from wavesim_amd.solver import PRECISION_FP32, Batch, Solver
+
+    ws = [configs.c3_reef_instance(seed) for seed in range(3000, 3000 + args.instances)]
+    solvers = [w.solver(Solver).with_precision(PRECISION_FP32) for w in ws]
+    states = [w.still_water() for w in ws]
+    cells = sum(w.grid.nx * w.grid.ny for w in ws)
+    b = Batch(solvers)
+    t = time.perf_counter()
+    b.run_steps(states, args.spinup)
+    print(f"[bench] sweep: {len(ws)} instances, {cells / 1e6:.1f} M cells, spin-up "
+          f"{args.spinup} steps in {time.perf_counter() - t:.1f} s, launch widths "
+          f"{b.launch_widths()}", file=sys.stderr, flush=True)
+    b.run_steps(states, args.warmup)
+    st0 = [g.dispersion_stats() for g in solvers]
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    done, _ = b.run_steps(states, args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t
+    st1 = [g.dispersion_stats() for g in solvers]
+    if min(done) != args.steps:
+        raise SystemExit(f"sweep: an instance stopped after {min(done)} steps")
+    iters = sum(b1.iterations - b0.iterations for b0, b1 in zip(st0, st1))
+    solves = sum(b1.solves - b0.solves for b0, b1 in zip(st0, st1))
+    value = cells * args.steps / elapsed
+    print(f"[bench] sweep batched: {args.steps} steps in {elapsed:.3f} s = {value / 1e6:.1f} M "
+          f"cell-updates/s, {iters / max(solves, 1):.1f} PCG iterations per solve",
+          file=sys.stderr, flush=True)
+    alone_steps = max(args.steps // 5, 2)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for g, s in zip(solvers, states):
+        g.run_steps(s, alone_steps)
+    torch.cuda.synchronize()
+    el1 = time.perf_counter() - t
+    v1 = cells * alone_steps / el1
+    print(f"[bench] sweep one at a time: {alone_steps} steps in {el1:.3f} s = {v1 / 1e6:.1f} M "
+          f"cell-updates/s", file=sys.stderr, flush=True)
+    sizes = {}
+    for w in ws:
+        sizes[f"{w.grid.nx}x{w.grid.ny}"] = sizes.get(f"{w.grid.nx}x{w.grid.ny}", 0) + 1
+    print(json.dumps({
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"reef_sheet_sweep_{len(ws)}", "instances": len(ws),
+                   "cells": cells, "sizes": sizes, "spinup_steps": args.spinup,
+                   "path": "FP32 form, one launch per phase over all instances",
+                   "launch_widths": list(b.launch_widths())},
+        "pcg_iters_per_solve": iters / max(solves, 1),
+        "one_at_a_time": {"value": v1, "steps": alone_steps, "batched_over_alone": value / v1},
+        "roofline": None, "cpu_baseline": None}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -240,6 +305,9 @@ def main():
     torch.cuda.init()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.workload == "sweep":
+        sweep(args)
         return
     w = {"rough_bed": configs.c1_rough_bed, "plane_beach": configs.c0_plane_beach,
          "large_reef": configs.c4_large_reef}[args.workload]()

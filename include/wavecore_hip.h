@@ -259,6 +259,26 @@ int wc_relaxation_cells(const wc_relaxation* r, uint64_t* i, uint64_t* j);
 int wc_relaxation_apply(wc_relaxation* r, double dt, const double* targets);
 int wc_relaxation_apply_device(wc_relaxation* r, double dt, const double* targets);
 
+/* Many instances in one launch (the batched sweep of independent runs: beds, swell angles,
+ * tides, grids of uneven size). A batch groups existing solvers of one device, all in the FP32
+ * form with the one-kernel Jacobi PCG, sharing the time stepping (Euler or RK2), the switch and
+ * the dispersion; each keeps its own grid, bed, settings, state and dt. wc_batch_run_steps
+ * advances every instance `steps` times with its own stable dt (as wc_run_steps does for one):
+ * every phase of a step is one launch over all instances, whose blocks walk each instance's
+ * tiles, so an instance ends bit for bit where it would alone. An instance whose dt is not
+ * finite and positive stops there (done_steps[q], last_dt[q] per instance, both may be NULL).
+ * While a batch steps its solvers, they must not be used on their own; afterwards their states
+ * are read with wc_state_get as usual. The first run tunes the launch widths (persistent grids
+ * of the 256- and 1024-thread launches) by timing candidates; wc_batch_launch_widths reports the
+ * choice, wc_batch_set_option("grid_nt" | "grid_pt", w) fixes it, ("tune", 1) tunes again,
+ * ("pcg_batch", k) sets the iterations queued between looks at the PCG flags. */
+typedef struct wc_batch wc_batch;
+int wc_batch_new(wc_solver* const* solvers, uint64_t n, wc_batch** out);
+void wc_batch_free(wc_batch* b);
+int wc_batch_set_option(wc_batch* b, const char* key, int64_t value);
+int wc_batch_launch_widths(const wc_batch* b, int32_t* grid_nt, int32_t* grid_pt);
+int wc_batch_run_steps(wc_batch* b, uint64_t steps, uint64_t* done_steps, double* last_dt);
+
 /* Measurement hooks for benchmarks: device time in milliseconds of each kernel family over the
  * steps since the last reset, measured with HIP events on the solver's stream, and the kernel
  * launch counts. names receives a ';'-separated list. on = 1 puts a pair of events around every

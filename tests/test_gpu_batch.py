@@ -1,0 +1,123 @@
+"""Many instances in one launch (wc_batch_*): every instance of a batch must end exactly where
+it ends when stepped alone (the FP32 form with the one-kernel PCG is deterministic, and a
+batched launch runs each instance's blocks with the single launch's arithmetic), whatever the
+persistent launch widths; an instance whose step stops being finite stops alone."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from wavesim_amd import configs
+from wavesim_amd.grid import State
+from wavesim_amd.solver import PRECISION_FP32, Batch, Solver
+
+pytestmark = pytest.mark.gpu
+
+
+def workloads():
+    """Four reef-sheet instances of uneven size (seeded like the sweep, the smaller sizes) and a
+    periodic rough bed."""
+    out = []
+    seed = 3000
+    while len(out) < 4:
+        w = configs.c3_reef_instance(seed)
+        if w.grid.nx <= 768 and all(o.grid.nx != w.grid.nx or o.notes["lobes"] != w.notes["lobes"]
+                                    for o in out):
+            out.append(w)
+        seed += 1
+    out.append(configs.c1_rough_bed(nx=320, ny=96, seed=5))
+    return out
+
+
+def fresh(w, arrays=None, t0=0.0):
+    g = w.solver(Solver).with_precision(PRECISION_FP32)
+    s = w.still_water() if arrays is None else State(*(a.copy() for a in arrays), t0)
+    return g, s
+
+
+def run_alone(ws, starts, steps):
+    out = []
+    for w, (arr, t0) in zip(ws, starts):
+        g, s = fresh(w, arr, t0)
+        n, last = g.run_steps(s, steps)
+        out.append((n, s.time, [a.copy() for a in s.arrays()[:3]], g.dispersion_stats()))
+    return out
+
+
+def run_batched(ws, starts, steps, widths=None, graph=True):
+    pairs = [fresh(w, arr, t0) for w, (arr, t0) in zip(ws, starts)]
+    b = Batch([g for g, _ in pairs])
+    b.set_option("graph", int(graph))
+    if widths:
+        b.set_option("grid_nt", widths[0]).set_option("grid_pt", widths[1])
+    done, last = b.run_steps([s for _, s in pairs], steps)
+    res = [(d, s.time, [a.copy() for a in s.arrays()[:3]], g.dispersion_stats())
+           for d, (g, s) in zip(done, pairs)]
+    return res, b.launch_widths(), last
+
+
+def spun_up(ws, steps):
+    starts = []
+    for w in ws:
+        g, s = fresh(w)
+        g.run_steps(s, steps)
+        starts.append(([a.copy() for a in s.arrays()], s.time))
+    return starts
+
+
+def assert_same(alone, batched):
+    for k, (a, b) in enumerate(zip(alone, batched)):
+        assert a[0] == b[0], k
+        assert a[1] == b[1], k
+        for x, y in zip(a[2], b[2]):
+            assert np.array_equal(x, y), k
+        assert a[3] == b[3], k
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_a_batch_steps_each_instance_as_it_steps_alone(graph):
+    """With the fixed stretches of each step replayed from HIP graphs (the default) and
+    without; compared after every 5 steps, so that every replay between two looks counts."""
+    ws = workloads()
+    starts = spun_up(ws, 120)
+    pairs = [fresh(w, arr, t0) for w, (arr, t0) in zip(ws, starts)]
+    solo = [fresh(w, arr, t0) for w, (arr, t0) in zip(ws, starts)]
+    b = Batch([g for g, _ in pairs])
+    b.set_option("graph", int(graph))
+    for chunk in range(5):
+        done, _ = b.run_steps([s for _, s in pairs], 5)
+        assert done == [5] * len(ws)
+        for k, ((g, s), (g1, s1)) in enumerate(zip(pairs, solo)):
+            assert g1.run_steps(s1, 5)[0] == 5
+            assert s.time == s1.time, (chunk, k)
+            for x, y in zip(s.arrays()[:3], s1.arrays()[:3]):
+                assert np.array_equal(x, y), (chunk, k)
+            assert g.dispersion_stats() == g1.dispersion_stats(), (chunk, k)
+    widths = b.launch_widths()
+    print("instances", [(w.grid.nx, w.grid.ny) for w in ws], "widths", widths,
+          "iterations", [g.dispersion_stats().iterations for g, _ in solo], flush=True)
+    assert all(g.dispersion_stats().iterations > 0 for g, _ in solo)
+    assert widths[0] > 0 and widths[1] > 0
+
+
+@pytest.mark.parametrize("widths", [(1, 1), (37, 5), (100000, 100000)])
+def test_the_launch_width_changes_nothing(widths):
+    ws = workloads()[:3]
+    starts = spun_up(ws, 60)
+    alone = run_alone(ws, starts, 8)
+    batched, got, _ = run_batched(ws, starts, 8, widths)
+    assert got == widths
+    assert_same(alone, batched)
+
+
+def test_an_instance_that_stops_stops_alone():
+    ws = workloads()[:3]
+    starts = spun_up(ws, 40)
+    h = starts[1][0][0].copy()
+    h[ws[1].grid.idx(20, 20)] = np.nan
+    starts[1] = ([h] + starts[1][0][1:], starts[1][1])
+    alone = run_alone([ws[0], ws[2]], [starts[0], starts[2]], 10)
+    batched, _, last = run_batched(ws, starts, 10)
+    assert batched[1][0] == 0 and np.isnan(last[1])
+    assert_same(alone, [batched[0], batched[2]])

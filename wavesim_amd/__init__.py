@@ -19,6 +19,7 @@ from .config import (  # noqa: F401
 from .grid import GHOST, G, H_DRY, THREADED_CELLS, Bathymetry, Grid, State  # noqa: F401
 from .solver import (  # noqa: F401,E402
     PRECISION_FP32,
+    Batch,
     PRECISION_FP64,
     DeviceError,
     Relaxation,

@@ -164,8 +164,9 @@ def gaussian_lobes(rng: Lcg, n: int, xr, yr, height, width):
 
 def c3_reef_instance(seed: int, dx: float = 3.0) -> Workload:
     """One instance of the batched spot-sheet sweep: a size drawn from {256x128, 512x256,
-    768x384, 1024x512, 1536x768}, a 1:30 plane plus 1-4 Gaussian reef lobes (4-10 m high,
-    40-200 m wide) plus a 3 m channel, swell angle -20..20 degrees, tide -0.5..0.5 m (the bed
+    768x384, 1024x512, 1536x768}, a 1:30 plane (levelling off at 40 m depth, so that the
+    wave-maker of the larger grids stays within the dispersive model's range) plus 1-4 Gaussian
+    reef lobes (4-10 m high, 40-200 m wide) plus a 3 m channel, swell angle -20..20 degrees, tide -0.5..0.5 m (the bed
     shifted down by the tide, as waveio's Bed::bathymetry does, crates/waveio/src/bed.rs:239-246),
     H 1-3 m, T 10-18 s; fifth order + SGN, 2000 steps."""
     rng = Lcg(seed)
@@ -185,7 +186,7 @@ def c3_reef_instance(seed: int, dx: float = 3.0) -> Workload:
     ch_y = rng.uniform(0.2 * ly, 0.8 * ly)
 
     def bed_fn(x, y):
-        plane = (x - shore) / 30.0
+        plane = np.maximum((x - shore) / 30.0, -40.0)
         channel = -3.0 * np.exp(-((y - ch_y) / 30.0) ** 2) * (x > 0.4 * lx)
         return np.minimum(plane + lobes(x, y), plane + 2.0) + channel - tide
 

@@ -102,6 +102,11 @@ SIGNATURES = {
     "wc_relaxation_cells": (_I32, [_VP, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
     "wc_relaxation_apply": (_I32, [_VP, _D, _P]),
     "wc_relaxation_apply_device": (_I32, [_VP, _D, _VP]),
+    "wc_batch_new": (_I32, [ctypes.POINTER(_VP), _U64, ctypes.POINTER(_VP)]),
+    "wc_batch_free": (None, [_VP]),
+    "wc_batch_set_option": (_I32, [_VP, ctypes.c_char_p, ctypes.c_int64]),
+    "wc_batch_launch_widths": (_I32, [_VP, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
+    "wc_batch_run_steps": (_I32, [_VP, _U64, ctypes.POINTER(_U64), _P]),
     "wc_profile_enable": (_I32, [_VP, _I32]),
     "wc_profile_reset": (_I32, [_VP]),
     "wc_profile_read": (_I32, [_VP, ctypes.c_char_p, _U64, _P, ctypes.POINTER(_U64), _U64,
@@ -514,6 +519,51 @@ def relaxation_targets(target, n: int) -> np.ndarray:
     else:
         out = np.array(target, dtype=np.float64).reshape(n, 3)
     return np.ascontiguousarray(out.reshape(-1))
+
+
+class Batch:
+    """Many independent solvers stepped together, every phase one launch over all of them
+    (wc_batch_*): FP32 form, one-kernel Jacobi PCG. ``run_steps(states, steps)`` advances each
+    solver's state ``steps`` times with its own stable dt, as ``Solver.run_steps`` does for one,
+    and returns the steps each took and its last dt."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        arr = (_VP * len(self.solvers))(*[s._h for s in self.solvers])
+        h = _VP()
+        _check(lib().wc_batch_new(arr, len(self.solvers), ctypes.byref(h)))
+        self._b = h
+
+    def __del__(self):
+        if getattr(self, "_b", None):
+            lib().wc_batch_free(self._b)
+            self._b = None
+
+    def set_option(self, key: str, value: int) -> "Batch":
+        _check(lib().wc_batch_set_option(self._b, key.encode(), int(value)))
+        return self
+
+    def launch_widths(self) -> tuple[int, int]:
+        """The persistent widths of the 256- and 1024-thread launches (tuned on the first run)."""
+        a, b = _I32(), _I32()
+        _check(lib().wc_batch_launch_widths(self._b, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def run_steps(self, states, steps: int):
+        if len(states) != len(self.solvers):
+            raise ValueError("one state per solver")
+        for g, s in zip(self.solvers, states):
+            g._hold(s)
+        n = len(self.solvers)
+        done = (_U64 * n)()
+        last = np.zeros(n)
+        _check(lib().wc_batch_run_steps(self._b, int(steps), done, _ptr(last)))
+        for g, s in zip(self.solvers, states):
+            tt = ctypes.c_double()
+            _check(lib().wc_state_get(g._h, None, None, None, None, ctypes.byref(tt)))
+            s.time = tt.value
+            s._host_valid = False
+        return [int(d) for d in done], last
 
 
 class Relaxation:
