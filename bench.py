@@ -76,6 +76,10 @@ def parse():
     p.add_argument("--spinup", type=int, default=1500)
     p.add_argument("--workload", default="rough_bed",
                    choices=["rough_bed", "plane_beach", "large_reef", "sweep"])
+    p.add_argument("--graph", type=int, default=None, choices=[0, 1],
+                   help="sweep: replay the fixed stretches of a step from HIP graphs (default 1)")
+    p.add_argument("--sweep-size", default=None,
+                   help="sweep: every instance at this size (e.g. 256x128) instead of the drawn ones")
     p.add_argument("--instances", type=int, default=96,
                    help="sweep: the number of reef-sheet instances (seeds 3000, 3001, ...)")
     p.add_argument("--profile-steps", type=int, default=20)
@@ -85,6 +89,8 @@ def parse():
     p.add_argument("--precision", default="both", choices=["fp64", "fp32", "both"])
     p.add_argument("--no-network", action="store_true",
                    help="skip the FP32 path with the learned preconditioner")
+    p.add_argument("--pcg-lanes", type=int, default=None, choices=[256, 512],
+                   help="blocks of the FP32 form's one-kernel PCG launches")
     p.add_argument("--pcg32", type=int, default=None,
                    help="PCG kernels of the FP32 form (3: two per iteration, 4: one)")
     p.add_argument("--calibrate-copy", action="store_true",
@@ -106,7 +112,8 @@ def calibrate_copy():
                       "launches": 10}))
 
 
-def measure(w, precision, arrays, t0, warmup, steps, profile_steps, pcg32=None, network=None):
+def measure(w, precision, arrays, t0, warmup, steps, profile_steps, pcg32=None, network=None,
+            lanes=None):
     """Time `steps` steps of one precision from the given state; per-kernel times with HIP
     events on the solver's stream: for the one-kernel PCG iteration of the FP32 form, one pair
     of events around each batch of its launches inside the timed window itself; for the kernel
@@ -122,6 +129,8 @@ def measure(w, precision, arrays, t0, warmup, steps, profile_steps, pcg32=None, 
         g.with_precision(PRECISION_FP32)
         if pcg32:
             g.set_option("pcg_kernels", pcg32)
+        if lanes:
+            g.set_option("pcg_lanes", lanes)
         if network is not None:
             g.with_preconditioner(network)
     h, hu, hv, br = arrays
@@ -235,11 +244,18 @@ def sweep(args):
     from wavesim_amd import configs
     from wavesim_amd.solver import PRECISION_FP32, Batch, Solver
 
-    ws = [configs.c3_reef_instance(seed) for seed in range(3000, 3000 + args.instances)]
+    size = tuple(int(v) for v in args.sweep_size.split("x")) if args.sweep_size else None
+    ws = [configs.c3_reef_instance(seed, size=size)
+          for seed in range(3000, 3000 + args.instances)]
     solvers = [w.solver(Solver).with_precision(PRECISION_FP32) for w in ws]
+    if args.pcg_lanes:
+        for g in solvers:
+            g.set_option("pcg_lanes", args.pcg_lanes)
     states = [w.still_water() for w in ws]
     cells = sum(w.grid.nx * w.grid.ny for w in ws)
     b = Batch(solvers)
+    if args.graph is not None:
+        b.set_option("graph", args.graph)
     t = time.perf_counter()
     b.run_steps(states, args.spinup)
     print(f"[bench] sweep: {len(ws)} instances, {cells / 1e6:.1f} M cells, spin-up "
@@ -279,7 +295,9 @@ def sweep(args):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"reef_sheet_sweep_{len(ws)}", "instances": len(ws),
+        "config": {"workload": f"reef_sheet_sweep_{len(ws)}" + (f"_{args.sweep_size}"
+                                                                 if args.sweep_size else ""),
+                   "instances": len(ws),
                    "cells": cells, "sizes": sizes, "spinup_steps": args.spinup,
                    "path": "FP32 form, one launch per phase over all instances",
                    "launch_widths": list(b.launch_widths())},
@@ -323,12 +341,13 @@ def main():
     del g
     precisions = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
     res = {p: measure(w, p, arrays, t0, args.warmup, args.steps, args.profile_steps,
-                      args.pcg32)
+                      args.pcg32, lanes=args.pcg_lanes)
            for p in precisions}
     net_file = os.path.join(ROOT, "wavesim_amd", "data", "precond_rough_bed.npy")
     if "fp32" in res and not args.no_network and os.path.exists(net_file):
         res["fp32+network"] = measure(w, "fp32", arrays, t0, args.warmup, args.steps,
-                                      args.profile_steps, args.pcg32, np.load(net_file))
+                                      args.profile_steps, args.pcg32, np.load(net_file),
+                                      lanes=args.pcg_lanes)
     accs = {p: accuracy(w, res["fp64"], res[p]) for p in res if p != "fp64"} \
         if "fp64" in res else {}
     # the chosen form: the fastest path within the accuracy bounds against FP64

@@ -60,10 +60,15 @@ typedef struct {
   int32_t west, east, south, north;
 } wc_sponge_t;
 
-/* Breaking (dispersion.rs:44-74), without the eddy viscosity, which is not built. */
+/* Breaking (dispersion.rs:44-74); its `viscosity` is wc_solver_with_eddy_viscosity. */
 typedef struct {
   double eta_ratio_lo, eta_ratio_hi, slope_lo, slope_hi;
 } wc_breaking_t;
+
+/* EddyViscosity (eddy.rs:63-89): Kennedy et al.'s defaults are 0.65, 0.15, 5, 1.2, 5. */
+typedef struct {
+  double initial, settled, duration, mixing, linger;
+} wc_eddy_t;
 
 /* Dispersion (dispersion.rs:76-97). */
 typedef struct {
@@ -108,6 +113,16 @@ int wc_solver_with_manning(wc_solver* s, double n);                   /* solver.
 int wc_solver_with_sponge(wc_solver* s, const wc_sponge_t* sponge);   /* solver.rs:314 */
 int wc_solver_with_wavemaker(wc_solver* s, const wc_wavemaker_t* m); /* solver.rs:321 */
 int wc_solver_with_dispersion(wc_solver* s, const wc_dispersion_t* d); /* solver.rs:330 */
+/* Breaking::viscosity = Some(*e) (NULL: None), after wc_solver_with_dispersion (which resets
+ * it): the breaking fronts are followed in the state's breaking array after every step, keep
+ * the switch off where they are and behind them, and get their eddy viscosity
+ * (solver.rs:512-532, eddy.rs). In the FP32 form this part runs on the FP64 state. */
+int wc_solver_with_eddy_viscosity(wc_solver* s, const wc_eddy_t* e);
+/* Breaking::viscosity = Some(*e) (NULL: None), after wc_solver_with_dispersion (which resets
+ * it): the breaking fronts are followed in the state's breaking array after every step, keep
+ * the switch off where they are and behind them, and get their eddy viscosity
+ * (solver.rs:512-532, eddy.rs). In the FP32 form this part runs on the FP64 state. */
+int wc_solver_with_eddy_viscosity(wc_solver* s, const wc_eddy_t* e);
 int wc_solver_with_periodic_y(wc_solver* s);                          /* solver.rs:414 */
 /* solver.rs:376: one byte per interior cell, row-major with y outer, nonzero = held. */
 int wc_solver_with_shallow_cells(wc_solver* s, const uint8_t* cells, uint64_t n);
@@ -258,6 +273,29 @@ uint64_t wc_relaxation_len(const wc_relaxation* r);
 int wc_relaxation_cells(const wc_relaxation* r, uint64_t* i, uint64_t* j);
 int wc_relaxation_apply(wc_relaxation* r, double dt, const double* targets);
 int wc_relaxation_apply_device(wc_relaxation* r, double dt, const double* targets);
+
+/* The coarse-to-fine sampler of nested runs (wavesim's Sampler, crates/wavesim/src/
+ * lib.rs:1548-1616, and its use after each fine step, lib.rs:1866-1897), for a relaxation zone
+ * of the fine solver:
+ *   wc_sampler_new    Sampler::new: the zone's cells, the fine grid's first column fine_x0 in
+ *                     the bed file and its cell size, the coarse grid (cropped by `crop`
+ *                     columns) and its cell size; four coarse corners and bilinear weights per
+ *                     zone cell, clamped to one ghost layer.
+ *   wc_sampler_sample Sampler::sample of the coarse solver's current state (ghosts as walls)
+ *                     into slot 0 (before the coarse step that brackets the fine step's end)
+ *                     or 1 (after it), with the state's time.
+ *   wc_sampler_apply  the fine step's relaxation: targets interpolated at (end - t0) / (t1 - t0)
+ *                     clamped to [0, 1] (the nearer sample where one is dry), then
+ *                     Relaxation::apply with dt on the zone's solver.
+ *   wc_sampler_read   a sample (slot 0, 1) or the last targets (2): 3 doubles per zone cell. */
+typedef struct wc_sampler wc_sampler;
+int wc_sampler_new(const wc_relaxation* zone, uint64_t fine_x0, double fine_dx, double fine_dy,
+                   uint64_t coarse_nx, uint64_t coarse_ny, double coarse_dx, double coarse_dy,
+                   uint64_t crop, wc_sampler** out);
+void wc_sampler_free(wc_sampler* sm);
+int wc_sampler_sample(wc_sampler* sm, wc_solver* coarse, int32_t slot);
+int wc_sampler_apply(wc_sampler* sm, wc_relaxation* zone, double end, double dt);
+int wc_sampler_read(const wc_sampler* sm, int32_t slot, double* out, double* time);
 
 /* Many instances in one launch (the batched sweep of independent runs: beds, swell angles,
  * tides, grids of uneven size). A batch groups existing solvers of one device, all in the FP32

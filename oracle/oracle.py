@@ -28,6 +28,7 @@ from wavesim_amd.config import (
     Sponge,
     WaveMaker,
     c_dispersion,
+    c_eddy,
     c_sponge,
     c_wavemaker,
 )
@@ -72,6 +73,11 @@ def lib():
             "wco_with_sponge": (_I32, [_VP, ctypes.POINTER(CSponge)]),
             "wco_with_wavemaker": (_I32, [_VP, ctypes.POINTER(CWaveMaker)]),
             "wco_with_dispersion": (_I32, [_VP, ctypes.POINTER(CDispersion)]),
+            "wco_with_eddy_viscosity": (_I32, [_VP, ctypes.c_void_p]),
+            "wco_eddy_follow": (_I32, [_U64, _U64, _D, _D, ctypes.c_void_p, _P, _P, _P, _P, _P,
+                                       _D, _P]),
+            "wco_eddy_diffuse": (_I32, [_U64, _U64, _D, _D, _P, _P, _P, _P, _D,
+                                        ctypes.POINTER(_U64)]),
             "wco_with_periodic_y": (_I32, [_VP]),
             "wco_with_shallow_cells": (_I32, [_VP, ctypes.POINTER(ctypes.c_uint8), _U64]),
             "wco_bed": (_I32, [_VP, _P]),
@@ -192,6 +198,9 @@ class OracleSolver:
     def with_dispersion(self, d: Dispersion) -> "OracleSolver":
         c = c_dispersion(d)
         _check(lib().wco_with_dispersion(self._h, ctypes.byref(c)))
+        if d.breaking.viscosity is not None:
+            e = c_eddy(d.breaking.viscosity)
+            _check(lib().wco_with_eddy_viscosity(self._h, ctypes.byref(e)))
         self._dispersion = d
         return self
 
@@ -436,3 +445,81 @@ def dispersion_apply(grid: Grid, periodic: bool, d: Dispersion, h, hu, hv, phi, 
                                       ctypes.byref(c), *(ptr(a) for a in ins), ptr(lhu),
                                       ptr(lhv), dot_order))
     return lhu, lhv
+
+
+class OracleSampler:
+    """wavesim's Sampler (crates/wavesim/src/lib.rs:1548-1616) and the nested run's time
+    interpolation of its targets (lib.rs:1877-1890), in plain Python floats (IEEE doubles, the
+    reference's operation order). ``cells`` are the zone's padded (i, j)."""
+
+    def __init__(self, cells, fine_x0, fine_dx, fine_dy, cnx, cny, cdx, cdy, crop=0):
+        import math
+
+        g = 3  # GHOST
+        w = cnx + 2 * g
+        self.corners = []
+        for i, j in cells:
+            x = float(fine_x0 + i - g) * fine_dx + 0.5 * fine_dx
+            y = float(j - g) * fine_dy + 0.5 * fine_dy
+            cx = x / cdx - float(crop) - 0.5
+            cy = y / cdy - 0.5
+            ix, iy = math.floor(cx), math.floor(cy)
+            wx, wy = cx - ix, cy - iy
+
+            def at(di, dj, ix=ix, iy=iy):
+                pi = min(max(int(ix + di) + g, g - 1), cnx + g)
+                pj = min(max(int(iy + dj) + g, g - 1), cny + g)
+                return pj * w + pi
+
+            self.corners.append([(at(0.0, 0.0), (1.0 - wx) * (1.0 - wy)),
+                                 (at(1.0, 0.0), wx * (1.0 - wy)),
+                                 (at(0.0, 1.0), (1.0 - wx) * wy),
+                                 (at(1.0, 1.0), wx * wy)])
+
+    def sample(self, h, hu, hv, b):
+        """(eta, u, v) per zone cell from padded arrays whose ghosts are filled (walls)."""
+        out = []
+        for corners in self.corners:
+            eta = u = v = weight = 0.0
+            for k, wk in corners:
+                if h[k] > 1e-3 and wk > 0.0:
+                    eta += wk * (h[k] + b[k])
+                    u += wk * hu[k] / h[k]
+                    v += wk * hv[k] / h[k]
+                    weight += wk
+            out.append((eta / weight, u / weight, v / weight) if weight > 0.0
+                       else (float("-inf"), 0.0, 0.0))
+        return np.array(out)
+
+    @staticmethod
+    def lerp(frm, to, w):
+        out = np.empty_like(frm)
+        for m in range(len(frm)):
+            a, c = frm[m], to[m]
+            if np.isfinite(a[0]) and np.isfinite(c[0]):
+                out[m] = (a[0] + w * (c[0] - a[0]), a[1] + w * (c[1] - a[1]),
+                          a[2] + w * (c[2] - a[2]))
+            else:
+                out[m] = a if w < 0.5 else c
+        return out
+
+
+def eddy_follow(grid, e, h, hu, hv, breaking, phi, dt):
+    """EddyViscosity::follow (eddy.rs:163-221) on padded arrays: (new breaking, nu)."""
+    h, hu, hv, phi = (np.ascontiguousarray(a, dtype=np.float64).copy() for a in (h, hu, hv, phi))
+    br = np.ascontiguousarray(breaking, dtype=np.float64).copy()
+    nu = np.zeros_like(h)
+    ce = c_eddy(e)
+    _check(lib().wco_eddy_follow(grid.nx, grid.ny, grid.dx, grid.dy, ctypes.byref(ce), ptr(h),
+                                 ptr(hu), ptr(hv), ptr(br), ptr(phi), float(dt), ptr(nu)))
+    return br, nu
+
+
+def eddy_diffuse(grid, h, hu, hv, nu, dt):
+    """diffuse (eddy.rs:225-302) on padded arrays, walls refilled between sub-steps:
+    (hu, hv, sub-steps)."""
+    h, hu, hv, nu = (np.ascontiguousarray(a, dtype=np.float64).copy() for a in (h, hu, hv, nu))
+    n = _U64()
+    _check(lib().wco_eddy_diffuse(grid.nx, grid.ny, grid.dx, grid.dy, ptr(h), ptr(hu), ptr(hv),
+                                  ptr(nu), float(dt), ctypes.byref(n)))
+    return hu, hv, int(n.value)

@@ -155,8 +155,21 @@ struct State {
   }
 };
 
+// EddyViscosity (eddy.rs:63-89) and its threshold (eddy.rs:108-131)
+struct Eddy {
+  double initial = 0.65, settled = 0.15, duration = 5.0, mixing = 1.2, linger = 5.0;
+  double span(double h) const { return duration * sqrt(h / G); }
+  double threshold(double age, double h) const {
+    const double sp = span(h);
+    if (age >= sp) return settled;
+    return initial + (settled - initial) * age / sp;
+  }
+};
+
 struct Breaking {
   double er_lo = 0.30, er_hi = 0.55, sl_lo = 0.40, sl_hi = 0.65;  // dispersion.rs:67-73
+  bool viscosity = false;  // Breaking::viscosity is Some(eddy)
+  Eddy eddy;
 };
 
 struct DispersionCfg {
@@ -640,7 +653,7 @@ void dispersion_apply(const Grid& g, const Par& par, int dot_order, const Disper
   });
 }
 
-// dispersion.rs:120-160 (the eddy-viscosity branch is never on here: viscosity is off).
+// dispersion.rs:120-160; with the eddy viscosity on, a cell with breaking memory stays off.
 void switch_phi(const Grid& g, const Par& par, const Mathlib& m, const double* bed,
                 const State& s, const Breaking& br, bool periodic, double* phi) {
   const size_t w = g.width();
@@ -659,13 +672,125 @@ void switch_phi(const Grid& g, const Par& par, const Mathlib& m, const double* b
       const double ex = (eta(k + 1) - eta(k - 1)) * ix;
       const double ey = (eta(k + w) - eta(k - w)) * iy;
       const double ratio = wc_max(eta(k) / h[k], 0.0);
-      const double off = wc_max(wc_smoothstep(br.er_lo, br.er_hi, ratio),
-                                wc_smoothstep(br.sl_lo, br.sl_hi, m.hypot(ex, ey)));
+      double off = wc_max(wc_smoothstep(br.er_lo, br.er_hi, ratio),
+                          wc_smoothstep(br.sl_lo, br.sl_hi, m.hypot(ex, ey)));
+      if (br.viscosity && s.breaking[k] != 0.0) off = 1.0;  // EddyViscosity::keeps_off
       phi[k] = 1.0 - off;
     }
   }
   mirror_ghosts(g, phi);
   if (periodic) wrap_ghosts_y(g, phi);
+}
+
+// ---- breaking fronts and their eddy viscosity (eddy.rs) --------------------------------------
+constexpr double H_BREAKING_MIN = 0.01;  // eddy.rs:55
+constexpr double BREAKING_ONSET = 0.8;   // eddy.rs:59
+
+struct EddyWork {  // eddy.rs:93-100
+  std::vector<double> rise, nu, memory, hu, hv;
+  void init(const Grid& g) {
+    rise.assign(g.cells(), 0.0);
+    nu.assign(g.cells(), 0.0);
+    memory.assign(g.cells(), 0.0);
+    hu.assign(g.cells(), 0.0);
+    hv.assign(g.cells(), 0.0);
+  }
+};
+
+// EddyViscosity::follow (eddy.rs:163-221): the fronts into s.breaking, their viscosity into nu.
+void eddy_follow(const Grid& g, const Eddy& e, State& s, const double* phi, double dt,
+                 EddyWork& work) {
+  const size_t w = g.width(), nx = g.nx, ny = g.ny;
+  const double ix = 0.5 / g.dx, iy = 0.5 / g.dy;
+  const double *h = s.h.data(), *hu = s.hu.data(), *hv = s.hv.data();
+  double* rise = work.rise.data();
+  std::fill(work.rise.begin(), work.rise.end(), 0.0);
+  for (size_t j = GH; j < ny + GH; ++j)
+    for (size_t i = GH; i < nx + GH; ++i) {
+      const size_t k = j * w + i;
+      rise[k] = -((hu[k + 1] - hu[k - 1]) * ix + (hv[k + w] - hv[k - w]) * iy);
+    }
+  const double* old = s.breaking.data();
+  std::fill(work.memory.begin(), work.memory.end(), 0.0);
+  std::fill(work.nu.begin(), work.nu.end(), 0.0);
+  double *rm = work.memory.data(), *rn = work.nu.data();
+  for (size_t j = GH; j < ny + GH; ++j)
+    for (size_t i = GH; i < nx + GH; ++i) {
+      const size_t k = j * w + i;
+      const double depth = h[k];
+      if (depth < H_BREAKING_MIN) continue;
+      const double c = sqrt(G * depth);
+      double front = 0.0;
+      for (size_t m : {k, k - 1, k + 1, k - w, k + w}) front = wc_max(front, old[m]);
+      const bool starts = depth >= H_DISP_MIN && 1.0 - phi[k] >= BREAKING_ONSET && rise[k] > 0.0;
+      const bool goes_on = front > 0.0 && rise[k] > e.threshold(front, depth) * c;
+      const double age = (front > 0.0 && (starts || goes_on)) ? front + dt : starts ? dt : 0.0;
+      if (age > 0.0) {
+        double b = rise[k] / (e.threshold(age, depth) * c) - 1.0;
+        b = b < 0.0 ? 0.0 : (b > 1.0 ? 1.0 : b);  // f64::clamp(0, 1)
+        rm[k] = age;
+        rn[k] = b * e.mixing * e.mixing * depth * rise[k];
+      } else if (old[k] > 0.0) {
+        rm[k] = -e.linger * sqrt(depth / G);
+      } else if (old[k] < 0.0) {
+        rm[k] = wc_min(old[k] + dt, 0.0);
+      }
+    }
+  std::swap(s.breaking, work.memory);
+}
+
+// diffuse (eddy.rs:225-302): nu applied to the momentum in explicit sub-steps; returns them.
+template <class Fill>
+size_t eddy_diffuse(const Grid& g, State& s, double dt, EddyWork& work, Fill fill) {
+  const size_t w = g.width(), nx = g.nx, ny = g.ny;
+  const double* nu = work.nu.data();
+  double top = 0.0;
+  for (size_t j = GH; j < GH + ny; ++j)
+    for (size_t i = GH; i < GH + nx; ++i) top = wc_max(top, nu[j * w + i]);
+  if (top <= 0.0) return 0;
+  const double idx2 = 1.0 / (g.dx * g.dx), idy2 = 1.0 / (g.dy * g.dy);
+  const double limit = 0.5 / (top * wc_max(2.0 * idx2 + idy2, idx2 + 2.0 * idy2));
+  const size_t steps = (size_t)wc_max(ceil(dt / limit), 1.0);
+  const double sub = dt / (double)steps;
+  const double cross = 0.25 / (g.dx * g.dy);
+  for (size_t step = 0; step < steps; ++step) {
+    if (step > 0) fill(s);
+    const double *h = s.h.data(), *hu = s.hu.data(), *hv = s.hv.data();
+    auto link = [&](size_t a, size_t b) {
+      return (h[a] < H_BREAKING_MIN || h[b] < H_BREAKING_MIN) ? 0.0 : 0.5 * (nu[a] + nu[b]);
+    };
+    auto wet = [&](size_t a) { return h[a] < H_BREAKING_MIN ? 0.0 : nu[a]; };
+    double *ru = work.hu.data(), *rv = work.hv.data();
+    for (size_t j = GH; j < ny + GH; ++j)
+      for (size_t i = GH; i < nx + GH; ++i) {
+        const size_t k = j * w + i;
+        if (h[k] < H_BREAKING_MIN) {
+          ru[k] = hu[k];
+          rv[k] = hv[k];
+          continue;
+        }
+        const size_t e = k + 1, west = k - 1, north = k + w, south = k - w;
+        auto xx = [&](const double* q) {
+          return (link(k, e) * (q[e] - q[k]) - link(k, west) * (q[k] - q[west])) * idx2;
+        };
+        auto yy = [&](const double* q) {
+          return (link(k, north) * (q[north] - q[k]) - link(k, south) * (q[k] - q[south])) * idy2;
+        };
+        auto yx = [&](const double* q) {
+          return (wet(north) * (q[north + 1] - q[north - 1]) -
+                  wet(south) * (q[south + 1] - q[south - 1])) * cross;
+        };
+        auto xy = [&](const double* q) {
+          return (wet(e) * (q[e + w] - q[e - w]) - wet(west) * (q[west + w] - q[west - w])) *
+                 cross;
+        };
+        ru[k] = hu[k] + sub * (xx(hu) + 0.5 * (yy(hu) + yx(hv)));
+        rv[k] = hv[k] + sub * (yy(hv) + 0.5 * (xx(hv) + xy(hu)));
+      }
+    std::swap(s.hu, work.hu);
+    std::swap(s.hv, work.hv);
+  }
+  return steps;
 }
 
 }  // namespace
@@ -686,6 +811,7 @@ struct wco_solver {
   bool dispersive = false;
   DispersionCfg disp;
   Work work;
+  EddyWork eddy_work;
   std::vector<double> phi;
   bool has_shallow = false;
   std::vector<uint8_t> shallow;
@@ -874,7 +1000,22 @@ struct wco_solver {
       clean_dry(s);
     }
     damp(s, dt);
+    follow_breaking(s, dt);
     s.time = t + dt;
+  }
+
+  // solver.rs:512-532
+  void follow_breaking(State& st, double dt) {
+    if (!(dispersive && disp.breaking.viscosity)) return;
+    st.fill_boundaries(g, periodic);
+    Breaking fresh = breaking_config();
+    fresh.viscosity = false;  // the switch as the surface alone reads it
+    switch_phi(g, par, math, bed.data(), st, fresh, periodic, phi.data());
+    const Eddy& e = disp.breaking.eddy;
+    eddy_follow(g, e, st, phi.data(), dt, eddy_work);
+    if (e.mixing > 0.0)
+      eddy_diffuse(g, st, dt, eddy_work, [&](State& x) { x.fill_boundaries(g, periodic); });
+    clean_dry(st);
   }
 
   // solver.rs:95-100
@@ -1045,9 +1186,67 @@ int wco_with_dispersion(wco_solver* s, const wco_dispersion_t* d) {
   s->disp.nonlinear = d->nonlinear != 0;
   s->disp.tolerance = d->tolerance;
   s->disp.max_iterations = d->max_iterations;
+  s->disp.breaking.viscosity = false;
   s->dispersive = true;
   s->work.init(s->g);
   return s->rebuild_drive() ? 0 : 1;
+}
+
+int wco_with_eddy_viscosity(wco_solver* s, const wco_eddy_t* e) {
+  if (!s->dispersive) return fail("the eddy viscosity is part of the dispersion (with_dispersion first)");
+  if (!e) {
+    s->disp.breaking.viscosity = false;
+    return 0;
+  }
+  s->disp.breaking.viscosity = true;
+  s->disp.breaking.eddy = Eddy{e->initial, e->settled, e->duration, e->mixing, e->linger};
+  s->eddy_work.init(s->g);
+  return 0;
+}
+
+// Test hooks: EddyViscosity::follow and diffuse on given arrays (eddy.rs:163-302).
+int wco_eddy_follow(uint64_t nx, uint64_t ny, double dx, double dy, const wco_eddy_t* e,
+                    double* h, double* hu, double* hv, double* breaking, const double* phi,
+                    double dt, double* nu_out) {
+  Grid g;
+  g.nx = nx;
+  g.ny = ny;
+  g.dx = dx;
+  g.dy = dy;
+  State s;
+  s.h.assign(h, h + g.cells());
+  s.hu.assign(hu, hu + g.cells());
+  s.hv.assign(hv, hv + g.cells());
+  s.breaking.assign(breaking, breaking + g.cells());
+  EddyWork work;
+  work.init(g);
+  eddy_follow(g, Eddy{e->initial, e->settled, e->duration, e->mixing, e->linger}, s, phi, dt,
+              work);
+  std::copy(s.breaking.begin(), s.breaking.end(), breaking);
+  if (nu_out) std::copy(work.nu.begin(), work.nu.end(), nu_out);
+  return 0;
+}
+
+int wco_eddy_diffuse(uint64_t nx, uint64_t ny, double dx, double dy, double* h, double* hu,
+                     double* hv, const double* nu, double dt, uint64_t* steps) {
+  Grid g;
+  g.nx = nx;
+  g.ny = ny;
+  g.dx = dx;
+  g.dy = dy;
+  State s;
+  s.h.assign(h, h + g.cells());
+  s.hu.assign(hu, hu + g.cells());
+  s.hv.assign(hv, hv + g.cells());
+  s.breaking.assign(g.cells(), 0.0);
+  EddyWork work;
+  work.init(g);
+  std::copy(nu, nu + g.cells(), work.nu.begin());
+  const size_t n = eddy_diffuse(g, s, dt, work, [&](State& x) { x.fill_walls(g); });
+  std::copy(s.hu.begin(), s.hu.end(), hu);
+  std::copy(s.hv.begin(), s.hv.end(), hv);
+  if (steps) *steps = n;
+  return 0;
 }
 
 int wco_with_periodic_y(wco_solver* s) {

@@ -58,6 +58,11 @@ typedef struct {
   uint64_t solves, iterations, unconverged;
 } wco_dispersion_stats_t;
 
+/* EddyViscosity (eddy.rs:63-89). */
+typedef struct {
+  double initial, settled, duration, mixing, linger;
+} wco_eddy_t;
+
 typedef struct {
   uint64_t i, j;
   double depth, speed;
@@ -94,6 +99,14 @@ int wco_state_get(const wco_solver* s, double* h, double* hu, double* hv, double
 int wco_stable_dt(wco_solver* s, double* dt);
 int wco_step(wco_solver* s, double dt);
 int wco_limiting_cell(wco_solver* s, int32_t* found, wco_limiting_t* out);
+/* Breaking::viscosity = Some(e) (NULL: None), after wco_with_dispersion (solver.rs:512-532). */
+int wco_with_eddy_viscosity(wco_solver* s, const wco_eddy_t* e);
+/* EddyViscosity::follow and diffuse on padded arrays, for the unit tests (eddy.rs:163-302). */
+int wco_eddy_follow(uint64_t nx, uint64_t ny, double dx, double dy, const wco_eddy_t* e,
+                    double* h, double* hu, double* hv, double* breaking, const double* phi,
+                    double dt, double* nu_out);
+int wco_eddy_diffuse(uint64_t nx, uint64_t ny, double dx, double dy, double* h, double* hu,
+                     double* hv, const double* nu, double dt, uint64_t* steps);
 int wco_breaking_indicator(wco_solver* s, double* out);
 int wco_dispersion_stats(const wco_solver* s, int32_t* has, wco_dispersion_stats_t* out);
 int wco_state_volume(wco_solver* s, double* out);

@@ -24,7 +24,10 @@ pytestmark = pytest.mark.gpu
 
 
 def fp32(grid, bed, pcg=None):
+    """pcg: 3 (two kernels per iteration), 4 (one, 256 blocks), 512 (one, 512 blocks)."""
     s = Solver(grid, bed).with_precision(PRECISION_FP32)
+    if pcg == 512:
+        return s.set_option("pcg_kernels", 4).set_option("pcg_lanes", 512)
     return s.set_option("pcg_kernels", pcg) if pcg else s
 
 
@@ -62,7 +65,7 @@ def run_pair(w, steps, t0=0.0, state=None, pcg=None):
     return g64, g32, s64, s32
 
 
-@pytest.mark.parametrize("pcg", [3, 4])
+@pytest.mark.parametrize("pcg", [3, 4, 512])
 def test_fp32_rough_bed_matches_fp64_within_its_bounds(pcg):
     w = configs.c1_rough_bed(nx=256, ny=128)
     g64, g32, s64, s32 = run_pair(w, 600, t0=40.0, pcg=pcg)
@@ -83,7 +86,7 @@ def test_fp32_plane_beach_fv_matches_fp64():
     assert r["eta_max_err_m"] < 1e-3, r
 
 
-@pytest.mark.parametrize("pcg", [3, 4])
+@pytest.mark.parametrize("pcg", [3, 4, 512])
 def test_fp32_lake_at_rest_and_closed_basin_volume(pcg):
     # well_balanced.rs:16-62 and 87-121 on the FP32 form
     for mean_depth, amplitude in [(8.0, 3.0), (0.5, 2.5)]:
@@ -100,7 +103,7 @@ def test_fp32_lake_at_rest_and_closed_basin_volume(pcg):
                                                  cases=[(False, True), (True, True)])
 
 
-@pytest.mark.parametrize("pcg", [3, 4])
+@pytest.mark.parametrize("pcg", [3, 4, 512])
 def test_fp32_standing_wave_frequency(pcg):
     out = ka.check_standing_wave_sgn_frequency(fp32_with(pcg), orders=(Order.Fifth,))
     assert abs(out["Fifth"] - 1.4261) < 3e-3, out
@@ -121,3 +124,26 @@ def test_switching_precision_converts_the_state():
         assert np.max(np.abs(inner)) <= 1e-12 * max(1.0, np.max(np.abs(b)))
     n, dt = g.run_steps(s, 20)
     assert n == 20 and math.isfinite(dt)
+
+
+def test_fp32_breaking_fronts_with_eddy_viscosity_stay_near_fp64():
+    """Breaking::viscosity in the FP32 form (the fronts and the viscosity run on the FP64 state
+    between the FP32 steps): the surface stays within the form's bound of the FP64 path."""
+    from wavesim_amd.config import Breaking, EddyViscosity
+    from wavesim_amd.grid import GHOST
+
+    grid = Grid(80, 24, 0.5, 0.5)
+    bed = Bathymetry.flat(grid, 1.0)
+    disp = Dispersion(breaking=Breaking(viscosity=EddyViscosity()), tolerance=1e-8)
+    s = State.lake_at_rest(grid, bed, 0.0)
+    x, y = grid.centres()
+    X, Y = np.meshgrid(x[GHOST:GHOST + grid.nx], y[GHOST:GHOST + grid.ny])
+    grid.inner(s.h)[:, :] += np.exp(-((X - 20.0) ** 2 + (Y - 6.0) ** 2) / 12.5)
+    g64 = Solver(grid, bed).with_order(Order.Fifth).with_dispersion(disp)
+    g32 = fp32(grid, bed).with_order(Order.Fifth).with_dispersion(disp)
+    s64, s32 = s.clone(), s.clone()
+    assert g64.run_steps(s64, 160)[0] == 160 and g32.run_steps(s32, 160)[0] == 160
+    h64, h32 = grid.inner(s64.arrays()[0]), grid.inner(s32.arrays()[0])
+    assert np.max(np.abs(h64 - h32)) < 1e-3
+    b64, b32 = s64.arrays()[3], s32.arrays()[3]
+    assert np.any(b64 > 0) and np.mean((b64 != 0) == (b32 != 0)) >= 0.99

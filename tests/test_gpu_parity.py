@@ -218,3 +218,29 @@ def test_invalid_setups_raise_like_the_reference_panics():
         Solver(grid, bed).with_wavemaker(WaveMaker(5.0, 0.1, 10.0, 0.3, 2.0, 1.0)).with_periodic_y()
     with pytest.raises(ValueError):
         Solver(Grid(2, 10, 1.0, 1.0), Bathymetry.flat(Grid(2, 10, 1.0, 1.0), 1.0))
+
+
+@pytest.mark.parametrize("pcg", [1, 3])
+@pytest.mark.parametrize("periodic", [False, True])
+def test_breaking_fronts_and_eddy_viscosity(periodic, pcg):
+    """Breaking::viscosity on (solver.rs:512-532, eddy.rs): a hump as tall as the water is deep
+    breaks at once; the fronts, their memory in the breaking array, the switch kept off behind
+    them and the sub-stepped eddy viscosity must match the oracle bit for bit."""
+    from wavesim_amd.config import Breaking, EddyViscosity
+
+    grid = Grid(80, 24, 0.5, 0.5)
+    bed = Bathymetry.flat(grid, 1.0)
+    disp = Dispersion(breaking=Breaking(viscosity=EddyViscosity()), tolerance=1e-8)
+
+    def configure(s):
+        s = s.with_order(Order.Fifth).with_dispersion(disp)
+        return s.with_periodic_y() if periodic else s
+
+    s = hump_state(grid, bed, amp=1.0, x0=20.0, y0=3.0 if periodic else 6.0, r=2.5)
+    g, o, sg, so = lockstep(grid, bed, configure, s, 160,
+                            f"eddy periodic={periodic} pcg={pcg}", options={"pcg_kernels": pcg})
+    br = sg.arrays()[3]
+    assert np.any(br > 0.0) and np.any(br < 0.0)  # fronts now, and lingering behind them
+    ind = g.breaking_indicator(sg)
+    assert np.array_equal(ind, o.breaking_indicator(so))
+    assert np.all(ind[grid.inner(br).ravel() != 0.0] == 1.0)

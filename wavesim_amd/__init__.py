@@ -6,6 +6,7 @@ The compute runs in hand-written HIP kernels for gfx950 in ``libwavecore_hip.so`
 
 from .config import (  # noqa: F401
     Breaking,
+    EddyViscosity,
     Dispersion,
     DispersionStats,
     Limiting,
@@ -23,6 +24,7 @@ from .solver import (  # noqa: F401,E402
     PRECISION_FP64,
     DeviceError,
     Relaxation,
+    Sampler,
     Solver,
     Stats,
     StepWatch,

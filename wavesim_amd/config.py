@@ -25,16 +25,29 @@ class Order(enum.IntEnum):
 
 
 @dataclass(frozen=True)
+class EddyViscosity:
+    """Breaking fronts followed from step to step and their eddy viscosity (eddy.rs:63-89),
+    Kennedy et al.'s (2000) coefficients by default."""
+
+    initial: float = 0.65
+    settled: float = 0.15
+    duration: float = 5.0
+    mixing: float = 1.2
+    linger: float = 5.0
+
+    def span(self, h: float) -> float:
+        """eddy.rs:104-106"""
+        return self.duration * math.sqrt(h / G)
+
+
+@dataclass(frozen=True)
 class Breaking:
-    """When to switch the dispersive terms off (dispersion.rs:43-74); viscosity is not built."""
+    """When to switch the dispersive terms off (dispersion.rs:43-74); `viscosity` follows the
+    breaking fronts (None: the switch is read afresh at every step)."""
 
     eta_ratio: tuple[float, float] = (0.30, 0.55)
     slope: tuple[float, float] = (0.40, 0.65)
-    viscosity: None = None
-
-    def __post_init__(self) -> None:
-        if self.viscosity is not None:
-            raise NotImplementedError("breaking-front eddy viscosity is not part of this build")
+    viscosity: EddyViscosity | None = None
 
 
 @dataclass(frozen=True)
@@ -153,6 +166,15 @@ class CDispersion(ctypes.Structure):
         ("tolerance", ctypes.c_double),
         ("max_iterations", ctypes.c_uint64),
     ]
+
+
+class CEddy(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("initial", "settled", "duration", "mixing",
+                                               "linger")]
+
+
+def c_eddy(e: EddyViscosity) -> CEddy:
+    return CEddy(e.initial, e.settled, e.duration, e.mixing, e.linger)
 
 
 class CDispersionStats(ctypes.Structure):

@@ -162,16 +162,19 @@ def gaussian_lobes(rng: Lcg, n: int, xr, yr, height, width):
     return f
 
 
-def c3_reef_instance(seed: int, dx: float = 3.0) -> Workload:
+def c3_reef_instance(seed: int, dx: float = 3.0, size: tuple[int, int] | None = None) -> Workload:
     """One instance of the batched spot-sheet sweep: a size drawn from {256x128, 512x256,
     768x384, 1024x512, 1536x768}, a 1:30 plane (levelling off at 40 m depth, so that the
     wave-maker of the larger grids stays within the dispersive model's range) plus 1-4 Gaussian
-    reef lobes (4-10 m high, 40-200 m wide) plus a 3 m channel, swell angle -20..20 degrees, tide -0.5..0.5 m (the bed
-    shifted down by the tide, as waveio's Bed::bathymetry does, crates/waveio/src/bed.rs:239-246),
-    H 1-3 m, T 10-18 s; fifth order + SGN, 2000 steps."""
+    reef lobes (4-10 m high, 40-200 m wide) plus a 3 m channel, swell angle -20..20 degrees,
+    tide -0.5..0.5 m (the bed shifted down by the tide, as waveio's Bed::bathymetry does,
+    crates/waveio/src/bed.rs:239-246), H 1-3 m, T 10-18 s; fifth order + SGN, 2000 steps.
+    `size` replaces the drawn size (the rest of the draw is unchanged)."""
     rng = Lcg(seed)
     sizes = [(256, 128), (512, 256), (768, 384), (1024, 512), (1536, 768)]
     nx, ny = sizes[min(int(rng.uniform() * len(sizes)), len(sizes) - 1)]
+    if size is not None:
+        nx, ny = size
     grid = Grid(nx, ny, dx, dx)
     lx, ly = nx * dx, ny * dx
     angle = math.radians(rng.uniform(-20.0, 20.0))

@@ -130,6 +130,10 @@ struct wc_solver {
 
   bool dispersive = false;
   wc_dispersion_t disp{};
+  // breaking fronts and their eddy viscosity (Breaking::viscosity, eddy.rs)
+  bool eddy_on = false;
+  EddyP eddy{};
+  double *e_rise = nullptr, *e_mem = nullptr, *e_nu = nullptr, *e_hu = nullptr, *e_hv = nullptr;
   bool has_held = false;
   double time = 0.0;
   unsigned long long prev_it[2] = {0, 0};
@@ -172,7 +176,8 @@ struct wc_solver {
 
   // implementation options (wc_solver_set_option)
   int pcg_kernels = 0;  // 0: the default of each precision (FP64 3, FP32 4)
-  double* lanes = nullptr;  // 8 x 256 lane sums of the 256-lane PCGs
+  double* lanes = nullptr;  // 8 x 512 lane sums of the lane PCGs
+  int pcg_lanes = 256;      // blocks of the one-kernel PCG's launches: 256 or 512
   int flux_kernel = 2;
   unsigned long long pcg_batch = 4;
   uint8_t* held = nullptr;
@@ -286,11 +291,13 @@ dim3 tiles(const wc_solver* s) { return dim3((unsigned)(s->geo.ntx * s->geo.nty)
 dim3 over(long long n) { return dim3((unsigned)((n + NT - 1) / NT)); }
 
 Breaking breaking_of(const wc_solver* s) {
-  // Solver::breaking_config (solver.rs:619-623): the dispersive model's, or the defaults.
+  // Solver::breaking_config (solver.rs:619-623): the dispersive model's (with the breaking
+  // memory when the fronts are followed), or the defaults.
   if (s->dispersive)
     return Breaking{s->disp.breaking.eta_ratio_lo, s->disp.breaking.eta_ratio_hi,
-                    s->disp.breaking.slope_lo, s->disp.breaking.slope_hi};
-  return Breaking{0.30, 0.55, 0.40, 0.65};
+                    s->disp.breaking.slope_lo, s->disp.breaking.slope_hi,
+                    s->eddy_on ? s->br : nullptr};
+  return Breaking{0.30, 0.55, 0.40, 0.65, nullptr};
 }
 
 // Drive::new (forcing.rs:88-117) and the static per-column part of Drive::source.
@@ -683,6 +690,53 @@ int fill(wc_solver* s, double* h, double* hu, double* hv, double* br, double* ah
                 br, ah, ahu, ahv);
 }
 
+// Solver::follow_breaking (solver.rs:512-532) on the FP64 state: the switch read afresh
+// (without the fronts), the fronts followed (eddy.rs:163-221), their viscosity applied in
+// explicit sub-steps (eddy.rs:225-302), dry cells cleaned. The memory and the momentum swap
+// buffers with the scratch arrays, as the reference swaps its vectors.
+int follow_breaking(wc_solver* s, double dt) {
+  if (!(s->dispersive && s->eddy_on)) return WC_OK;
+  const Geo& g = s->geo;
+  TRY(fill(s, s->h, s->hu, s->hv, s->br, nullptr, nullptr, nullptr));
+  Breaking fresh = breaking_of(s);
+  fresh.memory = nullptr;
+  TRY(launch(s, F_SWITCH, k_switch, over((long long)g.w * g.rows), dim3(NT), g,
+             (const double*)s->h, (const double*)s->b, fresh, (const uint8_t*)nullptr, s->phi));
+  TRY(launch(s, F_SWITCH, k_eddy_rise, over((long long)g.w * g.rows), dim3(NT), g,
+             (const double*)s->hu, (const double*)s->hv, s->e_rise));
+  TRY(launch(s, F_SWITCH, k_eddy_follow, over((long long)g.w * g.rows), dim3(NT), g, s->eddy,
+             (const double*)s->h, (const double*)s->br, (const double*)s->phi,
+             (const double*)s->e_rise, dt, s->e_mem, s->e_nu));
+  std::swap(s->br, s->e_mem);
+  if (s->eddy.mixing > 0.0) {
+    HIP_OK(hipMemsetAsync(s->red64, 0, sizeof(unsigned long long), s->stream));
+    TRY(launch(s, F_REDUCE, k_eddy_top, over((long long)g.nx * g.ny), dim3(NT), g,
+               (const double*)s->e_nu, s->red64));
+    HIP_OK(hipMemcpyAsync(s->red64_host, s->red64, sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    double top;
+    memcpy(&top, &s->red64_host[0], sizeof top);
+    if (top > 0.0) {
+      const double idx2 = 1.0 / (g.dx * g.dx), idy2 = 1.0 / (g.dy * g.dy);
+      const double limit = 0.5 / (top * wc_max(2.0 * idx2 + idy2, idx2 + 2.0 * idy2));
+      const unsigned long long steps = (unsigned long long)wc_max(ceil(dt / limit), 1.0);
+      const double sub = dt / (double)steps;
+      for (unsigned long long q = 0; q < steps; ++q) {
+        if (q > 0) TRY(fill(s, s->h, s->hu, s->hv, s->br, nullptr, nullptr, nullptr));
+        TRY(launch(s, F_SGN_FINISH, k_eddy_diffuse, over((long long)g.nx * g.ny), dim3(NT), g,
+                   (const double*)s->h, (const double*)s->hu, (const double*)s->hv,
+                   (const double*)s->e_nu, sub, s->e_hu, s->e_hv));
+        std::swap(s->hu, s->e_hu);
+        std::swap(s->hv, s->e_hv);
+      }
+    }
+  }
+  TRY(launch(s, F_SGN_FINISH, k_clean_dry, over((long long)g.nx * g.ny), dim3(NT), g, s->h, s->hu,
+             s->hv));
+  return WC_OK;
+}
+
 // Solver::step (solver.rs:481-507).
 int do_step(wc_solver* s, double dt) {
   const double t = s->time;
@@ -696,6 +750,7 @@ int do_step(wc_solver* s, double dt) {
     TRY(fill(s, s->h1, s->hu1, s->hv1, nullptr, s->h, s->hu, s->hv));
     TRY(stage(s, dt, 1, MODE_FINAL));
   }
+  TRY(follow_breaking(s, dt));
   s->time = t + dt;
   return WC_OK;
 }
@@ -824,17 +879,18 @@ CgG cg1_args(wc_solver* s, int stage) {
     a.s[m] = s->g_s[m];
   }
   a.lanes_b = s->lanes;
-  a.lanes = s->lanes + 256;
+  a.lanes = s->lanes + s->pcg_lanes;
   a.sc = s->sc;
   a.tolerance = s->disp.tolerance;
   a.max_iterations = s->disp.max_iterations;
   return a;
 }
 
-int pcg1_32(wc_solver* s, int stage, unsigned long long earlier = 0) {
+template <int L>
+int pcg1_32l(wc_solver* s, int stage, unsigned long long earlier) {
   const Sgn32 sa = sgn32_args(s, stage);
   const CgG a = cg1_args(s, stage);
-  const dim3 grid(256), block(1024);
+  const dim3 grid(L), block(1024);
   const size_t fbytes = s->cells * sizeof(float);
   static constexpr unsigned long long cap_it[wc_solver::CAP_SLOTS] = {0, 2, 5, 10};
   // training pairs: the depth, and (r_n, w_n) before iteration n of the solve
@@ -845,9 +901,9 @@ int pcg1_32(wc_solver* s, int stage, unsigned long long earlier = 0) {
     return WC_OK;
   };
   if (s->capture) HIP_OK(hipMemcpyAsync(s->cap_h, sa.h, fbytes, hipMemcpyDeviceToDevice, s->stream));
-  TRY(launch(s, F_CG_INIT, k_cg1_init, grid, block, a));
+  TRY(launch(s, F_CG_INIT, k_cg1_init<L>, grid, block, a));
   if (s->capture) TRY(capture_at(0, s->g_r[0]));
-  TRY(launch(s, F_CG_INIT, k_cg1_start, grid, block, a));
+  TRY(launch(s, F_CG_INIT, k_cg1_start<L>, grid, block, a));
   const unsigned long long maxit = s->disp.max_iterations;
   unsigned long long n = 1;
   unsigned long long batch = std::max<unsigned long long>(s->prev_it[stage], 1);
@@ -861,7 +917,7 @@ int pcg1_32(wc_solver* s, int stage, unsigned long long earlier = 0) {
     }
     const unsigned long long first = n;
     for (; n < stop; ++n) {
-      TRY(launch(s, F_CG_ITER, k_cg1_iter, grid, block, a, n));
+      TRY(launch(s, F_CG_ITER, k_cg1_iter<L>, grid, block, a, n));
       if (s->capture)  // K_n leaves r_n in r[n & 1] and w_n in w
         for (int slot = 1; slot < wc_solver::CAP_SLOTS; ++slot)
           if (n == cap_it[slot]) TRY(capture_at(slot, s->g_r[n & 1]));
@@ -870,7 +926,7 @@ int pcg1_32(wc_solver* s, int stage, unsigned long long earlier = 0) {
       HIP_OK(hipEventRecord(b1, s->stream));
       s->ev_pending.push_back({F_CG_ITER, b0, b1, n - first});
     }
-    TRY(launch(s, F_CG_REDUCE, k_cg1_check, dim3(1), dim3(64), a, n));
+    TRY(launch(s, F_CG_REDUCE, k_cg1_check<L>, dim3(1), dim3(64), a, n));
     HIP_OK(hipMemcpyAsync(s->sc_host, s->sc, sizeof(CgScalars), hipMemcpyDeviceToHost,
                           s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
@@ -881,6 +937,10 @@ int pcg1_32(wc_solver* s, int stage, unsigned long long earlier = 0) {
   s->cap_last_it = s->sc_host->it;
   TRY(launch(s, F_CG_STATS, k_cg_stats, dim3(1), dim3(64), s->sc, earlier));
   return WC_OK;
+}
+
+int pcg1_32(wc_solver* s, int stage, unsigned long long earlier = 0) {
+  return s->pcg_lanes == 512 ? pcg1_32l<512>(s, stage, earlier) : pcg1_32l<256>(s, stage, earlier);
 }
 
 // Flexible PCG with the network (wc_fpcg.hip), FP32 form: batches of iterations, each the input
@@ -1123,6 +1183,11 @@ int do_step32(wc_solver* s, double dt) {
                s->f_s1hl, s->f_s1hu, s->f_s1hv));
     TRY(stage32(s, dt, 1, MODE_FINAL));
   }
+  if (s->dispersive && s->eddy_on) {  // the fronts and their viscosity run in FP64
+    TRY(join_state(s));
+    TRY(follow_breaking(s, dt));
+    TRY(split_state(s));
+  }
   s->time = t + dt;
   return WC_OK;
 }
@@ -1212,7 +1277,7 @@ int wc_solver_new(uint64_t nx, uint64_t ny, double dx, double dy, const double* 
   const size_t ntiles = (size_t)g.ntx * g.nty;
   if ((rc = dalloc(s, &s->part, 4 * ntiles)) != WC_OK) return cleanup(rc);
   if ((rc = dalloc(s, &s->sc, 1)) != WC_OK) return cleanup(rc);
-  if ((rc = dalloc(s, &s->lanes, 8 * 256)) != WC_OK) return cleanup(rc);
+  if ((rc = dalloc(s, &s->lanes, 8 * 512)) != WC_OK) return cleanup(rc);
   if ((rc = dalloc(s, &s->red64, 4)) != WC_OK) return cleanup(rc);
   if ((rc = dalloc(s, &s->red32, 4)) != WC_OK) return cleanup(rc);
   if ((rc = dalloc(s, &s->redf, 4)) != WC_OK) return cleanup(rc);
@@ -1334,6 +1399,7 @@ int wc_solver_with_dispersion(wc_solver* s, const wc_dispersion_t* d) {
   if (d->reserved != 0) return fail(WC_EINVAL, "wc_dispersion_t.reserved must be 0");
   s->disp = *d;
   s->dispersive = true;
+  s->eddy_on = false;  // Breaking::viscosity comes with wc_solver_with_eddy_viscosity
   TRY(alloc_dispersion(s));
   // Work::new (dispersion.rs:237-266): fresh scratch and statistics.
   for (double* p : {s->c, s->dgx, s->dgy, s->bx, s->by, s->wx, s->wy, s->rx, s->ry, s->px, s->py,
@@ -1343,6 +1409,26 @@ int wc_solver_with_dispersion(wc_solver* s, const wc_dispersion_t* d) {
   HIP_OK(hipStreamSynchronize(s->stream));
   s->prev_it[0] = s->prev_it[1] = 0;
   return rebuild_drive(s);
+}
+
+int wc_solver_with_eddy_viscosity(wc_solver* s, const wc_eddy_t* e) {
+  if (!s) return fail(WC_EINVAL, "null solver");
+  if (!s->dispersive)
+    return fail(WC_EINVAL, "the eddy viscosity is part of the dispersion: set it after "
+                           "wc_solver_with_dispersion");
+  TRY(set_device(s));
+  if (!e) {
+    s->eddy_on = false;
+    return WC_OK;
+  }
+  if (!s->e_rise) {  // EddyWork::new (eddy.rs:102-113): zeros
+    for (double** p : {&s->e_rise, &s->e_mem, &s->e_nu, &s->e_hu, &s->e_hv})
+      TRY(dalloc(s, p, s->cells));
+  }
+  s->eddy = EddyP{e->initial, e->settled, e->duration, e->mixing, e->linger};
+  s->eddy_on = true;
+  HIP_OK(hipStreamSynchronize(s->stream));
+  return WC_OK;
 }
 
 int wc_solver_with_periodic_y(wc_solver* s) {
@@ -1385,6 +1471,15 @@ int wc_solver_set_option(wc_solver* s, const char* key, int64_t value) {
   if (strcmp(key, "pcg_kernels") == 0) {
     if (value < 1 || value > 4) return fail(WC_EINVAL, "pcg_kernels must be 1, 2, 3 or 4");
     s->pcg_kernels = (int)value;
+    return WC_OK;
+  }
+  if (strcmp(key, "pcg_lanes") == 0) {
+    if (value != 256 && value != 512) return fail(WC_EINVAL, "pcg_lanes must be 256 or 512");
+    TRY(set_device(s));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    s->pcg_lanes = (int)value;
+    // lanes past a launch's blocks must read zero
+    HIP_OK(hipMemsetAsync(s->lanes, 0, 8 * 512 * sizeof(double), s->stream));
     return WC_OK;
   }
   if (strcmp(key, "capture") == 0) {
@@ -2127,6 +2222,153 @@ int wc_relaxation_apply_device(wc_relaxation* r, double dt, const double* target
   return WC_OK;
 }
 
+}  // extern "C"
+
+// ---- the coarse-to-fine sampler of nested runs (crates/wavesim/src/lib.rs:1548-1616) --------
+
+struct wc_sampler {
+  int device = 0;
+  long long n = 0;       // zone cells
+  int cnx = 0, cny = 0;  // the coarse grid the corners index
+  int* idx = nullptr;    // device: four padded coarse indices per zone cell
+  double* w = nullptr;   // device: their bilinear weights
+  double* smp[2] = {nullptr, nullptr};  // device: (eta, u, v) per zone cell, then and now
+  double t[2] = {NAN, NAN};
+  double* targets = nullptr;  // device: the interpolated targets of the last apply
+  std::vector<void*> owned;
+};
+
+extern "C" {
+
+// Sampler::new (lib.rs:1559-1590): the fine zone cell (i, j) sits at x = (x0 + i - GHOST + 0.5)
+// fine cells of the bed file; its coarse neighbours are clamped to one ghost layer.
+int wc_sampler_new(const wc_relaxation* zone, uint64_t fine_x0, double fine_dx, double fine_dy,
+                   uint64_t coarse_nx, uint64_t coarse_ny, double coarse_dx, double coarse_dy,
+                   uint64_t crop, wc_sampler** out) {
+  if (!zone || !out) return fail(WC_EINVAL, "null argument");
+  if (!zone->s) return fail(WC_EINVAL, "the relaxation's solver has been freed");
+  if (coarse_nx < 3 || coarse_ny < 3 || !(coarse_dx > 0.0) || !(coarse_dy > 0.0))
+    return fail(WC_EINVAL, "the coarse grid needs at least 3 x 3 cells of positive size");
+  *out = nullptr;
+  const size_t n = zone->ci.size();
+  const long long cw = (long long)coarse_nx + 2 * GH;
+  std::vector<int> idx(4 * std::max<size_t>(n, 1), 0);
+  std::vector<double> w(4 * std::max<size_t>(n, 1), 0.0);
+  for (size_t m = 0; m < n; ++m) {
+    const double x = (double)(fine_x0 + zone->ci[m] - GH) * fine_dx + 0.5 * fine_dx;
+    const double y = (double)(zone->cj[m] - GH) * fine_dy + 0.5 * fine_dy;
+    const double cx = x / coarse_dx - (double)crop - 0.5;
+    const double cy = y / coarse_dy - 0.5;
+    const double ix = std::floor(cx), iy = std::floor(cy);
+    const double wx = cx - ix, wy = cy - iy;
+    auto at = [&](double di, double dj) {
+      const long long pi = std::clamp((long long)(ix + di) + GH, (long long)GH - 1,
+                                      (long long)coarse_nx + GH);
+      const long long pj = std::clamp((long long)(iy + dj) + GH, (long long)GH - 1,
+                                      (long long)coarse_ny + GH);
+      return (int)(pj * cw + pi);
+    };
+    idx[4 * m] = at(0.0, 0.0);
+    idx[4 * m + 1] = at(1.0, 0.0);
+    idx[4 * m + 2] = at(0.0, 1.0);
+    idx[4 * m + 3] = at(1.0, 1.0);
+    w[4 * m] = (1.0 - wx) * (1.0 - wy);
+    w[4 * m + 1] = wx * (1.0 - wy);
+    w[4 * m + 2] = (1.0 - wx) * wy;
+    w[4 * m + 3] = wx * wy;
+  }
+  auto sm = std::make_unique<wc_sampler>();
+  sm->device = zone->s->device;
+  sm->n = (long long)n;
+  sm->cnx = (int)coarse_nx;
+  sm->cny = (int)coarse_ny;
+  HIP_OK(hipSetDevice(sm->device));
+  struct Undo {
+    std::unique_ptr<wc_sampler>& sm;
+    ~Undo() {
+      if (sm)
+        for (void* p : sm->owned) (void)hipFree(p);
+    }
+  } undo{sm};
+  auto get = [&](void** p, size_t bytes) -> int {
+    HIP_OK(hipMalloc(p, std::max<size_t>(bytes, 8)));
+    sm->owned.push_back(*p);
+    return WC_OK;
+  };
+  TRY(get((void**)&sm->idx, idx.size() * sizeof(int)));
+  TRY(get((void**)&sm->w, w.size() * sizeof(double)));
+  for (int q = 0; q < 2; ++q) TRY(get((void**)&sm->smp[q], 3 * n * sizeof(double)));
+  TRY(get((void**)&sm->targets, 3 * n * sizeof(double)));
+  HIP_OK(hipMemcpy(sm->idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(sm->w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+  *out = sm.release();
+  return WC_OK;
+}
+
+void wc_sampler_free(wc_sampler* sm) {
+  if (!sm) return;
+  (void)hipSetDevice(sm->device);
+  for (void* p : sm->owned) (void)hipFree(p);
+  delete sm;
+}
+
+// Sampler::sample (lib.rs:1592-1614) of the coarse solver's current state into slot 0 (the
+// state before the coarse step that brackets the fine one) or 1 (after it), at its time.
+int wc_sampler_sample(wc_sampler* sm, wc_solver* coarse, int32_t slot) {
+  if (!sm || !coarse) return fail(WC_EINVAL, "null argument");
+  if (slot != 0 && slot != 1) return fail(WC_EINVAL, "slot must be 0 or 1");
+  if (coarse->geo.nx != sm->cnx || coarse->geo.ny != sm->cny)
+    return fail(WC_EINVAL, "the coarse solver's grid is not the sampler's");
+  if (coarse->device != sm->device) return fail(WC_EINVAL, "the sampler is on another device");
+  TRY(set_device(coarse));
+  TRY(join_state(coarse));
+  Geo g = coarse->geo;
+  g.periodic = 0;  // fill_boundaries(grid, false): walls
+  if (sm->n > 0)
+    TRY(launch(coarse, F_RELAX, k_sample, over(sm->n), dim3(NT), (long long)sm->n,
+               (const int*)sm->idx, (const double*)sm->w, g, (const double*)coarse->h,
+               (const double*)coarse->hu, (const double*)coarse->hv,
+               (const double*)coarse->b, sm->smp[slot]));
+  sm->t[slot] = coarse->time;
+  HIP_OK(hipStreamSynchronize(coarse->stream));
+  return WC_OK;
+}
+
+// The nested run's relaxation after a fine step of dt ending at `end` (lib.rs:1866-1897): the
+// targets interpolated between the two samples at w = (end - t0) / (t1 - t0) clamped to [0, 1],
+// then Relaxation::apply on the fine solver.
+int wc_sampler_apply(wc_sampler* sm, wc_relaxation* zone, double end, double dt) {
+  if (!sm || !zone) return fail(WC_EINVAL, "null argument");
+  if (!zone->s) return fail(WC_EINVAL, "the relaxation's solver has been freed");
+  if ((long long)zone->ci.size() != sm->n)
+    return fail(WC_EINVAL, "the relaxation is not the sampler's zone");
+  if (!(std::isfinite(sm->t[0]) && std::isfinite(sm->t[1])))
+    return fail(WC_EINVAL, "both samples must be taken before the targets are interpolated");
+  wc_solver* s = zone->s;
+  TRY(set_device(s));
+  const double w = std::clamp((end - sm->t[0]) / (sm->t[1] - sm->t[0]), 0.0, 1.0);
+  if (sm->n > 0)
+    TRY(launch(s, F_RELAX, k_lerp_targets, over(sm->n), dim3(NT), (long long)sm->n,
+               (const double*)sm->smp[0], (const double*)sm->smp[1], w, sm->targets));
+  TRY(relax_apply(zone, dt, sm->targets));
+  if (!s->prof) HIP_OK(hipStreamSynchronize(s->stream));
+  return WC_OK;
+}
+
+// slot 0 / 1: a sample, 2: the last targets; 3 n doubles (eta, u, v) per zone cell, and the
+// sample's time.
+int wc_sampler_read(const wc_sampler* sm, int32_t slot, double* out, double* time) {
+  if (!sm) return fail(WC_EINVAL, "null sampler");
+  if (slot < 0 || slot > 2) return fail(WC_EINVAL, "slot must be 0, 1 or 2");
+  HIP_OK(hipSetDevice(sm->device));
+  HIP_OK(hipDeviceSynchronize());
+  const double* src = slot == 2 ? sm->targets : sm->smp[slot];
+  if (out && sm->n)
+    HIP_OK(hipMemcpy(out, src, 3 * sm->n * sizeof(double), hipMemcpyDeviceToHost));
+  if (time) *time = slot == 2 ? NAN : sm->t[slot];
+  return WC_OK;
+}
+
 int wc_run_steps(wc_solver* s, uint64_t steps, uint64_t* done_steps, double* last_dt) {
   TRY(set_device(s));
   uint64_t n = 0;
@@ -2323,6 +2565,7 @@ struct wc_batch {
   std::vector<wc_solver*> inst;
   std::vector<int> active;  // indices of the instances still stepping
   bool rk2 = true, sw = false, dispersive = false;
+  int lanes = 256;  // the instances' one-kernel PCG lanes
   unsigned long long maxit = 0, pcg_batch = 4;
   int grid_nt = 2048, grid_pt = 512;  // persistent widths of the 256- and 1024-thread launches
   bool tuned = false;
@@ -2394,13 +2637,19 @@ int bhost(wc_batch* b, T** p, size_t n) {
   return WC_OK;
 }
 
-int pcg_blocks(const wc_solver* s) {  // the one-kernel PCG's 64x16 tiles, at most 256 lanes
+int pcg_blocks(const wc_solver* s) {  // the one-kernel PCG's 64x16 tiles, at most its lanes
   const int nt = ((s->geo.nx + 63) / 64) * ((s->geo.ny + 15) / 16);
-  return std::min(nt, 256);
+  return std::min(nt, s->pcg_lanes);
 }
 
 // (Re)builds every argument block and launch map for the active instances, in one upload.
 int batch_build(wc_batch* b) {
+  // largest instances first: their virtual blocks are the longest (a PCG block walks several
+  // tiles of a large grid), so they start first and the short ones fill the tail of a launch
+  std::stable_sort(b->active.begin(), b->active.end(), [&](int x, int y) {
+    const Geo &gx = b->inst[x]->geo, &gy = b->inst[y]->geo;
+    return (long long)gx.nx * gx.ny > (long long)gy.nx * gy.ny;
+  });
   const int n = (int)b->active.size();
   std::vector<int> f_fill(n + 1, 0), f_sw(n + 1, 0), f_tiles(n + 1, 0), f_rate(n + 1, 0),
       f_cg(n + 1, 0);
@@ -2540,12 +2789,38 @@ void enq_stage_head(wc_batch* b, int st) {  // the switch, the rates, the SGN se
                        b->stream, b->sgn[st], b->m_tiles);
 }
 
-void enq_pcg_head(wc_batch* b, int st, unsigned long long k) {  // init, start, iterations 1..k
+template <int L>
+void enq_pcg_head_l(wc_batch* b, int st, unsigned long long k) {
   const dim3 gp = bgrid(b->m_cg.total, b->grid_pt);
-  hipLaunchKernelGGL(kb_cg1_init, gp, dim3(1024), 0, b->stream, b->cg[st], b->m_cg);
-  hipLaunchKernelGGL(kb_cg1_start, gp, dim3(1024), 0, b->stream, b->cg[st], b->m_cg);
+  hipLaunchKernelGGL(kb_cg1_init<L>, gp, dim3(1024), 0, b->stream, b->cg[st], b->m_cg);
+  hipLaunchKernelGGL(kb_cg1_start<L>, gp, dim3(1024), 0, b->stream, b->cg[st], b->m_cg);
   for (unsigned long long it = 1; it <= k; ++it)
-    hipLaunchKernelGGL(kb_cg1_iter, gp, dim3(1024), 0, b->stream, b->cg[st], it, b->m_cg);
+    hipLaunchKernelGGL(kb_cg1_iter<L>, gp, dim3(1024), 0, b->stream, b->cg[st], it, b->m_cg);
+}
+
+void enq_pcg_head(wc_batch* b, int st, unsigned long long k) {  // init, start, iterations 1..k
+  if (b->lanes == 512)
+    enq_pcg_head_l<512>(b, st, k);
+  else
+    enq_pcg_head_l<256>(b, st, k);
+}
+
+void enq_iter(wc_batch* b, int st, unsigned long long it) {
+  const dim3 gp = bgrid(b->m_cg.total, b->grid_pt);
+  if (b->lanes == 512)
+    hipLaunchKernelGGL(kb_cg1_iter<512>, gp, dim3(1024), 0, b->stream, b->cg[st], it, b->m_cg);
+  else
+    hipLaunchKernelGGL(kb_cg1_iter<256>, gp, dim3(1024), 0, b->stream, b->cg[st], it, b->m_cg);
+}
+
+void enq_check(wc_batch* b, int st, unsigned long long it, int stats) {
+  const int n = (int)b->active.size();
+  if (b->lanes == 512)
+    hipLaunchKernelGGL(kb_cg1_check<512>, dim3(n), dim3(64), 0, b->stream, b->cg[st], it, stats,
+                       b->d_done, b->d_it);
+  else
+    hipLaunchKernelGGL(kb_cg1_check<256>, dim3(n), dim3(64), 0, b->stream, b->cg[st], it, stats,
+                       b->d_done, b->d_it);
 }
 
 void enq_update(wc_batch* b, int st, int mode) {
@@ -2557,7 +2832,6 @@ void enq_update(wc_batch* b, int st, int mode) {
 // flags per batch, then the solves are counted.
 int batch_pcg_tail(wc_batch* b, int st, unsigned long long k) {
   const int n = (int)b->active.size();
-  const dim3 gp = bgrid(b->m_cg.total, b->grid_pt);
   unsigned long long it = k + 1;
   unsigned long long batch = k > 0 ? b->pcg_batch : 1;
   if (k == 0) {
@@ -2569,13 +2843,11 @@ int batch_pcg_tail(wc_batch* b, int st, unsigned long long k) {
   while (true) {
     if (!look) {
       const unsigned long long stop = std::min(it + batch, b->maxit + 1);
-      for (; it < stop; ++it)
-        hipLaunchKernelGGL(kb_cg1_iter, gp, dim3(1024), 0, b->stream, b->cg[st], it, b->m_cg);
+      for (; it < stop; ++it) enq_iter(b, st, it);
       batch = b->pcg_batch;
     }
     look = false;
-    hipLaunchKernelGGL(kb_cg1_check, dim3(n), dim3(64), 0, b->stream, b->cg[st], it, 0,
-                       b->d_done, b->d_it);
+    enq_check(b, st, it, 0);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(b->h_done, b->d_done, n * sizeof(int), hipMemcpyDeviceToHost,
                           b->stream));
@@ -2584,8 +2856,7 @@ int batch_pcg_tail(wc_batch* b, int st, unsigned long long k) {
     for (int q = 0; q < n; ++q) all = all && b->h_done[q];
     if (all || it > b->maxit) break;
   }
-  hipLaunchKernelGGL(kb_cg1_check, dim3(n), dim3(64), 0, b->stream, b->cg[st], it, 1, b->d_done,
-                     b->d_it);
+  enq_check(b, st, it, 1);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(b->h_it, b->d_it, n * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                         b->stream));
@@ -2708,8 +2979,12 @@ int batch_tune(wc_batch* b) {
       double us = 0.0;
       const int rc = [&]() -> int {
         us = time([&] {
-          hipLaunchKernelGGL(kb_cg1_start, bgrid(b->m_cg.total, width), dim3(1024), 0, b->stream,
-                             b->cg[0], b->m_cg);
+          if (b->lanes == 512)
+            hipLaunchKernelGGL(kb_cg1_start<512>, bgrid(b->m_cg.total, width), dim3(1024), 0,
+                               b->stream, b->cg[0], b->m_cg);
+          else
+            hipLaunchKernelGGL(kb_cg1_start<256>, bgrid(b->m_cg.total, width), dim3(1024), 0,
+                               b->stream, b->cg[0], b->m_cg);
         });
         return WC_OK;
       }();
@@ -2743,6 +3018,7 @@ int wc_batch_new(wc_solver* const* solvers, uint64_t n, wc_batch** out) {
     if (s->precond != WC_PRECOND_JACOBI || s->capture || !(s->pcg_kernels == 0 || s->pcg_kernels == 4))
       return bfail("a batch runs the one-kernel Jacobi PCG, without capture");
     if (s->device != s0->device) return bfail("a batch's instances share one device");
+    if (s->pcg_lanes != s0->pcg_lanes) return bfail("a batch's instances share pcg_lanes");
     if ((s->order == WC_ORDER_FIRST) != (s0->order == WC_ORDER_FIRST) ||
         switch_on(s) != switch_on(s0) || s->dispersive != s0->dispersive)
       return bfail("a batch's instances share the time stepping, the switch and the dispersion");
@@ -2770,9 +3046,10 @@ int wc_batch_new(wc_solver* const* solvers, uint64_t n, wc_batch** out) {
     b->active.push_back((int)q);
     b->maxit = std::max<unsigned long long>(b->maxit, s->disp.max_iterations);
     // lanes past an instance's blocks must read zero
-    HIP_OK(hipMemsetAsync(s->lanes, 0, 8 * 256 * sizeof(double), b->stream));
+    HIP_OK(hipMemsetAsync(s->lanes, 0, 8 * 512 * sizeof(double), b->stream));
   }
   b->rk2 = s0->order != WC_ORDER_FIRST;
+  b->lanes = s0->pcg_lanes;
   b->sw = switch_on(s0);
   b->dispersive = s0->dispersive;
   b->pcg_batch = s0->pcg_batch;

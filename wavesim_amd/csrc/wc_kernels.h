@@ -11,6 +11,9 @@ enum { REDUCE_INIT = 0, REDUCE_ALPHA = 1, REDUCE_BETA = 2 };
 
 struct Breaking {
   double er_lo, er_hi, sl_lo, sl_hi;
+  // with the eddy viscosity on (Breaking::viscosity), the breaking memory of the state:
+  // where it is not zero the switch stays off (EddyViscosity::keeps_off, eddy.rs:116-118)
+  const double* memory;
 };
 
 // breaking_into's indicator at interior cell (i, j), padded index k (dispersion.rs:183-201):
@@ -25,8 +28,9 @@ __device__ __forceinline__ double breaking_at(const double* h, const double* b, 
   const double ey =
       ((ghosted(h, g, i, j + 1) + b[k + g.w]) - (ghosted(h, g, i, j - 1) + b[k - g.w])) * iy;
   const double ratio = wc_max((hc + b[k]) / hc, 0.0);
-  const double off = wc_max(wc_smoothstep(br.er_lo, br.er_hi, ratio),
-                            wc_smoothstep(br.sl_lo, br.sl_hi, wc_hypot(ex, ey)));
+  double off = wc_max(wc_smoothstep(br.er_lo, br.er_hi, ratio),
+                      wc_smoothstep(br.sl_lo, br.sl_hi, wc_hypot(ex, ey)));
+  if (br.memory && br.memory[k] != 0.0) off = 1.0;
   return 1.0 - (1.0 - off);
 }
 
@@ -166,9 +170,14 @@ __global__ void k_fcg_up(FcgArgs a, unsigned long long n);
 __global__ void k_fcg_check(FcgArgs a, unsigned long long n);
 __global__ void k_bed_features(Geo g, const double* b, double scale, float* bgx, float* bgy);
 
+// L: the blocks (and lanes) of one instance's launch, 256 or 512
+template <int L>
 __global__ void k_cg1_init(CgG a);
+template <int L>
 __global__ void k_cg1_start(CgG a);
+template <int L>
 __global__ void k_cg1_iter(CgG a, unsigned long long n);
+template <int L>
 __global__ void k_cg1_check(CgG a, unsigned long long n);
 
 template <typename V>
@@ -305,6 +314,10 @@ struct RelaxArgs {
   const double* b;
 };
 __global__ void k_relax64(RelaxArgs a, double* h, double* hu, double* hv);
+__global__ void k_sample(long long n, const int* idx, const double* w, Geo g, const double* h,
+                         const double* hu, const double* hv, const double* b, double* out);
+__global__ void k_lerp_targets(long long n, const double* from, const double* to, double w,
+                               double* out);
 __global__ void k_relax32(RelaxArgs a, float* h, float* hl, float* hu, float* hul, float* hv,
                           float* hvl);
 
@@ -355,10 +368,28 @@ __global__ void kb_sgn_setup32(const Sgn32* args, BMap m);
 __global__ void kb_stage32(const Stage32* args, const double* dt, int mode, BMap m);
 __global__ void kb_max_rate32(const BRate* args, BMap m);
 __global__ void kb_scatter(const double* src, const long long* off, double* const* dst, int n);
+template <int L>
 __global__ void kb_cg1_init(const CgG* args, BMap m);
+template <int L>
 __global__ void kb_cg1_start(const CgG* args, BMap m);
+template <int L>
 __global__ void kb_cg1_iter(const CgG* args, unsigned long long n, BMap m);
+template <int L>
 __global__ void kb_cg1_check(const CgG* args, unsigned long long n, int stats, int* done,
                              unsigned long long* it);
+
+// ---- breaking fronts and their eddy viscosity (eddy.rs; wc_eddy.hip) ---------------------
+struct EddyP {
+  double initial, settled, duration, mixing, linger;
+};
+// EddyViscosity::follow's two passes and diffuse's sub-step (eddy.rs:163-302)
+__global__ void k_eddy_rise(Geo g, const double* hu, const double* hv, double* rise);
+__global__ void k_eddy_follow(Geo g, EddyP e, const double* h, const double* old,
+                              const double* phi, const double* rise, double dt, double* memory,
+                              double* nu);
+__global__ void k_eddy_top(Geo g, const double* nu, unsigned long long* top);
+__global__ void k_eddy_diffuse(Geo g, const double* h, const double* hu, const double* hv,
+                               const double* nu, double sub, double* out_hu, double* out_hv);
+__global__ void k_clean_dry(Geo g, double* h, double* hu, double* hv);
 
 }  // namespace wc

@@ -60,8 +60,9 @@ __device__ __forceinline__ double switch_at(const Geo& g, const double* h, const
   const double ex = ((h[k + 1] + b[k + 1]) - (h[k - 1] + b[k - 1])) * ix;
   const double ey = ((h[k + g.w] + b[k + g.w]) - (h[k - g.w] + b[k - g.w])) * iy;
   const double ratio = wc_max((h[k] + b[k]) / h[k], 0.0);
-  const double off = wc_max(wc_smoothstep(br.er_lo, br.er_hi, ratio),
-                            wc_smoothstep(br.sl_lo, br.sl_hi, wc_hypot(ex, ey)));
+  double off = wc_max(wc_smoothstep(br.er_lo, br.er_hi, ratio),
+                      wc_smoothstep(br.sl_lo, br.sl_hi, wc_hypot(ex, ey)));
+  if (br.memory && br.memory[k] != 0.0) off = 1.0;  // a front, or the water just behind it
   return 1.0 - off;
 }
 

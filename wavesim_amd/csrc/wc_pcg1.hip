@@ -35,6 +35,7 @@ using namespace t16;
 
 // The scalars of iteration i = n - 1, from K_{n-1}'s lanes (or the start's); false when the
 // solve is over. Every wave computes the same numbers; block 0 records them.
+template <int L>
 __device__ __forceinline__ bool start(const CgG& a, unsigned long long n, int blk, double& alpha,
                                       double& beta) {
   CgScalars* sc = a.sc;
@@ -42,14 +43,14 @@ __device__ __forceinline__ bool start(const CgG& a, unsigned long long n, int bl
   double target;
   if (n == 1) {
     double b[1];
-    lane_sums<1>(a.lanes_b, b);
+    lane_sums<1, L>(a.lanes_b, b);
     target = a.tolerance * sqrt(b[0]) + 1e-10;
   } else {
     if (sc->done) return false;
     target = sc->target;
   }
   double s[3];
-  lane_sums<3>(a.lanes + 3 * LANES * (i & 1), s);
+  lane_sums<3, L>(a.lanes + 3 * L * (i & 1), s);
   const double gamma = s[0], delta = s[1];
   const int conv = sqrt(s[2]) <= target ? 1 : 0;
   const bool done = conv || i >= a.max_iterations;
@@ -77,14 +78,15 @@ __device__ __forceinline__ float2 f2(double x, double y) { return make_float2((f
 
 // r_0 = b - A w (w zero where masked, as set up), the inverse Jacobi diagonal
 // (dispersion.rs:466-480), p = s_{-1} = 0, lanes of b.b.
+template <int L>
 __device__ __forceinline__ void cg1_init_body(const CgG& a, int blk) {
   const Geo& g = a.g;
   const int nt = tiles_all(g);
   __shared__ double s_u[2][S2Y][S2X], s_f[S1Y][S1X];
   __shared__ double sums[2][BY][1];
   double acc[1] = {0.0};
-  for (int r = 0; blk + LANES * r < nt; ++r) {
-    const int tile = blk + LANES * r;
+  for (int r = 0; blk + L * r < nt; ++r) {
+    const int tile = blk + L * r;
     bool own;
     int k;
     double ux, uy, ax, ay;
@@ -115,17 +117,19 @@ __device__ __forceinline__ void cg1_init_body(const CgG& a, int blk) {
   if (threadIdx.x == 0) a.lanes_b[blk] = acc[0];
 }
 
-__global__ void __launch_bounds__(PT) k_cg1_init(CgG a) { cg1_init_body(a, blockIdx.x); }
+template <int L>
+__global__ void __launch_bounds__(PT) k_cg1_init(CgG a) { cg1_init_body<L>(a, blockIdx.x); }
 
 // u_0 = D^-1 r_0, q_0 = A u_0; lanes of gamma_0 = r.u, delta_0 = q.u, r.r (slot 0).
+template <int L>
 __device__ __forceinline__ void cg1_start_body(const CgG& a, int blk) {
   const Geo& g = a.g;
   const int nt = tiles_all(g);
   __shared__ double s_u[2][S2Y][S2X], s_f[S1Y][S1X];
   __shared__ double sums[2][BY][3];
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int r = 0; blk + LANES * r < nt; ++r) {
-    const int tile = blk + LANES * r;
+  for (int r = 0; blk + L * r < nt; ++r) {
+    const int tile = blk + L * r;
     bool own;
     int k;
     double ux, uy, ax, ay;
@@ -149,17 +153,19 @@ __device__ __forceinline__ void cg1_start_body(const CgG& a, int blk) {
     add_rows<3>(v, sums, r, acc);
   }
   if (threadIdx.x == 0)
-    for (int m = 0; m < 3; ++m) a.lanes[m * LANES + blk] = acc[m];
+    for (int m = 0; m < 3; ++m) a.lanes[m * L + blk] = acc[m];
 }
 
-__global__ void __launch_bounds__(PT) k_cg1_start(CgG a) { cg1_start_body(a, blockIdx.x); }
+template <int L>
+__global__ void __launch_bounds__(PT) k_cg1_start(CgG a) { cg1_start_body<L>(a, blockIdx.x); }
 
 // Iteration i = n - 1 (see the top of the file).
+template <int L>
 __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n, int blk) {
   const Geo& g = a.g;
   const int nt = tiles_all(g);
   double alpha = 0.0, beta = 0.0;
-  if (blk < nt && !start(a, n, blk, alpha, beta)) return;
+  if (blk < nt && !start<L>(a, n, blk, alpha, beta)) return;
   const unsigned long long i = n - 1;
   const int cur = (int)(i & 1), nxt = cur ^ 1;
   const float2 *r0 = a.r[cur], *q0 = a.q[cur];
@@ -168,8 +174,8 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
   __shared__ double s_u[2][S2Y][S2X], s_f[S1Y][S1X];
   __shared__ double sums[2][BY][3];
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int r = 0; blk + LANES * r < nt; ++r) {
-    const int tile = blk + LANES * r;
+  for (int r = 0; blk + L * r < nt; ++r) {
+    const int tile = blk + L * r;
     bool own;
     int k;
     double ux, uy, ax, ay;
@@ -206,54 +212,60 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
     add_rows<3>(v, sums, r, acc);
   }
   if (threadIdx.x == 0)
-    for (int m = 0; m < 3; ++m) a.lanes[3 * LANES * (n & 1) + m * LANES + blk] = acc[m];
+    for (int m = 0; m < 3; ++m) a.lanes[3 * L * (n & 1) + m * L + blk] = acc[m];
 }
 
+template <int L>
 __global__ void __launch_bounds__(PT) k_cg1_iter(CgG a, unsigned long long n) {
-  cg1_iter_body(a, n, blockIdx.x);
+  cg1_iter_body<L>(a, n, blockIdx.x);
 }
 
 // K_n's prologue alone, one wave: whether the solve is over before iteration n - 1.
+template <int L>
 __global__ void __launch_bounds__(64) k_cg1_check(CgG a, unsigned long long n) {
   double alpha, beta;
-  start(a, n, 0, alpha, beta);
+  start<L>(a, n, 0, alpha, beta);
 }
 
 // ---- many instances in one launch (see kb_fill32) -------------------------------------------
 // Instance q runs min(256, its tiles) virtual blocks; its lanes past those stay zero, as the
 // single launch's empty blocks write them, so the sums and every iterate are the same.
+template <int L>
 __global__ void __launch_bounds__(PT) kb_cg1_init(const CgG* args, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
-    cg1_init_body(args[q], vb - m.first[q]);
+    cg1_init_body<L>(args[q], vb - m.first[q]);
     __syncthreads();
   }
 }
 
+template <int L>
 __global__ void __launch_bounds__(PT) kb_cg1_start(const CgG* args, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
-    cg1_start_body(args[q], vb - m.first[q]);
+    cg1_start_body<L>(args[q], vb - m.first[q]);
     __syncthreads();
   }
 }
 
+template <int L>
 __global__ void __launch_bounds__(PT) kb_cg1_iter(const CgG* args, unsigned long long n, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
-    cg1_iter_body(args[q], n, vb - m.first[q]);
+    cg1_iter_body<L>(args[q], n, vb - m.first[q]);
     __syncthreads();
   }
 }
 
 // One wave per instance: k_cg1_check, then the instance's flag and iteration into done[q],
 // it[q]; with `stats` (the solve is over) DispersionStats counts the solve as k_cg_stats does.
+template <int L>
 __global__ void __launch_bounds__(64) kb_cg1_check(const CgG* args, unsigned long long n,
                                                    int stats, int* done, unsigned long long* it) {
   const int q = blockIdx.x;
   const CgG& a = args[q];
   double alpha, beta;
-  start(a, n, 0, alpha, beta);
+  start<L>(a, n, 0, alpha, beta);
   if (threadIdx.x == 0) {
     CgScalars* sc = a.sc;
     done[q] = sc->done;
@@ -265,5 +277,19 @@ __global__ void __launch_bounds__(64) kb_cg1_check(const CgG* args, unsigned lon
     }
   }
 }
+
+#define WC_CG1_LANES(L)                                                                      \
+  template __global__ void k_cg1_init<L>(CgG);                                                 \
+  template __global__ void k_cg1_start<L>(CgG);                                                \
+  template __global__ void k_cg1_iter<L>(CgG, unsigned long long);                             \
+  template __global__ void k_cg1_check<L>(CgG, unsigned long long);                            \
+  template __global__ void kb_cg1_init<L>(const CgG*, BMap);                                   \
+  template __global__ void kb_cg1_start<L>(const CgG*, BMap);                                  \
+  template __global__ void kb_cg1_iter<L>(const CgG*, unsigned long long, BMap);               \
+  template __global__ void kb_cg1_check<L>(const CgG*, unsigned long long, int, int*,          \
+                                           unsigned long long*);
+WC_CG1_LANES(256)
+WC_CG1_LANES(512)
+#undef WC_CG1_LANES
 
 }  // namespace wc

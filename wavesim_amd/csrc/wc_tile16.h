@@ -16,17 +16,27 @@ constexpr int S2X = BX + 4, S2Y = BY + 4;
 constexpr int S1X = BX + 2, S1Y = BY + 2;
 constexpr int H2 = S2X * S2Y, H1 = S1X * S1Y;
 
-template <int N>
+template <int N, int L = LANES>
 __device__ __forceinline__ void lane_sums(const double* acc, double (&out)[N]) {
+  static_assert(L == 256 || L == 512, "256 or 512 lanes");
   const int lane = threadIdx.x & 63;
-  double v[N][4];
+  constexpr int G = L / 64;
+  double v[N][G];
 #pragma unroll
   for (int n = 0; n < N; ++n)
 #pragma unroll
-    for (int w = 0; w < 4; ++w) v[n][w] = acc[n * LANES + 64 * w + lane];
+    for (int w = 0; w < G; ++w) v[n][w] = acc[n * L + 64 * w + lane];
 #pragma unroll
-  for (int n = 0; n < N; ++n)
-    out[n] = (wave_sum(v[n][0]) + wave_sum(v[n][1])) + (wave_sum(v[n][2]) + wave_sum(v[n][3]));
+  for (int n = 0; n < N; ++n) {
+    const double lo = (wave_sum(v[n][0]) + wave_sum(v[n][1])) + (wave_sum(v[n][2]) + wave_sum(v[n][3]));
+    if constexpr (G == 4) {
+      out[n] = lo;
+    } else {
+      const double hi =
+          (wave_sum(v[n][4]) + wave_sum(v[n][5])) + (wave_sum(v[n][6]) + wave_sum(v[n][7]));
+      out[n] = lo + hi;
+    }
+  }
 }
 
 __device__ __forceinline__ int tiles_x(const Geo& g) { return (g.nx + BX - 1) / BX; }

@@ -30,6 +30,7 @@ from .config import (
     Sponge,
     WaveMaker,
     c_dispersion,
+    c_eddy,
     c_sponge,
     c_wavemaker,
 )
@@ -102,6 +103,13 @@ SIGNATURES = {
     "wc_relaxation_cells": (_I32, [_VP, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
     "wc_relaxation_apply": (_I32, [_VP, _D, _P]),
     "wc_relaxation_apply_device": (_I32, [_VP, _D, _VP]),
+    "wc_solver_with_eddy_viscosity": (_I32, [_VP, ctypes.c_void_p]),
+    "wc_sampler_new": (_I32, [_VP, _U64, _D, _D, _U64, _U64, _D, _D, _U64,
+                              ctypes.POINTER(_VP)]),
+    "wc_sampler_free": (None, [_VP]),
+    "wc_sampler_sample": (_I32, [_VP, _VP, _I32]),
+    "wc_sampler_apply": (_I32, [_VP, _VP, _D, _D]),
+    "wc_sampler_read": (_I32, [_VP, _I32, _P, ctypes.POINTER(_D)]),
     "wc_batch_new": (_I32, [ctypes.POINTER(_VP), _U64, ctypes.POINTER(_VP)]),
     "wc_batch_free": (None, [_VP]),
     "wc_batch_set_option": (_I32, [_VP, ctypes.c_char_p, ctypes.c_int64]),
@@ -222,6 +230,9 @@ class Solver:
     def with_dispersion(self, d: Dispersion) -> "Solver":
         c = c_dispersion(d)
         _check(lib().wc_solver_with_dispersion(self._h, ctypes.byref(c)))
+        if d.breaking.viscosity is not None:
+            e = c_eddy(d.breaking.viscosity)
+            _check(lib().wc_solver_with_eddy_viscosity(self._h, ctypes.byref(e)))
         self._dispersion = d
         return self
 
@@ -519,6 +530,44 @@ def relaxation_targets(target, n: int) -> np.ndarray:
     else:
         out = np.array(target, dtype=np.float64).reshape(n, 3)
     return np.ascontiguousarray(out.reshape(-1))
+
+
+class Sampler:
+    """wavesim's coarse-to-fine Sampler (crates/wavesim/src/lib.rs:1548-1616) for a fine
+    solver's relaxation zone: ``sample(coarse, state, slot)`` before (slot 0) and after (slot 1)
+    the coarse step that brackets the fine step, ``apply(state, end, dt)`` after the fine step
+    (targets interpolated in time, then Relaxation::apply)."""
+
+    def __init__(self, zone: "Relaxation", fine_x0: int, fine_grid, coarse_grid, crop: int = 0):
+        self.zone = zone
+        self.n = len(zone)
+        h = _VP()
+        _check(lib().wc_sampler_new(zone._r, int(fine_x0), float(fine_grid.dx),
+                                    float(fine_grid.dy), coarse_grid.nx, coarse_grid.ny,
+                                    float(coarse_grid.dx), float(coarse_grid.dy), int(crop),
+                                    ctypes.byref(h)))
+        self._s = h
+
+    def __del__(self):
+        if getattr(self, "_s", None):
+            lib().wc_sampler_free(self._s)
+            self._s = None
+
+    def sample(self, coarse: "Solver", state: State, slot: int) -> None:
+        coarse._hold(state)
+        _check(lib().wc_sampler_sample(self._s, coarse._h, int(slot)))
+
+    def apply(self, state: State, end: float, dt: float) -> None:
+        self.zone.solver._hold(state)
+        _check(lib().wc_sampler_apply(self._s, self.zone._r, float(end), float(dt)))
+        state._host_valid = False
+
+    def read(self, slot: int):
+        """(targets as an (n, 3) array, time) of a sample (0, 1) or the last targets (2)."""
+        out = np.empty(3 * max(self.n, 1))
+        t = ctypes.c_double()
+        _check(lib().wc_sampler_read(self._s, int(slot), _ptr(out), ctypes.byref(t)))
+        return out[:3 * self.n].reshape(self.n, 3), t.value
 
 
 class Batch:
