@@ -74,6 +74,9 @@ def parse():
     p.add_argument("--steps", type=int, default=300)
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--spinup", type=int, default=1500)
+    p.add_argument("--spinup-precision", default="fp32", choices=["fp64", "fp32"],
+                   help="the path that brings the run to its mid-run state (every measured path "
+                        "then starts from that one state)")
     p.add_argument("--workload", default="rough_bed",
                    choices=["rough_bed", "plane_beach", "large_reef", "sweep"])
     p.add_argument("--graph", type=int, default=None, choices=[0, 1],
@@ -89,6 +92,11 @@ def parse():
     p.add_argument("--precision", default="both", choices=["fp64", "fp32", "both"])
     p.add_argument("--no-network", action="store_true",
                    help="skip the FP32 path with the learned preconditioner")
+    p.add_argument("--no-breakdown", action="store_true",
+                   help="skip the FP32 path with the reference's warm start (the gain's split)")
+    p.add_argument("--warm-start", type=int, default=None, choices=[0, 1, 2, 3],
+                   help="FP32 form: order of the extrapolated warm start (default 3; 0: the "
+                        "reference's start from the previous solution)")
     p.add_argument("--pcg-lanes", type=int, default=None, choices=[256, 512],
                    help="blocks of the FP32 form's one-kernel PCG launches")
     p.add_argument("--pcg32", type=int, default=None,
@@ -113,7 +121,7 @@ def calibrate_copy():
 
 
 def measure(w, precision, arrays, t0, warmup, steps, profile_steps, pcg32=None, network=None,
-            lanes=None):
+            lanes=None, warm_start=None, label=None):
     """Time `steps` steps of one precision from the given state; per-kernel times with HIP
     events on the solver's stream: for the one-kernel PCG iteration of the FP32 form, one pair
     of events around each batch of its launches inside the timed window itself; for the kernel
@@ -124,9 +132,11 @@ def measure(w, precision, arrays, t0, warmup, steps, profile_steps, pcg32=None, 
     from wavesim_amd.solver import PRECISION_FP32, Solver
 
     g = w.solver(Solver)
-    label = precision if network is None else precision + "+network"
+    label = label or (precision if network is None else precision + "+network")
     if precision == "fp32":
         g.with_precision(PRECISION_FP32)
+        if warm_start is not None:
+            g.set_option("warm_start", warm_start)
         if pcg32:
             g.set_option("pcg_kernels", pcg32)
         if lanes:
@@ -315,7 +325,7 @@ def main():
     import torch
 
     from wavesim_amd import configs
-    from wavesim_amd.solver import Solver
+    from wavesim_amd.solver import PRECISION_FP32, Solver
 
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no GPU"}))
@@ -330,8 +340,11 @@ def main():
     w = {"rough_bed": configs.c1_rough_bed, "plane_beach": configs.c0_plane_beach,
          "large_reef": configs.c4_large_reef}[args.workload]()
     g = w.solver(Solver)
+    if args.spinup_precision == "fp32":
+        g.with_precision(PRECISION_FP32)
     s = w.still_water()
-    print(f"[bench] {w.name}: spin-up {args.spinup} steps", file=sys.stderr, flush=True)
+    print(f"[bench] {w.name}: spin-up {args.spinup} steps ({args.spinup_precision})",
+          file=sys.stderr, flush=True)
     t = time.perf_counter()
     done, _ = g.run_steps(s, args.spinup)
     print(f"[bench] spin-up {done} steps in {time.perf_counter() - t:.1f} s, t = {s.time:.1f} s",
@@ -341,13 +354,18 @@ def main():
     del g
     precisions = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
     res = {p: measure(w, p, arrays, t0, args.warmup, args.steps, args.profile_steps,
-                      args.pcg32, lanes=args.pcg_lanes)
+                      args.pcg32, lanes=args.pcg_lanes, warm_start=args.warm_start)
            for p in precisions}
+    if "fp32" in res and not args.no_breakdown and args.warm_start != 0:
+        # the FP32 form with the reference's warm start: what storage alone gains
+        res["fp32+previous_start"] = measure(w, "fp32", arrays, t0, args.warmup, args.steps,
+                                             args.profile_steps, args.pcg32, lanes=args.pcg_lanes,
+                                             warm_start=0, label="fp32+previous_start")
     net_file = os.path.join(ROOT, "wavesim_amd", "data", "precond_rough_bed.npy")
     if "fp32" in res and not args.no_network and os.path.exists(net_file):
         res["fp32+network"] = measure(w, "fp32", arrays, t0, args.warmup, args.steps,
                                       args.profile_steps, args.pcg32, np.load(net_file),
-                                      lanes=args.pcg_lanes)
+                                      lanes=args.pcg_lanes, warm_start=args.warm_start)
     accs = {p: accuracy(w, res["fp64"], res[p]) for p in res if p != "fp64"} \
         if "fp64" in res else {}
     # the chosen form: the fastest path within the accuracy bounds against FP64
@@ -390,8 +408,14 @@ def main():
                    "preconditioner": ("learned network, flexible PCG, Jacobi fallback"
                                       if chosen.endswith("network")
                                       else "Jacobi (the reference's)"),
+                   "warm_start": ("the previous solution (the reference's)"
+                                  if not chosen.startswith("fp32") or chosen.endswith(
+                                      "previous_start") or args.warm_start == 0
+                                  else f"extrapolated, order "
+                                       f"{3 if args.warm_start is None else args.warm_start}"),
                    "pcg_tolerance": 1e-8,
-                   "spinup_steps": args.spinup, "t_start_s": round(t0, 3), **w.notes},
+                   "spinup_steps": args.spinup, "spinup_path": args.spinup_precision,
+                   "t_start_s": round(t0, 3), **w.notes},
         "pcg_iters_per_solve": r["pcg_iters_per_solve"],
         "roofline": {"bound": "hbm", "kernel": r["dominant"], "achieved": r["achieved"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["achieved"] / HBM_PEAK_GBS,

@@ -84,6 +84,8 @@ def lib():
             "wco_set_dot_order": (_I32, [_VP, _I32]),
             "wco_set_math": (_I32, [_VP, _I32]),
             "wco_set_threads": (_I32, [_VP, _I32]),
+            "wco_set_warm_start": (_I32, [_VP, _I32]),
+            "wco_ws_extrapolate": (_I32, [_I32, _I32, _U64, _P, _P]),
             "wco_state_set": (_I32, [_VP, _P, _P, _P, _P, _D]),
             "wco_state_get": (_I32, [_VP, _P, _P, _P, _P, _P]),
             "wco_stable_dt": (_I32, [_VP, _P]),
@@ -178,6 +180,12 @@ class OracleSolver:
     def with_order(self, order: Order) -> "OracleSolver":
         _check(lib().wco_with_order(self._h, int(order)))
         self.order = Order(order)
+        return self
+
+    def with_warm_start(self, order: int) -> "OracleSolver":
+        """The FP32 form's extrapolated warm start of the SGN solves (order 0: the reference's
+        start from the previous solution). Set after the dispersion."""
+        _check(lib().wco_set_warm_start(self._h, int(order)))
         return self
 
     def with_manning(self, n: float) -> "OracleSolver":
@@ -523,3 +531,13 @@ def eddy_diffuse(grid, h, hu, hv, nu, dt):
     _check(lib().wco_eddy_diffuse(grid.nx, grid.ny, grid.dx, grid.dy, ptr(h), ptr(hu), ptr(hv),
                                   ptr(nu), float(dt), ctypes.byref(n)))
     return hu, hv, int(n.value)
+
+
+def ws_extrapolate(order: int, period: int, past) -> float:
+    """The warm start of solve n for one cell from past[i] = x_{n-1-i}, as the oracle's solver
+    (and the FP32 form) computes it."""
+    a = np.ascontiguousarray(past, dtype=np.float64)
+    out = ctypes.c_double()
+    _check(lib().wco_ws_extrapolate(int(order), int(period), a.size,
+                                    a.ctypes.data_as(_P), ctypes.byref(out)))
+    return out.value

@@ -464,10 +464,31 @@ double dot(const Grid& g, const Par& par, int order, const double* ax, const dou
 }
 
 // ---------------------------------------------------------------- SGN dispersion
+// The FP32 form's extrapolated warm start (not the reference's: it starts each solve from the
+// previous solution, dispersion.rs:220-222). Solves come `period` per step; with d_n = x_n -
+// x_{n-1} the start of solve n is x_{n-1} + sum_{j=1..m} (-1)^(j+1) C(m, j) d_{n - j period}:
+// the m-th backward difference (stride `period`) of the steps d set to zero. Exact when the
+// solutions are a polynomial of degree <= m in time plus, per kind of solve (RK stage), one of
+// degree <= m - 1: then each kind's steps are of degree <= m - 1. `step(j)` is d_{n - j period}.
+constexpr int WS_SLOTS = 8;
+template <class D>
+double ws_extrapolate(double x1, int order, D step) {
+  double x = x1, cf = 1.0;
+  for (int j = 1; j <= order; ++j) {
+    cf = cf * (double)(order - j + 1) / (double)j;
+    x += ((j & 1) ? cf : -cf) * step(j);
+  }
+  return x;
+}
+
 struct Work {
   std::vector<double> u, v, mask, um, vm, div, c, f, heff, diag_x, diag_y, wx, wy, bx, by, rx, ry,
       zx, zy, px, py, ax, ay;
   wco_dispersion_stats_t stats{0, 0, 0};
+  // the extrapolated warm start (order 0: the reference's): steps d of the last WS_SLOTS solves,
+  // slot n % WS_SLOTS for solve n (which holds x_{n-1} while solve n runs)
+  int ws_order = 0, ws_period = 2;
+  std::vector<double> ws_x[WS_SLOTS], ws_y[WS_SLOTS];
   void init(const Grid& g) {
     const size_t n = g.cells();
     for (auto* a : {&u, &v, &mask, &um, &vm, &div, &c, &f, &heff, &diag_x, &diag_y, &wx, &wy,
@@ -589,12 +610,36 @@ void dispersion_apply(const Grid& g, const Par& par, int dot_order, const Disper
       by[k] = lhv[k] - v[k] * lh[k] + (f[k + w] - f[k - w]) * iy;
     }
   });
-  interior(g, par, [&](size_t k) {
-    if (mask[k] == 0.0) {
-      wx[k] = 0.0;
-      wy[k] = 0.0;
-    }
-  });
+  if (wk.ws_order > 0) {
+    const uint64_t n = wk.stats.solves;
+    for (int m = 0; m < WS_SLOTS; ++m)
+      if (wk.ws_x[m].size() != g.cells()) {
+        wk.ws_x[m].assign(g.cells(), 0.0);
+        wk.ws_y[m].assign(g.cells(), 0.0);
+      }
+    double *ex = wk.ws_x[n % WS_SLOTS].data(), *ey = wk.ws_y[n % WS_SLOTS].data();
+    auto slot = [&](int j) { return (n - (uint64_t)(j * wk.ws_period)) % WS_SLOTS; };
+    interior(g, par, [&](size_t k) {
+      if (mask[k] == 0.0) {
+        wx[k] = 0.0;
+        wy[k] = 0.0;
+        ex[k] = 0.0;
+        ey[k] = 0.0;
+      } else {
+        ex[k] = wx[k];
+        ey[k] = wy[k];
+        wx[k] = ws_extrapolate(wx[k], wk.ws_order, [&](int j) { return wk.ws_x[slot(j)][k]; });
+        wy[k] = ws_extrapolate(wy[k], wk.ws_order, [&](int j) { return wk.ws_y[slot(j)][k]; });
+      }
+    });
+  } else {
+    interior(g, par, [&](size_t k) {
+      if (mask[k] == 0.0) {
+        wx[k] = 0.0;
+        wy[k] = 0.0;
+      }
+    });
+  }
 
   matvec(g, par, periodic, heff, mask, c, f, wx, wy, ax, ay);
   interior(g, par, [&](size_t k) {
@@ -651,6 +696,14 @@ void dispersion_apply(const Grid& g, const Par& par, int dot_order, const Disper
       lhv[k] = h[k] * wy[k] + v[k] * lh[k];
     }
   });
+  if (wk.ws_order > 0) {  // slot n: x_{n-1} -> d_n = x_n - x_{n-1}
+    double *ex = wk.ws_x[(wk.stats.solves - 1) % WS_SLOTS].data(),
+           *ey = wk.ws_y[(wk.stats.solves - 1) % WS_SLOTS].data();
+    interior(g, par, [&](size_t k) {
+      ex[k] = wx[k] - ex[k];
+      ey[k] = wy[k] - ey[k];
+    });
+  }
 }
 
 // dispersion.rs:120-160; with the eddy viscosity on, a cell with breaking memory stays off.
@@ -1078,6 +1131,7 @@ int wco_set_cfl(wco_solver* s, double cfl) {
 int wco_with_order(wco_solver* s, int32_t order) {
   if (order < 0 || order > 2) return fail("order must be 0 (first), 1 (second) or 2 (fifth)");
   s->order = order;
+  s->work.ws_period = order == ORDER_FIRST ? 1 : 2;
   return 0;
 }
 
@@ -1276,6 +1330,30 @@ int wco_set_dot_order(wco_solver* s, int32_t order) {
 int wco_set_math(wco_solver* s, int32_t mode) {
   if (mode != 0 && mode != 1) return fail("math must be 0 (glibc) or 1 (portable)");
   s->math.portable = mode == 1;
+  return 0;
+}
+
+int wco_set_warm_start(wco_solver* s, int32_t order) {
+  if (order < 0 || order > 3) return fail("warm start order must be in [0, 3]");
+  s->work.ws_order = order;
+  s->work.ws_period = s->order == ORDER_FIRST ? 1 : 2;
+  for (int m = 0; m < WS_SLOTS; ++m) {
+    s->work.ws_x[m].clear();
+    s->work.ws_y[m].clear();
+  }
+  return 0;
+}
+
+int wco_ws_extrapolate(int32_t order, int32_t period, uint64_t n_past, const double* past,
+                       double* out) {
+  if (order < 0 || order > 3 || period < 1 || period > 2)
+    return fail("order must be in [0, 3] and period 1 or 2");
+  if (n_past < 1) return fail("need x_{n-1}");
+  // past[i] = x_{n-1-i}; d_{n-m} = past[m-1] - past[m], zero where the history runs out
+  *out = ws_extrapolate(past[0], order, [&](int j) {
+    const uint64_t m = (uint64_t)(j * period);
+    return m < n_past ? past[m - 1] - past[m] : 0.0;
+  });
   return 0;
 }
 

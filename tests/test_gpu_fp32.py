@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 import known_answers as ka
+from helpers import bore_state
 from wavesim_amd import configs
 from wavesim_amd.config import Dispersion, Order
 from wavesim_amd.grid import GHOST, Bathymetry, Grid, State
@@ -130,15 +131,11 @@ def test_fp32_breaking_fronts_with_eddy_viscosity_stay_near_fp64():
     """Breaking::viscosity in the FP32 form (the fronts and the viscosity run on the FP64 state
     between the FP32 steps): the surface stays within the form's bound of the FP64 path."""
     from wavesim_amd.config import Breaking, EddyViscosity
-    from wavesim_amd.grid import GHOST
 
     grid = Grid(80, 24, 0.5, 0.5)
     bed = Bathymetry.flat(grid, 1.0)
     disp = Dispersion(breaking=Breaking(viscosity=EddyViscosity()), tolerance=1e-8)
-    s = State.lake_at_rest(grid, bed, 0.0)
-    x, y = grid.centres()
-    X, Y = np.meshgrid(x[GHOST:GHOST + grid.nx], y[GHOST:GHOST + grid.ny])
-    grid.inner(s.h)[:, :] += np.exp(-((X - 20.0) ** 2 + (Y - 6.0) ** 2) / 12.5)
+    s = bore_state(grid, 1.0)
     g64 = Solver(grid, bed).with_order(Order.Fifth).with_dispersion(disp)
     g32 = fp32(grid, bed).with_order(Order.Fifth).with_dispersion(disp)
     s64, s32 = s.clone(), s.clone()

@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 import known_answers as ka
+from helpers import bore_state
 from oracle import oracle as orc
 from wavesim_amd import configs
 from wavesim_amd.config import Dispersion, Order, Sponge, WaveMaker, linear_wave
@@ -223,9 +224,9 @@ def test_invalid_setups_raise_like_the_reference_panics():
 @pytest.mark.parametrize("pcg", [1, 3])
 @pytest.mark.parametrize("periodic", [False, True])
 def test_breaking_fronts_and_eddy_viscosity(periodic, pcg):
-    """Breaking::viscosity on (solver.rs:512-532, eddy.rs): a hump as tall as the water is deep
-    breaks at once; the fronts, their memory in the breaking array, the switch kept off behind
-    them and the sub-stepped eddy viscosity must match the oracle bit for bit."""
+    """Breaking::viscosity on (solver.rs:512-532, eddy.rs): a running hump as tall as the water
+    is deep breaks at once; the fronts, their memory in the breaking array, the switch kept off
+    behind them and the sub-stepped eddy viscosity must match the oracle bit for bit."""
     from wavesim_amd.config import Breaking, EddyViscosity
 
     grid = Grid(80, 24, 0.5, 0.5)
@@ -236,7 +237,7 @@ def test_breaking_fronts_and_eddy_viscosity(periodic, pcg):
         s = s.with_order(Order.Fifth).with_dispersion(disp)
         return s.with_periodic_y() if periodic else s
 
-    s = hump_state(grid, bed, amp=1.0, x0=20.0, y0=3.0 if periodic else 6.0, r=2.5)
+    s = bore_state(grid, 1.0)
     g, o, sg, so = lockstep(grid, bed, configure, s, 160,
                             f"eddy periodic={periodic} pcg={pcg}", options={"pcg_kernels": pcg})
     br = sg.arrays()[3]

@@ -160,6 +160,9 @@ struct wc_solver {
   // the one-kernel PCG's own vectors (wc_pcg1.hip), interleaved (x, y)
   float2 *g_r[2] = {nullptr, nullptr}, *g_q[2] = {nullptr, nullptr},
          *g_s[2] = {nullptr, nullptr}, *g_p = nullptr, *g_di = nullptr;
+  // the extrapolated warm start of the FP32 form's solves (WarmStart in wc_kernels.h)
+  float2* ws_ring = nullptr;
+  int ws_order = WS_MAX_ORDER;  // 0: the reference's warm start
 
   // the learned preconditioner (wc_fpcg.hip) and the capture of solves for its training
   int precond = WC_PRECOND_JACOBI;
@@ -289,6 +292,7 @@ int prof_collect(wc_solver* s) {
 
 dim3 tiles(const wc_solver* s) { return dim3((unsigned)(s->geo.ntx * s->geo.nty)); }
 dim3 over(long long n) { return dim3((unsigned)((n + NT - 1) / NT)); }
+dim3 rate_blocks(long long n) { return dim3((unsigned)((n + RATE_CELLS - 1) / RATE_CELLS)); }
 
 Breaking breaking_of(const wc_solver* s) {
   // Solver::breaking_config (solver.rs:619-623): the dispersive model's (with the breaking
@@ -758,7 +762,7 @@ int do_step(wc_solver* s, double dt) {
 int stable_dt(wc_solver* s, double* dt) {
   const Geo& g = s->geo;
   HIP_OK(hipMemsetAsync(s->red64, 0, sizeof(unsigned long long), s->stream));
-  TRY(launch(s, F_REDUCE, k_max_rate, over((long long)g.nx * g.ny), dim3(NT), g,
+  TRY(launch(s, F_REDUCE, k_max_rate, rate_blocks((long long)g.nx * g.ny), dim3(NT), g,
              (const double*)s->h, (const double*)s->hu, (const double*)s->hv, s->red64));
   HIP_OK(hipMemcpyAsync(s->red64_host, s->red64, sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, s->stream));
@@ -798,7 +802,28 @@ int alloc_fp32(wc_solver* s) {
   for (float2** p : {&s->g_r[0], &s->g_r[1], &s->g_q[0], &s->g_q[1], &s->g_s[0], &s->g_s[1],
                      &s->g_p, &s->g_di})
     TRY(dalloc(s, p, n));
+  TRY(dalloc(s, &s->ws_ring, WS_SLOTS * n));
   return WC_OK;
+}
+
+// The warm start's record of past solution steps, cleared: the next solves start as the
+// reference's (from the previous solution) until the record refills.
+int ws_reset(wc_solver* s) {
+  if (s->ws_ring)
+    HIP_OK(hipMemsetAsync(s->ws_ring, 0, WS_SLOTS * s->cells * sizeof(float2), s->stream));
+  return WC_OK;
+}
+
+WarmStart ws_args(wc_solver* s) {
+  WarmStart w{};
+  if (s->ws_order > 0 && s->dispersive && s->precision == WC_PRECISION_FP32 && s->ws_ring) {
+    w.ring = s->ws_ring;
+    w.stride = s->cells;
+    w.sc = s->sc;
+    w.order = s->ws_order;
+    w.period = s->order == WC_ORDER_FIRST ? 1 : 2;
+  }
+  return w;
 }
 
 // FP64 state arrays -> FP32 pairs (after an upload, or when switching to FP32).
@@ -843,6 +868,8 @@ Sgn32 sgn32_args(wc_solver* s, int stage) {
   a.by = s->f_by;
   a.wx = s->f_wx;
   a.wy = s->f_wy;
+  a.solx = s->wx;
+  a.soly = s->wy;
   a.rx = s->f_rx;
   a.ry = s->f_ry;
   a.px = s->f_px;
@@ -856,6 +883,7 @@ Sgn32 sgn32_args(wc_solver* s, int stage) {
   a.nonlinear = s->disp.nonlinear ? 1 : 0;
   a.tolerance = s->disp.tolerance;
   a.max_iterations = s->disp.max_iterations;
+  a.ws = ws_args(s);
   return a;
 }
 
@@ -871,6 +899,8 @@ CgG cg1_args(wc_solver* s, int stage) {
   a.by = s->f_by;
   a.wx = s->f_wx;
   a.wy = s->f_wy;
+  a.solx = s->wx;
+  a.soly = s->wy;
   a.p = s->g_p;
   a.di = s->g_di;
   for (int m = 0; m < 2; ++m) {
@@ -956,6 +986,8 @@ int pcg_net32(wc_solver* s, int stage) {
   a.by = s->f_by;
   a.wx = s->f_wx;
   a.wy = s->f_wy;
+  a.solx = s->wx;
+  a.soly = s->wy;
   a.r = s->g_r[0];
   a.q = s->g_q[0];
   a.z = s->g_s[0];
@@ -1031,6 +1063,8 @@ int pcg32(wc_solver* s, int stage) {
     v.c = s->f_c;
     v.bx = s->f_bx;
     v.by = s->f_by;
+    v.solx = s->wx;
+    v.soly = s->wy;
     v.wx = s->f_wx;
     v.wy = s->f_wy;
     v.rx = s->f_rx;
@@ -1145,8 +1179,11 @@ Stage32 stage_args32(wc_solver* s, double dt, int stage) {
   a.lhv = s->f_lhv;
   a.wx = s->f_wx;
   a.wy = s->f_wy;
+  a.solx = s->wx;
+  a.soly = s->wy;
   a.dispersive = s->dispersive ? 1 : 0;
   a.damp = step_args(s, dt, stage);
+  a.ws = ws_args(s);
   return a;
 }
 
@@ -1195,7 +1232,7 @@ int do_step32(wc_solver* s, double dt) {
 int stable_dt32(wc_solver* s, double* dt) {
   const Geo& g = s->geo;
   HIP_OK(hipMemsetAsync(s->red64, 0, sizeof(unsigned long long), s->stream));
-  TRY(launch(s, F_REDUCE, k_max_rate32, over((long long)g.nx * g.ny), dim3(NT), g,
+  TRY(launch(s, F_REDUCE, k_max_rate32, rate_blocks((long long)g.nx * g.ny), dim3(NT), g,
              (const float*)s->f_sh, (const float*)s->f_slh, (const float*)s->f_shu,
              (const float*)s->f_slhu, (const float*)s->f_shv, (const float*)s->f_slhv,
              s->red64));
@@ -1341,6 +1378,8 @@ int wc_solver_with_order(wc_solver* s, int32_t order) {
   if (order < WC_ORDER_FIRST || order > WC_ORDER_FIFTH)
     return fail(WC_EINVAL, "order must be 0 (first), 1 (second) or 2 (fifth)");
   s->order = order;
+  TRY(set_device(s));
+  TRY(ws_reset(s));  // the rhythm of the solves changes with the order
   return WC_OK;
 }
 
@@ -1406,6 +1445,7 @@ int wc_solver_with_dispersion(wc_solver* s, const wc_dispersion_t* d) {
                     s->ax, s->ay})
     HIP_OK(hipMemsetAsync(p, 0, s->cells * sizeof(double), s->stream));
   HIP_OK(hipMemsetAsync(s->sc, 0, sizeof(CgScalars), s->stream));
+  TRY(ws_reset(s));
   HIP_OK(hipStreamSynchronize(s->stream));
   s->prev_it[0] = s->prev_it[1] = 0;
   return rebuild_drive(s);
@@ -1516,6 +1556,15 @@ int wc_solver_set_option(wc_solver* s, const char* key, int64_t value) {
     s->pcg_batch = (unsigned long long)value;
     return WC_OK;
   }
+  if (strcmp(key, "warm_start") == 0) {
+    if (value < 0 || value > WS_MAX_ORDER)
+      return fail(WC_EINVAL, "warm_start must be in [0, 3] (0: the previous solution)");
+    TRY(set_device(s));
+    s->ws_order = (int)value;
+    TRY(ws_reset(s));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    return WC_OK;
+  }
   return fail(WC_EINVAL, std::string("unknown option ") + key);
 }
 
@@ -1528,6 +1577,7 @@ int wc_solver_set_precision(wc_solver* s, int32_t precision) {
     TRY(alloc_fp32(s));
     s->precision = precision;
     TRY(split_state(s));
+    TRY(ws_reset(s));
   } else {
     TRY(join_state(s));
     s->precision = precision;
@@ -1548,6 +1598,7 @@ int wc_state_set(wc_solver* s, const double* h, const double* hu, const double* 
   else
     HIP_OK(hipMemsetAsync(s->br, 0, bytes, s->stream));
   if (s->precision == WC_PRECISION_FP32) TRY(split_state(s));
+  TRY(ws_reset(s));
   HIP_OK(hipStreamSynchronize(s->stream));
   s->time = time;
   return WC_OK;
@@ -2660,7 +2711,7 @@ int batch_build(wc_batch* b) {
     f_fill[q + 1] = f_fill[q] + (int)((6LL * g.w + 6LL * g.ny + NT - 1) / NT);
     f_sw[q + 1] = f_sw[q] + (int)(((long long)g.w * g.rows + NT - 1) / NT);
     f_tiles[q + 1] = f_tiles[q] + g.ntx * g.nty;
-    f_rate[q + 1] = f_rate[q] + (int)(((long long)g.nx * g.ny + NT - 1) / NT);
+    f_rate[q + 1] = f_rate[q] + (int)rate_blocks((long long)g.nx * g.ny).x;
     f_cg[q + 1] = f_cg[q] + pcg_blocks(s);
     b->off[q + 1] = b->off[q] + (long long)s->prm_len;
   }

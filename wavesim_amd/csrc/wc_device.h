@@ -18,6 +18,9 @@ constexpr int TX = 64;  // tile width: one wavefront along x, reads and writes c
 constexpr int TY = 4;   // tile height: four wavefronts per 256-thread block
 constexpr int NT = TX * TY;
 constexpr int NXCD = 8;  // MI355X: 8 XCDs, each with its own L2
+// Cells per block of the largest-rate reductions (stable_dt): one atomic per block, not per
+// wave, so that 8 M cells take 2048 atomics on the one address instead of 131,072.
+constexpr int RATE_CELLS = 16 * NT;
 
 // Grid geometry as kernels see it.
 struct Geo {
@@ -102,6 +105,26 @@ __device__ __forceinline__ double wave_sum(double v) {
   const double r0 = lane_value(v, 0), r1 = lane_value(v, 16);
   const double r2 = lane_value(v, 32), r3 = lane_value(v, 48);
   return (r0 + r1) + (r2 + r3);
+}
+
+// The largest of the block's values (IEEE bit patterns of non-negative doubles, or NaN, whose
+// patterns exceed every number's, so NaN wins as the reference's max_or_nan) into *out: a
+// wave butterfly, the block's waves through LDS, one atomic per block. Maxima do not depend on
+// the order they are taken in, so any grouping gives the same bits.
+__device__ __forceinline__ void block_max_to(unsigned long long bits, unsigned long long* out) {
+  __shared__ unsigned long long lds_max[NT / 64];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const unsigned long long o = __shfl_xor(bits, off, 64);
+    bits = o > bits ? o : bits;
+  }
+  if ((threadIdx.x & 63) == 0) lds_max[threadIdx.x >> 6] = bits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < NT / 64; ++w) bits = lds_max[w] > bits ? lds_max[w] : bits;
+    if (bits != 0ull) atomicMax(out, bits);
+  }
+  __syncthreads();
 }
 
 template <int N>

@@ -119,6 +119,53 @@ __global__ void __launch_bounds__(NT) k_switch32(Geo g, const float* h, const fl
   switch32_body(g, h, hl, b, br, held, phi, blockIdx.x);
 }
 
+// The start of solve n = sc->solves at interior cell k (Sgn32, WarmStart): the FP32 correction
+// w = 0; masked cells x0 = 0 (dispersion.rs:488-490); wet cells x0 = the previous solution, plus
+// the extrapolated step when the warm start is on (the step also into slot n).
+__device__ __forceinline__ void solve_start(const Sgn32& a, int k, bool wet) {
+  a.wx[k] = 0.f;
+  a.wy[k] = 0.f;
+  const WarmStart& ws = a.ws;
+  float2* e = ws.ring ? ws.ring + (ws.sc->solves % WS_SLOTS) * ws.stride : nullptr;
+  if (!wet) {
+    a.solx[k] = 0.0;
+    a.soly[k] = 0.0;
+    if (e) e[k] = make_float2(0.f, 0.f);
+    return;
+  }
+  if (!e) return;
+  const unsigned long long n = ws.sc->solves;
+  double px = 0.0, py = 0.0, cf = 1.0;
+  for (int j = 1; j <= ws.order; ++j) {
+    cf = cf * (double)(ws.order - j + 1) / (double)j;  // C(order, j), exact
+    const unsigned long long m = n - (unsigned long long)(j * ws.period);  // mod 2^64
+    const float2 d = ws.ring[(m % WS_SLOTS) * ws.stride + k];
+    const double sg = (j & 1) ? cf : -cf;
+    px += sg * (double)d.x;
+    py += sg * (double)d.y;
+  }
+  const float2 step = make_float2((float)px, (float)py);
+  e[k] = step;
+  a.solx[k] += (double)step.x;
+  a.soly[k] += (double)step.y;
+}
+
+// After solve n - 1 = sc->solves - 1 (counted by now), at interior cell k: the solution x = x0
+// + w into solx, soly (returned), and slot n - 1 from the start's step to d = x - x_{n-2}.
+__device__ __forceinline__ void solve_end(const Stage32& a, int k, double& x, double& y) {
+  const double wx = (double)a.wx[k], wy = (double)a.wy[k];
+  x = a.solx[k] + wx;
+  y = a.soly[k] + wy;
+  a.solx[k] = x;
+  a.soly[k] = y;
+  const WarmStart& ws = a.ws;
+  if (ws.ring) {
+    float2* e = ws.ring + ((ws.sc->solves - 1) % WS_SLOTS) * ws.stride;
+    const float2 st = e[k];
+    e[k] = make_float2((float)((double)st.x + wx), (float)((double)st.y + wy));
+  }
+}
+
 // SGN setup (dispersion.rs:385-490) on FP32 storage; arithmetic in FP64.
 __device__ __forceinline__ void sgn_setup32_body(const Sgn32& a, int blk, int nblk) {
   const Geo& g = a.g;
@@ -183,10 +230,10 @@ __device__ __forceinline__ void sgn_setup32_body(const Sgn32& a, int blk, int nb
   if (!(h >= WC_H_DISPERSION_MIN)) {
     a.bx[k] = 0.f;
     a.by[k] = 0.f;
-    a.wx[k] = 0.f;
-    a.wy[k] = 0.f;
+    solve_start(a, k, false);
     return;
   }
+  solve_start(a, k, true);
   const int x1 = lx + 1, y1 = ly + 1;
   const double u = wc_velocity(h, (double)a.hu[k]);
   const double v = wc_velocity(h, (double)a.hv[k]);
@@ -259,8 +306,8 @@ __global__ void __launch_bounds__(NT) k_cg_init32(Sgn32 a) {
     double vx = 0.0, vy = 0.0;
     if (i <= g.nx + GH + 1 && j <= g.ny + GH + 1) {
       const int m = map_y(j, g.ny, g.periodic) * g.w + map_x(i, g.nx);
-      vx = (double)a.wx[m];
-      vy = (double)a.wy[m];
+      vx = a.solx[m] + (double)a.wx[m];
+      vy = a.soly[m] + (double)a.wy[m];
       if (is_ghost_x(i, g.nx)) vx = -vx;
       if (!g.periodic && is_ghost_y(j, g.ny)) vy = -vy;
     }
@@ -386,9 +433,13 @@ __device__ __forceinline__ void stage32_body(const Stage32& a, double dt, int mo
   const double ihv = mode == MODE_FINAL ? (double)a.s1hv[k] : (double)a.shv[k];
   const double lh = a.lh[k];
   double lhu = (double)a.lhu[k], lhv = (double)a.lhv[k];
-  if (a.dispersive && ih >= WC_H_DISPERSION_MIN) {
-    lhu = ih * (double)a.wx[k] + wc_velocity(ih, ihu) * lh;
-    lhv = ih * (double)a.wy[k] + wc_velocity(ih, ihv) * lh;
+  if (a.dispersive) {
+    double wx, wy;
+    solve_end(a, k, wx, wy);
+    if (ih >= WC_H_DISPERSION_MIN) {
+      lhu = ih * wx + wc_velocity(ih, ihu) * lh;
+      lhv = ih * wy + wc_velocity(ih, ihv) * lh;
+    }
   }
   // the state s as the pair
   const double sh = (double)a.sh[k] + (double)a.sl_h[k];
@@ -453,23 +504,19 @@ __device__ __forceinline__ void max_rate32_body(const Geo& g, const float* h, co
                                                 const float* hu, const float* hul,
                                                 const float* hv, const float* hvl,
                                                 unsigned long long* out, int blk) {
-  const int t = blk * NT + threadIdx.x;
+  const int n = g.nx * g.ny;
   unsigned long long bits = 0ull;
-  if (t < g.nx * g.ny) {
+  for (int t = blk * RATE_CELLS + threadIdx.x; t < min(n, (blk + 1) * RATE_CELLS); t += NT) {
     const int k = (GH + t / g.nx) * g.w + GH + t % g.nx;
     const double hc = (double)h[k] + (double)hl[k];
     const double c = sqrt(WC_G * hc);
     const double u = fabs(wc_velocity(hc, (double)hu[k] + (double)hul[k]));
     const double v = fabs(wc_velocity(hc, (double)hv[k] + (double)hvl[k]));
     const double r = (u + c) / g.dx + (v + c) / g.dy;
-    bits = (unsigned long long)__double_as_longlong(r);
+    const unsigned long long b = (unsigned long long)__double_as_longlong(r);
+    bits = b > bits ? b : bits;
   }
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    const unsigned long long o = __shfl_xor(bits, off, 64);
-    bits = o > bits ? o : bits;
-  }
-  if ((threadIdx.x & 63) == 0 && bits != 0ull) atomicMax(out, bits);
+  block_max_to(bits, out);
 }
 
 __global__ void __launch_bounds__(NT) k_max_rate32(Geo g, const float* h, const float* hl,

@@ -108,9 +108,9 @@ __global__ void __launch_bounds__(PT) k_fcg_init(FcgArgs a) {
     int k;
     double ux, uy, ax, ay;
     stencil(g, a.c, a.h, tile, s_u, s_f,
-            [&](int m, double& vx, double& vy) {
-              vx = (double)a.wx[m];
-              vy = (double)a.wy[m];
+            [&](int m, double& vx, double& vy) {  // the iterate x0 + w
+              vx = a.solx[m] + (double)a.wx[m];
+              vy = a.soly[m] + (double)a.wy[m];
             },
             own, k, ux, uy, ax, ay);
     double v[2] = {0.0, 0.0};
@@ -133,6 +133,7 @@ __global__ void __launch_bounds__(PT) k_fcg_init(FcgArgs a) {
     }
     add_rows<2>(v, sums, r, acc);
   }
+  rows_total<2>(acc, sums);
   if (threadIdx.x == 0) {
     a.lanes[blockIdx.x] = acc[0];
     a.lanes[LANES + blockIdx.x] = acc[1];
@@ -220,6 +221,7 @@ __global__ void __launch_bounds__(PT) k_fcg_out(FcgArgs a, unsigned long long n)
     }
     add_rows<2>(v, sums, r, acc);
   }
+  rows_total<2>(acc, sums);
   if (threadIdx.x == 0) {
     a.lanes[2 * LANES + blockIdx.x] = acc[0];
     a.lanes[3 * LANES + blockIdx.x] = acc[1];
@@ -272,6 +274,7 @@ __global__ void __launch_bounds__(PT) k_fcg_mv(FcgArgs a, unsigned long long n) 
     }
     add_rows<1>(v, sums, r, acc);
   }
+  rows_total<1>(acc, sums);
   if (threadIdx.x == 0) a.lanes[4 * LANES + blockIdx.x] = acc[0];
 }
 
@@ -304,6 +307,7 @@ __global__ void __launch_bounds__(PT) k_fcg_up(FcgArgs a, unsigned long long n) 
     }
     add_rows<1>(v, sums, r, acc);
   }
+  rows_total<1>(acc, sums);
   if (threadIdx.x == 0) a.lanes[LANES + blockIdx.x] = acc[0];
 }
 

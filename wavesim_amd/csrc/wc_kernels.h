@@ -114,6 +114,7 @@ template <typename V>
 struct CgV {
   Geo g;
   const V *h, *c, *bx, *by;
+  const double *solx, *soly;  // FP32 form: the solve's FP64 start x0 (w its correction); FP64: null
   V *wx, *wy, *rx, *ry, *px, *py, *zx, *zy, *ax, *ay;
   double *acc_init, *acc_pap, *acc_update;  // 3 x 256, 256, 2 x 256 lane sums
   CgScalars* sc;
@@ -127,7 +128,8 @@ struct CgV {
 struct CgG {
   Geo g;
   const float *h, *c, *bx, *by;
-  float *wx, *wy;
+  const double *solx, *soly;  // the solve's start x0 (FP64, see Sgn32)
+  float *wx, *wy;             // the correction x - x0 (FP32)
   float2 *p, *di;
   float2* r[2];
   float2* q[2];
@@ -144,7 +146,8 @@ struct CgG {
 struct FcgArgs {
   Geo g;
   const float *h, *c, *bx, *by;
-  float *wx, *wy;
+  const double *solx, *soly;  // the solve's start x0 (FP64, see Sgn32)
+  float *wx, *wy;             // the correction x - x0 (FP32)
   float2 *r, *q, *z, *di;
   float2* p[2];
   const float *bgx, *bgy;  // bed-gradient features
@@ -191,8 +194,38 @@ template <typename V>
 __global__ void k_cg3_check(CgV<V> a, unsigned long long n);
 
 // The FP32 chosen form (wc_fp32.hip).
+// The extrapolated warm start of the FP32 form's SGN solves. The reference starts each solve
+// from the previous solution x_{k-1} (dispersion.rs:220-222, 488-490). Solves come in a fixed
+// rhythm of `period` per step (2 for SSP-RK2, 1 for Euler), so the solution of solve k is a
+// smooth function of time sampled every `period` solves, and
+//   x_k ~ x_{k-1} + sum_{j=1..m} (-1)^(j+1) C(m, j) d_{k - j period},   d_n = x_n - x_{n-1}
+// (the m-th backward difference, stride `period`, of the steps d set to zero: exact when the
+// solutions are a polynomial of degree <= m in time plus, per RK stage, one of degree <= m - 1;
+// the error on a smooth run is O(dt^(m+1))). The ring holds
+// the last WS_SLOTS steps d, slot n % WS_SLOTS for solve n, indexed by the solve counter in
+// CgScalars (so replays of captured graphs and batched instances find their own slots). The
+// set-up stores the extrapolation x0 - x_{k-1} into slot k and starts the solve at x0; the stage
+// update, after the solve, adds the solve's correction, so slot k holds d_k = x_k - x_{k-1}.
+// Masked cells start from zero, as the reference's, and record d = 0. The start never changes what a solve converges to (same
+// residual test, same tolerance), only how many iterations it takes.
+constexpr int WS_SLOTS = 8;
+constexpr int WS_MAX_ORDER = 3;  // order * period < WS_SLOTS
+struct WarmStart {
+  float2* ring;                    // WS_SLOTS x stride, interleaved (x, y); null: the reference's
+  unsigned long long stride;       // cells per slot
+  const CgScalars* sc;             // sc->solves: solves completed so far
+  int order, period;
+};
+
+// The SGN solution of the FP32 form is kept in FP64 (solx, soly: the warm start, shared with the
+// FP64 path's), and each solve iterates on an FP32 correction w = x - x0 from its start x0: the
+// bulk PCG traffic stays FP32 while the solution, which the warm start carries from solve to
+// solve, keeps FP64 accuracy (FP32 rounding of x itself would be amplified by the extrapolation
+// and cost iterations). The set-up writes x0 and w = 0, the iterations update w, the stage
+// update folds x = x0 + w and uses it.
 struct Sgn32 {
   Geo g;
+  double *solx, *soly;
   const float *h, *hu, *hv, *phi;  // stage state (high parts) and the switch
   const double* lh;                // mass rate (FP64: the faces' fluxes cancel exactly)
   const float *lhu, *lhv;          // momentum rates
@@ -202,6 +235,7 @@ struct Sgn32 {
   int nonlinear;
   double tolerance;
   unsigned long long max_iterations;
+  WarmStart ws;
 };
 
 struct Stage32 {
@@ -214,9 +248,11 @@ struct Stage32 {
   float *l1hu, *l1hv;
   const double* lh;                              // this stage's shallow-water rates
   const float *lhu, *lhv;
-  const float *wx, *wy;                          // the SGN solution
+  const float *wx, *wy;                          // the SGN solution's FP32 correction
+  double *solx, *soly;                           // its FP64 start, then the solution
   int dispersive;
   StepArgs damp;                                 // sponges and friction (dt, tables, manning)
+  WarmStart ws;                                  // records this stage's solution step
 };
 
 __global__ void k_split32(long long n, const double* a, float* hi, float* lo);

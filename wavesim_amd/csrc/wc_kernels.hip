@@ -578,23 +578,20 @@ __global__ void __launch_bounds__(NT) k_sgn_finish(StepArgs a, const double* wx,
 // every number: an unsigned max of the bits is max_or_nan.
 __global__ void __launch_bounds__(NT) k_max_rate(Geo g, const double* h, const double* hu,
                                                  const double* hv, unsigned long long* out) {
-  const int t = blockIdx.x * NT + threadIdx.x;
+  const int n = g.nx * g.ny;
   unsigned long long bits = 0ull;
-  if (t < g.nx * g.ny) {
+  for (int t = blockIdx.x * RATE_CELLS + threadIdx.x; t < min(n, (int)(blockIdx.x + 1) * RATE_CELLS);
+       t += NT) {
     const int k = (GH + t / g.nx) * g.w + GH + t % g.nx;
     const double hc = h[k];
     const double c = sqrt(WC_G * hc);
     const double u = fabs(wc_velocity(hc, hu[k]));
     const double v = fabs(wc_velocity(hc, hv[k]));
     const double r = (u + c) / g.dx + (v + c) / g.dy;
-    bits = (unsigned long long)__double_as_longlong(r);
+    const unsigned long long b = (unsigned long long)__double_as_longlong(r);
+    bits = b > bits ? b : bits;
   }
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    const unsigned long long o = __shfl_xor(bits, off, 64);
-    bits = o > bits ? o : bits;
-  }
-  if ((threadIdx.x & 63) == 0 && bits != 0ull) atomicMax(out, bits);
+  block_max_to(bits, out);
 }
 
 // limiting_cell (solver.rs:455-477), pass 1: the first NaN cell (row-major), and the bits of

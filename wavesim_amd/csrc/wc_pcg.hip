@@ -252,8 +252,13 @@ __global__ void __launch_bounds__(PT) k_cg3_init(CgV<V> a) {
         bool fx, fy;
         double vx = 0.0, vy = 0.0;
         if (tile[s] >= 0 && ring2(g, tile[s], t, m, fx, fy)) {
-          vx = (double)a.wx[m];
-          vy = (double)a.wy[m];
+          if (a.solx) {  // the FP32 form's iterate x0 + w
+            vx = a.solx[m] + (double)a.wx[m];
+            vy = a.soly[m] + (double)a.wy[m];
+          } else {
+            vx = (double)a.wx[m];
+            vy = (double)a.wy[m];
+          }
           if (fx) vx = -vx;
           if (fy) vy = -vy;
         }

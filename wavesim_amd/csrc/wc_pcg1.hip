@@ -91,9 +91,9 @@ __device__ __forceinline__ void cg1_init_body(const CgG& a, int blk) {
     int k;
     double ux, uy, ax, ay;
     stencil(g, a.c, a.h, tile, s_u, s_f,
-            [&](int m, double& vx, double& vy) {
-              vx = (double)a.wx[m];
-              vy = (double)a.wy[m];
+            [&](int m, double& vx, double& vy) {  // the iterate x0 + w
+              vx = a.solx[m] + (double)a.wx[m];
+              vy = a.soly[m] + (double)a.wy[m];
             },
             own, k, ux, uy, ax, ay);
     double v[1] = {0.0};
@@ -114,6 +114,7 @@ __device__ __forceinline__ void cg1_init_body(const CgG& a, int blk) {
     }
     add_rows<1>(v, sums, r, acc);
   }
+  rows_total<1>(acc, sums);
   if (threadIdx.x == 0) a.lanes_b[blk] = acc[0];
 }
 
@@ -152,6 +153,7 @@ __device__ __forceinline__ void cg1_start_body(const CgG& a, int blk) {
     }
     add_rows<3>(v, sums, r, acc);
   }
+  rows_total<3>(acc, sums);
   if (threadIdx.x == 0)
     for (int m = 0; m < 3; ++m) a.lanes[m * L + blk] = acc[m];
 }
@@ -211,6 +213,7 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
     }
     add_rows<3>(v, sums, r, acc);
   }
+  rows_total<3>(acc, sums);
   if (threadIdx.x == 0)
     for (int m = 0; m < 3; ++m) a.lanes[3 * L * (n & 1) + m * L + blk] = acc[m];
 }

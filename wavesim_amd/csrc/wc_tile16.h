@@ -44,20 +44,35 @@ __device__ __forceinline__ int tiles_all(const Geo& g) {
   return tiles_x(g) * ((g.ny + BY - 1) / BY);
 }
 
-// The round's row sums (one wave per tile row), then thread 0 adds the rows in order.
+// Adds this tile's terms into the thread's own running sums (its cells of the block's tiles, in
+// tile order), and keeps the next tile's LDS staging from overwriting what this one still reads.
+// The block reduction runs once, after the last tile (rows_total), not once per tile.
 template <int N>
-__device__ __forceinline__ void add_rows(double (&v)[N], double (*sums)[BY][N], int r,
+__device__ __forceinline__ void add_rows(double (&v)[N], double (*)[BY][N], int,
                                          double (&acc)[N]) {
+#pragma unroll
+  for (int n = 0; n < N; ++n) acc[n] = acc[n] + v[n];
+  __syncthreads();
+}
+
+// The block's total of the threads' running sums into thread 0's acc: each row of 64 threads by
+// the wave butterfly, then the rows in order.
+template <int N>
+__device__ __forceinline__ void rows_total(double (&acc)[N], double (*sums)[BY][N]) {
   const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
 #pragma unroll
   for (int n = 0; n < N; ++n) {
-    const double t = wave_sum(v[n]);
-    if (lane == 0) sums[r & 1][row][n] = t;
+    const double t = wave_sum(acc[n]);
+    if (lane == 0) sums[0][row][n] = t;
   }
   __syncthreads();
   if (threadIdx.x == 0)
-    for (int y = 0; y < BY; ++y)
-      for (int n = 0; n < N; ++n) acc[n] = acc[n] + sums[r & 1][y][n];
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      double t = 0.0;
+      for (int y = 0; y < BY; ++y) t = t + sums[0][y][n];
+      acc[n] = t;
+    }
 }
 
 // Stages the vector u (given per halo cell by fn(m, vx, vy), m the mapped interior index; the
