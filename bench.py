@@ -1,13 +1,17 @@
 """Throughput of wavecore's time step on one MI355X: cell-updates/s, PCG iterations per solve,
 and the roofline of the dominant kernel, with the CPU FP64 path timed beside it.
 
-    python bench.py [--steps K] [--warmup W] [--spinup S] [--workload rough_bed] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--spinup S] [--workload large_reef] [--gpus 1]
 
-Workload (default): the dispersive fifth-order run, 1024x512 cells at 3 m (1:40 plane from
-20 m plus 16 random Fourier modes, seed 1), SGN by PCG to 1e-8, breaking switch on, FP64.
+Workload (default): the large single reef, 4096x2048 cells at 3 m (a 40 m shelf shoaling at
+1:50 onto a 1 m reef flat, 8 Gaussian lobes, a 16-mode rough-bed overlay, a dry beach strip;
+wave-maker H 3 m, T 16 s at 15 degrees, sponges W, S, N, Manning 0.025), fifth order, SGN by
+PCG to 1e-8, breaking switch on. `--workload rough_bed` is the 1024x512 dispersive run.
 A step is one full Solver::step (both RK stages, every PCG iteration) with dt = stable_dt,
 driven on the device by wc_run_steps. The timed window starts from a mid-run state: S untimed
-spin-up steps (the wave has crossed the shelf by then), W untimed warm-up steps, then K timed.
+spin-up steps (the wave has crossed the shelf by then), W untimed warm-up steps, then K timed,
+for each path from that same state: the FP64 plain path, the FP32 chosen form, the FP32 form
+with the reference's warm start, and the FP32 form with the learned preconditioner.
 Inputs are resident in HBM for the whole window.
 """
 
@@ -71,13 +75,13 @@ def measured_traffic(kernel: str):
 def parse():
     p = argparse.ArgumentParser(description=__doc__)
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=300)
+    p.add_argument("--steps", type=int, default=100)
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--spinup", type=int, default=1500)
     p.add_argument("--spinup-precision", default="fp32", choices=["fp64", "fp32"],
                    help="the path that brings the run to its mid-run state (every measured path "
                         "then starts from that one state)")
-    p.add_argument("--workload", default="rough_bed",
+    p.add_argument("--workload", default="large_reef",
                    choices=["rough_bed", "plane_beach", "large_reef", "sweep"])
     p.add_argument("--graph", type=int, default=None, choices=[0, 1],
                    help="sweep: replay the fixed stretches of a step from HIP graphs (default 1)")
@@ -94,11 +98,18 @@ def parse():
                    help="skip the FP32 path with the learned preconditioner")
     p.add_argument("--no-breakdown", action="store_true",
                    help="skip the FP32 path with the reference's warm start (the gain's split)")
-    p.add_argument("--warm-start", type=int, default=None, choices=[0, 1, 2, 3],
-                   help="FP32 form: order of the extrapolated warm start (default 3; 0: the "
-                        "reference's start from the previous solution)")
+    p.add_argument("--warm-start", type=int, default=None, choices=[0, 1, 2, 3, 4, 5],
+                   help="FP32 form: order of the extrapolated warm start (0: the reference's "
+                        "start from the previous solution)")
+    p.add_argument("--warm-start-window", type=int, default=None,
+                   help="FP32 form: past solution steps the warm start weighs (>= its order)")
     p.add_argument("--pcg-lanes", type=int, default=None, choices=[256, 512],
                    help="blocks of the FP32 form's one-kernel PCG launches")
+    p.add_argument("--pcg-march", type=int, default=None, choices=[0, 1],
+                   help="one-kernel PCG layout: blocks march down column strips (1, default) or "
+                        "take 64x16 tiles (0)")
+    p.add_argument("--recon", type=int, default=None, choices=[32, 64],
+                   help="FP32 form: the flux reconstruction's arithmetic (default 32)")
     p.add_argument("--pcg32", type=int, default=None,
                    help="PCG kernels of the FP32 form (3: two per iteration, 4: one)")
     p.add_argument("--calibrate-copy", action="store_true",
@@ -121,7 +132,7 @@ def calibrate_copy():
 
 
 def measure(w, precision, arrays, t0, warmup, steps, profile_steps, pcg32=None, network=None,
-            lanes=None, warm_start=None, label=None):
+            lanes=None, warm_start=None, label=None, march=None, recon=None, window=None):
     """Time `steps` steps of one precision from the given state; per-kernel times with HIP
     events on the solver's stream: for the one-kernel PCG iteration of the FP32 form, one pair
     of events around each batch of its launches inside the timed window itself; for the kernel
@@ -137,10 +148,16 @@ def measure(w, precision, arrays, t0, warmup, steps, profile_steps, pcg32=None, 
         g.with_precision(PRECISION_FP32)
         if warm_start is not None:
             g.set_option("warm_start", warm_start)
+        if window is not None:
+            g.set_option("warm_start_window", window)
         if pcg32:
             g.set_option("pcg_kernels", pcg32)
         if lanes:
             g.set_option("pcg_lanes", lanes)
+        if march is not None:
+            g.set_option("pcg_march", march)
+        if recon is not None:
+            g.set_option("flux_reconstruction", recon)
         if network is not None:
             g.with_preconditioner(network)
     h, hu, hv, br = arrays
@@ -354,18 +371,21 @@ def main():
     del g
     precisions = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
     res = {p: measure(w, p, arrays, t0, args.warmup, args.steps, args.profile_steps,
-                      args.pcg32, lanes=args.pcg_lanes, warm_start=args.warm_start)
+                      args.pcg32, lanes=args.pcg_lanes, march=args.pcg_march, recon=args.recon,
+                      window=args.warm_start_window, warm_start=args.warm_start)
            for p in precisions}
     if "fp32" in res and not args.no_breakdown and args.warm_start != 0:
         # the FP32 form with the reference's warm start: what storage alone gains
         res["fp32+previous_start"] = measure(w, "fp32", arrays, t0, args.warmup, args.steps,
-                                             args.profile_steps, args.pcg32, lanes=args.pcg_lanes,
+                                             args.profile_steps, args.pcg32, lanes=args.pcg_lanes, march=args.pcg_march, recon=args.recon,
+                      window=args.warm_start_window,
                                              warm_start=0, label="fp32+previous_start")
     net_file = os.path.join(ROOT, "wavesim_amd", "data", "precond_rough_bed.npy")
     if "fp32" in res and not args.no_network and os.path.exists(net_file):
         res["fp32+network"] = measure(w, "fp32", arrays, t0, args.warmup, args.steps,
                                       args.profile_steps, args.pcg32, np.load(net_file),
-                                      lanes=args.pcg_lanes, warm_start=args.warm_start)
+                                      lanes=args.pcg_lanes, march=args.pcg_march, recon=args.recon,
+                      window=args.warm_start_window, warm_start=args.warm_start)
     accs = {p: accuracy(w, res["fp64"], res[p]) for p in res if p != "fp64"} \
         if "fp64" in res else {}
     # the chosen form: the fastest path within the accuracy bounds against FP64

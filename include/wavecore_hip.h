@@ -129,15 +129,30 @@ int wc_solver_with_shallow_cells(wc_solver* s, const uint8_t* cells, uint64_t n)
 /* The bed as the solver holds it (with_periodic_y wraps its ghost rows; solver.rs:416). */
 int wc_solver_bed(const wc_solver* s, double* out);
 /* Implementation choices (no counterpart in the reference). In FP64 they change speed, never
- * results; in the FP32 form "pcg_kernels" 4 also changes the rounding:
+ * results; in the FP32 form the PCG layout and the warm start also change the rounding, and the
+ * warm start the iteration count (never the tolerance a solve meets):
  *   "pcg_kernels" 1 = three kernels and two reductions per PCG iteration; 2 = two kernels and
  *                 two reductions; 3 = two kernels, reductions folded into them (FP64 default);
  *                 4 = one kernel per iteration, Chronopoulos-Gear recurrences (FP32 default;
  *                 FP64 runs 3 instead);
+ *   "pcg_lanes"   blocks of the one-kernel PCG: 256 (default) or 512;
+ *   "pcg_march"   one-kernel PCG layout: 0 = 64x16 tiles (default), 1 = blocks marching down
+ *                 64-column strips with circular LDS row buffers;
  *   "pcg_batch"   PCG iterations queued between looks at the convergence flag (default 4);
  *   "flux_kernel" 1 = one thread per cell, as the reference's loop; 2 = LDS tiles, each cell
- *                 reconstructed and each face fluxed once (default). */
+ *                 reconstructed and each face fluxed once (default);
+ *   "flux_reconstruction" FP32 form: 32 = reconstruct in FP32, 64 = in FP64 (default);
+ *   "warm_start"  FP32 form: the SGN solves start from an extrapolation of the past solutions,
+ *                 exact for solution steps of degree < m (m = 0..5; 0 = the previous
+ *                 solution, as the reference; default 3);
+ *   "warm_start_window" the past steps weighed, K >= m (0 = m; default 0): the least-norm
+ *                 weights exact to that degree (wc_warm_start_weights). */
 int wc_solver_set_option(wc_solver* s, const char* key, int64_t value);
+/* The warm start's weights c_1..c_K (host only, no GPU): x_n ~ x_{n-1} + sum c_j d_{n - j P},
+ * d the steps between successive solutions of one RK stage, exact for steps of degree < order
+ * and of least norm among such weights. K = max(window, order) values into out. No counterpart
+ * in the reference (its solves start from the previous solution, dispersion.rs:220-222). */
+int wc_warm_start_weights(int32_t order, int32_t window, double* out);
 /* Selects the arithmetic (WC_PRECISION_*); FP64 by default. Switching converts the held state. */
 int wc_solver_set_precision(wc_solver* s, int32_t precision);
 

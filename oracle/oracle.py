@@ -84,8 +84,9 @@ def lib():
             "wco_set_dot_order": (_I32, [_VP, _I32]),
             "wco_set_math": (_I32, [_VP, _I32]),
             "wco_set_threads": (_I32, [_VP, _I32]),
-            "wco_set_warm_start": (_I32, [_VP, _I32]),
-            "wco_ws_extrapolate": (_I32, [_I32, _I32, _U64, _P, _P]),
+            "wco_set_warm_start": (_I32, [_VP, _I32, _I32]),
+            "wco_ws_weights": (_I32, [_I32, _I32, _P]),
+            "wco_ws_extrapolate": (_I32, [_I32, _I32, _I32, _U64, _P, _P]),
             "wco_state_set": (_I32, [_VP, _P, _P, _P, _P, _D]),
             "wco_state_get": (_I32, [_VP, _P, _P, _P, _P, _P]),
             "wco_stable_dt": (_I32, [_VP, _P]),
@@ -182,10 +183,11 @@ class OracleSolver:
         self.order = Order(order)
         return self
 
-    def with_warm_start(self, order: int) -> "OracleSolver":
+    def with_warm_start(self, order: int, window: int = 0) -> "OracleSolver":
         """The FP32 form's extrapolated warm start of the SGN solves (order 0: the reference's
-        start from the previous solution). Set after the dispersion."""
-        _check(lib().wco_set_warm_start(self._h, int(order)))
+        start from the previous solution; window: past steps weighed, at least the order). Set
+        after the dispersion."""
+        _check(lib().wco_set_warm_start(self._h, int(order), int(window)))
         return self
 
     def with_manning(self, n: float) -> "OracleSolver":
@@ -533,11 +535,18 @@ def eddy_diffuse(grid, h, hu, hv, nu, dt):
     return hu, hv, int(n.value)
 
 
-def ws_extrapolate(order: int, period: int, past) -> float:
+def ws_extrapolate(order: int, period: int, past, window: int = 0) -> float:
     """The warm start of solve n for one cell from past[i] = x_{n-1-i}, as the oracle's solver
     (and the FP32 form) computes it."""
     a = np.ascontiguousarray(past, dtype=np.float64)
     out = ctypes.c_double()
-    _check(lib().wco_ws_extrapolate(int(order), int(period), a.size,
+    _check(lib().wco_ws_extrapolate(int(order), int(window), int(period), a.size,
                                     a.ctypes.data_as(_P), ctypes.byref(out)))
     return out.value
+
+
+def ws_weights(order: int, window: int = 0) -> np.ndarray:
+    """The oracle's warm-start weights c_1..c_K, K = max(order, window)."""
+    out = np.zeros(max(order, window))
+    _check(lib().wco_ws_weights(int(order), int(window), out.ctypes.data_as(_P)))
+    return out

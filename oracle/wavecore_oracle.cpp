@@ -466,18 +466,51 @@ double dot(const Grid& g, const Par& par, int order, const double* ax, const dou
 // ---------------------------------------------------------------- SGN dispersion
 // The FP32 form's extrapolated warm start (not the reference's: it starts each solve from the
 // previous solution, dispersion.rs:220-222). Solves come `period` per step; with d_n = x_n -
-// x_{n-1} the start of solve n is x_{n-1} + sum_{j=1..m} (-1)^(j+1) C(m, j) d_{n - j period}:
-// the m-th backward difference (stride `period`) of the steps d set to zero. Exact when the
-// solutions are a polynomial of degree <= m in time plus, per kind of solve (RK stage), one of
-// degree <= m - 1: then each kind's steps are of degree <= m - 1. `step(j)` is d_{n - j period}.
-constexpr int WS_SLOTS = 8;
-template <class D>
-double ws_extrapolate(double x1, int order, D step) {
-  double x = x1, cf = 1.0;
-  for (int j = 1; j <= order; ++j) {
-    cf = cf * (double)(order - j + 1) / (double)j;
-    x += ((j & 1) ? cf : -cf) * step(j);
+// x_{n-1} the start of solve n is x_{n-1} + sum_{j=1..K} c_j d_{n - j period}, the weights exact
+// for steps d of degree < m in time (each kind of solve, RK stage, its own) and of least norm
+// among such weights when K > m; K = m gives the m-th backward difference (stride `period`) of
+// the steps set to zero, c_j = (-1)^(j+1) C(m, j). Restated here from the mathematics: the
+// least-norm weights solve the normal equations (V V^T) y = e_0, c = V^T y, V[r][j] = (-j)^r.
+constexpr int WS_SLOTS = 32;
+constexpr int WS_MAX_WINDOW = 12;
+void ws_weights(int m, int K, double* c) {
+  if (K == m) {
+    double cf = 1.0;
+    for (int j = 1; j <= m; ++j) {
+      cf = cf * (double)(m - j + 1) / (double)j;
+      c[j - 1] = (j & 1) ? cf : -cf;
+    }
+    return;
   }
+  std::vector<std::vector<double>> a(m, std::vector<double>(m + 1, 0.0));
+  auto v = [](int r, int j) { return pow(-(double)j, (double)r); };
+  for (int r = 0; r < m; ++r) {
+    for (int q = 0; q < m; ++q)
+      for (int j = 1; j <= K; ++j) a[r][q] += v(r, j) * v(q, j);
+    a[r][m] = r == 0 ? 1.0 : 0.0;
+  }
+  for (int col = 0; col < m; ++col) {  // Gauss-Jordan with partial pivoting
+    int piv = col;
+    for (int r = col + 1; r < m; ++r)
+      if (fabs(a[r][col]) > fabs(a[piv][col])) piv = r;
+    std::swap(a[col], a[piv]);
+    for (int r = 0; r < m; ++r) {
+      if (r == col) continue;
+      const double f = a[r][col] / a[col][col];
+      for (int q = col; q <= m; ++q) a[r][q] -= f * a[col][q];
+    }
+  }
+  for (int j = 1; j <= K; ++j) {
+    double t = 0.0;
+    for (int r = 0; r < m; ++r) t += v(r, j) * (a[r][m] / a[r][r]);
+    c[j - 1] = t;
+  }
+}
+
+template <class D>
+double ws_extrapolate(double x1, int K, const double* c, D step) {
+  double x = x1;
+  for (int j = 1; j <= K; ++j) x += c[j - 1] * step(j);
   return x;
 }
 
@@ -486,8 +519,9 @@ struct Work {
       zx, zy, px, py, ax, ay;
   wco_dispersion_stats_t stats{0, 0, 0};
   // the extrapolated warm start (order 0: the reference's): steps d of the last WS_SLOTS solves,
-  // slot n % WS_SLOTS for solve n (which holds x_{n-1} while solve n runs)
-  int ws_order = 0, ws_period = 2;
+  // slot n % WS_SLOTS for solve n (which holds x_{n-1} while solve n runs); K weights
+  int ws_order = 0, ws_period = 2, ws_terms = 0;
+  double ws_c[WS_MAX_WINDOW] = {};
   std::vector<double> ws_x[WS_SLOTS], ws_y[WS_SLOTS];
   void init(const Grid& g) {
     const size_t n = g.cells();
@@ -628,8 +662,10 @@ void dispersion_apply(const Grid& g, const Par& par, int dot_order, const Disper
       } else {
         ex[k] = wx[k];
         ey[k] = wy[k];
-        wx[k] = ws_extrapolate(wx[k], wk.ws_order, [&](int j) { return wk.ws_x[slot(j)][k]; });
-        wy[k] = ws_extrapolate(wy[k], wk.ws_order, [&](int j) { return wk.ws_y[slot(j)][k]; });
+        wx[k] = ws_extrapolate(wx[k], wk.ws_terms, wk.ws_c,
+                               [&](int j) { return wk.ws_x[slot(j)][k]; });
+        wy[k] = ws_extrapolate(wy[k], wk.ws_terms, wk.ws_c,
+                               [&](int j) { return wk.ws_y[slot(j)][k]; });
       }
     });
   } else {
@@ -1333,9 +1369,12 @@ int wco_set_math(wco_solver* s, int32_t mode) {
   return 0;
 }
 
-int wco_set_warm_start(wco_solver* s, int32_t order) {
-  if (order < 0 || order > 3) return fail("warm start order must be in [0, 3]");
+int wco_set_warm_start(wco_solver* s, int32_t order, int32_t window) {
+  if (order < 0 || order > 5 || window < 0 || window > WS_MAX_WINDOW)
+    return fail("warm start order must be in [0, 5], window in [0, 12]");
   s->work.ws_order = order;
+  s->work.ws_terms = order > 0 ? std::max(order, window) : 0;
+  if (order > 0) ws_weights(order, s->work.ws_terms, s->work.ws_c);
   s->work.ws_period = s->order == ORDER_FIRST ? 1 : 2;
   for (int m = 0; m < WS_SLOTS; ++m) {
     s->work.ws_x[m].clear();
@@ -1344,13 +1383,24 @@ int wco_set_warm_start(wco_solver* s, int32_t order) {
   return 0;
 }
 
-int wco_ws_extrapolate(int32_t order, int32_t period, uint64_t n_past, const double* past,
-                       double* out) {
-  if (order < 0 || order > 3 || period < 1 || period > 2)
-    return fail("order must be in [0, 3] and period 1 or 2");
+int wco_ws_weights(int32_t order, int32_t window, double* out) {
+  if (order < 1 || order > 5 || window < 0 || window > WS_MAX_WINDOW)
+    return fail("order must be in [1, 5], window in [0, 12]");
+  ws_weights(order, std::max(order, window), out);
+  return 0;
+}
+
+int wco_ws_extrapolate(int32_t order, int32_t window, int32_t period, uint64_t n_past,
+                       const double* past, double* out) {
+  if (order < 0 || order > 5 || window < 0 || window > WS_MAX_WINDOW || period < 1 ||
+      period > 2)
+    return fail("order must be in [0, 5], window in [0, 12] and period 1 or 2");
   if (n_past < 1) return fail("need x_{n-1}");
+  double c[WS_MAX_WINDOW] = {};
+  const int K = order > 0 ? std::max(order, window) : 0;
+  if (order > 0) ws_weights(order, K, c);
   // past[i] = x_{n-1-i}; d_{n-m} = past[m-1] - past[m], zero where the history runs out
-  *out = ws_extrapolate(past[0], order, [&](int j) {
+  *out = ws_extrapolate(past[0], K, c, [&](int j) {
     const uint64_t m = (uint64_t)(j * period);
     return m < n_past ? past[m - 1] - past[m] : 0.0;
   });

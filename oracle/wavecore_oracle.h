@@ -90,12 +90,14 @@ int wco_bed(const wco_solver* s, double* out);
 /* Oracle switches (see above) and OpenMP threads for the row loops (1 = serial). */
 int wco_set_dot_order(wco_solver* s, int32_t order);
 int wco_set_math(wco_solver* s, int32_t mode);
-// The FP32 form's extrapolated warm start (order 0: the reference's previous solution); set it
-// after the order and the dispersion. wco_ws_extrapolate: the start of solve n from
-// past[i] = x_{n-1-i}, i < n_past (one cell), as the solver computes it.
-int wco_set_warm_start(wco_solver* s, int32_t order);
-int wco_ws_extrapolate(int32_t order, int32_t period, uint64_t n_past, const double* past,
-                       double* out);
+// The FP32 form's extrapolated warm start (order 0: the reference's previous solution): weights
+// exact for solution steps of degree < order, of least norm over max(order, window) past steps;
+// set it after the order and the dispersion. wco_ws_weights: those weights. wco_ws_extrapolate:
+// the start of solve n from past[i] = x_{n-1-i}, i < n_past (one cell), as the solver computes it.
+int wco_set_warm_start(wco_solver* s, int32_t order, int32_t window);
+int wco_ws_weights(int32_t order, int32_t window, double* out);
+int wco_ws_extrapolate(int32_t order, int32_t window, int32_t period, uint64_t n_past,
+                       const double* past, double* out);
 int wco_set_threads(wco_solver* s, int32_t threads);
 
 int wco_state_set(wco_solver* s, const double* h, const double* hu, const double* hv,

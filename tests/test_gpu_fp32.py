@@ -25,10 +25,13 @@ pytestmark = pytest.mark.gpu
 
 
 def fp32(grid, bed, pcg=None):
-    """pcg: 3 (two kernels per iteration), 4 (one, 256 blocks), 512 (one, 512 blocks)."""
+    """pcg: 3 (two kernels per iteration), 4 (one, 256 blocks marching down column strips),
+    512 (one, 512 blocks marching), "tiles" (one, 256 blocks over 64x16 tiles)."""
     s = Solver(grid, bed).with_precision(PRECISION_FP32)
     if pcg == 512:
         return s.set_option("pcg_kernels", 4).set_option("pcg_lanes", 512)
+    if pcg == "tiles":
+        return s.set_option("pcg_kernels", 4).set_option("pcg_march", 0)
     return s.set_option("pcg_kernels", pcg) if pcg else s
 
 
@@ -66,7 +69,7 @@ def run_pair(w, steps, t0=0.0, state=None, pcg=None):
     return g64, g32, s64, s32
 
 
-@pytest.mark.parametrize("pcg", [3, 4, 512])
+@pytest.mark.parametrize("pcg", [3, 4, 512, "tiles"])
 def test_fp32_rough_bed_matches_fp64_within_its_bounds(pcg):
     w = configs.c1_rough_bed(nx=256, ny=128)
     g64, g32, s64, s32 = run_pair(w, 600, t0=40.0, pcg=pcg)
@@ -87,7 +90,7 @@ def test_fp32_plane_beach_fv_matches_fp64():
     assert r["eta_max_err_m"] < 1e-3, r
 
 
-@pytest.mark.parametrize("pcg", [3, 4, 512])
+@pytest.mark.parametrize("pcg", [3, 4, 512, "tiles"])
 def test_fp32_lake_at_rest_and_closed_basin_volume(pcg):
     # well_balanced.rs:16-62 and 87-121 on the FP32 form
     for mean_depth, amplitude in [(8.0, 3.0), (0.5, 2.5)]:
@@ -104,7 +107,7 @@ def test_fp32_lake_at_rest_and_closed_basin_volume(pcg):
                                                  cases=[(False, True), (True, True)])
 
 
-@pytest.mark.parametrize("pcg", [3, 4, 512])
+@pytest.mark.parametrize("pcg", [3, 4, 512, "tiles"])
 def test_fp32_standing_wave_frequency(pcg):
     out = ka.check_standing_wave_sgn_frequency(fp32_with(pcg), orders=(Order.Fifth,))
     assert abs(out["Fifth"] - 1.4261) < 3e-3, out

@@ -12,6 +12,7 @@ whole run, the same waves (both solves meet the same residual test) in fewer ite
 from __future__ import annotations
 
 import math
+import os
 
 import numpy as np
 import pytest
@@ -57,6 +58,48 @@ def test_exact_for_polynomials_of_the_order_and_not_beyond(order, period):
         for fn in higher:
             miss = orc.ws_extrapolate(order, period, history(fn, n, period))
             assert abs(miss - fn(n % period, n)) > 1e-7
+
+
+@pytest.mark.parametrize("order,window", [(2, 4), (3, 6), (3, 8), (4, 10)])
+def test_wider_windows_stay_exact_to_their_order(order, window):
+    rng = np.random.default_rng(100 * order + window)
+    for trial in range(10):
+        n = 60 + trial
+        fn = polynomials(rng, 2, order, order - 1)
+        got = orc.ws_extrapolate(order, 2, history(fn, n, 2), window=window)
+        exact = fn(n % 2, n)
+        assert abs(got - exact) <= 1e-8 * max(1.0, abs(exact))
+
+
+def least_norm(order, window):
+    """numpy's least-norm solution of V c = e_0, V[r][j] = (-j)^r (r < order, j = 1..window)."""
+    V = np.array([[(-j) ** r for j in range(1, window + 1)] for r in range(order)], dtype=float)
+    e = np.zeros(order)
+    e[0] = 1.0
+    return np.linalg.lstsq(V, e, rcond=None)[0]
+
+
+def test_weights_are_the_least_norm_exact_ones():
+    from wavesim_amd import solver as gpu
+
+    have_lib = os.path.exists(gpu.LIB_PATH)
+    for order in range(1, 6):
+        for window in range(order, 13):
+            want = least_norm(order, window)
+            got = orc.ws_weights(order, window)
+            assert np.allclose(got, want, rtol=1e-9, atol=1e-9 * np.abs(want).max()), (
+                order, window, got, want)
+            if window == order:  # the binomial weights, exactly
+                binom = [(-1) ** (j + 1) * math.comb(order, j) for j in range(1, order + 1)]
+                assert list(got) == binom
+            if have_lib:  # the product's host-side weights (no GPU needed)
+                prod = gpu.warm_start_weights(order, window)
+                assert np.allclose(prod, got, rtol=1e-12, atol=1e-12 * np.abs(got).max())
+
+
+def test_wider_windows_let_less_noise_through():
+    # sum c_j^2 is what carries the past solutions' rounding into the start
+    assert np.sum(orc.ws_weights(3, 8) ** 2) < 0.4 * np.sum(orc.ws_weights(3, 3) ** 2)
 
 
 def test_order_zero_is_the_previous_solution():

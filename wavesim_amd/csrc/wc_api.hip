@@ -162,7 +162,8 @@ struct wc_solver {
          *g_s[2] = {nullptr, nullptr}, *g_p = nullptr, *g_di = nullptr;
   // the extrapolated warm start of the FP32 form's solves (WarmStart in wc_kernels.h)
   float2* ws_ring = nullptr;
-  int ws_order = WS_MAX_ORDER;  // 0: the reference's warm start
+  int ws_order = 3;   // exact for steps of degree < 3; 0: the reference's warm start
+  int ws_window = 0;  // past steps weighed (at least ws_order)
 
   // the learned preconditioner (wc_fpcg.hip) and the capture of solves for its training
   int precond = WC_PRECOND_JACOBI;
@@ -181,6 +182,9 @@ struct wc_solver {
   int pcg_kernels = 0;  // 0: the default of each precision (FP64 3, FP32 4)
   double* lanes = nullptr;  // 8 x 512 lane sums of the lane PCGs
   int pcg_lanes = 256;      // blocks of the one-kernel PCG's launches: 256 or 512
+  int pcg_march = 0;        // one-kernel PCG: blocks march down column strips (1) or take
+                            // 64x16 tiles (0)
+  int recon64 = 1;          // FP32 form: reconstruct the FP32 state in FP64 (1) or FP32 (0)
   int flux_kernel = 2;
   unsigned long long pcg_batch = 4;
   uint8_t* held = nullptr;
@@ -814,14 +818,59 @@ int ws_reset(wc_solver* s) {
   return WC_OK;
 }
 
+// The warm start's weights c_1..c_K (WarmStart): exact for steps of degree < m, of least norm.
+// K = m: the binomial weights (-1)^(j+1) C(m, j). K > m: c = V^T (V V^T)^-1 e_0 with
+// V[r][j] = (-j)^r, r < m, j = 1..K (the least-norm solution of V c = e_0), by Gaussian
+// elimination with partial pivoting on the m x m system.
+void ws_coefficients(int m, int K, double* c) {
+  if (K == m) {
+    double cf = 1.0;
+    for (int j = 1; j <= m; ++j) {
+      cf = cf * (double)(m - j + 1) / (double)j;
+      c[j - 1] = (j & 1) ? cf : -cf;
+    }
+    return;
+  }
+  double V[WS_MAX_ORDER][WS_MAX_WINDOW], A[WS_MAX_ORDER][WS_MAX_ORDER + 1];
+  for (int r = 0; r < m; ++r)
+    for (int j = 1; j <= K; ++j) V[r][j - 1] = std::pow(-(double)j, (double)r);
+  for (int r = 0; r < m; ++r) {
+    for (int q = 0; q < m; ++q) {
+      double t = 0.0;
+      for (int j = 0; j < K; ++j) t += V[r][j] * V[q][j];
+      A[r][q] = t;
+    }
+    A[r][m] = r == 0 ? 1.0 : 0.0;
+  }
+  for (int col = 0; col < m; ++col) {
+    int piv = col;
+    for (int r = col + 1; r < m; ++r)
+      if (std::fabs(A[r][col]) > std::fabs(A[piv][col])) piv = r;
+    for (int q = 0; q <= m; ++q) std::swap(A[col][q], A[piv][q]);
+    for (int r = 0; r < m; ++r) {
+      if (r == col) continue;
+      const double f = A[r][col] / A[col][col];
+      for (int q = col; q <= m; ++q) A[r][q] -= f * A[col][q];
+    }
+  }
+  double y[WS_MAX_ORDER];
+  for (int r = 0; r < m; ++r) y[r] = A[r][m] / A[r][r];
+  for (int j = 0; j < K; ++j) {
+    double t = 0.0;
+    for (int r = 0; r < m; ++r) t += V[r][j] * y[r];
+    c[j] = t;
+  }
+}
+
 WarmStart ws_args(wc_solver* s) {
   WarmStart w{};
   if (s->ws_order > 0 && s->dispersive && s->precision == WC_PRECISION_FP32 && s->ws_ring) {
     w.ring = s->ws_ring;
     w.stride = s->cells;
     w.sc = s->sc;
-    w.order = s->ws_order;
+    w.terms = std::max(s->ws_window, s->ws_order);
     w.period = s->order == WC_ORDER_FIRST ? 1 : 2;
+    ws_coefficients(s->ws_order, w.terms, w.coef);
   }
   return w;
 }
@@ -931,9 +980,12 @@ int pcg1_32l(wc_solver* s, int stage, unsigned long long earlier) {
     return WC_OK;
   };
   if (s->capture) HIP_OK(hipMemcpyAsync(s->cap_h, sa.h, fbytes, hipMemcpyDeviceToDevice, s->stream));
-  TRY(launch(s, F_CG_INIT, k_cg1_init<L>, grid, block, a));
+  const bool mar = s->pcg_march != 0;
+  TRY(mar ? launch(s, F_CG_INIT, k_cg1m_init<L>, grid, block, a)
+          : launch(s, F_CG_INIT, k_cg1_init<L>, grid, block, a));
   if (s->capture) TRY(capture_at(0, s->g_r[0]));
-  TRY(launch(s, F_CG_INIT, k_cg1_start<L>, grid, block, a));
+  TRY(mar ? launch(s, F_CG_INIT, k_cg1m_start<L>, grid, block, a)
+          : launch(s, F_CG_INIT, k_cg1_start<L>, grid, block, a));
   const unsigned long long maxit = s->disp.max_iterations;
   unsigned long long n = 1;
   unsigned long long batch = std::max<unsigned long long>(s->prev_it[stage], 1);
@@ -947,7 +999,8 @@ int pcg1_32l(wc_solver* s, int stage, unsigned long long earlier) {
     }
     const unsigned long long first = n;
     for (; n < stop; ++n) {
-      TRY(launch(s, F_CG_ITER, k_cg1_iter<L>, grid, block, a, n));
+      TRY(mar ? launch(s, F_CG_ITER, k_cg1m_iter<L>, grid, block, a, n)
+              : launch(s, F_CG_ITER, k_cg1_iter<L>, grid, block, a, n));
       if (s->capture)  // K_n leaves r_n in r[n & 1] and w_n in w
         for (int slot = 1; slot < wc_solver::CAP_SLOTS; ++slot)
           if (n == cap_it[slot]) TRY(capture_at(slot, s->g_r[n & 1]));
@@ -1153,6 +1206,7 @@ BFlux flux_args32(wc_solver* s, int stage) {
   a.lh = s->f_lh;
   a.lhu = s->f_lhu;
   a.lhv = s->f_lhv;
+  a.recon64 = s->recon64;
   return a;
 }
 
@@ -1197,7 +1251,8 @@ int stage32(wc_solver* s, double dt, int stage, int mode) {
                w.b, w.br, w.held, w.phi));
   }
   const BFlux f = flux_args32(s, stage);
-  TRY(launch(s, F_FLUX, k_flux_tile32, tiles(s), dim3(NT), f.f, f.lh, f.lhu, f.lhv));
+  TRY(s->recon64 ? launch(s, F_FLUX, k_flux_tile32r, tiles(s), dim3(NT), f.f, f.lh, f.lhu, f.lhv)
+                  : launch(s, F_FLUX, k_flux_tile32, tiles(s), dim3(NT), f.f, f.lh, f.lhu, f.lhv));
   if (s->dispersive) {
     TRY(launch(s, F_SGN_SETUP, k_sgn_setup32, tiles(s), dim3(NT), sgn32_args(s, stage)));
     TRY(pcg32(s, stage));
@@ -1506,6 +1561,13 @@ int wc_solver_bed(const wc_solver* s, double* out) {
   return WC_OK;
 }
 
+int wc_warm_start_weights(int32_t order, int32_t window, double* out) {
+  if (order < 1 || order > WS_MAX_ORDER || window < 0 || window > WS_MAX_WINDOW || !out)
+    return fail(WC_EINVAL, "order must be in [1, 5], window in [0, 12], out not null");
+  ws_coefficients(order, std::max<int>(window, order), out);
+  return WC_OK;
+}
+
 int wc_solver_set_option(wc_solver* s, const char* key, int64_t value) {
   if (!key) return fail(WC_EINVAL, "null option name");
   if (strcmp(key, "pcg_kernels") == 0) {
@@ -1551,6 +1613,16 @@ int wc_solver_set_option(wc_solver* s, const char* key, int64_t value) {
     s->flux_kernel = (int)value;
     return WC_OK;
   }
+  if (strcmp(key, "flux_reconstruction") == 0) {
+    if (value != 32 && value != 64) return fail(WC_EINVAL, "flux_reconstruction must be 32 or 64");
+    s->recon64 = value == 64 ? 1 : 0;
+    return WC_OK;
+  }
+  if (strcmp(key, "pcg_march") == 0) {
+    if (value != 0 && value != 1) return fail(WC_EINVAL, "pcg_march must be 0 or 1");
+    s->pcg_march = (int)value;
+    return WC_OK;
+  }
   if (strcmp(key, "pcg_batch") == 0) {
     if (value < 1 || value > 1000) return fail(WC_EINVAL, "pcg_batch must be in [1, 1000]");
     s->pcg_batch = (unsigned long long)value;
@@ -1558,9 +1630,18 @@ int wc_solver_set_option(wc_solver* s, const char* key, int64_t value) {
   }
   if (strcmp(key, "warm_start") == 0) {
     if (value < 0 || value > WS_MAX_ORDER)
-      return fail(WC_EINVAL, "warm_start must be in [0, 3] (0: the previous solution)");
+      return fail(WC_EINVAL, "warm_start must be in [0, 5] (0: the previous solution)");
     TRY(set_device(s));
     s->ws_order = (int)value;
+    TRY(ws_reset(s));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    return WC_OK;
+  }
+  if (strcmp(key, "warm_start_window") == 0) {
+    if (value < 0 || value > WS_MAX_WINDOW)
+      return fail(WC_EINVAL, "warm_start_window must be in [0, 12] (0: the order)");
+    TRY(set_device(s));
+    s->ws_window = (int)value;
     TRY(ws_reset(s));
     HIP_OK(hipStreamSynchronize(s->stream));
     return WC_OK;
@@ -2617,6 +2698,8 @@ struct wc_batch {
   std::vector<int> active;  // indices of the instances still stepping
   bool rk2 = true, sw = false, dispersive = false;
   int lanes = 256;  // the instances' one-kernel PCG lanes
+  bool march = true;  // and its layout (pcg_march)
+  bool recon64 = false;  // the instances' flux reconstruction in FP64
   unsigned long long maxit = 0, pcg_batch = 4;
   int grid_nt = 2048, grid_pt = 512;  // persistent widths of the 256- and 1024-thread launches
   bool tuned = false;
@@ -2688,9 +2771,17 @@ int bhost(wc_batch* b, T** p, size_t n) {
   return WC_OK;
 }
 
-int pcg_blocks(const wc_solver* s) {  // the one-kernel PCG's 64x16 tiles, at most its lanes
+// The one-kernel PCG's blocks that have work: 64x16 tiles, or strip units when marching
+// (mrch::segments), at most its lanes.
+int pcg_blocks(const wc_solver* s) {
+  const int L = s->pcg_lanes;
+  if (s->pcg_march) {
+    const int strips = (s->geo.nx + 63) / 64;
+    const int segs = std::min(strips >= L ? 1 : L / strips, s->geo.ny);
+    return std::min(strips * segs, L);
+  }
   const int nt = ((s->geo.nx + 63) / 64) * ((s->geo.ny + 15) / 16);
-  return std::min(nt, s->pcg_lanes);
+  return std::min(nt, L);
 }
 
 // (Re)builds every argument block and launch map for the active instances, in one upload.
@@ -2833,8 +2924,8 @@ void enq_stage_head(wc_batch* b, int st) {  // the switch, the rates, the SGN se
   if (b->sw)
     hipLaunchKernelGGL(kb_switch32, bgrid(b->m_sw.total, b->grid_nt), dim3(NT), 0, b->stream,
                        b->swa[st], b->m_sw);
-  hipLaunchKernelGGL(kb_flux32, bgrid(b->m_tiles.total, b->grid_nt), dim3(NT), 0, b->stream,
-                     b->flux[st], b->m_tiles);
+  hipLaunchKernelGGL(b->recon64 ? kb_flux32r : kb_flux32, bgrid(b->m_tiles.total, b->grid_nt),
+                     dim3(NT), 0, b->stream, b->flux[st], b->m_tiles);
   if (b->dispersive)
     hipLaunchKernelGGL(kb_sgn_setup32, bgrid(b->m_tiles.total, b->grid_nt), dim3(NT), 0,
                        b->stream, b->sgn[st], b->m_tiles);
@@ -2843,6 +2934,13 @@ void enq_stage_head(wc_batch* b, int st) {  // the switch, the rates, the SGN se
 template <int L>
 void enq_pcg_head_l(wc_batch* b, int st, unsigned long long k) {
   const dim3 gp = bgrid(b->m_cg.total, b->grid_pt);
+  if (b->march) {
+    hipLaunchKernelGGL(kbm_cg1_init<L>, gp, dim3(1024), 0, b->stream, b->cg[st], b->m_cg);
+    hipLaunchKernelGGL(kbm_cg1_start<L>, gp, dim3(1024), 0, b->stream, b->cg[st], b->m_cg);
+    for (unsigned long long it = 1; it <= k; ++it)
+      hipLaunchKernelGGL(kbm_cg1_iter<L>, gp, dim3(1024), 0, b->stream, b->cg[st], it, b->m_cg);
+    return;
+  }
   hipLaunchKernelGGL(kb_cg1_init<L>, gp, dim3(1024), 0, b->stream, b->cg[st], b->m_cg);
   hipLaunchKernelGGL(kb_cg1_start<L>, gp, dim3(1024), 0, b->stream, b->cg[st], b->m_cg);
   for (unsigned long long it = 1; it <= k; ++it)
@@ -2858,6 +2956,13 @@ void enq_pcg_head(wc_batch* b, int st, unsigned long long k) {  // init, start, 
 
 void enq_iter(wc_batch* b, int st, unsigned long long it) {
   const dim3 gp = bgrid(b->m_cg.total, b->grid_pt);
+  if (b->march) {
+    if (b->lanes == 512)
+      hipLaunchKernelGGL(kbm_cg1_iter<512>, gp, dim3(1024), 0, b->stream, b->cg[st], it, b->m_cg);
+    else
+      hipLaunchKernelGGL(kbm_cg1_iter<256>, gp, dim3(1024), 0, b->stream, b->cg[st], it, b->m_cg);
+    return;
+  }
   if (b->lanes == 512)
     hipLaunchKernelGGL(kb_cg1_iter<512>, gp, dim3(1024), 0, b->stream, b->cg[st], it, b->m_cg);
   else
@@ -3013,8 +3118,8 @@ int batch_tune(wc_batch* b) {
     double us = 0.0;
     const int rc = [&]() -> int {
       us = time([&] {
-        hipLaunchKernelGGL(kb_flux32, bgrid(b->m_tiles.total, width), dim3(NT), 0, b->stream,
-                           b->flux[0], b->m_tiles);
+        hipLaunchKernelGGL(b->recon64 ? kb_flux32r : kb_flux32, bgrid(b->m_tiles.total, width),
+                           dim3(NT), 0, b->stream, b->flux[0], b->m_tiles);
       });
       return WC_OK;
     }();
@@ -3030,7 +3135,13 @@ int batch_tune(wc_batch* b) {
       double us = 0.0;
       const int rc = [&]() -> int {
         us = time([&] {
-          if (b->lanes == 512)
+          if (b->march && b->lanes == 512)
+            hipLaunchKernelGGL(kbm_cg1_start<512>, bgrid(b->m_cg.total, width), dim3(1024), 0,
+                               b->stream, b->cg[0], b->m_cg);
+          else if (b->march)
+            hipLaunchKernelGGL(kbm_cg1_start<256>, bgrid(b->m_cg.total, width), dim3(1024), 0,
+                               b->stream, b->cg[0], b->m_cg);
+          else if (b->lanes == 512)
             hipLaunchKernelGGL(kb_cg1_start<512>, bgrid(b->m_cg.total, width), dim3(1024), 0,
                                b->stream, b->cg[0], b->m_cg);
           else
@@ -3069,7 +3180,9 @@ int wc_batch_new(wc_solver* const* solvers, uint64_t n, wc_batch** out) {
     if (s->precond != WC_PRECOND_JACOBI || s->capture || !(s->pcg_kernels == 0 || s->pcg_kernels == 4))
       return bfail("a batch runs the one-kernel Jacobi PCG, without capture");
     if (s->device != s0->device) return bfail("a batch's instances share one device");
-    if (s->pcg_lanes != s0->pcg_lanes) return bfail("a batch's instances share pcg_lanes");
+    if (s->pcg_lanes != s0->pcg_lanes || s->pcg_march != s0->pcg_march ||
+        s->recon64 != s0->recon64)
+      return bfail("a batch's instances share pcg_lanes, pcg_march and flux_reconstruction");
     if ((s->order == WC_ORDER_FIRST) != (s0->order == WC_ORDER_FIRST) ||
         switch_on(s) != switch_on(s0) || s->dispersive != s0->dispersive)
       return bfail("a batch's instances share the time stepping, the switch and the dispersion");
@@ -3101,6 +3214,8 @@ int wc_batch_new(wc_solver* const* solvers, uint64_t n, wc_batch** out) {
   }
   b->rk2 = s0->order != WC_ORDER_FIRST;
   b->lanes = s0->pcg_lanes;
+  b->march = s0->pcg_march != 0;
+  b->recon64 = s0->recon64 != 0;
   b->sw = switch_on(s0);
   b->dispersive = s0->dispersive;
   b->pcg_batch = s0->pcg_batch;

@@ -3,12 +3,14 @@
 // every face twice; both are pure functions of their inputs, so doing each once gives the same
 // numbers).
 //
-// Template parameter R is the type the reconstruction runs in (face_values on the surface and
-// the velocities, solver.rs:125-138). The surface-minus-bed differences (face depths, the
-// hydrostatic reconstruction, solver.rs:164-190), the HLL flux and the rates stay in FP64 for
-// both: with R = double this is the reference's arithmetic bit for bit; with R = float it is
-// the FP32 chosen form, whose lake at rest stays at rest because eta - b is never formed in
-// FP32.
+// Template parameter S is the storage type of the state, R the type the reconstruction runs in
+// (face_values on the surface and the velocities, solver.rs:125-138). The surface-minus-bed
+// differences (face depths, the hydrostatic reconstruction, solver.rs:164-190), the HLL flux and
+// the rates stay in FP64 for all: with S = R = double this is the reference's arithmetic bit for
+// bit; with S = float (the FP32 form's state pairs) R = float is the FP32 chosen form, whose
+// lake at rest stays at rest because eta - b is never formed in FP32, and R = double
+// reconstructs the FP32 state in FP64 (same time on this kernel, which is bound by latency and
+// LDS occupancy rather than by its bytes, and rates with FP64 rounding instead of FP32's).
 
 #include "wc_kernels.h"
 #include "wc_stage.h"
@@ -126,8 +128,8 @@ struct TileLds {
 
 // Rates of every tile cell, then the stage epilogue of `mode` (see stage_out).
 // in_lo: the low part of the depth in the FP32 chosen form (null in FP64).
-template <typename R>
-__device__ __forceinline__ void flux_tile(const FluxTileArgs<R>& a, int mode, int blk, int nblk,
+template <typename S, typename R>
+__device__ __forceinline__ void flux_tile(const FluxTileArgs<S>& a, int mode, int blk, int nblk,
                                           double& rh, double& rhu, double& rhv, int& i_out,
                                           int& j_out, bool& valid) {
   const Geo& g = a.g;
@@ -148,7 +150,7 @@ __device__ __forceinline__ void flux_tile(const FluxTileArgs<R>& a, int mode, in
       bb = a.b[k];
       u = wc_velocity(hd, (double)a.hu[k]);
       v = wc_velocity(hd, (double)a.hv[k]);
-      ph = a.phi[k];
+      ph = (R)a.phi[k];
     }
     L.u.box.eta[ly][lx] = (R)(hd + bb);
     L.u.box.h[ly][lx] = hd;
@@ -219,19 +221,20 @@ __global__ void __launch_bounds__(NT) k_flux_tile64(FluxTileArgs<double> f, Step
   double rh, rhu, rhv;
   int i, j;
   bool valid;
-  flux_tile<double>(f, mode, blockIdx.x, gridDim.x, rh, rhu, rhv, i, j, valid);
+  flux_tile<double, double>(f, mode, blockIdx.x, gridDim.x, rh, rhu, rhv, i, j, valid);
   if (!valid) return;
   stage_out(a, mode, i, j, j * a.g.w + i, rh, rhu, rhv);
 }
 
 // FP32 chosen form: the rates for the stage update that follows; the mass rate stays FP64 so
 // the volume is conserved to FP64 rounding, the momentum rates are rounded to FP32.
+template <typename R>
 __device__ __forceinline__ void flux32_body(const FluxTileArgs<float>& f, double* lh, float* lhu,
                                             float* lhv, int blk, int nblk) {
   double rh, rhu, rhv;
   int i, j;
   bool valid;
-  flux_tile<float>(f, MODE_RATES, blk, nblk, rh, rhu, rhv, i, j, valid);
+  flux_tile<float, R>(f, MODE_RATES, blk, nblk, rh, rhu, rhv, i, j, valid);
   if (!valid) return;
   const int k = j * f.g.w + i;
   lh[k] = rh;
@@ -241,17 +244,32 @@ __device__ __forceinline__ void flux32_body(const FluxTileArgs<float>& f, double
 
 __global__ void __launch_bounds__(NT) k_flux_tile32(FluxTileArgs<float> f, double* lh, float* lhu,
                                                     float* lhv) {
-  flux32_body(f, lh, lhu, lhv, blockIdx.x, gridDim.x);
+  flux32_body<float>(f, lh, lhu, lhv, blockIdx.x, gridDim.x);
+}
+
+// The FP32 form's state reconstructed in FP64.
+__global__ void __launch_bounds__(NT) k_flux_tile32r(FluxTileArgs<float> f, double* lh,
+                                                     float* lhu, float* lhv) {
+  flux32_body<double>(f, lh, lhu, lhv, blockIdx.x, gridDim.x);
 }
 
 // Many instances in one launch (see kb_fill32).
-__global__ void __launch_bounds__(NT) kb_flux32(const BFlux* args, BMap m) {
+template <typename R>
+__device__ __forceinline__ void kb_flux32_body(const BFlux* args, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
     const BFlux& a = args[q];
-    flux32_body(a.f, a.lh, a.lhu, a.lhv, vb - m.first[q], m.first[q + 1] - m.first[q]);
+    flux32_body<R>(a.f, a.lh, a.lhu, a.lhv, vb - m.first[q], m.first[q + 1] - m.first[q]);
     __syncthreads();  // the next virtual block reuses the LDS
   }
+}
+
+__global__ void __launch_bounds__(NT) kb_flux32(const BFlux* args, BMap m) {
+  kb_flux32_body<float>(args, m);
+}
+
+__global__ void __launch_bounds__(NT) kb_flux32r(const BFlux* args, BMap m) {
+  kb_flux32_body<double>(args, m);
 }
 
 }  // namespace wc

@@ -135,14 +135,12 @@ __device__ __forceinline__ void solve_start(const Sgn32& a, int k, bool wet) {
   }
   if (!e) return;
   const unsigned long long n = ws.sc->solves;
-  double px = 0.0, py = 0.0, cf = 1.0;
-  for (int j = 1; j <= ws.order; ++j) {
-    cf = cf * (double)(ws.order - j + 1) / (double)j;  // C(order, j), exact
+  double px = 0.0, py = 0.0;
+  for (int j = 1; j <= ws.terms; ++j) {
     const unsigned long long m = n - (unsigned long long)(j * ws.period);  // mod 2^64
     const float2 d = ws.ring[(m % WS_SLOTS) * ws.stride + k];
-    const double sg = (j & 1) ? cf : -cf;
-    px += sg * (double)d.x;
-    py += sg * (double)d.y;
+    px += ws.coef[j - 1] * (double)d.x;
+    py += ws.coef[j - 1] * (double)d.y;
   }
   const float2 step = make_float2((float)px, (float)py);
   e[k] = step;

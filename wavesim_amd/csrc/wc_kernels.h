@@ -182,6 +182,13 @@ template <int L>
 __global__ void k_cg1_iter(CgG a, unsigned long long n);
 template <int L>
 __global__ void k_cg1_check(CgG a, unsigned long long n);
+// the same, blocks marching down column strips (wc_march.h)
+template <int L>
+__global__ void k_cg1m_init(CgG a);
+template <int L>
+__global__ void k_cg1m_start(CgG a);
+template <int L>
+__global__ void k_cg1m_iter(CgG a, unsigned long long n);
 
 template <typename V>
 __global__ void k_cg3_init(CgV<V> a);
@@ -195,26 +202,31 @@ __global__ void k_cg3_check(CgV<V> a, unsigned long long n);
 
 // The FP32 chosen form (wc_fp32.hip).
 // The extrapolated warm start of the FP32 form's SGN solves. The reference starts each solve
-// from the previous solution x_{k-1} (dispersion.rs:220-222, 488-490). Solves come in a fixed
-// rhythm of `period` per step (2 for SSP-RK2, 1 for Euler), so the solution of solve k is a
-// smooth function of time sampled every `period` solves, and
-//   x_k ~ x_{k-1} + sum_{j=1..m} (-1)^(j+1) C(m, j) d_{k - j period},   d_n = x_n - x_{n-1}
-// (the m-th backward difference, stride `period`, of the steps d set to zero: exact when the
-// solutions are a polynomial of degree <= m in time plus, per RK stage, one of degree <= m - 1;
-// the error on a smooth run is O(dt^(m+1))). The ring holds
-// the last WS_SLOTS steps d, slot n % WS_SLOTS for solve n, indexed by the solve counter in
-// CgScalars (so replays of captured graphs and batched instances find their own slots). The
-// set-up stores the extrapolation x0 - x_{k-1} into slot k and starts the solve at x0; the stage
-// update, after the solve, adds the solve's correction, so slot k holds d_k = x_k - x_{k-1}.
-// Masked cells start from zero, as the reference's, and record d = 0. The start never changes what a solve converges to (same
-// residual test, same tolerance), only how many iterations it takes.
-constexpr int WS_SLOTS = 8;
-constexpr int WS_MAX_ORDER = 3;  // order * period < WS_SLOTS
+// from the previous solution x_{n-1} (dispersion.rs:220-222, 488-490). Solves come in a fixed
+// rhythm of `period` per step (2 for SSP-RK2, 1 for Euler), so solve n's solution is a smooth
+// function of time, sampled every `period` solves for each kind (RK stage), and
+//   x_n ~ x_{n-1} + sum_{j=1..K} c_j d_{n - j period},   d_n = x_n - x_{n-1}.
+// The weights c are exact for steps d that are polynomials of degree < m in time (each kind its
+// own: then the solutions are a polynomial of degree <= m plus, per kind, one of degree < m) and,
+// when the window K exceeds m, of least norm among those: sum c_j^2 sets how much of the
+// solutions' own noise (the FP32 rates' rounding, the solves' tolerance) reaches the start.
+// K = m gives the binomial weights of the m-th backward difference set to zero (3, -3, 1 for
+// m = 3). The ring holds the last WS_SLOTS steps d, slot n % WS_SLOTS for solve n, indexed by
+// the solve counter in CgScalars (so replays of captured graphs and batched instances find
+// their own slots). The set-up stores the extrapolation x0 - x_{n-1} into slot n and starts the
+// solve at x0; the stage update, after the solve, adds the solve's correction, so slot n holds
+// d_n = x_n - x_{n-1}. Masked cells start from zero, as the reference's, and record d = 0. The
+// start never changes what a solve converges to (same residual test, same tolerance), only how
+// many iterations it takes.
+constexpr int WS_SLOTS = 32;
+constexpr int WS_MAX_ORDER = 5;
+constexpr int WS_MAX_WINDOW = 12;  // WS_MAX_WINDOW * period < WS_SLOTS
 struct WarmStart {
   float2* ring;                    // WS_SLOTS x stride, interleaved (x, y); null: the reference's
   unsigned long long stride;       // cells per slot
   const CgScalars* sc;             // sc->solves: solves completed so far
-  int order, period;
+  int terms, period;               // K, solves per step
+  double coef[WS_MAX_WINDOW];      // c_1 .. c_K
 };
 
 // The SGN solution of the FP32 form is kept in FP64 (solx, soly: the warm start, shared with the
@@ -291,6 +303,7 @@ __global__ void k_cg_update2(SgnArgs a, const double* pcx, const double* pcy);
 __global__ void k_sgn_finish(StepArgs a, const double* wx, const double* wy, int mode);
 __global__ void k_flux_tile64(FluxTileArgs<double> f, StepArgs a, int mode);
 __global__ void k_flux_tile32(FluxTileArgs<float> f, double* lh, float* lhu, float* lhv);
+__global__ void k_flux_tile32r(FluxTileArgs<float> f, double* lh, float* lhu, float* lhv);
 __global__ void k_max_rate(Geo g, const double* h, const double* hu, const double* hv,
                            unsigned long long* out);
 __global__ void k_limit_scan(Geo g, const double* h, const double* hu, const double* hv,
@@ -391,6 +404,7 @@ struct BFlux {
   FluxTileArgs<float> f;
   double* lh;
   float *lhu, *lhv;
+  int recon64;  // reconstruct in FP64 (k_flux_tile32r, kb_flux32r) rather than FP32
 };
 struct BRate {
   Geo g;
@@ -400,6 +414,7 @@ struct BRate {
 __global__ void kb_fill32(const BFill* args, BMap m);
 __global__ void kb_switch32(const BSwitch* args, BMap m);
 __global__ void kb_flux32(const BFlux* args, BMap m);
+__global__ void kb_flux32r(const BFlux* args, BMap m);  // reconstruction in FP64
 __global__ void kb_sgn_setup32(const Sgn32* args, BMap m);
 __global__ void kb_stage32(const Stage32* args, const double* dt, int mode, BMap m);
 __global__ void kb_max_rate32(const BRate* args, BMap m);
@@ -410,6 +425,12 @@ template <int L>
 __global__ void kb_cg1_start(const CgG* args, BMap m);
 template <int L>
 __global__ void kb_cg1_iter(const CgG* args, unsigned long long n, BMap m);
+template <int L>
+__global__ void kbm_cg1_init(const CgG* args, BMap m);
+template <int L>
+__global__ void kbm_cg1_start(const CgG* args, BMap m);
+template <int L>
+__global__ void kbm_cg1_iter(const CgG* args, unsigned long long n, BMap m);
 template <int L>
 __global__ void kb_cg1_check(const CgG* args, unsigned long long n, int stats, int* done,
                              unsigned long long* it);

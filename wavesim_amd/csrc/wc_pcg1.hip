@@ -25,6 +25,7 @@
 // b+256, ...; each adds its tiles' sums (rows in order) into one lane, and every wave of the
 // next kernel reduces the 256 lanes itself, so no separate reduction kernel runs.
 
+#include "wc_march.h"
 #include "wc_tile16.h"
 
 namespace wc {
@@ -230,6 +231,132 @@ __global__ void __launch_bounds__(64) k_cg1_check(CgG a, unsigned long long n) {
   start<L>(a, n, 0, alpha, beta);
 }
 
+// ---- the same three kernels marching down column strips (wc_march.h) -----------------------
+// Same arithmetic per cell as the tile kernels above; only the grouping of cells into blocks,
+// and so the order of the lane sums, differs.
+
+template <int L>
+__device__ __forceinline__ void cg1m_init_body(const CgG& a, int blk) {
+  const Geo& g = a.g;
+  __shared__ mrch::LdsU s_u;
+  __shared__ mrch::LdsF s_f;
+  __shared__ double sums[2][BY][1];
+  double acc[1] = {0.0};
+  mrch::march<1>(
+      g, a.c, a.h, blk, L, s_u, s_f,
+      [&](int m, double& vx, double& vy) {  // the iterate x0 + w
+        vx = a.solx[m] + (double)a.wx[m];
+        vy = a.soly[m] + (double)a.wy[m];
+      },
+      [&](bool own, int k, double, double, double ax, double ay, double (&v)[1]) {
+        if (!own) return;
+        const double h = (double)a.h[k];
+        double dgx = 1.0, dgy = 1.0;
+        if (h >= WC_H_DISPERSION_MIN) {
+          const double heff = wc_max(h, WC_H_DRY);
+          dgx = heff + ((double)a.c[k + 1] + (double)a.c[k - 1]) * 0.25 / (g.dx * g.dx);
+          dgy = heff + ((double)a.c[k + g.w] + (double)a.c[k - g.w]) * 0.25 / (g.dy * g.dy);
+        }
+        const double bx = (double)a.bx[k], by = (double)a.by[k];
+        a.r[0][k] = f2(bx - ax, by - ay);
+        a.di[k] = f2(1.0 / dgx, 1.0 / dgy);
+        a.p[k] = make_float2(0.f, 0.f);
+        a.s[1][k] = make_float2(0.f, 0.f);
+        v[0] = bx * bx + by * by;
+      },
+      acc);
+  rows_total<1>(acc, sums);
+  if (threadIdx.x == 0) a.lanes_b[blk] = acc[0];
+}
+
+template <int L>
+__device__ __forceinline__ void cg1m_start_body(const CgG& a, int blk) {
+  const Geo& g = a.g;
+  __shared__ mrch::LdsU s_u;
+  __shared__ mrch::LdsF s_f;
+  __shared__ double sums[2][BY][3];
+  double acc[3] = {0.0, 0.0, 0.0};
+  mrch::march<3>(
+      g, a.c, a.h, blk, L, s_u, s_f,
+      [&](int m, double& vx, double& vy) {
+        const float2 rr = a.r[0][m], d = a.di[m];
+        vx = (double)rr.x * (double)d.x;
+        vy = (double)rr.y * (double)d.y;
+      },
+      [&](bool own, int k, double ux, double uy, double ax, double ay, double (&v)[3]) {
+        if (!own) return;
+        const double qx = (double)(float)ax, qy = (double)(float)ay;
+        a.q[0][k] = f2(qx, qy);
+        const float2 rr = a.r[0][k];
+        const double rx = rr.x, ry = rr.y;
+        v[0] = rx * ux + ry * uy;
+        v[1] = qx * ux + qy * uy;
+        v[2] = rx * rx + ry * ry;
+      },
+      acc);
+  rows_total<3>(acc, sums);
+  if (threadIdx.x == 0)
+    for (int m = 0; m < 3; ++m) a.lanes[m * L + blk] = acc[m];
+}
+
+template <int L>
+__device__ __forceinline__ void cg1m_iter_body(const CgG& a, unsigned long long n, int blk) {
+  const Geo& g = a.g;
+  double alpha = 0.0, beta = 0.0;
+  if (!start<L>(a, n, blk, alpha, beta)) return;
+  const unsigned long long i = n - 1;
+  const int cur = (int)(i & 1), nxt = cur ^ 1;
+  const float2 *r0 = a.r[cur], *q0 = a.q[cur];
+  const float2* s0 = a.s[nxt];  // s_{i-1}: written by K_{n-1} into nxt
+  const double na = -alpha;
+  __shared__ mrch::LdsU s_u;
+  __shared__ mrch::LdsF s_f;
+  __shared__ double sums[2][BY][3];
+  double acc[3] = {0.0, 0.0, 0.0};
+  mrch::march<3>(
+      g, a.c, a.h, blk, L, s_u, s_f,
+      [&](int m, double& vx, double& vy) {  // u_{i+1} = D^-1 (r_i - alpha (q_i + beta s_{i-1}))
+        const float2 qq = q0[m], ss = s0[m], rr = r0[m], d = a.di[m];
+        const double sx = (double)(float)((double)qq.x + beta * (double)ss.x);
+        const double sy = (double)(float)((double)qq.y + beta * (double)ss.y);
+        vx = (double)(float)((double)rr.x + na * sx) * (double)d.x;
+        vy = (double)(float)((double)rr.y + na * sy) * (double)d.y;
+      },
+      [&](bool own, int k, double ux, double uy, double ax, double ay, double (&v)[3]) {
+        if (!own) return;
+        const float2 qq = q0[k], ss = s0[k], rr = r0[k], d = a.di[k], pp = a.p[k];
+        const double sx = (double)(float)((double)qq.x + beta * (double)ss.x);
+        const double sy = (double)(float)((double)qq.y + beta * (double)ss.y);
+        const double px = (double)(float)((double)rr.x * (double)d.x + beta * (double)pp.x);
+        const double py = (double)(float)((double)rr.y * (double)d.y + beta * (double)pp.y);
+        a.p[k] = f2(px, py);
+        a.wx[k] = (float)((double)a.wx[k] + alpha * px);
+        a.wy[k] = (float)((double)a.wy[k] + alpha * py);
+        a.s[cur][k] = f2(sx, sy);
+        const double nrx = (double)(float)((double)rr.x + na * sx);
+        const double nry = (double)(float)((double)rr.y + na * sy);
+        a.r[nxt][k] = f2(nrx, nry);
+        const double qx = (double)(float)ax, qy = (double)(float)ay;
+        a.q[nxt][k] = f2(qx, qy);
+        v[0] = nrx * ux + nry * uy;
+        v[1] = qx * ux + qy * uy;
+        v[2] = nrx * nrx + nry * nry;
+      },
+      acc);
+  rows_total<3>(acc, sums);
+  if (threadIdx.x == 0)
+    for (int m = 0; m < 3; ++m) a.lanes[3 * L * (n & 1) + m * L + blk] = acc[m];
+}
+
+template <int L>
+__global__ void __launch_bounds__(PT) k_cg1m_init(CgG a) { cg1m_init_body<L>(a, blockIdx.x); }
+template <int L>
+__global__ void __launch_bounds__(PT) k_cg1m_start(CgG a) { cg1m_start_body<L>(a, blockIdx.x); }
+template <int L>
+__global__ void __launch_bounds__(PT) k_cg1m_iter(CgG a, unsigned long long n) {
+  cg1m_iter_body<L>(a, n, blockIdx.x);
+}
+
 // ---- many instances in one launch (see kb_fill32) -------------------------------------------
 // Instance q runs min(256, its tiles) virtual blocks; its lanes past those stay zero, as the
 // single launch's empty blocks write them, so the sums and every iterate are the same.
@@ -260,6 +387,35 @@ __global__ void __launch_bounds__(PT) kb_cg1_iter(const CgG* args, unsigned long
   }
 }
 
+// The marching kernels over many instances: instance q runs min(L, its strip units) virtual
+// blocks (mrch::segments), the same blocks its own launch runs work in.
+template <int L>
+__global__ void __launch_bounds__(PT) kbm_cg1_init(const CgG* args, BMap m) {
+  for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
+    const int q = batch_instance(m, vb);
+    cg1m_init_body<L>(args[q], vb - m.first[q]);
+    __syncthreads();
+  }
+}
+
+template <int L>
+__global__ void __launch_bounds__(PT) kbm_cg1_start(const CgG* args, BMap m) {
+  for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
+    const int q = batch_instance(m, vb);
+    cg1m_start_body<L>(args[q], vb - m.first[q]);
+    __syncthreads();
+  }
+}
+
+template <int L>
+__global__ void __launch_bounds__(PT) kbm_cg1_iter(const CgG* args, unsigned long long n, BMap m) {
+  for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
+    const int q = batch_instance(m, vb);
+    cg1m_iter_body<L>(args[q], n, vb - m.first[q]);
+    __syncthreads();
+  }
+}
+
 // One wave per instance: k_cg1_check, then the instance's flag and iteration into done[q],
 // it[q]; with `stats` (the solve is over) DispersionStats counts the solve as k_cg_stats does.
 template <int L>
@@ -286,9 +442,15 @@ __global__ void __launch_bounds__(64) kb_cg1_check(const CgG* args, unsigned lon
   template __global__ void k_cg1_start<L>(CgG);                                                \
   template __global__ void k_cg1_iter<L>(CgG, unsigned long long);                             \
   template __global__ void k_cg1_check<L>(CgG, unsigned long long);                            \
+  template __global__ void k_cg1m_init<L>(CgG);                                                \
+  template __global__ void k_cg1m_start<L>(CgG);                                               \
+  template __global__ void k_cg1m_iter<L>(CgG, unsigned long long);                            \
   template __global__ void kb_cg1_init<L>(const CgG*, BMap);                                   \
   template __global__ void kb_cg1_start<L>(const CgG*, BMap);                                  \
   template __global__ void kb_cg1_iter<L>(const CgG*, unsigned long long, BMap);               \
+  template __global__ void kbm_cg1_init<L>(const CgG*, BMap);                                  \
+  template __global__ void kbm_cg1_start<L>(const CgG*, BMap);                                 \
+  template __global__ void kbm_cg1_iter<L>(const CgG*, unsigned long long, BMap);              \
   template __global__ void kb_cg1_check<L>(const CgG*, unsigned long long, int, int*,          \
                                            unsigned long long*);
 WC_CG1_LANES(256)

@@ -82,6 +82,7 @@ SIGNATURES = {
     "wc_state_min_depth": (_I32, [_VP, _P]),
     "wc_linear_wave": (_I32, [_D, _D, _I32, ctypes.POINTER(CLinearWave)]),
     "wc_manning_factor": (_D, [_D, _D, _D, _D]),
+    "wc_warm_start_weights": (_I32, [_I32, _I32, _P]),
     "wc_run_steps": (_I32, [_VP, _U64, ctypes.POINTER(_U64), _P]),
     "wc_solver_set_preconditioner": (_I32, [_VP, _I32, ctypes.POINTER(ctypes.c_float), _U64]),
     "wc_solver_precond_fallbacks": (_I32, [_VP, ctypes.POINTER(_U64)]),
@@ -667,3 +668,11 @@ def linear_wave(omega: float, h: float, dispersive: bool) -> LinearWave:
 
 def manning_factor(h: float, speed: float, dt: float, n: float) -> float:
     return lib().wc_manning_factor(h, speed, dt, n)
+
+
+def warm_start_weights(order: int, window: int = 0) -> np.ndarray:
+    """The FP32 form's warm-start weights c_1..c_K (host only): exact for solution steps of
+    degree < order, of least norm over K = max(window, order) past steps."""
+    out = np.zeros(max(int(order), int(window), 1))
+    _check(lib().wc_warm_start_weights(int(order), int(window), _ptr(out)))
+    return out[:max(int(order), int(window))]
