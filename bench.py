@@ -46,24 +46,41 @@ BYTES_PER_CELL = {
     "fp32": {
         "cg_matvec": 4 * (2 + 2 + 1 + 1 + 2 + 2),
         "cg_update": 4 * (2 + 2 + 2 + 2 + 1 + 1 + 2 + 2 + 2),
-        # one kernel per iteration: read r, Au, Ap, 1/diag, c, h, p, w; write p, Ap, w, r, Au
-        "cg_iter": 4 * (2 + 2 + 2 + 2 + 1 + 1 + 2 + 2 + 2 + 2 + 2 + 2 + 2),
-        "cg_init": 4 * (2 + 1 + 1 + 2 + 2 + 2),  # read w, c, h, b; write r, p
-        "sgn_setup": 4 * (3 + 1 + 2 + 3) + 8,  # read h, hu, hv, phi, lhu, lhv, c; lh FP64; write c, b
+        # one kernel per iteration: read r, Au, Ap, 1/diag, p (pairs), c, h; write p, Ap, r, Au
+        # (pairs); the FP32 correction w (a pair) read and written every other iteration
+        "cg_iter": 4 * (2 * 5 + 1 + 1 + 2 * 4) + 4 * 2,
+        # init (read x0 FP64, w, c, h, b; write r, 1/diag, p, s) and start (read r, 1/diag,
+        # c, h; write Au), the two launches of the family averaged
+        "cg_init": (8 * 2 + 4 * (2 + 1 + 1 + 2) + 4 * 8 + 4 * (2 + 2 + 1 + 1) + 4 * 2) // 2,
+        # read h, hu, hv, phi, lhu, lhv, lh (FP64); write c, b; warm start: read x (FP64) and
+        # six past steps, write x0 (FP64), w = 0 and the step
+        "sgn_setup": 4 * (3 + 1 + 2) + 8 + 4 * 3 + (8 * 2 + 4 * 2 * 6) + (8 * 2 + 4 * 2 + 4 * 2),
         "flux": 4 * (2 + 2 + 1 + 2) + 8 + 8,  # read h pair, hu, hv, phi; b FP64; write lhu, lhv; lh FP64
-        "sgn_finish": 4 * (6 + 3 + 2 + 2 + 6) + 8 * 3,  # pair, s1, w, rates; l1h, lh, b; write pair
+        # pair, s1, w, rates; l1h, lh, b; write pair; the solution x (FP64) and its step read
+        # and written
+        "sgn_finish": 4 * (6 + 3 + 2 + 2 + 6) + 8 * 3 + 2 * (8 * 2 + 4 * 2),
         "switch": 4 * 3 + 8,
     },
 }
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "cg_iter_rough_bed_1024x512_pmc.md")
+# rocprofv3 --pmc summaries of the dominant kernel per workload (traffic per launch)
+TRAFFIC_PROFILES = {
+    "rough_bed_1024x512_fifth_sgn":
+        os.path.join(ROOT, "profiles", "cg_iter_rough_bed_1024x512_pmc.md"),
+    "large_reef_4096x2048_fifth_sgn":
+        os.path.join(ROOT, "profiles", "cg_iter_large_reef_4096x2048_pmc.md"),
+}
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary (FETCH_SIZE
-    and WRITE_SIZE, each scaled by what the counter reads for a copy of known size), or None."""
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the workload's committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads for a copy of known size),
+    or None."""
+    path = TRAFFIC_PROFILES.get(workload)
+    if path is None:
+        return None
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(path) as f:
             for line in f:
                 if line.startswith(f"traffic_bytes_per_launch {kernel} "):
                     return float(line.split()[2])
@@ -77,7 +94,9 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=100)
     p.add_argument("--warmup", type=int, default=20)
-    p.add_argument("--spinup", type=int, default=1500)
+    p.add_argument("--spinup", type=int, default=None,
+                   help="untimed steps before the window (default: 4000 for the large reef, by "
+                        "when its waves break on the reef edge; 1500 otherwise)")
     p.add_argument("--spinup-precision", default="fp32", choices=["fp64", "fp32"],
                    help="the path that brings the run to its mid-run state (every measured path "
                         "then starts from that one state)")
@@ -101,6 +120,9 @@ def parse():
     p.add_argument("--warm-start", type=int, default=None, choices=[0, 1, 2, 3, 4, 5],
                    help="FP32 form: order of the extrapolated warm start (0: the reference's "
                         "start from the previous solution)")
+    p.add_argument("--warm-start-guard", type=int, default=None,
+                   help="FP32 form: cells whose switch reads below this (per mille) start from "
+                        "the previous solution")
     p.add_argument("--warm-start-window", type=int, default=None,
                    help="FP32 form: past solution steps the warm start weighs (>= its order)")
     p.add_argument("--pcg-lanes", type=int, default=None, choices=[256, 512],
@@ -132,7 +154,8 @@ def calibrate_copy():
 
 
 def measure(w, precision, arrays, t0, warmup, steps, profile_steps, pcg32=None, network=None,
-            lanes=None, warm_start=None, label=None, march=None, recon=None, window=None):
+            lanes=None, warm_start=None, label=None, march=None, recon=None, window=None,
+            guard=None):
     """Time `steps` steps of one precision from the given state; per-kernel times with HIP
     events on the solver's stream: for the one-kernel PCG iteration of the FP32 form, one pair
     of events around each batch of its launches inside the timed window itself; for the kernel
@@ -150,6 +173,8 @@ def measure(w, precision, arrays, t0, warmup, steps, profile_steps, pcg32=None, 
             g.set_option("warm_start", warm_start)
         if window is not None:
             g.set_option("warm_start_window", window)
+        if guard is not None:
+            g.set_option("warm_start_guard", guard)
         if pcg32:
             g.set_option("pcg_kernels", pcg32)
         if lanes:
@@ -335,6 +360,8 @@ def sweep(args):
 
 def main():
     args = parse()
+    if args.spinup is None:
+        args.spinup = 4000 if args.workload == "large_reef" else 1500
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
                                    "(DESIGN.md), not a measured path", "n_gpus": args.gpus}))
@@ -372,20 +399,22 @@ def main():
     precisions = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
     res = {p: measure(w, p, arrays, t0, args.warmup, args.steps, args.profile_steps,
                       args.pcg32, lanes=args.pcg_lanes, march=args.pcg_march, recon=args.recon,
-                      window=args.warm_start_window, warm_start=args.warm_start)
+                      window=args.warm_start_window, warm_start=args.warm_start,
+                      guard=args.warm_start_guard)
            for p in precisions}
     if "fp32" in res and not args.no_breakdown and args.warm_start != 0:
         # the FP32 form with the reference's warm start: what storage alone gains
-        res["fp32+previous_start"] = measure(w, "fp32", arrays, t0, args.warmup, args.steps,
-                                             args.profile_steps, args.pcg32, lanes=args.pcg_lanes, march=args.pcg_march, recon=args.recon,
-                      window=args.warm_start_window,
-                                             warm_start=0, label="fp32+previous_start")
+        res["fp32+previous_start"] = measure(
+            w, "fp32", arrays, t0, args.warmup, args.steps, args.profile_steps, args.pcg32,
+            lanes=args.pcg_lanes, march=args.pcg_march, recon=args.recon, warm_start=0,
+            label="fp32+previous_start")
     net_file = os.path.join(ROOT, "wavesim_amd", "data", "precond_rough_bed.npy")
     if "fp32" in res and not args.no_network and os.path.exists(net_file):
         res["fp32+network"] = measure(w, "fp32", arrays, t0, args.warmup, args.steps,
                                       args.profile_steps, args.pcg32, np.load(net_file),
                                       lanes=args.pcg_lanes, march=args.pcg_march, recon=args.recon,
-                      window=args.warm_start_window, warm_start=args.warm_start)
+                      window=args.warm_start_window, warm_start=args.warm_start,
+                      guard=args.warm_start_guard)
     accs = {p: accuracy(w, res["fp64"], res[p]) for p in res if p != "fp64"} \
         if "fp64" in res else {}
     # the chosen form: the fastest path within the accuracy bounds against FP64
@@ -422,24 +451,29 @@ def main():
         "dtype": "f32" if chosen.startswith("fp32") else "f64",
         "data": "synthetic",
         "config": {"workload": w.name, "nx": w.grid.nx, "ny": w.grid.ny, "dx_m": w.grid.dx,
-                   "path": ("FP32 chosen form (FP32 storage, FP64 inner products, norms, PCG "
-                            "scalars, surface-minus-bed differences)" if chosen.startswith("fp32")
+                   "path": ("FP32 chosen form (FP32 storage of the state pairs, rates and PCG "
+                            "vectors; FP64 inner products, norms, PCG scalars, surface-minus-bed "
+                            "differences, SGN solution)" if chosen.startswith("fp32")
                             else "FP64 plain path"),
+                   "flux_reconstruction": (("fp32" if args.recon == 32 else "fp64")
+                                           if chosen.startswith("fp32") else "fp64"),
                    "preconditioner": ("learned network, flexible PCG, Jacobi fallback"
                                       if chosen.endswith("network")
                                       else "Jacobi (the reference's)"),
                    "warm_start": ("the previous solution (the reference's)"
                                   if not chosen.startswith("fp32") or chosen.endswith(
                                       "previous_start") or args.warm_start == 0
-                                  else f"extrapolated, order "
-                                       f"{3 if args.warm_start is None else args.warm_start}"),
+                                  else "extrapolated, order {}, window {}".format(
+                                      3 if args.warm_start is None else args.warm_start,
+                                      6 if args.warm_start_window is None
+                                      else args.warm_start_window)),
                    "pcg_tolerance": 1e-8,
                    "spinup_steps": args.spinup, "spinup_path": args.spinup_precision,
                    "t_start_s": round(t0, 3), **w.notes},
         "pcg_iters_per_solve": r["pcg_iters_per_solve"],
         "roofline": {"bound": "hbm", "kernel": r["dominant"], "achieved": r["achieved"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["achieved"] / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(r["dominant"]),
+                     "traffic": measured_traffic(w.name, r["dominant"]),
                      "algorithmic_bytes_per_launch": r["bytes_per_cell"] * w.grid.nx * w.grid.ny,
                      "bytes_per_cell": r["bytes_per_cell"],
                      "avg_launch_us": 1e3 * r["avg_ms"], "launches": r["launches"],

@@ -145,8 +145,8 @@ int wc_solver_bed(const wc_solver* s, double* out);
  *   "warm_start"  FP32 form: the SGN solves start from an extrapolation of the past solutions,
  *                 exact for solution steps of degree < m (m = 0..5; 0 = the previous
  *                 solution, as the reference; default 3);
- *   "warm_start_window" the past steps weighed, K >= m (0 = m; default 0): the least-norm
- *                 weights exact to that degree (wc_warm_start_weights). */
+ *   "warm_start_window" the past steps weighed, K = max(window, m) (default 6): the
+ *                 least-norm weights exact to that degree (wc_warm_start_weights). */
 int wc_solver_set_option(wc_solver* s, const char* key, int64_t value);
 /* The warm start's weights c_1..c_K (host only, no GPU): x_n ~ x_{n-1} + sum c_j d_{n - j P},
  * d the steps between successive solutions of one RK stage, exact for steps of degree < order

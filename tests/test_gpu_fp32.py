@@ -82,6 +82,24 @@ def test_fp32_rough_bed_matches_fp64_within_its_bounds(pcg):
     assert st.unconverged == 0
 
 
+@pytest.mark.parametrize("recon", [64, 32])
+def test_fp32_large_reef_tile_matches_fp64_within_its_bounds(recon):
+    """A tile of the 4096x2048 reef (its slope from the wave-maker towards the reef edge, lobes,
+    overlay, Manning, sponges), both flux reconstructions of the FP32 form."""
+    w = configs.c4_crop()
+    g64 = w.solver(Solver)
+    g32 = w.solver(fp32_with(None)).set_option("flux_reconstruction", recon)
+    s64 = w.still_water()
+    s64.time = 60.0
+    s32 = s64.clone()
+    assert g64.run_steps(s64, 400)[0] == 400 and g32.run_steps(s32, 400)[0] == 400
+    r = compare(w, s64, s32, g64, g32)
+    print(f"fp32 (reconstruction {recon}) vs fp64, large reef tile:", r, flush=True)
+    assert r["eta_max_err_m"] < 1e-3, r
+    assert r["mask_agreement"] >= 0.999, r
+    assert g32.dispersion_stats().unconverged == 0
+
+
 def test_fp32_plane_beach_fv_matches_fp64():
     w = configs.c0_plane_beach()
     g64, g32, s64, s32 = run_pair(w, 2000)

@@ -137,6 +137,17 @@ def test_sgn_rough_bed_tight_tolerance_with_maker_sponges_and_friction(pcg):
              options={"pcg_kernels": pcg, "pcg_batch": 3, "flux_kernel": 1 + pcg % 2})
 
 
+def test_large_reef_tile_is_bitwise_the_oracle():
+    """A 256x96 tile of the 4096x2048 reef's bed (the slope from the wave-maker at the 15 m
+    contour to the 2 m contour, with the reef lobes and the rough overlay), with its physics:
+    fifth order, SGN to 1e-8, Manning, sponges, the oblique wave-maker started near full
+    amplitude."""
+    w = configs.c4_crop()
+    s = w.still_water()
+    s.time = 60.0
+    lockstep(w.grid, w.bed, w.configure, s, 150, "large reef tile")
+
+
 def test_sgn_second_order_linear_dispersion_and_held_cells():
     grid = Grid(120, 8, 1.0, 1.0)
     bed = Bathymetry.flat(grid, 2.0)

@@ -29,25 +29,27 @@ def run(solver, w, steps, t0=40.0):
 
 
 def test_iterations_follow_the_oracle_and_the_waves_stay():
-    """Each order's iteration count on the GPU lies between the oracle's (FP64) at that order
-    and the oracle's at the order below: the FP32 form's rounding costs it a little against
-    FP64 (most at order 3, whose coefficients add up to 8 and amplify the rounding in the past
-    solutions most), a wrong ring slot or coefficient would cost it much more."""
+    """Each warm start's iteration count on the GPU lies between the oracle's (FP64) with the
+    same weights and the oracle's at the order below: the FP32 form's rounding costs it a little
+    against FP64 (most for weights whose squares add up most, which carry the most of the past
+    solutions' rounding into the start), a wrong ring slot or weight would cost it much more."""
     w = configs.c1_rough_bed(nx=160, ny=64, steps=80)
-    oracle_it, gpu = {}, {}
-    for order in (0, 1, 2, 3):
-        o = w.solver(lambda gr, b: orc.OracleSolver(gr, b)).with_warm_start(order)
-        so, oracle_it[order], uo = run(o, w, w.steps)
+    oracle_it = {}
+    for order, window in ((0, 0), (1, 1), (2, 2), (3, 3), (3, 6)):
+        o = w.solver(lambda gr, b: orc.OracleSolver(gr, b)).with_warm_start(order, window)
+        so, oracle_it[(order, window)], uo = run(o, w, w.steps)
         assert uo == 0
         if order == 0:
             continue
-        g = w.solver(Solver).with_precision(PRECISION_FP32).set_option("warm_start", order)
-        sg, gpu[order], ug = run(g, w, w.steps)
+        g = (w.solver(Solver).with_precision(PRECISION_FP32).set_option("warm_start", order)
+             .set_option("warm_start_window", window))
+        sg, itg, ug = run(g, w, w.steps)
         assert ug == 0
-        print(f"order {order}: GPU FP32 {gpu[order]:.2f} iterations per solve, oracle FP64 "
-              f"{oracle_it[order]:.2f} (order {order - 1}: {oracle_it[order - 1]:.2f})",
+        below = oracle_it[(order - 1, order - 1)]
+        print(f"order {order} window {window}: GPU FP32 {itg:.2f} iterations per solve, oracle "
+              f"FP64 {oracle_it[(order, window)]:.2f} (order {order - 1}: {below:.2f})",
               flush=True)
-        assert oracle_it[order] - 1.0 <= gpu[order] < oracle_it[order - 1]
+        assert oracle_it[(order, window)] - 1.0 <= itg < below
         dh = np.abs(w.grid.inner(sg.arrays()[0]) - w.grid.inner(so.arrays()[0]))
         assert float(np.max(dh)) < 1e-4
 
@@ -55,7 +57,7 @@ def test_iterations_follow_the_oracle_and_the_waves_stay():
 def test_fewer_iterations_than_the_reference_start_and_the_same_waves():
     w = configs.c1_rough_bed(nx=160, ny=64, steps=80)
     runs = {}
-    for order in (0, 3):
+    for order in (0, 3):  # the default window
         g = w.solver(Solver).with_precision(PRECISION_FP32).set_option("warm_start", order)
         runs[order] = run(g, w, w.steps)
     (s0, it0, u0), (s3, it3, u3) = runs[0], runs[3]
@@ -70,6 +72,8 @@ def test_the_order_is_checked():
     w = configs.c1_rough_bed(nx=32, ny=16, steps=1)
     g = w.solver(Solver).with_precision(PRECISION_FP32)
     with pytest.raises(ValueError):
-        g.set_option("warm_start", 4)
+        g.set_option("warm_start", 6)
+    with pytest.raises(ValueError):
+        g.set_option("warm_start_window", 13)
     with pytest.raises(ValueError):
         g.set_option("warm_start", -1)

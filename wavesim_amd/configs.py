@@ -217,12 +217,21 @@ def c4_large_reef(nx: int = 4096, ny: int = 2048, dx: float = 3.0, seed: int = 4
     """4096x2048 cells at 3 m: a shelf 40 m deep shoaling at 1:50 onto a reef flat 1 m deep,
     8 Gaussian reef lobes, the rough-bed Fourier overlay (16 modes, 30-300 m, 0-0.4 m), a dry
     beach strip; wave-maker H 3 m, T 16 s at 15 degrees, sponges on the west, south and north,
-    Manning 0.025, fifth order + SGN by PCG to 1e-8, breaking switch on, 6000 steps."""
+    Manning 0.025, fifth order + SGN by PCG to 1e-8, breaking switch on, 6000 steps.
+
+    Layout: the west sponge and two wavelengths of the 40 m shelf, then the 1:50 slope (1950 m)
+    and the reef flat to the beach strip at 95% of the domain. The wave-maker sits on the slope
+    at the 15 m contour, so that the waves reach the reef edge and break there within the
+    run's 6000 steps (about 190 s of simulated time); from the 40 m shelf they would need more
+    than 200 s to get there."""
     grid = Grid(nx, ny, dx, dx)
     lx, ly = nx * dx, ny * dx
     rng = Lcg(seed)
     overlay = fourier_overlay(rng, 16, (30.0, 300.0), (0.0, 0.4))
-    x_edge = 0.55 * lx  # shelf edge
+    period, height = 16.0, 3.0
+    sponge = _sponge_cells(period, 40.0, dx, nx)
+    wavelength = 2.0 * math.pi / linear_wave(2.0 * math.pi / period, 40.0, True).k
+    x_edge = sponge * dx + 2.0 * wavelength  # shelf edge
     x_flat = x_edge + 39.0 * 50.0  # 40 m -> 1 m at 1:50
     x_beach = 0.95 * lx  # dry beach strip beyond
     lobes = gaussian_lobes(rng, 8, (x_edge, x_beach), (0.0, ly), (4.0, 10.0), (40.0, 200.0))
@@ -234,9 +243,7 @@ def c4_large_reef(nx: int = 4096, ny: int = 2048, dx: float = 3.0, seed: int = 4
         return np.minimum(base + lobes(x, y), 3.0) + overlay(x, y)
 
     bed = Bathymetry.from_fn(grid, bed_fn)
-    period, height = 16.0, 3.0
-    sponge = _sponge_cells(period, 40.0, dx, nx)
-    maker_x = (sponge + 4.0 * SIGMA_CELLS) * dx
+    maker_x = x_edge + (40.0 - 15.0) * 50.0  # the 15 m contour of the slope
     maker = WaveMaker(maker_x, height / 2.0, period, math.radians(15.0), SIGMA_CELLS * dx,
                       RAMP_PERIODS)
 
@@ -251,3 +258,39 @@ def c4_large_reef(nx: int = 4096, ny: int = 2048, dx: float = 3.0, seed: int = 4
 
     notes = {"seed": seed, "sponge_cells": sponge, "maker_x_m": maker_x}
     return Workload(f"large_reef_{nx}x{ny}_fifth_sgn", grid, bed, configure, steps, notes)
+
+
+def c4_crop(i0: int = 700, j0: int = 960, nx: int = 256, ny: int = 96, sponge: int = 8,
+            steps: int = 200) -> Workload:
+    """A tile of the large reef: interior columns [i0, i0+nx) and rows [j0, j0+ny) of its bed
+    (by default the slope around the wave-maker, 15 m contour, up to the 4 m contour where the
+    waves break), with its physics: fifth order, SGN to 1e-8, Manning 0.025, the same wave-maker
+    at its place in the tile (the angle snapped to the tile's width, as wavesim snaps it), and
+    sponges `sponge` cells wide on the tile's west, south and north edges."""
+    full = c4_large_reef()
+    g = full.grid
+    inner = full.grid.inner(full.bed.b)[j0:j0 + ny, i0:i0 + nx]
+    grid = Grid(nx, ny, g.dx, g.dy)
+    bed = Bathymetry.from_interior(grid, inner)
+    maker_x = full.notes["maker_x_m"] - i0 * g.dx
+    if not 0.0 < maker_x < nx * g.dx:
+        raise ValueError("the crop must hold the wave-maker")
+    i = round(maker_x / g.dx - 0.5) + GHOST
+    depth = -bed.b[grid.idx(i, GHOST + ny // 2)]
+    period = 16.0
+    k = linear_wave(2.0 * math.pi / period, depth, True).k
+    # periodic in y is not set, so any angle is allowed; keep 15 degrees
+    maker = WaveMaker(maker_x, 1.5, period, math.radians(15.0), SIGMA_CELLS * g.dx,
+                      RAMP_PERIODS)
+
+    def configure(s):
+        s = (s.with_order(Order.Fifth)
+             .with_dispersion(Dispersion(tolerance=1e-8, max_iterations=200))
+             .with_wavemaker(maker)
+             .with_sponge(Sponge(sponge, SPONGE_STRENGTH).edges(True, False, True, True))
+             .with_manning(0.025))
+        s.cfl = 0.45
+        return s
+
+    notes = {"i0": i0, "j0": j0, "maker_depth_m": depth, "k": k}
+    return Workload(f"large_reef_crop_{nx}x{ny}", grid, bed, configure, steps, notes)

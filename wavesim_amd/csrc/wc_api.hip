@@ -68,13 +68,14 @@ enum Family {
   F_CG_ITER,
   F_NET,
   F_FRAME,
+  F_CG_FLUSH,
   F_COUNT
 };
 const char* kFamilyNames[F_COUNT] = {"fill_state", "switch",    "flux",      "finish_euler",
                                      "sgn_setup",  "cg_init",   "cg_matvec", "cg_update",
                                      "cg_pupdate", "cg_stats",  "sgn_finish", "reduce",
                                      "cg_reduce",  "relax",     "cg_iter",   "net",
-                                     "frame"};
+                                     "frame",      "cg_flush"};
 
 struct HostSponge {
   uint64_t width;
@@ -163,7 +164,8 @@ struct wc_solver {
   // the extrapolated warm start of the FP32 form's solves (WarmStart in wc_kernels.h)
   float2* ws_ring = nullptr;
   int ws_order = 3;   // exact for steps of degree < 3; 0: the reference's warm start
-  int ws_window = 0;  // past steps weighed (at least ws_order)
+  int ws_window = 6;  // past steps weighed (at least ws_order)
+  int ws_guard = 0;   // per mille: cells whose switch reads below start from x_{n-1}
 
   // the learned preconditioner (wc_fpcg.hip) and the capture of solves for its training
   int precond = WC_PRECOND_JACOBI;
@@ -871,6 +873,7 @@ WarmStart ws_args(wc_solver* s) {
     w.terms = std::max(s->ws_window, s->ws_order);
     w.period = s->order == WC_ORDER_FIRST ? 1 : 2;
     ws_coefficients(s->ws_order, w.terms, w.coef);
+    w.guard = (float)s->ws_guard * 1e-3f;
   }
   return w;
 }
@@ -962,6 +965,7 @@ CgG cg1_args(wc_solver* s, int stage) {
   a.sc = s->sc;
   a.tolerance = s->disp.tolerance;
   a.max_iterations = s->disp.max_iterations;
+  a.defer = s->capture ? 0 : 1;  // captured iterates must hold every step
   return a;
 }
 
@@ -1018,6 +1022,7 @@ int pcg1_32l(wc_solver* s, int stage, unsigned long long earlier) {
   }
   s->prev_it[stage] = s->sc_host->it;
   s->cap_last_it = s->sc_host->it;
+  TRY(launch(s, F_CG_FLUSH, k_cg1_flush, tiles(s), dim3(NT), a));
   TRY(launch(s, F_CG_STATS, k_cg_stats, dim3(1), dim3(64), s->sc, earlier));
   return WC_OK;
 }
@@ -1635,6 +1640,12 @@ int wc_solver_set_option(wc_solver* s, const char* key, int64_t value) {
     s->ws_order = (int)value;
     TRY(ws_reset(s));
     HIP_OK(hipStreamSynchronize(s->stream));
+    return WC_OK;
+  }
+  if (strcmp(key, "warm_start_guard") == 0) {
+    if (value < 0 || value > 1001)
+      return fail(WC_EINVAL, "warm_start_guard must be in [0, 1001] (per mille of the switch)");
+    s->ws_guard = (int)value;
     return WC_OK;
   }
   if (strcmp(key, "warm_start_window") == 0) {
@@ -3013,6 +3024,8 @@ int batch_pcg_tail(wc_batch* b, int st, unsigned long long k) {
     if (all || it > b->maxit) break;
   }
   enq_check(b, st, it, 1);
+  hipLaunchKernelGGL(kb_cg1_flush, bgrid(b->m_tiles.total, b->grid_nt), dim3(NT), 0, b->stream,
+                     b->cg[st], b->m_tiles);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(b->h_it, b->d_it, n * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                         b->stream));

@@ -160,31 +160,37 @@ __device__ __forceinline__ void flux_tile(const FluxTileArgs<S>& a, int mode, in
     L.u.box.phi[ly][lx] = ph;
   }
   __syncthreads();
-  // x: cells of tile columns -1..64 of every tile row; stencil along the row.
-  for (int q = threadIdx.x; q < TY * RXN; q += NT) {
-    const int cx = q % RXN, row = q / RXN;
-    const int y = row + 3, x0 = cx;  // stencil columns cx..cx+4 in the box = tile cols cx-3..cx+1
-    reconstruct_r<R>(&L.u.box.eta[y][x0], &L.u.box.h[y][x0], &L.u.box.un_x[y][x0],
-                     &L.u.box.un_y[y][x0], &L.u.box.b[y][x0], &L.u.box.phi[y][x0], 1, a.order,
-                     L.xlo[row][cx], L.xhi[row][cx]);
-  }
-  // y: tile columns 0..63 of tile rows -1..4; stencil along the column.
-  for (int q = threadIdx.x; q < RYN * TX; q += NT) {
-    const int col = q % TX, ry = q / TX;
-    const int x = col + 3, y0 = ry;
-    reconstruct_r<R>(&L.u.box.eta[y0][x], &L.u.box.h[y0][x], &L.u.box.un_y[y0][x],
-                     &L.u.box.un_x[y0][x], &L.u.box.b[y0][x], &L.u.box.phi[y0][x], BX, a.order,
-                     L.ylo[ry][col], L.yhi[ry][col]);
+  // One work list for both axes, so that no pass runs with only a few threads busy:
+  // x: cells of tile columns -1..64 of every tile row, stencil along the row; then
+  // y: tile columns 0..63 of tile rows -1..4, stencil along the column.
+  for (int q = threadIdx.x; q < TY * RXN + RYN * TX; q += NT) {
+    if (q < TY * RXN) {
+      const int cx = q % RXN, row = q / RXN;
+      const int y = row + 3, x0 = cx;  // stencil columns cx..cx+4 = tile cols cx-3..cx+1
+      reconstruct_r<R>(&L.u.box.eta[y][x0], &L.u.box.h[y][x0], &L.u.box.un_x[y][x0],
+                       &L.u.box.un_y[y][x0], &L.u.box.b[y][x0], &L.u.box.phi[y][x0], 1, a.order,
+                       L.xlo[row][cx], L.xhi[row][cx]);
+    } else {
+      const int qy = q - TY * RXN;
+      const int col = qy % TX, ry = qy / TX;
+      const int x = col + 3, y0 = ry;
+      reconstruct_r<R>(&L.u.box.eta[y0][x], &L.u.box.h[y0][x], &L.u.box.un_y[y0][x],
+                       &L.u.box.un_x[y0][x], &L.u.box.b[y0][x], &L.u.box.phi[y0][x], BX,
+                       a.order, L.ylo[ry][col], L.yhi[ry][col]);
+    }
   }
   __syncthreads();
-  // faces: x-face f of a row sits between tile columns f-1 and f.
-  for (int q = threadIdx.x; q < TY * (TX + 1); q += NT) {
-    const int f = q % (TX + 1), row = q / (TX + 1);
-    L.u.faces.fx[row][f] = face_r<R>(L.xhi[row][f], L.xlo[row][f + 1]);
-  }
-  for (int q = threadIdx.x; q < (TY + 1) * TX; q += NT) {
-    const int col = q % TX, f = q / TX;
-    L.u.faces.fy[f][col] = face_r<R>(L.yhi[f][col], L.ylo[f + 1][col]);
+  // faces, one list again: x-face f of a row sits between tile columns f-1 and f; y-face f of a
+  // column between tile rows f-1 and f.
+  for (int q = threadIdx.x; q < TY * (TX + 1) + (TY + 1) * TX; q += NT) {
+    if (q < TY * (TX + 1)) {
+      const int f = q % (TX + 1), row = q / (TX + 1);
+      L.u.faces.fx[row][f] = face_r<R>(L.xhi[row][f], L.xlo[row][f + 1]);
+    } else {
+      const int qy = q - TY * (TX + 1);
+      const int col = qy % TX, f = qy / TX;
+      L.u.faces.fy[f][col] = face_r<R>(L.yhi[f][col], L.ylo[f + 1][col]);
+    }
   }
   __syncthreads();
   const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;

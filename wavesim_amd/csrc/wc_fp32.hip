@@ -136,7 +136,8 @@ __device__ __forceinline__ void solve_start(const Sgn32& a, int k, bool wet) {
   if (!e) return;
   const unsigned long long n = ws.sc->solves;
   double px = 0.0, py = 0.0;
-  for (int j = 1; j <= ws.terms; ++j) {
+  const int terms = a.phi[k] < ws.guard ? 0 : ws.terms;
+  for (int j = 1; j <= terms; ++j) {
     const unsigned long long m = n - (unsigned long long)(j * ws.period);  // mod 2^64
     const float2 d = ws.ring[(m % WS_SLOTS) * ws.stride + k];
     px += ws.coef[j - 1] * (double)d.x;

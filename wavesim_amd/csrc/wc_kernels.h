@@ -139,6 +139,10 @@ struct CgG {
   CgScalars* sc;
   double tolerance;
   unsigned long long max_iterations;
+  // w += alpha p is taken two iterations at a time, in the odd ones (w += alpha_{i-1} p_{i-1}
+  // + alpha_i p_i, p_{i-1} being read there anyway), which halves the traffic of w; a solve
+  // that ends after an odd number of iterations adds its last step in k_cg1_flush.
+  int defer;
 };
 // The flexible PCG with the learned preconditioner (wc_fpcg.hip), FP32 form. Vectors are
 // interleaved (x, y) pairs in grid layout; lanes: [0] b.b, [1] r.r, [2] r.z, [3] r.z_old,
@@ -182,6 +186,7 @@ template <int L>
 __global__ void k_cg1_iter(CgG a, unsigned long long n);
 template <int L>
 __global__ void k_cg1_check(CgG a, unsigned long long n);
+__global__ void k_cg1_flush(CgG a);  // 64x4 tiles
 // the same, blocks marching down column strips (wc_march.h)
 template <int L>
 __global__ void k_cg1m_init(CgG a);
@@ -227,6 +232,7 @@ struct WarmStart {
   const CgScalars* sc;             // sc->solves: solves completed so far
   int terms, period;               // K, solves per step
   double coef[WS_MAX_WINDOW];      // c_1 .. c_K
+  float guard;                     // cells whose switch reads below it start unextrapolated
 };
 
 // The SGN solution of the FP32 form is kept in FP64 (solx, soly: the warm start, shared with the
@@ -425,6 +431,7 @@ template <int L>
 __global__ void kb_cg1_start(const CgG* args, BMap m);
 template <int L>
 __global__ void kb_cg1_iter(const CgG* args, unsigned long long n, BMap m);
+__global__ void kb_cg1_flush(const CgG* args, BMap m);  // m: the instances' 64x4 tiles
 template <int L>
 __global__ void kbm_cg1_init(const CgG* args, BMap m);
 template <int L>

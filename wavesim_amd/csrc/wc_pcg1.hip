@@ -38,7 +38,7 @@ using namespace t16;
 // solve is over. Every wave computes the same numbers; block 0 records them.
 template <int L>
 __device__ __forceinline__ bool start(const CgG& a, unsigned long long n, int blk, double& alpha,
-                                      double& beta) {
+                                      double& beta, double& alpha_prev) {
   CgScalars* sc = a.sc;
   const unsigned long long i = n - 1;
   double target;
@@ -58,9 +58,11 @@ __device__ __forceinline__ bool start(const CgG& a, unsigned long long n, int bl
   if (i == 0) {
     beta = 0.0;
     alpha = gamma / delta;
+    alpha_prev = 0.0;
   } else {
+    alpha_prev = sc->alphas[(i - 1) & 1];
     beta = gamma / sc->rzs[(i - 1) & 1];
-    alpha = gamma / (delta - beta * gamma / sc->alphas[(i - 1) & 1]);
+    alpha = gamma / (delta - beta * gamma / alpha_prev);
   }
   if (blk == 0 && threadIdx.x == 0) {
     if (n == 1) sc->target = target;
@@ -167,8 +169,8 @@ template <int L>
 __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n, int blk) {
   const Geo& g = a.g;
   const int nt = tiles_all(g);
-  double alpha = 0.0, beta = 0.0;
-  if (blk < nt && !start<L>(a, n, blk, alpha, beta)) return;
+  double alpha = 0.0, beta = 0.0, alpha_prev = 0.0;
+  if (blk < nt && !start<L>(a, n, blk, alpha, beta, alpha_prev)) return;
   const unsigned long long i = n - 1;
   const int cur = (int)(i & 1), nxt = cur ^ 1;
   const float2 *r0 = a.r[cur], *q0 = a.q[cur];
@@ -177,21 +179,32 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
   __shared__ double s_u[2][S2Y][S2X], s_f[S1Y][S1X];
   __shared__ double sums[2][BY][3];
   double acc[3] = {0.0, 0.0, 0.0};
+  // u_{i+1} = D^-1 (r_i - alpha (q_i + beta s_{i-1})) on the tile + 2, from q_i, s_{i-1}, r_i
+  // and D^-1 prefetched a tile ahead
+  auto fetch = [&](int m, float2 (&v)[4]) {
+    v[0] = q0[m];
+    v[1] = s0[m];
+    v[2] = r0[m];
+    v[3] = a.di[m];
+  };
+  auto make = [&](const float2 (&v)[4], double& vx, double& vy) {
+    const double sx = (double)(float)((double)v[0].x + beta * (double)v[1].x);
+    const double sy = (double)(float)((double)v[0].y + beta * (double)v[1].y);
+    vx = (double)(float)((double)v[2].x + na * sx) * (double)v[3].x;
+    vy = (double)(float)((double)v[2].y + na * sy) * (double)v[3].y;
+  };
+  Prefetch<4> P;
+  if (blk < nt) prefetch(g, blk, fetch, P);
   for (int r = 0; blk + L * r < nt; ++r) {
     const int tile = blk + L * r;
+    stage(P, s_u, make);
+    __syncthreads();
+    if (tile + L < nt) prefetch(g, tile + L, fetch, P);
+    stencil_f(g, a.c, tile, s_u, s_f);
     bool own;
     int k;
     double ux, uy, ax, ay;
-    // u_{i+1} = D^-1 (r_i - alpha (q_i + beta s_{i-1})) on the tile + 2
-    stencil(g, a.c, a.h, tile, s_u, s_f,
-            [&](int m, double& vx, double& vy) {
-              const float2 qq = q0[m], ss = s0[m], rr = r0[m], d = a.di[m];
-              const double sx = (double)(float)((double)qq.x + beta * (double)ss.x);
-              const double sy = (double)(float)((double)qq.y + beta * (double)ss.y);
-              vx = (double)(float)((double)rr.x + na * sx) * (double)d.x;
-              vy = (double)(float)((double)rr.y + na * sy) * (double)d.y;
-            },
-            own, k, ux, uy, ax, ay);
+    stencil_own(g, a.h, tile, s_u, s_f, own, k, ux, uy, ax, ay);
     double v[3] = {0.0, 0.0, 0.0};
     if (own) {
       const float2 qq = q0[k], ss = s0[k], rr = r0[k], d = a.di[k], pp = a.p[k];
@@ -200,8 +213,13 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
       const double px = (double)(float)((double)rr.x * (double)d.x + beta * (double)pp.x);
       const double py = (double)(float)((double)rr.y * (double)d.y + beta * (double)pp.y);
       a.p[k] = f2(px, py);
-      a.wx[k] = (float)((double)a.wx[k] + alpha * px);
-      a.wy[k] = (float)((double)a.wy[k] + alpha * py);
+      if (!a.defer) {
+        a.wx[k] = (float)((double)a.wx[k] + alpha * px);
+        a.wy[k] = (float)((double)a.wy[k] + alpha * py);
+      } else if (i & 1) {  // iterations i - 1 and i together (CgG::defer)
+        a.wx[k] = (float)((double)a.wx[k] + (alpha_prev * (double)pp.x + alpha * px));
+        a.wy[k] = (float)((double)a.wy[k] + (alpha_prev * (double)pp.y + alpha * py));
+      }
       a.s[cur][k] = f2(sx, sy);
       const double nrx = (double)(float)((double)rr.x + na * sx);
       const double nry = (double)(float)((double)rr.y + na * sy);
@@ -227,8 +245,34 @@ __global__ void __launch_bounds__(PT) k_cg1_iter(CgG a, unsigned long long n) {
 // K_n's prologue alone, one wave: whether the solve is over before iteration n - 1.
 template <int L>
 __global__ void __launch_bounds__(64) k_cg1_check(CgG a, unsigned long long n) {
-  double alpha, beta;
-  start<L>(a, n, 0, alpha, beta);
+  double alpha, beta, alpha_prev;
+  start<L>(a, n, 0, alpha, beta, alpha_prev);
+}
+
+// The step deferred from the last iteration of a solve that ended after an odd number of
+// iterations (CgG::defer): w += alpha_{it-1} p_{it-1}, at the cells of one 64x4 tile.
+__device__ __forceinline__ void cg1_flush_body(const CgG& a, int tile) {
+  const CgScalars* sc = a.sc;
+  const unsigned long long it = sc->it;
+  if (!a.defer || !(it & 1)) return;
+  const double alpha = sc->alphas[(it - 1) & 1];
+  const Geo& g = a.g;
+  const int i = GH + (tile % g.ntx) * TX + (threadIdx.x & 63);
+  const int j = GH + (tile / g.ntx) * TY + (threadIdx.x >> 6);
+  if (i >= g.nx + GH || j >= g.ny + GH) return;
+  const int k = j * g.w + i;
+  const float2 pp = a.p[k];
+  a.wx[k] = (float)((double)a.wx[k] + alpha * (double)pp.x);
+  a.wy[k] = (float)((double)a.wy[k] + alpha * (double)pp.y);
+}
+
+__global__ void __launch_bounds__(NT) k_cg1_flush(CgG a) { cg1_flush_body(a, blockIdx.x); }
+
+__global__ void __launch_bounds__(NT) kb_cg1_flush(const CgG* args, BMap m) {
+  for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
+    const int q = batch_instance(m, vb);
+    cg1_flush_body(args[q], vb - m.first[q]);
+  }
 }
 
 // ---- the same three kernels marching down column strips (wc_march.h) -----------------------
@@ -302,8 +346,8 @@ __device__ __forceinline__ void cg1m_start_body(const CgG& a, int blk) {
 template <int L>
 __device__ __forceinline__ void cg1m_iter_body(const CgG& a, unsigned long long n, int blk) {
   const Geo& g = a.g;
-  double alpha = 0.0, beta = 0.0;
-  if (!start<L>(a, n, blk, alpha, beta)) return;
+  double alpha = 0.0, beta = 0.0, alpha_prev = 0.0;
+  if (!start<L>(a, n, blk, alpha, beta, alpha_prev)) return;
   const unsigned long long i = n - 1;
   const int cur = (int)(i & 1), nxt = cur ^ 1;
   const float2 *r0 = a.r[cur], *q0 = a.q[cur];
@@ -330,8 +374,13 @@ __device__ __forceinline__ void cg1m_iter_body(const CgG& a, unsigned long long 
         const double px = (double)(float)((double)rr.x * (double)d.x + beta * (double)pp.x);
         const double py = (double)(float)((double)rr.y * (double)d.y + beta * (double)pp.y);
         a.p[k] = f2(px, py);
-        a.wx[k] = (float)((double)a.wx[k] + alpha * px);
-        a.wy[k] = (float)((double)a.wy[k] + alpha * py);
+        if (!a.defer) {
+          a.wx[k] = (float)((double)a.wx[k] + alpha * px);
+          a.wy[k] = (float)((double)a.wy[k] + alpha * py);
+        } else if (i & 1) {  // iterations i - 1 and i together (CgG::defer)
+          a.wx[k] = (float)((double)a.wx[k] + (alpha_prev * (double)pp.x + alpha * px));
+          a.wy[k] = (float)((double)a.wy[k] + (alpha_prev * (double)pp.y + alpha * py));
+        }
         a.s[cur][k] = f2(sx, sy);
         const double nrx = (double)(float)((double)rr.x + na * sx);
         const double nry = (double)(float)((double)rr.y + na * sy);
@@ -423,8 +472,8 @@ __global__ void __launch_bounds__(64) kb_cg1_check(const CgG* args, unsigned lon
                                                    int stats, int* done, unsigned long long* it) {
   const int q = blockIdx.x;
   const CgG& a = args[q];
-  double alpha, beta;
-  start<L>(a, n, 0, alpha, beta);
+  double alpha, beta, alpha_prev;
+  start<L>(a, n, 0, alpha, beta, alpha_prev);
   if (threadIdx.x == 0) {
     CgScalars* sc = a.sc;
     done[q] = sc->done;

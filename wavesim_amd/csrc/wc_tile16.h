@@ -75,33 +75,13 @@ __device__ __forceinline__ void rows_total(double (&acc)[N], double (*sums)[BY][
     }
 }
 
-// Stages the vector u (given per halo cell by fn(m, vx, vy), m the mapped interior index; the
-// wall flips are applied here) on the tile + 2 and c (grad . u) on the tile + 1; returns A u
-// at the thread's cell (and u there).
-template <class Fn>
-__device__ __forceinline__ void stencil(const Geo& g, const float* c, const float* hh, int tile,
-                                        double (*s_u)[S2Y][S2X], double (*s_f)[S1X], Fn fn,
-                                        bool& own, int& k, double& ux, double& uy, double& ax,
-                                        double& ay) {
-  const int t0 = threadIdx.x;
+// c (grad . u) on the tile + 1 from the staged u, then a barrier.
+__device__ __forceinline__ void stencil_f(const Geo& g, const float* c, int tile,
+                                          double (*s_u)[S2Y][S2X], double (*s_f)[S1X]) {
   const int tx = tiles_x(g);
   const int i0 = GH + (tile % tx) * BX, j0 = GH + (tile / tx) * BY;
   const double ix = 0.5 / g.dx, iy = 0.5 / g.dy;
-  for (int t = t0; t < H2; t += PT) {
-    const int lx = t % S2X, ly = t / S2X;
-    const int i = i0 - 2 + lx, j = j0 - 2 + ly;
-    double vx = 0.0, vy = 0.0;
-    if (i <= g.nx + GH + 1 && j <= g.ny + GH + 1) {
-      const int m = map_y(j, g.ny, g.periodic) * g.w + map_x(i, g.nx);
-      fn(m, vx, vy);
-      if (is_ghost_x(i, g.nx)) vx = -vx;
-      if (!g.periodic && is_ghost_y(j, g.ny)) vy = -vy;
-    }
-    s_u[0][ly][lx] = vx;
-    s_u[1][ly][lx] = vy;
-  }
-  __syncthreads();
-  for (int t = t0; t < H1; t += PT) {
+  for (int t = threadIdx.x; t < H1; t += PT) {
     const int lx = t % S1X, ly = t / S1X;
     const int i = i0 - 1 + lx, j = j0 - 1 + ly;
     double f = 0.0;
@@ -113,6 +93,17 @@ __device__ __forceinline__ void stencil(const Geo& g, const float* c, const floa
     s_f[ly][lx] = f;
   }
   __syncthreads();
+}
+
+// A u at the thread's cell (and u there) from the staged u and f.
+__device__ __forceinline__ void stencil_own(const Geo& g, const float* hh, int tile,
+                                            double (*s_u)[S2Y][S2X], double (*s_f)[S1X],
+                                            bool& own, int& k, double& ux, double& uy,
+                                            double& ax, double& ay) {
+  const int t0 = threadIdx.x;
+  const int tx = tiles_x(g);
+  const int i0 = GH + (tile % tx) * BX, j0 = GH + (tile / tx) * BY;
+  const double ix = 0.5 / g.dx, iy = 0.5 / g.dy;
   const int lx = t0 & 63, ly = t0 >> 6;
   const int i = i0 + lx, j = j0 + ly;
   own = i < g.nx + GH && j < g.ny + GH;
@@ -130,6 +121,85 @@ __device__ __forceinline__ void stencil(const Geo& g, const float* c, const floa
   }
 }
 
+// Stages the vector u (given per halo cell by fn(m, vx, vy), m the mapped interior index; the
+// wall flips are applied here) on the tile + 2 and c (grad . u) on the tile + 1; returns A u
+// at the thread's cell (and u there).
+template <class Fn>
+__device__ __forceinline__ void stencil(const Geo& g, const float* c, const float* hh, int tile,
+                                        double (*s_u)[S2Y][S2X], double (*s_f)[S1X], Fn fn,
+                                        bool& own, int& k, double& ux, double& uy, double& ax,
+                                        double& ay) {
+  const int t0 = threadIdx.x;
+  const int tx = tiles_x(g);
+  const int i0 = GH + (tile % tx) * BX, j0 = GH + (tile / tx) * BY;
+  for (int t = t0; t < H2; t += PT) {
+    const int lx = t % S2X, ly = t / S2X;
+    const int i = i0 - 2 + lx, j = j0 - 2 + ly;
+    double vx = 0.0, vy = 0.0;
+    if (i <= g.nx + GH + 1 && j <= g.ny + GH + 1) {
+      const int m = map_y(j, g.ny, g.periodic) * g.w + map_x(i, g.nx);
+      fn(m, vx, vy);
+      if (is_ghost_x(i, g.nx)) vx = -vx;
+      if (!g.periodic && is_ghost_y(j, g.ny)) vy = -vy;
+    }
+    s_u[0][ly][lx] = vx;
+    s_u[1][ly][lx] = vy;
+  }
+  __syncthreads();
+  stencil_f(g, c, tile, s_u, s_f);
+  stencil_own(g, hh, tile, s_u, s_f, own, k, ux, uy, ax, ay);
+}
+
+// The staged vector's inputs at a tile's halo points (at most two per thread), fetched into
+// registers ahead of their use, so that the next tile's loads are in flight while this tile
+// computes. NV float2 values per point.
+template <int NV>
+struct Prefetch {
+  float2 v[2][NV];
+  int flags[2];  // 1: inside the grid's two rings; 2, 4: flip x, y (wall ghosts)
+};
+
+template <int NV, class Fetch>
+__device__ __forceinline__ void prefetch(const Geo& g, int tile, Fetch fetch, Prefetch<NV>& P) {
+  const int tx = tiles_x(g);
+  const int i0 = GH + (tile % tx) * BX, j0 = GH + (tile / tx) * BY;
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int t = threadIdx.x + e * PT;
+    P.flags[e] = 0;
+    if (t < H2) {
+      const int lx = t % S2X, ly = t / S2X;
+      const int i = i0 - 2 + lx, j = j0 - 2 + ly;
+      if (i <= g.nx + GH + 1 && j <= g.ny + GH + 1) {
+        const int m = map_y(j, g.ny, g.periodic) * g.w + map_x(i, g.nx);
+        fetch(m, P.v[e]);
+        P.flags[e] = 1 | (is_ghost_x(i, g.nx) ? 2 : 0) |
+                     (!g.periodic && is_ghost_y(j, g.ny) ? 4 : 0);
+      }
+    }
+  }
+}
+
+// u from the prefetched inputs by make(v, vx, vy), into LDS on the tile + 2.
+template <int NV, class Make>
+__device__ __forceinline__ void stage(const Prefetch<NV>& P, double (*s_u)[S2Y][S2X],
+                                      Make make) {
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int t = threadIdx.x + e * PT;
+    if (t < H2) {
+      const int lx = t % S2X, ly = t / S2X;
+      double vx = 0.0, vy = 0.0;
+      if (P.flags[e] & 1) {
+        make(P.v[e], vx, vy);
+        if (P.flags[e] & 2) vx = -vx;
+        if (P.flags[e] & 4) vy = -vy;
+      }
+      s_u[0][ly][lx] = vx;
+      s_u[1][ly][lx] = vy;
+    }
+  }
+}
 
 }  // namespace t16
 }  // namespace wc
