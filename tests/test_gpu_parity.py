@@ -148,6 +148,32 @@ def test_large_reef_tile_is_bitwise_the_oracle():
     lockstep(w.grid, w.bed, w.configure, s, 150, "large reef tile")
 
 
+@pytest.mark.parametrize("order,window", [(3, 6), (2, 2)])
+def test_fp64_extrapolated_warm_start_is_bitwise_the_oracle(order, window):
+    """The extrapolated warm start (no counterpart in the reference: its solves start from the
+    previous solution) as an option of the FP64 path: ring slots, weights and the start, then
+    the reference's PCG from it, bit for bit the oracle's with_warm_start."""
+    w = configs.c1_rough_bed(nx=160, ny=64, steps=120)
+    s = w.still_water()
+    s.time = 40.0
+
+    def configure(sv):
+        return w.configure(sv)
+
+    g = Solver(w.grid, w.bed.copy())
+    g = configure(g).set_option("fp64_warm_start", order).set_option("warm_start_window", window)
+    o = configure(oracle(w.grid, w.bed.copy())).with_warm_start(order, window)
+    sg, so = s.clone(), s.clone()
+    for n in range(w.steps):
+        dg, do = g.stable_dt(sg), o.stable_dt(so)
+        assert same_float(dg, do), (n, dg, do)
+        g.step(sg, dg)
+        o.step(so, do)
+    assert_states_equal(sg, so, f"fp64 warm start {order} {window}")
+    assert g.dispersion_stats() == o.dispersion_stats()
+    print(f"fp64 warm start {order} window {window}: {g.dispersion_stats()}", flush=True)
+
+
 def test_sgn_second_order_linear_dispersion_and_held_cells():
     grid = Grid(120, 8, 1.0, 1.0)
     bed = Bathymetry.flat(grid, 2.0)

@@ -92,9 +92,10 @@ def test_weights_are_the_least_norm_exact_ones():
             if window == order:  # the binomial weights, exactly
                 binom = [(-1) ** (j + 1) * math.comb(order, j) for j in range(1, order + 1)]
                 assert list(got) == binom
-            if have_lib:  # the product's host-side weights (no GPU needed)
+            if have_lib:  # the product's host-side weights (no GPU needed), bit for bit:
+                # the FP64 path's warm start is held to the oracle's bitwise
                 prod = gpu.warm_start_weights(order, window)
-                assert np.allclose(prod, got, rtol=1e-12, atol=1e-12 * np.abs(got).max())
+                assert np.array_equal(prod, got), (order, window, prod, got)
 
 
 def test_wider_windows_let_less_noise_through():

@@ -163,6 +163,8 @@ struct wc_solver {
          *g_s[2] = {nullptr, nullptr}, *g_p = nullptr, *g_di = nullptr;
   // the extrapolated warm start of the FP32 form's solves (WarmStart in wc_kernels.h)
   float2* ws_ring = nullptr;
+  double2* ws_ring64 = nullptr;  // the FP64 path's, allocated when "fp64_warm_start" is set
+  int ws_order64 = 0;            // the FP64 path's warm start: 0, the reference's
   int ws_order = 3;   // exact for steps of degree < 3; 0: the reference's warm start
   int ws_window = 6;  // past steps weighed (at least ws_order)
   int ws_guard = 0;   // per mille: cells whose switch reads below start from x_{n-1}
@@ -501,6 +503,8 @@ StepArgs step_args(wc_solver* s, double dt, int stage) {
   return a;
 }
 
+WarmStart ws_args(wc_solver* s);  // below, with the FP32 form
+
 SgnArgs sgn_args(wc_solver* s, int stage) {
   SgnArgs a{};
   a.g = s->geo;
@@ -531,6 +535,7 @@ SgnArgs sgn_args(wc_solver* s, int stage) {
   a.nonlinear = s->disp.nonlinear ? 1 : 0;
   a.tolerance = s->disp.tolerance;
   a.max_iterations = s->disp.max_iterations;
+  a.ws = ws_args(s);
   return a;
 }
 
@@ -689,7 +694,7 @@ int stage(wc_solver* s, double dt, int stage, int mode) {
   TRY(launch(s, F_SGN_SETUP, k_sgn_setup, tiles(s), dim3(NT), sgn_args(s, stage)));
   TRY(pcg(s, stage));
   TRY(launch(s, F_SGN_FINISH, k_sgn_finish, tiles(s), dim3(NT), a, (const double*)s->wx,
-             (const double*)s->wy, mode));
+             (const double*)s->wy, mode, ws_args(s)));
   return WC_OK;
 }
 
@@ -817,6 +822,8 @@ int alloc_fp32(wc_solver* s) {
 int ws_reset(wc_solver* s) {
   if (s->ws_ring)
     HIP_OK(hipMemsetAsync(s->ws_ring, 0, WS_SLOTS * s->cells * sizeof(float2), s->stream));
+  if (s->ws_ring64)
+    HIP_OK(hipMemsetAsync(s->ws_ring64, 0, WS_SLOTS * s->cells * sizeof(double2), s->stream));
   return WC_OK;
 }
 
@@ -866,6 +873,14 @@ void ws_coefficients(int m, int K, double* c) {
 
 WarmStart ws_args(wc_solver* s) {
   WarmStart w{};
+  if (s->ws_order64 > 0 && s->dispersive && s->precision == WC_PRECISION_FP64 && s->ws_ring64) {
+    w.ring64 = s->ws_ring64;
+    w.stride = s->cells;
+    w.sc = s->sc;
+    w.terms = std::max(s->ws_window, s->ws_order64);
+    w.period = s->order == WC_ORDER_FIRST ? 1 : 2;
+    ws_coefficients(s->ws_order64, w.terms, w.coef);
+  }
   if (s->ws_order > 0 && s->dispersive && s->precision == WC_PRECISION_FP32 && s->ws_ring) {
     w.ring = s->ws_ring;
     w.stride = s->cells;
@@ -1638,6 +1653,16 @@ int wc_solver_set_option(wc_solver* s, const char* key, int64_t value) {
       return fail(WC_EINVAL, "warm_start must be in [0, 5] (0: the previous solution)");
     TRY(set_device(s));
     s->ws_order = (int)value;
+    TRY(ws_reset(s));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    return WC_OK;
+  }
+  if (strcmp(key, "fp64_warm_start") == 0) {
+    if (value < 0 || value > WS_MAX_ORDER)
+      return fail(WC_EINVAL, "fp64_warm_start must be in [0, 5] (0: the previous solution)");
+    TRY(set_device(s));
+    if (value > 0 && !s->ws_ring64) TRY(dalloc(s, &s->ws_ring64, WS_SLOTS * s->cells));
+    s->ws_order64 = (int)value;
     TRY(ws_reset(s));
     HIP_OK(hipStreamSynchronize(s->stream));
     return WC_OK;

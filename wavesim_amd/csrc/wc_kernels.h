@@ -96,6 +96,37 @@ struct FluxTileArgs {
   int ramp_prm, sin_prm;
 };
 
+// The extrapolated warm start of the SGN solves (the FP32 form's default; an option in FP64). The reference starts each solve
+// from the previous solution x_{n-1} (dispersion.rs:220-222, 488-490). Solves come in a fixed
+// rhythm of `period` per step (2 for SSP-RK2, 1 for Euler), so solve n's solution is a smooth
+// function of time, sampled every `period` solves for each kind (RK stage), and
+//   x_n ~ x_{n-1} + sum_{j=1..K} c_j d_{n - j period},   d_n = x_n - x_{n-1}.
+// The weights c are exact for steps d that are polynomials of degree < m in time (each kind its
+// own: then the solutions are a polynomial of degree <= m plus, per kind, one of degree < m) and,
+// when the window K exceeds m, of least norm among those: sum c_j^2 sets how much of the
+// solutions' own noise (the FP32 rates' rounding, the solves' tolerance) reaches the start.
+// K = m gives the binomial weights of the m-th backward difference set to zero (3, -3, 1 for
+// m = 3). The ring holds the last WS_SLOTS steps d, slot n % WS_SLOTS for solve n, indexed by
+// the solve counter in CgScalars (so replays of captured graphs and batched instances find
+// their own slots). The set-up stores the extrapolation x0 - x_{n-1} into slot n and starts the
+// solve at x0; the stage update, after the solve, adds the solve's correction, so slot n holds
+// d_n = x_n - x_{n-1}. Masked cells start from zero, as the reference's, and record d = 0. The
+// start never changes what a solve converges to (same residual test, same tolerance), only how
+// many iterations it takes.
+constexpr int WS_SLOTS = 32;
+constexpr int WS_MAX_ORDER = 5;
+constexpr int WS_MAX_WINDOW = 12;  // WS_MAX_WINDOW * period < WS_SLOTS
+struct WarmStart {
+  float2* ring;                    // WS_SLOTS x stride, interleaved (x, y); null: the reference's
+  double2* ring64;                 // the FP64 path's ring (its slot n holds x_{n-1} during solve
+                                   // n, as the oracle's), or null
+  unsigned long long stride;       // cells per slot
+  const CgScalars* sc;             // sc->solves: solves completed so far
+  int terms, period;               // K, solves per step
+  double coef[WS_MAX_WINDOW];      // c_1 .. c_K
+  float guard;                     // cells whose switch reads below it start unextrapolated
+};
+
 struct SgnArgs {
   Geo g;
   const double *h, *hu, *hv, *phi;
@@ -107,6 +138,7 @@ struct SgnArgs {
   int nonlinear;
   double tolerance;
   unsigned long long max_iterations;
+  WarmStart ws;  // ring64 set: the FP64 path's extrapolated warm start
 };
 
 // The 256-lane PCG (wc_pcg.hip), on FP64 (V = double) or FP32 (V = float) storage.
@@ -206,34 +238,6 @@ template <typename V>
 __global__ void k_cg3_check(CgV<V> a, unsigned long long n);
 
 // The FP32 chosen form (wc_fp32.hip).
-// The extrapolated warm start of the FP32 form's SGN solves. The reference starts each solve
-// from the previous solution x_{n-1} (dispersion.rs:220-222, 488-490). Solves come in a fixed
-// rhythm of `period` per step (2 for SSP-RK2, 1 for Euler), so solve n's solution is a smooth
-// function of time, sampled every `period` solves for each kind (RK stage), and
-//   x_n ~ x_{n-1} + sum_{j=1..K} c_j d_{n - j period},   d_n = x_n - x_{n-1}.
-// The weights c are exact for steps d that are polynomials of degree < m in time (each kind its
-// own: then the solutions are a polynomial of degree <= m plus, per kind, one of degree < m) and,
-// when the window K exceeds m, of least norm among those: sum c_j^2 sets how much of the
-// solutions' own noise (the FP32 rates' rounding, the solves' tolerance) reaches the start.
-// K = m gives the binomial weights of the m-th backward difference set to zero (3, -3, 1 for
-// m = 3). The ring holds the last WS_SLOTS steps d, slot n % WS_SLOTS for solve n, indexed by
-// the solve counter in CgScalars (so replays of captured graphs and batched instances find
-// their own slots). The set-up stores the extrapolation x0 - x_{n-1} into slot n and starts the
-// solve at x0; the stage update, after the solve, adds the solve's correction, so slot n holds
-// d_n = x_n - x_{n-1}. Masked cells start from zero, as the reference's, and record d = 0. The
-// start never changes what a solve converges to (same residual test, same tolerance), only how
-// many iterations it takes.
-constexpr int WS_SLOTS = 32;
-constexpr int WS_MAX_ORDER = 5;
-constexpr int WS_MAX_WINDOW = 12;  // WS_MAX_WINDOW * period < WS_SLOTS
-struct WarmStart {
-  float2* ring;                    // WS_SLOTS x stride, interleaved (x, y); null: the reference's
-  unsigned long long stride;       // cells per slot
-  const CgScalars* sc;             // sc->solves: solves completed so far
-  int terms, period;               // K, solves per step
-  double coef[WS_MAX_WINDOW];      // c_1 .. c_K
-  float guard;                     // cells whose switch reads below it start unextrapolated
-};
 
 // The SGN solution of the FP32 form is kept in FP64 (solx, soly: the warm start, shared with the
 // FP64 path's), and each solve iterates on an FP32 correction w = x - x0 from its start x0: the
@@ -306,7 +310,8 @@ __global__ void k_cg_stats(CgScalars* sc, unsigned long long earlier);
 __global__ void k_cg_matvec2(SgnArgs a, int first, const double* psx, const double* psy,
                              double* pdx, double* pdy);
 __global__ void k_cg_update2(SgnArgs a, const double* pcx, const double* pcy);
-__global__ void k_sgn_finish(StepArgs a, const double* wx, const double* wy, int mode);
+__global__ void k_sgn_finish(StepArgs a, const double* wx, const double* wy, int mode,
+                             WarmStart ws);
 __global__ void k_flux_tile64(FluxTileArgs<double> f, StepArgs a, int mode);
 __global__ void k_flux_tile32(FluxTileArgs<float> f, double* lh, float* lhu, float* lhv);
 __global__ void k_flux_tile32r(FluxTileArgs<float> f, double* lh, float* lhu, float* lhv);

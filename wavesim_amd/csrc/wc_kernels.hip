@@ -220,6 +220,29 @@ __global__ void __launch_bounds__(NT) k_sgn_setup(SgnArgs a) {
   const double heff = wc_max(h, WC_H_DRY);
   const bool masked = !(h >= WC_H_DISPERSION_MIN);
   double dgx, dgy, bx, by;
+  if (a.ws.ring64) {  // the extrapolated start (the oracle's with_warm_start, bit for bit)
+    const WarmStart& ws = a.ws;
+    const unsigned long long n = ws.sc->solves;
+    double2* e = ws.ring64 + (n % WS_SLOTS) * ws.stride;
+    if (masked) {
+      e[k] = make_double2(0.0, 0.0);
+    } else {
+      double x = a.wx[k], y = a.wy[k];
+      e[k] = make_double2(x, y);
+      for (int q = 1; q <= ws.terms; ++q) {
+        const unsigned long long m = n - (unsigned long long)(q * ws.period);  // mod 2^64
+        const double2 d = ws.ring64[(m % WS_SLOTS) * ws.stride + k];
+        x += ws.coef[q - 1] * d.x;
+      }
+      for (int q = 1; q <= ws.terms; ++q) {
+        const unsigned long long m = n - (unsigned long long)(q * ws.period);
+        const double2 d = ws.ring64[(m % WS_SLOTS) * ws.stride + k];
+        y += ws.coef[q - 1] * d.y;
+      }
+      a.wx[k] = x;
+      a.wy[k] = y;
+    }
+  }
   if (masked) {
     dgx = 1.0;
     dgy = 1.0;
@@ -554,7 +577,7 @@ __global__ void k_cg_stats(CgScalars* sc, unsigned long long earlier) {
 // d(hu)/dt = h w + u dh/dt where the cell takes part (dispersion.rs:547-562), then the stage
 // update of k_flux.
 __global__ void __launch_bounds__(NT) k_sgn_finish(StepArgs a, const double* wx, const double* wy,
-                                                   int mode) {
+                                                   int mode, WarmStart ws) {
   const Geo& g = a.g;
   const int tile = tile_of_block(blockIdx.x, gridDim.x);
   const int tx = tile % g.ntx, ty = tile / g.ntx;
@@ -562,6 +585,11 @@ __global__ void __launch_bounds__(NT) k_sgn_finish(StepArgs a, const double* wx,
   const int j = GH + ty * TY + (threadIdx.x >> 6);
   if (i >= g.nx + GH || j >= g.ny + GH) return;
   const int k = j * g.w + i;
+  if (ws.ring64) {  // the solve just counted: its slot from x_{n-1} to d_n = x_n - x_{n-1}
+    double2* e = ws.ring64 + ((ws.sc->solves - 1) % WS_SLOTS) * ws.stride;
+    const double2 x1 = e[k];
+    e[k] = make_double2(wx[k] - x1.x, wy[k] - x1.y);
+  }
   const double h = a.in_h[k];
   const double lh = a.lh[k];
   double lhu = a.lhu[k], lhv = a.lhv[k];
