@@ -49,16 +49,17 @@ BYTES_PER_CELL = {
         # one kernel per iteration: read r, Au, Ap, 1/diag, p (pairs), c, h; write p, Ap, r, Au
         # (pairs); the FP32 correction w (a pair) read and written every other iteration
         "cg_iter": 4 * (2 * 5 + 1 + 1 + 2 * 4) + 4 * 2,
-        # init (read x0 FP64, w, c, h, b; write r, 1/diag, p, s) and start (read r, 1/diag,
-        # c, h; write Au), the two launches of the family averaged
-        "cg_init": (8 * 2 + 4 * (2 + 1 + 1 + 2) + 4 * 8 + 4 * (2 + 2 + 1 + 1) + 4 * 2) // 2,
-        # read h, hu, hv, phi, lhu, lhv, lh (FP64); write c, b; warm start: read x (FP64) and
-        # six past steps, write x0 (FP64), w = 0 and the step
-        "sgn_setup": 4 * (3 + 1 + 2) + 8 + 4 * 3 + (8 * 2 + 4 * 2 * 6) + (8 * 2 + 4 * 2 + 4 * 2),
-        "flux": 4 * (2 + 2 + 1 + 2) + 8 + 8,  # read h pair, hu, hv, phi; b FP64; write lhu, lhv; lh FP64
-        # pair, s1, w, rates; l1h, lh, b; write pair; the solution x (FP64) and its step read
-        # and written
-        "sgn_finish": 4 * (6 + 3 + 2 + 2 + 6) + 8 * 3 + 2 * (8 * 2 + 4 * 2),
+        # init (read x0 and b FP64, w, c, h; write r, 1/diag, p, s) and start (read r,
+        # 1/diag, c, h; write Au), the two launches of the family averaged
+        "cg_init": (8 * 4 + 4 * (2 + 1 + 1) + 4 * 8 + 4 * (2 + 2 + 1 + 1) + 4 * 2) // 2,
+        # read h, hu, hv, phi, lhu and lhv (FP64), lh (FP64); write c, b (FP64); warm start:
+        # read x (FP64) and six past steps, write x0 (FP64), w = 0 and the step
+        "sgn_setup": 4 * (3 + 1) + 8 * 3 + 4 + 8 * 2 + (8 * 2 + 4 * 2 * 6) + (8 * 2 + 4 * 2 + 4 * 2),
+        # read h pair, hu, hv, phi, b (FP64); write lh, lhu, lhv (FP64)
+        "flux": 4 * (2 + 2 + 1) + 8 + 8 * 3,
+        # pair, s1, w, stage-1 momentum rates; l1h, lh, lhu, lhv, b (FP64); write pair; the
+        # solution x (FP64) and its step read and written
+        "sgn_finish": 4 * (6 + 4 + 2 + 2 + 6) + 8 * 5 + 2 * (8 * 2 + 4 * 2),
         "switch": 4 * 3 + 8,
     },
 }

@@ -151,11 +151,13 @@ struct wc_solver {
   // the FP32 chosen form
   float *f_sh = nullptr, *f_shu = nullptr, *f_shv = nullptr, *f_slh = nullptr, *f_slhu = nullptr,
         *f_slhv = nullptr;
-  float *f_s1h = nullptr, *f_s1hl = nullptr, *f_s1hu = nullptr, *f_s1hv = nullptr;
+  float *f_s1h = nullptr, *f_s1hl = nullptr, *f_s1hu = nullptr, *f_s1hv = nullptr,
+        *f_s1hul = nullptr, *f_s1hvl = nullptr;
   double *f_l1h = nullptr, *f_lh = nullptr;  // mass rates stay FP64
   float *f_l1hu = nullptr, *f_l1hv = nullptr;
-  float *f_lhu = nullptr, *f_lhv = nullptr, *f_phi = nullptr;
-  float *f_c = nullptr, *f_bx = nullptr, *f_by = nullptr, *f_wx = nullptr, *f_wy = nullptr,
+  double *f_lhu = nullptr, *f_lhv = nullptr, *f_bx = nullptr, *f_by = nullptr;  // FP64
+  float *f_phi = nullptr;
+  float *f_c = nullptr, *f_wx = nullptr, *f_wy = nullptr,
         *f_rx = nullptr, *f_ry = nullptr, *f_px = nullptr, *f_py = nullptr, *f_qx = nullptr,
         *f_qy = nullptr, *f_zx = nullptr, *f_zy = nullptr, *f_ax = nullptr, *f_ay = nullptr;
   // the one-kernel PCG's own vectors (wc_pcg1.hip), interleaved (x, y)
@@ -802,11 +804,12 @@ int alloc_dispersion(wc_solver* s) {
 int alloc_fp32(wc_solver* s) {
   if (s->f_sh) return WC_OK;
   const size_t n = s->cells;
-  TRY(dalloc(s, &s->f_l1h, n));
-  TRY(dalloc(s, &s->f_lh, n));
+  for (double** p : {&s->f_l1h, &s->f_lh, &s->f_lhu, &s->f_lhv, &s->f_bx, &s->f_by})
+    TRY(dalloc(s, p, n));
   for (float** p : {&s->f_sh, &s->f_shu, &s->f_shv, &s->f_slh, &s->f_slhu, &s->f_slhv, &s->f_s1h,
-                    &s->f_s1hl, &s->f_s1hu, &s->f_s1hv, &s->f_l1hu, &s->f_l1hv,
-                    &s->f_lhu, &s->f_lhv, &s->f_phi, &s->f_c, &s->f_bx, &s->f_by, &s->f_wx,
+                    &s->f_s1hl, &s->f_s1hu, &s->f_s1hv, &s->f_s1hul, &s->f_s1hvl,
+                    &s->f_l1hu, &s->f_l1hv,
+                    &s->f_phi, &s->f_c, &s->f_wx,
                     &s->f_wy, &s->f_rx, &s->f_ry, &s->f_px, &s->f_py, &s->f_qx, &s->f_qy,
                     &s->f_zx, &s->f_zy, &s->f_ax, &s->f_ay})
     TRY(dalloc(s, p, n));
@@ -926,6 +929,9 @@ Sgn32 sgn32_args(wc_solver* s, int stage) {
   a.h = stage == 0 ? s->f_sh : s->f_s1h;
   a.hu = stage == 0 ? s->f_shu : s->f_s1hu;
   a.hv = stage == 0 ? s->f_shv : s->f_s1hv;
+  a.hl = stage == 0 ? s->f_slh : s->f_s1hl;
+  a.hul = stage == 0 ? s->f_slhu : s->f_s1hul;
+  a.hvl = stage == 0 ? s->f_slhv : s->f_s1hvl;
   a.phi = s->f_phi;
   a.lh = s->f_lh;
   a.lhu = s->f_lhu;
@@ -1214,6 +1220,8 @@ BFlux flux_args32(wc_solver* s, int stage) {
   f.h_lo = stage == 0 ? s->f_slh : s->f_s1hl;
   f.hu = stage == 0 ? s->f_shu : s->f_s1hu;
   f.hv = stage == 0 ? s->f_shv : s->f_s1hv;
+  f.hu_lo = stage == 0 ? s->f_slhu : s->f_s1hul;
+  f.hv_lo = stage == 0 ? s->f_slhv : s->f_s1hvl;
   f.b = s->b;
   f.phi = s->f_phi;
   f.has_drive = s->has_drive ? 1 : 0;
@@ -1245,6 +1253,8 @@ Stage32 stage_args32(wc_solver* s, double dt, int stage) {
   a.s1hl = s->f_s1hl;
   a.s1hu = s->f_s1hu;
   a.s1hv = s->f_s1hv;
+  a.s1hul = s->f_s1hul;
+  a.s1hvl = s->f_s1hvl;
   a.l1h = s->f_l1h;
   a.l1hu = s->f_l1hu;
   a.l1hv = s->f_l1hv;
@@ -1286,13 +1296,13 @@ int do_step32(wc_solver* s, double dt) {
   const Geo& g = s->geo;
   TRY(upload_step_params(s, t, dt));
   TRY(launch(s, F_FILL, k_fill32, over(6LL * g.w + 6LL * g.ny), dim3(NT), g, s->f_sh, s->f_slh,
-             s->f_shu, s->f_shv));
+             s->f_shu, s->f_shv, s->f_slhu, s->f_slhv));
   if (s->order == WC_ORDER_FIRST) {
     TRY(stage32(s, dt, 0, MODE_EULER));
   } else {
     TRY(stage32(s, dt, 0, MODE_S1));
     TRY(launch(s, F_FILL, k_fill32, over(6LL * g.w + 6LL * g.ny), dim3(NT), g, s->f_s1h,
-               s->f_s1hl, s->f_s1hu, s->f_s1hv));
+               s->f_s1hl, s->f_s1hu, s->f_s1hv, s->f_s1hul, s->f_s1hvl));
     TRY(stage32(s, dt, 1, MODE_FINAL));
   }
   if (s->dispersive && s->eddy_on) {  // the fronts and their viscosity run in FP64
@@ -2867,6 +2877,8 @@ int batch_build(wc_batch* b) {
       f.hl = st == 0 ? s->f_slh : s->f_s1hl;
       f.hu = st == 0 ? s->f_shu : s->f_s1hu;
       f.hv = st == 0 ? s->f_shv : s->f_s1hv;
+      f.hul = st == 0 ? s->f_slhu : s->f_s1hul;
+      f.hvl = st == 0 ? s->f_slhv : s->f_s1hvl;
       fill[st].push_back(f);
       sw[st].push_back(switch_args32(s, st));
       flux[st].push_back(flux_args32(s, st));

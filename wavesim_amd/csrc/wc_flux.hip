@@ -148,8 +148,13 @@ __device__ __forceinline__ void flux_tile(const FluxTileArgs<S>& a, int mode, in
       hd = (double)a.h[k];
       if (a.h_lo) hd = hd + (double)a.h_lo[k];
       bb = a.b[k];
-      u = wc_velocity(hd, (double)a.hu[k]);
-      v = wc_velocity(hd, (double)a.hv[k]);
+      double mu = (double)a.hu[k], mv = (double)a.hv[k];
+      if (a.hu_lo) {
+        mu = mu + (double)a.hu_lo[k];
+        mv = mv + (double)a.hv_lo[k];
+      }
+      u = wc_velocity(hd, mu);
+      v = wc_velocity(hd, mv);
       ph = (R)a.phi[k];
     }
     L.u.box.eta[ly][lx] = (R)(hd + bb);
@@ -232,11 +237,13 @@ __global__ void __launch_bounds__(NT) k_flux_tile64(FluxTileArgs<double> f, Step
   stage_out(a, mode, i, j, j * a.g.w + i, rh, rhu, rhv);
 }
 
-// FP32 chosen form: the rates for the stage update that follows; the mass rate stays FP64 so
-// the volume is conserved to FP64 rounding, the momentum rates are rounded to FP32.
+// FP32 chosen form: the rates for the stage update that follows, in FP64 (the mass rate so
+// that the volume is conserved to FP64 rounding; the momentum rates because their rounding
+// would reach the SGN solves' right-hand side, and through the extrapolated starts cost
+// iterations).
 template <typename R>
-__device__ __forceinline__ void flux32_body(const FluxTileArgs<float>& f, double* lh, float* lhu,
-                                            float* lhv, int blk, int nblk) {
+__device__ __forceinline__ void flux32_body(const FluxTileArgs<float>& f, double* lh, double* lhu,
+                                            double* lhv, int blk, int nblk) {
   double rh, rhu, rhv;
   int i, j;
   bool valid;
@@ -244,18 +251,18 @@ __device__ __forceinline__ void flux32_body(const FluxTileArgs<float>& f, double
   if (!valid) return;
   const int k = j * f.g.w + i;
   lh[k] = rh;
-  lhu[k] = (float)rhu;
-  lhv[k] = (float)rhv;
+  lhu[k] = rhu;
+  lhv[k] = rhv;
 }
 
-__global__ void __launch_bounds__(NT) k_flux_tile32(FluxTileArgs<float> f, double* lh, float* lhu,
-                                                    float* lhv) {
+__global__ void __launch_bounds__(NT) k_flux_tile32(FluxTileArgs<float> f, double* lh,
+                                                    double* lhu, double* lhv) {
   flux32_body<float>(f, lh, lhu, lhv, blockIdx.x, gridDim.x);
 }
 
 // The FP32 form's state reconstructed in FP64.
 __global__ void __launch_bounds__(NT) k_flux_tile32r(FluxTileArgs<float> f, double* lh,
-                                                     float* lhu, float* lhv) {
+                                                     double* lhu, double* lhv) {
   flux32_body<double>(f, lh, lhu, lhv, blockIdx.x, gridDim.x);
 }
 

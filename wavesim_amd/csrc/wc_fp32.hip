@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(NT) k_join32(long long n, const float* hi, con
 // Ghost layers of the FP32 state (both parts of h; the high parts of hu, hv, which is all the
 // rates read of them).
 __device__ __forceinline__ void fill32_body(const Geo& g, float* h, float* hl, float* hu,
-                                            float* hv, int blk) {
+                                            float* hv, float* hul, float* hvl, int blk) {
   const int n = 6 * g.w + 6 * g.ny;
   const int t = blk * NT + threadIdx.x;
   if (t >= n) return;
@@ -78,12 +78,18 @@ __device__ __forceinline__ void fill32_body(const Geo& g, float* h, float* hl, f
   const int m = map_y(j, g.ny, g.periodic) * g.w + map_x(i, g.nx);
   h[k] = h[m];
   if (hl) hl[k] = hl[m];
-  hu[k] = is_ghost_x(i, g.nx) ? -hu[m] : hu[m];
-  hv[k] = (!g.periodic && is_ghost_y(j, g.ny)) ? -hv[m] : hv[m];
+  const bool fx = is_ghost_x(i, g.nx), fy = !g.periodic && is_ghost_y(j, g.ny);
+  hu[k] = fx ? -hu[m] : hu[m];
+  hv[k] = fy ? -hv[m] : hv[m];
+  if (hul) {  // the momentum pairs' low parts, flipped with them
+    hul[k] = fx ? -hul[m] : hul[m];
+    hvl[k] = fy ? -hvl[m] : hvl[m];
+  }
 }
 
-__global__ void __launch_bounds__(NT) k_fill32(Geo g, float* h, float* hl, float* hu, float* hv) {
-  fill32_body(g, h, hl, hu, hv, blockIdx.x);
+__global__ void __launch_bounds__(NT) k_fill32(Geo g, float* h, float* hl, float* hu, float* hv,
+                                               float* hul, float* hvl) {
+  fill32_body(g, h, hl, hu, hv, hul, hvl, blockIdx.x);
 }
 
 // The switch (dispersion.rs:120-160) in FP64 from the FP32 pair, stored in FP32.
@@ -180,10 +186,10 @@ __device__ __forceinline__ void sgn_setup32_body(const Sgn32& a, int blk, int nb
     double h = 0.0, um = 0.0, vm = 0.0;
     if (i < g.w && j < g.rows) {
       const int k = j * g.w + i;
-      h = (double)a.h[k];
+      h = (double)a.h[k] + (double)a.hl[k];  // the FP32 pairs
       const double m = h >= WC_H_DISPERSION_MIN ? 1.0 : 0.0;
-      um = m * wc_velocity(h, (double)a.hu[k]);
-      vm = m * wc_velocity(h, (double)a.hv[k]);
+      um = m * wc_velocity(h, (double)a.hu[k] + (double)a.hul[k]);
+      vm = m * wc_velocity(h, (double)a.hv[k] + (double)a.hvl[k]);
     }
     s_h[ly][lx] = h;
     s_um[ly][lx] = um;
@@ -226,19 +232,21 @@ __device__ __forceinline__ void sgn_setup32_body(const Sgn32& a, int blk, int nb
   if (i >= g.nx + GH || j >= g.ny + GH) return;
   const int k = j * g.w + i;
   const double h = s_h[ly + 3][lx + 3];
-  if (!(h >= WC_H_DISPERSION_MIN)) {
-    a.bx[k] = 0.f;
-    a.by[k] = 0.f;
+  // which cells take part is read from the depth's high part, as the PCG kernels read it, so
+  // that a cell the solve masks always starts (and stays) at zero with a zero right-hand side
+  if (!((double)a.h[k] >= WC_H_DISPERSION_MIN)) {
+    a.bx[k] = 0.0;
+    a.by[k] = 0.0;
     solve_start(a, k, false);
     return;
   }
   solve_start(a, k, true);
   const int x1 = lx + 1, y1 = ly + 1;
-  const double u = wc_velocity(h, (double)a.hu[k]);
-  const double v = wc_velocity(h, (double)a.hv[k]);
+  const double u = wc_velocity(h, (double)a.hu[k] + (double)a.hul[k]);
+  const double v = wc_velocity(h, (double)a.hv[k] + (double)a.hvl[k]);
   const double lh = a.lh[k];
-  a.bx[k] = (float)((double)a.lhu[k] - u * lh + (s_f[y1][x1 + 1] - s_f[y1][x1 - 1]) * ix);
-  a.by[k] = (float)((double)a.lhv[k] - v * lh + (s_f[y1 + 1][x1] - s_f[y1 - 1][x1]) * iy);
+  a.bx[k] = a.lhu[k] - u * lh + (s_f[y1][x1 + 1] - s_f[y1][x1 - 1]) * ix;
+  a.by[k] = a.lhv[k] - v * lh + (s_f[y1 + 1][x1] - s_f[y1 - 1][x1]) * iy;
 }
 
 __device__ __forceinline__ void diag32(const Sgn32& a, int k, double h, double& dgx,
@@ -428,8 +436,10 @@ __device__ __forceinline__ void stage32_body(const Stage32& a, double dt, int mo
   // the stage input, as the rates saw it
   const double ih = mode == MODE_FINAL ? (double)a.s1h[k] + (double)a.s1hl[k]
                                        : (double)a.sh[k] + (double)a.sl_h[k];
-  const double ihu = mode == MODE_FINAL ? (double)a.s1hu[k] : (double)a.shu[k];
-  const double ihv = mode == MODE_FINAL ? (double)a.s1hv[k] : (double)a.shv[k];
+  const double ihu = mode == MODE_FINAL ? (double)a.s1hu[k] + (double)a.s1hul[k]
+                                        : (double)a.shu[k] + (double)a.sl_hu[k];
+  const double ihv = mode == MODE_FINAL ? (double)a.s1hv[k] + (double)a.s1hvl[k]
+                                        : (double)a.shv[k] + (double)a.sl_hv[k];
   const double lh = a.lh[k];
   double lhu = (double)a.lhu[k], lhv = (double)a.lhv[k];
   if (a.dispersive) {
@@ -451,8 +461,12 @@ __device__ __forceinline__ void stage32_body(const Stage32& a, double dt, int mo
     split(h, hi, lo);
     a.s1h[k] = hi;
     a.s1hl[k] = lo;
-    a.s1hu[k] = (float)hu;
-    a.s1hv[k] = (float)hv;
+    split(hu, hi, lo);
+    a.s1hu[k] = hi;
+    a.s1hul[k] = lo;
+    split(hv, hi, lo);
+    a.s1hv[k] = hi;
+    a.s1hvl[k] = lo;
     a.l1h[k] = lh;
     a.l1hu[k] = (float)lhu;
     a.l1hv[k] = (float)lhv;
@@ -466,7 +480,8 @@ __device__ __forceinline__ void stage32_body(const Stage32& a, double dt, int mo
     clean_dry(h, hu, hv);
   } else {
     const double s1h = (double)a.s1h[k] + (double)a.s1hl[k];
-    const double s1hu = (double)a.s1hu[k], s1hv = (double)a.s1hv[k];
+    const double s1hu = (double)a.s1hu[k] + (double)a.s1hul[k];
+    const double s1hv = (double)a.s1hv[k] + (double)a.s1hvl[k];
     double h2 = s1h + dt * lh, hu2 = s1hu + dt * lhu, hv2 = s1hv + dt * lhv;
     const bool cleaned = !(sh + dt * a.l1h[k] >= WC_H_DRY) || !(h2 >= WC_H_DRY);
     clean_dry(h2, hu2, hv2);
@@ -567,7 +582,7 @@ __global__ void __launch_bounds__(NT) kb_fill32(const BFill* args, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
     const BFill& a = args[q];
-    fill32_body(a.g, a.h, a.hl, a.hu, a.hv, vb - m.first[q]);
+    fill32_body(a.g, a.h, a.hl, a.hu, a.hv, a.hul, a.hvl, vb - m.first[q]);
   }
 }
 

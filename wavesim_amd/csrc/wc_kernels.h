@@ -87,6 +87,7 @@ struct FluxTileArgs {
   Geo g;
   int order;
   const R *h, *h_lo, *hu, *hv;  // h_lo: low part of the FP32 depth pair, null in FP64
+  const R *hu_lo, *hv_lo;       // low parts of the FP32 momentum pairs, null in FP64
   const double* b;
   const R* phi;
   int has_drive;
@@ -145,7 +146,8 @@ struct SgnArgs {
 template <typename V>
 struct CgV {
   Geo g;
-  const V *h, *c, *bx, *by;
+  const V *h, *c;
+  const double *bx, *by;
   const double *solx, *soly;  // FP32 form: the solve's FP64 start x0 (w its correction); FP64: null
   V *wx, *wy, *rx, *ry, *px, *py, *zx, *zy, *ax, *ay;
   double *acc_init, *acc_pap, *acc_update;  // 3 x 256, 256, 2 x 256 lane sums
@@ -159,7 +161,8 @@ struct CgV {
 // inverse Jacobi diagonal; r, q, s, p and di are interleaved (x, y) pairs.
 struct CgG {
   Geo g;
-  const float *h, *c, *bx, *by;
+  const float *h, *c;
+  const double *bx, *by;
   const double *solx, *soly;  // the solve's start x0 (FP64, see Sgn32)
   float *wx, *wy;             // the correction x - x0 (FP32)
   float2 *p, *di;
@@ -181,7 +184,8 @@ struct CgG {
 // [4] p.Ap, 256 each.
 struct FcgArgs {
   Geo g;
-  const float *h, *c, *bx, *by;
+  const float *h, *c;
+  const double *bx, *by;
   const double *solx, *soly;  // the solve's start x0 (FP64, see Sgn32)
   float *wx, *wy;             // the correction x - x0 (FP32)
   float2 *r, *q, *z, *di;
@@ -249,9 +253,12 @@ struct Sgn32 {
   Geo g;
   double *solx, *soly;
   const float *h, *hu, *hv, *phi;  // stage state (high parts) and the switch
+  const float *hl, *hul, *hvl;     // and its low parts
   const double* lh;                // mass rate (FP64: the faces' fluxes cancel exactly)
-  const float *lhu, *lhv;          // momentum rates
-  float *c, *bx, *by, *wx, *wy, *rx, *ry, *px, *py, *ax, *ay, *zx, *zy;
+  const double *lhu, *lhv;         // momentum rates (FP64: their rounding would reach b)
+  float* c;
+  double *bx, *by;                 // the right-hand side (FP64, as the rates)
+  float *wx, *wy, *rx, *ry, *px, *py, *ax, *ay, *zx, *zy;
   double* part;
   CgScalars* sc;
   int nonlinear;
@@ -265,11 +272,12 @@ struct Stage32 {
   double dt;
   const double* b;
   float *sh, *shu, *shv, *sl_h, *sl_hu, *sl_hv;  // the state as FP32 pairs (hi, lo)
-  float *s1h, *s1hl, *s1hu, *s1hv;               // the stage-1 state (depth as a pair)
+  float *s1h, *s1hl, *s1hu, *s1hv;               // the stage-1 state as FP32 pairs:
+  float *s1hul, *s1hvl;                          // hi s1h, s1hu, s1hv; lo s1hl, s1hul, s1hvl
   double* l1h;                                   // the stage-1 rates (mass in FP64)
   float *l1hu, *l1hv;
   const double* lh;                              // this stage's shallow-water rates
-  const float *lhu, *lhv;
+  const double *lhu, *lhv;
   const float *wx, *wy;                          // the SGN solution's FP32 correction
   double *solx, *soly;                           // its FP64 start, then the solution
   int dispersive;
@@ -279,7 +287,8 @@ struct Stage32 {
 
 __global__ void k_split32(long long n, const double* a, float* hi, float* lo);
 __global__ void k_join32(long long n, const float* hi, const float* lo, double* a);
-__global__ void k_fill32(Geo g, float* h, float* hl, float* hu, float* hv);
+__global__ void k_fill32(Geo g, float* h, float* hl, float* hu, float* hv, float* hul,
+                         float* hvl);
 __global__ void k_switch32(Geo g, const float* h, const float* hl, const double* b, Breaking br,
                            const uint8_t* held, float* phi);
 __global__ void k_sgn_setup32(Sgn32 a);
@@ -313,8 +322,8 @@ __global__ void k_cg_update2(SgnArgs a, const double* pcx, const double* pcy);
 __global__ void k_sgn_finish(StepArgs a, const double* wx, const double* wy, int mode,
                              WarmStart ws);
 __global__ void k_flux_tile64(FluxTileArgs<double> f, StepArgs a, int mode);
-__global__ void k_flux_tile32(FluxTileArgs<float> f, double* lh, float* lhu, float* lhv);
-__global__ void k_flux_tile32r(FluxTileArgs<float> f, double* lh, float* lhu, float* lhv);
+__global__ void k_flux_tile32(FluxTileArgs<float> f, double* lh, double* lhu, double* lhv);
+__global__ void k_flux_tile32r(FluxTileArgs<float> f, double* lh, double* lhu, double* lhv);
 __global__ void k_max_rate(Geo g, const double* h, const double* hu, const double* hv,
                            unsigned long long* out);
 __global__ void k_limit_scan(Geo g, const double* h, const double* hu, const double* hv,
@@ -401,7 +410,7 @@ __device__ __forceinline__ int batch_instance(const BMap& m, int vb) {
 }
 struct BFill {
   Geo g;
-  float *h, *hl, *hu, *hv;
+  float *h, *hl, *hu, *hv, *hul, *hvl;
 };
 struct BSwitch {
   Geo g;
@@ -413,8 +422,7 @@ struct BSwitch {
 };
 struct BFlux {
   FluxTileArgs<float> f;
-  double* lh;
-  float *lhu, *lhv;
+  double *lh, *lhu, *lhv;
   int recon64;  // reconstruct in FP64 (k_flux_tile32r, kb_flux32r) rather than FP32
 };
 struct BRate {
