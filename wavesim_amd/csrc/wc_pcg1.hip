@@ -193,11 +193,12 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
     vx = (double)(float)((double)v[2].x + na * sx) * (double)v[3].x;
     vy = (double)(float)((double)v[2].y + na * sy) * (double)v[3].y;
   };
+  __shared__ float2 s_own[4][BY][BX];  // q_i, s_{i-1}, r_i, D^-1 at the tile's cells
   Prefetch<4> P;
   if (blk < nt) prefetch(g, blk, fetch, P);
   for (int r = 0; blk + L * r < nt; ++r) {
     const int tile = blk + L * r;
-    stage(P, s_u, make);
+    stage(P, s_u, s_own, make);
     __syncthreads();
     if (tile + L < nt) prefetch(g, tile + L, fetch, P);
     stencil_f(g, a.c, tile, s_u, s_f);
@@ -207,7 +208,9 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
     stencil_own(g, a.h, tile, s_u, s_f, own, k, ux, uy, ax, ay);
     double v[3] = {0.0, 0.0, 0.0};
     if (own) {
-      const float2 qq = q0[k], ss = s0[k], rr = r0[k], d = a.di[k], pp = a.p[k];
+      const int ox = threadIdx.x & 63, oy = threadIdx.x >> 6;
+      const float2 qq = s_own[0][oy][ox], ss = s_own[1][oy][ox], rr = s_own[2][oy][ox];
+      const float2 d = s_own[3][oy][ox], pp = a.p[k];
       const double sx = (double)(float)((double)qq.x + beta * (double)ss.x);
       const double sy = (double)(float)((double)qq.y + beta * (double)ss.y);
       const double px = (double)(float)((double)rr.x * (double)d.x + beta * (double)pp.x);

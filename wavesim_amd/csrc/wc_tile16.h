@@ -180,10 +180,12 @@ __device__ __forceinline__ void prefetch(const Geo& g, int tile, Fetch fetch, Pr
   }
 }
 
-// u from the prefetched inputs by make(v, vx, vy), into LDS on the tile + 2.
+// u from the prefetched inputs by make(v, vx, vy), into LDS on the tile + 2; the inputs of the
+// tile's own cells also into `own` (NV x 64x16), where the cell epilogue reads them instead of
+// loading them again (by then L2 may have let them go).
 template <int NV, class Make>
 __device__ __forceinline__ void stage(const Prefetch<NV>& P, double (*s_u)[S2Y][S2X],
-                                      Make make) {
+                                      float2 (*own)[BY][BX], Make make) {
 #pragma unroll
   for (int e = 0; e < 2; ++e) {
     const int t = threadIdx.x + e * PT;
@@ -197,6 +199,9 @@ __device__ __forceinline__ void stage(const Prefetch<NV>& P, double (*s_u)[S2Y][
       }
       s_u[0][ly][lx] = vx;
       s_u[1][ly][lx] = vy;
+      if (lx >= 2 && lx < BX + 2 && ly >= 2 && ly < BY + 2)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) own[q][ly - 2][lx - 2] = P.v[e][q];
     }
   }
 }
