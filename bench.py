@@ -66,8 +66,6 @@ BYTES_PER_CELL = {
 
 # rocprofv3 --pmc summaries of the dominant kernel per workload (traffic per launch)
 TRAFFIC_PROFILES = {
-    "rough_bed_1024x512_fifth_sgn":
-        os.path.join(ROOT, "profiles", "cg_iter_rough_bed_1024x512_pmc.md"),
     "large_reef_4096x2048_fifth_sgn":
         os.path.join(ROOT, "profiles", "cg_iter_large_reef_4096x2048_pmc.md"),
 }
