@@ -127,8 +127,11 @@ def parse():
     p.add_argument("--pcg-lanes", type=int, default=None, choices=[256, 512],
                    help="blocks of the FP32 form's one-kernel PCG launches")
     p.add_argument("--pcg-march", type=int, default=None, choices=[0, 1],
-                   help="one-kernel PCG layout: blocks march down column strips (1, default) or "
-                        "take 64x16 tiles (0)")
+                   help="one-kernel PCG layout: blocks march down column strips (1) or take "
+                        "64x16 tiles (0, default)")
+    p.add_argument("--pcg-walk", type=int, default=None,
+                   help="one-kernel PCG on 64x16 tiles: the fewest tiles a block takes (solver "
+                        "default 1; the sweep defaults to 8)")
     p.add_argument("--recon", type=int, default=None, choices=[32, 64],
                    help="FP32 form: the flux reconstruction's arithmetic (default 32)")
     p.add_argument("--pcg32", type=int, default=None,
@@ -303,9 +306,11 @@ def sweep(args):
     ws = [configs.c3_reef_instance(seed, size=size)
           for seed in range(3000, 3000 + args.instances)]
     solvers = [w.solver(Solver).with_precision(PRECISION_FP32) for w in ws]
-    if args.pcg_lanes:
-        for g in solvers:
+    walk = 8 if args.pcg_walk is None else args.pcg_walk
+    for g in solvers:
+        if args.pcg_lanes:
             g.set_option("pcg_lanes", args.pcg_lanes)
+        g.set_option("pcg_tiles_per_block", walk)
     states = [w.still_water() for w in ws]
     cells = sum(w.grid.nx * w.grid.ny for w in ws)
     b = Batch(solvers)
@@ -338,15 +343,22 @@ def sweep(args):
           f"cell-updates/s, {iters / max(solves, 1):.1f} PCG iterations per solve",
           file=sys.stderr, flush=True)
     alone_steps = max(args.steps // 5, 2)
+    for g in solvers:  # alone, a small grid is fastest with a PCG block per tile
+        g.set_option("pcg_tiles_per_block", 1)
+    st0 = [g.dispersion_stats() for g in solvers]
     torch.cuda.synchronize()
     t = time.perf_counter()
     for g, s in zip(solvers, states):
         g.run_steps(s, alone_steps)
     torch.cuda.synchronize()
     el1 = time.perf_counter() - t
+    st1 = [g.dispersion_stats() for g in solvers]
+    iters1 = sum(b1.iterations - b0.iterations for b0, b1 in zip(st0, st1))
+    solves1 = sum(b1.solves - b0.solves for b0, b1 in zip(st0, st1))
     v1 = cells * alone_steps / el1
     print(f"[bench] sweep one at a time: {alone_steps} steps in {el1:.3f} s = {v1 / 1e6:.1f} M "
-          f"cell-updates/s", file=sys.stderr, flush=True)
+          f"cell-updates/s, {iters1 / max(solves1, 1):.1f} PCG iterations per solve",
+          file=sys.stderr, flush=True)
     sizes = {}
     for w in ws:
         sizes[f"{w.grid.nx}x{w.grid.ny}"] = sizes.get(f"{w.grid.nx}x{w.grid.ny}", 0) + 1
@@ -360,9 +372,11 @@ def sweep(args):
                    "instances": len(ws),
                    "cells": cells, "sizes": sizes, "spinup_steps": args.spinup,
                    "path": "FP32 form, one launch per phase over all instances",
+                   "pcg_tiles_per_block": walk,
                    "launch_widths": list(b.launch_widths())},
         "pcg_iters_per_solve": iters / max(solves, 1),
-        "one_at_a_time": {"value": v1, "steps": alone_steps, "batched_over_alone": value / v1},
+        "one_at_a_time": {"value": v1, "steps": alone_steps, "batched_over_alone": value / v1,
+                          "pcg_iters_per_solve": iters1 / max(solves1, 1)},
         "roofline": None, "cpu_baseline": None}))
 
 

@@ -138,6 +138,9 @@ int wc_solver_bed(const wc_solver* s, double* out);
  *   "pcg_lanes"   blocks of the one-kernel PCG: 256 (default) or 512;
  *   "pcg_march"   one-kernel PCG layout: 0 = 64x16 tiles (default), 1 = blocks marching down
  *                 64-column strips with circular LDS row buffers;
+ *   "pcg_tiles_per_block" 64x16 tiles: the fewest tiles one block takes, 1..64 (default 1;
+ *                 larger values give a small grid fewer, longer blocks, which pays when many
+ *                 instances share a launch);
  *   "pcg_batch"   PCG iterations queued between looks at the convergence flag (default 4);
  *   "flux_kernel" 1 = one thread per cell, as the reference's loop; 2 = LDS tiles, each cell
  *                 reconstructed and each face fluxed once (default);
@@ -322,8 +325,9 @@ int wc_sampler_read(const wc_sampler* sm, int32_t slot, double* out, double* tim
  * finite and positive stops there (done_steps[q], last_dt[q] per instance, both may be NULL).
  * While a batch steps its solvers, they must not be used on their own; afterwards their states
  * are read with wc_state_get as usual. The first run tunes the launch widths (persistent grids
- * of the 256- and 1024-thread launches) by timing candidates; wc_batch_launch_widths reports the
- * choice, wc_batch_set_option("grid_nt" | "grid_pt", w) fixes it, ("tune", 1) tunes again,
+ * of the flux and the PCG launches; the other launches run a block per virtual block) by timing
+ * candidates; wc_batch_launch_widths reports the choice, wc_batch_set_option("grid_nt" |
+ * "grid_pt", w) fixes it ("grid_nt" then for every 256-thread launch), ("tune", 1) tunes again,
  * ("pcg_batch", k) sets the iterations queued between looks at the PCG flags. */
 typedef struct wc_batch wc_batch;
 int wc_batch_new(wc_solver* const* solvers, uint64_t n, wc_batch** out);

@@ -30,8 +30,10 @@ def workloads():
     return out
 
 
-def fresh(w, arrays=None, t0=0.0):
+def fresh(w, arrays=None, t0=0.0, walk=1):
     g = w.solver(Solver).with_precision(PRECISION_FP32)
+    if walk != 1:
+        g.set_option("pcg_tiles_per_block", walk)
     s = w.still_water() if arrays is None else State(*(a.copy() for a in arrays), t0)
     return g, s
 
@@ -75,14 +77,15 @@ def assert_same(alone, batched):
         assert a[3] == b[3], k
 
 
-@pytest.mark.parametrize("graph", [True, False])
-def test_a_batch_steps_each_instance_as_it_steps_alone(graph):
+@pytest.mark.parametrize("graph,walk", [(True, 1), (False, 1), (True, 4)])
+def test_a_batch_steps_each_instance_as_it_steps_alone(graph, walk):
     """With the fixed stretches of each step replayed from HIP graphs (the default) and
-    without; compared after every 5 steps, so that every replay between two looks counts."""
+    without, and with PCG blocks taking 4 tiles or more; compared after every 5 steps, so that
+    every replay between two looks counts."""
     ws = workloads()
     starts = spun_up(ws, 120)
-    pairs = [fresh(w, arr, t0) for w, (arr, t0) in zip(ws, starts)]
-    solo = [fresh(w, arr, t0) for w, (arr, t0) in zip(ws, starts)]
+    pairs = [fresh(w, arr, t0, walk) for w, (arr, t0) in zip(ws, starts)]
+    solo = [fresh(w, arr, t0, walk) for w, (arr, t0) in zip(ws, starts)]
     b = Batch([g for g, _ in pairs])
     b.set_option("graph", int(graph))
     for chunk in range(5):

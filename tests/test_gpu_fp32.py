@@ -26,12 +26,16 @@ pytestmark = pytest.mark.gpu
 
 def fp32(grid, bed, pcg=None):
     """pcg: 3 (two kernels per iteration), 4 (one, 256 blocks marching down column strips),
-    512 (one, 512 blocks marching), "tiles" (one, 256 blocks over 64x16 tiles)."""
+    512 (one, 512 blocks marching), "tiles" (one, 256 blocks over 64x16 tiles), "walk8" (one,
+    blocks over 64x16 tiles taking 8 tiles each or more)."""
     s = Solver(grid, bed).with_precision(PRECISION_FP32)
     if pcg == 512:
         return s.set_option("pcg_kernels", 4).set_option("pcg_lanes", 512)
     if pcg == "tiles":
         return s.set_option("pcg_kernels", 4).set_option("pcg_march", 0)
+    if pcg == "walk8":
+        return (s.set_option("pcg_kernels", 4).set_option("pcg_march", 0)
+                .set_option("pcg_tiles_per_block", 8))
     return s.set_option("pcg_kernels", pcg) if pcg else s
 
 
@@ -69,7 +73,7 @@ def run_pair(w, steps, t0=0.0, state=None, pcg=None):
     return g64, g32, s64, s32
 
 
-@pytest.mark.parametrize("pcg", [3, 4, 512, "tiles"])
+@pytest.mark.parametrize("pcg", [3, 4, 512, "tiles", "walk8"])
 def test_fp32_rough_bed_matches_fp64_within_its_bounds(pcg):
     w = configs.c1_rough_bed(nx=256, ny=128)
     g64, g32, s64, s32 = run_pair(w, 600, t0=40.0, pcg=pcg)

@@ -191,6 +191,7 @@ struct wc_solver {
   int pcg_march = 0;        // one-kernel PCG: blocks march down column strips (1) or take
                             // 64x16 tiles (0)
   int recon64 = 1;          // FP32 form: reconstruct the FP32 state in FP64 (1) or FP32 (0)
+  int pcg_walk = 1;         // one-kernel PCG on 64x16 tiles: the fewest tiles a block takes
   int flux_kernel = 2;
   unsigned long long pcg_batch = 4;
   uint8_t* held = nullptr;
@@ -960,6 +961,14 @@ Sgn32 sgn32_args(wc_solver* s, int stage) {
   return a;
 }
 
+// The one-kernel PCG on 64x16 tiles: the blocks that take tiles, at most its lanes and few
+// enough that each takes pcg_walk tiles (block b takes b, b + stride, ...). The lanes past them
+// read zero.
+int tile_stride(const wc_solver* s) {
+  const int nt = ((s->geo.nx + 63) / 64) * ((s->geo.ny + 15) / 16);
+  return std::max(1, std::min(s->pcg_lanes, (nt + s->pcg_walk - 1) / s->pcg_walk));
+}
+
 // The one-kernel PCG (wc_pcg1.hip), FP32 form: start kernels, then one kernel per iteration in
 // batches, a one-wave check closing every batch.
 CgG cg1_args(wc_solver* s, int stage) {
@@ -987,6 +996,7 @@ CgG cg1_args(wc_solver* s, int stage) {
   a.tolerance = s->disp.tolerance;
   a.max_iterations = s->disp.max_iterations;
   a.defer = s->capture ? 0 : 1;  // captured iterates must hold every step
+  a.stride = tile_stride(s);
   return a;
 }
 
@@ -1603,6 +1613,14 @@ int wc_solver_set_option(wc_solver* s, const char* key, int64_t value) {
   if (strcmp(key, "pcg_kernels") == 0) {
     if (value < 1 || value > 4) return fail(WC_EINVAL, "pcg_kernels must be 1, 2, 3 or 4");
     s->pcg_kernels = (int)value;
+    return WC_OK;
+  }
+  if (strcmp(key, "pcg_tiles_per_block") == 0) {
+    if (value < 1 || value > 64) return fail(WC_EINVAL, "pcg_tiles_per_block must be in [1, 64]");
+    TRY(set_device(s));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    s->pcg_walk = (int)value;
+    HIP_OK(hipMemsetAsync(s->lanes, 0, 8 * 512 * sizeof(double), s->stream));
     return WC_OK;
   }
   if (strcmp(key, "pcg_lanes") == 0) {
@@ -2747,7 +2765,9 @@ struct wc_batch {
   bool march = true;  // and its layout (pcg_march)
   bool recon64 = false;  // the instances' flux reconstruction in FP64
   unsigned long long maxit = 0, pcg_batch = 4;
-  int grid_nt = 2048, grid_pt = 512;  // persistent widths of the 256- and 1024-thread launches
+  int grid_nt = 2048, grid_pt = 512;  // persistent widths of the flux and the PCG launches
+  int grid_st = 1 << 30;  // and of the other 256-thread launches: a block per virtual block
+                          // (their blocks are short and few registers deep), unless set by hand
   bool tuned = false;
   std::vector<std::pair<int, double>> tune_nt, tune_pt;  // (width, us per launch) as measured
 
@@ -2826,8 +2846,7 @@ int pcg_blocks(const wc_solver* s) {
     const int segs = std::min(strips >= L ? 1 : L / strips, s->geo.ny);
     return std::min(strips * segs, L);
   }
-  const int nt = ((s->geo.nx + 63) / 64) * ((s->geo.ny + 15) / 16);
-  return std::min(nt, L);
+  return tile_stride(s);
 }
 
 // (Re)builds every argument block and launch map for the active instances, in one upload.
@@ -2964,18 +2983,18 @@ dim3 bgrid(int total, int width) { return dim3((unsigned)std::max(1, std::min(to
 
 // The launches of one stretch of a step, in order (captured into a graph or issued directly).
 void enq_fill(wc_batch* b, int st) {
-  hipLaunchKernelGGL(kb_fill32, bgrid(b->m_fill.total, b->grid_nt), dim3(NT), 0, b->stream,
+  hipLaunchKernelGGL(kb_fill32, bgrid(b->m_fill.total, b->grid_st), dim3(NT), 0, b->stream,
                      b->fill[st], b->m_fill);
 }
 
 void enq_stage_head(wc_batch* b, int st) {  // the switch, the rates, the SGN set-up
   if (b->sw)
-    hipLaunchKernelGGL(kb_switch32, bgrid(b->m_sw.total, b->grid_nt), dim3(NT), 0, b->stream,
+    hipLaunchKernelGGL(kb_switch32, bgrid(b->m_sw.total, b->grid_st), dim3(NT), 0, b->stream,
                        b->swa[st], b->m_sw);
   hipLaunchKernelGGL(b->recon64 ? kb_flux32r : kb_flux32, bgrid(b->m_tiles.total, b->grid_nt),
                      dim3(NT), 0, b->stream, b->flux[st], b->m_tiles);
   if (b->dispersive)
-    hipLaunchKernelGGL(kb_sgn_setup32, bgrid(b->m_tiles.total, b->grid_nt), dim3(NT), 0,
+    hipLaunchKernelGGL(kb_sgn_setup32, bgrid(b->m_tiles.total, b->grid_st), dim3(NT), 0,
                        b->stream, b->sgn[st], b->m_tiles);
 }
 
@@ -3028,22 +3047,21 @@ void enq_check(wc_batch* b, int st, unsigned long long it, int stats) {
 }
 
 void enq_update(wc_batch* b, int st, int mode) {
-  hipLaunchKernelGGL(kb_stage32, bgrid(b->m_tiles.total, b->grid_nt), dim3(NT), 0, b->stream,
+  hipLaunchKernelGGL(kb_stage32, bgrid(b->m_tiles.total, b->grid_st), dim3(NT), 0, b->stream,
                      b->stg[st], (const double*)b->d_stage, mode, b->m_tiles);
 }
 
-// The rest of stage st's solves after iterations 1..k: batches of iterations, one look at all
-// flags per batch, then the solves are counted.
+// The rest of stage st's solves after iterations 1..k: first up to the largest count any
+// instance needed last time, then batches of pcg_batch iterations, one look at all flags per
+// batch; then the solves are counted. An instance that has converged leaves the launches after
+// reading its flag, so queueing past its count changes none of its numbers.
 int batch_pcg_tail(wc_batch* b, int st, unsigned long long k) {
   const int n = (int)b->active.size();
   unsigned long long it = k + 1;
-  unsigned long long batch = k > 0 ? b->pcg_batch : 1;
-  if (k == 0) {
-    unsigned long long first = ~0ull;
-    for (int q = 0; q < n; ++q) first = std::min(first, b->prev_it[st][q]);
-    batch = std::max<unsigned long long>(first, 1);
-  }
-  bool look = k > 0;  // iterations 1..k were queued: look at the flags first
+  unsigned long long last = 0;
+  for (int q = 0; q < n; ++q) last = std::max(last, b->prev_it[st][q]);
+  unsigned long long batch = last > k ? last - k : b->pcg_batch;
+  bool look = k > 0 && last <= k;  // nothing to queue before the first look
   while (true) {
     if (!look) {
       const unsigned long long stop = std::min(it + batch, b->maxit + 1);
@@ -3061,7 +3079,7 @@ int batch_pcg_tail(wc_batch* b, int st, unsigned long long k) {
     if (all || it > b->maxit) break;
   }
   enq_check(b, st, it, 1);
-  hipLaunchKernelGGL(kb_cg1_flush, bgrid(b->m_tiles.total, b->grid_nt), dim3(NT), 0, b->stream,
+  hipLaunchKernelGGL(kb_cg1_flush, bgrid(b->m_tiles.total, b->grid_st), dim3(NT), 0, b->stream,
                      b->cg[st], b->m_tiles);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(b->h_it, b->d_it, n * sizeof(unsigned long long), hipMemcpyDeviceToHost,
@@ -3299,11 +3317,15 @@ int wc_batch_set_option(wc_batch* b, const char* key, int64_t value) {
     HIP_OK(hipStreamSynchronize(b->stream));
     drop_graphs(b);  // the widths are part of the captured launches
     (k == "grid_nt" ? b->grid_nt : b->grid_pt) = (int)std::min<int64_t>(value, 1 << 30);
+    if (k == "grid_nt") b->grid_st = b->grid_nt;
     b->tuned = true;  // set by hand: no tuning
     return WC_OK;
   }
   if (k == "tune") {
     b->tuned = value == 0;
+    if (value) b->grid_st = 1 << 30;
+    HIP_OK(hipStreamSynchronize(b->stream));
+    drop_graphs(b);
     return WC_OK;
   }
   if (k == "pcg_batch") {
@@ -3341,7 +3363,7 @@ int wc_batch_run_steps(wc_batch* b, uint64_t steps, uint64_t* done_steps, double
     int n = (int)b->active.size();
     // stable_dt of every instance (solver.rs:434-450), one read
     HIP_OK(hipMemsetAsync(b->d_red, 0, n * sizeof(unsigned long long), b->stream));
-    hipLaunchKernelGGL(kb_max_rate32, bgrid(b->m_rate.total, b->grid_nt), dim3(NT), 0, b->stream,
+    hipLaunchKernelGGL(kb_max_rate32, bgrid(b->m_rate.total, b->grid_st), dim3(NT), 0, b->stream,
                        b->rate, b->m_rate);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(b->h_red, b->d_red, n * sizeof(unsigned long long),

@@ -178,6 +178,7 @@ struct CgG {
   // + alpha_i p_i, p_{i-1} being read there anyway), which halves the traffic of w; a solve
   // that ends after an odd number of iterations adds its last step in k_cg1_flush.
   int defer;
+  int stride;  // 64x16-tile layout: the launch's blocks; block b takes tiles b, b + stride, ...
 };
 // The flexible PCG with the learned preconditioner (wc_fpcg.hip), FP32 form. Vectors are
 // interleaved (x, y) pairs in grid layout; lanes: [0] b.b, [1] r.r, [2] r.z, [3] r.z_old,

@@ -84,12 +84,12 @@ __device__ __forceinline__ float2 f2(double x, double y) { return make_float2((f
 template <int L>
 __device__ __forceinline__ void cg1_init_body(const CgG& a, int blk) {
   const Geo& g = a.g;
-  const int nt = tiles_all(g);
+  const int nt = blk < a.stride ? tiles_all(g) : 0;  // blocks past the stride: zero lanes
   __shared__ double s_u[2][S2Y][S2X], s_f[S1Y][S1X];
   __shared__ double sums[2][BY][1];
   double acc[1] = {0.0};
-  for (int r = 0; blk + L * r < nt; ++r) {
-    const int tile = blk + L * r;
+  for (int r = 0; blk + a.stride * r < nt; ++r) {
+    const int tile = blk + a.stride * r;
     bool own;
     int k;
     double ux, uy, ax, ay;
@@ -128,12 +128,12 @@ __global__ void __launch_bounds__(PT) k_cg1_init(CgG a) { cg1_init_body<L>(a, bl
 template <int L>
 __device__ __forceinline__ void cg1_start_body(const CgG& a, int blk) {
   const Geo& g = a.g;
-  const int nt = tiles_all(g);
+  const int nt = blk < a.stride ? tiles_all(g) : 0;  // blocks past the stride: zero lanes
   __shared__ double s_u[2][S2Y][S2X], s_f[S1Y][S1X];
   __shared__ double sums[2][BY][3];
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int r = 0; blk + L * r < nt; ++r) {
-    const int tile = blk + L * r;
+  for (int r = 0; blk + a.stride * r < nt; ++r) {
+    const int tile = blk + a.stride * r;
     bool own;
     int k;
     double ux, uy, ax, ay;
@@ -168,7 +168,7 @@ __global__ void __launch_bounds__(PT) k_cg1_start(CgG a) { cg1_start_body<L>(a, 
 template <int L>
 __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n, int blk) {
   const Geo& g = a.g;
-  const int nt = tiles_all(g);
+  const int nt = blk < a.stride ? tiles_all(g) : 0;  // blocks past the stride: zero lanes
   double alpha = 0.0, beta = 0.0, alpha_prev = 0.0;
   if (blk < nt && !start<L>(a, n, blk, alpha, beta, alpha_prev)) return;
   const unsigned long long i = n - 1;
@@ -196,11 +196,11 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
   __shared__ float2 s_own[4][BY][BX];  // q_i, s_{i-1}, r_i, D^-1 at the tile's cells
   Prefetch<4> P;
   if (blk < nt) prefetch(g, blk, fetch, P);
-  for (int r = 0; blk + L * r < nt; ++r) {
-    const int tile = blk + L * r;
+  for (int r = 0; blk + a.stride * r < nt; ++r) {
+    const int tile = blk + a.stride * r;
     stage(P, s_u, s_own, make);
     __syncthreads();
-    if (tile + L < nt) prefetch(g, tile + L, fetch, P);
+    if (tile + a.stride < nt) prefetch(g, tile + a.stride, fetch, P);
     stencil_f(g, a.c, tile, s_u, s_f);
     bool own;
     int k;
