@@ -268,7 +268,7 @@ __global__ void __launch_bounds__(NT) k_flux_tile32r(FluxTileArgs<float> f, doub
 
 // Many instances in one launch (see kb_fill32).
 template <typename R>
-__device__ __forceinline__ void kb_flux32_body(const BFlux* args, BMap m) {
+__device__ __forceinline__ void kb_flux32_body(const BFlux* __restrict__ args, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
     const BFlux& a = args[q];
@@ -277,11 +277,11 @@ __device__ __forceinline__ void kb_flux32_body(const BFlux* args, BMap m) {
   }
 }
 
-__global__ void __launch_bounds__(NT) kb_flux32(const BFlux* args, BMap m) {
+__global__ void __launch_bounds__(NT) kb_flux32(const BFlux* __restrict__ args, BMap m) {
   kb_flux32_body<float>(args, m);
 }
 
-__global__ void __launch_bounds__(NT) kb_flux32r(const BFlux* args, BMap m) {
+__global__ void __launch_bounds__(NT) kb_flux32r(const BFlux* __restrict__ args, BMap m) {
   kb_flux32_body<double>(args, m);
 }
 

@@ -578,7 +578,7 @@ __global__ void __launch_bounds__(NT) k_cg_reduce32(Sgn32 a, int kind) {
 // The grid walks every instance's virtual blocks, gridDim.x at a time; a virtual block runs the
 // single-instance body with its instance's arguments, so each instance computes exactly what
 // its own launch would.
-__global__ void __launch_bounds__(NT) kb_fill32(const BFill* args, BMap m) {
+__global__ void __launch_bounds__(NT) kb_fill32(const BFill* __restrict__ args, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
     const BFill& a = args[q];
@@ -586,7 +586,7 @@ __global__ void __launch_bounds__(NT) kb_fill32(const BFill* args, BMap m) {
   }
 }
 
-__global__ void __launch_bounds__(NT) kb_switch32(const BSwitch* args, BMap m) {
+__global__ void __launch_bounds__(NT) kb_switch32(const BSwitch* __restrict__ args, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
     const BSwitch& a = args[q];
@@ -594,7 +594,7 @@ __global__ void __launch_bounds__(NT) kb_switch32(const BSwitch* args, BMap m) {
   }
 }
 
-__global__ void __launch_bounds__(NT) kb_sgn_setup32(const Sgn32* args, BMap m) {
+__global__ void __launch_bounds__(NT) kb_sgn_setup32(const Sgn32* __restrict__ args, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
     sgn_setup32_body(args[q], vb - m.first[q], m.first[q + 1] - m.first[q]);
@@ -602,7 +602,7 @@ __global__ void __launch_bounds__(NT) kb_sgn_setup32(const Sgn32* args, BMap m) 
   }
 }
 
-__global__ void __launch_bounds__(NT) kb_stage32(const Stage32* args, const double* dt, int mode,
+__global__ void __launch_bounds__(NT) kb_stage32(const Stage32* __restrict__ args, const double* dt, int mode,
                                                  BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
@@ -610,7 +610,7 @@ __global__ void __launch_bounds__(NT) kb_stage32(const Stage32* args, const doub
   }
 }
 
-__global__ void __launch_bounds__(NT) kb_max_rate32(const BRate* args, BMap m) {
+__global__ void __launch_bounds__(NT) kb_max_rate32(const BRate* __restrict__ args, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
     const BRate& a = args[q];

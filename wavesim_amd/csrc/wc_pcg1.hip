@@ -271,7 +271,7 @@ __device__ __forceinline__ void cg1_flush_body(const CgG& a, int tile) {
 
 __global__ void __launch_bounds__(NT) k_cg1_flush(CgG a) { cg1_flush_body(a, blockIdx.x); }
 
-__global__ void __launch_bounds__(NT) kb_cg1_flush(const CgG* args, BMap m) {
+__global__ void __launch_bounds__(NT) kb_cg1_flush(const CgG* __restrict__ args, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
     cg1_flush_body(args[q], vb - m.first[q]);
@@ -413,7 +413,7 @@ __global__ void __launch_bounds__(PT) k_cg1m_iter(CgG a, unsigned long long n) {
 // Instance q runs min(256, its tiles) virtual blocks; its lanes past those stay zero, as the
 // single launch's empty blocks write them, so the sums and every iterate are the same.
 template <int L>
-__global__ void __launch_bounds__(PT) kb_cg1_init(const CgG* args, BMap m) {
+__global__ void __launch_bounds__(PT) kb_cg1_init(const CgG* __restrict__ args, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
     cg1_init_body<L>(args[q], vb - m.first[q]);
@@ -422,7 +422,7 @@ __global__ void __launch_bounds__(PT) kb_cg1_init(const CgG* args, BMap m) {
 }
 
 template <int L>
-__global__ void __launch_bounds__(PT) kb_cg1_start(const CgG* args, BMap m) {
+__global__ void __launch_bounds__(PT) kb_cg1_start(const CgG* __restrict__ args, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
     cg1_start_body<L>(args[q], vb - m.first[q]);
@@ -431,7 +431,7 @@ __global__ void __launch_bounds__(PT) kb_cg1_start(const CgG* args, BMap m) {
 }
 
 template <int L>
-__global__ void __launch_bounds__(PT) kb_cg1_iter(const CgG* args, unsigned long long n, BMap m) {
+__global__ void __launch_bounds__(PT) kb_cg1_iter(const CgG* __restrict__ args, unsigned long long n, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
     cg1_iter_body<L>(args[q], n, vb - m.first[q]);
@@ -442,7 +442,7 @@ __global__ void __launch_bounds__(PT) kb_cg1_iter(const CgG* args, unsigned long
 // The marching kernels over many instances: instance q runs min(L, its strip units) virtual
 // blocks (mrch::segments), the same blocks its own launch runs work in.
 template <int L>
-__global__ void __launch_bounds__(PT) kbm_cg1_init(const CgG* args, BMap m) {
+__global__ void __launch_bounds__(PT) kbm_cg1_init(const CgG* __restrict__ args, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
     cg1m_init_body<L>(args[q], vb - m.first[q]);
@@ -451,7 +451,7 @@ __global__ void __launch_bounds__(PT) kbm_cg1_init(const CgG* args, BMap m) {
 }
 
 template <int L>
-__global__ void __launch_bounds__(PT) kbm_cg1_start(const CgG* args, BMap m) {
+__global__ void __launch_bounds__(PT) kbm_cg1_start(const CgG* __restrict__ args, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
     cg1m_start_body<L>(args[q], vb - m.first[q]);
@@ -460,7 +460,7 @@ __global__ void __launch_bounds__(PT) kbm_cg1_start(const CgG* args, BMap m) {
 }
 
 template <int L>
-__global__ void __launch_bounds__(PT) kbm_cg1_iter(const CgG* args, unsigned long long n, BMap m) {
+__global__ void __launch_bounds__(PT) kbm_cg1_iter(const CgG* __restrict__ args, unsigned long long n, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
     cg1m_iter_body<L>(args[q], n, vb - m.first[q]);
@@ -471,7 +471,7 @@ __global__ void __launch_bounds__(PT) kbm_cg1_iter(const CgG* args, unsigned lon
 // One wave per instance: k_cg1_check, then the instance's flag and iteration into done[q],
 // it[q]; with `stats` (the solve is over) DispersionStats counts the solve as k_cg_stats does.
 template <int L>
-__global__ void __launch_bounds__(64) kb_cg1_check(const CgG* args, unsigned long long n,
+__global__ void __launch_bounds__(64) kb_cg1_check(const CgG* __restrict__ args, unsigned long long n,
                                                    int stats, int* done, unsigned long long* it) {
   const int q = blockIdx.x;
   const CgG& a = args[q];
