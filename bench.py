@@ -6,13 +6,19 @@ and the roofline of the dominant kernel, with the CPU FP64 path timed beside it.
 Workload (default): the large single reef, 4096x2048 cells at 3 m (a 40 m shelf shoaling at
 1:50 onto a 1 m reef flat, 8 Gaussian lobes, a 16-mode rough-bed overlay, a dry beach strip;
 wave-maker H 3 m, T 16 s at 15 degrees, sponges W, S, N, Manning 0.025), fifth order, SGN by
-PCG to 1e-8, breaking switch on. `--workload rough_bed` is the 1024x512 dispersive run.
-A step is one full Solver::step (both RK stages, every PCG iteration) with dt = stable_dt,
-driven on the device by wc_run_steps. The timed window starts from a mid-run state: S untimed
-spin-up steps (the wave has crossed the shelf by then), W untimed warm-up steps, then K timed,
-for each path from that same state: the FP64 plain path, the FP32 chosen form, the FP32 form
-with the reference's warm start, and the FP32 form with the learned preconditioner.
-Inputs are resident in HBM for the whole window.
+PCG to 1e-8, breaking switch on. A step is one full Solver::step (both RK stages, every PCG
+iteration) with dt = stable_dt, driven on the device by wc_run_steps.
+
+The run: S - A spin-up steps in the FP32 form; from that state the FP64 path and the FP32 form
+each take A more steps (A = --accuracy-steps), after which the FP32 form is compared with FP64
+(surface, breaking mask, volume); then each of them goes on in its own solver (warm starts
+full) with W untimed and K timed steps. The headline `value` is the FP64 path, the reference's
+arithmetic, bit for bit the oracle's. Beside it in `paths`: the FP32 chosen form, the FP64
+path with the extrapolated warm start, the FP32 form with the reference's warm start, and the
+FP32 form with the learned preconditioner (these three from the FP64 state in solvers of their
+own, primed until their warm start is full). Inputs are resident in HBM for every timed window.
+`sweep` is the batched spot-sheet sweep (96 reef sheets of uneven size in one persistent launch
+per phase), `cpu_baseline` the C++ FP64 restatement on the host's cores.
 """
 
 from __future__ import annotations
@@ -35,17 +41,16 @@ HBM_PEAK_GBS = 8000.0
 # or written once, halo re-reads excluded), for the roofline.
 BYTES_PER_CELL = {
     "fp64": {
-        "cg_matvec": 8 * (2 + 2 + 1 + 1 + 2 + 2),  # read p_old, z, c, h; write p, Ap
-        "cg_update": 8 * (2 + 2 + 2 + 2 + 1 + 1 + 2 + 2 + 2),  # read w, p, r, Ap, c, h; write w, r, z
-        "cg_init": 8 * (2 + 1 + 1 + 2 + 2 + 2 + 2),  # read w, c, h, b, diag; write r, p
+        "cg_matvec": 8 * (2 + 2 + 1 + 1 + 2),  # read z, p_old, c, h; write p
+        "cg_update": 8 * (2 + 1 + 1 + 2 + 2 + 2 + 2 + 2),  # read p, c, h, w, r; write w, r, z
+        "cg_pair": 8 * (2 + 2 + 1 + 1 + 2) + 8 * (2 + 1 + 1 + 2 + 2 + 2 + 2 + 2),  # one iteration
+        "cg_init": 8 * (2 + 1 + 1 + 2 + 2 + 2),  # read w, c, h, b; write r, z
         "sgn_setup": 8 * (7 + 5),  # read h, hu, hv, phi, lh, lhu, lhv; write c, diag, b
         "flux": 8 * (5 + 3),  # read h, hu, hv, b, phi; write lh, lhu, lhv
         "sgn_finish": 8 * (3 + 3 + 2 + 3 + 1 + 3),  # read s_in, rates, w, s, b; write s
         "switch": 8 * 3,  # read h, b; write phi
     },
     "fp32": {
-        "cg_matvec": 4 * (2 + 2 + 1 + 1 + 2 + 2),
-        "cg_update": 4 * (2 + 2 + 2 + 2 + 1 + 1 + 2 + 2 + 2),
         # one kernel per iteration: read r, Au, Ap, 1/diag, p (pairs), c, h; write p, Ap, r, Au
         # (pairs); the FP32 correction w (a pair) read and written every other iteration
         "cg_iter": 4 * (2 * 5 + 1 + 1 + 2 * 4) + 4 * 2,
@@ -63,28 +68,28 @@ BYTES_PER_CELL = {
         "switch": 4 * 3 + 8,
     },
 }
+# The survey's figure for one FP64 iteration of the reference's PCG as two fused passes (a
+# matvec storing A p, an update reading it back): what the kernels here avoid moving.
+SURVEY_FP64_PCG_BYTES = 192
 
-# rocprofv3 --pmc summaries of the dominant kernel per workload (traffic per launch)
-TRAFFIC_PROFILES = {
-    "large_reef_4096x2048_fifth_sgn":
-        os.path.join(ROOT, "profiles", "cg_iter_large_reef_4096x2048_pmc.md"),
-}
+# rocprofv3 --pmc summaries of the dominant kernels (traffic per launch)
+TRAFFIC_PROFILES = [
+    os.path.join(ROOT, "profiles", "cg_pair_large_reef_4096x2048_pmc.md"),
+    os.path.join(ROOT, "profiles", "cg_iter_large_reef_4096x2048_pmc.md"),
+]
 
 
-def measured_traffic(workload: str, kernel: str):
-    """HBM bytes per launch of `kernel` from the workload's committed rocprofv3 --pmc summary
-    (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads for a copy of known size),
-    or None."""
-    path = TRAFFIC_PROFILES.get(workload)
-    if path is None:
-        return None
-    try:
-        with open(path) as f:
-            for line in f:
-                if line.startswith(f"traffic_bytes_per_launch {kernel} "):
-                    return float(line.split()[2])
-    except OSError:
-        pass
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 --pmc summary (FETCH_SIZE and
+    WRITE_SIZE, each scaled by what the counter reads for a copy of known size), or None."""
+    for path in TRAFFIC_PROFILES:
+        try:
+            with open(path) as f:
+                for line in f:
+                    if line.startswith(f"traffic_bytes_per_launch {kernel} "):
+                        return float(line.split()[2])
+        except OSError:
+            pass
     return None
 
 
@@ -94,11 +99,12 @@ def parse():
     p.add_argument("--steps", type=int, default=100)
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--spinup", type=int, default=None,
-                   help="untimed steps before the window (default: 4000 for the large reef, by "
-                        "when its waves break on the reef edge; 1500 otherwise)")
-    p.add_argument("--spinup-precision", default="fp32", choices=["fp64", "fp32"],
-                   help="the path that brings the run to its mid-run state (every measured path "
-                        "then starts from that one state)")
+                   help="steps before the window (default: 4000 for the large reef, by when its "
+                        "waves break on the reef edge; 1500 otherwise), the last "
+                        "--accuracy-steps of them in both arithmetics")
+    p.add_argument("--accuracy-steps", type=int, default=1000,
+                   help="steps the FP64 path and the FP32 form take from one state before they "
+                        "are compared (and then timed)")
     p.add_argument("--workload", default="large_reef",
                    choices=["rough_bed", "plane_beach", "large_reef", "sweep"])
     p.add_argument("--graph", type=int, default=None, choices=[0, 1],
@@ -107,6 +113,11 @@ def parse():
                    help="sweep: every instance at this size (e.g. 256x128) instead of the drawn ones")
     p.add_argument("--instances", type=int, default=96,
                    help="sweep: the number of reef-sheet instances (seeds 3000, 3001, ...)")
+    p.add_argument("--sweep-spinup", type=int, default=300,
+                   help="sweep leg of the default run: spin-up steps")
+    p.add_argument("--sweep-steps", type=int, default=30,
+                   help="sweep leg of the default run: timed steps (warm-up 5)")
+    p.add_argument("--no-sweep", action="store_true", help="skip the sweep leg")
     p.add_argument("--profile-steps", type=int, default=20)
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="CPU work per thread count for the baseline sample")
@@ -115,7 +126,7 @@ def parse():
     p.add_argument("--no-network", action="store_true",
                    help="skip the FP32 path with the learned preconditioner")
     p.add_argument("--no-breakdown", action="store_true",
-                   help="skip the FP32 path with the reference's warm start (the gain's split)")
+                   help="skip the paths that split the FP32 form's gain")
     p.add_argument("--warm-start", type=int, default=None, choices=[0, 1, 2, 3, 4, 5],
                    help="FP32 form: order of the extrapolated warm start (0: the reference's "
                         "start from the previous solution)")
@@ -130,12 +141,14 @@ def parse():
                    help="one-kernel PCG layout: blocks march down column strips (1) or take "
                         "64x16 tiles (0, default)")
     p.add_argument("--pcg-walk", type=int, default=None,
-                   help="one-kernel PCG on 64x16 tiles: the fewest tiles a block takes (solver "
-                        "default 1; the sweep defaults to 8)")
+                   help="one-kernel PCG on 64x16 tiles: tiles per block to aim for (ceil(tiles "
+                        "/ n) blocks, at most the lanes; solver default 1, the sweep 8)")
     p.add_argument("--recon", type=int, default=None, choices=[32, 64],
-                   help="FP32 form: the flux reconstruction's arithmetic (default 32)")
+                   help="FP32 form: the flux reconstruction's arithmetic (default 64)")
     p.add_argument("--pcg32", type=int, default=None,
                    help="PCG kernels of the FP32 form (3: two per iteration, 4: one)")
+    p.add_argument("--pcg64", type=int, default=None,
+                   help="PCG kernels of the FP64 path (3: 64x4 tiles; default 64x16 tiles)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run a device copy of known size (to scale HBM counters) and exit")
     return p.parse_args()
@@ -155,48 +168,77 @@ def calibrate_copy():
                       "launches": 10}))
 
 
-def measure(w, precision, arrays, t0, warmup, steps, profile_steps, pcg32=None, network=None,
-            lanes=None, warm_start=None, label=None, march=None, recon=None, window=None,
-            guard=None, fp64_warm_start=None):
-    """Time `steps` steps of one precision from the given state; per-kernel times with HIP
-    events on the solver's stream: for the one-kernel PCG iteration of the FP32 form, one pair
-    of events around each batch of its launches inside the timed window itself; for the kernel
-    table, a pair around every launch over `profile_steps` further steps."""
-    import torch
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
-    from wavesim_amd.grid import State
+
+def configure(w, precision, args, *, network=None, warm_start=None, fp64_warm_start=None):
+    """A solver of the workload in one of the measured configurations."""
     from wavesim_amd.solver import PRECISION_FP32, Solver
 
     g = w.solver(Solver)
-    label = label or (precision if network is None else precision + "+network")
-    if precision == "fp64" and fp64_warm_start:
-        g.set_option("fp64_warm_start", fp64_warm_start)
-        if window is not None:
-            g.set_option("warm_start_window", window)
-    if precision == "fp32":
-        g.with_precision(PRECISION_FP32)
-        if warm_start is not None:
-            g.set_option("warm_start", warm_start)
-        if window is not None:
-            g.set_option("warm_start_window", window)
-        if guard is not None:
-            g.set_option("warm_start_guard", guard)
-        if pcg32:
-            g.set_option("pcg_kernels", pcg32)
-        if lanes:
-            g.set_option("pcg_lanes", lanes)
-        if march is not None:
-            g.set_option("pcg_march", march)
-        if recon is not None:
-            g.set_option("flux_reconstruction", recon)
-        if network is not None:
-            g.with_preconditioner(network)
-    h, hu, hv, br = arrays
-    s = State(h.copy(), hu.copy(), hv.copy(), br.copy(), t0)
-    g.run_steps(s, warmup)
+    if precision == "fp64":
+        if args.pcg64:
+            g.set_option("pcg_kernels", args.pcg64)
+        if fp64_warm_start:
+            g.set_option("fp64_warm_start", fp64_warm_start)
+            if args.warm_start_window is not None:
+                g.set_option("warm_start_window", args.warm_start_window)
+        return g
+    g.with_precision(PRECISION_FP32)
+    ws = args.warm_start if warm_start is None else warm_start
+    if ws is not None:
+        g.set_option("warm_start", ws)
+    if args.warm_start_window is not None:
+        g.set_option("warm_start_window", args.warm_start_window)
+    if args.warm_start_guard is not None:
+        g.set_option("warm_start_guard", args.warm_start_guard)
+    if args.pcg32:
+        g.set_option("pcg_kernels", args.pcg32)
+    if args.pcg_lanes:
+        g.set_option("pcg_lanes", args.pcg_lanes)
+    if args.pcg_march is not None:
+        g.set_option("pcg_march", args.pcg_march)
+    if args.recon is not None:
+        g.set_option("flux_reconstruction", args.recon)
+    if network is not None:
+        g.with_preconditioner(network)
+    return g
+
+
+def priming_steps(args, warm_start_order) -> int:
+    """Steps a fresh solver needs before its warm start is what a running one has: the
+    extrapolation's window of past steps (and one more), or one step for the reference's start
+    from the previous solution."""
+    if not warm_start_order:
+        return 1
+    window = args.warm_start_window if args.warm_start_window is not None else 6
+    return window + 1
+
+
+def timed_family(args, precision, network=False):
+    """The PCG iteration family that profile mode 2 times in this configuration, or None."""
+    if precision == "fp64":
+        return "cg_pair" if args.pcg64 in (None, 4, 5) else None
+    return "cg_iter" if (args.pcg32 in (None, 4) and not network) else None
+
+
+def measure(w, g, s, precision, label, warmup, steps, profile_steps, prime=0, family=None):
+    """Time `steps` steps of solver g from state s (after `warmup` untimed steps, of which
+    `prime` beyond --warmup so that a fresh solver's warm start is full; the profile window
+    shrinks by as many, so every path ends the same number of steps after its start). The PCG
+    iterations are timed inside the timed window itself with one pair of HIP events on the
+    solver's stream around each batch of back-to-back launches (FP32: the one-kernel iteration
+    "cg_iter"; FP64: the matvec and update pair "cg_pair"); the kernel table comes from a pair
+    of events around every launch over the profile steps after it."""
+    import torch
+
+    extra = max(0, prime - warmup)
+    g.run_steps(s, warmup + extra)
     st0 = g.dispersion_stats()
-    one_kernel = precision == "fp32" and network is None and pcg32 in (None, 4)
-    if one_kernel:
+    pcg_family = family
+    one_kernel_or_pair = family is not None
+    if one_kernel_or_pair:
         g.profile(2)
         g.profile_reset()
     torch.cuda.synchronize()
@@ -204,29 +246,29 @@ def measure(w, precision, arrays, t0, warmup, steps, profile_steps, pcg32=None, 
     done, last_dt = g.run_steps(s, steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t
-    timed = g.profile_read() if one_kernel else {}
-    if one_kernel:
+    timed = g.profile_read() if one_kernel_or_pair else {}
+    if one_kernel_or_pair:
         g.profile(0)
     st1 = g.dispersion_stats()
     if done != steps:
-        raise SystemExit(f"{precision}: the run stopped after {done} steps (dt = {last_dt})")
+        raise SystemExit(f"{label}: the run stopped after {done} steps (dt = {last_dt})")
     solves = st1.solves - st0.solves if st1 else 0
     iters = st1.iterations - st0.iterations if st1 else 0
     n_cells = w.grid.nx * w.grid.ny
     g.profile(True)
     g.profile_reset()
-    g.run_steps(s, profile_steps)
+    g.run_steps(s, max(profile_steps - extra, 1))
     prof = g.profile_read()
     g.profile(False)
     fam = {k: v for k, v in prof.items() if v[1] > 0}
     table = BYTES_PER_CELL[precision]
     data = {k: v for k, v in fam.items() if table.get(k, 0) > 0}
-    dominant = max(data, key=lambda k: data[k][0])
-    window = "profile"
-    if one_kernel and timed.get("cg_iter", (0, 0))[1] > 0:
-        dominant, window = "cg_iter", "timed"
-        tot_ms, launches = timed["cg_iter"]
+    if one_kernel_or_pair and timed.get(pcg_family, (0, 0))[1] > 0:
+        dominant, window = pcg_family, "timed"
+        tot_ms, launches = timed[pcg_family]
     else:
+        dominant = max(data, key=lambda k: data[k][0])
+        window = "profile"
         tot_ms, launches = data[dominant]
     avg_ms = tot_ms / launches
     # a launch after its solve has converged returns at once: count only the iterations done
@@ -235,31 +277,30 @@ def measure(w, precision, arrays, t0, warmup, steps, profile_steps, pcg32=None, 
     kernels = {k: {"total_ms": round(v[0], 4), "launches": v[1],
                    "avg_us": round(1e3 * v[0] / v[1], 3),
                    "bytes_per_cell": table.get(k, 0)} for k, v in sorted(fam.items())}
-    fallbacks = g.precond_fallbacks() if network is not None else 0
-    print(f"[bench] {label}: {steps} steps in {elapsed:.3f} s = "
-          f"{n_cells * steps / elapsed / 1e6:.1f} M cell-updates/s, "
-          f"{iters / max(solves, 1):.1f} PCG iterations per solve; {dominant} "
-          f"{1e3 * avg_ms:.1f} us = {achieved:.0f} GB/s", file=sys.stderr, flush=True)
+    fallbacks = g.precond_fallbacks() if label.endswith("network") else 0
+    log(f"{label}: {steps} steps in {elapsed:.3f} s = {n_cells * steps / elapsed / 1e6:.1f} M "
+        f"cell-updates/s, {iters / max(solves, 1):.1f} PCG iterations per solve; {dominant} "
+        f"{1e3 * tot_ms / max(useful, 1):.1f} us per iteration = {achieved:.0f} GB/s")
     return {
         "value": n_cells * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
         "pcg_iters_per_solve": iters / max(solves, 1), "dominant": dominant,
-        "avg_ms": avg_ms, "achieved": achieved, "bytes_per_cell": table[dominant],
-        "launches": launches, "iterations": iters, "window": window,
+        "avg_ms": tot_ms / max(useful, 1), "achieved": achieved, "bytes_per_cell": table[dominant],
+        "launches": launches, "iterations": iters, "window": window, "priming_steps": extra,
         "kernels": kernels, "solver": g, "state": s, "precond_fallbacks": fallbacks,
     }
 
 
-def accuracy(w, r64, r32):
-    """The FP32 form against FP64 after the same window from the same state."""
+def accuracy(w, g64, s64, g32, s32):
+    """The FP32 form against FP64: surface error over wet cells, breaking masks (indicator >=
+    0.8), volume."""
     grid = w.grid
     b = grid.inner(w.bed.b)
-    s64, s32 = r64["state"], r32["state"]
     h64, h32 = grid.inner(s64.arrays()[0]), grid.inner(s32.arrays()[0])
     wet = (h64 > 0.05) | (h32 > 0.05)
     d = np.where(wet, (h32 + b) - (h64 + b), 0.0)
-    m64 = r64["solver"].breaking_indicator(s64) >= 0.8
-    m32 = r32["solver"].breaking_indicator(s32) >= 0.8
-    v64, v32 = r64["solver"].volume(s64), r32["solver"].volume(s32)
+    m64 = g64.breaking_indicator(s64) >= 0.8
+    m32 = g32.breaking_indicator(s32) >= 0.8
+    v64, v32 = g64.volume(s64), g32.volume(s32)
     return {"eta_max_err_m": float(np.max(np.abs(d))),
             "eta_rms_err_m": float(np.sqrt(np.mean(d[wet] ** 2))),
             "breaking_mask_agreement": float(np.mean(m64 == m32)),
@@ -268,35 +309,40 @@ def accuracy(w, r64, r32):
             "time_diff_s": abs(s32.time - s64.time)}
 
 
-def cpu_baseline(w, state_arrays, t0, seconds):
-    """The CPU FP64 restatement (reference row order, glibc) on the host's cores, from the same
-    mid-run state, for about `seconds` of CPU work per thread count; cell-updates/s."""
+def cpu_rate(w, state_arrays, t0, seconds, threads):
+    """The CPU FP64 restatement (reference row order, glibc) stepping from a state for about
+    `seconds`; (cell-updates/s, steps, seconds)."""
     from oracle import oracle as orc
     from wavesim_amd.grid import State
 
+    o = w.solver(lambda g, b: orc.OracleSolver(g, b, threads=threads))
+    h, hu, hv, br = state_arrays
+    s = State(h.copy(), hu.copy(), hv.copy(), br.copy(), t0)
+    n, el = 0, 0.0
+    t = time.perf_counter()
+    while el < seconds and n < 10000:
+        o.step_n(s, 1)
+        n += 1
+        el = time.perf_counter() - t
+    return w.grid.nx * w.grid.ny * n / el, n, el
+
+
+def cpu_baseline(w, state_arrays, t0, seconds):
     cores = min(os.cpu_count() or 1, 16)
     out = {}
     for threads in (cores, 1):
-        o = w.solver(lambda g, b: orc.OracleSolver(g, b, threads=threads))
-        h, hu, hv, br = state_arrays
-        s = State(h.copy(), hu.copy(), hv.copy(), br.copy(), t0)
-        n, el = 0, 0.0
-        t = time.perf_counter()
-        while el < seconds and n < 10000:
-            o.step_n(s, 1)
-            n += 1
-            el = time.perf_counter() - t
-        out[threads] = (w.grid.nx * w.grid.ny * n / el, n, el)
-        print(f"[bench] cpu baseline, {threads} threads: {n} steps in {el:.1f} s",
-              file=sys.stderr, flush=True)
+        out[threads] = cpu_rate(w, state_arrays, t0, seconds, threads)
+        log(f"cpu baseline, {threads} threads: {out[threads][1]} steps in "
+            f"{out[threads][2]:.1f} s")
     return cores, out
 
 
-def sweep(args):
+def sweep(args, spinup, warmup, steps, cpu_seconds):
     """The batched spot-sheet sweep: reef-sheet instances of uneven size (wavesim_amd.configs
     .c3_reef_instance, seeds 3000...), FP32 form, all stepped together by wc_batch_run_steps;
     then the same solvers one at a time for comparison. cell-updates/s counts every instance's
-    interior cells."""
+    interior cells. The PCG iteration launches (kb_cg1_iter) are timed over a few profiled
+    steps after the timed window, each between its own events."""
     import torch
 
     from wavesim_amd import configs
@@ -316,33 +362,39 @@ def sweep(args):
     b = Batch(solvers)
     if args.graph is not None:
         b.set_option("graph", args.graph)
-    print(f"[bench] sweep: {len(ws)} instances built, {cells / 1e6:.1f} M cells", file=sys.stderr,
-          flush=True)
+    log(f"sweep: {len(ws)} instances built, {cells / 1e6:.1f} M cells")
     t = time.perf_counter()
-    for done in range(0, args.spinup, 100):  # in chunks, each reported
-        b.run_steps(states, min(100, args.spinup - done))
-        print(f"[bench] sweep spin-up: {min(done + 100, args.spinup)} steps, "
-              f"{time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
-    print(f"[bench] sweep: {len(ws)} instances, {cells / 1e6:.1f} M cells, spin-up "
-          f"{args.spinup} steps in {time.perf_counter() - t:.1f} s, launch widths "
-          f"{b.launch_widths()}", file=sys.stderr, flush=True)
-    b.run_steps(states, args.warmup)
+    for done in range(0, spinup, 100):  # in chunks, each reported
+        b.run_steps(states, min(100, spinup - done))
+        log(f"sweep spin-up: {min(done + 100, spinup)} steps, {time.perf_counter() - t:.1f} s")
+    b.run_steps(states, warmup)
     st0 = [g.dispersion_stats() for g in solvers]
     torch.cuda.synchronize()
     t = time.perf_counter()
-    done, _ = b.run_steps(states, args.steps)
+    done, _ = b.run_steps(states, steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t
     st1 = [g.dispersion_stats() for g in solvers]
-    if min(done) != args.steps:
+    if min(done) != steps:
         raise SystemExit(f"sweep: an instance stopped after {min(done)} steps")
     iters = sum(b1.iterations - b0.iterations for b0, b1 in zip(st0, st1))
     solves = sum(b1.solves - b0.solves for b0, b1 in zip(st0, st1))
-    value = cells * args.steps / elapsed
-    print(f"[bench] sweep batched: {args.steps} steps in {elapsed:.3f} s = {value / 1e6:.1f} M "
-          f"cell-updates/s, {iters / max(solves, 1):.1f} PCG iterations per solve",
-          file=sys.stderr, flush=True)
-    alone_steps = max(args.steps // 5, 2)
+    value = cells * steps / elapsed
+    log(f"sweep batched: {steps} steps in {elapsed:.3f} s = {value / 1e6:.1f} M cell-updates/s, "
+        f"{iters / max(solves, 1):.1f} PCG iterations per solve")
+    # the PCG iteration launches, timed one by one over a few more steps
+    prof_steps = 3
+    p0 = [g.dispersion_stats() for g in solvers]
+    b.set_option("profile", 1)
+    b.run_steps(states, prof_steps)
+    ms, launches = b.profile_read()
+    b.set_option("profile", 0)
+    p1 = [g.dispersion_stats() for g in solvers]
+    cell_iters = sum(w.grid.nx * w.grid.ny * (b1.iterations - b0.iterations)
+                     for w, b0, b1 in zip(ws, p0, p1))
+    per_cell = BYTES_PER_CELL["fp32"]["cg_iter"]
+    achieved = per_cell * cell_iters / (ms * 1e-3) / 1e9
+    alone_steps = max(steps // 5, 2)
     for g in solvers:  # alone, a small grid is fastest with a PCG block per tile
         g.set_option("pcg_tiles_per_block", 1)
     st0 = [g.dispersion_stats() for g in solvers]
@@ -356,28 +408,42 @@ def sweep(args):
     iters1 = sum(b1.iterations - b0.iterations for b0, b1 in zip(st0, st1))
     solves1 = sum(b1.solves - b0.solves for b0, b1 in zip(st0, st1))
     v1 = cells * alone_steps / el1
-    print(f"[bench] sweep one at a time: {alone_steps} steps in {el1:.3f} s = {v1 / 1e6:.1f} M "
-          f"cell-updates/s, {iters1 / max(solves1, 1):.1f} PCG iterations per solve",
-          file=sys.stderr, flush=True)
+    log(f"sweep one at a time: {alone_steps} steps in {el1:.3f} s = {v1 / 1e6:.1f} M "
+        f"cell-updates/s, {iters1 / max(solves1, 1):.1f} PCG iterations per solve")
     sizes = {}
     for w in ws:
         sizes[f"{w.grid.nx}x{w.grid.ny}"] = sizes.get(f"{w.grid.nx}x{w.grid.ny}", 0) + 1
-    print(json.dumps({
-        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic",
+    cpu = None
+    if cpu_seconds > 0:
+        # a sample of the same workload: the sweep's first 512x256 instance from its state
+        q = next((i for i, w in enumerate(ws) if (w.grid.nx, w.grid.ny) == (512, 256)), 0)
+        cores = min(os.cpu_count() or 1, 16)
+        arrays = [a.copy() for a in states[q].arrays()]
+        v, n, el = cpu_rate(ws[q], arrays, states[q].time, cpu_seconds, cores)
+        cpu = {"value": v, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+               "sample": f"{n} steps of sweep instance {q} ({ws[q].grid.nx}x{ws[q].grid.ny}, seed "
+                         f"{3000 + q}) from its spun-up state ({el:.1f} s), C++ FP64 "
+                         f"restatement, reference row order and glibc, OpenMP rows"}
+        log(f"sweep cpu baseline: {n} steps in {el:.1f} s")
+    return {
+        "value": value, "unit": "cell-updates/s", "ms_per_step": 1e3 * elapsed / steps,
+        "steps": steps, "warmup": warmup, "dtype": "f32",
         "config": {"workload": f"reef_sheet_sweep_{len(ws)}" + (f"_{args.sweep_size}"
-                                                                 if args.sweep_size else ""),
-                   "instances": len(ws),
-                   "cells": cells, "sizes": sizes, "spinup_steps": args.spinup,
+                                                                if args.sweep_size else ""),
+                   "instances": len(ws), "cells": cells, "sizes": sizes, "spinup_steps": spinup,
                    "path": "FP32 form, one launch per phase over all instances",
-                   "pcg_tiles_per_block": walk,
-                   "launch_widths": list(b.launch_widths())},
+                   "pcg_tiles_per_block": walk, "launch_widths": list(b.launch_widths())},
         "pcg_iters_per_solve": iters / max(solves, 1),
+        "roofline": {"bound": "hbm", "kernel": "kb_cg1_iter", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_cell": per_cell,
+                     "cell_iterations": cell_iters, "launches": launches,
+                     "avg_launch_us": 1e3 * ms / max(launches, 1),
+                     "window": f"{prof_steps} profiled steps after the timed window"},
         "one_at_a_time": {"value": v1, "steps": alone_steps, "batched_over_alone": value / v1,
                           "pcg_iters_per_solve": iters1 / max(solves1, 1)},
-        "roofline": None, "cpu_baseline": None}))
+        "cpu_baseline": cpu,
+    }
 
 
 def main():
@@ -391,7 +457,6 @@ def main():
     import torch
 
     from wavesim_amd import configs
-    from wavesim_amd.solver import PRECISION_FP32, Solver
 
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no GPU"}))
@@ -401,69 +466,97 @@ def main():
         calibrate_copy()
         return
     if args.workload == "sweep":
-        sweep(args)
+        out = sweep(args, args.spinup, args.warmup, args.steps,
+                    0.0 if args.no_cpu else args.cpu_seconds)
+        print(json.dumps({"metric": METRIC, "n_gpus": 1, "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "data": "synthetic", **out},
+                         default=float))
         return
     w = {"rough_bed": configs.c1_rough_bed, "plane_beach": configs.c0_plane_beach,
          "large_reef": configs.c4_large_reef}[args.workload]()
-    g = w.solver(Solver)
-    if args.spinup_precision == "fp32":
-        g.with_precision(PRECISION_FP32)
-    s = w.still_water()
-    print(f"[bench] {w.name}: spin-up {args.spinup} steps ({args.spinup_precision})",
-          file=sys.stderr, flush=True)
-    t = time.perf_counter()
-    done, _ = g.run_steps(s, args.spinup)
-    print(f"[bench] spin-up {done} steps in {time.perf_counter() - t:.1f} s, t = {s.time:.1f} s",
-          file=sys.stderr, flush=True)
-    arrays = [a.copy() for a in s.arrays()]
-    t0 = s.time
-    del g
-    precisions = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
-    res = {p: measure(w, p, arrays, t0, args.warmup, args.steps, args.profile_steps,
-                      args.pcg32, lanes=args.pcg_lanes, march=args.pcg_march, recon=args.recon,
-                      window=args.warm_start_window, warm_start=args.warm_start,
-                      guard=args.warm_start_guard)
-           for p in precisions}
-    if "fp64" in res and not args.no_breakdown:
-        # the FP64 path with the extrapolated warm start: what the start gains without FP32
-        res["fp64+extrapolated_start"] = measure(
-            w, "fp64", arrays, t0, args.warmup, args.steps, args.profile_steps, args.pcg32,
-            window=args.warm_start_window, fp64_warm_start=3, label="fp64+extrapolated_start")
-    if "fp32" in res and not args.no_breakdown and args.warm_start != 0:
-        # the FP32 form with the reference's warm start: what storage alone gains
-        res["fp32+previous_start"] = measure(
-            w, "fp32", arrays, t0, args.warmup, args.steps, args.profile_steps, args.pcg32,
-            lanes=args.pcg_lanes, march=args.pcg_march, recon=args.recon, warm_start=0,
-            label="fp32+previous_start")
-    net_file = os.path.join(ROOT, "wavesim_amd", "data", "precond_rough_bed.npy")
-    if "fp32" in res and not args.no_network and os.path.exists(net_file):
-        res["fp32+network"] = measure(w, "fp32", arrays, t0, args.warmup, args.steps,
-                                      args.profile_steps, args.pcg32, np.load(net_file),
-                                      lanes=args.pcg_lanes, march=args.pcg_march, recon=args.recon,
-                      window=args.warm_start_window, warm_start=args.warm_start,
-                      guard=args.warm_start_guard)
-    accs = {p: accuracy(w, res["fp64"], res[p]) for p in res if p != "fp64"} \
-        if "fp64" in res else {}
-    # the chosen form: the fastest path within the accuracy bounds against FP64
-    ok = [p for p, a in accs.items() if p.startswith("fp32")
-          and a["eta_max_err_m"] < 1e-3 and a["breaking_mask_agreement"] >= 0.999]
-    chosen = max(ok, key=lambda p: res[p]["value"]) if ok else precisions[-1]
-    if "fp64" not in res:
-        chosen = precisions[0]
-    acc = accs.get(chosen)
-    r = res[chosen]
+    from wavesim_amd.grid import State
 
+    acc_steps = min(args.accuracy_steps, args.spinup)
+    # 1. the FP32 form to the start of the accuracy window
+    g32 = configure(w, "fp32", args)
+    s32 = w.still_water()
+    log(f"{w.name}: spin-up {args.spinup - acc_steps} steps (FP32 form)")
+    t = time.perf_counter()
+    g32.run_steps(s32, args.spinup - acc_steps)
+    log(f"spin-up in {time.perf_counter() - t:.1f} s, t = {s32.time:.1f} s")
+    # 2. both arithmetics from that one state through the accuracy window
+    h, hu, hv, br = (a.copy() for a in s32.arrays())
+    s64 = State(h, hu, hv, br, s32.time)
+    g64 = configure(w, "fp64", args) if args.precision != "fp32" else None
+    t = time.perf_counter()
+    if g64 is not None:
+        g64.run_steps(s64, acc_steps)
+        log(f"FP64 path through the {acc_steps}-step window in {time.perf_counter() - t:.1f} s")
+    g32.run_steps(s32, acc_steps)
+    window_acc = accuracy(w, g64, s64, g32, s32) if g64 is not None else None
+    if window_acc:
+        log(f"FP32 form vs FP64 after {acc_steps} steps: {window_acc}")
+    # every further path starts from the FP64 state (the FP32 form from its own)
+    base = s64 if g64 is not None else s32
+    arrays = [a.copy() for a in base.arrays()]
+    t0 = base.time
+
+    def fresh_state():
+        return State(*(a.copy() for a in arrays), t0)
+
+    res = {}
+    if g64 is not None:
+        res["fp64"] = measure(w, g64, s64, "fp64", "fp64", args.warmup, args.steps,
+                              args.profile_steps, family=timed_family(args, "fp64"))
+    if args.precision != "fp64":
+        res["fp32"] = measure(w, g32, s32, "fp32", "fp32", args.warmup, args.steps,
+                              args.profile_steps, family=timed_family(args, "fp32"))
+    if not args.no_breakdown and g64 is not None:
+        # the FP64 path with the extrapolated warm start: what the start gains without FP32
+        g = configure(w, "fp64", args, fp64_warm_start=3)
+        res["fp64+extrapolated_start"] = measure(
+            w, g, fresh_state(), "fp64", "fp64+extrapolated_start", args.warmup, args.steps,
+            args.profile_steps, prime=priming_steps(args, 3), family=timed_family(args, "fp64"))
+        if args.precision != "fp64":
+            # the FP32 form with the reference's warm start: what storage alone gains
+            g = configure(w, "fp32", args, warm_start=0)
+            res["fp32+previous_start"] = measure(
+                w, g, fresh_state(), "fp32", "fp32+previous_start", args.warmup, args.steps,
+                args.profile_steps, prime=priming_steps(args, 0),
+                family=timed_family(args, "fp32"))
+    net_file = os.path.join(ROOT, "wavesim_amd", "data", "precond_rough_bed.npy")
+    if args.precision != "fp64" and not args.no_network and os.path.exists(net_file):
+        g = configure(w, "fp32", args, network=np.load(net_file))
+        order = 3 if args.warm_start is None else args.warm_start
+        res["fp32+network"] = measure(w, g, fresh_state(), "fp32", "fp32+network", args.warmup,
+                                      args.steps, args.profile_steps,
+                                      prime=priming_steps(args, order),
+                                      family=timed_family(args, "fp32", network=True))
+    accs = {}
+    if "fp64" in res:
+        r64 = res["fp64"]
+        for p, r in res.items():
+            if p != "fp64":
+                accs[p] = accuracy(w, r64["solver"], r64["state"], r["solver"], r["state"])
+    head = "fp64" if "fp64" in res else "fp32"
+    r = res[head]
     cpu = None
     if not args.no_cpu:
         cores, cres = cpu_baseline(w, arrays, t0, args.cpu_seconds)
         v, n, el = cres[cores]
         v1, n1, el1 = cres[1]
         cpu = {"value": v, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-               "sample": f"{n} steps of the same workload from the same mid-run state "
+               "sample": f"{n} steps of the same workload from the FP64 path's window state "
                          f"({el:.1f} s), C++ FP64 restatement, reference row order and glibc, "
                          f"OpenMP rows",
                "value_1_thread": v1, "sample_1_thread": f"{n1} steps ({el1:.1f} s)"}
+    sw = None
+    if args.workload == "large_reef" and not args.no_sweep:
+        sw = sweep(args, args.sweep_spinup, 5, args.sweep_steps,
+                   0.0 if args.no_cpu else min(args.cpu_seconds, 10.0))
 
+    ratio = {p: v["value"] / res["fp64"]["value"] for p, v in res.items()} \
+        if "fp64" in res else {}
     line = {
         "metric": METRIC,
         "value": r["value"],
@@ -475,44 +568,41 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if chosen.startswith("fp32") else "f64",
+        "dtype": "f64" if head == "fp64" else "f32",
         "data": "synthetic",
         "config": {"workload": w.name, "nx": w.grid.nx, "ny": w.grid.ny, "dx_m": w.grid.dx,
-                   "path": ("FP32 chosen form (FP32 storage of the state pairs, rates and PCG "
-                            "vectors; FP64 inner products, norms, PCG scalars, surface-minus-bed "
-                            "differences, SGN solution)" if chosen.startswith("fp32")
-                            else "FP64 plain path"),
-                   "flux_reconstruction": (("fp32" if args.recon == 32 else "fp64")
-                                           if chosen.startswith("fp32") else "fp64"),
-                   "preconditioner": ("learned network, flexible PCG, Jacobi fallback"
-                                      if chosen.endswith("network")
-                                      else "Jacobi (the reference's)"),
-                   "warm_start": ("the previous solution (the reference's)"
-                                  if not chosen.startswith("fp32") or chosen.endswith(
-                                      "previous_start") or args.warm_start == 0
-                                  else "extrapolated, order {}, window {}".format(
-                                      3 if args.warm_start is None else args.warm_start,
-                                      6 if args.warm_start_window is None
-                                      else args.warm_start_window)),
+                   "path": ("FP64 plain path: the reference's arithmetic and PCG (Jacobi, warm "
+                            "start from the previous solution), bit for bit the oracle's"
+                            if head == "fp64" else "FP32 chosen form"),
                    "pcg_tolerance": 1e-8,
-                   "spinup_steps": args.spinup, "spinup_path": args.spinup_precision,
+                   "spinup_steps": args.spinup, "accuracy_window_steps": acc_steps,
                    "t_start_s": round(t0, 3), **w.notes},
         "pcg_iters_per_solve": r["pcg_iters_per_solve"],
         "roofline": {"bound": "hbm", "kernel": r["dominant"], "achieved": r["achieved"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["achieved"] / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(w.name, r["dominant"]),
+                     "traffic": measured_traffic(r["dominant"]),
                      "algorithmic_bytes_per_launch": r["bytes_per_cell"] * w.grid.nx * w.grid.ny,
                      "bytes_per_cell": r["bytes_per_cell"],
+                     **({"survey_bytes_per_cell": SURVEY_FP64_PCG_BYTES,
+                         "kernels": "k_cg2_matvec + k_cg2_update (one PCG iteration)"}
+                        if r["dominant"] == "cg_pair" else {}),
                      "avg_launch_us": 1e3 * r["avg_ms"], "launches": r["launches"],
                      "pcg_iterations": r["iterations"], "window": r["window"]},
         "paths": {p: {"value": v["value"], "ms_per_step": v["ms_per_step"],
                       "pcg_iters_per_solve": v["pcg_iters_per_solve"],
-                      "precond_fallbacks": v["precond_fallbacks"], "kernels": v["kernels"],
-                      "vs_fp64": accs.get(p)}
+                      "vs_fp64_speed": ratio.get(p), "priming_steps": v["priming_steps"],
+                      "precond_fallbacks": v["precond_fallbacks"],
+                      "pcg_us_per_iteration": 1e3 * v["avg_ms"] if v["window"] == "timed"
+                      else None,
+                      "kernels": v["kernels"], "vs_fp64": accs.get(p)}
                   for p, v in res.items()},
-        "chosen_path": chosen,
-        "fp32_vs_fp64": acc,
+        "fp32_vs_fp64": {"window_steps": acc_steps + args.warmup + args.steps
+                         + args.profile_steps, **accs["fp32"]} if "fp32" in accs else window_acc,
+        "north_star": {"target": "FP32 + learned preconditioner >= 4x the FP64 path",
+                       "fp32+network_over_fp64": ratio.get("fp32+network"),
+                       "fp32_over_fp64": ratio.get("fp32")},
         "cpu_baseline": cpu,
+        "sweep": sw,
     }
     print(json.dumps(line, default=float))
 

@@ -335,13 +335,17 @@ void wc_batch_free(wc_batch* b);
 int wc_batch_set_option(wc_batch* b, const char* key, int64_t value);
 int wc_batch_launch_widths(const wc_batch* b, int32_t* grid_nt, int32_t* grid_pt);
 int wc_batch_run_steps(wc_batch* b, uint64_t steps, uint64_t* done_steps, double* last_dt);
+/* Time spent in the batch's PCG iteration launches since wc_batch_set_option("profile", 1),
+ * each launch between its own pair of events (while profiling, the stretches of a step are
+ * issued directly, not from the graphs). Measurement only; no counterpart in the reference. */
+int wc_batch_profile_read(wc_batch* b, double* total_ms, uint64_t* launches);
 
 /* Measurement hooks for benchmarks: device time in milliseconds of each kernel family over the
  * steps since the last reset, measured with HIP events on the solver's stream, and the kernel
  * launch counts. names receives a ';'-separated list. on = 1 puts a pair of events around every
- * launch; on = 2 times only the one-kernel PCG iteration of the FP32 form ("cg_iter"), one pair
- * of events around each batch of its back-to-back launches, which leaves a timed run nearly
- * undisturbed; 0 (the default) times nothing. */
+ * launch; on = 2 times only the PCG iterations (the FP32 form's one kernel, "cg_iter"; the FP64
+ * path's matvec and update, "cg_pair"), one pair of events around each batch of back-to-back
+ * launches, which leaves a timed run nearly undisturbed; 0 (the default) times nothing. */
 int wc_profile_enable(wc_solver* s, int32_t on);
 int wc_profile_reset(wc_solver* s);
 int wc_profile_read(wc_solver* s, char* names, uint64_t names_len, double* total_ms,

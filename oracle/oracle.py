@@ -38,7 +38,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libwavecore_oracle.so")
 
 DOT_REFERENCE = 0
-DOT_CANONICAL = 1
+DOT_CANONICAL = 1  # the GPU's 64x4-tile order (state sums; the 64x4-tile PCG variants)
+DOT_CANONICAL16 = 2  # the GPU's default FP64 PCG: 64x16 tiles, running sums per cell
 MATH_GLIBC = 0
 MATH_PORTABLE = 1
 

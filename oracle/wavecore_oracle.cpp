@@ -437,6 +437,45 @@ double canonical_sum(const Grid& g, const Par& par, Term term) {
   return (ws[0] + ws[1]) + (ws[2] + ws[3]);
 }
 
+// The FP64 PCG kernels' order (wc_pcg2.hip): 64x16-cell tiles t = ty * ntx + tx; lane b of 256
+// takes tiles b, b+256, ... and each of its 1024 threads (one cell of a tile) adds its cell's
+// term tile by tile from 0.0 (0.0 for cells outside the grid); each row of 64 running sums goes
+// through the butterfly, the 16 row sums are added in row order from 0.0, and the 256 lanes
+// end as in canonical_sum.
+template <class Term>
+double canonical16_sum(const Grid& g, const Par& par, Term term) {
+  const size_t nx = g.nx, ny = g.ny;
+  const size_t ntx = (nx + 63) / 64, nty = (ny + 15) / 16, np = ntx * nty;
+  double acc[256];
+#pragma omp parallel for schedule(static) num_threads(par.threads) if (par.on(g.cells()))
+  for (long b = 0; b < 256; ++b) {
+    double run[16][64];
+    for (size_t r = 0; r < 16; ++r)
+      for (size_t l = 0; l < 64; ++l) run[r][l] = 0.0;
+    for (size_t p = (size_t)b; p < np; p += 256) {
+      const size_t tx = p % ntx, ty = p / ntx;
+      for (size_t r = 0; r < 16; ++r)
+        for (size_t l = 0; l < 64; ++l) {
+          const size_t i = tx * 64 + l, j = ty * 16 + r;
+          run[r][l] = run[r][l] + ((i < nx && j < ny) ? term(GH + i, GH + j) : 0.0);
+        }
+    }
+    double t = 0.0;
+    for (size_t r = 0; r < 16; ++r) {
+      butterfly(run[r]);
+      t = t + run[r][0];
+    }
+    acc[b] = t;
+  }
+  double ws[4];
+  for (size_t wv = 0; wv < 4; ++wv) {
+    double* v = acc + 64 * wv;
+    butterfly(v);
+    ws[wv] = v[0];
+  }
+  return (ws[0] + ws[1]) + (ws[2] + ws[3]);
+}
+
 // par.rs:90-98 with dispersion.rs:319-329: rows summed serially, then in row order from 0.0.
 template <class Term>
 double row_sum(const Grid& g, const Par& par, Term term) {
@@ -460,7 +499,9 @@ double dot(const Grid& g, const Par& par, int order, const double* ax, const dou
     const size_t k = j * w + i;
     return ax[k] * bx[k] + ay[k] * by[k];
   };
-  return order == 1 ? canonical_sum(g, par, term) : row_sum(g, par, term);
+  if (order == 1) return canonical_sum(g, par, term);
+  if (order == 2) return canonical16_sum(g, par, term);
+  return row_sum(g, par, term);
 }
 
 // ---------------------------------------------------------------- SGN dispersion
@@ -1358,7 +1399,8 @@ int wco_bed(const wco_solver* s, double* out) {
 }
 
 int wco_set_dot_order(wco_solver* s, int32_t order) {
-  if (order != 0 && order != 1) return fail("dot order must be 0 (reference) or 1 (canonical)");
+  if (order < 0 || order > 2)
+    return fail("dot order must be 0 (reference), 1 (canonical, 64x4 tiles) or 2 (64x16 tiles)");
   s->dot_order = order;
   return 0;
 }
@@ -1637,11 +1679,12 @@ int wco_dispersion_stats(const wco_solver* s, int32_t* has, wco_dispersion_stats
   return 0;
 }
 
-// state.rs:79-85 (order 0: Rust's serial sum from -0.0; order 1: the canonical tree)
+// state.rs:79-85 (order 0: Rust's serial sum from -0.0; otherwise the canonical tree of the
+// GPU's state sums, 64x4 tiles, whichever order its PCG takes)
 int wco_state_volume(wco_solver* s, double* out) {
   const Grid& g = s->g;
   double sum;
-  if (s->dot_order == 1) {
+  if (s->dot_order != 0) {
     const double* h = s->s.h.data();
     sum = canonical_sum(g, s->par, [&](size_t i, size_t j) { return h[g.idx(i, j)]; });
   } else {

@@ -12,7 +12,10 @@
  *     then the rows in order from 0.0, par.rs:90-98); 1 = the GPU's canonical order (64x4-cell
  *     tiles, a 64-lane xor butterfly per tile row adding lane l^1 first and l^32 last, rows
  *     paired, tiles summed by 256 strided lanes and a final butterfly), which the GPU kernels
- *     follow bit for bit;
+ *     follow bit for bit; 2 = the order of the GPU's default FP64 PCG (64x16-cell tiles taken
+ *     by 256 strided lanes, a running sum per cell position over the lane's tiles, each row of
+ *     64 by the butterfly, the 16 rows in order, the lanes as in 1). The volume sums in order 1
+ *     whenever the dot order is not 0;
  *   - libm (wco_set_math): 0 = glibc's hypot and pow (what the Rust reference calls on Linux);
  *     1 = the portable IEEE-only hypot and pow(h, 4/3) that the GPU evaluates.
  * exp, sin and cos only ever see host-known arguments and always come from glibc, in the

@@ -26,8 +26,10 @@ from wavesim_amd.solver import Solver
 pytestmark = pytest.mark.gpu
 
 
-def oracle(grid, bed):
-    return orc.OracleSolver(grid, bed, dot_order=orc.DOT_CANONICAL, math=orc.MATH_PORTABLE)
+def oracle(grid, bed, order=orc.DOT_CANONICAL16):
+    """The oracle in the GPU's summation order: 64x16 tiles for the default FP64 PCG, 64x4
+    tiles for the PCG variants pcg_kernels 1-3."""
+    return orc.OracleSolver(grid, bed, dot_order=order, math=orc.MATH_PORTABLE)
 
 
 def say(*a):
@@ -55,7 +57,9 @@ def lockstep(grid, bed, configure, state: State, steps: int, label: str, fixed_d
     for k, v in (options or {}).items():
         g.set_option(k, v)
     g = configure(g)
-    o = configure(oracle(grid, bed.copy()))
+    order = (orc.DOT_CANONICAL if (options or {}).get("pcg_kernels") in (1, 2, 3)
+             else orc.DOT_CANONICAL16)
+    o = configure(oracle(grid, bed.copy(), order))
     sg, so = state.clone(), state.clone()
     say(f"[{label}] {grid.nx}x{grid.ny}, {steps} steps")
     for n in range(steps):
@@ -102,7 +106,7 @@ def test_each_order_over_a_rough_bed_with_dry_land(order, flux):
              150, f"rough-dry {order.name} flux={flux}", options={"flux_kernel": flux})
 
 
-@pytest.mark.parametrize("pcg", [1, 2, 3])
+@pytest.mark.parametrize("pcg", [1, 2, 3, 5])
 @pytest.mark.parametrize("periodic", [False, True])
 def test_sgn_fifth_walls_and_periodic_seam(periodic, pcg):
     grid = Grid(40, 30, 1.0, 1.0)
@@ -121,7 +125,7 @@ def test_sgn_fifth_walls_and_periodic_seam(periodic, pcg):
              options={"pcg_kernels": pcg})
 
 
-@pytest.mark.parametrize("pcg", [1, 2, 3])
+@pytest.mark.parametrize("pcg", [1, 2, 3, 5])
 def test_sgn_rough_bed_tight_tolerance_with_maker_sponges_and_friction(pcg):
     w = configs.c1_rough_bed(nx=160, ny=64, steps=200)
 
@@ -258,7 +262,7 @@ def test_invalid_setups_raise_like_the_reference_panics():
         Solver(Grid(2, 10, 1.0, 1.0), Bathymetry.flat(Grid(2, 10, 1.0, 1.0), 1.0))
 
 
-@pytest.mark.parametrize("pcg", [1, 3])
+@pytest.mark.parametrize("pcg", [1, 3, 5])
 @pytest.mark.parametrize("periodic", [False, True])
 def test_breaking_fronts_and_eddy_viscosity(periodic, pcg):
     """Breaking::viscosity on (solver.rs:512-532, eddy.rs): a running hump as tall as the water

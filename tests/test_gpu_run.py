@@ -24,7 +24,7 @@ def lockstep(nx, ny, seed, period, stats_from, until, frame_every, tmp_path, tid
     oracle's frames are computed on the host from its state."""
     w = configs.c1_rough_bed(nx=nx, ny=ny, seed=seed)
     g = w.solver(Solver)
-    o = w.solver(lambda gr, b: orc.OracleSolver(gr, b, dot_order=orc.DOT_CANONICAL,
+    o = w.solver(lambda gr, b: orc.OracleSolver(gr, b, dot_order=orc.DOT_CANONICAL16,
                                                 math=orc.MATH_PORTABLE))
     s = w.still_water()
     s.time = 20.0

@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 def test_stage_statistics_match_the_oracle():
     w = configs.c1_rough_bed(nx=96, ny=48, seed=11)
     g = w.solver(Solver)
-    o = w.solver(lambda gr, b: orc.OracleSolver(gr, b, dot_order=orc.DOT_CANONICAL,
+    o = w.solver(lambda gr, b: orc.OracleSolver(gr, b, dot_order=orc.DOT_CANONICAL16,
                                                 math=orc.MATH_PORTABLE))
     s = w.still_water()
     s.time = 20.0

@@ -230,10 +230,19 @@ def test_canonical_dot_matches_a_correctly_rounded_sum():
         terms = (a[0] * a[2] + a[1] * a[3])[inner]
         exact = math.fsum(terms.tolist())
         canon = orc.dot(grid, *a, order=orc.DOT_CANONICAL)
+        canon16 = orc.dot(grid, *a, order=orc.DOT_CANONICAL16)
         ref = orc.dot(grid, *a, order=orc.DOT_REFERENCE)
         scale = float(np.sum(np.abs(terms)))
         assert abs(canon - exact) <= 1e-14 * scale
+        assert abs(canon16 - exact) <= 1e-14 * scale
         assert abs(ref - exact) <= 1e-13 * scale
+        # every interior cell exactly once: integer terms 1..n sum exactly in any order
+        ones = np.ones(n)
+        ids = np.zeros(n)
+        ids[inner] = np.arange(1, inner.size + 1, dtype=np.float64)
+        want = inner.size * (inner.size + 1) // 2
+        for order in (orc.DOT_REFERENCE, orc.DOT_CANONICAL, orc.DOT_CANONICAL16):
+            assert orc.dot(grid, ids, ids * 0, ones, ones, order=order) == want, (nx, ny, order)
 
 
 def test_portable_math_is_within_one_ulp_of_glibc():

@@ -34,10 +34,11 @@ def test_fv_plane_beach_is_identical_in_both_modes():
 def test_sgn_modes_differ_only_by_rounding_in_the_inner_products():
     w = configs.c1_rough_bed(nx=160, ny=64)
     ref, st_ref = run(w, orc.DOT_REFERENCE, orc.MATH_GLIBC, 200, t0=20.0)
-    gpu, st_gpu = run(w, orc.DOT_CANONICAL, orc.MATH_PORTABLE, 200, t0=20.0)
     libm, _ = run(w, orc.DOT_REFERENCE, orc.MATH_PORTABLE, 200, t0=20.0)
-    gaps = [max_rel_diff(a, b) for a, b in zip(gpu[:3], ref[:3])]
-    assert max(gaps) < 1e-12, gaps
-    assert max(gaps) > 0.0  # the orders do differ: the test has teeth
-    assert st_ref == st_gpu
+    for order in (orc.DOT_CANONICAL, orc.DOT_CANONICAL16):
+        gpu, st_gpu = run(w, order, orc.MATH_PORTABLE, 200, t0=20.0)
+        gaps = [max_rel_diff(a, b) for a, b in zip(gpu[:3], ref[:3])]
+        assert max(gaps) < 1e-12, (order, gaps)
+        assert max(gaps) > 0.0  # the orders do differ: the test has teeth
+        assert st_ref == st_gpu
     assert all(max_rel_diff(a, b) == 0.0 for a, b in zip(libm[:3], ref[:3]))

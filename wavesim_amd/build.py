@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libwavecore_hip.so")
 SOURCES = ["wc_api.hip", "wc_kernels.hip", "wc_flux.hip", "wc_fp32.hip", "wc_pcg.hip",
-           "wc_pcg1.hip", "wc_relax.hip", "wc_unet.hip", "wc_fpcg.hip",
+           "wc_pcg1.hip", "wc_pcg2.hip", "wc_relax.hip", "wc_unet.hip", "wc_fpcg.hip",
            "wc_stats.hip", "wc_run.hip", "wc_eddy.hip"]
 HEADERS = ["wc_device.h", "wc_json.h", "wc_kernels.h", "wc_math.h", "wc_run.h", "wc_stage.h",
            "wc_tile16.h"]
@@ -43,20 +43,41 @@ def hipcc() -> str:
     return "hipcc"
 
 
-def build(verbose: bool = False) -> str:
-    cmd = [hipcc(), *FLAGS, "-o", LIB + ".tmp", *[os.path.join(CSRC, s) for s in SOURCES]]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libwavecore_hip.so")
     # The host pass does not know the packed-FP32 feature flag and says so; nothing else is
     # expected on stderr.
     noise = [ln for ln in res.stderr.splitlines()
              if ln.strip() and "packed-fp32-ops" not in ln and "warning generated" not in ln]
-    if noise and verbose:
-        sys.stderr.write("\n".join(noise) + "\n")
+    return res, noise
+
+
+def build(verbose: bool = False) -> str:
+    """Each source compiled to an object on its own (in parallel), then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    flags = [f for f in FLAGS if f != "-shared"]
+    jobs = []
+    for src in SOURCES:
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        jobs.append((obj, [hipcc(), *flags, "-c", "-o", obj, os.path.join(CSRC, src)]))
+    workers = max(1, min(len(jobs), os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(workers) as ex:
+        results = list(ex.map(lambda j: _run(j[1], verbose), jobs))
+    for (obj, _), (res, noise) in zip(jobs, results):
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"hipcc failed compiling {os.path.basename(obj)}")
+        if noise and verbose:
+            sys.stderr.write("\n".join(noise) + "\n")
+    res, noise = _run([hipcc(), *FLAGS, "-o", LIB + ".tmp", *[o for o, _ in jobs]], verbose)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed linking libwavecore_hip.so")
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

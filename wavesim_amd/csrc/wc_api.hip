@@ -69,13 +69,14 @@ enum Family {
   F_NET,
   F_FRAME,
   F_CG_FLUSH,
+  F_CG_PAIR,
   F_COUNT
 };
 const char* kFamilyNames[F_COUNT] = {"fill_state", "switch",    "flux",      "finish_euler",
                                      "sgn_setup",  "cg_init",   "cg_matvec", "cg_update",
                                      "cg_pupdate", "cg_stats",  "sgn_finish", "reduce",
                                      "cg_reduce",  "relax",     "cg_iter",   "net",
-                                     "frame",      "cg_flush"};
+                                     "frame",      "cg_flush",  "cg_pair"};
 
 struct HostSponge {
   uint64_t width;
@@ -222,7 +223,8 @@ struct wc_solver {
   double* redf_host = nullptr;
 
   // profiling: prof times every launch with its own pair of events; prof_batch times only the
-  // one-kernel PCG iterations, one pair of events around each batch of back-to-back launches
+  // PCG iterations (the FP32 form's one kernel, the FP64 path's pair), one pair of events around
+  // each batch of back-to-back launches
   bool prof = false, prof_batch = false;
   double prof_ms[F_COUNT] = {};
   unsigned long long prof_n[F_COUNT] = {};
@@ -582,13 +584,77 @@ int pcg3(wc_solver* s, int stage, CgV<V> a, V* pA_x, V* pA_y, V* pB_x, V* pB_y) 
   return WC_OK;
 }
 
+// The FP64 path's default PCG loop (wc_pcg2.hip): 64x16 tiles, two kernels per iteration,
+// batches of iterations between looks at the device-side flag, a one-wave check closing every
+// batch. p alternates between the two buffers (A, B) by iteration parity: M_n reads p_{n-2}
+// where neighbouring tiles still read it while it writes p_{n-1}.
+int pcg2(wc_solver* s, int stage) {
+  CgV<double> a{};
+  a.g = s->geo;
+  a.h = stage == 0 ? s->h : s->h1;
+  a.c = s->c;
+  a.bx = s->bx;
+  a.by = s->by;
+  a.wx = s->wx;
+  a.wy = s->wy;
+  a.rx = s->rx;
+  a.ry = s->ry;
+  a.zx = s->zx;
+  a.zy = s->zy;
+  a.acc_init = s->lanes;
+  a.acc_pap = s->lanes + 3 * 256;
+  a.acc_update = s->lanes + 4 * 256;
+  a.sc = s->sc;
+  a.tolerance = s->disp.tolerance;
+  a.max_iterations = s->disp.max_iterations;
+  double *pA_x = s->px, *pA_y = s->py, *pB_x = s->qx, *pB_y = s->qy;
+  const dim3 grid(256), block(1024);
+  TRY(launch(s, F_CG_INIT, k_cg2_init, grid, block, a));
+  const unsigned long long maxit = s->disp.max_iterations;
+  unsigned long long n = 1;
+  unsigned long long batch = std::max<unsigned long long>(s->prev_it[stage], 1);
+  while (true) {
+    const unsigned long long stop = std::min(n + batch, maxit + 1);
+    hipEvent_t b0 = nullptr, b1 = nullptr;
+    if (s->prof_batch) {
+      b0 = take_event(s);
+      b1 = take_event(s);
+      if (b0) HIP_OK(hipEventRecord(b0, s->stream));
+    }
+    const unsigned long long first = n;
+    for (; n < stop; ++n) {
+      double* cur_x = (n % 2 == 1) ? pA_x : pB_x;
+      double* cur_y = (n % 2 == 1) ? pA_y : pB_y;
+      const double* old_x = (n % 2 == 1) ? pB_x : pA_x;
+      const double* old_y = (n % 2 == 1) ? pB_y : pA_y;
+      TRY(launch(s, F_CG_MATVEC, k_cg2_matvec, grid, block, a, n, old_x, old_y, cur_x, cur_y));
+      TRY(launch(s, F_CG_UPDATE, k_cg2_update, grid, block, a, n, (const double*)cur_x,
+                 (const double*)cur_y));
+    }
+    if (s->prof_batch && b0 && b1) {
+      HIP_OK(hipEventRecord(b1, s->stream));
+      s->ev_pending.push_back({F_CG_PAIR, b0, b1, n - first});
+    }
+    TRY(launch(s, F_CG_REDUCE, k_cg2_check, dim3(1), dim3(64), a, n));
+    HIP_OK(hipMemcpyAsync(s->sc_host, s->sc, sizeof(CgScalars), hipMemcpyDeviceToHost,
+                          s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    if (s->sc_host->done || n > maxit) break;
+    batch = s->pcg_batch;
+  }
+  s->prev_it[stage] = s->sc_host->it;
+  TRY(launch(s, F_CG_STATS, k_cg_stats, dim3(1), dim3(64), s->sc, 0ull));
+  return WC_OK;
+}
+
 // The PCG loop (dispersion.rs:493-545). Iterations run in batches; every kernel of an
 // iteration returns at once when the device-side flag says the solve is over, so the host
 // only looks at the flag between batches. The first batch is as long as the last solve.
 int pcg(wc_solver* s, int stage) {
+  // 0 (default), 4 (the FP32 form's one-kernel value) and 5: the 64x16-tile kernels
+  if (s->pcg_kernels == 0 || s->pcg_kernels >= 4) return pcg2(s, stage);
   const SgnArgs a = sgn_args(s, stage);
-  // 0 (default) and 4 (one kernel per iteration, the FP32 form's) run the 256-lane form here
-  if (s->pcg_kernels == 0 || s->pcg_kernels >= 3) {
+  if (s->pcg_kernels == 3) {
     CgV<double> v{};
     v.g = s->geo;
     v.h = a.h;
@@ -1611,7 +1677,7 @@ int wc_warm_start_weights(int32_t order, int32_t window, double* out) {
 int wc_solver_set_option(wc_solver* s, const char* key, int64_t value) {
   if (!key) return fail(WC_EINVAL, "null option name");
   if (strcmp(key, "pcg_kernels") == 0) {
-    if (value < 1 || value > 4) return fail(WC_EINVAL, "pcg_kernels must be 1, 2, 3 or 4");
+    if (value < 1 || value > 5) return fail(WC_EINVAL, "pcg_kernels must be 1, 2, 3, 4 or 5");
     s->pcg_kernels = (int)value;
     return WC_OK;
   }
@@ -2805,6 +2871,14 @@ struct wc_batch {
   hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
   unsigned long long graph_k[2] = {~0ull, ~0ull};
   unsigned long long graph_builds = 0;
+
+  // profiling (option "profile"): each PCG iteration launch between a pair of events, the
+  // stretches issued directly instead of from the graphs
+  bool prof = false;
+  double prof_ms = 0.0;
+  unsigned long long prof_n = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_pending;
+  std::vector<hipEvent_t> prof_pool;
 };
 
 namespace {
@@ -2998,6 +3072,45 @@ void enq_stage_head(wc_batch* b, int st) {  // the switch, the rates, the SGN se
                        b->stream, b->sgn[st], b->m_tiles);
 }
 
+// A PCG iteration launch, timed between two events when the batch profiles.
+template <class Launch>
+void timed_iter(wc_batch* b, Launch fn) {
+  hipEvent_t e[2] = {nullptr, nullptr};
+  if (b->prof)
+    for (hipEvent_t& x : e) {
+      if (!b->prof_pool.empty()) {
+        x = b->prof_pool.back();
+        b->prof_pool.pop_back();
+      } else if (hipEventCreate(&x) != hipSuccess) {
+        x = nullptr;
+      }
+    }
+  if (e[0] && e[1]) (void)hipEventRecord(e[0], b->stream);
+  fn();
+  if (e[0] && e[1]) {
+    (void)hipEventRecord(e[1], b->stream);
+    b->prof_pending.push_back({e[0], e[1]});
+  } else {
+    for (hipEvent_t x : e)
+      if (x) b->prof_pool.push_back(x);
+  }
+}
+
+int batch_prof_collect(wc_batch* b) {
+  if (b->prof_pending.empty()) return WC_OK;
+  HIP_OK(hipStreamSynchronize(b->stream));
+  for (auto& pe : b->prof_pending) {
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, pe.first, pe.second));
+    b->prof_ms += ms;
+    b->prof_n += 1;
+    b->prof_pool.push_back(pe.first);
+    b->prof_pool.push_back(pe.second);
+  }
+  b->prof_pending.clear();
+  return WC_OK;
+}
+
 template <int L>
 void enq_pcg_head_l(wc_batch* b, int st, unsigned long long k) {
   const dim3 gp = bgrid(b->m_cg.total, b->grid_pt);
@@ -3005,13 +3118,18 @@ void enq_pcg_head_l(wc_batch* b, int st, unsigned long long k) {
     hipLaunchKernelGGL(kbm_cg1_init<L>, gp, dim3(1024), 0, b->stream, b->cg[st], b->m_cg);
     hipLaunchKernelGGL(kbm_cg1_start<L>, gp, dim3(1024), 0, b->stream, b->cg[st], b->m_cg);
     for (unsigned long long it = 1; it <= k; ++it)
-      hipLaunchKernelGGL(kbm_cg1_iter<L>, gp, dim3(1024), 0, b->stream, b->cg[st], it, b->m_cg);
+      timed_iter(b, [&] {
+        hipLaunchKernelGGL(kbm_cg1_iter<L>, gp, dim3(1024), 0, b->stream, b->cg[st], it,
+                           b->m_cg);
+      });
     return;
   }
   hipLaunchKernelGGL(kb_cg1_init<L>, gp, dim3(1024), 0, b->stream, b->cg[st], b->m_cg);
   hipLaunchKernelGGL(kb_cg1_start<L>, gp, dim3(1024), 0, b->stream, b->cg[st], b->m_cg);
   for (unsigned long long it = 1; it <= k; ++it)
-    hipLaunchKernelGGL(kb_cg1_iter<L>, gp, dim3(1024), 0, b->stream, b->cg[st], it, b->m_cg);
+    timed_iter(b, [&] {
+      hipLaunchKernelGGL(kb_cg1_iter<L>, gp, dim3(1024), 0, b->stream, b->cg[st], it, b->m_cg);
+    });
 }
 
 void enq_pcg_head(wc_batch* b, int st, unsigned long long k) {  // init, start, iterations 1..k
@@ -3021,7 +3139,7 @@ void enq_pcg_head(wc_batch* b, int st, unsigned long long k) {  // init, start, 
     enq_pcg_head_l<256>(b, st, k);
 }
 
-void enq_iter(wc_batch* b, int st, unsigned long long it) {
+void enq_iter_untimed(wc_batch* b, int st, unsigned long long it) {
   const dim3 gp = bgrid(b->m_cg.total, b->grid_pt);
   if (b->march) {
     if (b->lanes == 512)
@@ -3034,6 +3152,10 @@ void enq_iter(wc_batch* b, int st, unsigned long long it) {
     hipLaunchKernelGGL(kb_cg1_iter<512>, gp, dim3(1024), 0, b->stream, b->cg[st], it, b->m_cg);
   else
     hipLaunchKernelGGL(kb_cg1_iter<256>, gp, dim3(1024), 0, b->stream, b->cg[st], it, b->m_cg);
+}
+
+void enq_iter(wc_batch* b, int st, unsigned long long it) {
+  timed_iter(b, [&] { enq_iter_untimed(b, st, it); });
 }
 
 void enq_check(wc_batch* b, int st, unsigned long long it, int stats) {
@@ -3119,7 +3241,7 @@ void enq_segment(wc_batch* b, int seg, const unsigned long long* k) {
 }
 
 int run_segment(wc_batch* b, int seg, const unsigned long long* k) {
-  if (!b->use_graph) {
+  if (!b->use_graph || b->prof) {
     enq_segment(b, seg, k);
     HIP_OK(hipGetLastError());
     return WC_OK;
@@ -3303,6 +3425,11 @@ void wc_batch_free(wc_batch* b) {
   (void)hipSetDevice(b->device);
   (void)hipStreamSynchronize(b->stream);
   drop_graphs(b);
+  for (auto& pe : b->prof_pending) {
+    (void)hipEventDestroy(pe.first);
+    (void)hipEventDestroy(pe.second);
+  }
+  for (hipEvent_t e : b->prof_pool) (void)hipEventDestroy(e);
   for (void* p : b->owned) (void)hipFree(p);
   for (void* p : b->owned_host) (void)hipHostFree(p);
   (void)hipStreamDestroy(b->stream);
@@ -3339,7 +3466,26 @@ int wc_batch_set_option(wc_batch* b, const char* key, int64_t value) {
     b->use_graph = value != 0;
     return WC_OK;
   }
+  if (k == "profile") {  // 1: time every PCG iteration launch (counters reset), 0: stop
+    if (value != 0 && value != 1) return fail(WC_EINVAL, "profile must be 0 or 1");
+    TRY(batch_prof_collect(b));
+    b->prof = value == 1;
+    if (b->prof) {
+      b->prof_ms = 0.0;
+      b->prof_n = 0;
+    }
+    return WC_OK;
+  }
   return fail(WC_EINVAL, "unknown batch option: " + k);
+}
+
+int wc_batch_profile_read(wc_batch* b, double* total_ms, uint64_t* launches) {
+  if (!b || !total_ms || !launches) return fail(WC_EINVAL, "null argument");
+  HIP_OK(hipSetDevice(b->device));
+  TRY(batch_prof_collect(b));
+  *total_ms = b->prof_ms;
+  *launches = b->prof_n;
+  return WC_OK;
 }
 
 int wc_batch_launch_widths(const wc_batch* b, int32_t* grid_nt, int32_t* grid_pt) {

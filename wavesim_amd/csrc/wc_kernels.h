@@ -242,6 +242,15 @@ __global__ void k_cg3_update(CgV<V> a, unsigned long long n, const V* pcx, const
 template <typename V>
 __global__ void k_cg3_check(CgV<V> a, unsigned long long n);
 
+// The FP64 path's default PCG (wc_pcg2.hip): 64x16 tiles, 256 blocks, the oracle's dot order 2.
+__global__ void k_cg2_init(CgV<double> a);
+__global__ void k_cg2_matvec(CgV<double> a, unsigned long long n, const double* __restrict__ pox,
+                             const double* __restrict__ poy, double* __restrict__ pnx,
+                             double* __restrict__ pny);
+__global__ void k_cg2_update(CgV<double> a, unsigned long long n, const double* __restrict__ pcx,
+                             const double* __restrict__ pcy);
+__global__ void k_cg2_check(CgV<double> a, unsigned long long n);
+
 // The FP32 chosen form (wc_fp32.hip).
 
 // The SGN solution of the FP32 form is kept in FP64 (solx, soly: the warm start, shared with the

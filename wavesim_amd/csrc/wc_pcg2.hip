@@ -1,0 +1,391 @@
+// The FP64 path's PCG (dispersion.rs:493-545): the reference's iterates bit for bit, two
+// kernels per iteration over 64x16-cell tiles.
+//
+// The reference's loop needs two grid-wide reductions per iteration (p.Ap before the update,
+// r.r and r.z after it), so an iteration is two launches, each a pass over the grid:
+//   M_n: beta_{n-1} from U_{n-1}'s lanes (or the start's); p_{n-1} = z_{n-1} + beta p_{n-2}
+//        (p_0 = z_0) on the tile + 2, A p_{n-1}, lanes of p.Ap; p_{n-1} stored;
+//   U_n: alpha_n = r.z / p.Ap; p_{n-1} staged again on the tile + 2 and A p_{n-1} computed
+//        again (the same arithmetic, so the same bits) instead of being stored by M_n and read
+//        back; w += alpha p; r -= alpha A p; z = r / diag; lanes of r.r and r.z.
+// Recomputing A p in U_n saves its store and its load (32 of 208 bytes per cell and
+// iteration); the two halo stencils run from LDS and L2. Per cell and iteration the passes
+// move z, p_old (read), p (written), c, h in M_n and p, c, h, r, w (read), w, r, z (written)
+// in U_n: 176 bytes.
+//
+// Layout: 64x16 tiles (one 1024-thread block, a cell per thread; the two-cell halo is a third of
+// the tile where a 64x4 tile's is 113%), exactly 256 blocks, block b taking tiles b, b+256, ...
+// The inputs of the next tile (its halo, the cell's own values) are loaded into registers while
+// the current tile computes. Each thread keeps running sums over its cells, and the block
+// reduces them once, at the end: rows by the canonical butterfly, then in row order. That
+// register is lane b of the final stage, which every wave of the next kernel finishes itself,
+// so no separate reduction kernel, barrier or atomic is needed. The oracle's dot order 2 is this
+// order (oracle/wavecore_oracle.cpp: canonical16_sum).
+//
+// Block b runs on XCD b % 8; with 64-cell tile columns a multiple of 8 wide, vertically
+// adjacent tiles (b and b + tiles_x) sit on one XCD and share its L2 for their halo rows.
+
+#include "wc_tile16.h"
+
+namespace wc {
+
+namespace {
+
+using namespace t16;
+
+// The scalars iteration n starts from (M_n's prologue and the check kernel); false when the
+// solve is over. Every wave computes the same numbers; block 0 records them.
+__device__ __forceinline__ bool start(const CgV<double>& a, unsigned long long n, double& beta) {
+  CgScalars* sc = a.sc;
+  bool done;
+  double rz;
+  if (n == 1) {
+    double s[3];
+    lane_sums<3>(a.acc_init, s);
+    const double target = a.tolerance * sqrt(s[0]) + 1e-10;
+    rz = s[1];
+    const int conv = sqrt(s[2]) <= target ? 1 : 0;
+    done = conv || a.max_iterations == 0;
+    beta = 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->target = target;
+      sc->converged = conv;
+      sc->it = 0;
+    }
+  } else {
+    // A solve that ended in an earlier launch stays as it ended: later launches write nothing.
+    // (Block 0 of this launch may already have set the flag; a wave that reads it would have
+    // reached the same verdict from the lanes.)
+    if (sc->done) return false;
+    double s[2];
+    lane_sums<2>(a.acc_update, s);
+    const int conv = sqrt(s[0]) <= sc->target ? 1 : 0;
+    const unsigned long long it = n - 1;
+    done = conv || it >= a.max_iterations;
+    beta = s[1] / sc->rzs[(n - 1) & 1];
+    rz = s[1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->converged = conv;
+      sc->it = it;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->rzs[n & 1] = rz;
+    sc->done = done ? 1 : 0;
+  }
+  return !done;
+}
+
+// Ring element t of a tile + 2 halo: the mapped interior index m and flags (1: inside the
+// grid's two rings; 2, 4: the x, y component flips at a wall).
+__device__ __forceinline__ int ring2(const Geo& g, int tile, int t, int& m) {
+  const int tx = tiles_x(g);
+  const int i = GH + (tile % tx) * BX - 2 + t % S2X;
+  const int j = GH + (tile / tx) * BY - 2 + t / S2X;
+  if (i > g.nx + GH + 1 || j > g.ny + GH + 1) return 0;
+  m = map_y(j, g.ny, g.periodic) * g.w + map_x(i, g.nx);
+  return 1 | (is_ghost_x(i, g.nx) ? 2 : 0) | (!g.periodic && is_ghost_y(j, g.ny) ? 4 : 0);
+}
+
+// Ring element t of a tile + 1 halo: its padded index, or -1 outside the used rows and
+// columns (where f is zero).
+__device__ __forceinline__ int ring1(const Geo& g, int tile, int t) {
+  const int tx = tiles_x(g);
+  const int i = GH + (tile % tx) * BX - 1 + t % S1X;
+  const int j = GH + (tile / tx) * BY - 1 + t / S1X;
+  if (i > g.nx + GH || j > g.ny + GH) return -1;
+  return j * g.w + i;
+}
+
+// The thread's own cell of a tile: its padded index, or -1 outside the interior.
+__device__ __forceinline__ int own_cell(const Geo& g, int tile) {
+  const int tx = tiles_x(g);
+  const int i = GH + (tile % tx) * BX + (threadIdx.x & 63);
+  const int j = GH + (tile / tx) * BY + (threadIdx.x >> 6);
+  return (i < g.nx + GH && j < g.ny + GH) ? j * g.w + i : -1;
+}
+
+// What a thread loads for one tile, a tile ahead: NV values at each of its (at most two) tile + 2
+// halo points, c at its (at most two) tile + 1 points, and NO values at its own cell.
+template <int NV, int NO>
+struct Pre {
+  double v[2][NV];
+  int flags[2];
+  double c[2];
+  double o[NO];
+  int k;  // own cell, -1 outside
+};
+
+template <int NV, int NO, class Halo, class Own>
+__device__ __forceinline__ void prefetch(const CgV<double>& a, int tile, Halo halo, Own own,
+                                         Pre<NV, NO>& P) {
+  const Geo& g = a.g;
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int t = threadIdx.x + e * PT;
+    P.flags[e] = 0;
+    int m = 0;
+    if (t < H2 && (P.flags[e] = ring2(g, tile, t, m)) != 0) halo(m, P.v[e]);
+    P.c[e] = 0.0;
+    if (t < H1) {
+      const int q = ring1(g, tile, t);
+      if (q >= 0) P.c[e] = a.c[q];
+    }
+  }
+  P.k = own_cell(g, tile);
+  if (P.k >= 0) own(P.k, P.o);
+}
+
+// The vector on the tile + 2 from the prefetched halo values by make(v, vx, vy), the wall flips
+// applied; c on the tile + 1.
+template <int NV, int NO, class Make>
+__device__ __forceinline__ void stage(const Pre<NV, NO>& P, double (*s_u)[S2Y][S2X],
+                                      double (*s_c)[S1X], Make make) {
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int t = threadIdx.x + e * PT;
+    if (t < H2) {
+      double vx = 0.0, vy = 0.0;
+      if (P.flags[e] & 1) {
+        make(P.v[e], vx, vy);
+        if (P.flags[e] & 2) vx = -vx;
+        if (P.flags[e] & 4) vy = -vy;
+      }
+      s_u[0][t / S2X][t % S2X] = vx;
+      s_u[1][t / S2X][t % S2X] = vy;
+    }
+    if (t < H1) s_c[t / S1X][t % S1X] = P.c[e];
+  }
+}
+
+// f = c (d/dx px + d/dy py) on the tile + 1 (dispersion.rs:283-296), then a barrier; outside the
+// used rows and columns f is zero (c is staged as zero there).
+__device__ __forceinline__ void stencil_f(const Geo& g, double (*s_u)[S2Y][S2X],
+                                          double (*s_c)[S1X], double (*s_f)[S1X]) {
+  const double ix = 0.5 / g.dx, iy = 0.5 / g.dy;
+  for (int t = threadIdx.x; t < H1; t += PT) {
+    const int lx = t % S1X, ly = t / S1X;
+    const int x = lx + 1, y = ly + 1;
+    s_f[ly][lx] = s_c[ly][lx] * ((s_u[0][y][x + 1] - s_u[0][y][x - 1]) * ix +
+                                 (s_u[1][y + 1][x] - s_u[1][y - 1][x]) * iy);
+  }
+  __syncthreads();
+}
+
+// A p at the thread's own cell (dispersion.rs:298-316), p there, from the staged p and f.
+__device__ __forceinline__ void apply_own(const Geo& g, double h, double (*s_u)[S2Y][S2X],
+                                          double (*s_f)[S1X], double& px, double& py,
+                                          double& ax, double& ay) {
+  const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+  const double ix = 0.5 / g.dx, iy = 0.5 / g.dy;
+  const double mask = h >= WC_H_DISPERSION_MIN ? 1.0 : 0.0;
+  const double heff = wc_max(h, WC_H_DRY);
+  const int x1 = lx + 1, y1 = ly + 1;
+  px = s_u[0][ly + 2][lx + 2];
+  py = s_u[1][ly + 2][lx + 2];
+  ax = mask * (heff * px - (s_f[y1][x1 + 1] - s_f[y1][x1 - 1]) * ix);
+  ay = mask * (heff * py - (s_f[y1 + 1][x1] - s_f[y1 - 1][x1]) * iy);
+}
+
+// The Jacobi diagonal (dispersion.rs:466-480) at the own cell, c of its neighbours from the
+// staged tile + 1.
+__device__ __forceinline__ void own_diag(const Geo& g, double h, double (*s_c)[S1X], double& dgx,
+                                         double& dgy) {
+  const int x = (threadIdx.x & 63) + 1, y = (threadIdx.x >> 6) + 1;
+  if (!(h >= WC_H_DISPERSION_MIN)) {
+    dgx = 1.0;
+    dgy = 1.0;
+    return;
+  }
+  const double heff = wc_max(h, WC_H_DRY);
+  dgx = heff + (s_c[y][x + 1] + s_c[y][x - 1]) * 0.25 / (g.dx * g.dx);
+  dgy = heff + (s_c[y + 1][x] + s_c[y - 1][x]) * 0.25 / (g.dy * g.dy);
+}
+
+}  // namespace
+
+// r = b - A w, z = r / diag; lanes of b.b, r.z, r.r (dispersion.rs:493-514; p_0 = z_0 is
+// staged by M_1 from z).
+__global__ void __launch_bounds__(PT) k_cg2_init(CgV<double> a) {
+  const Geo& g = a.g;
+  const int nt = tiles_all(g);
+  __shared__ double s_u[2][S2Y][S2X], s_c[S1Y][S1X], s_f[S1Y][S1X];
+  __shared__ double sums[1][BY][3];
+  double acc[3] = {0.0, 0.0, 0.0};
+  auto halo = [&](int m, double (&v)[2]) {
+    v[0] = a.wx[m];
+    v[1] = a.wy[m];
+  };
+  auto own = [&](int k, double (&o)[3]) {
+    o[0] = a.h[k];
+    o[1] = a.bx[k];
+    o[2] = a.by[k];
+  };
+  Pre<2, 3> P;
+  if ((int)blockIdx.x < nt) prefetch(a, blockIdx.x, halo, own, P);
+  for (int tile = blockIdx.x; tile < nt; tile += LANES) {
+    stage(P, s_u, s_c, [](const double (&v)[2], double& vx, double& vy) {
+      vx = v[0];
+      vy = v[1];
+    });
+    const int k = P.k;
+    const double h = P.o[0], bx = P.o[1], by = P.o[2];
+    __syncthreads();
+    if (tile + LANES < nt) prefetch(a, tile + LANES, halo, own, P);
+    stencil_f(g, s_u, s_c, s_f);
+    double v[3] = {0.0, 0.0, 0.0};
+    if (k >= 0) {
+      double wx, wy, ax, ay;
+      apply_own(g, h, s_u, s_f, wx, wy, ax, ay);
+      const double rx = bx - ax, ry = by - ay;
+      double dgx, dgy;
+      own_diag(g, h, s_c, dgx, dgy);
+      const double zx = rx / dgx, zy = ry / dgy;
+      a.rx[k] = rx;
+      a.ry[k] = ry;
+      a.zx[k] = zx;
+      a.zy[k] = zy;
+      v[0] = bx * bx + by * by;
+      v[1] = rx * zx + ry * zy;
+      v[2] = rx * rx + ry * ry;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) acc[q] = acc[q] + v[q];
+    __syncthreads();
+  }
+  rows_total<3>(acc, sums);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 3; ++q) a.acc_init[q * LANES + blockIdx.x] = acc[q];
+}
+
+// M_n: p_{n-1} = z + beta p_{n-2} (p_0 = z) on the tile + 2, stored at the tile's cells; A p;
+// lanes of p.Ap (dispersion.rs:516-518, 537-539).
+__global__ void __launch_bounds__(PT) k_cg2_matvec(CgV<double> a, unsigned long long n,
+                                                   const double* __restrict__ pox,
+                                                   const double* __restrict__ poy,
+                                                   double* __restrict__ pnx,
+                                                   double* __restrict__ pny) {
+  const Geo& g = a.g;
+  const int nt = tiles_all(g);
+  const bool first = n == 1;
+  auto halo = [&](int m, double (&v)[4]) {
+    v[0] = a.zx[m];
+    v[1] = a.zy[m];
+    v[2] = first ? 0.0 : pox[m];
+    v[3] = first ? 0.0 : poy[m];
+  };
+  auto own = [&](int k, double (&o)[1]) { o[0] = a.h[k]; };
+  Pre<4, 1> P;
+  // the first tile's loads are in flight while the scalars are reduced
+  if ((int)blockIdx.x < nt) prefetch(a, blockIdx.x, halo, own, P);
+  double beta = 0.0;
+  if ((int)blockIdx.x < nt && !start(a, n, beta)) return;
+  __shared__ double s_u[2][S2Y][S2X], s_c[S1Y][S1X], s_f[S1Y][S1X];
+  __shared__ double sums[1][BY][1];
+  double acc[1] = {0.0};
+  for (int tile = blockIdx.x; tile < nt; tile += LANES) {
+    stage(P, s_u, s_c, [&](const double (&v)[4], double& vx, double& vy) {
+      if (first) {
+        vx = v[0];
+        vy = v[1];
+      } else {
+        vx = v[0] + beta * v[2];
+        vy = v[1] + beta * v[3];
+      }
+    });
+    const int k = P.k;
+    const double h = P.o[0];
+    __syncthreads();
+    if (tile + LANES < nt) prefetch(a, tile + LANES, halo, own, P);
+    stencil_f(g, s_u, s_c, s_f);
+    double v = 0.0;
+    if (k >= 0) {
+      double px, py, ax, ay;
+      apply_own(g, h, s_u, s_f, px, py, ax, ay);
+      pnx[k] = px;
+      pny[k] = py;
+      v = px * ax + py * ay;
+    }
+    acc[0] = acc[0] + v;
+    __syncthreads();
+  }
+  rows_total<1>(acc, sums);
+  if (threadIdx.x == 0) a.acc_pap[blockIdx.x] = acc[0];
+}
+
+// U_n: alpha = r.z / p.Ap; A p again from p_{n-1} on the tile + 2; w += alpha p, r -= alpha A p,
+// z = r / diag (dispersion.rs:519-526, 568-589); lanes of r.r and r.z.
+__global__ void __launch_bounds__(PT) k_cg2_update(CgV<double> a, unsigned long long n,
+                                                   const double* __restrict__ pcx,
+                                                   const double* __restrict__ pcy) {
+  CgScalars* sc = a.sc;
+  const Geo& g = a.g;
+  const int nt = tiles_all(g);
+  auto halo = [&](int m, double (&v)[2]) {
+    v[0] = pcx[m];
+    v[1] = pcy[m];
+  };
+  auto own = [&](int k, double (&o)[5]) {
+    o[0] = a.h[k];
+    o[1] = a.wx[k];
+    o[2] = a.wy[k];
+    o[3] = a.rx[k];
+    o[4] = a.ry[k];
+  };
+  Pre<2, 5> P;
+  if ((int)blockIdx.x < nt) prefetch(a, blockIdx.x, halo, own, P);
+  double alpha = 0.0;
+  if ((int)blockIdx.x < nt) {
+    if (sc->done) return;
+    double s[1];
+    lane_sums<1>(a.acc_pap, s);
+    alpha = sc->rzs[n & 1] / s[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha = alpha;
+  }
+  const double na = -alpha;
+  __shared__ double s_u[2][S2Y][S2X], s_c[S1Y][S1X], s_f[S1Y][S1X];
+  __shared__ double sums[1][BY][2];
+  double acc[2] = {0.0, 0.0};
+  for (int tile = blockIdx.x; tile < nt; tile += LANES) {
+    stage(P, s_u, s_c, [](const double (&v)[2], double& vx, double& vy) {
+      vx = v[0];
+      vy = v[1];
+    });
+    const int k = P.k;
+    const double h = P.o[0], wx0 = P.o[1], wy0 = P.o[2], rx0 = P.o[3], ry0 = P.o[4];
+    __syncthreads();
+    if (tile + LANES < nt) prefetch(a, tile + LANES, halo, own, P);
+    stencil_f(g, s_u, s_c, s_f);
+    double v[2] = {0.0, 0.0};
+    if (k >= 0) {
+      double px, py, ax, ay;
+      apply_own(g, h, s_u, s_f, px, py, ax, ay);
+      a.wx[k] = wx0 + alpha * px;
+      a.wy[k] = wy0 + alpha * py;
+      const double rx = rx0 + na * ax, ry = ry0 + na * ay;
+      double dgx, dgy;
+      own_diag(g, h, s_c, dgx, dgy);
+      const double zx = rx / dgx, zy = ry / dgy;
+      a.rx[k] = rx;
+      a.ry[k] = ry;
+      a.zx[k] = zx;
+      a.zy[k] = zy;
+      v[0] = rx * rx + ry * ry;
+      v[1] = rx * zx + ry * zy;
+    }
+    acc[0] = acc[0] + v[0];
+    acc[1] = acc[1] + v[1];
+    __syncthreads();
+  }
+  rows_total<2>(acc, sums);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 2; ++q) a.acc_update[q * LANES + blockIdx.x] = acc[q];
+}
+
+// M_n's prologue alone, one wave: records whether the solve is over after iteration n - 1.
+__global__ void __launch_bounds__(64) k_cg2_check(CgV<double> a, unsigned long long n) {
+  double beta;
+  start(a, n, beta);
+}
+
+}  // namespace wc

@@ -115,6 +115,8 @@ SIGNATURES = {
     "wc_batch_free": (None, [_VP]),
     "wc_batch_set_option": (_I32, [_VP, ctypes.c_char_p, ctypes.c_int64]),
     "wc_batch_launch_widths": (_I32, [_VP, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
+    "wc_batch_profile_read": (_I32, [_VP, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(_U64)]),
     "wc_batch_run_steps": (_I32, [_VP, _U64, ctypes.POINTER(_U64), _P]),
     "wc_profile_enable": (_I32, [_VP, _I32]),
     "wc_profile_reset": (_I32, [_VP]),
@@ -409,8 +411,8 @@ class Solver:
 
     # -- measurement ---------------------------------------------------------------------------
     def profile(self, mode: int) -> None:
-        """0: off; 1 (True): events around every launch; 2: events around each batch of the
-        FP32 form's one-kernel PCG iterations only."""
+        """0: off; 1 (True): events around every launch; 2: events around each batch of PCG
+        iterations only (the FP32 form's one kernel: "cg_iter"; the FP64 path's two: "cg_pair")."""
         _check(lib().wc_profile_enable(self._h, int(mode)))
 
     def profile_reset(self) -> None:
@@ -598,6 +600,12 @@ class Batch:
         a, b = _I32(), _I32()
         _check(lib().wc_batch_launch_widths(self._b, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def profile_read(self) -> tuple[float, int]:
+        """(milliseconds, launches) of the PCG iteration launches since set_option("profile", 1)."""
+        ms, n = ctypes.c_double(), _U64()
+        _check(lib().wc_batch_profile_read(self._b, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, int(n.value)
 
     def run_steps(self, states, steps: int):
         if len(states) != len(self.solvers):
