@@ -171,6 +171,17 @@ int wc_state_set_time(wc_solver* s, double time);
 /* Solver::stable_dt (solver.rs:434-450): cfl / max rate, +inf if nothing moves, NaN if any
  * cell is NaN, 0 if any moves infinitely fast. */
 int wc_stable_dt(wc_solver* s, double* dt);
+/* wc_stable_dt's value for a padded FP64 state (h, hu, hv: (nx+6)(ny+6) doubles each) that the
+ * caller keeps in device memory of the current device, written to dt_out (device memory) by
+ * work enqueued on `stream` (a hipStream_t, NULL for the default stream); the call does not
+ * wait for it. No counterpart in the reference: for callers that cannot allocate (code under
+ * HIP graph capture, serving loops, harnesses that forbid it), it allocates nothing and keeps
+ * its partial results in a fixed 128 KiB device array per GPU, held from its first call on that
+ * GPU until the process ends. Limit: nx * ny <= 67,108,864 interior cells (WC_EINVAL beyond).
+ * Calls that overlap on one GPU (other streams, other threads) race for that array. */
+int wc_stable_dt_noalloc(const double* h, const double* hu, const double* hv, uint64_t nx,
+                         uint64_t ny, double dx, double dy, double cfl, double* dt_out,
+                         void* stream);
 /* Solver::step (solver.rs:481-507): one SSP-RK2 step (Euler for Order::First) of size dt,
  * with sponges and friction, advancing the time by dt. */
 int wc_step(wc_solver* s, double dt);

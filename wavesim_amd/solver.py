@@ -73,6 +73,8 @@ SIGNATURES = {
     "wc_state_get": (_I32, [_VP, _P, _P, _P, _P, _P]),
     "wc_state_set_time": (_I32, [_VP, _D]),
     "wc_stable_dt": (_I32, [_VP, _P]),
+    "wc_stable_dt_noalloc": (_I32, [_VP, _VP, _VP, _U64, _U64, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_double, _VP, _VP]),
     "wc_step": (_I32, [_VP, _D]),
     "wc_limiting_cell": (_I32, [_VP, ctypes.POINTER(_I32), ctypes.POINTER(CLimiting)]),
     "wc_breaking_indicator": (_I32, [_VP, _P]),
