@@ -15,8 +15,8 @@
 //
 // Layout: 64x16 tiles (one 1024-thread block, a cell per thread; the two-cell halo is a third of
 // the tile where a 64x4 tile's is 113%), exactly 256 blocks, block b taking tiles b, b+256, ...
-// The inputs of the next tile (its halo, the cell's own values) are loaded into registers while
-// the current tile computes. Each thread keeps running sums over its cells, and the block
+// The inputs of the next two tiles (their halos, the cells' own values) are loaded into
+// registers while the current tile computes. Each thread keeps running sums over its cells, and the block
 // reduces them once, at the end: rows by the canonical butterfly, then in row order. That
 // register is lane b of the final stage, which every wave of the next kernel finishes itself,
 // so no separate reduction kernel, barrier or atomic is needed. The oracle's dot order 2 is this
@@ -275,16 +275,19 @@ __global__ void __launch_bounds__(PT) k_cg2_matvec(CgV<double> a, unsigned long 
     v[3] = first ? 0.0 : poy[m];
   };
   auto own = [&](int k, double (&o)[1]) { o[0] = a.h[k]; };
-  Pre<4, 1> P;
-  // the first tile's loads are in flight while the scalars are reduced
-  if ((int)blockIdx.x < nt) prefetch(a, blockIdx.x, halo, own, P);
+  // two tiles' loads in flight: the first two while the scalars are reduced, then each tile's
+  // successor-but-one as soon as the tile is staged
+  Pre<4, 1> P[2];
+  const int t0 = blockIdx.x;
+  if (t0 < nt) prefetch(a, t0, halo, own, P[0]);
+  if (t0 + LANES < nt) prefetch(a, t0 + LANES, halo, own, P[1]);
   double beta = 0.0;
-  if ((int)blockIdx.x < nt && !start(a, n, beta)) return;
+  if (t0 < nt && !start(a, n, beta)) return;
   __shared__ double s_u[2][S2Y][S2X], s_c[S1Y][S1X], s_f[S1Y][S1X];
   __shared__ double sums[1][BY][1];
   double acc[1] = {0.0};
-  for (int tile = blockIdx.x; tile < nt; tile += LANES) {
-    stage(P, s_u, s_c, [&](const double (&v)[4], double& vx, double& vy) {
+  auto body = [&](Pre<4, 1>& Q, int tile) {
+    stage(Q, s_u, s_c, [&](const double (&v)[4], double& vx, double& vy) {
       if (first) {
         vx = v[0];
         vy = v[1];
@@ -293,10 +296,10 @@ __global__ void __launch_bounds__(PT) k_cg2_matvec(CgV<double> a, unsigned long 
         vy = v[1] + beta * v[3];
       }
     });
-    const int k = P.k;
-    const double h = P.o[0];
+    const int k = Q.k;
+    const double h = Q.o[0];
     __syncthreads();
-    if (tile + LANES < nt) prefetch(a, tile + LANES, halo, own, P);
+    if (tile + 2 * LANES < nt) prefetch(a, tile + 2 * LANES, halo, own, Q);
     stencil_f(g, s_u, s_c, s_f);
     double v = 0.0;
     if (k >= 0) {
@@ -308,6 +311,10 @@ __global__ void __launch_bounds__(PT) k_cg2_matvec(CgV<double> a, unsigned long 
     }
     acc[0] = acc[0] + v;
     __syncthreads();
+  };
+  for (int tile = t0; tile < nt; tile += 2 * LANES) {
+    body(P[0], tile);
+    if (tile + LANES < nt) body(P[1], tile + LANES);
   }
   rows_total<1>(acc, sums);
   if (threadIdx.x == 0) a.acc_pap[blockIdx.x] = acc[0];
@@ -332,10 +339,12 @@ __global__ void __launch_bounds__(PT) k_cg2_update(CgV<double> a, unsigned long 
     o[3] = a.rx[k];
     o[4] = a.ry[k];
   };
-  Pre<2, 5> P;
-  if ((int)blockIdx.x < nt) prefetch(a, blockIdx.x, halo, own, P);
+  Pre<2, 5> P[2];  // two tiles' loads in flight, as in M_n
+  const int t0 = blockIdx.x;
+  if (t0 < nt) prefetch(a, t0, halo, own, P[0]);
+  if (t0 + LANES < nt) prefetch(a, t0 + LANES, halo, own, P[1]);
   double alpha = 0.0;
-  if ((int)blockIdx.x < nt) {
+  if (t0 < nt) {
     if (sc->done) return;
     double s[1];
     lane_sums<1>(a.acc_pap, s);
@@ -346,15 +355,15 @@ __global__ void __launch_bounds__(PT) k_cg2_update(CgV<double> a, unsigned long 
   __shared__ double s_u[2][S2Y][S2X], s_c[S1Y][S1X], s_f[S1Y][S1X];
   __shared__ double sums[1][BY][2];
   double acc[2] = {0.0, 0.0};
-  for (int tile = blockIdx.x; tile < nt; tile += LANES) {
-    stage(P, s_u, s_c, [](const double (&v)[2], double& vx, double& vy) {
+  auto body = [&](Pre<2, 5>& Q, int tile) {
+    stage(Q, s_u, s_c, [](const double (&v)[2], double& vx, double& vy) {
       vx = v[0];
       vy = v[1];
     });
-    const int k = P.k;
-    const double h = P.o[0], wx0 = P.o[1], wy0 = P.o[2], rx0 = P.o[3], ry0 = P.o[4];
+    const int k = Q.k;
+    const double h = Q.o[0], wx0 = Q.o[1], wy0 = Q.o[2], rx0 = Q.o[3], ry0 = Q.o[4];
     __syncthreads();
-    if (tile + LANES < nt) prefetch(a, tile + LANES, halo, own, P);
+    if (tile + 2 * LANES < nt) prefetch(a, tile + 2 * LANES, halo, own, Q);
     stencil_f(g, s_u, s_c, s_f);
     double v[2] = {0.0, 0.0};
     if (k >= 0) {
@@ -376,6 +385,10 @@ __global__ void __launch_bounds__(PT) k_cg2_update(CgV<double> a, unsigned long 
     acc[0] = acc[0] + v[0];
     acc[1] = acc[1] + v[1];
     __syncthreads();
+  };
+  for (int tile = t0; tile < nt; tile += 2 * LANES) {
+    body(P[0], tile);
+    if (tile + LANES < nt) body(P[1], tile + LANES);
   }
   rows_total<2>(acc, sums);
   if (threadIdx.x == 0)
