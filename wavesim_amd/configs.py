@@ -199,16 +199,22 @@ def c3_reef_instance(seed: int, dx: float = 3.0, size: tuple[int, int] | None = 
     maker_x = (sponge + 4.0 * SIGMA_CELLS) * dx
     maker = WaveMaker(maker_x, height / 2.0, period, angle, SIGMA_CELLS * dx, RAMP_PERIODS)
 
+    # the side sponges at most an eighth of the sheet's width each: 1.5 wavelengths on both
+    # sides of a 384 m sheet would damp all of it, and no wave would reach the reef
+    side = min(sponge, max(ny // 8, 4))
+
     def configure(s):
         s = (s.with_order(Order.Fifth)
              .with_dispersion(Dispersion(tolerance=1e-8, max_iterations=200))
              .with_wavemaker(maker)
-             .with_sponge(Sponge(sponge, SPONGE_STRENGTH).edges(True, False, True, True)))
+             .with_sponge(Sponge(sponge, SPONGE_STRENGTH).edges(True, False, False, False))
+             .with_sponge(Sponge(side, SPONGE_STRENGTH).edges(False, False, True, True)))
         s.cfl = 0.45
         return s
 
     notes = {"seed": seed, "nx": nx, "ny": ny, "angle_deg": math.degrees(angle), "tide_m": tide,
-             "H_m": height, "T_s": period, "lobes": n_lobes, "sponge_cells": sponge}
+             "H_m": height, "T_s": period, "lobes": n_lobes, "sponge_cells": sponge,
+             "side_sponge_cells": side}
     return Workload(f"reef_sheet_{seed}_{nx}x{ny}", grid, bed, configure, 2000, notes)
 
 

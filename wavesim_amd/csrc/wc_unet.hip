@@ -739,6 +739,10 @@ int wcu_net_new(uint64_t nx, uint64_t ny, wcu_net** out) {
   n->lblocks = 1024;
   if ((rc = nalloc(n.get(), &n->lpart, 2 * (size_t)n->lblocks))) return rc;
   HIPC(hipHostMalloc(&n->lpart_host, 2 * (size_t)n->lblocks * sizeof(double), hipHostMallocDefault));
+  // The zero fills above run on the null stream, which a caller's non-blocking stream does not
+  // wait for: finish them before the network is used on any stream (the activations' zero
+  // borders are read by the first forward pass).
+  HIPC(hipDeviceSynchronize());
   *out = n.release();
   return WC_OK;
 }
