@@ -118,11 +118,6 @@ int wc_solver_with_dispersion(wc_solver* s, const wc_dispersion_t* d); /* solver
  * the switch off where they are and behind them, and get their eddy viscosity
  * (solver.rs:512-532, eddy.rs). In the FP32 form this part runs on the FP64 state. */
 int wc_solver_with_eddy_viscosity(wc_solver* s, const wc_eddy_t* e);
-/* Breaking::viscosity = Some(*e) (NULL: None), after wc_solver_with_dispersion (which resets
- * it): the breaking fronts are followed in the state's breaking array after every step, keep
- * the switch off where they are and behind them, and get their eddy viscosity
- * (solver.rs:512-532, eddy.rs). In the FP32 form this part runs on the FP64 state. */
-int wc_solver_with_eddy_viscosity(wc_solver* s, const wc_eddy_t* e);
 int wc_solver_with_periodic_y(wc_solver* s);                          /* solver.rs:414 */
 /* solver.rs:376: one byte per interior cell, row-major with y outer, nonzero = held. */
 int wc_solver_with_shallow_cells(wc_solver* s, const uint8_t* cells, uint64_t n);
@@ -132,15 +127,18 @@ int wc_solver_bed(const wc_solver* s, double* out);
  * results; in the FP32 form the PCG layout and the warm start also change the rounding, and the
  * warm start the iteration count (never the tolerance a solve meets):
  *   "pcg_kernels" 1 = three kernels and two reductions per PCG iteration; 2 = two kernels and
- *                 two reductions; 3 = two kernels, reductions folded into them (FP64 default);
+ *                 two reductions; 3 = two kernels, reductions folded into them, 64x4 tiles;
  *                 4 = one kernel per iteration, Chronopoulos-Gear recurrences (FP32 default;
- *                 FP64 runs 3 instead);
+ *                 FP64 runs 5 instead); 5 = FP64: two kernels over 64x16 tiles, the update
+ *                 recomputing A p (FP64 default);
  *   "pcg_lanes"   blocks of the one-kernel PCG: 256 (default) or 512;
  *   "pcg_march"   one-kernel PCG layout: 0 = 64x16 tiles (default), 1 = blocks marching down
  *                 64-column strips with circular LDS row buffers;
- *   "pcg_tiles_per_block" 64x16 tiles: the fewest tiles one block takes, 1..64 (default 1;
- *                 larger values give a small grid fewer, longer blocks, which pays when many
- *                 instances share a launch);
+ *   "pcg_tiles_per_block" n, 1..64: the one-kernel PCG on 64x16 tiles runs ceil(tiles / n)
+ *                 blocks, at most pcg_lanes, so a block takes about n tiles (at most n unless
+ *                 the lanes cap the block count; then more); default 1. Larger values give a
+ *                 small grid fewer, longer blocks, which pays when many instances share a
+ *                 launch. Ignored by pcg_march 1, pcg_kernels 1-3 and the network's PCG;
  *   "pcg_batch"   PCG iterations queued between looks at the convergence flag (default 4);
  *   "flux_kernel" 1 = one thread per cell, as the reference's loop; 2 = LDS tiles, each cell
  *                 reconstructed and each face fluxed once (default);

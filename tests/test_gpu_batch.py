@@ -77,7 +77,7 @@ def assert_same(alone, batched):
         assert a[3] == b[3], k
 
 
-@pytest.mark.parametrize("graph,walk", [(True, 1), (False, 1), (True, 4)])
+@pytest.mark.parametrize("graph,walk", [(True, 1), (False, 1), (True, 4), (False, 4)])
 def test_a_batch_steps_each_instance_as_it_steps_alone(graph, walk):
     """With the fixed stretches of each step replayed from HIP graphs (the default) and
     without, and with PCG blocks taking 4 tiles or more; compared after every 5 steps, so that
@@ -112,6 +112,40 @@ def test_the_launch_width_changes_nothing(widths):
     batched, got, _ = run_batched(ws, starts, 8, widths)
     assert got == widths
     assert_same(alone, batched)
+
+
+def test_a_solo_run_between_batch_runs_changes_nothing():
+    """Instances stepped alone with other PCG layouts between two batch runs (other lanes and
+    blocks write lanes the batch relies on reading zero) still step in the batch as alone; a
+    batch refuses an instance whose settings changed since it was made."""
+    ws = workloads()[:3]
+    starts = spun_up(ws, 40)
+    pairs = [fresh(w, arr, t0) for w, (arr, t0) in zip(ws, starts)]
+    solo = [fresh(w, arr, t0) for w, (arr, t0) in zip(ws, starts)]
+    b = Batch([g for g, _ in pairs])
+    assert b.run_steps([s for _, s in pairs], 3)[0] == [3] * 3
+    for (g, s), (g1, s1) in zip(pairs, solo):
+        g1.run_steps(s1, 3)
+        # (setting pcg_lanes clears the lanes itself, so it comes first and the layouts that
+        # leave other lanes written come last)
+        for opts in ({"pcg_lanes": 512}, {"pcg_lanes": 256, "pcg_march": 1},
+                     {"pcg_march": 0, "pcg_kernels": 3}):
+            for k, v in opts.items():
+                g.set_option(k, v)
+                g1.set_option(k, v)
+            g.run_steps(s, 2)
+            g1.run_steps(s1, 2)
+        g.set_option("pcg_kernels", 4)
+        g1.set_option("pcg_kernels", 4)
+    assert b.run_steps([s for _, s in pairs], 5)[0] == [5] * 3
+    for (g, s), (g1, s1) in zip(pairs, solo):
+        assert g1.run_steps(s1, 5)[0] == 5
+        assert s.time == s1.time
+        for x, y in zip(s.arrays()[:3], s1.arrays()[:3]):
+            assert np.array_equal(x, y)
+    pairs[1][0].set_option("pcg_lanes", 512)
+    with pytest.raises(ValueError, match="settings changed"):
+        b.run_steps([s for _, s in pairs], 1)
 
 
 def test_an_instance_that_stops_stops_alone():

@@ -112,7 +112,7 @@ def test_fp32_plane_beach_fv_matches_fp64():
     assert r["eta_max_err_m"] < 1e-3, r
 
 
-@pytest.mark.parametrize("pcg", [3, 4, 512, "tiles"])
+@pytest.mark.parametrize("pcg", [3, 4, 512, "tiles", "walk8"])
 def test_fp32_lake_at_rest_and_closed_basin_volume(pcg):
     # well_balanced.rs:16-62 and 87-121 on the FP32 form
     for mean_depth, amplitude in [(8.0, 3.0), (0.5, 2.5)]:

@@ -192,7 +192,7 @@ struct wc_solver {
   int pcg_march = 0;        // one-kernel PCG: blocks march down column strips (1) or take
                             // 64x16 tiles (0)
   int recon64 = 1;          // FP32 form: reconstruct the FP32 state in FP64 (1) or FP32 (0)
-  int pcg_walk = 1;         // one-kernel PCG on 64x16 tiles: the fewest tiles a block takes
+  int pcg_walk = 1;         // one-kernel PCG on 64x16 tiles: tiles per block to aim for
   int flux_kernel = 2;
   unsigned long long pcg_batch = 4;
   uint8_t* held = nullptr;
@@ -3180,8 +3180,16 @@ void enq_update(wc_batch* b, int st, int mode) {
 int batch_pcg_tail(wc_batch* b, int st, unsigned long long k) {
   const int n = (int)b->active.size();
   unsigned long long it = k + 1;
-  unsigned long long last = 0;
-  for (int q = 0; q < n; ++q) last = std::max(last, b->prev_it[st][q]);
+  // the largest count of the instances' last solves at this stage, leaving out solves that ran
+  // out of iterations (one instance at max_iterations would otherwise queue that many unchecked
+  // launches for all), and at most pcg_batch past the smallest
+  unsigned long long last = 0, least = ~0ull;
+  for (int q = 0; q < n; ++q) {
+    const unsigned long long v = b->prev_it[st][q];
+    least = std::min(least, v);
+    if (v < b->maxit) last = std::max(last, v);
+  }
+  if (least != ~0ull) last = std::min(last, least + b->pcg_batch);
   unsigned long long batch = last > k ? last - k : b->pcg_batch;
   bool look = k > 0 && last <= k;  // nothing to queue before the first look
   while (true) {
@@ -3358,6 +3366,24 @@ int batch_tune(wc_batch* b) {
 
 extern "C" {
 
+// Whether instance s can step in a batch whose settings are those of s0 (at creation, s0 is the
+// first instance; before every run, each instance against its settings at creation, which the
+// batch's launches were built from).
+int batch_compatible(const wc_solver* s, const wc_solver* s0) {
+  if (s->precision != WC_PRECISION_FP32)
+    return bfail("a batch runs the FP32 form (wc_solver_set_precision)");
+  if (s->precond != WC_PRECOND_JACOBI || s->capture || !(s->pcg_kernels == 0 || s->pcg_kernels == 4))
+    return bfail("a batch runs the one-kernel Jacobi PCG, without capture");
+  if (s->device != s0->device) return bfail("a batch's instances share one device");
+  if (s->pcg_lanes != s0->pcg_lanes || s->pcg_march != s0->pcg_march ||
+      s->recon64 != s0->recon64)
+    return bfail("a batch's instances share pcg_lanes, pcg_march and flux_reconstruction");
+  if ((s->order == WC_ORDER_FIRST) != (s0->order == WC_ORDER_FIRST) ||
+      switch_on(s) != switch_on(s0) || s->dispersive != s0->dispersive)
+    return bfail("a batch's instances share the time stepping, the switch and the dispersion");
+  return WC_OK;
+}
+
 int wc_batch_new(wc_solver* const* solvers, uint64_t n, wc_batch** out) {
   if (!solvers || !out || n == 0) return fail(WC_EINVAL, "null argument");
   *out = nullptr;
@@ -3365,17 +3391,7 @@ int wc_batch_new(wc_solver* const* solvers, uint64_t n, wc_batch** out) {
   for (uint64_t q = 0; q < n; ++q) {
     wc_solver* s = solvers[q];
     if (!s) return bfail("null solver");
-    if (s->precision != WC_PRECISION_FP32)
-      return bfail("a batch runs the FP32 form (wc_solver_set_precision)");
-    if (s->precond != WC_PRECOND_JACOBI || s->capture || !(s->pcg_kernels == 0 || s->pcg_kernels == 4))
-      return bfail("a batch runs the one-kernel Jacobi PCG, without capture");
-    if (s->device != s0->device) return bfail("a batch's instances share one device");
-    if (s->pcg_lanes != s0->pcg_lanes || s->pcg_march != s0->pcg_march ||
-        s->recon64 != s0->recon64)
-      return bfail("a batch's instances share pcg_lanes, pcg_march and flux_reconstruction");
-    if ((s->order == WC_ORDER_FIRST) != (s0->order == WC_ORDER_FIRST) ||
-        switch_on(s) != switch_on(s0) || s->dispersive != s0->dispersive)
-      return bfail("a batch's instances share the time stepping, the switch and the dispersion");
+    TRY(batch_compatible(s, s0));
     for (uint64_t p = 0; p < q; ++p)
       if (solvers[p] == s) return bfail("an instance appears twice in the batch");
   }
@@ -3501,6 +3517,19 @@ int wc_batch_run_steps(wc_batch* b, uint64_t steps, uint64_t* done_steps, double
   const size_t all = b->inst.size();
   if (done_steps)
     for (size_t q = 0; q < all; ++q) done_steps[q] = 0;
+  // The instances may have been used alone since the last run: their settings must still be
+  // the batch's, and their PCG lanes past the blocks the batch gives them must read zero again
+  // (a solve alone with other lanes or blocks may have written them).
+  for (size_t q = 0; q < all; ++q) {
+    const wc_solver* s = b->inst[q];
+    if (s->pcg_lanes != b->lanes || (s->pcg_march != 0) != b->march ||
+        (s->recon64 != 0) != b->recon64 || (s->order != WC_ORDER_FIRST) != b->rk2 ||
+        switch_on(s) != b->sw || s->dispersive != b->dispersive)
+      return bfail("an instance's settings changed since the batch was made");
+    TRY(batch_compatible(s, b->inst[0]));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    HIP_OK(hipMemsetAsync(s->lanes, 0, 8 * 512 * sizeof(double), b->stream));
+  }
   if (!b->tuned && !b->active.empty()) {
     TRY(batch_tune(b));
     drop_graphs(b);
