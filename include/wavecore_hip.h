@@ -140,8 +140,10 @@ int wc_solver_bed(const wc_solver* s, double* out);
  *                 small grid fewer, longer blocks, which pays when many instances share a
  *                 launch. Ignored by pcg_march 1, pcg_kernels 1-3 and the network's PCG;
  *   "pcg_batch"   PCG iterations queued between looks at the convergence flag (default 4);
- *   "flux_kernel" 1 = one thread per cell, as the reference's loop; 2 = LDS tiles, each cell
- *                 reconstructed and each face fluxed once (default);
+ *   "flux_kernel" 1 = one thread per cell, as the reference's loop (FP64; the FP32 form runs
+ *                 2 instead); 2 = 64x4 LDS tiles, each cell reconstructed and each face fluxed
+ *                 once; 3 = one wave per 58-column strip walking down a band of rows, the y
+ *                 stencil in registers and the x stencil in the wave's LDS row (default);
  *   "flux_reconstruction" FP32 form: 32 = reconstruct in FP32, 64 = in FP64 (default);
  *   "warm_start"  FP32 form: the SGN solves start from an extrapolation of the past solutions,
  *                 exact for solution steps of degree < m (m = 0..5; 0 = the previous

@@ -169,3 +169,22 @@ def test_fp32_breaking_fronts_with_eddy_viscosity_stay_near_fp64():
     assert np.max(np.abs(h64 - h32)) < 1e-3
     b64, b32 = s64.arrays()[3], s32.arrays()[3]
     assert np.any(b64 > 0) and np.mean((b64 != 0) == (b32 != 0)) >= 0.99
+
+
+@pytest.mark.parametrize("recon", [64, 32])
+def test_fp32_marching_flux_equals_the_tile_flux_bit_for_bit(recon):
+    """The marching flux (one wave per strip) and the 64x4 tile flux reconstruct every cell and
+    flux every face with the same arithmetic: the FP32 form steps to the same bits with either,
+    over a rough bed with dry land (the first-order fallbacks) and the reef tile's physics."""
+    for w in (configs.c1_rough_bed(nx=200, ny=72, steps=60), configs.c4_crop()):
+        out = []
+        for flux in (2, 3):
+            g = w.solver(Solver).with_precision(PRECISION_FP32)
+            g.set_option("flux_reconstruction", recon).set_option("flux_kernel", flux)
+            s = w.still_water()
+            s.time = 30.0
+            assert g.run_steps(s, 60)[0] == 60
+            out.append([a.copy() for a in s.arrays()[:3]] + [s.time])
+        for x, y in zip(out[0][:3], out[1][:3]):
+            assert np.array_equal(x, y), w.name
+        assert out[0][3] == out[1][3]

@@ -193,7 +193,8 @@ struct wc_solver {
                             // 64x16 tiles (0)
   int recon64 = 1;          // FP32 form: reconstruct the FP32 state in FP64 (1) or FP32 (0)
   int pcg_walk = 1;         // one-kernel PCG on 64x16 tiles: tiles per block to aim for
-  int flux_kernel = 2;
+  int flux_kernel = 3;
+  int march_rows = 0;  // band height of the marching flux launches (0: not yet chosen)
   unsigned long long pcg_batch = 4;
   uint8_t* held = nullptr;
   double* part = nullptr;
@@ -304,6 +305,29 @@ int prof_collect(wc_solver* s) {
 }
 
 dim3 tiles(const wc_solver* s) { return dim3((unsigned)(s->geo.ntx * s->geo.nty)); }
+
+// The marching flux launch (wc_flux.hip): one wave per 58-column strip and band of rows. The
+// band height is chosen once per solver so that all waves of a launch fit on the GPU at once
+// (as many bands as the free wave slots allow the strips, bands of at least 8 rows): a second,
+// partial round of waves would leave most of the GPU idle while it runs.
+constexpr int MARCH_MW = TX - 6;
+template <typename... KArgs>
+int march_launch(wc_solver* s, void (*kernel)(KArgs...), dim3* grid, int* rows) {
+  if (!s->march_rows) {
+    int per_cu = 0, cus = 0;
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, TX, 0));
+    HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
+    const int strips = (s->geo.nx + MARCH_MW - 1) / MARCH_MW;
+    const int slots = std::max(per_cu, 1) * std::max(cus, 1);
+    const int bands = std::max(1, slots / strips);
+    s->march_rows = std::max(8, (s->geo.ny + bands - 1) / bands);
+  }
+  const int strips = (s->geo.nx + MARCH_MW - 1) / MARCH_MW;
+  const int bands = (s->geo.ny + s->march_rows - 1) / s->march_rows;
+  *grid = dim3((unsigned)(strips * bands));
+  *rows = s->march_rows;
+  return WC_OK;
+}
 dim3 over(long long n) { return dim3((unsigned)((n + NT - 1) / NT)); }
 dim3 rate_blocks(long long n) { return dim3((unsigned)((n + RATE_CELLS - 1) / RATE_CELLS)); }
 
@@ -756,6 +780,27 @@ int stage(wc_solver* s, double dt, int stage, int mode) {
     f.ramp_prm = a.ramp_prm;
     f.sin_prm = a.sin_prm;
     TRY(launch(s, F_FLUX, k_flux_tile64, tiles(s), dim3(NT), f, a, flux_mode));
+  } else if (s->flux_kernel == 3) {
+    FluxTileArgs<double> f{};
+    f.g = a.g;
+    f.order = a.order;
+    f.h = a.in_h;
+    f.h_lo = nullptr;
+    f.hu = a.in_hu;
+    f.hv = a.in_hv;
+    f.b = a.b;
+    f.phi = a.phi;
+    f.has_drive = a.has_drive;
+    f.src_i0 = a.src_i0;
+    f.src_i1 = a.src_i1;
+    f.src_col = a.src_col;
+    f.prm = a.prm;
+    f.ramp_prm = a.ramp_prm;
+    f.sin_prm = a.sin_prm;
+    dim3 grid;
+    int rows;
+    TRY(march_launch(s, k_flux_march64, &grid, &rows));
+    TRY(launch(s, F_FLUX, k_flux_march64, grid, dim3(TX), f, a, flux_mode, rows));
   } else {
     TRY(launch(s, F_FLUX, k_flux, tiles(s), dim3(NT), a, flux_mode));
   }
@@ -1357,8 +1402,18 @@ int stage32(wc_solver* s, double dt, int stage, int mode) {
                w.b, w.br, w.held, w.phi));
   }
   const BFlux f = flux_args32(s, stage);
-  TRY(s->recon64 ? launch(s, F_FLUX, k_flux_tile32r, tiles(s), dim3(NT), f.f, f.lh, f.lhu, f.lhv)
-                  : launch(s, F_FLUX, k_flux_tile32, tiles(s), dim3(NT), f.f, f.lh, f.lhu, f.lhv));
+  if (s->flux_kernel == 3) {
+    dim3 grid;
+    int rows;
+    TRY(march_launch(s, s->recon64 ? k_flux_march32r : k_flux_march32, &grid, &rows));
+    TRY(s->recon64
+            ? launch(s, F_FLUX, k_flux_march32r, grid, dim3(TX), f.f, f.lh, f.lhu, f.lhv, rows)
+            : launch(s, F_FLUX, k_flux_march32, grid, dim3(TX), f.f, f.lh, f.lhu, f.lhv, rows));
+  } else {
+    TRY(s->recon64
+            ? launch(s, F_FLUX, k_flux_tile32r, tiles(s), dim3(NT), f.f, f.lh, f.lhu, f.lhv)
+            : launch(s, F_FLUX, k_flux_tile32, tiles(s), dim3(NT), f.f, f.lh, f.lhu, f.lhv));
+  }
   if (s->dispersive) {
     TRY(launch(s, F_SGN_SETUP, k_sgn_setup32, tiles(s), dim3(NT), sgn32_args(s, stage)));
     TRY(pcg32(s, stage));
@@ -1723,13 +1778,14 @@ int wc_solver_set_option(wc_solver* s, const char* key, int64_t value) {
     return WC_OK;
   }
   if (strcmp(key, "flux_kernel") == 0) {
-    if (value != 1 && value != 2) return fail(WC_EINVAL, "flux_kernel must be 1 or 2");
+    if (value < 1 || value > 3) return fail(WC_EINVAL, "flux_kernel must be 1, 2 or 3");
     s->flux_kernel = (int)value;
     return WC_OK;
   }
   if (strcmp(key, "flux_reconstruction") == 0) {
     if (value != 32 && value != 64) return fail(WC_EINVAL, "flux_reconstruction must be 32 or 64");
     s->recon64 = value == 64 ? 1 : 0;
+    s->march_rows = 0;
     return WC_OK;
   }
   if (strcmp(key, "pcg_march") == 0) {
@@ -1784,6 +1840,7 @@ int wc_solver_set_precision(wc_solver* s, int32_t precision) {
   if (precision != WC_PRECISION_FP64 && precision != WC_PRECISION_FP32)
     return fail(WC_EINVAL, "precision must be WC_PRECISION_FP64 or WC_PRECISION_FP32");
   if (precision == s->precision) return WC_OK;
+  s->march_rows = 0;  // the other precision's flux kernel has its own occupancy
   if (precision == WC_PRECISION_FP32) {
     TRY(alloc_fp32(s));
     s->precision = precision;

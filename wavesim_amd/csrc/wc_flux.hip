@@ -54,35 +54,34 @@ __device__ __forceinline__ void face_values_r(R low2, R low, R c, R high, R high
   hi = mc_hi + blend * (fifth_hi - mc_hi);
 }
 
-// solver.rs:146-183 for the cell in the middle of five, reading the staged tile along a line:
-// e[], d[] (depth, FP64), n[] (normal velocity), t[] (tangential), b[] (bed, FP64), p[] (phi).
+// solver.rs:146-183 for the cell in the middle of five consecutive cells along a line: e[]
+// (surface), d[] (depth, FP64), n[] (normal velocity), t[] (tangential), b[] (bed, FP64), p[]
+// (phi).
 template <typename R>
-__device__ __forceinline__ void reconstruct_r(const R* e, const double* d, const R* n, const R* t,
-                                              const double* b, const R* p, int stride, int order,
-                                              SideT<R>& lo_out, SideT<R>& hi_out) {
-  const double h1 = d[stride], h2 = d[2 * stride], h3 = d[3 * stride];
-  const SideT<R> first{h2, b[2 * stride], n[2 * stride], t[2 * stride]};
+__device__ __forceinline__ void reconstruct5(const R (&e)[5], const double (&d)[5],
+                                             const R (&n)[5], const R (&t)[5],
+                                             const double (&b)[5], const R (&p)[5], int order,
+                                             SideT<R>& lo_out, SideT<R>& hi_out) {
+  const double h1 = d[1], h2 = d[2], h3 = d[3];
+  const SideT<R> first{h2, b[2], n[2], t[2]};
   if (order == 0 || h1 < WC_H_SECOND || h2 < WC_H_SECOND || h3 < WC_H_SECOND) {
     lo_out = first;
     hi_out = first;
     return;
   }
   R blend = R(0);
-  if (order == 2 && d[0] >= WC_H_SECOND && d[4 * stride] >= WC_H_SECOND) {
+  if (order == 2 && d[0] >= WC_H_SECOND && d[4] >= WC_H_SECOND) {
     double bl = 1.0;
 #pragma unroll
-    for (int m = 0; m < 5; ++m) bl = wc_min(bl, (double)p[m * stride]);
+    for (int m = 0; m < 5; ++m) bl = wc_min(bl, (double)p[m]);
     blend = (R)bl;
   }
   R eta_lo, eta_hi, un_lo, un_hi, ut_lo, ut_hi;
-  face_values_r<R>(e[0], e[stride], e[2 * stride], e[3 * stride], e[4 * stride], blend, eta_lo,
-                   eta_hi);
-  face_values_r<R>(n[0], n[stride], n[2 * stride], n[3 * stride], n[4 * stride], blend, un_lo,
-                   un_hi);
-  face_values_r<R>(t[0], t[stride], t[2 * stride], t[3 * stride], t[4 * stride], blend, ut_lo,
-                   ut_hi);
-  const double cb = b[2 * stride];
-  const double s_z = 0.25 * (b[3 * stride] - b[stride]);
+  face_values_r<R>(e[0], e[1], e[2], e[3], e[4], blend, eta_lo, eta_hi);
+  face_values_r<R>(n[0], n[1], n[2], n[3], n[4], blend, un_lo, un_hi);
+  face_values_r<R>(t[0], t[1], t[2], t[3], t[4], blend, ut_lo, ut_hi);
+  const double cb = b[2];
+  const double s_z = 0.25 * (b[3] - b[1]);
   SideT<R> l, r;
   l.z = cb + -1.0 * s_z;
   l.h = (double)eta_lo - l.z;
@@ -99,6 +98,25 @@ __device__ __forceinline__ void reconstruct_r(const R* e, const double* d, const
   }
   lo_out = l;
   hi_out = r;
+}
+
+// The same from a staged line: five values `stride` apart.
+template <typename R>
+__device__ __forceinline__ void reconstruct_r(const R* e, const double* d, const R* n, const R* t,
+                                              const double* b, const R* p, int stride, int order,
+                                              SideT<R>& lo_out, SideT<R>& hi_out) {
+  R e5[5], n5[5], t5[5], p5[5];
+  double d5[5], b5[5];
+#pragma unroll
+  for (int m = 0; m < 5; ++m) {
+    e5[m] = e[m * stride];
+    d5[m] = d[m * stride];
+    n5[m] = n[m * stride];
+    t5[m] = t[m * stride];
+    b5[m] = b[m * stride];
+    p5[m] = p[m * stride];
+  }
+  reconstruct5<R>(e5, d5, n5, t5, b5, p5, order, lo_out, hi_out);
 }
 
 template <typename R>
@@ -226,6 +244,156 @@ __device__ __forceinline__ void flux_tile(const FluxTileArgs<S>& a, int mode, in
   }
 }
 
+// ---- the marching wave ------------------------------------------------------------------------
+// One 64-lane wave per strip of MW = 58 interior columns and a band of rows: lane l holds padded
+// column c0 + l (the strip's columns and three more on each side). The wave walks down its band
+// one row at a time. Each lane keeps its column's five rows around the row being reconstructed
+// along y in registers (the y stencil, the y sides and the y faces need no neighbour); the
+// x stencil of a row goes through the wave's own LDS row. Per cell: one reconstruction and one
+// face per axis (the 64x4 tile takes 2.5 and 2.3), the inputs read once per band plus six rows,
+// and a few KB of LDS per wave instead of 75 KB per 256-thread tile. The arithmetic per cell is
+// the tile kernel's (reconstruct5, face_r, the rates), so the results are the same bits.
+constexpr int MW = TX - 6;
+
+// What a lane stages per cell: depth (FP64, with the low part of the FP32 pair), the velocities
+// and phi in R, the bed.
+template <typename R>
+struct Cell5 {
+  double h, b;
+  R u, v, p;
+};
+
+template <typename S, typename R>
+__device__ __forceinline__ Cell5<R> load_cell(const FluxTileArgs<S>& a, int i, int j) {
+  Cell5<R> c{0.0, 0.0, R(0), R(0), R(0)};
+  if (i < a.g.w) {
+    const int k = j * a.g.w + i;
+    double hd = (double)a.h[k];
+    if (a.h_lo) hd = hd + (double)a.h_lo[k];
+    double mu = (double)a.hu[k], mv = (double)a.hv[k];
+    if (a.hu_lo) {
+      mu = mu + (double)a.hu_lo[k];
+      mv = mv + (double)a.hv_lo[k];
+    }
+    c.h = hd;
+    c.b = a.b[k];
+    c.u = (R)wc_velocity(hd, mu);
+    c.v = (R)wc_velocity(hd, mv);
+    c.p = (R)a.phi[k];
+  }
+  return c;
+}
+
+template <typename R>
+struct MarchLds {
+  double h[TX], b[TX];
+  R u[TX], v[TX], p[TX];
+  SideT<R> xhi[TX];
+  Face fx[TX];
+};
+
+// The rates of the band's cells, handed to out(i, j, k, rh, rhu, rhv) (padded i, j). `rows` is
+// the band height; block b takes strip b % strips, band b / strips.
+template <typename S, typename R, class Out>
+__device__ __forceinline__ void flux_march(const FluxTileArgs<S>& a, int rows, Out out) {
+  const Geo& g = a.g;
+  __shared__ MarchLds<R> L;
+  const int strips = (g.nx + MW - 1) / MW;
+  const int strip = blockIdx.x % strips, band = blockIdx.x / strips;
+  const int l = threadIdx.x;
+  const int i = strip * MW + l;  // padded column: interior column strip * MW - 3 + l
+  const int j0 = GH + band * rows, j1 = min(j0 + rows, GH + g.ny);
+  if (j0 >= j1) return;
+  // the five rows around the row being reconstructed along y
+  Cell5<R> w[5];
+#pragma unroll
+  for (int m = 0; m < 5; ++m) w[m] = load_cell<S, R>(a, i, j0 - 3 + m);
+  SideT<R> ylo_prev{}, yhi_prev{};
+  Face fy_prev{};
+  const double dx = g.dx, dy = g.dy;
+  for (int jr = j0 - 1; jr <= j1; ++jr) {
+    Cell5<R> nxt{};
+    if (jr < j1) nxt = load_cell<S, R>(a, i, jr + 3);  // in flight while this row computes
+    // y: the sides of row jr, the face between jr - 1 and jr
+    SideT<R> ylo, yhi;
+    {
+      R e[5], n[5], t[5], p[5];
+      double d[5], b[5];
+#pragma unroll
+      for (int m = 0; m < 5; ++m) {
+        e[m] = (R)(w[m].h + w[m].b);
+        d[m] = w[m].h;
+        n[m] = w[m].v;
+        t[m] = w[m].u;
+        b[m] = w[m].b;
+        p[m] = w[m].p;
+      }
+      reconstruct5<R>(e, d, n, t, b, p, a.order, ylo, yhi);
+    }
+    Face fy{};
+    if (jr >= j0) fy = face_r<R>(yhi_prev, ylo);
+    if (jr >= j0 + 1) {
+      // x: row jr - 1 (w[1]) through the wave's LDS row
+      const int j = jr - 1;
+      L.h[l] = w[1].h;
+      L.b[l] = w[1].b;
+      L.u[l] = w[1].u;
+      L.v[l] = w[1].v;
+      L.p[l] = w[1].p;
+      __syncthreads();
+      SideT<R> xlo{}, xhi{};
+      if (l >= 2 && l < TX - 2) {
+        R e[5], n[5], t[5], p[5];
+        double d[5], b[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+          const int q = l - 2 + m;
+          e[m] = (R)(L.h[q] + L.b[q]);
+          d[m] = L.h[q];
+          n[m] = L.u[q];
+          t[m] = L.v[q];
+          b[m] = L.b[q];
+          p[m] = L.p[q];
+        }
+        reconstruct5<R>(e, d, n, t, b, p, a.order, xlo, xhi);
+      }
+      L.xhi[l] = xhi;
+      __syncthreads();
+      if (l >= 3 && l < TX - 2) L.fx[l] = face_r<R>(L.xhi[l - 1], xlo);
+      __syncthreads();
+      const int ii = i;  // padded
+      if (l >= 3 && l < TX - 3 && ii < g.nx + GH) {
+        const Face& xl = L.fx[l];
+        const Face& xh = L.fx[l + 1];
+        const Face& yl = fy_prev;
+        const Face& yh = fy;
+        const double xs = -WC_G * 0.5 * (xlo.h + xhi.h) * (xhi.z - xlo.z) / dx;
+        const double ys = -WC_G * 0.5 * (ylo_prev.h + yhi_prev.h) * (yhi_prev.z - ylo_prev.z) / dy;
+        double rh = -(xh.f0 - xl.f0) / dx - (yh.f0 - yl.f0) / dy;
+        const double rhu = -((xh.f1 + xh.cl) - (xl.f1 + xl.cr)) / dx + xs - (yh.f2 - yl.f2) / dy;
+        const double rhv = -((yh.f1 + yh.cl) - (yl.f1 + yl.cr)) / dy + ys - (xh.f2 - xl.f2) / dx;
+        if (a.has_drive) {
+          const int ic = ii - GH;
+          double src = 0.0;
+          if (ic >= a.src_i0 && ic < a.src_i1)
+            src = a.src_col[ic] * a.prm[a.ramp_prm] * a.prm[a.sin_prm + (j - GH)];
+          rh = rh + src;
+        }
+        out(ii, j, j * g.w + ii, rh, rhu, rhv);
+      }
+      __syncthreads();  // the next row's staging overwrites the LDS row
+    }
+    ylo_prev = ylo;
+    yhi_prev = yhi;
+    fy_prev = fy;
+    if (jr < j1) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) w[m] = w[m + 1];
+      w[4] = nxt;
+    }
+  }
+}
+
 // FP64: the tile rates with the stage epilogue of k_flux.
 __global__ void __launch_bounds__(NT) k_flux_tile64(FluxTileArgs<double> f, StepArgs a,
                                                     int mode) {
@@ -235,6 +403,14 @@ __global__ void __launch_bounds__(NT) k_flux_tile64(FluxTileArgs<double> f, Step
   flux_tile<double, double>(f, mode, blockIdx.x, gridDim.x, rh, rhu, rhv, i, j, valid);
   if (!valid) return;
   stage_out(a, mode, i, j, j * a.g.w + i, rh, rhu, rhv);
+}
+
+// The same, marching (flux_kernel 3, the default).
+__global__ void __launch_bounds__(TX) k_flux_march64(FluxTileArgs<double> f, StepArgs a, int mode,
+                                                     int rows) {
+  flux_march<double, double>(f, rows, [&](int i, int j, int k, double rh, double rhu, double rhv) {
+    stage_out(a, mode, i, j, k, rh, rhu, rhv);
+  });
 }
 
 // FP32 chosen form: the rates for the stage update that follows, in FP64 (the mass rate so
@@ -264,6 +440,26 @@ __global__ void __launch_bounds__(NT) k_flux_tile32(FluxTileArgs<float> f, doubl
 __global__ void __launch_bounds__(NT) k_flux_tile32r(FluxTileArgs<float> f, double* lh,
                                                      double* lhu, double* lhv) {
   flux32_body<double>(f, lh, lhu, lhv, blockIdx.x, gridDim.x);
+}
+
+template <typename R>
+__device__ __forceinline__ void flux32_march(const FluxTileArgs<float>& f, double* lh,
+                                             double* lhu, double* lhv, int rows) {
+  flux_march<float, R>(f, rows, [&](int, int, int k, double rh, double rhu, double rhv) {
+    lh[k] = rh;
+    lhu[k] = rhu;
+    lhv[k] = rhv;
+  });
+}
+
+__global__ void __launch_bounds__(TX) k_flux_march32(FluxTileArgs<float> f, double* lh,
+                                                     double* lhu, double* lhv, int rows) {
+  flux32_march<float>(f, lh, lhu, lhv, rows);
+}
+
+__global__ void __launch_bounds__(TX) k_flux_march32r(FluxTileArgs<float> f, double* lh,
+                                                      double* lhu, double* lhv, int rows) {
+  flux32_march<double>(f, lh, lhu, lhv, rows);
 }
 
 // Many instances in one launch (see kb_fill32).

@@ -334,6 +334,12 @@ __global__ void k_sgn_finish(StepArgs a, const double* wx, const double* wy, int
 __global__ void k_flux_tile64(FluxTileArgs<double> f, StepArgs a, int mode);
 __global__ void k_flux_tile32(FluxTileArgs<float> f, double* lh, double* lhu, double* lhv);
 __global__ void k_flux_tile32r(FluxTileArgs<float> f, double* lh, double* lhu, double* lhv);
+// One wave per strip of 58 columns walking down a band of `rows` rows (wc_flux.hip).
+__global__ void k_flux_march64(FluxTileArgs<double> f, StepArgs a, int mode, int rows);
+__global__ void k_flux_march32(FluxTileArgs<float> f, double* lh, double* lhu, double* lhv,
+                               int rows);
+__global__ void k_flux_march32r(FluxTileArgs<float> f, double* lh, double* lhu, double* lhv,
+                                int rows);
 __global__ void k_max_rate(Geo g, const double* h, const double* hu, const double* hv,
                            unsigned long long* out);
 __global__ void k_limit_scan(Geo g, const double* h, const double* hu, const double* hv,
