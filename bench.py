@@ -41,10 +41,10 @@ HBM_PEAK_GBS = 8000.0
 # or written once, halo re-reads excluded), for the roofline.
 BYTES_PER_CELL = {
     "fp64": {
-        "cg_matvec": 8 * (2 + 2 + 1 + 1 + 2),  # read z, p_old, c, h; write p
-        "cg_update": 8 * (2 + 1 + 1 + 2 + 2 + 2 + 2 + 2),  # read p, c, h, w, r; write w, r, z
-        "cg_pair": 8 * (2 + 2 + 1 + 1 + 2) + 8 * (2 + 1 + 1 + 2 + 2 + 2 + 2 + 2),  # one iteration
-        "cg_init": 8 * (2 + 1 + 1 + 2 + 2 + 2),  # read w, c, h, b; write r, z
+        "cg_matvec": 8 * (2 + 2 + 2 + 1 + 1 + 2),  # read r, p_old, diag, c, h; write p
+        "cg_update": 8 * (2 + 1 + 1 + 2 + 2 + 2 + 2),  # read p, c, h, w, r; write w, r
+        "cg_pair": 8 * (2 + 2 + 2 + 1 + 1 + 2) + 8 * (2 + 1 + 1 + 2 + 2 + 2 + 2),  # one iteration
+        "cg_init": 8 * (2 + 1 + 1 + 2 + 2),  # read w, c, h, b; write r
         "sgn_setup": 8 * (7 + 5),  # read h, hu, hv, phi, lh, lhu, lhv; write c, diag, b
         "flux": 8 * (5 + 3),  # read h, hu, hv, b, phi; write lh, lhu, lhv
         "sgn_finish": 8 * (3 + 3 + 2 + 3 + 1 + 3),  # read s_in, rates, w, s, b; write s

@@ -617,6 +617,8 @@ int pcg2(wc_solver* s, int stage) {
   a.g = s->geo;
   a.h = stage == 0 ? s->h : s->h1;
   a.c = s->c;
+  a.dgx = s->dgx;
+  a.dgy = s->dgy;
   a.bx = s->bx;
   a.by = s->by;
   a.wx = s->wx;

@@ -147,6 +147,7 @@ template <typename V>
 struct CgV {
   Geo g;
   const V *h, *c;
+  const V *dgx, *dgy;  // the Jacobi diagonal as the set-up stored it (wc_pcg2.hip)
   const double *bx, *by;
   const double *solx, *soly;  // FP32 form: the solve's FP64 start x0 (w its correction); FP64: null
   V *wx, *wy, *rx, *ry, *px, *py, *zx, *zy, *ax, *ay;

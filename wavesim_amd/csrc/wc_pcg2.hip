@@ -4,14 +4,15 @@
 // The reference's loop needs two grid-wide reductions per iteration (p.Ap before the update,
 // r.r and r.z after it), so an iteration is two launches, each a pass over the grid:
 //   M_n: beta_{n-1} from U_{n-1}'s lanes (or the start's); p_{n-1} = z_{n-1} + beta p_{n-2}
-//        (p_0 = z_0) on the tile + 2, A p_{n-1}, lanes of p.Ap; p_{n-1} stored;
+//        (p_0 = z_0) on the tile + 2, with z = r / diag taken there from r and the diagonal the
+//        set-up stored; A p_{n-1}, lanes of p.Ap; p_{n-1} stored;
 //   U_n: alpha_n = r.z / p.Ap; p_{n-1} staged again on the tile + 2 and A p_{n-1} computed
 //        again (the same arithmetic, so the same bits) instead of being stored by M_n and read
 //        back; w += alpha p; r -= alpha A p; z = r / diag; lanes of r.r and r.z.
 // Recomputing A p in U_n saves its store and its load (32 of 208 bytes per cell and
-// iteration); the two halo stencils run from LDS and L2. Per cell and iteration the passes
-// move z, p_old (read), p (written), c, h in M_n and p, c, h, r, w (read), w, r, z (written)
-// in U_n: 176 bytes.
+// iteration), and M_n's z from r and the stored diagonal replaces U_n's store of z by a load;
+// the halo work runs from LDS and L2. Per cell and iteration the passes move r, p_old, diag
+// (read), p (written), c, h in M_n and p, c, h, r, w (read), w, r (written) in U_n: 176 bytes.
 //
 // Layout: 64x16 tiles (one 1024-thread block, a cell per thread; the two-cell halo is a third of
 // the tile where a 64x4 tile's is 113%), exactly 256 blocks, block b taking tiles b, b+256, ...
@@ -187,25 +188,30 @@ __device__ __forceinline__ void apply_own(const Geo& g, double h, double (*s_u)[
   ay = mask * (heff * py - (s_f[y1 + 1][x1] - s_f[y1 - 1][x1]) * iy);
 }
 
-// The Jacobi diagonal (dispersion.rs:466-480) at the own cell, c of its neighbours from the
-// staged tile + 1.
-__device__ __forceinline__ void own_diag(const Geo& g, double h, double (*s_c)[S1X], double& dgx,
-                                         double& dgy) {
-  const int x = (threadIdx.x & 63) + 1, y = (threadIdx.x >> 6) + 1;
+// The Jacobi diagonal (dispersion.rs:466-480) of a cell from its depth and its neighbours' c.
+__device__ __forceinline__ void diag_at(const Geo& g, double h, double cxm, double cxp,
+                                        double cym, double cyp, double& dgx, double& dgy) {
   if (!(h >= WC_H_DISPERSION_MIN)) {
     dgx = 1.0;
     dgy = 1.0;
     return;
   }
   const double heff = wc_max(h, WC_H_DRY);
-  dgx = heff + (s_c[y][x + 1] + s_c[y][x - 1]) * 0.25 / (g.dx * g.dx);
-  dgy = heff + (s_c[y + 1][x] + s_c[y - 1][x]) * 0.25 / (g.dy * g.dy);
+  dgx = heff + (cxp + cxm) * 0.25 / (g.dx * g.dx);
+  dgy = heff + (cyp + cym) * 0.25 / (g.dy * g.dy);
+}
+
+// The same at the own cell, c of its neighbours from the staged tile + 1.
+__device__ __forceinline__ void own_diag(const Geo& g, double h, double (*s_c)[S1X], double& dgx,
+                                         double& dgy) {
+  const int x = (threadIdx.x & 63) + 1, y = (threadIdx.x >> 6) + 1;
+  diag_at(g, h, s_c[y][x - 1], s_c[y][x + 1], s_c[y - 1][x], s_c[y + 1][x], dgx, dgy);
 }
 
 }  // namespace
 
-// r = b - A w, z = r / diag; lanes of b.b, r.z, r.r (dispersion.rs:493-514; p_0 = z_0 is
-// staged by M_1 from z).
+// r = b - A w; lanes of b.b, r.z, r.r with z = r / diag (dispersion.rs:493-514; p_0 = z_0 is
+// staged by M_1 from r).
 __global__ void __launch_bounds__(PT) k_cg2_init(CgV<double> a) {
   const Geo& g = a.g;
   const int nt = tiles_all(g);
@@ -243,8 +249,6 @@ __global__ void __launch_bounds__(PT) k_cg2_init(CgV<double> a) {
       const double zx = rx / dgx, zy = ry / dgy;
       a.rx[k] = rx;
       a.ry[k] = ry;
-      a.zx[k] = zx;
-      a.zy[k] = zy;
       v[0] = bx * bx + by * by;
       v[1] = rx * zx + ry * zy;
       v[2] = rx * rx + ry * ry;
@@ -268,16 +272,20 @@ __global__ void __launch_bounds__(PT) k_cg2_matvec(CgV<double> a, unsigned long 
   const Geo& g = a.g;
   const int nt = tiles_all(g);
   const bool first = n == 1;
-  auto halo = [&](int m, double (&v)[4]) {
-    v[0] = a.zx[m];
-    v[1] = a.zy[m];
+  // z = r / diag at the halo's (mapped) cells from r and the diagonal the set-up stored
+  // (k_sgn_setup, dispersion.rs:466-480): no z array
+  auto halo = [&](int m, double (&v)[6]) {
+    v[0] = a.rx[m];
+    v[1] = a.ry[m];
     v[2] = first ? 0.0 : pox[m];
     v[3] = first ? 0.0 : poy[m];
+    v[4] = a.dgx[m];
+    v[5] = a.dgy[m];
   };
   auto own = [&](int k, double (&o)[1]) { o[0] = a.h[k]; };
   // two tiles' loads in flight: the first two while the scalars are reduced, then each tile's
   // successor-but-one as soon as the tile is staged
-  Pre<4, 1> P[2];
+  Pre<6, 1> P[2];
   const int t0 = blockIdx.x;
   if (t0 < nt) prefetch(a, t0, halo, own, P[0]);
   if (t0 + LANES < nt) prefetch(a, t0 + LANES, halo, own, P[1]);
@@ -286,14 +294,15 @@ __global__ void __launch_bounds__(PT) k_cg2_matvec(CgV<double> a, unsigned long 
   __shared__ double s_u[2][S2Y][S2X], s_c[S1Y][S1X], s_f[S1Y][S1X];
   __shared__ double sums[1][BY][1];
   double acc[1] = {0.0};
-  auto body = [&](Pre<4, 1>& Q, int tile) {
-    stage(Q, s_u, s_c, [&](const double (&v)[4], double& vx, double& vy) {
+  auto body = [&](Pre<6, 1>& Q, int tile) {
+    stage(Q, s_u, s_c, [&](const double (&v)[6], double& vx, double& vy) {
+      const double zx = v[0] / v[4], zy = v[1] / v[5];
       if (first) {
-        vx = v[0];
-        vy = v[1];
+        vx = zx;
+        vy = zy;
       } else {
-        vx = v[0] + beta * v[2];
-        vy = v[1] + beta * v[3];
+        vx = zx + beta * v[2];
+        vy = zy + beta * v[3];
       }
     });
     const int k = Q.k;
@@ -339,7 +348,7 @@ __global__ void __launch_bounds__(PT) k_cg2_update(CgV<double> a, unsigned long 
     o[3] = a.rx[k];
     o[4] = a.ry[k];
   };
-  Pre<2, 5> P[2];  // two tiles' loads in flight, as in M_n
+  Pre<2, 5> P[2];  // two tiles' loads in flight
   const int t0 = blockIdx.x;
   if (t0 < nt) prefetch(a, t0, halo, own, P[0]);
   if (t0 + LANES < nt) prefetch(a, t0 + LANES, halo, own, P[1]);
@@ -377,8 +386,6 @@ __global__ void __launch_bounds__(PT) k_cg2_update(CgV<double> a, unsigned long 
       const double zx = rx / dgx, zy = ry / dgy;
       a.rx[k] = rx;
       a.ry[k] = ry;
-      a.zx[k] = zx;
-      a.zy[k] = zy;
       v[0] = rx * rx + ry * ry;
       v[1] = rx * zx + ry * zy;
     }
