@@ -524,7 +524,8 @@ def main():
                 w, g, fresh_state(), "fp32", "fp32+previous_start", args.warmup, args.steps,
                 args.profile_steps, prime=priming_steps(args, 0),
                 family=timed_family(args, "fp32"))
-    net_file = os.path.join(ROOT, "wavesim_amd", "data", "precond_rough_bed.npy")
+    # the network trained on the residuals the FP32 form's extrapolated starts leave
+    net_file = os.path.join(ROOT, "wavesim_amd", "data", "precond_extrapolated.npy")
     if args.precision != "fp64" and not args.no_network and os.path.exists(net_file):
         g = configure(w, "fp32", args, network=np.load(net_file))
         order = 3 if args.warm_start is None else args.warm_start

@@ -19,6 +19,7 @@ touches (h >= 0.05). Training: the library's forward, backward and Adam kernels
 from __future__ import annotations
 
 import argparse
+import json
 import sys
 import time
 
@@ -119,14 +120,18 @@ def train(samples, grid: Grid, steps: int, lr: float = 1e-3, seed: int = 0, log=
     return params.cpu().numpy(), history
 
 
-def evaluate(params, seeds, nx: int, ny: int, spinup: int, steps: int, log=print) -> list:
-    """PCG iterations per solve and time per step, Jacobi against the network, on each bed from
-    the same mid-run state (FP32 form)."""
+def evaluate(params, seeds, nx: int, ny: int, spinup: int, steps: int, log=print,
+             warm_starts=(3,), nets=None) -> list:
+    """PCG iterations per solve, time per step and fallbacks, Jacobi against each network, on
+    each seeded bed from the same mid-run state (FP32 form), for each warm start (0: the
+    reference's start from the previous solution; 3: the FP32 form's extrapolated start).
+    `nets` maps names to parameter arrays (default {"network": params})."""
     import torch
 
     from . import configs
     from .grid import State
 
+    nets = nets if nets is not None else {"network": params}
     rows = []
     for seed in seeds:
         w = configs.c1_rough_bed(nx=nx, ny=ny, seed=int(seed))
@@ -135,41 +140,59 @@ def evaluate(params, seeds, nx: int, ny: int, spinup: int, steps: int, log=print
         g.run_steps(s, spinup)
         arrays = [a.copy() for a in s.arrays()]
         t0 = s.time
-        row = {"seed": int(seed), "t": t0}
-        for name, p in (("jacobi", None), ("network", params)):
-            gg = w.solver(Solver).with_precision(PRECISION_FP32)
-            if p is not None:
-                gg.with_preconditioner(p)
-            ss = State(*(a.copy() for a in arrays), t0)
-            gg.run_steps(ss, 2)  # warm starts
-            st0 = gg.dispersion_stats()
-            torch.cuda.synchronize()
-            t = time.perf_counter()
-            gg.run_steps(ss, steps)
-            torch.cuda.synchronize()
-            el = time.perf_counter() - t
-            st1 = gg.dispersion_stats()
-            row[name] = {"iters_per_solve": (st1.iterations - st0.iterations) /
-                         max(st1.solves - st0.solves, 1),
-                         "ms_per_step": 1e3 * el / steps,
-                         "unconverged": st1.unconverged - st0.unconverged,
-                         "fallbacks": gg.precond_fallbacks() if p is not None else 0}
-            if p is not None:
-                row["eta_diff_m"] = float(np.max(np.abs(ss.arrays()[0] - ref_h)))
-                hist = gg.residual_history()
-                row["history"] = (np.sqrt(hist / hist[0]) if hist.size else hist).tolist()
-            else:
-                ref_h = ss.arrays()[0].copy()
-        rows.append(row)
-        if log:
-            j, nn = row["jacobi"], row["network"]
-            log(f"[eval] seed {seed}: jacobi {j['iters_per_solve']:.1f} it/solve "
-                f"{j['ms_per_step']:.2f} ms/step; network {nn['iters_per_solve']:.1f} it/solve "
-                f"{nn['ms_per_step']:.2f} ms/step, {nn['fallbacks']} fallbacks, "
-                f"max |dh| {row['eta_diff_m']:.2e} m")
-            log("[eval]   network |r|/|r0| of the last solve: " +
-                " ".join(f"{v:.1e}" for v in row["history"][:40]))
+        for ws in warm_starts:
+            row = {"seed": int(seed), "t": t0, "warm_start": int(ws)}
+            ref_h = None
+            for name, p in (("jacobi", None), *nets.items()):
+                gg = w.solver(Solver).with_precision(PRECISION_FP32).set_option("warm_start", ws)
+                if p is not None:
+                    gg.with_preconditioner(p)
+                ss = State(*(a.copy() for a in arrays), t0)
+                gg.run_steps(ss, 8)  # warm starts full (the extrapolation's window of 6 steps)
+                st0 = gg.dispersion_stats()
+                f0 = gg.precond_fallbacks() if p is not None else 0
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                gg.run_steps(ss, steps)
+                torch.cuda.synchronize()
+                el = time.perf_counter() - t
+                st1 = gg.dispersion_stats()
+                solves = max(st1.solves - st0.solves, 1)
+                row[name] = {"iters_per_solve": (st1.iterations - st0.iterations) / solves,
+                             "ms_per_step": 1e3 * el / steps,
+                             "unconverged": st1.unconverged - st0.unconverged,
+                             "fallback_fraction": ((gg.precond_fallbacks() - f0) / solves
+                                                   if p is not None else 0.0)}
+                if p is None:
+                    ref_h = ss.arrays()[0].copy()
+                else:
+                    row[name]["eta_diff_m"] = float(np.max(np.abs(ss.arrays()[0] - ref_h)))
+            rows.append(row)
+            if log:
+                j = row["jacobi"]
+                msg = (f"[eval] seed {seed} start {ws}: jacobi {j['iters_per_solve']:.1f} it/solve "
+                       f"{j['ms_per_step']:.2f} ms/step")
+                for name in nets:
+                    nn = row[name]
+                    msg += (f"; {name} {nn['iters_per_solve']:.1f} it/solve "
+                            f"{nn['ms_per_step']:.2f} ms/step, fallbacks "
+                            f"{100 * nn['fallback_fraction']:.0f}%")
+                log(msg)
     return rows
+
+
+def summarize(rows, nets) -> dict:
+    """Means over the beds per warm start: iterations per solve, ms per step, fallback share."""
+    out = {}
+    for ws in sorted({r["warm_start"] for r in rows}):
+        rs = [r for r in rows if r["warm_start"] == ws]
+        entry = {"beds": len(rs)}
+        for name in ("jacobi", *nets):
+            entry[name] = {k: float(np.mean([r[name][k] for r in rs]))
+                           for k in ("iters_per_solve", "ms_per_step", "fallback_fraction")}
+            entry[name]["unconverged"] = int(sum(r[name]["unconverged"] for r in rs))
+        out[f"warm_start_{ws}"] = entry
+    return out
 
 
 def main(argv=None) -> int:
@@ -188,10 +211,30 @@ def main(argv=None) -> int:
     p.add_argument("--eval-spinup", type=int, default=1500)
     p.add_argument("--eval-steps", type=int, default=40)
     p.add_argument("--out", default=None)
+    p.add_argument("--eval-only", nargs="+", default=None, metavar="NAME=PARAMS.npy",
+                   help="skip the training; evaluate these trained networks against Jacobi")
+    p.add_argument("--eval-seeds", default="5000:5004",
+                   help="held-out beds a:b (the configuration's evaluation set is 2000:2064)")
+    p.add_argument("--warm-starts", default="3",
+                   help="warm-start orders to evaluate from, e.g. 0,3")
     a = p.parse_args(argv)
 
     def log(m):
         print(m, file=sys.stderr, flush=True)
+
+    lo, hi = (int(v) for v in a.eval_seeds.split(":"))
+    starts = tuple(int(v) for v in a.warm_starts.split(","))
+    if a.eval_only:
+        nets = {}
+        for item in a.eval_only:
+            name, path = item.split("=", 1)
+            nets[name] = np.load(path)
+        rows = evaluate(None, range(lo, hi), a.eval_nx, a.eval_ny, a.eval_spinup, a.eval_steps,
+                        log, warm_starts=starts, nets=nets)
+        print(json.dumps({"grid": f"{a.eval_nx}x{a.eval_ny}", "seeds": f"{lo}:{hi}",
+                          "spinup": a.eval_spinup, "steps": a.eval_steps,
+                          "summary": summarize(rows, nets), "beds": rows}))
+        return 0
 
     grid = Grid(a.nx, a.ny, 3.0, 3.0)
     t = time.perf_counter()
@@ -202,8 +245,9 @@ def main(argv=None) -> int:
         np.save(a.out, params.astype(np.float32))
         log(f"[train] saved {a.out}")
     if a.eval_beds:
-        evaluate(params, range(5000, 5000 + a.eval_beds), a.eval_nx, a.eval_ny, a.eval_spinup,
-                 a.eval_steps, log)
+        rows = evaluate(params, range(lo, lo + a.eval_beds), a.eval_nx, a.eval_ny,
+                        a.eval_spinup, a.eval_steps, log, warm_starts=starts)
+        log(f"[eval] {json.dumps(summarize(rows, {'network': params}))}")
     return 0
 
 
