@@ -68,9 +68,9 @@ BYTES_PER_CELL = {
         "switch": 4 * 3 + 8,
     },
 }
-# The survey's figure for one FP64 iteration of the reference's PCG as two fused passes (a
-# matvec storing A p, an update reading it back): what the kernels here avoid moving.
-SURVEY_FP64_PCG_BYTES = 192
+# Bytes per cell of one FP64 iteration of the reference's PCG as two fused passes (a matvec
+# storing A p, an update reading it back): what the kernels here avoid moving.
+TWO_PASS_FP64_PCG_BYTES = 192
 
 # rocprofv3 --pmc summaries of the dominant kernels (traffic per launch)
 TRAFFIC_PROFILES = [
@@ -451,8 +451,8 @@ def main():
     if args.spinup is None:
         args.spinup = 4000 if args.workload == "large_reef" else 1500
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
-                                   "(DESIGN.md), not a measured path", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "wavesim_amd steps on one GPU; sharding over several "
+                                   "is designed, not built or measured", "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
 
@@ -584,7 +584,7 @@ def main():
                      "traffic": measured_traffic(r["dominant"]),
                      "algorithmic_bytes_per_launch": r["bytes_per_cell"] * w.grid.nx * w.grid.ny,
                      "bytes_per_cell": r["bytes_per_cell"],
-                     **({"survey_bytes_per_cell": SURVEY_FP64_PCG_BYTES,
+                     **({"two_pass_bytes_per_cell": TWO_PASS_FP64_PCG_BYTES,
                          "kernels": "k_cg2_matvec + k_cg2_update (one PCG iteration)"}
                         if r["dominant"] == "cg_pair" else {}),
                      "avg_launch_us": 1e3 * r["avg_ms"], "launches": r["launches"],
@@ -599,7 +599,7 @@ def main():
                   for p, v in res.items()},
         "fp32_vs_fp64": {"window_steps": acc_steps + args.warmup + args.steps
                          + args.profile_steps, **accs["fp32"]} if "fp32" in accs else window_acc,
-        "north_star": {"target": "FP32 + learned preconditioner >= 4x the FP64 path",
+        "speedup_target": {"target": "FP32 + learned preconditioner >= 4x the FP64 path",
                        "fp32+network_over_fp64": ratio.get("fp32+network"),
                        "fp32_over_fp64": ratio.get("fp32")},
         "cpu_baseline": cpu,

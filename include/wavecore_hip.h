@@ -1,8 +1,8 @@
 /* wavecore_hip — wavecore's time step (FV + SGN) on MI355X (gfx950), behind a C ABI.
  *
  * Each entry point replaces one item of wavecore's public API (crates/wavecore/src/lib.rs:60-66)
- * and keeps its argument meaning. The Rust crate wavecore-hip binds these
- * one to one; the Python mirror wavesim_amd and the C++ header wavecore_hip.hpp wrap them too.
+ * and keeps its argument meaning. A Rust crate, wavecore-hip, would bind these one to one;
+ * the Python mirror wavesim_amd and the C++ header wavecore_hip.hpp wrap them too.
  *
  * Ownership and threading. A wc_solver owns, on one GPU, the bed, the settings, the current
  * state, every scratch array and the dispersive solve's warm start (the reference keeps the
@@ -12,7 +12,7 @@
  * are independent. Every call is synchronous unless its comment says otherwise.
  *
  * Arrays are padded like the reference's (grid.rs:6-58): (nx+6)*(ny+6) doubles, row-major with
- * y outer, interior cells at [3, nx+3) x [3, ny+3). This build needs nx >= 3 and ny >= 3.
+ * y outer, interior cells at [3, nx+3) x [3, ny+3). The library needs nx >= 3 and ny >= 3.
  *
  * Errors. Every int-returning call returns WC_OK or an error code and leaves a message for
  * wc_last_error() (thread-local) where the reference would panic (grid.rs:21-27,

@@ -1,11 +1,12 @@
 """The learned preconditioner inside the solver (flexible PCG, FP32 form) and the capture of
 training pairs, on the GPU.
 
-The method stays the judge: with any network (here an untrained one) every solve must reach the
-reference's tolerance, falling back to Jacobi when the network stalls, so the state after a
-run matches the Jacobi run's to the tolerance's effect. The trained network
-(wavesim_amd/data/precond_rough_bed.npy, trained by wavesim_amd.precond_train on the GPU) must cut
-the PCG iterations per solve on a bed it was not trained on.
+The tolerance decides, not the network: with any network (here an untrained one) every solve
+must reach the reference's tolerance, falling back to Jacobi when the network stalls, so the
+state after a run matches the Jacobi run's to the tolerance's effect. The trained networks
+(wavesim_amd/data/precond_rough_bed.npy and precond_extrapolated.npy, trained by
+wavesim_amd.precond_train on the GPU on the residuals of the two warm starts) must cut the PCG
+iterations per solve on beds they were not trained on.
 """
 
 from __future__ import annotations

@@ -1493,7 +1493,7 @@ int wc_solver_new(uint64_t nx, uint64_t ny, double dx, double dy, const double* 
   if (!(nx > 0 && ny > 0)) return fail(WC_EINVAL, "grid must have at least one interior cell");
   if (!(dx > 0.0 && dy > 0.0)) return fail(WC_EINVAL, "cell size must be positive");
   if (nx < 3 || ny < 3)
-    return fail(WC_EINVAL, "this build needs at least 3 interior cells along x and along y");
+    return fail(WC_EINVAL, "the library needs at least 3 interior cells along x and along y");
   if ((nx + 6) * (ny + 6) > (uint64_t)0x7fffffff / 2)
     return fail(WC_EINVAL, "grid too large for 32-bit cell indices");
   if (!bed) return fail(WC_EINVAL, "null bed");
@@ -1609,7 +1609,7 @@ int wc_solver_with_manning(wc_solver* s, double n) {
 int wc_solver_with_sponge(wc_solver* s, const wc_sponge_t* sp) {
   TRY(set_device(s));
   if ((int)s->sponges.size() >= MAX_SPONGES)
-    return fail(WC_EINVAL, "at most 16 sponges in this build");
+    return fail(WC_EINVAL, "at most 16 sponges per solver");
   HostSponge hs{};
   hs.width = sp->width;
   hs.strength = sp->strength;
