@@ -15,7 +15,7 @@
 // (read), p (written), c, h in M_n and p, c, h, r, w (read), w, r (written) in U_n: 176 bytes.
 //
 // Layout: 64x16 tiles (one 1024-thread block, a cell per thread; the two-cell halo is a third of
-// the tile where a 64x4 tile's is 113%), exactly 256 blocks, block b taking tiles b, b+256, ...
+// the tile where a 64x4 tile's is 113%), exactly 256 blocks, lane b taking tiles b, b+256, ...
 // The inputs of the next two tiles (their halos, the cells' own values) are loaded into
 // registers while the current tile computes. Each thread keeps running sums over its cells, and the block
 // reduces them once, at the end: rows by the canonical butterfly, then in row order. That
@@ -23,8 +23,9 @@
 // so no separate reduction kernel, barrier or atomic is needed. The oracle's dot order 2 is this
 // order (oracle/wavecore_oracle.cpp: canonical16_sum).
 //
-// Block b runs on XCD b % 8; with 64-cell tile columns a multiple of 8 wide, vertically
-// adjacent tiles (b and b + tiles_x) sit on one XCD and share its L2 for their halo rows.
+// Block b runs on XCD b % 8, and carries the lane lane_of_block gives it: each XCD takes the
+// lanes of an eighth of the tile columns, so a tile's neighbours on all four sides share its
+// XCD's L2 for their halo lines.
 
 #include "wc_tile16.h"
 
@@ -218,6 +219,7 @@ __global__ void __launch_bounds__(PT) k_cg2_init(CgV<double> a) {
   __shared__ double s_u[2][S2Y][S2X], s_c[S1Y][S1X], s_f[S1Y][S1X];
   __shared__ double sums[1][BY][3];
   double acc[3] = {0.0, 0.0, 0.0};
+  const int lane = lane_of_block(g);
   auto halo = [&](int m, double (&v)[2]) {
     v[0] = a.wx[m];
     v[1] = a.wy[m];
@@ -228,8 +230,8 @@ __global__ void __launch_bounds__(PT) k_cg2_init(CgV<double> a) {
     o[2] = a.by[k];
   };
   Pre<2, 3> P;
-  if ((int)blockIdx.x < nt) prefetch(a, blockIdx.x, halo, own, P);
-  for (int tile = blockIdx.x; tile < nt; tile += LANES) {
+  if (lane < nt) prefetch(a, lane, halo, own, P);
+  for (int tile = lane; tile < nt; tile += LANES) {
     stage(P, s_u, s_c, [](const double (&v)[2], double& vx, double& vy) {
       vx = v[0];
       vy = v[1];
@@ -259,7 +261,7 @@ __global__ void __launch_bounds__(PT) k_cg2_init(CgV<double> a) {
   }
   rows_total<3>(acc, sums);
   if (threadIdx.x == 0)
-    for (int q = 0; q < 3; ++q) a.acc_init[q * LANES + blockIdx.x] = acc[q];
+    for (int q = 0; q < 3; ++q) a.acc_init[q * LANES + lane] = acc[q];
 }
 
 // M_n: p_{n-1} = z + beta p_{n-2} (p_0 = z) on the tile + 2, stored at the tile's cells; A p;
@@ -286,7 +288,7 @@ __global__ void __launch_bounds__(PT) k_cg2_matvec(CgV<double> a, unsigned long 
   // two tiles' loads in flight: the first two while the scalars are reduced, then each tile's
   // successor-but-one as soon as the tile is staged
   Pre<6, 1> P[2];
-  const int t0 = blockIdx.x;
+  const int t0 = lane_of_block(g);
   if (t0 < nt) prefetch(a, t0, halo, own, P[0]);
   if (t0 + LANES < nt) prefetch(a, t0 + LANES, halo, own, P[1]);
   double beta = 0.0;
@@ -326,7 +328,7 @@ __global__ void __launch_bounds__(PT) k_cg2_matvec(CgV<double> a, unsigned long 
     if (tile + LANES < nt) body(P[1], tile + LANES);
   }
   rows_total<1>(acc, sums);
-  if (threadIdx.x == 0) a.acc_pap[blockIdx.x] = acc[0];
+  if (threadIdx.x == 0) a.acc_pap[t0] = acc[0];
 }
 
 // U_n: alpha = r.z / p.Ap; A p again from p_{n-1} on the tile + 2; w += alpha p, r -= alpha A p,
@@ -349,7 +351,7 @@ __global__ void __launch_bounds__(PT) k_cg2_update(CgV<double> a, unsigned long 
     o[4] = a.ry[k];
   };
   Pre<2, 5> P[2];  // two tiles' loads in flight
-  const int t0 = blockIdx.x;
+  const int t0 = lane_of_block(g);
   if (t0 < nt) prefetch(a, t0, halo, own, P[0]);
   if (t0 + LANES < nt) prefetch(a, t0 + LANES, halo, own, P[1]);
   double alpha = 0.0;
@@ -399,7 +401,7 @@ __global__ void __launch_bounds__(PT) k_cg2_update(CgV<double> a, unsigned long 
   }
   rows_total<2>(acc, sums);
   if (threadIdx.x == 0)
-    for (int q = 0; q < 2; ++q) a.acc_update[q * LANES + blockIdx.x] = acc[q];
+    for (int q = 0; q < 2; ++q) a.acc_update[q * LANES + t0] = acc[q];
 }
 
 // M_n's prologue alone, one wave: records whether the solve is over after iteration n - 1.
