@@ -122,7 +122,9 @@ __device__ __forceinline__ void cg1_init_body(const CgG& a, int blk) {
 }
 
 template <int L>
-__global__ void __launch_bounds__(PT) k_cg1_init(CgG a) { cg1_init_body<L>(a, blockIdx.x); }
+__global__ void __launch_bounds__(PT) k_cg1_init(CgG a) {
+  cg1_init_body<L>(a, lane_of_block(a.g, a.stride));
+}
 
 // u_0 = D^-1 r_0, q_0 = A u_0; lanes of gamma_0 = r.u, delta_0 = q.u, r.r (slot 0).
 template <int L>
@@ -162,7 +164,9 @@ __device__ __forceinline__ void cg1_start_body(const CgG& a, int blk) {
 }
 
 template <int L>
-__global__ void __launch_bounds__(PT) k_cg1_start(CgG a) { cg1_start_body<L>(a, blockIdx.x); }
+__global__ void __launch_bounds__(PT) k_cg1_start(CgG a) {
+  cg1_start_body<L>(a, lane_of_block(a.g, a.stride));
+}
 
 // Iteration i = n - 1 (see the top of the file).
 template <int L>
@@ -242,7 +246,7 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
 
 template <int L>
 __global__ void __launch_bounds__(PT) k_cg1_iter(CgG a, unsigned long long n) {
-  cg1_iter_body<L>(a, n, blockIdx.x);
+  cg1_iter_body<L>(a, n, lane_of_block(a.g, a.stride));
 }
 
 // K_n's prologue alone, one wave: whether the solve is over before iteration n - 1.

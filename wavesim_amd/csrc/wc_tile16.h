@@ -44,19 +44,22 @@ __device__ __forceinline__ int tiles_all(const Geo& g) {
   return tiles_x(g) * ((g.ny + BY - 1) / BY);
 }
 
-// The summation lane this block carries. Lane b takes tiles b, b + LANES, ... and adds their
-// sums in that order whichever block runs it, so the choice changes no bit. Blocks run on XCD
-// blockIdx.x % 8. When the tile columns are a multiple of 8 and divide the lanes, each XCD
-// takes the lanes of its own eighth of the tile columns (at 64 tile columns: 8 columns by the
-// 4 tile rows of each pass over 256 tiles), so a tile's left and right neighbours, whose halo
-// columns share its 128-byte lines, run on the same XCD and meet in its L2, as the tiles above
-// and below do. Otherwise lane b runs on block b. Block 0 carries lane 0 either way.
-__device__ __forceinline__ int lane_of_block(const Geo& g) {
+// The summation lane this block carries, in a launch of `stride` blocks where lane b takes
+// tiles b, b + stride, ... and adds their sums in that order whichever block runs it, so the
+// choice changes no bit. Blocks run on XCD blockIdx.x % 8. When the tile columns are a multiple
+// of 8 and divide the stride, each XCD takes the lanes of its own eighth of the tile columns (at
+// 64 tile columns and 256 lanes: 8 columns by the 4 tile rows of each pass), so a tile's left
+// and right neighbours, whose halo columns share its 128-byte lines, run on the same XCD and
+// meet in its L2, as the tiles above and below do. Otherwise lane b runs on block b. Block 0
+// carries lane 0 either way.
+__device__ __forceinline__ int lane_of_block(const Geo& g, int stride = LANES) {
   const int tx = tiles_x(g);
   const int b = blockIdx.x;
-  if (tx < 8 || (tx & 7) != 0 || LANES % tx != 0) return b;
+  if ((int)gridDim.x != stride || (stride & 7) != 0 || tx < 8 || (tx & 7) != 0 ||
+      stride % tx != 0)
+    return b;
   const int cols = tx / 8;           // tile columns per XCD
-  const int x = b & 7, s = b >> 3;  // XCD, slot (0 .. LANES / 8 - 1)
+  const int x = b & 7, s = b >> 3;  // XCD, slot (0 .. stride / 8 - 1)
   return (s / cols) * tx + x * cols + s % cols;
 }
 
