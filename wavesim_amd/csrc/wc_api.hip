@@ -148,6 +148,9 @@ struct wc_solver {
   double *wx = nullptr, *wy = nullptr, *rx = nullptr, *ry = nullptr, *px = nullptr,
          *py = nullptr, *ax = nullptr, *ay = nullptr;
   double *zx = nullptr, *zy = nullptr, *qx = nullptr, *qy = nullptr;  // fused PCG: z, 2nd p
+  // the default FP64 PCG (wc_pcg2.hip): the diagonal, r and the two p buffers as interleaved
+  // (x, y) pairs, one 16-byte load per halo point; allocated at its first use
+  double2 *v_dg = nullptr, *v_r = nullptr, *v_pa = nullptr, *v_pb = nullptr;
 
   // the FP32 chosen form
   float *f_sh = nullptr, *f_shu = nullptr, *f_shv = nullptr, *f_slh = nullptr, *f_slhu = nullptr,
@@ -534,6 +537,16 @@ StepArgs step_args(wc_solver* s, double dt, int stage) {
 
 WarmStart ws_args(wc_solver* s);  // below, with the FP32 form
 
+// Whether the FP64 path's solve runs the 64x16-tile kernels of wc_pcg2.hip (pcg dispatches the
+// same way).
+bool uses_pcg2(const wc_solver* s) { return s->pcg_kernels == 0 || s->pcg_kernels >= 4; }
+
+int alloc_pcg2(wc_solver* s) {
+  if (s->v_dg) return WC_OK;
+  for (double2** p : {&s->v_dg, &s->v_r, &s->v_pa, &s->v_pb}) TRY(dalloc(s, p, s->cells));
+  return WC_OK;
+}
+
 SgnArgs sgn_args(wc_solver* s, int stage) {
   SgnArgs a{};
   a.g = s->geo;
@@ -559,6 +572,7 @@ SgnArgs sgn_args(wc_solver* s, int stage) {
   a.ay = s->ay;
   a.zx = s->zx;
   a.zy = s->zy;
+  a.dg2 = uses_pcg2(s) ? s->v_dg : nullptr;
   a.part = s->part;
   a.sc = s->sc;
   a.nonlinear = s->disp.nonlinear ? 1 : 0;
@@ -613,27 +627,24 @@ int pcg3(wc_solver* s, int stage, CgV<V> a, V* pA_x, V* pA_y, V* pB_x, V* pB_y) 
 // batch. p alternates between the two buffers (A, B) by iteration parity: M_n reads p_{n-2}
 // where neighbouring tiles still read it while it writes p_{n-1}.
 int pcg2(wc_solver* s, int stage) {
+  if (!s->v_dg) return fail(WC_EINVAL, "the FP64 PCG's vectors were not allocated before the set-up");
   CgV<double> a{};
   a.g = s->geo;
   a.h = stage == 0 ? s->h : s->h1;
   a.c = s->c;
-  a.dgx = s->dgx;
-  a.dgy = s->dgy;
+  a.dg2 = s->v_dg;
+  a.r2 = s->v_r;
   a.bx = s->bx;
   a.by = s->by;
   a.wx = s->wx;
   a.wy = s->wy;
-  a.rx = s->rx;
-  a.ry = s->ry;
-  a.zx = s->zx;
-  a.zy = s->zy;
   a.acc_init = s->lanes;
   a.acc_pap = s->lanes + 3 * 256;
   a.acc_update = s->lanes + 4 * 256;
   a.sc = s->sc;
   a.tolerance = s->disp.tolerance;
   a.max_iterations = s->disp.max_iterations;
-  double *pA_x = s->px, *pA_y = s->py, *pB_x = s->qx, *pB_y = s->qy;
+  double2 *pA = s->v_pa, *pB = s->v_pb;
   const dim3 grid(256), block(1024);
   TRY(launch(s, F_CG_INIT, k_cg2_init, grid, block, a));
   const unsigned long long maxit = s->disp.max_iterations;
@@ -649,13 +660,10 @@ int pcg2(wc_solver* s, int stage) {
     }
     const unsigned long long first = n;
     for (; n < stop; ++n) {
-      double* cur_x = (n % 2 == 1) ? pA_x : pB_x;
-      double* cur_y = (n % 2 == 1) ? pA_y : pB_y;
-      const double* old_x = (n % 2 == 1) ? pB_x : pA_x;
-      const double* old_y = (n % 2 == 1) ? pB_y : pA_y;
-      TRY(launch(s, F_CG_MATVEC, k_cg2_matvec, grid, block, a, n, old_x, old_y, cur_x, cur_y));
-      TRY(launch(s, F_CG_UPDATE, k_cg2_update, grid, block, a, n, (const double*)cur_x,
-                 (const double*)cur_y));
+      double2* cur = (n % 2 == 1) ? pA : pB;
+      const double2* old = (n % 2 == 1) ? pB : pA;
+      TRY(launch(s, F_CG_MATVEC, k_cg2_matvec, grid, block, a, n, old, cur));
+      TRY(launch(s, F_CG_UPDATE, k_cg2_update, grid, block, a, n, (const double2*)cur));
     }
     if (s->prof_batch && b0 && b1) {
       HIP_OK(hipEventRecord(b1, s->stream));
@@ -807,6 +815,7 @@ int stage(wc_solver* s, double dt, int stage, int mode) {
     TRY(launch(s, F_FLUX, k_flux, tiles(s), dim3(NT), a, flux_mode));
   }
   if (!s->dispersive) return WC_OK;
+  if (uses_pcg2(s)) TRY(alloc_pcg2(s));
   TRY(launch(s, F_SGN_SETUP, k_sgn_setup, tiles(s), dim3(NT), sgn_args(s, stage)));
   TRY(pcg(s, stage));
   TRY(launch(s, F_SGN_FINISH, k_sgn_finish, tiles(s), dim3(NT), a, (const double*)s->wx,

@@ -134,6 +134,7 @@ struct SgnArgs {
   const double *lh, *lhu, *lhv;
   double *c, *dgx, *dgy, *bx, *by, *wx, *wy, *rx, *ry, *px, *py, *ax, *ay;
   double *zx, *zy;  // z = r / diag, kept between the two kernels of a fused iteration
+  double2* dg2;     // set: the diagonal is stored interleaved here instead (wc_pcg2.hip)
   double* part;
   CgScalars* sc;
   int nonlinear;
@@ -147,7 +148,9 @@ template <typename V>
 struct CgV {
   Geo g;
   const V *h, *c;
-  const V *dgx, *dgy;  // the Jacobi diagonal as the set-up stored it (wc_pcg2.hip)
+  const V *dgx, *dgy;  // the Jacobi diagonal as the set-up stored it
+  const double2* dg2;  // wc_pcg2.hip: the diagonal, r and p as interleaved (x, y) pairs
+  double2* r2;
   const double *bx, *by;
   const double *solx, *soly;  // FP32 form: the solve's FP64 start x0 (w its correction); FP64: null
   V *wx, *wy, *rx, *ry, *px, *py, *zx, *zy, *ax, *ay;
@@ -245,11 +248,9 @@ __global__ void k_cg3_check(CgV<V> a, unsigned long long n);
 
 // The FP64 path's default PCG (wc_pcg2.hip): 64x16 tiles, 256 blocks, the oracle's dot order 2.
 __global__ void k_cg2_init(CgV<double> a);
-__global__ void k_cg2_matvec(CgV<double> a, unsigned long long n, const double* __restrict__ pox,
-                             const double* __restrict__ poy, double* __restrict__ pnx,
-                             double* __restrict__ pny);
-__global__ void k_cg2_update(CgV<double> a, unsigned long long n, const double* __restrict__ pcx,
-                             const double* __restrict__ pcy);
+__global__ void k_cg2_matvec(CgV<double> a, unsigned long long n, const double2* __restrict__ po,
+                             double2* __restrict__ pn);
+__global__ void k_cg2_update(CgV<double> a, unsigned long long n, const double2* __restrict__ pc);
 __global__ void k_cg2_check(CgV<double> a, unsigned long long n);
 
 // The FP32 chosen form (wc_fp32.hip).

@@ -258,8 +258,12 @@ __global__ void __launch_bounds__(NT) k_sgn_setup(SgnArgs a) {
     bx = a.lhu[k] - u * a.lh[k] + (s_f[y1][x1 + 1] - s_f[y1][x1 - 1]) * ix;
     by = a.lhv[k] - v * a.lh[k] + (s_f[y1 + 1][x1] - s_f[y1 - 1][x1]) * iy;
   }
-  a.dgx[k] = dgx;
-  a.dgy[k] = dgy;
+  if (a.dg2) {
+    a.dg2[k] = make_double2(dgx, dgy);
+  } else {
+    a.dgx[k] = dgx;
+    a.dgy[k] = dgy;
+  }
   a.bx[k] = bx;
   a.by[k] = by;
 }

@@ -14,6 +14,11 @@
 // the halo work runs from LDS and L2. Per cell and iteration the passes move r, p_old, diag
 // (read), p (written), c, h in M_n and p, c, h, r, w (read), w, r (written) in U_n: 176 bytes.
 //
+// r, p and the diagonal are stored as interleaved (x, y) pairs (double2): a halo point of each
+// is one 16-byte load, and a halo row's partial 128-byte lines are shared by both components.
+// The kernels are bound by how many loads and L2 requests each CU keeps in flight, not by the
+// bytes alone, so halving both for these three vectors shortens both passes.
+//
 // Layout: 64x16 tiles (one 1024-thread block, a cell per thread; the two-cell halo is a third of
 // the tile where a 64x4 tile's is 113%), exactly 256 blocks, lane b taking tiles b, b+256, ...
 // The inputs of the next two tiles (their halos, the cells' own values) are loaded into
@@ -249,8 +254,7 @@ __global__ void __launch_bounds__(PT) k_cg2_init(CgV<double> a) {
       double dgx, dgy;
       own_diag(g, h, s_c, dgx, dgy);
       const double zx = rx / dgx, zy = ry / dgy;
-      a.rx[k] = rx;
-      a.ry[k] = ry;
+      a.r2[k] = make_double2(rx, ry);
       v[0] = bx * bx + by * by;
       v[1] = rx * zx + ry * zy;
       v[2] = rx * rx + ry * ry;
@@ -267,22 +271,22 @@ __global__ void __launch_bounds__(PT) k_cg2_init(CgV<double> a) {
 // M_n: p_{n-1} = z + beta p_{n-2} (p_0 = z) on the tile + 2, stored at the tile's cells; A p;
 // lanes of p.Ap (dispersion.rs:516-518, 537-539).
 __global__ void __launch_bounds__(PT) k_cg2_matvec(CgV<double> a, unsigned long long n,
-                                                   const double* __restrict__ pox,
-                                                   const double* __restrict__ poy,
-                                                   double* __restrict__ pnx,
-                                                   double* __restrict__ pny) {
+                                                   const double2* __restrict__ po,
+                                                   double2* __restrict__ pn) {
   const Geo& g = a.g;
   const int nt = tiles_all(g);
   const bool first = n == 1;
   // z = r / diag at the halo's (mapped) cells from r and the diagonal the set-up stored
   // (k_sgn_setup, dispersion.rs:466-480): no z array
   auto halo = [&](int m, double (&v)[6]) {
-    v[0] = a.rx[m];
-    v[1] = a.ry[m];
-    v[2] = first ? 0.0 : pox[m];
-    v[3] = first ? 0.0 : poy[m];
-    v[4] = a.dgx[m];
-    v[5] = a.dgy[m];
+    const double2 r = a.r2[m], d = a.dg2[m];
+    const double2 p = first ? make_double2(0.0, 0.0) : po[m];
+    v[0] = r.x;
+    v[1] = r.y;
+    v[2] = p.x;
+    v[3] = p.y;
+    v[4] = d.x;
+    v[5] = d.y;
   };
   auto own = [&](int k, double (&o)[1]) { o[0] = a.h[k]; };
   // two tiles' loads in flight: the first two while the scalars are reduced, then each tile's
@@ -316,8 +320,7 @@ __global__ void __launch_bounds__(PT) k_cg2_matvec(CgV<double> a, unsigned long 
     if (k >= 0) {
       double px, py, ax, ay;
       apply_own(g, h, s_u, s_f, px, py, ax, ay);
-      pnx[k] = px;
-      pny[k] = py;
+      pn[k] = make_double2(px, py);
       v = px * ax + py * ay;
     }
     acc[0] = acc[0] + v;
@@ -334,21 +337,22 @@ __global__ void __launch_bounds__(PT) k_cg2_matvec(CgV<double> a, unsigned long 
 // U_n: alpha = r.z / p.Ap; A p again from p_{n-1} on the tile + 2; w += alpha p, r -= alpha A p,
 // z = r / diag (dispersion.rs:519-526, 568-589); lanes of r.r and r.z.
 __global__ void __launch_bounds__(PT) k_cg2_update(CgV<double> a, unsigned long long n,
-                                                   const double* __restrict__ pcx,
-                                                   const double* __restrict__ pcy) {
+                                                   const double2* __restrict__ pc) {
   CgScalars* sc = a.sc;
   const Geo& g = a.g;
   const int nt = tiles_all(g);
   auto halo = [&](int m, double (&v)[2]) {
-    v[0] = pcx[m];
-    v[1] = pcy[m];
+    const double2 p = pc[m];
+    v[0] = p.x;
+    v[1] = p.y;
   };
   auto own = [&](int k, double (&o)[5]) {
+    const double2 r = a.r2[k];
     o[0] = a.h[k];
     o[1] = a.wx[k];
     o[2] = a.wy[k];
-    o[3] = a.rx[k];
-    o[4] = a.ry[k];
+    o[3] = r.x;
+    o[4] = r.y;
   };
   Pre<2, 5> P[2];  // two tiles' loads in flight
   const int t0 = lane_of_block(g);
@@ -386,8 +390,7 @@ __global__ void __launch_bounds__(PT) k_cg2_update(CgV<double> a, unsigned long 
       double dgx, dgy;
       own_diag(g, h, s_c, dgx, dgy);
       const double zx = rx / dgx, zy = ry / dgy;
-      a.rx[k] = rx;
-      a.ry[k] = ry;
+      a.r2[k] = make_double2(rx, ry);
       v[0] = rx * rx + ry * ry;
       v[1] = rx * zx + ry * zy;
     }
