@@ -123,9 +123,10 @@ int wc_solver_with_periodic_y(wc_solver* s);                          /* solver.
 int wc_solver_with_shallow_cells(wc_solver* s, const uint8_t* cells, uint64_t n);
 /* The bed as the solver holds it (with_periodic_y wraps its ghost rows; solver.rs:416). */
 int wc_solver_bed(const wc_solver* s, double* out);
-/* Implementation choices (no counterpart in the reference). In FP64 they change speed, never
- * results; in the FP32 form the PCG layout and the warm start also change the rounding, and the
- * warm start the iteration count (never the tolerance a solve meets):
+/* Implementation choices (no counterpart in the reference). In FP64 they change speed, and
+ * pcg_kernels the summation order of the PCG's dot products (1-3: the oracle's order 1, 5: its
+ * order 2), nothing else; in the FP32 form the PCG layout and the warm start also change the
+ * rounding, and the warm start the iteration count (never the tolerance a solve meets):
  *   "pcg_kernels" 1 = three kernels and two reductions per PCG iteration; 2 = two kernels and
  *                 two reductions; 3 = two kernels, reductions folded into them, 64x4 tiles;
  *                 4 = one kernel per iteration, Chronopoulos-Gear recurrences (FP32 default;
@@ -144,6 +145,9 @@ int wc_solver_bed(const wc_solver* s, double* out);
  *                 2 instead); 2 = 64x4 LDS tiles, each cell reconstructed and each face fluxed
  *                 once; 3 = one wave per 58-column strip walking down a band of rows, the y
  *                 stencil in registers and the x stencil in the wave's LDS row (default);
+ *   "flux_rows"   flux_kernel 3's band height, rows per wave (each wave's tile is 58 x rows
+ *                 cells; the bits do not depend on it): 0 = chosen by the tuner at the first
+ *                 flux launch, timing a few heights (default); n > 0 = n rows;
  *   "flux_reconstruction" FP32 form: 32 = reconstruct in FP32, 64 = in FP64 (default);
  *   "warm_start"  FP32 form: the SGN solves start from an extrapolation of the past solutions,
  *                 exact for solution steps of degree < m (m = 0..5; 0 = the previous
@@ -151,6 +155,12 @@ int wc_solver_bed(const wc_solver* s, double* out);
  *   "warm_start_window" the past steps weighed, K = max(window, m) (default 6): the
  *                 least-norm weights exact to that degree (wc_warm_start_weights). */
 int wc_solver_set_option(wc_solver* s, const char* key, int64_t value);
+/* flux_kernel 3's band height in use (0 before the first flux launch) into *rows, and the
+ * tuner's measurements: up to `capacity` (height, microseconds per launch) pairs into
+ * tried_rows / tried_us, their count into *tried (0 when the height was fixed or not timed).
+ * No counterpart in the reference (its flux is one loop over the cells, solver.rs:625-718). */
+int wc_solver_flux_rows(const wc_solver* s, int32_t* rows, int32_t* tried_rows, double* tried_us,
+                        int32_t capacity, int32_t* tried);
 /* The warm start's weights c_1..c_K (host only, no GPU): x_n ~ x_{n-1} + sum c_j d_{n - j P},
  * d the steps between successive solutions of one RK stage, exact for steps of degree < order
  * and of least norm among such weights. K = max(window, order) values into out. No counterpart
@@ -335,16 +345,25 @@ int wc_sampler_read(const wc_sampler* sm, int32_t slot, double* out, double* tim
  * tiles, so an instance ends bit for bit where it would alone. An instance whose dt is not
  * finite and positive stops there (done_steps[q], last_dt[q] per instance, both may be NULL).
  * While a batch steps its solvers, they must not be used on their own; afterwards their states
- * are read with wc_state_get as usual. The first run tunes the launch widths (persistent grids
- * of the flux and the PCG launches; the other launches run a block per virtual block) by timing
- * candidates; wc_batch_launch_widths reports the choice, wc_batch_set_option("grid_nt" |
- * "grid_pt", w) fixes it ("grid_nt" then for every 256-thread launch), ("tune", 1) tunes again,
- * ("pcg_batch", k) sets the iterations queued between looks at the PCG flags. */
+ * are read with wc_state_get as usual. The instances share the flux layout too: with
+ * flux_kernel 3 (the default) the flux marches one wave per 58-column strip and band of rows.
+ * The first run tunes, by timing candidates, the marching flux's band height (the tile shape of
+ * its waves, 58 x rows cells; wc_batch_flux_rows reports it and the timings,
+ * wc_batch_set_option("flux_rows", r) fixes it, 0 tunes it again) and the launch widths
+ * (persistent grids of the 64x4 tile flux and the PCG launches; the other launches run a block
+ * per virtual block); wc_batch_launch_widths reports them, wc_batch_set_option("grid_nt" |
+ * "grid_pt", w) fixes them ("grid_nt" then for every 256-thread launch), ("tune", 1) tunes
+ * again, ("pcg_batch", k) sets the iterations queued between looks at the PCG flags. None of
+ * these choices changes a bit of the results. */
 typedef struct wc_batch wc_batch;
 int wc_batch_new(wc_solver* const* solvers, uint64_t n, wc_batch** out);
 void wc_batch_free(wc_batch* b);
 int wc_batch_set_option(wc_batch* b, const char* key, int64_t value);
 int wc_batch_launch_widths(const wc_batch* b, int32_t* grid_nt, int32_t* grid_pt);
+/* The marching flux's band height in use (0 with the tile flux) and the tuner's (height,
+ * microseconds per launch) measurements, as wc_solver_flux_rows. */
+int wc_batch_flux_rows(const wc_batch* b, int32_t* rows, int32_t* tried_rows, double* tried_us,
+                       int32_t capacity, int32_t* tried);
 int wc_batch_run_steps(wc_batch* b, uint64_t steps, uint64_t* done_steps, double* last_dt);
 /* Time spent in the batch's PCG iteration launches since wc_batch_set_option("profile", 1),
  * each launch between its own pair of events (while profiling, the stretches of a step are

@@ -114,6 +114,37 @@ def test_the_launch_width_changes_nothing(widths):
     assert_same(alone, batched)
 
 
+@pytest.mark.parametrize("rows", [0, 8, 13, 100000, "tiles"])
+def test_the_flux_band_height_changes_nothing(rows):
+    """The batched marching flux's band height (each wave's tile is 58 x rows cells): the
+    tuner's choice (timed at the first run, the fastest of its candidates), fixed heights that
+    divide no grid or exceed every grid, and the 64x4 tile flux all step each instance to the
+    bits it reaches alone."""
+    ws = workloads()[:3]
+    starts = spun_up(ws, 60)
+    alone = run_alone(ws, starts, 8)
+    pairs = [fresh(w, arr, t0) for w, (arr, t0) in zip(ws, starts)]
+    if rows == "tiles":
+        for g, _ in pairs:
+            g.set_option("flux_kernel", 2)
+    b = Batch([g for g, _ in pairs])
+    if rows != "tiles" and rows:
+        b.set_option("flux_rows", rows)
+    done, _ = b.run_steps([s for _, s in pairs], 8)
+    assert done == [8] * len(ws)
+    used, tried = b.flux_rows()
+    print(f"flux rows {rows}: in use {used}, timed {tried}", flush=True)
+    if rows == "tiles":
+        assert used == 0 and tried == []
+    elif rows == 0:
+        assert len(tried) >= 2 and used == min(tried, key=lambda t: t[1])[0]
+    else:
+        assert used == rows and tried == []
+    batched = [(d, s.time, [a.copy() for a in s.arrays()[:3]], g.dispersion_stats())
+               for d, (g, s) in zip(done, pairs)]
+    assert_same(alone, batched)
+
+
 def test_a_solo_run_between_batch_runs_changes_nothing():
     """Instances stepped alone with other PCG layouts between two batch runs (other lanes and
     blocks write lanes the batch relies on reading zero) still step in the batch as alone; a

@@ -86,6 +86,34 @@ def test_fp32_rough_bed_matches_fp64_within_its_bounds(pcg):
     assert st.unconverged == 0
 
 
+def test_fp32_rough_bed_over_its_whole_4000_step_run():
+    """The rough-bed configuration at its own size and length: 1024x512 cells, the 4000 steps
+    of its run from still water, the FP32 form and the FP64 path each stepped alone (no shared
+    state after t = 0). Compared every 1000 steps, so that drift shows if it grows: surface
+    within 1e-3 m, breaking masks equal on at least 99.9% of cells, volume within 1e-9."""
+    w = configs.c1_rough_bed()
+    assert (w.grid.nx, w.grid.ny, w.steps) == (1024, 512, 4000)
+    g64, g32 = w.solver(Solver), w.solver(fp32_with(None))
+    s64 = w.still_water()
+    s32 = s64.clone()
+    breaking = 0
+    for done in range(1000, w.steps + 1, 1000):
+        assert g64.run_steps(s64, 1000)[0] == 1000
+        assert g32.run_steps(s32, 1000)[0] == 1000
+        r = compare(w, s64, s32, g64, g32)
+        print(f"step {done}, t {s64.time:.1f} s: {r}", flush=True)
+        assert r["eta_max_err_m"] < 1e-3, (done, r)
+        assert r["mask_agreement"] >= 0.999, (done, r)
+        assert r["volume_rel_diff"] < 1e-9, (done, r)
+        assert abs(s32.time - s64.time) < 1e-6 * s64.time
+        breaking = max(breaking, r["breaking_cells"])
+    st64, st32 = g64.dispersion_stats(), g32.dispersion_stats()
+    print(f"iterations per solve: FP64 {st64.iterations / st64.solves:.1f}, FP32 form "
+          f"{st32.iterations / st32.solves:.1f}", flush=True)
+    assert st64.unconverged == 0 and st32.unconverged == 0
+    assert breaking > 0  # the masks are compared where waves break, not on still water alone
+
+
 @pytest.mark.parametrize("recon", [64, 32])
 def test_fp32_large_reef_tile_matches_fp64_within_its_bounds(recon):
     """A tile of the 4096x2048 reef (its slope from the wave-maker towards the reef edge, lobes,
@@ -188,3 +216,34 @@ def test_fp32_marching_flux_equals_the_tile_flux_bit_for_bit(recon):
         for x, y in zip(out[0][:3], out[1][:3]):
             assert np.array_equal(x, y), w.name
         assert out[0][3] == out[1][3]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_flux_band_height_changes_no_bit_and_the_tuner_keeps_the_fastest(precision):
+    """The marching flux's band height (rows per wave: each wave's tile is 58 x rows cells) is a
+    speed choice only: fixed heights of 8 rows, 13 (which divides neither grid) and more than
+    the grid has step to the same bits as the tuner's choice, in both arithmetics, on a rough
+    bed with dry land and on the reef tile. The tuner times its candidates at the first flux
+    launch, reports them, and keeps the fastest."""
+    for w in (configs.c1_rough_bed(nx=200, ny=72, steps=60), configs.c4_crop()):
+        out = []
+        for rows in (0, 8, 13, 100000):
+            g = w.solver(Solver)
+            if precision == "fp32":
+                g.with_precision(PRECISION_FP32)
+            g.set_option("flux_rows", rows)
+            s = w.still_water()
+            s.time = 30.0
+            assert g.run_steps(s, 40)[0] == 40
+            used, tried = g.flux_rows()
+            if rows == 0:
+                print(f"{w.name} {precision}: tuned band height {used} of {tried}", flush=True)
+                assert len(tried) >= 2 and used == min(tried, key=lambda t: t[1])[0]
+                assert all(8 <= r <= w.grid.ny and us > 0 for r, us in tried)
+            else:
+                assert used == min(rows, w.grid.ny) and tried == []
+            out.append([a.copy() for a in s.arrays()[:3]] + [s.time])
+        for other in out[1:]:
+            for x, y in zip(out[0][:3], other[:3]):
+                assert np.array_equal(x, y), w.name
+            assert out[0][3] == other[3]

@@ -198,6 +198,8 @@ struct wc_solver {
   int pcg_walk = 1;         // one-kernel PCG on 64x16 tiles: tiles per block to aim for
   int flux_kernel = 3;
   int march_rows = 0;  // band height of the marching flux launches (0: not yet chosen)
+  int flux_rows = 0;   // option: a fixed band height, or 0 for the tuner's choice
+  std::vector<std::pair<int, double>> march_tried;  // (band height, us per launch) as timed
   unsigned long long pcg_batch = 4;
   uint8_t* held = nullptr;
   double* part = nullptr;
@@ -309,25 +311,68 @@ int prof_collect(wc_solver* s) {
 
 dim3 tiles(const wc_solver* s) { return dim3((unsigned)(s->geo.ntx * s->geo.nty)); }
 
-// The marching flux launch (wc_flux.hip): one wave per 58-column strip and band of rows. The
-// band height is chosen once per solver so that all waves of a launch fit on the GPU at once
-// (as many bands as the free wave slots allow the strips, bands of at least 8 rows): a second,
-// partial round of waves would leave most of the GPU idle while it runs.
+// The marching flux launch (wc_flux.hip): one wave per 58-column strip and band of rows, a
+// tile of 58 x rows cells per wave. Every band height computes the same bits; which is fastest
+// depends on the grid and the GPU (taller bands recompute fewer halo rows, shorter ones give
+// more waves to hide latency with). The tuner picks it at a solver's first flux launch: it
+// times the launch itself (which only writes rates from the state, so extra launches change
+// nothing) at the height where all waves of a launch fit on the GPU at once and at a few others
+// around it, and keeps the fastest. Only a launch that may run again unchanged is timed (one
+// that writes rates or stage 1 from other arrays; stage 2 averages into the state in place):
+// otherwise, and under stream capture, the fitting height serves untimed, and a launch that
+// may not be repeated leaves the choice to the next one that may. Option flux_rows fixes the
+// height instead.
 constexpr int MARCH_MW = TX - 6;
-template <typename... KArgs>
-int march_launch(wc_solver* s, void (*kernel)(KArgs...), dim3* grid, int* rows) {
+template <typename... KArgs, class Run>
+int march_launch(wc_solver* s, void (*kernel)(KArgs...), bool repeatable, dim3* grid, int* rows,
+                 Run run) {
+  const int strips = (s->geo.nx + MARCH_MW - 1) / MARCH_MW;
+  auto grid_of = [&](int r) { return dim3((unsigned)(strips * ((s->geo.ny + r - 1) / r))); };
+  if (s->flux_rows > 0) s->march_rows = std::min(s->flux_rows, s->geo.ny);
   if (!s->march_rows) {
     int per_cu = 0, cus = 0;
     HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, TX, 0));
     HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
-    const int strips = (s->geo.nx + MARCH_MW - 1) / MARCH_MW;
     const int slots = std::max(per_cu, 1) * std::max(cus, 1);
     const int bands = std::max(1, slots / strips);
-    s->march_rows = std::max(8, (s->geo.ny + bands - 1) / bands);
+    const int fit = std::min(s->geo.ny, std::max(8, (s->geo.ny + bands - 1) / bands));
+    if (!repeatable) {
+      *grid = grid_of(fit);
+      *rows = fit;
+      return WC_OK;
+    }
+    s->march_rows = fit;
+    s->march_tried.clear();
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIP_OK(hipStreamIsCapturing(s->stream, &cap));
+    if (cap == hipStreamCaptureStatusNone) {
+      std::vector<int> cand;
+      for (int r : {fit, fit / 2, (fit * 3) / 4, (fit * 3) / 2, 2 * fit, 16, 32, 64, 128}) {
+        r = std::min(s->geo.ny, std::max(8, r));
+        if (std::find(cand.begin(), cand.end(), r) == cand.end()) cand.push_back(r);
+      }
+      hipEvent_t e0, e1;
+      HIP_OK(hipEventCreate(&e0));
+      HIP_OK(hipEventCreate(&e1));
+      double best = 1e30;
+      for (int r : cand) {
+        run(grid_of(r), r);  // warm
+        HIP_OK(hipEventRecord(e0, s->stream));
+        for (int q = 0; q < 3; ++q) run(grid_of(r), r);
+        HIP_OK(hipEventRecord(e1, s->stream));
+        HIP_OK(hipEventSynchronize(e1));
+        HIP_OK(hipGetLastError());
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        const double us = 1e3 * ms / 3.0;
+        s->march_tried.push_back({r, us});
+        if (us < best) best = us, s->march_rows = r;
+      }
+      HIP_OK(hipEventDestroy(e0));
+      HIP_OK(hipEventDestroy(e1));
+    }
   }
-  const int strips = (s->geo.nx + MARCH_MW - 1) / MARCH_MW;
-  const int bands = (s->geo.ny + s->march_rows - 1) / s->march_rows;
-  *grid = dim3((unsigned)(strips * bands));
+  *grid = grid_of(s->march_rows);
   *rows = s->march_rows;
   return WC_OK;
 }
@@ -809,7 +854,11 @@ int stage(wc_solver* s, double dt, int stage, int mode) {
     f.sin_prm = a.sin_prm;
     dim3 grid;
     int rows;
-    TRY(march_launch(s, k_flux_march64, &grid, &rows));
+    TRY(march_launch(s, k_flux_march64, flux_mode != MODE_FINAL, &grid, &rows,
+                     [&](dim3 gr, int r) {
+                       hipLaunchKernelGGL(k_flux_march64, gr, dim3(TX), 0, s->stream, f, a,
+                                          flux_mode, r);
+                     }));
     TRY(launch(s, F_FLUX, k_flux_march64, grid, dim3(TX), f, a, flux_mode, rows));
   } else {
     TRY(launch(s, F_FLUX, k_flux, tiles(s), dim3(NT), a, flux_mode));
@@ -1416,7 +1465,15 @@ int stage32(wc_solver* s, double dt, int stage, int mode) {
   if (s->flux_kernel == 3) {
     dim3 grid;
     int rows;
-    TRY(march_launch(s, s->recon64 ? k_flux_march32r : k_flux_march32, &grid, &rows));
+    TRY(march_launch(s, s->recon64 ? k_flux_march32r : k_flux_march32, true, &grid, &rows,
+                     [&](dim3 gr, int r) {
+                       if (s->recon64)
+                         hipLaunchKernelGGL(k_flux_march32r, gr, dim3(TX), 0, s->stream, f.f,
+                                            f.lh, f.lhu, f.lhv, r);
+                       else
+                         hipLaunchKernelGGL(k_flux_march32, gr, dim3(TX), 0, s->stream, f.f,
+                                            f.lh, f.lhu, f.lhv, r);
+                     }));
     TRY(s->recon64
             ? launch(s, F_FLUX, k_flux_march32r, grid, dim3(TX), f.f, f.lh, f.lhu, f.lhv, rows)
             : launch(s, F_FLUX, k_flux_march32, grid, dim3(TX), f.f, f.lh, f.lhu, f.lhv, rows));
@@ -1793,6 +1850,13 @@ int wc_solver_set_option(wc_solver* s, const char* key, int64_t value) {
     s->flux_kernel = (int)value;
     return WC_OK;
   }
+  if (strcmp(key, "flux_rows") == 0) {
+    if (value < 0 || value > (1 << 30))
+      return fail(WC_EINVAL, "flux_rows must be 0 (tuned) or a band height of rows");
+    s->flux_rows = (int)value;
+    s->march_rows = 0;
+    return WC_OK;
+  }
   if (strcmp(key, "flux_reconstruction") == 0) {
     if (value != 32 && value != 64) return fail(WC_EINVAL, "flux_reconstruction must be 32 or 64");
     s->recon64 = value == 64 ? 1 : 0;
@@ -1844,6 +1908,20 @@ int wc_solver_set_option(wc_solver* s, const char* key, int64_t value) {
     return WC_OK;
   }
   return fail(WC_EINVAL, std::string("unknown option ") + key);
+}
+
+int wc_solver_flux_rows(const wc_solver* s, int32_t* rows, int32_t* tried_rows, double* tried_us,
+                        int32_t capacity, int32_t* tried) {
+  if (!s) return fail(WC_EINVAL, "null solver");
+  if (capacity < 0) return fail(WC_EINVAL, "capacity must be >= 0");
+  if (rows) *rows = s->march_rows;
+  const int n = (int)s->march_tried.size();
+  for (int q = 0; q < std::min(n, (int)capacity); ++q) {
+    if (tried_rows) tried_rows[q] = s->march_tried[q].first;
+    if (tried_us) tried_us[q] = s->march_tried[q].second;
+  }
+  if (tried) *tried = n;
+  return WC_OK;
 }
 
 int wc_solver_set_precision(wc_solver* s, int32_t precision) {
@@ -2898,6 +2976,10 @@ struct wc_batch {
   int lanes = 256;  // the instances' one-kernel PCG lanes
   bool march = true;  // and its layout (pcg_march)
   bool recon64 = false;  // the instances' flux reconstruction in FP64
+  bool march_flux = true;  // the instances' flux: marching strips (flux_kernel 3) or 64x4 tiles
+  int flux_rows = 32;      // the marching flux's band height (tuned unless fixed by hand)
+  bool flux_rows_fixed = false;
+  std::vector<std::pair<int, double>> tune_rows;  // (band height, us per launch) as measured
   unsigned long long maxit = 0, pcg_batch = 4;
   int grid_nt = 2048, grid_pt = 512;  // persistent widths of the flux and the PCG launches
   int grid_st = 1 << 30;  // and of the other 256-thread launches: a block per virtual block
@@ -2915,7 +2997,7 @@ struct wc_batch {
   CgG* cg[2] = {};
   Stage32* stg[2] = {};
   BRate* rate = nullptr;
-  BMap m_fill{}, m_sw{}, m_tiles{}, m_rate{}, m_cg{};
+  BMap m_fill{}, m_sw{}, m_tiles{}, m_rate{}, m_cg{}, m_march{};
   long long* d_off = nullptr;
   double** d_dst = nullptr;
   double* d_stage = nullptr;  // [dt of each active instance | their step parameters]
@@ -3001,7 +3083,8 @@ int batch_build(wc_batch* b) {
   });
   const int n = (int)b->active.size();
   std::vector<int> f_fill(n + 1, 0), f_sw(n + 1, 0), f_tiles(n + 1, 0), f_rate(n + 1, 0),
-      f_cg(n + 1, 0);
+      f_cg(n + 1, 0), f_march(n + 1, 0);
+  auto rows_of = [&](const Geo& g) { return std::max(1, std::min(b->flux_rows, g.ny)); };
   b->off.assign(n + 1, 0);
   for (int q = 0; q < n; ++q) {
     const wc_solver* s = b->inst[b->active[q]];
@@ -3011,6 +3094,8 @@ int batch_build(wc_batch* b) {
     f_tiles[q + 1] = f_tiles[q] + g.ntx * g.nty;
     f_rate[q + 1] = f_rate[q] + (int)rate_blocks((long long)g.nx * g.ny).x;
     f_cg[q + 1] = f_cg[q] + pcg_blocks(s);
+    f_march[q + 1] = f_march[q] + ((g.nx + MARCH_MW - 1) / MARCH_MW) *
+                                      ((g.ny + rows_of(g) - 1) / rows_of(g));
     b->off[q + 1] = b->off[q] + (long long)s->prm_len;
   }
   // host image: the blocks, then the maps and the parameter routes
@@ -3043,6 +3128,7 @@ int batch_build(wc_batch* b) {
       fill[st].push_back(f);
       sw[st].push_back(switch_args32(s, st));
       flux[st].push_back(flux_args32(s, st));
+      flux[st].back().rows = rows_of(s->geo);
       sgn[st].push_back(sgn32_args(s, st));
       cg[st].push_back(cg1_args(s, st));
       stg[st].push_back(stage_args32(s, 0.0, st));  // dt comes per step from d_stage
@@ -3074,6 +3160,7 @@ int batch_build(wc_batch* b) {
   const size_t o_ftiles = put(f_tiles.data(), (n + 1) * sizeof(int));
   const size_t o_frate = put(f_rate.data(), (n + 1) * sizeof(int));
   const size_t o_fcg = put(f_cg.data(), (n + 1) * sizeof(int));
+  const size_t o_fmarch = put(f_march.data(), (n + 1) * sizeof(int));
   const size_t o_off = put(b->off.data(), (n + 1) * sizeof(long long));
   const size_t o_dst = put(dst.data(), n * sizeof(double*));
   if (img.size() > b->args_cap) {
@@ -3103,6 +3190,7 @@ int batch_build(wc_batch* b) {
   b->m_tiles = BMap{reinterpret_cast<const int*>(d + o_ftiles), n, f_tiles[n]};
   b->m_rate = BMap{reinterpret_cast<const int*>(d + o_frate), n, f_rate[n]};
   b->m_cg = BMap{reinterpret_cast<const int*>(d + o_fcg), n, f_cg[n]};
+  b->m_march = BMap{reinterpret_cast<const int*>(d + o_fmarch), n, f_march[n]};
   b->d_off = reinterpret_cast<long long*>(d + o_off);
   b->d_dst = reinterpret_cast<double**>(d + o_dst);
   drop_graphs(b);  // the captured launches point at the old blocks
@@ -3129,12 +3217,23 @@ void enq_fill(wc_batch* b, int st) {
                      b->fill[st], b->m_fill);
 }
 
+// The rates of every instance: the marching flux (one wave per strip and band, a virtual block
+// each) or the 64x4 tile flux at its persistent width.
+void enq_flux(wc_batch* b, int st) {
+  if (b->march_flux)
+    hipLaunchKernelGGL(b->recon64 ? kb_flux_march32r : kb_flux_march32,
+                       bgrid(b->m_march.total, 1 << 30), dim3(TX), 0, b->stream, b->flux[st],
+                       b->m_march);
+  else
+    hipLaunchKernelGGL(b->recon64 ? kb_flux32r : kb_flux32, bgrid(b->m_tiles.total, b->grid_nt),
+                       dim3(NT), 0, b->stream, b->flux[st], b->m_tiles);
+}
+
 void enq_stage_head(wc_batch* b, int st) {  // the switch, the rates, the SGN set-up
   if (b->sw)
     hipLaunchKernelGGL(kb_switch32, bgrid(b->m_sw.total, b->grid_st), dim3(NT), 0, b->stream,
                        b->swa[st], b->m_sw);
-  hipLaunchKernelGGL(b->recon64 ? kb_flux32r : kb_flux32, bgrid(b->m_tiles.total, b->grid_nt),
-                     dim3(NT), 0, b->stream, b->flux[st], b->m_tiles);
+  enq_flux(b, st);
   if (b->dispersive)
     hipLaunchKernelGGL(kb_sgn_setup32, bgrid(b->m_tiles.total, b->grid_st), dim3(NT), 0,
                        b->stream, b->sgn[st], b->m_tiles);
@@ -3357,9 +3456,11 @@ int batch_step(wc_batch* b) {
   return WC_OK;
 }
 
-// The autotuner: the persistent width of the 256-thread launches (timed on the flux launch,
-// which only writes rates) and of the 1024-thread PCG launches (timed on kb_cg1_start, whose
-// outputs the next solve's start rewrites), each the fastest of a few candidates.
+// The autotuner, each choice the fastest of a few candidates timed on launches that change no
+// state: the marching flux's band height, the tile shape of its waves (58 x rows cells; timed
+// on the flux launch, which only writes rates), or with the 64x4 tile flux the persistent width
+// of its launch; and the persistent width of the 1024-thread PCG launches (timed on
+// kb_cg1_start, whose outputs the next solve's start rewrites). No choice changes a bit.
 int batch_tune(wc_batch* b) {
   hipEvent_t e0, e1;
   HIP_OK(hipEventCreate(&e0));
@@ -3378,7 +3479,23 @@ int batch_tune(wc_batch* b) {
   b->tune_pt.clear();
   int best_nt = b->grid_nt, best_pt = b->grid_pt;
   double t_nt = 1e30, t_pt = 1e30;
+  if (b->march_flux && !b->flux_rows_fixed) {
+    b->tune_rows.clear();
+    int best_rows = b->flux_rows;
+    double t_rows = 1e30;
+    for (int r : {8, 16, 24, 32, 48, 64, 96, 128}) {
+      b->flux_rows = r;
+      TRY(batch_build(b));  // the map of strips and bands for this height
+      const double us = time([&] { enq_flux(b, 0); });
+      HIP_OK(hipGetLastError());
+      b->tune_rows.push_back({r, us});
+      if (us < t_rows) t_rows = us, best_rows = r;
+    }
+    b->flux_rows = best_rows;
+    TRY(batch_build(b));
+  }
   for (int w : {512, 1024, 2048, 4096, 8192, 1 << 30}) {
+    if (b->march_flux) break;  // the tile flux's width only
     const int width = std::min(w, b->m_tiles.total);
     if (!b->tune_nt.empty() && b->tune_nt.back().first == width) continue;
     double us = 0.0;
@@ -3444,8 +3561,9 @@ int batch_compatible(const wc_solver* s, const wc_solver* s0) {
     return bfail("a batch runs the one-kernel Jacobi PCG, without capture");
   if (s->device != s0->device) return bfail("a batch's instances share one device");
   if (s->pcg_lanes != s0->pcg_lanes || s->pcg_march != s0->pcg_march ||
-      s->recon64 != s0->recon64)
-    return bfail("a batch's instances share pcg_lanes, pcg_march and flux_reconstruction");
+      s->recon64 != s0->recon64 || (s->flux_kernel == 3) != (s0->flux_kernel == 3))
+    return bfail("a batch's instances share pcg_lanes, pcg_march, flux_reconstruction and "
+                 "the flux layout (flux_kernel 3 or not)");
   if ((s->order == WC_ORDER_FIRST) != (s0->order == WC_ORDER_FIRST) ||
       switch_on(s) != switch_on(s0) || s->dispersive != s0->dispersive)
     return bfail("a batch's instances share the time stepping, the switch and the dispersion");
@@ -3490,6 +3608,7 @@ int wc_batch_new(wc_solver* const* solvers, uint64_t n, wc_batch** out) {
   b->lanes = s0->pcg_lanes;
   b->march = s0->pcg_march != 0;
   b->recon64 = s0->recon64 != 0;
+  b->march_flux = s0->flux_kernel == 3;
   b->sw = switch_on(s0);
   b->dispersive = s0->dispersive;
   b->pcg_batch = s0->pcg_batch;
@@ -3532,6 +3651,15 @@ int wc_batch_set_option(wc_batch* b, const char* key, int64_t value) {
     b->tuned = true;  // set by hand: no tuning
     return WC_OK;
   }
+  if (k == "flux_rows") {  // the marching flux's band height: 0 = tuned at the next run
+    if (value < 0 || value > (1 << 30)) return fail(WC_EINVAL, "flux_rows must be >= 0");
+    HIP_OK(hipStreamSynchronize(b->stream));
+    b->flux_rows_fixed = value > 0;
+    if (value > 0) b->flux_rows = (int)value;
+    b->tune_rows.clear();
+    if (value == 0) b->tuned = false;
+    return batch_build(b);
+  }
   if (k == "tune") {
     b->tuned = value == 0;
     if (value) b->grid_st = 1 << 30;
@@ -3572,6 +3700,20 @@ int wc_batch_profile_read(wc_batch* b, double* total_ms, uint64_t* launches) {
   return WC_OK;
 }
 
+int wc_batch_flux_rows(const wc_batch* b, int32_t* rows, int32_t* tried_rows, double* tried_us,
+                       int32_t capacity, int32_t* tried) {
+  if (!b) return fail(WC_EINVAL, "null batch");
+  if (capacity < 0) return fail(WC_EINVAL, "capacity must be >= 0");
+  if (rows) *rows = b->march_flux ? b->flux_rows : 0;
+  const int n = (int)b->tune_rows.size();
+  for (int q = 0; q < std::min(n, (int)capacity); ++q) {
+    if (tried_rows) tried_rows[q] = b->tune_rows[q].first;
+    if (tried_us) tried_us[q] = b->tune_rows[q].second;
+  }
+  if (tried) *tried = n;
+  return WC_OK;
+}
+
 int wc_batch_launch_widths(const wc_batch* b, int32_t* grid_nt, int32_t* grid_pt) {
   if (!b) return fail(WC_EINVAL, "null batch");
   if (grid_nt) *grid_nt = b->grid_nt;
@@ -3591,7 +3733,8 @@ int wc_batch_run_steps(wc_batch* b, uint64_t steps, uint64_t* done_steps, double
   for (size_t q = 0; q < all; ++q) {
     const wc_solver* s = b->inst[q];
     if (s->pcg_lanes != b->lanes || (s->pcg_march != 0) != b->march ||
-        (s->recon64 != 0) != b->recon64 || (s->order != WC_ORDER_FIRST) != b->rk2 ||
+        (s->recon64 != 0) != b->recon64 || (s->flux_kernel == 3) != b->march_flux ||
+        (s->order != WC_ORDER_FIRST) != b->rk2 ||
         switch_on(s) != b->sw || s->dispersive != b->dispersive)
       return bfail("an instance's settings changed since the batch was made");
     TRY(batch_compatible(s, b->inst[0]));

@@ -293,13 +293,13 @@ struct MarchLds {
 };
 
 // The rates of the band's cells, handed to out(i, j, k, rh, rhu, rhv) (padded i, j). `rows` is
-// the band height; block b takes strip b % strips, band b / strips.
+// the band height; (virtual) block b takes strip b % strips, band b / strips.
 template <typename S, typename R, class Out>
-__device__ __forceinline__ void flux_march(const FluxTileArgs<S>& a, int rows, Out out) {
+__device__ __forceinline__ void flux_march(const FluxTileArgs<S>& a, int rows, int blk, Out out) {
   const Geo& g = a.g;
   __shared__ MarchLds<R> L;
   const int strips = (g.nx + MW - 1) / MW;
-  const int strip = blockIdx.x % strips, band = blockIdx.x / strips;
+  const int strip = blk % strips, band = blk / strips;
   const int l = threadIdx.x;
   const int i = strip * MW + l;  // padded column: interior column strip * MW - 3 + l
   const int j0 = GH + band * rows, j1 = min(j0 + rows, GH + g.ny);
@@ -410,9 +410,10 @@ __global__ void __launch_bounds__(NT) k_flux_tile64(FluxTileArgs<double> f, Step
 // four spilled 81 registers.
 __global__ void __launch_bounds__(TX, 3) k_flux_march64(FluxTileArgs<double> f, StepArgs a, int mode,
                                                      int rows) {
-  flux_march<double, double>(f, rows, [&](int i, int j, int k, double rh, double rhu, double rhv) {
-    stage_out(a, mode, i, j, k, rh, rhu, rhv);
-  });
+  flux_march<double, double>(f, rows, blockIdx.x,
+                             [&](int i, int j, int k, double rh, double rhu, double rhv) {
+                               stage_out(a, mode, i, j, k, rh, rhu, rhv);
+                             });
 }
 
 // FP32 chosen form: the rates for the stage update that follows, in FP64 (the mass rate so
@@ -446,8 +447,8 @@ __global__ void __launch_bounds__(NT) k_flux_tile32r(FluxTileArgs<float> f, doub
 
 template <typename R>
 __device__ __forceinline__ void flux32_march(const FluxTileArgs<float>& f, double* lh,
-                                             double* lhu, double* lhv, int rows) {
-  flux_march<float, R>(f, rows, [&](int, int, int k, double rh, double rhu, double rhv) {
+                                             double* lhu, double* lhv, int rows, int blk) {
+  flux_march<float, R>(f, rows, blk, [&](int, int, int k, double rh, double rhu, double rhv) {
     lh[k] = rh;
     lhu[k] = rhu;
     lhv[k] = rhv;
@@ -456,12 +457,12 @@ __device__ __forceinline__ void flux32_march(const FluxTileArgs<float>& f, doubl
 
 __global__ void __launch_bounds__(TX) k_flux_march32(FluxTileArgs<float> f, double* lh,
                                                      double* lhu, double* lhv, int rows) {
-  flux32_march<float>(f, lh, lhu, lhv, rows);
+  flux32_march<float>(f, lh, lhu, lhv, rows, blockIdx.x);
 }
 
 __global__ void __launch_bounds__(TX) k_flux_march32r(FluxTileArgs<float> f, double* lh,
                                                       double* lhu, double* lhv, int rows) {
-  flux32_march<double>(f, lh, lhu, lhv, rows);
+  flux32_march<double>(f, lh, lhu, lhv, rows, blockIdx.x);
 }
 
 // Many instances in one launch (see kb_fill32).
@@ -481,6 +482,26 @@ __global__ void __launch_bounds__(NT) kb_flux32(const BFlux* __restrict__ args, 
 
 __global__ void __launch_bounds__(NT) kb_flux32r(const BFlux* __restrict__ args, BMap m) {
   kb_flux32_body<double>(args, m);
+}
+
+// The marching flux over many instances: instance q's virtual blocks are its strips times its
+// bands of a.rows rows (the batch's tuned band height, at most its rows).
+template <typename R>
+__device__ __forceinline__ void kb_flux_march32_body(const BFlux* __restrict__ args, BMap m) {
+  for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
+    const int q = batch_instance(m, vb);
+    const BFlux& a = args[q];
+    flux32_march<R>(a.f, a.lh, a.lhu, a.lhv, a.rows, vb - m.first[q]);
+    __syncthreads();  // the next virtual block reuses the LDS row
+  }
+}
+
+__global__ void __launch_bounds__(TX) kb_flux_march32(const BFlux* __restrict__ args, BMap m) {
+  kb_flux_march32_body<float>(args, m);
+}
+
+__global__ void __launch_bounds__(TX) kb_flux_march32r(const BFlux* __restrict__ args, BMap m) {
+  kb_flux_march32_body<double>(args, m);
 }
 
 }  // namespace wc

@@ -442,6 +442,7 @@ struct BFlux {
   FluxTileArgs<float> f;
   double *lh, *lhu, *lhv;
   int recon64;  // reconstruct in FP64 (k_flux_tile32r, kb_flux32r) rather than FP32
+  int rows;     // the marching flux's band height (kb_flux_march32*)
 };
 struct BRate {
   Geo g;
@@ -451,6 +452,8 @@ struct BRate {
 __global__ void kb_fill32(const BFill* args, BMap m);
 __global__ void kb_switch32(const BSwitch* args, BMap m);
 __global__ void kb_flux32(const BFlux* args, BMap m);
+__global__ void kb_flux_march32(const BFlux* args, BMap m);
+__global__ void kb_flux_march32r(const BFlux* args, BMap m);
 __global__ void kb_flux32r(const BFlux* args, BMap m);  // reconstruction in FP64
 __global__ void kb_sgn_setup32(const Sgn32* args, BMap m);
 __global__ void kb_stage32(const Stage32* args, const double* dt, int mode, BMap m);

@@ -69,6 +69,8 @@ SIGNATURES = {
     "wc_solver_bed": (_I32, [_VP, _P]),
     "wc_solver_set_precision": (_I32, [_VP, _I32]),
     "wc_solver_set_option": (_I32, [_VP, ctypes.c_char_p, ctypes.c_int64]),
+    "wc_solver_flux_rows": (_I32, [_VP, ctypes.POINTER(_I32), ctypes.POINTER(_I32), _P, _I32,
+                                   ctypes.POINTER(_I32)]),
     "wc_state_set": (_I32, [_VP, _P, _P, _P, _P, _D]),
     "wc_state_get": (_I32, [_VP, _P, _P, _P, _P, _P]),
     "wc_state_set_time": (_I32, [_VP, _D]),
@@ -117,6 +119,8 @@ SIGNATURES = {
     "wc_batch_free": (None, [_VP]),
     "wc_batch_set_option": (_I32, [_VP, ctypes.c_char_p, ctypes.c_int64]),
     "wc_batch_launch_widths": (_I32, [_VP, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
+    "wc_batch_flux_rows": (_I32, [_VP, ctypes.POINTER(_I32), ctypes.POINTER(_I32), _P, _I32,
+                                  ctypes.POINTER(_I32)]),
     "wc_batch_profile_read": (_I32, [_VP, ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(_U64)]),
     "wc_batch_run_steps": (_I32, [_VP, _U64, ctypes.POINTER(_U64), _P]),
@@ -256,6 +260,17 @@ class Solver:
         """Implementation choices that change speed, never results (wc_solver_set_option)."""
         _check(lib().wc_solver_set_option(self._h, key.encode(), int(value)))
         return self
+
+    def flux_rows(self) -> tuple[int, list[tuple[int, float]]]:
+        """The marching flux's band height in use (0 before the first flux launch) and the
+        tuner's (height, microseconds per launch) measurements (wc_solver_flux_rows)."""
+        rows, n = _I32(), _I32()
+        cap = 16
+        tr = (_I32 * cap)()
+        tu = (ctypes.c_double * cap)()
+        _check(lib().wc_solver_flux_rows(self._h, ctypes.byref(rows), tr, tu, cap,
+                                         ctypes.byref(n)))
+        return rows.value, [(tr[q], tu[q]) for q in range(min(n.value, cap))]
 
     def with_preconditioner(self, params=None) -> "Solver":
         """The dispersive solve's preconditioner: None for the reference's Jacobi diagonal, or
@@ -596,6 +611,17 @@ class Batch:
     def set_option(self, key: str, value: int) -> "Batch":
         _check(lib().wc_batch_set_option(self._b, key.encode(), int(value)))
         return self
+
+    def flux_rows(self) -> tuple[int, list[tuple[int, float]]]:
+        """The marching flux's band height (0 with the tile flux) and the tuner's (height,
+        microseconds per launch) measurements (wc_batch_flux_rows)."""
+        rows, n = _I32(), _I32()
+        cap = 16
+        tr = (_I32 * cap)()
+        tu = (ctypes.c_double * cap)()
+        _check(lib().wc_batch_flux_rows(self._b, ctypes.byref(rows), tr, tu, cap,
+                                        ctypes.byref(n)))
+        return rows.value, [(tr[q], tu[q]) for q in range(min(n.value, cap))]
 
     def launch_widths(self) -> tuple[int, int]:
         """The persistent widths of the 256- and 1024-thread launches (tuned on the first run)."""
