@@ -60,8 +60,8 @@ BYTES_PER_CELL = {
         # read h, hu, hv, phi, lhu and lhv (FP64), lh (FP64); write c, b (FP64); warm start:
         # read x (FP64) and six past steps, write x0 (FP64), w = 0 and the step
         "sgn_setup": 4 * (3 + 1) + 8 * 3 + 4 + 8 * 2 + (8 * 2 + 4 * 2 * 6) + (8 * 2 + 4 * 2 + 4 * 2),
-        # read h pair, hu, hv, phi, b (FP64); write lh, lhu, lhv (FP64)
-        "flux": 4 * (2 + 2 + 1) + 8 + 8 * 3,
+        # read the h, hu and hv pairs, phi, b (FP64); write lh, lhu, lhv (FP64)
+        "flux": 4 * (2 + 2 + 2 + 1) + 8 + 8 * 3,
         # pair, s1, w, stage-1 momentum rates; l1h, lh, lhu, lhv, b (FP64); write pair; the
         # solution x (FP64) and its step read and written
         "sgn_finish": 4 * (6 + 4 + 2 + 2 + 6) + 8 * 5 + 2 * (8 * 2 + 4 * 2),
@@ -76,6 +76,7 @@ TWO_PASS_FP64_PCG_BYTES = 192
 TRAFFIC_PROFILES = [
     os.path.join(ROOT, "profiles", "cg_pair_large_reef_4096x2048_pmc.md"),
     os.path.join(ROOT, "profiles", "cg_iter_large_reef_4096x2048_pmc.md"),
+    os.path.join(ROOT, "profiles", "flux_march_large_reef_4096x2048_pmc.md"),
 ]
 
 
