@@ -42,8 +42,10 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {
     "fp64": {
         "cg_matvec": 8 * (2 + 2 + 2 + 1 + 1 + 2),  # read r, p_old, diag, c, h; write p
-        "cg_update": 8 * (2 + 1 + 1 + 2 + 2 + 2 + 2),  # read p, c, h, w, r; write w, r
-        "cg_pair": 8 * (2 + 2 + 2 + 1 + 1 + 2) + 8 * (2 + 1 + 1 + 2 + 2 + 2 + 2),  # one iteration
+        # read p, c, h, r; write r; every other iteration also read w and p_{n-2}, write w
+        "cg_update": 8 * (2 + 1 + 1 + 2 + 2) + 8 * (2 + 2 + 2) // 2,
+        "cg_pair": 8 * (2 + 2 + 2 + 1 + 1 + 2) + 8 * (2 + 1 + 1 + 2 + 2) + 8 * (2 + 2 + 2) // 2,
+        "cg_flush": 8 * (2 + 2 + 2),  # a solve ending after an odd iteration: read w, p; write w
         "cg_init": 8 * (2 + 1 + 1 + 2 + 2),  # read w, c, h, b; write r
         "sgn_setup": 8 * (7 + 5),  # read h, hu, hv, phi, lh, lhu, lhv; write c, diag, b
         "flux": 8 * (5 + 3),  # read h, hu, hv, b, phi; write lh, lhu, lhv

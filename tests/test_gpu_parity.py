@@ -125,6 +125,27 @@ def test_sgn_fifth_walls_and_periodic_seam(periodic, pcg):
              options={"pcg_kernels": pcg})
 
 
+@pytest.mark.parametrize("cap", [7, 8])
+def test_sgn_solves_stopped_after_an_odd_or_an_even_iteration(cap):
+    """Every solve stopped by max_iterations after 7 (odd) or 8 (even) iterations, on a grid of
+    15 tiles: the FP64 PCG applies the solution's steps two iterations at a time and the last
+    one alone after an odd count, and the solution still ends where the oracle's does."""
+    grid = Grid(150, 70, 1.0, 1.0)
+    bed = Bathymetry.flat(grid, 2.0)
+
+    def configure(s):
+        return s.with_order(Order.Fifth).with_dispersion(
+            Dispersion(tolerance=1e-14, max_iterations=cap))
+
+    s = State.lake_at_rest(grid, bed, 0.0)
+    x, y = grid.centres()
+    X, Y = np.meshgrid(x[GHOST:GHOST + grid.nx], y[GHOST:GHOST + grid.ny])
+    grid.inner(s.h)[:, :] += 0.9 * np.exp(-((X - 60.0) ** 2 + (Y - 30.0) ** 2) / 40.0)
+    g, _, _, _ = lockstep(grid, bed, configure, s, 40, f"sgn capped at {cap}")
+    st = g.dispersion_stats()
+    assert st.solves > 0 and st.unconverged == st.solves and st.iterations == cap * st.solves
+
+
 @pytest.mark.parametrize("pcg", [1, 2, 3, 5])
 def test_sgn_rough_bed_tight_tolerance_with_maker_sponges_and_friction(pcg):
     w = configs.c1_rough_bed(nx=160, ny=64, steps=200)

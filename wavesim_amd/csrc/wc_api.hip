@@ -708,7 +708,8 @@ int pcg2(wc_solver* s, int stage) {
       double2* cur = (n % 2 == 1) ? pA : pB;
       const double2* old = (n % 2 == 1) ? pB : pA;
       TRY(launch(s, F_CG_MATVEC, k_cg2_matvec, grid, block, a, n, old, cur));
-      TRY(launch(s, F_CG_UPDATE, k_cg2_update, grid, block, a, n, (const double2*)cur));
+      TRY(launch(s, F_CG_UPDATE, n % 2 == 0 ? k_cg2_update<1> : k_cg2_update<0>, grid, block, a,
+                 n, (const double2*)cur, (const double2*)old));
     }
     if (s->prof_batch && b0 && b1) {
       HIP_OK(hipEventRecord(b1, s->stream));
@@ -721,7 +722,13 @@ int pcg2(wc_solver* s, int stage) {
     if (s->sc_host->done || n > maxit) break;
     batch = s->pcg_batch;
   }
-  s->prev_it[stage] = s->sc_host->it;
+  const unsigned long long it = s->sc_host->it;
+  // U_it of an odd it left its step for the next iteration, which did not run: w += alpha p_{it-1}
+  // (p_{it-1} is in the buffer that iteration it's matvec wrote, pA for odd it)
+  if (it % 2 == 1)
+    TRY(launch(s, F_CG_FLUSH, k_cg2_flush, over((long long)s->geo.nx * s->geo.ny), dim3(NT), a,
+               it, (const double2*)pA));
+  s->prev_it[stage] = it;
   TRY(launch(s, F_CG_STATS, k_cg_stats, dim3(1), dim3(64), s->sc, 0ull));
   return WC_OK;
 }

@@ -250,7 +250,10 @@ __global__ void k_cg3_check(CgV<V> a, unsigned long long n);
 __global__ void k_cg2_init(CgV<double> a);
 __global__ void k_cg2_matvec(CgV<double> a, unsigned long long n, const double2* __restrict__ po,
                              double2* __restrict__ pn);
-__global__ void k_cg2_update(CgV<double> a, unsigned long long n, const double2* __restrict__ pc);
+template <int PAIR>
+__global__ void k_cg2_update(CgV<double> a, unsigned long long n, const double2* __restrict__ pc,
+                             const double2* __restrict__ pp);
+__global__ void k_cg2_flush(CgV<double> a, unsigned long long it, const double2* __restrict__ p);
 __global__ void k_cg2_check(CgV<double> a, unsigned long long n);
 
 // The FP32 chosen form (wc_fp32.hip).

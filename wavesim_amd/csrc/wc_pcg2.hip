@@ -334,10 +334,19 @@ __global__ void __launch_bounds__(PT) k_cg2_matvec(CgV<double> a, unsigned long 
   if (threadIdx.x == 0) a.acc_pap[t0] = acc[0];
 }
 
-// U_n: alpha = r.z / p.Ap; A p again from p_{n-1} on the tile + 2; w += alpha p, r -= alpha A p,
-// z = r / diag (dispersion.rs:519-526, 568-589); lanes of r.r and r.z.
+// U_n: alpha = r.z / p.Ap; A p again from p_{n-1} on the tile + 2; r -= alpha A p,
+// z = r / diag (dispersion.rs:519-526, 568-589); lanes of r.r and r.z. The solution takes two
+// iterations' steps in one pass: U_n of odd n leaves w alone, U_n of even n (PAIR) makes
+// w = (w + alpha_{n-1} p_{n-2}) + alpha_n p_{n-1}, the reference's two updates in its order, so
+// w reaches the same bits as if each U_n had updated it (nothing reads w during the solve; a
+// solve that ends after an odd iteration gets its last step from k_cg2_flush). w is read and
+// written every other iteration, for one more read of p_{n-2}: 88 instead of 96 bytes per cell
+// and iteration on average.
+template <int PAIR>
 __global__ void __launch_bounds__(PT) k_cg2_update(CgV<double> a, unsigned long long n,
-                                                   const double2* __restrict__ pc) {
+                                                   const double2* __restrict__ pc,
+                                                   const double2* __restrict__ pp) {
+  constexpr int NO = PAIR ? 7 : 3;
   CgScalars* sc = a.sc;
   const Geo& g = a.g;
   const int nt = tiles_all(g);
@@ -346,37 +355,53 @@ __global__ void __launch_bounds__(PT) k_cg2_update(CgV<double> a, unsigned long 
     v[0] = p.x;
     v[1] = p.y;
   };
-  auto own = [&](int k, double (&o)[5]) {
+  auto own = [&](int k, double (&o)[NO]) {
     const double2 r = a.r2[k];
     o[0] = a.h[k];
-    o[1] = a.wx[k];
-    o[2] = a.wy[k];
-    o[3] = r.x;
-    o[4] = r.y;
+    o[1] = r.x;
+    o[2] = r.y;
+    if constexpr (PAIR) {
+      const double2 q = pp[k];  // p_{n-2}
+      o[3] = a.wx[k];
+      o[4] = a.wy[k];
+      o[5] = q.x;
+      o[6] = q.y;
+    }
   };
-  Pre<2, 5> P[2];  // two tiles' loads in flight
+  Pre<2, NO> P[2];  // two tiles' loads in flight
   const int t0 = lane_of_block(g);
   if (t0 < nt) prefetch(a, t0, halo, own, P[0]);
   if (t0 + LANES < nt) prefetch(a, t0 + LANES, halo, own, P[1]);
-  double alpha = 0.0;
+  double alpha = 0.0, alpha_prev = 0.0;
   if (t0 < nt) {
     if (sc->done) return;
     double s[1];
     lane_sums<1>(a.acc_pap, s);
     alpha = sc->rzs[n & 1] / s[0];
-    if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha = alpha;
+    if (PAIR) alpha_prev = sc->alphas[(n - 1) & 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->alpha = alpha;
+      sc->alphas[n & 1] = alpha;
+    }
   }
   const double na = -alpha;
   __shared__ double s_u[2][S2Y][S2X], s_c[S1Y][S1X], s_f[S1Y][S1X];
   __shared__ double sums[1][BY][2];
   double acc[2] = {0.0, 0.0};
-  auto body = [&](Pre<2, 5>& Q, int tile) {
+  auto body = [&](Pre<2, NO>& Q, int tile) {
     stage(Q, s_u, s_c, [](const double (&v)[2], double& vx, double& vy) {
       vx = v[0];
       vy = v[1];
     });
     const int k = Q.k;
-    const double h = Q.o[0], wx0 = Q.o[1], wy0 = Q.o[2], rx0 = Q.o[3], ry0 = Q.o[4];
+    const double h = Q.o[0], rx0 = Q.o[1], ry0 = Q.o[2];
+    double wx0 = 0.0, wy0 = 0.0, qx = 0.0, qy = 0.0;
+    if constexpr (PAIR) {
+      wx0 = Q.o[3];
+      wy0 = Q.o[4];
+      qx = Q.o[5];
+      qy = Q.o[6];
+    }
     __syncthreads();
     if (tile + 2 * LANES < nt) prefetch(a, tile + 2 * LANES, halo, own, Q);
     stencil_f(g, s_u, s_c, s_f);
@@ -384,8 +409,10 @@ __global__ void __launch_bounds__(PT) k_cg2_update(CgV<double> a, unsigned long 
     if (k >= 0) {
       double px, py, ax, ay;
       apply_own(g, h, s_u, s_f, px, py, ax, ay);
-      a.wx[k] = wx0 + alpha * px;
-      a.wy[k] = wy0 + alpha * py;
+      if constexpr (PAIR) {
+        a.wx[k] = (wx0 + alpha_prev * qx) + alpha * px;
+        a.wy[k] = (wy0 + alpha_prev * qy) + alpha * py;
+      }
       const double rx = rx0 + na * ax, ry = ry0 + na * ay;
       double dgx, dgy;
       own_diag(g, h, s_c, dgx, dgy);
@@ -405,6 +432,24 @@ __global__ void __launch_bounds__(PT) k_cg2_update(CgV<double> a, unsigned long 
   rows_total<2>(acc, sums);
   if (threadIdx.x == 0)
     for (int q = 0; q < 2; ++q) a.acc_update[q * LANES + t0] = acc[q];
+}
+
+template __global__ void k_cg2_update<0>(CgV<double>, unsigned long long, const double2*,
+                                         const double2*);
+template __global__ void k_cg2_update<1>(CgV<double>, unsigned long long, const double2*,
+                                         const double2*);
+
+// The step U_it left in a solve that ended after an odd iteration it: w += alpha_it p_{it-1}.
+__global__ void __launch_bounds__(NT) k_cg2_flush(CgV<double> a, unsigned long long it,
+                                                  const double2* __restrict__ p) {
+  const Geo& g = a.g;
+  const int t = blockIdx.x * NT + threadIdx.x;
+  if (t >= g.nx * g.ny) return;
+  const int k = (GH + t / g.nx) * g.w + GH + t % g.nx;
+  const double alpha = a.sc->alphas[it & 1];
+  const double2 q = p[k];
+  a.wx[k] = a.wx[k] + alpha * q.x;
+  a.wy[k] = a.wy[k] + alpha * q.y;
 }
 
 // M_n's prologue alone, one wave: records whether the solve is over after iteration n - 1.
