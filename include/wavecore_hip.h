@@ -370,6 +370,14 @@ int wc_batch_run_steps(wc_batch* b, uint64_t steps, uint64_t* done_steps, double
  * issued directly, not from the graphs). Measurement only; no counterpart in the reference. */
 int wc_batch_profile_read(wc_batch* b, double* total_ms, uint64_t* launches);
 
+/* Self-check of the flux kernels' division by a divisor every cell shares (the 60 of the
+ * fifth-order formula, dx, dy; a reciprocal computed once, three instructions per quotient where
+ * the division's scaling and fix-up steps would change nothing): shared[i] = x[i] / y that way and
+ * plain[i] = x[i] / y by the device's own division, on the current device, for n <= 2^26
+ * values. The two must agree bit for bit. No counterpart in the reference. */
+int wc_check_shared_division(double y, const double* x, uint64_t n, double* shared,
+                             double* plain);
+
 /* Measurement hooks for benchmarks: device time in milliseconds of each kernel family over the
  * steps since the last reset, measured with HIP events on the solver's stream, and the kernel
  * launch counts. names receives a ';'-separated list. on = 1 puts a pair of events around every

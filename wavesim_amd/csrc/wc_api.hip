@@ -2926,6 +2926,28 @@ double wc_manning_factor(double h, double speed, double dt, double n) {
   return 1.0 / (1.0 + dt * k * speed);
 }
 
+int wc_check_shared_division(double y, const double* x, uint64_t n, double* shared,
+                              double* plain) {
+  if (!x || !shared || !plain) return fail(WC_EINVAL, "null buffer");
+  if (n == 0) return WC_OK;
+  if (n > (1ull << 26)) return fail(WC_EINVAL, "at most 2^26 values per check");
+  double* d = nullptr;
+  HIP_OK(hipMalloc(&d, 3 * n * sizeof(double)));
+  auto run = [&]() -> int {
+    HIP_OK(hipMemcpy(d, x, n * sizeof(double), hipMemcpyHostToDevice));
+    const int nn = (int)n;
+    hipLaunchKernelGGL(k_division_check, dim3((nn + NT - 1) / NT), dim3(NT), 0, 0, y, d, nn,
+                       d + n, d + 2 * n);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpy(shared, d + n, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(plain, d + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost));
+    return WC_OK;
+  };
+  const int rc = run();
+  (void)hipFree(d);
+  return rc;
+}
+
 int wc_profile_enable(wc_solver* s, int32_t on) {
   TRY(set_device(s));
   TRY(prof_collect(s));

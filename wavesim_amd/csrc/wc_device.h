@@ -48,6 +48,50 @@ __device__ __forceinline__ int map_y(int j, int ny, int periodic) {
 __device__ __forceinline__ bool is_ghost_x(int i, int nx) { return i < GH || i >= nx + GH; }
 __device__ __forceinline__ bool is_ghost_y(int j, int ny) { return j < GH || j >= ny + GH; }
 
+// ---- division by a divisor every cell shares ------------------------------------------------
+// hipcc expands an FP64 division x / y into v_div_scale (of y, then of x), v_rcp_f64 of the
+// scaled y, two Newton steps on that reciprocal (four FMAs), the product m = x r, the remainder
+// e = fma(-y, m, x), v_div_fmas (fma(e, r, m), scaled back when v_div_scale scaled) and
+// v_div_fixup (zeros, infinities, NaN, overflow and underflow): eleven instructions, one of them
+// quarter rate, with the correctly rounded quotient. When y is the same for every cell (the 60
+// of the fifth-order formula, dx, dy), the reciprocal and its refinement are computed once
+// (quot_of), and where neither v_div_scale nor v_div_fixup changes anything (0 < |x| within
+// [2^-900, 2^700], 2^-60 <= |y| <= 2^60) the quotient takes the three instructions the expansion
+// runs there (div_q), so its bits are the division's; other x take the full division.
+struct Quot {
+  double y, r;
+  bool fast;
+};
+
+// The same double in every lane, held in scalar registers.
+__device__ __forceinline__ double wave_uniform(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// y must be the same in every lane.
+__device__ __forceinline__ Quot quot_of(double y) {
+  double v = y;
+  asm volatile("" : "+v"(v));  // a run-time value: the reciprocal is the instruction's, not folded
+  const double rcp = __builtin_amdgcn_rcp(v);
+  const double e0 = fma(-v, rcp, 1.0);
+  const double r1 = fma(rcp, e0, rcp);
+  const double e1 = fma(-v, r1, 1.0);
+  const double ay = fabs(y);
+  return {y, wave_uniform(fma(r1, e1, r1)), ay >= 0x1p-60 && ay <= 0x1p60};
+}
+
+__device__ __forceinline__ double div_q(double x, const Quot& d) {
+  const double ax = fabs(x);
+  if (d.fast && ax >= 0x1p-900 && ax <= 0x1p700) {
+    const double m = x * d.r;
+    return fma(fma(-d.y, m, x), d.r, m);
+  }
+  return x / d.y;
+}
+
 // A scalar field at any padded cell, from its interior.
 __device__ __forceinline__ double ghosted(const double* a, const Geo& g, int i, int j) {
   return a[map_y(j, g.ny, g.periodic) * g.w + map_x(i, g.nx)];

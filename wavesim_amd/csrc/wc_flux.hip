@@ -37,10 +37,30 @@ __device__ __forceinline__ R minmod3r(R a, R b, R c) {
   return zero;
 }
 
+// x / 60 of the fifth-order formula: in FP64 through the shared reciprocal (the division's bits).
+template <typename R>
+__device__ __forceinline__ R div60(R x, const Quot& q60);
+template <>
+__device__ __forceinline__ float div60<float>(float x, const Quot&) {
+  return x / 60.0f;
+}
+template <>
+__device__ __forceinline__ double div60<double>(double x, const Quot& q60) {
+  return div_q(x, q60);
+}
+
+// The divisors the rates share, their reciprocals computed once per wave.
+struct FluxDivs {
+  Quot q60, qdx, qdy;
+};
+__device__ __forceinline__ FluxDivs flux_divs(const Geo& g) {
+  return {quot_of(60.0), quot_of(g.dx), quot_of(g.dy)};
+}
+
 // solver.rs:125-138 in R arithmetic.
 template <typename R>
-__device__ __forceinline__ void face_values_r(R low2, R low, R c, R high, R high2, R blend, R& lo,
-                                              R& hi) {
+__device__ __forceinline__ void face_values_r(R low2, R low, R c, R high, R high2, R blend,
+                                              const Quot& q60, R& lo, R& hi) {
   const R s = minmod3r<R>(R(2) * (c - low), R(0.5) * ((c - low) + (high - c)), R(2) * (high - c));
   const R mc_lo = c - R(0.5) * s, mc_hi = c + R(0.5) * s;
   if (blend <= R(0)) {
@@ -48,8 +68,10 @@ __device__ __forceinline__ void face_values_r(R low2, R low, R c, R high, R high
     hi = mc_hi;
     return;
   }
-  const R fifth_hi = (R(2) * low2 - R(13) * low + R(47) * c + R(27) * high - R(3) * high2) / R(60);
-  const R fifth_lo = (R(2) * high2 - R(13) * high + R(47) * c + R(27) * low - R(3) * low2) / R(60);
+  const R fifth_hi =
+      div60<R>(R(2) * low2 - R(13) * low + R(47) * c + R(27) * high - R(3) * high2, q60);
+  const R fifth_lo =
+      div60<R>(R(2) * high2 - R(13) * high + R(47) * c + R(27) * low - R(3) * low2, q60);
   lo = mc_lo + blend * (fifth_lo - mc_lo);
   hi = mc_hi + blend * (fifth_hi - mc_hi);
 }
@@ -61,7 +83,8 @@ template <typename R>
 __device__ __forceinline__ void reconstruct5(const R (&e)[5], const double (&d)[5],
                                              const R (&n)[5], const R (&t)[5],
                                              const double (&b)[5], const R (&p)[5], int order,
-                                             SideT<R>& lo_out, SideT<R>& hi_out) {
+                                             const Quot& q60, SideT<R>& lo_out,
+                                             SideT<R>& hi_out) {
   const double h1 = d[1], h2 = d[2], h3 = d[3];
   const SideT<R> first{h2, b[2], n[2], t[2]};
   if (order == 0 || h1 < WC_H_SECOND || h2 < WC_H_SECOND || h3 < WC_H_SECOND) {
@@ -77,9 +100,9 @@ __device__ __forceinline__ void reconstruct5(const R (&e)[5], const double (&d)[
     blend = (R)bl;
   }
   R eta_lo, eta_hi, un_lo, un_hi, ut_lo, ut_hi;
-  face_values_r<R>(e[0], e[1], e[2], e[3], e[4], blend, eta_lo, eta_hi);
-  face_values_r<R>(n[0], n[1], n[2], n[3], n[4], blend, un_lo, un_hi);
-  face_values_r<R>(t[0], t[1], t[2], t[3], t[4], blend, ut_lo, ut_hi);
+  face_values_r<R>(e[0], e[1], e[2], e[3], e[4], blend, q60, eta_lo, eta_hi);
+  face_values_r<R>(n[0], n[1], n[2], n[3], n[4], blend, q60, un_lo, un_hi);
+  face_values_r<R>(t[0], t[1], t[2], t[3], t[4], blend, q60, ut_lo, ut_hi);
   const double cb = b[2];
   const double s_z = 0.25 * (b[3] - b[1]);
   SideT<R> l, r;
@@ -104,7 +127,8 @@ __device__ __forceinline__ void reconstruct5(const R (&e)[5], const double (&d)[
 template <typename R>
 __device__ __forceinline__ void reconstruct_r(const R* e, const double* d, const R* n, const R* t,
                                               const double* b, const R* p, int stride, int order,
-                                              SideT<R>& lo_out, SideT<R>& hi_out) {
+                                              const Quot& q60, SideT<R>& lo_out,
+                                              SideT<R>& hi_out) {
   R e5[5], n5[5], t5[5], p5[5];
   double d5[5], b5[5];
 #pragma unroll
@@ -116,7 +140,7 @@ __device__ __forceinline__ void reconstruct_r(const R* e, const double* d, const
     b5[m] = b[m * stride];
     p5[m] = p[m * stride];
   }
-  reconstruct5<R>(e5, d5, n5, t5, b5, p5, order, lo_out, hi_out);
+  reconstruct5<R>(e5, d5, n5, t5, b5, p5, order, q60, lo_out, hi_out);
 }
 
 template <typename R>
@@ -155,6 +179,7 @@ __device__ __forceinline__ void flux_tile(const FluxTileArgs<S>& a, int mode, in
   const int tx = tile % g.ntx, ty = tile / g.ntx;
   const int i0 = GH + tx * TX, j0 = GH + ty * TY;
   __shared__ TileLds<R> L;
+  const FluxDivs dv = flux_divs(g);
   // stage the tile and three rings: eta = h + b, h, u, v, b, phi
   for (int t = threadIdx.x; t < BX * BY; t += NT) {
     const int lx = t % BX, ly = t / BX;
@@ -192,14 +217,14 @@ __device__ __forceinline__ void flux_tile(const FluxTileArgs<S>& a, int mode, in
       const int y = row + 3, x0 = cx;  // stencil columns cx..cx+4 = tile cols cx-3..cx+1
       reconstruct_r<R>(&L.u.box.eta[y][x0], &L.u.box.h[y][x0], &L.u.box.un_x[y][x0],
                        &L.u.box.un_y[y][x0], &L.u.box.b[y][x0], &L.u.box.phi[y][x0], 1, a.order,
-                       L.xlo[row][cx], L.xhi[row][cx]);
+                       dv.q60, L.xlo[row][cx], L.xhi[row][cx]);
     } else {
       const int qy = q - TY * RXN;
       const int col = qy % TX, ry = qy / TX;
       const int x = col + 3, y0 = ry;
       reconstruct_r<R>(&L.u.box.eta[y0][x], &L.u.box.h[y0][x], &L.u.box.un_y[y0][x],
                        &L.u.box.un_x[y0][x], &L.u.box.b[y0][x], &L.u.box.phi[y0][x], BX,
-                       a.order, L.ylo[ry][col], L.yhi[ry][col]);
+                       a.order, dv.q60, L.ylo[ry][col], L.yhi[ry][col]);
     }
   }
   __syncthreads();
@@ -230,11 +255,11 @@ __device__ __forceinline__ void flux_tile(const FluxTileArgs<S>& a, int mode, in
   const SideT<R>& xs_hi = L.xhi[ly][lx + 1];
   const SideT<R>& ys_lo = L.ylo[ly + 1][lx];
   const SideT<R>& ys_hi = L.yhi[ly + 1][lx];
-  const double xs = -WC_G * 0.5 * (xs_lo.h + xs_hi.h) * (xs_hi.z - xs_lo.z) / g.dx;
-  const double ys = -WC_G * 0.5 * (ys_lo.h + ys_hi.h) * (ys_hi.z - ys_lo.z) / g.dy;
-  rh = -(xh.f0 - xl.f0) / g.dx - (yh.f0 - yl.f0) / g.dy;
-  rhu = -((xh.f1 + xh.cl) - (xl.f1 + xl.cr)) / g.dx + xs - (yh.f2 - yl.f2) / g.dy;
-  rhv = -((yh.f1 + yh.cl) - (yl.f1 + yl.cr)) / g.dy + ys - (xh.f2 - xl.f2) / g.dx;
+  const double xs = div_q(-WC_G * 0.5 * (xs_lo.h + xs_hi.h) * (xs_hi.z - xs_lo.z), dv.qdx);
+  const double ys = div_q(-WC_G * 0.5 * (ys_lo.h + ys_hi.h) * (ys_hi.z - ys_lo.z), dv.qdy);
+  rh = div_q(-(xh.f0 - xl.f0), dv.qdx) - div_q(yh.f0 - yl.f0, dv.qdy);
+  rhu = div_q(-((xh.f1 + xh.cl) - (xl.f1 + xl.cr)), dv.qdx) + xs - div_q(yh.f2 - yl.f2, dv.qdy);
+  rhv = div_q(-((yh.f1 + yh.cl) - (yl.f1 + yl.cr)), dv.qdy) + ys - div_q(xh.f2 - xl.f2, dv.qdx);
   if (a.has_drive) {
     const int ii = i - GH;
     double src = 0.0;
@@ -310,7 +335,7 @@ __device__ __forceinline__ void flux_march(const FluxTileArgs<S>& a, int rows, i
   for (int m = 0; m < 5; ++m) w[m] = load_cell<S, R>(a, i, j0 - 3 + m);
   SideT<R> ylo_prev{}, yhi_prev{};
   Face fy_prev{};
-  const double dx = g.dx, dy = g.dy;
+  const FluxDivs dv = flux_divs(g);
   for (int jr = j0 - 1; jr <= j1; ++jr) {
     Cell5<R> nxt{};
     if (jr < j1) nxt = load_cell<S, R>(a, i, jr + 3);  // in flight while this row computes
@@ -328,7 +353,7 @@ __device__ __forceinline__ void flux_march(const FluxTileArgs<S>& a, int rows, i
         b[m] = w[m].b;
         p[m] = w[m].p;
       }
-      reconstruct5<R>(e, d, n, t, b, p, a.order, ylo, yhi);
+      reconstruct5<R>(e, d, n, t, b, p, a.order, dv.q60, ylo, yhi);
     }
     Face fy{};
     if (jr >= j0) fy = face_r<R>(yhi_prev, ylo);
@@ -355,7 +380,7 @@ __device__ __forceinline__ void flux_march(const FluxTileArgs<S>& a, int rows, i
           b[m] = L.b[q];
           p[m] = L.p[q];
         }
-        reconstruct5<R>(e, d, n, t, b, p, a.order, xlo, xhi);
+        reconstruct5<R>(e, d, n, t, b, p, a.order, dv.q60, xlo, xhi);
       }
       L.xhi[l] = xhi;
       __syncthreads();
@@ -367,11 +392,14 @@ __device__ __forceinline__ void flux_march(const FluxTileArgs<S>& a, int rows, i
         const Face& xh = L.fx[l + 1];
         const Face& yl = fy_prev;
         const Face& yh = fy;
-        const double xs = -WC_G * 0.5 * (xlo.h + xhi.h) * (xhi.z - xlo.z) / dx;
-        const double ys = -WC_G * 0.5 * (ylo_prev.h + yhi_prev.h) * (yhi_prev.z - ylo_prev.z) / dy;
-        double rh = -(xh.f0 - xl.f0) / dx - (yh.f0 - yl.f0) / dy;
-        const double rhu = -((xh.f1 + xh.cl) - (xl.f1 + xl.cr)) / dx + xs - (yh.f2 - yl.f2) / dy;
-        const double rhv = -((yh.f1 + yh.cl) - (yl.f1 + yl.cr)) / dy + ys - (xh.f2 - xl.f2) / dx;
+        const double xs = div_q(-WC_G * 0.5 * (xlo.h + xhi.h) * (xhi.z - xlo.z), dv.qdx);
+        const double ys =
+            div_q(-WC_G * 0.5 * (ylo_prev.h + yhi_prev.h) * (yhi_prev.z - ylo_prev.z), dv.qdy);
+        double rh = div_q(-(xh.f0 - xl.f0), dv.qdx) - div_q(yh.f0 - yl.f0, dv.qdy);
+        const double rhu = div_q(-((xh.f1 + xh.cl) - (xl.f1 + xl.cr)), dv.qdx) + xs -
+                           div_q(yh.f2 - yl.f2, dv.qdy);
+        const double rhv = div_q(-((yh.f1 + yh.cl) - (yl.f1 + yl.cr)), dv.qdy) + ys -
+                           div_q(xh.f2 - xl.f2, dv.qdx);
         if (a.has_drive) {
           const int ic = ii - GH;
           double src = 0.0;
@@ -460,7 +488,7 @@ __global__ void __launch_bounds__(TX) k_flux_march32(FluxTileArgs<float> f, doub
   flux32_march<float>(f, lh, lhu, lhv, rows, blockIdx.x);
 }
 
-__global__ void __launch_bounds__(TX) k_flux_march32r(FluxTileArgs<float> f, double* lh,
+__global__ void __launch_bounds__(TX, 3) k_flux_march32r(FluxTileArgs<float> f, double* lh,
                                                       double* lhu, double* lhv, int rows) {
   flux32_march<double>(f, lh, lhu, lhv, rows, blockIdx.x);
 }
@@ -500,7 +528,7 @@ __global__ void __launch_bounds__(TX) kb_flux_march32(const BFlux* __restrict__ 
   kb_flux_march32_body<float>(args, m);
 }
 
-__global__ void __launch_bounds__(TX) kb_flux_march32r(const BFlux* __restrict__ args, BMap m) {
+__global__ void __launch_bounds__(TX, 3) kb_flux_march32r(const BFlux* __restrict__ args, BMap m) {
   kb_flux_march32_body<double>(args, m);
 }
 

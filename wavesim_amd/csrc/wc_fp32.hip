@@ -110,8 +110,7 @@ __device__ __forceinline__ void switch32_body(const Geo& g, const float* h, cons
       const double ex = ((depth(k + 1) + b[k + 1]) - (depth(k - 1) + b[k - 1])) * ix;
       const double ey = ((depth(k + g.w) + b[k + g.w]) - (depth(k - g.w) + b[k - g.w])) * iy;
       const double ratio = wc_max((hc + b[k]) / hc, 0.0);
-      double off = wc_max(wc_smoothstep(br.er_lo, br.er_hi, ratio),
-                          wc_smoothstep(br.sl_lo, br.sl_hi, wc_hypot(ex, ey)));
+      double off = switch_off(br, ratio, ex, ey);
       if (br.memory && br.memory[k] != 0.0) off = 1.0;
       v = 1.0 - off;
     }

@@ -16,6 +16,27 @@ struct Breaking {
   const double* memory;
 };
 
+// The switch's "off" (dispersion.rs:140-150): the larger of the erosion smoothstep of the
+// depth ratio and the steepness smoothstep of hypot(ex, ey). The portable hypot is the costly
+// part of the switch, and most cells do not need its last bits: where the larger slope component
+// alone already puts hypot past the steepness band (hypot >= max(|ex|, |ey|) and the bits of a
+// correctly rounded hypot stay within 2^-50 of it), the smoothstep is exactly 1; where 1.5 times
+// it stays below the band (hypot <= sqrt 2 max(|ex|, |ey|)), exactly 0; and where the erosion
+// smoothstep is 1 the larger of the two is 1 whatever the other. Those cells skip hypot; the
+// result has the bits of the full expression.
+__device__ __forceinline__ double switch_off(const Breaking& br, double ratio, double ex,
+                                             double ey) {
+  const double er = wc_smoothstep(br.er_lo, br.er_hi, ratio);
+  const double ax = fabs(ex), ay = fabs(ey);
+  const double mx = ax > ay ? ax : ay;
+  if (br.sl_lo < br.sl_hi && mx <= 0x1p1000) {  // finite slopes (NaN fails the test)
+    if (er == 1.0) return er;
+    if (mx >= br.sl_hi * (1.0 + 0x1p-40)) return wc_max(er, 1.0);
+    if (1.5 * mx <= br.sl_lo) return wc_max(er, 0.0);
+  }
+  return wc_max(er, wc_smoothstep(br.sl_lo, br.sl_hi, wc_hypot(ex, ey)));
+}
+
 // breaking_into's indicator at interior cell (i, j), padded index k (dispersion.rs:183-201):
 // 1 - phi with phi the switch of the thresholds br, 0 where the water is shallower than
 // H_DISPERSION_MIN.
@@ -28,8 +49,7 @@ __device__ __forceinline__ double breaking_at(const double* h, const double* b, 
   const double ey =
       ((ghosted(h, g, i, j + 1) + b[k + g.w]) - (ghosted(h, g, i, j - 1) + b[k - g.w])) * iy;
   const double ratio = wc_max((hc + b[k]) / hc, 0.0);
-  double off = wc_max(wc_smoothstep(br.er_lo, br.er_hi, ratio),
-                      wc_smoothstep(br.sl_lo, br.sl_hi, wc_hypot(ex, ey)));
+  double off = switch_off(br, ratio, ex, ey);
   if (br.memory && br.memory[k] != 0.0) off = 1.0;
   return 1.0 - (1.0 - off);
 }
@@ -313,6 +333,8 @@ __global__ void k_cg_matvec32(Sgn32 a, int first, const float* psx, const float*
 __global__ void k_cg_update32(Sgn32 a, const float* pcx, const float* pcy);
 __global__ void k_cg_reduce32(Sgn32 a, int kind);
 __global__ void k_stage32(Stage32 a, int mode);
+__global__ void k_division_check(double y, const double* x, int n, double* shared,
+                                 double* plain);
 __global__ void k_max_rate32(Geo g, const float* h, const float* hl, const float* hu,
                              const float* hul, const float* hv, const float* hvl,
                              unsigned long long* out);

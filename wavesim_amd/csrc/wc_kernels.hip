@@ -60,8 +60,7 @@ __device__ __forceinline__ double switch_at(const Geo& g, const double* h, const
   const double ex = ((h[k + 1] + b[k + 1]) - (h[k - 1] + b[k - 1])) * ix;
   const double ey = ((h[k + g.w] + b[k + g.w]) - (h[k - g.w] + b[k - g.w])) * iy;
   const double ratio = wc_max((h[k] + b[k]) / h[k], 0.0);
-  double off = wc_max(wc_smoothstep(br.er_lo, br.er_hi, ratio),
-                      wc_smoothstep(br.sl_lo, br.sl_hi, wc_hypot(ex, ey)));
+  double off = switch_off(br, ratio, ex, ey);
   if (br.memory && br.memory[k] != 0.0) off = 1.0;  // a front, or the water just behind it
   return 1.0 - off;
 }
@@ -694,6 +693,17 @@ __global__ void __launch_bounds__(NT) k_sum_tiles(const double* part, int np, do
   double s[1];
   final_sums<1>(part, np, s);
   if (threadIdx.x == 0) *out = s[0];
+}
+
+// x / y for n values, both through the shared reciprocal of the flux kernels (div_q) and by the
+// device's own division, for a host-side check that the two agree bit for bit.
+__global__ void __launch_bounds__(NT) k_division_check(double y, const double* x, int n,
+                                                       double* shared, double* plain) {
+  const Quot q = quot_of(y);
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= n) return;
+  shared[i] = div_q(x[i], q);
+  plain[i] = x[i] / y;
 }
 
 }  // namespace wc

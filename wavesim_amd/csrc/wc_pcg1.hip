@@ -169,7 +169,9 @@ __global__ void __launch_bounds__(PT) k_cg1_start(CgG a) {
 }
 
 // Iteration i = n - 1 (see the top of the file).
-template <int L>
+// AHEAD tiles' loads in flight: 2 alone; 1 in the batched kernel, whose argument block and
+// instance walk leave no registers for a second set.
+template <int L, int AHEAD>
 __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n, int blk) {
   const Geo& g = a.g;
   const int nt = blk < a.stride ? tiles_all(g) : 0;  // blocks past the stride: zero lanes
@@ -198,13 +200,36 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
     vy = (double)(float)((double)v[2].y + na * sy) * (double)v[3].y;
   };
   __shared__ float2 s_own[4][BY][BX];  // q_i, s_{i-1}, r_i, D^-1 at the tile's cells
-  Prefetch<4> P;
-  if (blk < nt) prefetch(g, blk, fetch, P);
-  for (int r = 0; blk + a.stride * r < nt; ++r) {
-    const int tile = blk + a.stride * r;
-    stage(P, s_u, s_own, make);
+  // the iteration reads w at its cells when it updates it (every iteration, or only the odd
+  // ones when the step is deferred)
+  const bool upd_w = !a.defer || (i & 1);
+  // AHEAD tiles' loads in flight (the halo, p and w at the own cell): the tile AHEAD places on
+  // is fetched as soon as a tile's halo is staged.
+  struct Ahead {
+    Prefetch<4> halo;
+    float2 p, w;
+  };
+  auto fetch_own = [&](int tile, Ahead& A) {
+    prefetch(g, tile, fetch, A.halo);
+    const int tx = tiles_x(g);
+    const int ii = GH + (tile % tx) * BX + (threadIdx.x & 63);
+    const int jj = GH + (tile / tx) * BY + (threadIdx.x >> 6);
+    A.p = A.w = make_float2(0.f, 0.f);
+    if (ii < g.nx + GH && jj < g.ny + GH) {
+      const int k = jj * g.w + ii;
+      A.p = a.p[k];
+      if (upd_w) A.w = make_float2(a.wx[k], a.wy[k]);
+    }
+  };
+  Ahead P[AHEAD];
+#pragma unroll
+  for (int q = 0; q < AHEAD; ++q)
+    if (blk + a.stride * q < nt) fetch_own(blk + a.stride * q, P[q]);
+  auto body = [&](Ahead& Q, int tile, int r) {
+    stage(Q.halo, s_u, s_own, make);
+    const float2 pp = Q.p, ww = Q.w;
     __syncthreads();
-    if (tile + a.stride < nt) prefetch(g, tile + a.stride, fetch, P);
+    if (tile + AHEAD * a.stride < nt) fetch_own(tile + AHEAD * a.stride, Q);
     stencil_f(g, a.c, tile, s_u, s_f);
     bool own;
     int k;
@@ -214,18 +239,18 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
     if (own) {
       const int ox = threadIdx.x & 63, oy = threadIdx.x >> 6;
       const float2 qq = s_own[0][oy][ox], ss = s_own[1][oy][ox], rr = s_own[2][oy][ox];
-      const float2 d = s_own[3][oy][ox], pp = a.p[k];
+      const float2 d = s_own[3][oy][ox];
       const double sx = (double)(float)((double)qq.x + beta * (double)ss.x);
       const double sy = (double)(float)((double)qq.y + beta * (double)ss.y);
       const double px = (double)(float)((double)rr.x * (double)d.x + beta * (double)pp.x);
       const double py = (double)(float)((double)rr.y * (double)d.y + beta * (double)pp.y);
       a.p[k] = f2(px, py);
       if (!a.defer) {
-        a.wx[k] = (float)((double)a.wx[k] + alpha * px);
-        a.wy[k] = (float)((double)a.wy[k] + alpha * py);
+        a.wx[k] = (float)((double)ww.x + alpha * px);
+        a.wy[k] = (float)((double)ww.y + alpha * py);
       } else if (i & 1) {  // iterations i - 1 and i together (CgG::defer)
-        a.wx[k] = (float)((double)a.wx[k] + (alpha_prev * (double)pp.x + alpha * px));
-        a.wy[k] = (float)((double)a.wy[k] + (alpha_prev * (double)pp.y + alpha * py));
+        a.wx[k] = (float)((double)ww.x + (alpha_prev * (double)pp.x + alpha * px));
+        a.wy[k] = (float)((double)ww.y + (alpha_prev * (double)pp.y + alpha * py));
       }
       a.s[cur][k] = f2(sx, sy);
       const double nrx = (double)(float)((double)rr.x + na * sx);
@@ -238,6 +263,11 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
       v[2] = nrx * nrx + nry * nry;
     }
     add_rows<3>(v, sums, r, acc);
+  };
+  for (int r = 0; blk + a.stride * r < nt; r += AHEAD) {
+#pragma unroll
+    for (int q = 0; q < AHEAD; ++q)
+      if (blk + a.stride * (r + q) < nt) body(P[q], blk + a.stride * (r + q), r + q);
   }
   rows_total<3>(acc, sums);
   if (threadIdx.x == 0)
@@ -246,7 +276,7 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
 
 template <int L>
 __global__ void __launch_bounds__(PT) k_cg1_iter(CgG a, unsigned long long n) {
-  cg1_iter_body<L>(a, n, lane_of_block(a.g, a.stride));
+  cg1_iter_body<L, 2>(a, n, lane_of_block(a.g, a.stride));
 }
 
 // K_n's prologue alone, one wave: whether the solve is over before iteration n - 1.
@@ -438,7 +468,7 @@ template <int L>
 __global__ void __launch_bounds__(PT) kb_cg1_iter(const CgG* __restrict__ args, unsigned long long n, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
-    cg1_iter_body<L>(args[q], n, vb - m.first[q]);
+    cg1_iter_body<L, 1>(args[q], n, vb - m.first[q]);
     __syncthreads();
   }
 }

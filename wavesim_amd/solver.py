@@ -87,6 +87,7 @@ SIGNATURES = {
     "wc_linear_wave": (_I32, [_D, _D, _I32, ctypes.POINTER(CLinearWave)]),
     "wc_manning_factor": (_D, [_D, _D, _D, _D]),
     "wc_warm_start_weights": (_I32, [_I32, _I32, _P]),
+    "wc_check_shared_division": (_I32, [_D, _P, _U64, _P, _P]),
     "wc_run_steps": (_I32, [_VP, _U64, ctypes.POINTER(_U64), _P]),
     "wc_solver_set_preconditioner": (_I32, [_VP, _I32, ctypes.POINTER(ctypes.c_float), _U64]),
     "wc_solver_precond_fallbacks": (_I32, [_VP, ctypes.POINTER(_U64)]),
@@ -704,6 +705,15 @@ def linear_wave(omega: float, h: float, dispersive: bool) -> LinearWave:
 
 def manning_factor(h: float, speed: float, dt: float, n: float) -> float:
     return lib().wc_manning_factor(h, speed, dt, n)
+
+
+def check_shared_division(y: float, x: np.ndarray):
+    """x / y on the device both through the flux kernels' shared reciprocal and by the device's
+    own division: (shared, plain), which must agree bit for bit."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    shared, plain = np.empty_like(x), np.empty_like(x)
+    _check(lib().wc_check_shared_division(float(y), _ptr(x), x.size, _ptr(shared), _ptr(plain)))
+    return shared, plain
 
 
 def warm_start_weights(order: int, window: int = 0) -> np.ndarray:
