@@ -200,6 +200,7 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
     vy = (double)(float)((double)v[2].y + na * sy) * (double)v[3].y;
   };
   __shared__ float2 s_own[4][BY][BX];  // q_i, s_{i-1}, r_i, D^-1 at the tile's cells
+  __shared__ float s_c[H1];              // c on the tile + 1
   // the iteration reads w at its cells when it updates it (every iteration, or only the odd
   // ones when the step is deferred)
   const bool upd_w = !a.defer || (i & 1);
@@ -207,17 +208,22 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
   // is fetched as soon as a tile's halo is staged.
   struct Ahead {
     Prefetch<4> halo;
+    float c[2];  // c at the tile + 1 points
     float2 p, w;
+    float h;
   };
   auto fetch_own = [&](int tile, Ahead& A) {
     prefetch(g, tile, fetch, A.halo);
+    prefetch_c(g, a.c, tile, A.c);
     const int tx = tiles_x(g);
     const int ii = GH + (tile % tx) * BX + (threadIdx.x & 63);
     const int jj = GH + (tile / tx) * BY + (threadIdx.x >> 6);
     A.p = A.w = make_float2(0.f, 0.f);
+    A.h = 0.f;
     if (ii < g.nx + GH && jj < g.ny + GH) {
       const int k = jj * g.w + ii;
       A.p = a.p[k];
+      A.h = a.h[k];
       if (upd_w) A.w = make_float2(a.wx[k], a.wy[k]);
     }
   };
@@ -227,14 +233,18 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
     if (blk + a.stride * q < nt) fetch_own(blk + a.stride * q, P[q]);
   auto body = [&](Ahead& Q, int tile, int r) {
     stage(Q.halo, s_u, s_own, make);
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+      if (threadIdx.x + e * PT < H1) s_c[threadIdx.x + e * PT] = Q.c[e];
     const float2 pp = Q.p, ww = Q.w;
+    const float hh = Q.h;
     __syncthreads();
     if (tile + AHEAD * a.stride < nt) fetch_own(tile + AHEAD * a.stride, Q);
-    stencil_f(g, a.c, tile, s_u, s_f);
+    stencil_f_staged(g, s_u, s_c, s_f);
     bool own;
     int k;
     double ux, uy, ax, ay;
-    stencil_own(g, a.h, tile, s_u, s_f, own, k, ux, uy, ax, ay);
+    stencil_own_h(g, hh, tile, s_u, s_f, own, k, ux, uy, ax, ay);
     double v[3] = {0.0, 0.0, 0.0};
     if (own) {
       const int ox = threadIdx.x & 63, oy = threadIdx.x >> 6;

@@ -114,7 +114,43 @@ __device__ __forceinline__ void stencil_f(const Geo& g, const float* c, int tile
   __syncthreads();
 }
 
+// c at the thread's (at most two) tile + 1 points, zero outside the used rows and columns (as
+// stencil_f has it), fetched a tile ahead for stencil_f_staged.
+__device__ __forceinline__ void prefetch_c(const Geo& g, const float* c, int tile, float (&v)[2]) {
+  const int tx = tiles_x(g);
+  const int i0 = GH + (tile % tx) * BX, j0 = GH + (tile / tx) * BY;
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int t = threadIdx.x + e * PT;
+    v[e] = 0.f;
+    if (t < H1) {
+      const int i = i0 - 1 + t % S1X, j = j0 - 1 + t / S1X;
+      if (i <= g.nx + GH && j <= g.ny + GH) v[e] = c[j * g.w + i];
+    }
+  }
+}
+
+// stencil_f with c staged in LDS (s_c, the tile + 1 in rows of S1X) instead of read from memory
+// after the barrier. Where c is staged as zero (outside the used rows and columns) f is a zero,
+// possibly of either sign, which no cell of the tile reads.
+__device__ __forceinline__ void stencil_f_staged(const Geo& g, double (*s_u)[S2Y][S2X],
+                                                 const float* s_c, double (*s_f)[S1X]) {
+  const double ix = 0.5 / g.dx, iy = 0.5 / g.dy;
+  for (int t = threadIdx.x; t < H1; t += PT) {
+    const int lx = t % S1X, ly = t / S1X;
+    const int x = lx + 1, y = ly + 1;
+    s_f[ly][lx] = (double)s_c[t] * ((s_u[0][y][x + 1] - s_u[0][y][x - 1]) * ix +
+                                    (s_u[1][y + 1][x] - s_u[1][y - 1][x]) * iy);
+  }
+  __syncthreads();
+}
+
 // A u at the thread's cell (and u there) from the staged u and f.
+__device__ __forceinline__ void stencil_own_h(const Geo& g, float h_own, int tile,
+                                              double (*s_u)[S2Y][S2X], double (*s_f)[S1X],
+                                              bool& own, int& k, double& ux, double& uy,
+                                              double& ax, double& ay);
+
 __device__ __forceinline__ void stencil_own(const Geo& g, const float* hh, int tile,
                                             double (*s_u)[S2Y][S2X], double (*s_f)[S1X],
                                             bool& own, int& k, double& ux, double& uy,
@@ -132,6 +168,32 @@ __device__ __forceinline__ void stencil_own(const Geo& g, const float* hh, int t
   ax = ay = 0.0;
   if (own) {
     const double h = (double)hh[k];
+    const double mask = h >= WC_H_DISPERSION_MIN ? 1.0 : 0.0;
+    const double heff = wc_max(h, WC_H_DRY);
+    const int x1 = lx + 1, y1 = ly + 1;
+    ax = mask * (heff * ux - (s_f[y1][x1 + 1] - s_f[y1][x1 - 1]) * ix);
+    ay = mask * (heff * uy - (s_f[y1 + 1][x1] - s_f[y1 - 1][x1]) * iy);
+  }
+}
+
+// stencil_own with the cell's depth h_own fetched ahead (any value outside the interior).
+__device__ __forceinline__ void stencil_own_h(const Geo& g, float h_own, int tile,
+                                              double (*s_u)[S2Y][S2X], double (*s_f)[S1X],
+                                              bool& own, int& k, double& ux, double& uy,
+                                              double& ax, double& ay) {
+  const int t0 = threadIdx.x;
+  const int tx = tiles_x(g);
+  const int i0 = GH + (tile % tx) * BX, j0 = GH + (tile / tx) * BY;
+  const double ix = 0.5 / g.dx, iy = 0.5 / g.dy;
+  const int lx = t0 & 63, ly = t0 >> 6;
+  const int i = i0 + lx, j = j0 + ly;
+  own = i < g.nx + GH && j < g.ny + GH;
+  k = j * g.w + i;
+  ux = s_u[0][ly + 2][lx + 2];
+  uy = s_u[1][ly + 2][lx + 2];
+  ax = ay = 0.0;
+  if (own) {
+    const double h = (double)h_own;
     const double mask = h >= WC_H_DISPERSION_MIN ? 1.0 : 0.0;
     const double heff = wc_max(h, WC_H_DRY);
     const int x1 = lx + 1, y1 = ly + 1;
