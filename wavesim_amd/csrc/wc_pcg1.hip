@@ -79,36 +79,107 @@ __device__ __forceinline__ float2 f2(double x, double y) { return make_float2((f
 
 }  // namespace
 
+// The tile's own cells: the thread's cell and whether it lies in the interior.
+__device__ __forceinline__ int own_index(const Geo& g, int tile, bool& inside) {
+  const int tx = tiles_x(g);
+  const int i = GH + (tile % tx) * BX + (threadIdx.x & 63);
+  const int j = GH + (tile / tx) * BY + (threadIdx.x >> 6);
+  inside = i < g.nx + GH && j < g.ny + GH;
+  return j * g.w + i;
+}
+
 // r_0 = b - A w (w zero where masked, as set up), the inverse Jacobi diagonal
-// (dispersion.rs:466-480), p = s_{-1} = 0, lanes of b.b.
-template <int L>
+// (dispersion.rs:466-480), p = s_{-1} = 0, lanes of b.b. AHEAD tiles' loads in flight (as in
+// the iteration: 2 alone, 1 in the batched kernel).
+template <int L, int AHEAD>
 __device__ __forceinline__ void cg1_init_body(const CgG& a, int blk) {
   const Geo& g = a.g;
   const int nt = blk < a.stride ? tiles_all(g) : 0;  // blocks past the stride: zero lanes
   __shared__ double s_u[2][S2Y][S2X], s_f[S1Y][S1X];
   __shared__ double sums[2][BY][1];
+  __shared__ float s_c[H1];
   double acc[1] = {0.0};
-  for (int r = 0; blk + a.stride * r < nt; ++r) {
-    const int tile = blk + a.stride * r;
+  // the iterate x0 + w at the tile + 2 points (its parts, added once loaded), c on the tile + 1,
+  // and the cell's depth and right-hand side
+  struct Ahead {
+    double x0[2][2];
+    float2 w[2];
+    int flags[2];
+    float c[2];
+    float h;
+    double bx, by;
+  };
+  auto fetch = [&](int tile, Ahead& A) {
+    const int tx = tiles_x(g);
+    const int i0 = GH + (tile % tx) * BX, j0 = GH + (tile / tx) * BY;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int t = threadIdx.x + e * PT;
+      A.flags[e] = 0;
+      if (t < H2) {
+        const int i = i0 - 2 + t % S2X, j = j0 - 2 + t / S2X;
+        if (i <= g.nx + GH + 1 && j <= g.ny + GH + 1) {
+          const int m = map_y(j, g.ny, g.periodic) * g.w + map_x(i, g.nx);
+          A.x0[e][0] = a.solx[m];
+          A.x0[e][1] = a.soly[m];
+          A.w[e] = make_float2(a.wx[m], a.wy[m]);
+          A.flags[e] = 1 | (is_ghost_x(i, g.nx) ? 2 : 0) |
+                       (!g.periodic && is_ghost_y(j, g.ny) ? 4 : 0);
+        }
+      }
+    }
+    prefetch_c(g, a.c, tile, A.c);
+    bool in;
+    const int k = own_index(g, tile, in);
+    A.h = 0.f;
+    A.bx = A.by = 0.0;
+    if (in) {
+      A.h = a.h[k];
+      A.bx = a.bx[k];
+      A.by = a.by[k];
+    }
+  };
+  Ahead P[AHEAD];
+#pragma unroll
+  for (int q = 0; q < AHEAD; ++q)
+    if (blk + a.stride * q < nt) fetch(blk + a.stride * q, P[q]);
+  auto body = [&](Ahead& Q, int tile, int r) {
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int t = threadIdx.x + e * PT;
+      if (t < H2) {
+        double vx = 0.0, vy = 0.0;
+        if (Q.flags[e] & 1) {
+          vx = Q.x0[e][0] + (double)Q.w[e].x;
+          vy = Q.x0[e][1] + (double)Q.w[e].y;
+          if (Q.flags[e] & 2) vx = -vx;
+          if (Q.flags[e] & 4) vy = -vy;
+        }
+        s_u[0][t / S2X][t % S2X] = vx;
+        s_u[1][t / S2X][t % S2X] = vy;
+      }
+      if (t < H1) s_c[t] = Q.c[e];
+    }
+    const float hh = Q.h;
+    const double bx = Q.bx, by = Q.by;
+    __syncthreads();
+    if (tile + AHEAD * a.stride < nt) fetch(tile + AHEAD * a.stride, Q);
+    stencil_f_staged(g, s_u, s_c, s_f);
     bool own;
     int k;
     double ux, uy, ax, ay;
-    stencil(g, a.c, a.h, tile, s_u, s_f,
-            [&](int m, double& vx, double& vy) {  // the iterate x0 + w
-              vx = a.solx[m] + (double)a.wx[m];
-              vy = a.soly[m] + (double)a.wy[m];
-            },
-            own, k, ux, uy, ax, ay);
+    stencil_own_h(g, hh, tile, s_u, s_f, own, k, ux, uy, ax, ay);
     double v[1] = {0.0};
     if (own) {
-      const double h = (double)a.h[k];
+      const double h = (double)hh;
       double dgx = 1.0, dgy = 1.0;
       if (h >= WC_H_DISPERSION_MIN) {
+        // c of the cell's neighbours, as staged on the tile + 1
+        const int c1 = ((threadIdx.x >> 6) + 1) * S1X + (threadIdx.x & 63) + 1;
         const double heff = wc_max(h, WC_H_DRY);
-        dgx = heff + ((double)a.c[k + 1] + (double)a.c[k - 1]) * 0.25 / (g.dx * g.dx);
-        dgy = heff + ((double)a.c[k + g.w] + (double)a.c[k - g.w]) * 0.25 / (g.dy * g.dy);
+        dgx = heff + ((double)s_c[c1 + 1] + (double)s_c[c1 - 1]) * 0.25 / (g.dx * g.dx);
+        dgy = heff + ((double)s_c[c1 + S1X] + (double)s_c[c1 - S1X]) * 0.25 / (g.dy * g.dy);
       }
-      const double bx = (double)a.bx[k], by = (double)a.by[k];
       a.r[0][k] = f2(bx - ax, by - ay);
       a.di[k] = f2(1.0 / dgx, 1.0 / dgy);
       a.p[k] = make_float2(0.f, 0.f);
@@ -116,6 +187,11 @@ __device__ __forceinline__ void cg1_init_body(const CgG& a, int blk) {
       v[0] = bx * bx + by * by;
     }
     add_rows<1>(v, sums, r, acc);
+  };
+  for (int r = 0; blk + a.stride * r < nt; r += AHEAD) {
+#pragma unroll
+    for (int q = 0; q < AHEAD; ++q)
+      if (blk + a.stride * (r + q) < nt) body(P[q], blk + a.stride * (r + q), r + q);
   }
   rows_total<1>(acc, sums);
   if (threadIdx.x == 0) a.lanes_b[blk] = acc[0];
@@ -123,40 +199,72 @@ __device__ __forceinline__ void cg1_init_body(const CgG& a, int blk) {
 
 template <int L>
 __global__ void __launch_bounds__(PT) k_cg1_init(CgG a) {
-  cg1_init_body<L>(a, lane_of_block(a.g, a.stride));
+  cg1_init_body<L, 2>(a, lane_of_block(a.g, a.stride));
 }
 
 // u_0 = D^-1 r_0, q_0 = A u_0; lanes of gamma_0 = r.u, delta_0 = q.u, r.r (slot 0).
-template <int L>
+template <int L, int AHEAD>
 __device__ __forceinline__ void cg1_start_body(const CgG& a, int blk) {
   const Geo& g = a.g;
   const int nt = blk < a.stride ? tiles_all(g) : 0;  // blocks past the stride: zero lanes
   __shared__ double s_u[2][S2Y][S2X], s_f[S1Y][S1X];
   __shared__ double sums[2][BY][3];
+  __shared__ float2 s_own[2][BY][BX];  // r_0, D^-1 at the tile's cells
+  __shared__ float s_c[H1];
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int r = 0; blk + a.stride * r < nt; ++r) {
-    const int tile = blk + a.stride * r;
+  struct Ahead {
+    Prefetch<2> halo;  // r_0 and D^-1 at the tile + 2 points
+    float c[2];
+    float h;
+  };
+  auto fetch_halo = [&](int m, float2 (&v)[2]) {
+    v[0] = a.r[0][m];
+    v[1] = a.di[m];
+  };
+  auto make = [](const float2 (&v)[2], double& vx, double& vy) {
+    vx = (double)v[0].x * (double)v[1].x;
+    vy = (double)v[0].y * (double)v[1].y;
+  };
+  auto fetch = [&](int tile, Ahead& A) {
+    prefetch(g, tile, fetch_halo, A.halo);
+    prefetch_c(g, a.c, tile, A.c);
+    bool in;
+    const int k = own_index(g, tile, in);
+    A.h = in ? a.h[k] : 0.f;
+  };
+  Ahead P[AHEAD];
+#pragma unroll
+  for (int q = 0; q < AHEAD; ++q)
+    if (blk + a.stride * q < nt) fetch(blk + a.stride * q, P[q]);
+  auto body = [&](Ahead& Q, int tile, int r) {
+    stage(Q.halo, s_u, s_own, make);
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+      if (threadIdx.x + e * PT < H1) s_c[threadIdx.x + e * PT] = Q.c[e];
+    const float hh = Q.h;
+    __syncthreads();
+    if (tile + AHEAD * a.stride < nt) fetch(tile + AHEAD * a.stride, Q);
+    stencil_f_staged(g, s_u, s_c, s_f);
     bool own;
     int k;
     double ux, uy, ax, ay;
-    stencil(g, a.c, a.h, tile, s_u, s_f,
-            [&](int m, double& vx, double& vy) {
-              const float2 rr = a.r[0][m], d = a.di[m];
-              vx = (double)rr.x * (double)d.x;
-              vy = (double)rr.y * (double)d.y;
-            },
-            own, k, ux, uy, ax, ay);
+    stencil_own_h(g, hh, tile, s_u, s_f, own, k, ux, uy, ax, ay);
     double v[3] = {0.0, 0.0, 0.0};
     if (own) {
       const double qx = (double)(float)ax, qy = (double)(float)ay;
       a.q[0][k] = f2(qx, qy);
-      const float2 rr = a.r[0][k];
+      const float2 rr = s_own[0][threadIdx.x >> 6][threadIdx.x & 63];
       const double rx = rr.x, ry = rr.y;
       v[0] = rx * ux + ry * uy;
       v[1] = qx * ux + qy * uy;
       v[2] = rx * rx + ry * ry;
     }
     add_rows<3>(v, sums, r, acc);
+  };
+  for (int r = 0; blk + a.stride * r < nt; r += AHEAD) {
+#pragma unroll
+    for (int q = 0; q < AHEAD; ++q)
+      if (blk + a.stride * (r + q) < nt) body(P[q], blk + a.stride * (r + q), r + q);
   }
   rows_total<3>(acc, sums);
   if (threadIdx.x == 0)
@@ -165,7 +273,7 @@ __device__ __forceinline__ void cg1_start_body(const CgG& a, int blk) {
 
 template <int L>
 __global__ void __launch_bounds__(PT) k_cg1_start(CgG a) {
-  cg1_start_body<L>(a, lane_of_block(a.g, a.stride));
+  cg1_start_body<L, 2>(a, lane_of_block(a.g, a.stride));
 }
 
 // Iteration i = n - 1 (see the top of the file).
@@ -460,7 +568,7 @@ template <int L>
 __global__ void __launch_bounds__(PT) kb_cg1_init(const CgG* __restrict__ args, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
-    cg1_init_body<L>(args[q], vb - m.first[q]);
+    cg1_init_body<L, 1>(args[q], vb - m.first[q]);
     __syncthreads();
   }
 }
@@ -469,7 +577,7 @@ template <int L>
 __global__ void __launch_bounds__(PT) kb_cg1_start(const CgG* __restrict__ args, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
-    cg1_start_body<L>(args[q], vb - m.first[q]);
+    cg1_start_body<L, 1>(args[q], vb - m.first[q]);
     __syncthreads();
   }
 }
