@@ -162,20 +162,51 @@ __global__ void __launch_bounds__(NT) k_sgn_setup(SgnArgs a) {
   __shared__ double s_div[SY2][SX2];
   __shared__ double s_c[SY1][SX1], s_f[SY1][SX1];
   const double ix = 0.5 / g.dx, iy = 0.5 / g.dy;
-  for (int t = threadIdx.x; t < SX3 * SY3; t += NT) {
-    const int lx = t % SX3, ly = t / SX3;
-    const int i = i0 - 3 + lx, j = j0 - 3 + ly;
-    double h = 0.0, um = 0.0, vm = 0.0;
-    if (i < g.w && j < g.rows) {
+  // Every load of the kernel is issued before the first is used: the state on the tile + 3
+  // (at most three points a thread), phi on the tile + 1 (at most two), the cell's rates.
+  constexpr int N3 = SX3 * SY3, R3 = (N3 + NT - 1) / NT;
+  constexpr int N1 = SX1 * SY1, R1 = (N1 + NT - 1) / NT;
+  double st[R3][3];
+#pragma unroll
+  for (int q = 0; q < R3; ++q) {
+    const int t = threadIdx.x + q * NT;
+    const int i = i0 - 3 + t % SX3, j = j0 - 3 + t / SX3;
+    st[q][0] = st[q][1] = st[q][2] = 0.0;
+    if (t < N3 && i < g.w && j < g.rows) {
       const int k = j * g.w + i;
-      h = a.h[k];
-      const double m = h >= WC_H_DISPERSION_MIN ? 1.0 : 0.0;
-      um = m * wc_velocity(h, a.hu[k]);
-      vm = m * wc_velocity(h, a.hv[k]);
+      st[q][0] = a.h[k];
+      st[q][1] = a.hu[k];
+      st[q][2] = a.hv[k];
     }
-    s_h[ly][lx] = h;
-    s_um[ly][lx] = um;
-    s_vm[ly][lx] = vm;
+  }
+  double ph[R1];
+#pragma unroll
+  for (int q = 0; q < R1; ++q) {
+    const int t = threadIdx.x + q * NT;
+    const int i = i0 - 1 + t % SX1, j = j0 - 1 + t / SX1;
+    ph[q] = 0.0;
+    if (t < N1 && i <= g.nx + GH && j <= g.ny + GH) ph[q] = a.phi[j * g.w + i];
+  }
+  const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+  const int i = i0 + lx, j = j0 + ly;
+  const bool own = i < g.nx + GH && j < g.ny + GH;
+  const int k = j * g.w + i;
+  double r_h = 0.0, r_hu = 0.0, r_hv = 0.0;
+  if (own) {
+    r_h = a.lh[k];
+    r_hu = a.lhu[k];
+    r_hv = a.lhv[k];
+  }
+#pragma unroll
+  for (int q = 0; q < R3; ++q) {
+    const int t = threadIdx.x + q * NT;
+    if (t < N3) {
+      const double h = st[q][0];  // zero outside the array: then um = vm = 0
+      const double m = h >= WC_H_DISPERSION_MIN ? 1.0 : 0.0;
+      s_h[t / SX3][t % SX3] = h;
+      s_um[t / SX3][t % SX3] = m * wc_velocity(h, st[q][1]);
+      s_vm[t / SX3][t % SX3] = m * wc_velocity(h, st[q][2]);
+    }
   }
   __syncthreads();
   for (int t = threadIdx.x; t < SX2 * SY2; t += NT) {
@@ -184,7 +215,10 @@ __global__ void __launch_bounds__(NT) k_sgn_setup(SgnArgs a) {
     s_div[ly][lx] = (s_um[y][x + 1] - s_um[y][x - 1]) * ix + (s_vm[y + 1][x] - s_vm[y - 1][x]) * iy;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < SX1 * SY1; t += NT) {
+#pragma unroll
+  for (int q = 0; q < R1; ++q) {
+    const int t = threadIdx.x + q * NT;
+    if (t >= N1) break;
     const int lx = t % SX1, ly = t / SX1;
     const int i = i0 - 1 + lx, j = j0 - 1 + ly;
     const int x3 = lx + 2, y3 = ly + 2;  // in the +3 frame
@@ -192,8 +226,7 @@ __global__ void __launch_bounds__(NT) k_sgn_setup(SgnArgs a) {
     double coeff = 0.0, f = 0.0;
     if (i <= g.nx + GH && j <= g.ny + GH) {
       const double depth = s_h[y3][x3];
-      coeff = depth < WC_H_DISPERSION_MIN ? 0.0
-                                          : a.phi[j * g.w + i] * depth * depth * depth / 3.0;
+      coeff = depth < WC_H_DISPERSION_MIN ? 0.0 : ph[q] * depth * depth * depth / 3.0;
       if (a.nonlinear && coeff > 0.0) {
         const double n = s_um[y3][x3] * (s_div[y2][x2 + 1] - s_div[y2][x2 - 1]) * ix +
                          s_vm[y3][x3] * (s_div[y2 + 1][x2] - s_div[y2 - 1][x2]) * iy -
@@ -210,10 +243,7 @@ __global__ void __launch_bounds__(NT) k_sgn_setup(SgnArgs a) {
     s_f[ly][lx] = f;
   }
   __syncthreads();
-  const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
-  const int i = i0 + lx, j = j0 + ly;
-  if (i >= g.nx + GH || j >= g.ny + GH) return;
-  const int k = j * g.w + i;
+  if (!own) return;
   const int x1 = lx + 1, y1 = ly + 1;
   const double h = s_h[ly + 3][lx + 3];
   const double heff = wc_max(h, WC_H_DRY);
@@ -252,10 +282,11 @@ __global__ void __launch_bounds__(NT) k_sgn_setup(SgnArgs a) {
   } else {
     dgx = heff + (s_c[y1][x1 + 1] + s_c[y1][x1 - 1]) * 0.25 / (g.dx * g.dx);
     dgy = heff + (s_c[y1 + 1][x1] + s_c[y1 - 1][x1]) * 0.25 / (g.dy * g.dy);
-    const double u = wc_velocity(h, a.hu[k]);
-    const double v = wc_velocity(h, a.hv[k]);
-    bx = a.lhu[k] - u * a.lh[k] + (s_f[y1][x1 + 1] - s_f[y1][x1 - 1]) * ix;
-    by = a.lhv[k] - v * a.lh[k] + (s_f[y1 + 1][x1] - s_f[y1 - 1][x1]) * iy;
+    // the cell's velocities as staged (the mask is 1 here, and 1 u is u)
+    const double u = s_um[ly + 3][lx + 3];
+    const double v = s_vm[ly + 3][lx + 3];
+    bx = r_hu - u * r_h + (s_f[y1][x1 + 1] - s_f[y1][x1 - 1]) * ix;
+    by = r_hv - v * r_h + (s_f[y1 + 1][x1] - s_f[y1 - 1][x1]) * iy;
   }
   if (a.dg2) {
     a.dg2[k] = make_double2(dgx, dgy);
