@@ -6,84 +6,28 @@
 // the sponges' exp(-rate dt)), with the same libm calls the reference makes, and ships those few
 // numbers each step.
 
-#include <hip/hip_runtime.h>
-#include <math.h>
-#include <stdio.h>
-#include <string.h>
-
-#include <algorithm>
-#include <limits>
-#include <memory>
-#include <string>
-#include <vector>
-
-#include "../../include/wavecore_hip.h"
-#include "../../include/wavecore_unet.h"
-#include "wc_kernels.h"
-#include "wc_run.h"
+#include "wc_host.h"
 
 namespace wc {
-int unet_body(wcu_net* n, hipStream_t st, const int* skip);
-void unet_views(wcu_net* n, void** x0, const void** y0, int* w0, int* h0, const float** wo);
-}  // namespace wc
-
-using namespace wc;
+namespace host {
 
 namespace {
-
-constexpr double TAU = 6.283185307179586476925286766559;
-constexpr double PI = 3.14159265358979323846264338327950;
-constexpr int PRM_SLOTS = 3;
-
 thread_local std::string g_err;
+}  // namespace
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
 
-#define HIP_OK(expr)                                                                      \
-  do {                                                                                    \
-    hipError_t e_ = (expr);                                                               \
-    if (e_ != hipSuccess)                                                                 \
-      return fail(e_ == hipErrorOutOfMemory ? WC_ENOMEM : WC_EDEVICE,                     \
-                  std::string(#expr) + " failed: " + hipGetErrorString(e_));              \
-  } while (0)
+const char* last_error() { return g_err.c_str(); }
 
-enum Family {
-  F_FILL,
-  F_SWITCH,
-  F_FLUX,
-  F_EULER,
-  F_SGN_SETUP,
-  F_CG_INIT,
-  F_CG_MATVEC,
-  F_CG_UPDATE,
-  F_CG_PUPDATE,
-  F_CG_STATS,
-  F_SGN_FINISH,
-  F_REDUCE,
-  F_CG_REDUCE,
-  F_RELAX,
-  F_CG_ITER,
-  F_NET,
-  F_FRAME,
-  F_CG_FLUSH,
-  F_CG_PAIR,
-  F_COUNT
-};
 const char* kFamilyNames[F_COUNT] = {"fill_state", "switch",    "flux",      "finish_euler",
                                      "sgn_setup",  "cg_init",   "cg_matvec", "cg_update",
                                      "cg_pupdate", "cg_stats",  "sgn_finish", "reduce",
                                      "cg_reduce",  "relax",     "cg_iter",   "net",
                                      "frame",      "cg_flush",  "cg_pair"};
 
-struct HostSponge {
-  uint64_t width;
-  double strength, level;
-  bool west, east, south, north;
-  long long offset, len;
-};
 
 // linear_wave (forcing.rs:53-75).
 bool linear_wave(double omega, double h, bool dispersive, wc_linear_wave_t* out) {
@@ -105,194 +49,16 @@ bool linear_wave(double omega, double h, bool dispersive, wc_linear_wave_t* out)
   return true;
 }
 
-}  // namespace
+}  // namespace host
+}  // namespace wc
 
-struct wc_solver {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  Geo geo{};
-  size_t cells = 0;
-  std::vector<double> bed;
+namespace wc {
+namespace host {
 
-  double cfl = 0.4;
-  int order = WC_ORDER_SECOND;
-  double manning = 0.0;
-  bool periodic = false;
-  int precision = WC_PRECISION_FP64;
 
-  // wave-maker and its Drive (forcing.rs:77-139)
-  bool has_maker = false;
-  wc_wavemaker_t maker{};
-  bool has_drive = false;
-  double omega = 0.0, ky = 0.0, strength = 0.0;
-  int src_i0 = 0, src_i1 = 0;
 
-  std::vector<HostSponge> sponges;
-  std::vector<double> sponge_rates;  // concatenated tables, static
 
-  bool dispersive = false;
-  wc_dispersion_t disp{};
-  // breaking fronts and their eddy viscosity (Breaking::viscosity, eddy.rs)
-  bool eddy_on = false;
-  EddyP eddy{};
-  double *e_rise = nullptr, *e_mem = nullptr, *e_nu = nullptr, *e_hu = nullptr, *e_hv = nullptr;
-  bool has_held = false;
-  double time = 0.0;
-  unsigned long long prev_it[2] = {0, 0};
 
-  // device arrays (padded, FP64)
-  double *h = nullptr, *hu = nullptr, *hv = nullptr, *br = nullptr;
-  double *h1 = nullptr, *hu1 = nullptr, *hv1 = nullptr;
-  double *b = nullptr, *phi = nullptr, *lh = nullptr, *lhu = nullptr, *lhv = nullptr;
-  double *c = nullptr, *dgx = nullptr, *dgy = nullptr, *bx = nullptr, *by = nullptr;
-  double *wx = nullptr, *wy = nullptr, *rx = nullptr, *ry = nullptr, *px = nullptr,
-         *py = nullptr, *ax = nullptr, *ay = nullptr;
-  double *zx = nullptr, *zy = nullptr, *qx = nullptr, *qy = nullptr;  // fused PCG: z, 2nd p
-  // the default FP64 PCG (wc_pcg2.hip): the diagonal, r and the two p buffers as interleaved
-  // (x, y) pairs, one 16-byte load per halo point; allocated at its first use
-  double2 *v_dg = nullptr, *v_r = nullptr, *v_pa = nullptr, *v_pb = nullptr;
-
-  // the FP32 chosen form
-  float *f_sh = nullptr, *f_shu = nullptr, *f_shv = nullptr, *f_slh = nullptr, *f_slhu = nullptr,
-        *f_slhv = nullptr;
-  float *f_s1h = nullptr, *f_s1hl = nullptr, *f_s1hu = nullptr, *f_s1hv = nullptr,
-        *f_s1hul = nullptr, *f_s1hvl = nullptr;
-  double *f_l1h = nullptr, *f_lh = nullptr;  // mass rates stay FP64
-  float *f_l1hu = nullptr, *f_l1hv = nullptr;
-  double *f_lhu = nullptr, *f_lhv = nullptr, *f_bx = nullptr, *f_by = nullptr;  // FP64
-  float *f_phi = nullptr;
-  float *f_c = nullptr, *f_wx = nullptr, *f_wy = nullptr,
-        *f_rx = nullptr, *f_ry = nullptr, *f_px = nullptr, *f_py = nullptr, *f_qx = nullptr,
-        *f_qy = nullptr, *f_zx = nullptr, *f_zy = nullptr, *f_ax = nullptr, *f_ay = nullptr;
-  // the one-kernel PCG's own vectors (wc_pcg1.hip), interleaved (x, y)
-  float2 *g_r[2] = {nullptr, nullptr}, *g_q[2] = {nullptr, nullptr},
-         *g_s[2] = {nullptr, nullptr}, *g_p = nullptr, *g_di = nullptr;
-  // the extrapolated warm start of the FP32 form's solves (WarmStart in wc_kernels.h)
-  float2* ws_ring = nullptr;
-  double2* ws_ring64 = nullptr;  // the FP64 path's, allocated when "fp64_warm_start" is set
-  int ws_order64 = 0;            // the FP64 path's warm start: 0, the reference's
-  int ws_order = 3;   // exact for steps of degree < 3; 0: the reference's warm start
-  int ws_window = 6;  // past steps weighed (at least ws_order)
-  int ws_guard = 0;   // per mille: cells whose switch reads below start from x_{n-1}
-
-  // the learned preconditioner (wc_fpcg.hip) and the capture of solves for its training
-  int precond = WC_PRECOND_JACOBI;
-  wcu_net* net = nullptr;
-  float *bgx = nullptr, *bgy = nullptr;
-  int capture = 0;
-  static constexpr int CAP_SLOTS = 4;  // the solve's state before iterations 0, 2, 5, 10
-  float *cap_wx[CAP_SLOTS] = {}, *cap_wy[CAP_SLOTS] = {}, *cap_h = nullptr;
-  float2* cap_r[CAP_SLOTS] = {};
-  unsigned long long cap_last_it = 0;
-  int precond_stall = 2;
-  int precond_ratio = 70;  // percent
-  double* history = nullptr;  // r.r per iteration of the last network-preconditioned solve
-
-  // implementation options (wc_solver_set_option)
-  int pcg_kernels = 0;  // 0: the default of each precision (FP64 3, FP32 4)
-  double* lanes = nullptr;  // 8 x 512 lane sums of the lane PCGs
-  int pcg_lanes = 256;      // blocks of the one-kernel PCG's launches: 256 or 512
-  int pcg_march = 0;        // one-kernel PCG: blocks march down column strips (1) or take
-                            // 64x16 tiles (0)
-  int recon64 = 1;          // FP32 form: reconstruct the FP32 state in FP64 (1) or FP32 (0)
-  int pcg_walk = 1;         // one-kernel PCG on 64x16 tiles: tiles per block to aim for
-  int flux_kernel = 3;
-  int march_rows = 0;  // band height of the marching flux launches (0: not yet chosen)
-  int flux_rows = 0;   // option: a fixed band height, or 0 for the tuner's choice
-  std::vector<std::pair<int, double>> march_tried;  // (band height, us per launch) as timed
-  unsigned long long pcg_batch = 4;
-  uint8_t* held = nullptr;
-  double* part = nullptr;
-  CgScalars* sc = nullptr;
-  double* src_col = nullptr;
-  double* sponge_rate_d = nullptr;
-  double* prm = nullptr;
-  size_t prm_len = 0;
-  unsigned long long* red64 = nullptr;  // [0] max rate bits, [1] max momentum, [2] min-h key
-  unsigned int* red32 = nullptr;        // [1] first NaN, [2] first max
-  double* redf = nullptr;               // [0] volume sum
-  std::vector<void*> owned;
-  // statistics and relaxation zones built on this solver: freeing the solver releases their
-  // device memory and leaves them detached, whichever order a caller frees things in
-  std::vector<wc_stats*> stats;
-  std::vector<wc_relaxation*> relaxations;
-
-  // pinned host staging
-  double* prm_host[PRM_SLOTS] = {nullptr, nullptr, nullptr};
-  hipEvent_t prm_ev[PRM_SLOTS] = {nullptr, nullptr, nullptr};
-  bool prm_used[PRM_SLOTS] = {false, false, false};
-  int prm_slot = 0;
-  size_t prm_cap = 0;
-  CgScalars* sc_host = nullptr;
-  unsigned long long* red64_host = nullptr;
-  unsigned int* red32_host = nullptr;
-  double* redf_host = nullptr;
-
-  // profiling: prof times every launch with its own pair of events; prof_batch times only the
-  // PCG iterations (the FP32 form's one kernel, the FP64 path's pair), one pair of events around
-  // each batch of back-to-back launches
-  bool prof = false, prof_batch = false;
-  double prof_ms[F_COUNT] = {};
-  unsigned long long prof_n[F_COUNT] = {};
-  struct PendingTime {
-    int fam;
-    hipEvent_t e0, e1;
-    unsigned long long launches;
-  };
-  std::vector<hipEvent_t> ev_pool;
-  std::vector<PendingTime> ev_pending;
-};
-
-namespace {
-
-int set_device(wc_solver* s) {
-  HIP_OK(hipSetDevice(s->device));
-  return WC_OK;
-}
-
-template <class T>
-int dalloc(wc_solver* s, T** p, size_t n) {
-  void* v = nullptr;
-  HIP_OK(hipMalloc(&v, std::max<size_t>(n, 1) * sizeof(T)));
-  HIP_OK(hipMemsetAsync(v, 0, std::max<size_t>(n, 1) * sizeof(T), s->stream));
-  s->owned.push_back(v);
-  *p = static_cast<T*>(v);
-  return WC_OK;
-}
-
-hipEvent_t take_event(wc_solver* s) {
-  if (!s->ev_pool.empty()) {
-    hipEvent_t e = s->ev_pool.back();
-    s->ev_pool.pop_back();
-    return e;
-  }
-  hipEvent_t e = nullptr;
-  if (hipEventCreate(&e) != hipSuccess) return nullptr;
-  return e;
-}
-
-template <typename... KArgs, typename... Args>
-int launch(wc_solver* s, int fam, void (*k)(KArgs...), dim3 grid, dim3 block, Args... args) {
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (s->prof) {
-    e0 = take_event(s);
-    e1 = take_event(s);
-    if (e0) HIP_OK(hipEventRecord(e0, s->stream));
-  }
-  hipLaunchKernelGGL(k, grid, block, 0, s->stream, args...);
-  HIP_OK(hipGetLastError());
-  if (s->prof && e0 && e1) {
-    HIP_OK(hipEventRecord(e1, s->stream));
-    s->ev_pending.push_back({fam, e0, e1, 1});
-  }
-  return WC_OK;
-}
-
-#define TRY(expr)               \
-  do {                          \
-    int rc_ = (expr);           \
-    if (rc_ != WC_OK) return rc_; \
-  } while (0)
 
 int prof_collect(wc_solver* s) {
   if (s->ev_pending.empty()) return WC_OK;
@@ -322,7 +88,6 @@ dim3 tiles(const wc_solver* s) { return dim3((unsigned)(s->geo.ntx * s->geo.nty)
 // otherwise, and under stream capture, the fitting height serves untimed, and a launch that
 // may not be repeated leaves the choice to the next one that may. Option flux_rows fixes the
 // height instead.
-constexpr int MARCH_MW = TX - 6;
 template <typename... KArgs, class Run>
 int march_launch(wc_solver* s, void (*kernel)(KArgs...), bool repeatable, dim3* grid, int* rows,
                  Run run) {
@@ -1542,7 +1307,8 @@ int stable_dt32(wc_solver* s, double* dt) {
   return WC_OK;
 }
 
-}  // namespace
+}  // namespace host
+}  // namespace wc
 
 // ================================================================== the C ABI
 namespace wc {
@@ -1550,14 +1316,9 @@ namespace wc {
 int api_fail(int code, const char* msg) { return fail(code, msg); }
 }  // namespace wc
 
-namespace wc {
-void detach(wc_stats* st);       // releases the device memory; the object stays, inert
-void detach(wc_relaxation* r);
-}  // namespace wc
-
 extern "C" {
 
-const char* wc_last_error(void) { return g_err.c_str(); }
+const char* wc_last_error(void) { return last_error(); }
 
 int wc_solver_new(uint64_t nx, uint64_t ny, double dx, double dy, const double* bed,
                   wc_solver** out) {
@@ -2078,712 +1839,6 @@ int wc_solver_residual_history(wc_solver* s, double* out, uint64_t max, uint64_t
   return WC_OK;
 }
 
-// ---- the run's statistics and the driver step (crates/wavesim/src/lib.rs:849-937,1099-1434) --
-
-struct wc_stats {
-  wc_solver* s = nullptr;
-  double from = 0.0, period = 0.0, next_wave = 0.0, seconds = 0.0, omega = 0.0;
-  double moments[6] = {0, 0, 0, 0, 0, 0};
-  uint64_t waves = 0;
-  double *breaking_seconds = nullptr, *eta_max = nullptr, *eta_min = nullptr,
-         *wave_max = nullptr, *wave_min = nullptr, *harm = nullptr, *wave_sum = nullptr,
-         *wave_first = nullptr, *wave_last = nullptr;
-  float* elevation = nullptr;
-  std::vector<void*> owned;
-};
-
-namespace {
-
-__global__ void k_fill_value(double* a, long long n, double v) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) a[t] = v;
-}
-
-}  // namespace
-
-int wc_stats_new(wc_solver* s, double from, double period, const float* elevation,
-                 wc_stats** out) {
-  if (!s || !out) return fail(WC_EINVAL, "null argument");
-  if (!(period > 0.0)) return fail(WC_EINVAL, "the wave period must be positive");
-  *out = nullptr;
-  TRY(set_device(s));
-  auto st = std::make_unique<wc_stats>();
-  st->s = s;
-  s->stats.push_back(st.get());
-  struct Undo {  // a failed construction leaves nothing behind
-    std::unique_ptr<wc_stats>& st;
-    ~Undo() {
-      if (st) wc::detach(st.get());
-    }
-  } undo{st};
-  st->from = from;
-  st->period = period;
-  st->next_wave = from + period;
-  st->omega = 2.0 * M_PI / period;
-  const long long n = (long long)s->geo.nx * s->geo.ny;
-  auto alloc = [&](double** p, long long count, double v) -> int {
-    HIP_OK(hipMalloc(p, std::max<long long>(count, 1) * sizeof(double)));
-    st->owned.push_back(*p);
-    hipLaunchKernelGGL(k_fill_value, over(count), dim3(NT), 0, s->stream, *p, count, v);
-    HIP_OK(hipGetLastError());
-    return WC_OK;
-  };
-  TRY(alloc(&st->breaking_seconds, n, 0.0));
-  TRY(alloc(&st->eta_max, n, -INFINITY));
-  TRY(alloc(&st->eta_min, n, INFINITY));
-  TRY(alloc(&st->wave_max, n, -INFINITY));
-  TRY(alloc(&st->wave_min, n, INFINITY));
-  TRY(alloc(&st->harm, 3 * n, 0.0));
-  TRY(alloc(&st->wave_sum, n, 0.0));
-  TRY(alloc(&st->wave_first, n, 0.0));
-  TRY(alloc(&st->wave_last, n, 0.0));
-  if (elevation) {
-    HIP_OK(hipMalloc(&st->elevation, n * sizeof(float)));
-    st->owned.push_back(st->elevation);
-    HIP_OK(hipMemcpyAsync(st->elevation, elevation, n * sizeof(float), hipMemcpyHostToDevice,
-                          s->stream));
-  }
-  HIP_OK(hipStreamSynchronize(s->stream));
-  *out = st.release();
-  return WC_OK;
-}
-
-void wc_stats_free(wc_stats* st) {
-  if (!st) return;
-  wc::detach(st);
-  delete st;
-}
-
-// Stage::record (lib.rs:1276-1300) after a step of dt that has just ended.
-int wc_stats_record(wc_stats* st, double dt) {
-  if (!st) return fail(WC_EINVAL, "null statistics");
-  if (!st->s) return fail(WC_EINVAL, "the statistics' solver has been freed");
-  wc_solver* s = st->s;
-  const double inside = std::min(dt, s->time - st->from);
-  if (inside <= 0.0) return WC_OK;
-  TRY(set_device(s));
-  TRY(join_state(s));
-  RecordArgs a{};
-  a.g = s->geo;
-  a.h = s->h;
-  a.b = s->b;
-  a.br = breaking_of(s);
-  a.elevation = st->elevation;
-  if (s->time > st->next_wave + 1e-9) {  // Stats::close_wave (lib.rs:918-932)
-    a.close_wave = 1;
-    st->waves += 1;
-    a.first_wave = st->waves == 1 ? 1 : 0;
-    st->next_wave += st->period;
-  }
-  // FirstHarmonic::weights (lib.rs:1118-1128)
-  // (omega t).sin_cos(): glibc's sincos, as the wave-maker's angle
-  double sn, cs;
-  sincos(st->omega * s->time, &sn, &cs);
-  const double v[6] = {1.0, cs, sn, cs * cs, cs * sn, sn * sn};
-  for (int m = 0; m < 6; ++m) st->moments[m] += inside * v[m];
-  a.inside = inside;
-  a.w0 = inside;
-  a.w1 = inside * cs;
-  a.w2 = inside * sn;
-  a.breaking_seconds = st->breaking_seconds;
-  a.eta_max = st->eta_max;
-  a.eta_min = st->eta_min;
-  a.wave_max = st->wave_max;
-  a.wave_min = st->wave_min;
-  a.harm = st->harm;
-  a.wave_sum = st->wave_sum;
-  a.wave_first = st->wave_first;
-  a.wave_last = st->wave_last;
-  TRY(launch(s, F_REDUCE, k_record, over((long long)s->geo.nx * s->geo.ny), dim3(NT), a));
-  st->seconds += inside;
-  if (!s->prof) HIP_OK(hipStreamSynchronize(s->stream));
-  return WC_OK;
-}
-
-int wc_stats_read(wc_stats* st, double* seconds, uint64_t* waves, double* breaking_seconds,
-                  double* eta_max, double* eta_min, double* wave_mean, double* wave_first,
-                  double* wave_last) {
-  if (!st) return fail(WC_EINVAL, "null statistics");
-  if (!st->s) return fail(WC_EINVAL, "the statistics' solver has been freed");
-  wc_solver* s = st->s;
-  TRY(set_device(s));
-  const size_t n = (size_t)s->geo.nx * s->geo.ny;
-  HIP_OK(hipStreamSynchronize(s->stream));
-  auto get = [&](double* dst, const double* src) -> int {
-    if (dst) HIP_OK(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyDeviceToHost));
-    return WC_OK;
-  };
-  if (seconds) *seconds = st->seconds;
-  if (waves) *waves = st->waves;
-  TRY(get(breaking_seconds, st->breaking_seconds));
-  TRY(get(eta_max, st->eta_max));
-  TRY(get(eta_min, st->eta_min));
-  TRY(get(wave_first, st->wave_first));
-  TRY(get(wave_last, st->wave_last));
-  if (wave_mean) {
-    TRY(get(wave_mean, st->wave_sum));
-    for (size_t k = 0; k < n; ++k)
-      wave_mean[k] = st->waves ? wave_mean[k] / (double)st->waves : 0.0;
-  }
-  return WC_OK;
-}
-
-// FirstHarmonic::amplitudes (lib.rs:1136-1160): (a, -b) per cell, *ok = 0 where the reference
-// returns None.
-int wc_stats_harmonic(wc_stats* st, int32_t* ok, double* a_out, double* b_out) {
-  if (!st || !ok) return fail(WC_EINVAL, "null argument");
-  if (!st->s) return fail(WC_EINVAL, "the statistics' solver has been freed");
-  wc_solver* s = st->s;
-  TRY(set_device(s));
-  const size_t n = (size_t)s->geo.nx * s->geo.ny;
-  const double s0 = st->moments[0], c = st->moments[1], sn = st->moments[2];
-  const double cc = st->moments[3], cs = st->moments[4], ss = st->moments[5];
-  const double m[3][3] = {{s0, c, sn}, {c, cc, cs}, {sn, cs, ss}};
-  auto cof = [&](int r, int k) {
-    const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, k1 = (k + 1) % 3, k2 = (k + 2) % 3;
-    return m[r1][k1] * m[r2][k2] - m[r1][k2] * m[r2][k1];
-  };
-  double det = 0.0;
-  for (int k = 0; k < 3; ++k) det += m[0][k] * cof(0, k);
-  if (std::isnan(det) || det <= 1e-3 * s0 * s0 * s0 / 4.0) {
-    *ok = 0;
-    return WC_OK;
-  }
-  *ok = 1;
-  std::vector<double> e(3 * n);
-  HIP_OK(hipStreamSynchronize(s->stream));
-  HIP_OK(hipMemcpy(e.data(), st->harm, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
-  auto inv = [&](int r, int k) { return cof(k, r) / det; };
-  for (size_t q = 0; q < n; ++q) {
-    auto row = [&](int r) {
-      double t = 0.0;
-      for (int k = 0; k < 3; ++k) t += inv(r, k) * e[3 * q + k];
-      return t;
-    };
-    if (a_out) a_out[q] = row(1);
-    if (b_out) b_out[q] = -row(2);
-  }
-  return WC_OK;
-}
-
-// The end of a run (lib.rs:1451-1453): the wave in progress is closed if it is due.
-int wc_stats_finish(wc_stats* st) {
-  if (!st) return fail(WC_EINVAL, "null statistics");
-  if (!st->s) return fail(WC_EINVAL, "the statistics' solver has been freed");
-  wc_solver* s = st->s;
-  if (!(s->time >= st->next_wave - 1e-9)) return WC_OK;
-  TRY(set_device(s));
-  RecordArgs a{};
-  a.g = s->geo;
-  st->waves += 1;
-  a.first_wave = st->waves == 1 ? 1 : 0;
-  st->next_wave += st->period;
-  a.wave_max = st->wave_max;
-  a.wave_min = st->wave_min;
-  a.wave_sum = st->wave_sum;
-  a.wave_first = st->wave_first;
-  a.wave_last = st->wave_last;
-  TRY(launch(s, F_REDUCE, k_close_wave, over((long long)s->geo.nx * s->geo.ny), dim3(NT), a));
-  if (!s->prof) HIP_OK(hipStreamSynchronize(s->stream));
-  return WC_OK;
-}
-
-// Stage::frame_if_due's frame (lib.rs:1348-1380) from the solver's state, through a pinned slot
-// of the run writer; the disk write is the writer thread's.
-int wc_run_frame_from(wc_run_writer* w, wc_solver* s, double tide, double* rise,
-                      int32_t* status) {
-  if (!w || !s || !rise || !status) return fail(WC_EINVAL, "null argument");
-  TRY(set_device(s));
-  TRY(join_state(s));
-  wc::run::DeviceFrame f;
-  TRY(wc::run::begin_device_frame(w, s->device, s->geo.nx, s->geo.ny, &f));
-  const size_t n = f.cells;
-  const int rc = [&]() -> int {
-    HIP_OK(hipMemsetAsync(f.scalars, 0, 2 * sizeof(unsigned long long), s->stream));
-    FrameArgs a{};
-    a.g = s->geo;
-    a.h = s->h;
-    a.b = s->b;
-    a.br = breaking_of(s);
-    a.elevation = f.elevation;
-    a.tide = tide;
-    a.eta = f.device + (size_t)f.eta * n;
-    a.breaking = f.device + (size_t)f.breaking * n;
-    a.rise_bits = f.scalars;
-    a.nonfinite = (unsigned int*)(f.scalars + 1);
-    TRY(launch(s, F_FRAME, k_frame, over((long long)n), dim3(NT), a));
-    HIP_OK(hipMemcpyAsync(f.pinned, f.device, 2 * n * sizeof(float), hipMemcpyDeviceToHost,
-                          s->stream));
-    HIP_OK(hipMemcpyAsync(f.scalars_host, f.scalars, 2 * sizeof(unsigned long long),
-                          hipMemcpyDeviceToHost, s->stream));
-    HIP_OK(hipStreamSynchronize(s->stream));
-    return WC_OK;
-  }();
-  if (rc) {
-    wc::run::abandon_frame(w, f);
-    return rc;
-  }
-  double r;
-  memcpy(&r, &f.scalars_host[0], sizeof r);
-  const bool nonfinite = (unsigned int)f.scalars_host[1] != 0;
-  *rise = r;
-  if (!std::isfinite(r) || r > 100.0 || nonfinite) {
-    wc::run::abandon_frame(w, f);
-    *status = WC_STAGE_UNSTABLE;
-    return WC_OK;
-  }
-  *status = WC_STAGE_OK;
-  return wc::run::commit_frame(w, f, s->time);
-}
-
-// Stage::finish's statistics (lib.rs:1436-1509) without the break line.
-int wc_run_stats_from(wc_run_writer* w, wc_stats* st) {
-  if (!w || !st) return fail(WC_EINVAL, "null argument");
-  if (!st->s) return fail(WC_EINVAL, "the statistics' solver has been freed");
-  wc_solver* s = st->s;
-  TRY(wc_stats_finish(st));
-  if (!(st->seconds > 0.0)) return WC_OK;
-  const long long nx = s->geo.nx, ny = s->geo.ny;
-  const size_t n = (size_t)nx * ny;
-  std::vector<double> bs(n), hi(n), lo(n), mean(n), first(n), last(n), ha(n), hb(n);
-  double seconds = 0.0;
-  uint64_t waves = 0;
-  TRY(wc_stats_read(st, &seconds, &waves, bs.data(), hi.data(), lo.data(), mean.data(),
-                    first.data(), last.data()));
-  int32_t ok = 0;
-  TRY(wc_stats_harmonic(st, &ok, ha.data(), hb.data()));
-  auto f32s = [&](const std::vector<double>& v) {
-    std::vector<float> o(v.size());
-    for (size_t q = 0; q < v.size(); ++q) o[q] = (float)v[q];
-    return o;
-  };
-  // Stage::breaking_fraction (lib.rs:1337-1341)
-  const double t = std::max(seconds, std::numeric_limits<double>::min());
-  std::vector<float> fraction(n);
-  for (size_t q = 0; q < n; ++q) fraction[q] = (float)(bs[q] / t);
-  std::vector<float> high = f32s(hi), low = f32s(lo), height = f32s(mean), re = f32s(ha),
-                     im = f32s(hb);
-  std::vector<const char*> names = {"break_fraction", "eta_max", "eta_min"};
-  std::vector<const float*> blocks = {fraction.data(), high.data(), low.data()};
-  if (waves > 0) {
-    wc::run::note(w, "waves_recorded", wc::json::Value::uint(waves));
-    // Stage::unsettled (lib.rs:1315-1335): the median over the middle half of the width of
-    // |last / first - 1| per period, over the cells whose first wave exceeded 0.3 m
-    if (waves >= 2) {
-      std::vector<double> change;
-      for (long long j = ny / 4; j < ny - ny / 4; ++j)
-        for (long long i = 0; i < nx; ++i) {
-          const size_t m = (size_t)(j * nx + i);
-          if (first[m] > 0.3)
-            change.push_back(std::fabs(last[m] / first[m] - 1.0) / (double)(waves - 1));
-        }
-      if (!change.empty()) {
-        std::sort(change.begin(), change.end(), [](double a, double b) {
-          return wc::json::total_order(a) < wc::json::total_order(b);
-        });
-        wc::run::note(w, "unsettled_per_period", wc::json::Value::number(change[change.size() / 2]));
-      }
-    }
-    names.push_back("wave_height");
-    blocks.push_back(height.data());
-    if (ok) {
-      names.push_back("harmonic_re");
-      names.push_back("harmonic_im");
-      blocks.push_back(re.data());
-      blocks.push_back(im.data());
-    }
-  }
-  return wc_run_write_stats(w, names.data(), blocks.data(), names.size(), nullptr, st->from,
-                            s->time);
-}
-
-// StepWatch (lib.rs:861-896)
-int wc_watch_init(wc_watch_t* w, double reference) {
-  if (!w) return fail(WC_EINVAL, "null watch");
-  w->reference = std::isfinite(reference) ? reference : 0.0;
-  w->steps = 0;
-  w->seconds = 0.0;
-  return WC_OK;
-}
-
-int wc_watch_collapsed(wc_watch_t* w, double stable, double* ratio) {
-  if (!w || !ratio) return fail(WC_EINVAL, "null argument");
-  *ratio = 0.0;
-  w->steps += 1;
-  w->seconds += stable;
-  if (w->steps < WC_WATCH_STEPS) return WC_OK;
-  const double mean = w->seconds / (double)w->steps;
-  w->steps = 0;
-  w->seconds = 0.0;
-  if (mean * WC_COLLAPSE < w->reference) *ratio = w->reference / mean;
-  return WC_OK;
-}
-
-// Stage::next_dt + advance + record (lib.rs:1385-1434), without frames: the next step as long
-// as is stable and ending no later than `until`.
-int wc_stage_step(wc_solver* s, wc_stats* st, wc_watch_t* w, double until, double* dt,
-                  int32_t* status, double* ratio) {
-  if (!s || !dt || !status) return fail(WC_EINVAL, "null argument");
-  if (st && st->s != s) return fail(WC_EINVAL, "the statistics belong to another solver");
-  *status = WC_STAGE_OK;
-  if (ratio) *ratio = 0.0;
-  double stable = 0.0;
-  TRY(wc_stable_dt(s, &stable));
-  *dt = stable;
-  if (std::isnan(stable) || stable <= 0.0) {
-    *status = WC_STAGE_UNSTABLE;
-    return WC_OK;
-  }
-  if (std::isinf(stable)) {
-    *status = WC_STAGE_NO_WATER;
-    return WC_OK;
-  }
-  if (w) {
-    double r = 0.0;
-    TRY(wc_watch_collapsed(w, stable, &r));
-    if (r > 0.0) {
-      *status = WC_STAGE_COLLAPSED;
-      if (ratio) *ratio = r;
-      return WC_OK;
-    }
-  }
-  *dt = std::min(stable, until - s->time);
-  TRY(wc_step(s, *dt));
-  if (st) TRY(wc_stats_record(st, *dt));
-  return WC_OK;
-}
-
-// ---- relaxation zones (forcing.rs:227-284) ---------------------------------------------------
-
-struct wc_relaxation {
-  wc_solver* s = nullptr;
-  std::vector<uint64_t> ci, cj;       // padded (i, j), the reference's order
-  std::vector<double> rates;          // the distinct rates, in order of first appearance
-  int* cell = nullptr;                // device: padded index of each cell
-  int* rate = nullptr;                // device: index into rates
-  double* factor = nullptr;           // device: exp(-rate dt) per distinct rate
-  double* target = nullptr;           // device: staging for host targets
-  double* factor_host = nullptr;      // pinned
-  hipEvent_t factor_ev = nullptr;     // the last copy out of factor_host
-  bool factor_pending = false;
-};
-
-int wc_relaxation_new(wc_solver* s, const wc_sponge_t* zones, uint64_t n_zones,
-                      wc_relaxation** out) {
-  if (!s || !out || (n_zones > 0 && !zones)) return fail(WC_EINVAL, "null argument");
-  *out = nullptr;
-  TRY(set_device(s));
-  const Geo& g = s->geo;
-  auto r = std::make_unique<wc_relaxation>();
-  r->s = s;
-  std::vector<int> cell, rate;
-  for (uint64_t j = GH; j < (uint64_t)g.ny + GH; ++j)
-    for (uint64_t i = GH; i < (uint64_t)g.nx + GH; ++i) {
-      // Sponge::rate (forcing.rs:181-206) of each zone, folded with max from 0.0
-      double best = 0.0;
-      for (uint64_t z = 0; z < n_zones; ++z) {
-        const wc_sponge_t& sp = zones[z];
-        const uint64_t a = i - GH, b = j - GH;
-        const double w = (double)sp.width;
-        auto ramp = [&](uint64_t d) {
-          if (d < sp.width) {
-            const double f = (w - (double)d) / w;
-            return f * f;
-          }
-          return 0.0;
-        };
-        double m = 0.0;
-        if (sp.west) m = std::max(m, ramp(a));
-        if (sp.east) m = std::max(m, ramp((uint64_t)g.nx - 1 - a));
-        if (sp.south) m = std::max(m, ramp(b));
-        if (sp.north) m = std::max(m, ramp((uint64_t)g.ny - 1 - b));
-        best = std::max(best, sp.strength * m);
-      }
-      if (!(best > 0.0)) continue;
-      size_t q = 0;
-      while (q < r->rates.size() && r->rates[q] != best) ++q;
-      if (q == r->rates.size()) r->rates.push_back(best);
-      r->ci.push_back(i);
-      r->cj.push_back(j);
-      cell.push_back((int)(j * g.w + i));
-      rate.push_back((int)q);
-    }
-  const size_t n = cell.size();
-  s->relaxations.push_back(r.get());
-  struct Undo {  // a failed construction leaves nothing behind
-    std::unique_ptr<wc_relaxation>& r;
-    ~Undo() {
-      if (r) wc::detach(r.get());
-    }
-  } undo{r};
-  HIP_OK(hipMalloc(&r->cell, std::max<size_t>(n, 1) * sizeof(int)));
-  HIP_OK(hipMalloc(&r->rate, std::max<size_t>(n, 1) * sizeof(int)));
-  HIP_OK(hipMalloc(&r->target, std::max<size_t>(3 * n, 1) * sizeof(double)));
-  HIP_OK(hipMalloc(&r->factor, std::max<size_t>(r->rates.size(), 1) * sizeof(double)));
-  HIP_OK(hipHostMalloc(&r->factor_host, std::max<size_t>(r->rates.size(), 1) * sizeof(double),
-                       hipHostMallocDefault));
-  HIP_OK(hipEventCreateWithFlags(&r->factor_ev, hipEventDisableTiming));
-  if (n) {
-    HIP_OK(hipMemcpy(r->cell, cell.data(), n * sizeof(int), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(r->rate, rate.data(), n * sizeof(int), hipMemcpyHostToDevice));
-  }
-  *out = r.release();
-  return WC_OK;
-}
-
-void wc_relaxation_free(wc_relaxation* r) {
-  if (!r) return;
-  wc::detach(r);
-  delete r;
-}
-
-}  // extern "C"
-
-namespace wc {
-
-void detach(wc_stats* st) {
-  wc_solver* s = st->s;
-  if (!s) return;
-  (void)hipSetDevice(s->device);
-  (void)hipStreamSynchronize(s->stream);
-  for (void* p : st->owned) (void)hipFree(p);
-  st->owned.clear();
-  s->stats.erase(std::remove(s->stats.begin(), s->stats.end(), st), s->stats.end());
-  st->s = nullptr;
-}
-
-void detach(wc_relaxation* r) {
-  wc_solver* s = r->s;
-  if (!s) return;
-  (void)hipSetDevice(s->device);
-  if (r->factor_ev) {
-    (void)hipEventSynchronize(r->factor_ev);
-    (void)hipEventDestroy(r->factor_ev);
-    r->factor_ev = nullptr;
-  }
-  (void)hipStreamSynchronize(s->stream);
-  (void)hipFree(r->cell);
-  (void)hipFree(r->rate);
-  (void)hipFree(r->target);
-  (void)hipFree(r->factor);
-  (void)hipHostFree(r->factor_host);
-  r->cell = r->rate = nullptr;
-  r->target = r->factor = r->factor_host = nullptr;
-  s->relaxations.erase(std::remove(s->relaxations.begin(), s->relaxations.end(), r),
-                       s->relaxations.end());
-  r->s = nullptr;
-}
-
-}  // namespace wc
-
-extern "C" {
-
-uint64_t wc_relaxation_len(const wc_relaxation* r) { return r ? (uint64_t)r->ci.size() : 0; }
-
-int wc_relaxation_cells(const wc_relaxation* r, uint64_t* i, uint64_t* j) {
-  if (!r) return fail(WC_EINVAL, "null relaxation");
-  for (size_t m = 0; m < r->ci.size(); ++m) {
-    if (i) i[m] = r->ci[m];
-    if (j) j[m] = r->cj[m];
-  }
-  return WC_OK;
-}
-
-namespace {
-
-int relax_apply(wc_relaxation* r, double dt, const double* targets_device) {
-  wc_solver* s = r->s;
-  const size_t n = r->ci.size();
-  if (n == 0) return WC_OK;
-  // exp(-rate dt) with the reference's expression, once per distinct rate; the pinned buffer
-  // is rewritten only after its previous copy has left it.
-  if (r->factor_pending) HIP_OK(hipEventSynchronize(r->factor_ev));
-  for (size_t q = 0; q < r->rates.size(); ++q) r->factor_host[q] = exp(-r->rates[q] * dt);
-  HIP_OK(hipMemcpyAsync(r->factor, r->factor_host, r->rates.size() * sizeof(double),
-                        hipMemcpyHostToDevice, s->stream));
-  HIP_OK(hipEventRecord(r->factor_ev, s->stream));
-  r->factor_pending = true;
-  const RelaxArgs a{(long long)n, r->cell, r->rate, r->factor, targets_device, s->b};
-  const dim3 grid((unsigned)((n + NT - 1) / NT));
-  if (s->precision == WC_PRECISION_FP32)
-    TRY(launch(s, F_RELAX, k_relax32, grid, dim3(NT), a, s->f_sh, s->f_slh, s->f_shu, s->f_slhu,
-               s->f_shv, s->f_slhv));
-  else
-    TRY(launch(s, F_RELAX, k_relax64, grid, dim3(NT), a, s->h, s->hu, s->hv));
-  return WC_OK;
-}
-
-}  // namespace
-
-int wc_relaxation_apply(wc_relaxation* r, double dt, const double* targets) {
-  if (!r || (!targets && !r->ci.empty())) return fail(WC_EINVAL, "null argument");
-  if (!r->s) return fail(WC_EINVAL, "the relaxation's solver has been freed");
-  TRY(set_device(r->s));
-  const size_t n = r->ci.size();
-  if (n)
-    HIP_OK(hipMemcpyAsync(r->target, targets, 3 * n * sizeof(double), hipMemcpyHostToDevice,
-                          r->s->stream));
-  TRY(relax_apply(r, dt, r->target));
-  if (!r->s->prof) HIP_OK(hipStreamSynchronize(r->s->stream));
-  return WC_OK;
-}
-
-int wc_relaxation_apply_device(wc_relaxation* r, double dt, const double* targets) {
-  if (!r || (!targets && !r->ci.empty())) return fail(WC_EINVAL, "null argument");
-  if (!r->s) return fail(WC_EINVAL, "the relaxation's solver has been freed");
-  TRY(set_device(r->s));
-  TRY(relax_apply(r, dt, targets));
-  if (!r->s->prof) HIP_OK(hipStreamSynchronize(r->s->stream));
-  return WC_OK;
-}
-
-}  // extern "C"
-
-// ---- the coarse-to-fine sampler of nested runs (crates/wavesim/src/lib.rs:1548-1616) --------
-
-struct wc_sampler {
-  int device = 0;
-  long long n = 0;       // zone cells
-  int cnx = 0, cny = 0;  // the coarse grid the corners index
-  int* idx = nullptr;    // device: four padded coarse indices per zone cell
-  double* w = nullptr;   // device: their bilinear weights
-  double* smp[2] = {nullptr, nullptr};  // device: (eta, u, v) per zone cell, then and now
-  double t[2] = {NAN, NAN};
-  double* targets = nullptr;  // device: the interpolated targets of the last apply
-  std::vector<void*> owned;
-};
-
-extern "C" {
-
-// Sampler::new (lib.rs:1559-1590): the fine zone cell (i, j) sits at x = (x0 + i - GHOST + 0.5)
-// fine cells of the bed file; its coarse neighbours are clamped to one ghost layer.
-int wc_sampler_new(const wc_relaxation* zone, uint64_t fine_x0, double fine_dx, double fine_dy,
-                   uint64_t coarse_nx, uint64_t coarse_ny, double coarse_dx, double coarse_dy,
-                   uint64_t crop, wc_sampler** out) {
-  if (!zone || !out) return fail(WC_EINVAL, "null argument");
-  if (!zone->s) return fail(WC_EINVAL, "the relaxation's solver has been freed");
-  if (coarse_nx < 3 || coarse_ny < 3 || !(coarse_dx > 0.0) || !(coarse_dy > 0.0))
-    return fail(WC_EINVAL, "the coarse grid needs at least 3 x 3 cells of positive size");
-  *out = nullptr;
-  const size_t n = zone->ci.size();
-  const long long cw = (long long)coarse_nx + 2 * GH;
-  std::vector<int> idx(4 * std::max<size_t>(n, 1), 0);
-  std::vector<double> w(4 * std::max<size_t>(n, 1), 0.0);
-  for (size_t m = 0; m < n; ++m) {
-    const double x = (double)(fine_x0 + zone->ci[m] - GH) * fine_dx + 0.5 * fine_dx;
-    const double y = (double)(zone->cj[m] - GH) * fine_dy + 0.5 * fine_dy;
-    const double cx = x / coarse_dx - (double)crop - 0.5;
-    const double cy = y / coarse_dy - 0.5;
-    const double ix = std::floor(cx), iy = std::floor(cy);
-    const double wx = cx - ix, wy = cy - iy;
-    auto at = [&](double di, double dj) {
-      const long long pi = std::clamp((long long)(ix + di) + GH, (long long)GH - 1,
-                                      (long long)coarse_nx + GH);
-      const long long pj = std::clamp((long long)(iy + dj) + GH, (long long)GH - 1,
-                                      (long long)coarse_ny + GH);
-      return (int)(pj * cw + pi);
-    };
-    idx[4 * m] = at(0.0, 0.0);
-    idx[4 * m + 1] = at(1.0, 0.0);
-    idx[4 * m + 2] = at(0.0, 1.0);
-    idx[4 * m + 3] = at(1.0, 1.0);
-    w[4 * m] = (1.0 - wx) * (1.0 - wy);
-    w[4 * m + 1] = wx * (1.0 - wy);
-    w[4 * m + 2] = (1.0 - wx) * wy;
-    w[4 * m + 3] = wx * wy;
-  }
-  auto sm = std::make_unique<wc_sampler>();
-  sm->device = zone->s->device;
-  sm->n = (long long)n;
-  sm->cnx = (int)coarse_nx;
-  sm->cny = (int)coarse_ny;
-  HIP_OK(hipSetDevice(sm->device));
-  struct Undo {
-    std::unique_ptr<wc_sampler>& sm;
-    ~Undo() {
-      if (sm)
-        for (void* p : sm->owned) (void)hipFree(p);
-    }
-  } undo{sm};
-  auto get = [&](void** p, size_t bytes) -> int {
-    HIP_OK(hipMalloc(p, std::max<size_t>(bytes, 8)));
-    sm->owned.push_back(*p);
-    return WC_OK;
-  };
-  TRY(get((void**)&sm->idx, idx.size() * sizeof(int)));
-  TRY(get((void**)&sm->w, w.size() * sizeof(double)));
-  for (int q = 0; q < 2; ++q) TRY(get((void**)&sm->smp[q], 3 * n * sizeof(double)));
-  TRY(get((void**)&sm->targets, 3 * n * sizeof(double)));
-  HIP_OK(hipMemcpy(sm->idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(sm->w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
-  *out = sm.release();
-  return WC_OK;
-}
-
-void wc_sampler_free(wc_sampler* sm) {
-  if (!sm) return;
-  (void)hipSetDevice(sm->device);
-  for (void* p : sm->owned) (void)hipFree(p);
-  delete sm;
-}
-
-// Sampler::sample (lib.rs:1592-1614) of the coarse solver's current state into slot 0 (the
-// state before the coarse step that brackets the fine one) or 1 (after it), at its time.
-int wc_sampler_sample(wc_sampler* sm, wc_solver* coarse, int32_t slot) {
-  if (!sm || !coarse) return fail(WC_EINVAL, "null argument");
-  if (slot != 0 && slot != 1) return fail(WC_EINVAL, "slot must be 0 or 1");
-  if (coarse->geo.nx != sm->cnx || coarse->geo.ny != sm->cny)
-    return fail(WC_EINVAL, "the coarse solver's grid is not the sampler's");
-  if (coarse->device != sm->device) return fail(WC_EINVAL, "the sampler is on another device");
-  TRY(set_device(coarse));
-  TRY(join_state(coarse));
-  Geo g = coarse->geo;
-  g.periodic = 0;  // fill_boundaries(grid, false): walls
-  if (sm->n > 0)
-    TRY(launch(coarse, F_RELAX, k_sample, over(sm->n), dim3(NT), (long long)sm->n,
-               (const int*)sm->idx, (const double*)sm->w, g, (const double*)coarse->h,
-               (const double*)coarse->hu, (const double*)coarse->hv,
-               (const double*)coarse->b, sm->smp[slot]));
-  sm->t[slot] = coarse->time;
-  HIP_OK(hipStreamSynchronize(coarse->stream));
-  return WC_OK;
-}
-
-// The nested run's relaxation after a fine step of dt ending at `end` (lib.rs:1866-1897): the
-// targets interpolated between the two samples at w = (end - t0) / (t1 - t0) clamped to [0, 1],
-// then Relaxation::apply on the fine solver.
-int wc_sampler_apply(wc_sampler* sm, wc_relaxation* zone, double end, double dt) {
-  if (!sm || !zone) return fail(WC_EINVAL, "null argument");
-  if (!zone->s) return fail(WC_EINVAL, "the relaxation's solver has been freed");
-  if ((long long)zone->ci.size() != sm->n)
-    return fail(WC_EINVAL, "the relaxation is not the sampler's zone");
-  if (!(std::isfinite(sm->t[0]) && std::isfinite(sm->t[1])))
-    return fail(WC_EINVAL, "both samples must be taken before the targets are interpolated");
-  wc_solver* s = zone->s;
-  TRY(set_device(s));
-  const double w = std::clamp((end - sm->t[0]) / (sm->t[1] - sm->t[0]), 0.0, 1.0);
-  if (sm->n > 0)
-    TRY(launch(s, F_RELAX, k_lerp_targets, over(sm->n), dim3(NT), (long long)sm->n,
-               (const double*)sm->smp[0], (const double*)sm->smp[1], w, sm->targets));
-  TRY(relax_apply(zone, dt, sm->targets));
-  if (!s->prof) HIP_OK(hipStreamSynchronize(s->stream));
-  return WC_OK;
-}
-
-// slot 0 / 1: a sample, 2: the last targets; 3 n doubles (eta, u, v) per zone cell, and the
-// sample's time.
-int wc_sampler_read(const wc_sampler* sm, int32_t slot, double* out, double* time) {
-  if (!sm) return fail(WC_EINVAL, "null sampler");
-  if (slot < 0 || slot > 2) return fail(WC_EINVAL, "slot must be 0, 1 or 2");
-  HIP_OK(hipSetDevice(sm->device));
-  HIP_OK(hipDeviceSynchronize());
-  const double* src = slot == 2 ? sm->targets : sm->smp[slot];
-  if (out && sm->n)
-    HIP_OK(hipMemcpy(out, src, 3 * sm->n * sizeof(double), hipMemcpyDeviceToHost));
-  if (time) *time = slot == 2 ? NAN : sm->t[slot];
-  return WC_OK;
-}
-
 int wc_run_steps(wc_solver* s, uint64_t steps, uint64_t* done_steps, double* last_dt) {
   TRY(set_device(s));
   uint64_t n = 0;
@@ -2984,845 +2039,6 @@ int wc_profile_read(wc_solver* s, char* names, uint64_t names_len, double* total
     names[names_len - 1] = '\0';
   }
   *n_kernels = F_COUNT;
-  return WC_OK;
-}
-
-}  // extern "C"
-
-// ================================================ many instances in one launch (wc_batch_*)
-// Every phase of a step is one launch over all instances (kb_* kernels): each instance keeps
-// its own arrays, dt and PCG scalars, and its virtual blocks run exactly the single-instance
-// bodies, so an instance advances bit for bit as it would alone (FP32 form, one-kernel PCG).
-// Per step the host reads the instances' rates once, evaluates their step parameters into one
-// pinned buffer, and reads the PCG flags once per batch of iterations.
-
-struct wc_batch {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::vector<wc_solver*> inst;
-  std::vector<int> active;  // indices of the instances still stepping
-  bool rk2 = true, sw = false, dispersive = false;
-  int lanes = 256;  // the instances' one-kernel PCG lanes
-  bool march = true;  // and its layout (pcg_march)
-  bool recon64 = false;  // the instances' flux reconstruction in FP64
-  bool march_flux = true;  // the instances' flux: marching strips (flux_kernel 3) or 64x4 tiles
-  int flux_rows = 32;      // the marching flux's band height (tuned unless fixed by hand)
-  bool flux_rows_fixed = false;
-  std::vector<std::pair<int, double>> tune_rows;  // (band height, us per launch) as measured
-  unsigned long long maxit = 0, pcg_batch = 4;
-  int grid_nt = 2048, grid_pt = 512;  // persistent widths of the flux and the PCG launches
-  int grid_st = 1 << 30;  // and of the other 256-thread launches: a block per virtual block
-                          // (their blocks are short and few registers deep), unless set by hand
-  bool tuned = false;
-  std::vector<std::pair<int, double>> tune_nt, tune_pt;  // (width, us per launch) as measured
-
-  // device: argument blocks of the active instances and the launch maps
-  char* d_args = nullptr;
-  size_t args_cap = 0;
-  BFill* fill[2] = {};
-  BSwitch* swa[2] = {};
-  BFlux* flux[2] = {};
-  Sgn32* sgn[2] = {};
-  CgG* cg[2] = {};
-  Stage32* stg[2] = {};
-  BRate* rate = nullptr;
-  BMap m_fill{}, m_sw{}, m_tiles{}, m_rate{}, m_cg{}, m_march{};
-  long long* d_off = nullptr;
-  double** d_dst = nullptr;
-  double* d_stage = nullptr;  // [dt of each active instance | their step parameters]
-  size_t stage_cap = 0;
-  unsigned long long* d_red = nullptr;
-  int* d_done = nullptr;
-  unsigned long long* d_it = nullptr;
-  // pinned
-  double* h_stage = nullptr;
-  unsigned long long* h_red = nullptr;
-  int* h_done = nullptr;
-  unsigned long long* h_it = nullptr;
-  std::vector<long long> off;  // step-parameter offsets of the active instances
-  std::vector<unsigned long long> prev_it[2];
-  std::vector<void*> owned, owned_host;
-
-  // HIP graphs of the fixed stretches of a step (use_graph): [0] stage 1 up to K[0] PCG
-  // iterations, [1] the stage-1 update and stage 2 up to K[1] iterations, [2] the stage-2
-  // update; rebuilt when the argument blocks move or a K changes
-  bool use_graph = true;
-  hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
-  unsigned long long graph_k[2] = {~0ull, ~0ull};
-  unsigned long long graph_builds = 0;
-
-  // profiling (option "profile"): each PCG iteration launch between a pair of events, the
-  // stretches issued directly instead of from the graphs
-  bool prof = false;
-  double prof_ms = 0.0;
-  unsigned long long prof_n = 0;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_pending;
-  std::vector<hipEvent_t> prof_pool;
-};
-
-namespace {
-
-int bfail(const std::string& m) { return fail(WC_EINVAL, m); }
-
-void drop_graphs(wc_batch* b) {
-  for (hipGraphExec_t& g : b->graph) {
-    if (g) (void)hipGraphExecDestroy(g);
-    g = nullptr;
-  }
-  b->graph_k[0] = b->graph_k[1] = ~0ull;
-}
-
-template <class T>
-int balloc(wc_batch* b, T** p, size_t n) {
-  void* v = nullptr;
-  HIP_OK(hipMalloc(&v, std::max<size_t>(n, 1) * sizeof(T)));
-  b->owned.push_back(v);
-  *p = static_cast<T*>(v);
-  return WC_OK;
-}
-
-template <class T>
-int bhost(wc_batch* b, T** p, size_t n) {
-  void* v = nullptr;
-  HIP_OK(hipHostMalloc(&v, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault));
-  b->owned_host.push_back(v);
-  *p = static_cast<T*>(v);
-  return WC_OK;
-}
-
-// The one-kernel PCG's blocks that have work: 64x16 tiles, or strip units when marching
-// (mrch::segments), at most its lanes.
-int pcg_blocks(const wc_solver* s) {
-  const int L = s->pcg_lanes;
-  if (s->pcg_march) {
-    const int strips = (s->geo.nx + 63) / 64;
-    const int segs = std::min(strips >= L ? 1 : L / strips, s->geo.ny);
-    return std::min(strips * segs, L);
-  }
-  return tile_stride(s);
-}
-
-// (Re)builds every argument block and launch map for the active instances, in one upload.
-int batch_build(wc_batch* b) {
-  // largest instances first: their virtual blocks are the longest (a PCG block walks several
-  // tiles of a large grid), so they start first and the short ones fill the tail of a launch
-  std::stable_sort(b->active.begin(), b->active.end(), [&](int x, int y) {
-    const Geo &gx = b->inst[x]->geo, &gy = b->inst[y]->geo;
-    return (long long)gx.nx * gx.ny > (long long)gy.nx * gy.ny;
-  });
-  const int n = (int)b->active.size();
-  std::vector<int> f_fill(n + 1, 0), f_sw(n + 1, 0), f_tiles(n + 1, 0), f_rate(n + 1, 0),
-      f_cg(n + 1, 0), f_march(n + 1, 0);
-  auto rows_of = [&](const Geo& g) { return std::max(1, std::min(b->flux_rows, g.ny)); };
-  b->off.assign(n + 1, 0);
-  for (int q = 0; q < n; ++q) {
-    const wc_solver* s = b->inst[b->active[q]];
-    const Geo& g = s->geo;
-    f_fill[q + 1] = f_fill[q] + (int)((6LL * g.w + 6LL * g.ny + NT - 1) / NT);
-    f_sw[q + 1] = f_sw[q] + (int)(((long long)g.w * g.rows + NT - 1) / NT);
-    f_tiles[q + 1] = f_tiles[q] + g.ntx * g.nty;
-    f_rate[q + 1] = f_rate[q] + (int)rate_blocks((long long)g.nx * g.ny).x;
-    f_cg[q + 1] = f_cg[q] + pcg_blocks(s);
-    f_march[q + 1] = f_march[q] + ((g.nx + MARCH_MW - 1) / MARCH_MW) *
-                                      ((g.ny + rows_of(g) - 1) / rows_of(g));
-    b->off[q + 1] = b->off[q] + (long long)s->prm_len;
-  }
-  // host image: the blocks, then the maps and the parameter routes
-  std::vector<char> img;
-  auto put = [&](const void* p, size_t bytes) {
-    size_t at = (img.size() + 255) & ~(size_t)255;
-    img.resize(at + bytes);
-    memcpy(img.data() + at, p, bytes);
-    return at;
-  };
-  std::vector<BFill> fill[2];
-  std::vector<BSwitch> sw[2];
-  std::vector<BFlux> flux[2];
-  std::vector<Sgn32> sgn[2];
-  std::vector<CgG> cg[2];
-  std::vector<Stage32> stg[2];
-  std::vector<BRate> rate;
-  std::vector<double*> dst;
-  for (int q = 0; q < n; ++q) {
-    wc_solver* s = b->inst[b->active[q]];
-    for (int st = 0; st < 2; ++st) {
-      BFill f{};
-      f.g = s->geo;
-      f.h = st == 0 ? s->f_sh : s->f_s1h;
-      f.hl = st == 0 ? s->f_slh : s->f_s1hl;
-      f.hu = st == 0 ? s->f_shu : s->f_s1hu;
-      f.hv = st == 0 ? s->f_shv : s->f_s1hv;
-      f.hul = st == 0 ? s->f_slhu : s->f_s1hul;
-      f.hvl = st == 0 ? s->f_slhv : s->f_s1hvl;
-      fill[st].push_back(f);
-      sw[st].push_back(switch_args32(s, st));
-      flux[st].push_back(flux_args32(s, st));
-      flux[st].back().rows = rows_of(s->geo);
-      sgn[st].push_back(sgn32_args(s, st));
-      cg[st].push_back(cg1_args(s, st));
-      stg[st].push_back(stage_args32(s, 0.0, st));  // dt comes per step from d_stage
-    }
-    BRate r{};
-    r.g = s->geo;
-    r.h = s->f_sh;
-    r.hl = s->f_slh;
-    r.hu = s->f_shu;
-    r.hul = s->f_slhu;
-    r.hv = s->f_shv;
-    r.hvl = s->f_slhv;
-    r.out = b->d_red + q;
-    rate.push_back(r);
-    dst.push_back(s->prm);
-  }
-  size_t o_fill[2], o_sw[2], o_flux[2], o_sgn[2], o_cg[2], o_stg[2];
-  for (int st = 0; st < 2; ++st) {
-    o_fill[st] = put(fill[st].data(), n * sizeof(BFill));
-    o_sw[st] = put(sw[st].data(), n * sizeof(BSwitch));
-    o_flux[st] = put(flux[st].data(), n * sizeof(BFlux));
-    o_sgn[st] = put(sgn[st].data(), n * sizeof(Sgn32));
-    o_cg[st] = put(cg[st].data(), n * sizeof(CgG));
-    o_stg[st] = put(stg[st].data(), n * sizeof(Stage32));
-  }
-  const size_t o_rate = put(rate.data(), n * sizeof(BRate));
-  const size_t o_ffill = put(f_fill.data(), (n + 1) * sizeof(int));
-  const size_t o_fsw = put(f_sw.data(), (n + 1) * sizeof(int));
-  const size_t o_ftiles = put(f_tiles.data(), (n + 1) * sizeof(int));
-  const size_t o_frate = put(f_rate.data(), (n + 1) * sizeof(int));
-  const size_t o_fcg = put(f_cg.data(), (n + 1) * sizeof(int));
-  const size_t o_fmarch = put(f_march.data(), (n + 1) * sizeof(int));
-  const size_t o_off = put(b->off.data(), (n + 1) * sizeof(long long));
-  const size_t o_dst = put(dst.data(), n * sizeof(double*));
-  if (img.size() > b->args_cap) {
-    if (b->d_args) {
-      HIP_OK(hipStreamSynchronize(b->stream));
-      b->owned.erase(std::find(b->owned.begin(), b->owned.end(), (void*)b->d_args));
-      HIP_OK(hipFree(b->d_args));
-      b->d_args = nullptr;
-    }
-    TRY(balloc(b, &b->d_args, img.size()));
-    b->args_cap = img.size();
-  }
-  HIP_OK(hipMemcpyAsync(b->d_args, img.data(), img.size(), hipMemcpyHostToDevice, b->stream));
-  HIP_OK(hipStreamSynchronize(b->stream));
-  char* d = b->d_args;
-  for (int st = 0; st < 2; ++st) {
-    b->fill[st] = reinterpret_cast<BFill*>(d + o_fill[st]);
-    b->swa[st] = reinterpret_cast<BSwitch*>(d + o_sw[st]);
-    b->flux[st] = reinterpret_cast<BFlux*>(d + o_flux[st]);
-    b->sgn[st] = reinterpret_cast<Sgn32*>(d + o_sgn[st]);
-    b->cg[st] = reinterpret_cast<CgG*>(d + o_cg[st]);
-    b->stg[st] = reinterpret_cast<Stage32*>(d + o_stg[st]);
-  }
-  b->rate = reinterpret_cast<BRate*>(d + o_rate);
-  b->m_fill = BMap{reinterpret_cast<const int*>(d + o_ffill), n, f_fill[n]};
-  b->m_sw = BMap{reinterpret_cast<const int*>(d + o_fsw), n, f_sw[n]};
-  b->m_tiles = BMap{reinterpret_cast<const int*>(d + o_ftiles), n, f_tiles[n]};
-  b->m_rate = BMap{reinterpret_cast<const int*>(d + o_frate), n, f_rate[n]};
-  b->m_cg = BMap{reinterpret_cast<const int*>(d + o_fcg), n, f_cg[n]};
-  b->m_march = BMap{reinterpret_cast<const int*>(d + o_fmarch), n, f_march[n]};
-  b->d_off = reinterpret_cast<long long*>(d + o_off);
-  b->d_dst = reinterpret_cast<double**>(d + o_dst);
-  drop_graphs(b);  // the captured launches point at the old blocks
-  const size_t need = (size_t)n + (size_t)b->off[n];
-  if (need > b->stage_cap) {
-    // the pinned and device staging are only ever reused after a synchronisation
-    TRY(bhost(b, &b->h_stage, need));
-    TRY(balloc(b, &b->d_stage, need));
-    b->stage_cap = need;
-  }
-  for (int st = 0; st < 2; ++st) {
-    std::vector<unsigned long long> keep(n, 0);
-    for (int q = 0; q < n; ++q) keep[q] = b->inst[b->active[q]]->prev_it[st];
-    b->prev_it[st] = keep;
-  }
-  return WC_OK;
-}
-
-dim3 bgrid(int total, int width) { return dim3((unsigned)std::max(1, std::min(total, width))); }
-
-// The launches of one stretch of a step, in order (captured into a graph or issued directly).
-void enq_fill(wc_batch* b, int st) {
-  hipLaunchKernelGGL(kb_fill32, bgrid(b->m_fill.total, b->grid_st), dim3(NT), 0, b->stream,
-                     b->fill[st], b->m_fill);
-}
-
-// The rates of every instance: the marching flux (one wave per strip and band, a virtual block
-// each) or the 64x4 tile flux at its persistent width.
-void enq_flux(wc_batch* b, int st) {
-  if (b->march_flux)
-    hipLaunchKernelGGL(b->recon64 ? kb_flux_march32r : kb_flux_march32,
-                       bgrid(b->m_march.total, 1 << 30), dim3(TX), 0, b->stream, b->flux[st],
-                       b->m_march);
-  else
-    hipLaunchKernelGGL(b->recon64 ? kb_flux32r : kb_flux32, bgrid(b->m_tiles.total, b->grid_nt),
-                       dim3(NT), 0, b->stream, b->flux[st], b->m_tiles);
-}
-
-void enq_stage_head(wc_batch* b, int st) {  // the switch, the rates, the SGN set-up
-  if (b->sw)
-    hipLaunchKernelGGL(kb_switch32, bgrid(b->m_sw.total, b->grid_st), dim3(NT), 0, b->stream,
-                       b->swa[st], b->m_sw);
-  enq_flux(b, st);
-  if (b->dispersive)
-    hipLaunchKernelGGL(kb_sgn_setup32, bgrid(b->m_tiles.total, b->grid_st), dim3(NT), 0,
-                       b->stream, b->sgn[st], b->m_tiles);
-}
-
-// A PCG iteration launch, timed between two events when the batch profiles.
-template <class Launch>
-void timed_iter(wc_batch* b, Launch fn) {
-  hipEvent_t e[2] = {nullptr, nullptr};
-  if (b->prof)
-    for (hipEvent_t& x : e) {
-      if (!b->prof_pool.empty()) {
-        x = b->prof_pool.back();
-        b->prof_pool.pop_back();
-      } else if (hipEventCreate(&x) != hipSuccess) {
-        x = nullptr;
-      }
-    }
-  if (e[0] && e[1]) (void)hipEventRecord(e[0], b->stream);
-  fn();
-  if (e[0] && e[1]) {
-    (void)hipEventRecord(e[1], b->stream);
-    b->prof_pending.push_back({e[0], e[1]});
-  } else {
-    for (hipEvent_t x : e)
-      if (x) b->prof_pool.push_back(x);
-  }
-}
-
-int batch_prof_collect(wc_batch* b) {
-  if (b->prof_pending.empty()) return WC_OK;
-  HIP_OK(hipStreamSynchronize(b->stream));
-  for (auto& pe : b->prof_pending) {
-    float ms = 0.f;
-    HIP_OK(hipEventElapsedTime(&ms, pe.first, pe.second));
-    b->prof_ms += ms;
-    b->prof_n += 1;
-    b->prof_pool.push_back(pe.first);
-    b->prof_pool.push_back(pe.second);
-  }
-  b->prof_pending.clear();
-  return WC_OK;
-}
-
-template <int L>
-void enq_pcg_head_l(wc_batch* b, int st, unsigned long long k) {
-  const dim3 gp = bgrid(b->m_cg.total, b->grid_pt);
-  if (b->march) {
-    hipLaunchKernelGGL(kbm_cg1_init<L>, gp, dim3(1024), 0, b->stream, b->cg[st], b->m_cg);
-    hipLaunchKernelGGL(kbm_cg1_start<L>, gp, dim3(1024), 0, b->stream, b->cg[st], b->m_cg);
-    for (unsigned long long it = 1; it <= k; ++it)
-      timed_iter(b, [&] {
-        hipLaunchKernelGGL(kbm_cg1_iter<L>, gp, dim3(1024), 0, b->stream, b->cg[st], it,
-                           b->m_cg);
-      });
-    return;
-  }
-  hipLaunchKernelGGL(kb_cg1_init<L>, gp, dim3(1024), 0, b->stream, b->cg[st], b->m_cg);
-  hipLaunchKernelGGL(kb_cg1_start<L>, gp, dim3(1024), 0, b->stream, b->cg[st], b->m_cg);
-  for (unsigned long long it = 1; it <= k; ++it)
-    timed_iter(b, [&] {
-      hipLaunchKernelGGL(kb_cg1_iter<L>, gp, dim3(1024), 0, b->stream, b->cg[st], it, b->m_cg);
-    });
-}
-
-void enq_pcg_head(wc_batch* b, int st, unsigned long long k) {  // init, start, iterations 1..k
-  if (b->lanes == 512)
-    enq_pcg_head_l<512>(b, st, k);
-  else
-    enq_pcg_head_l<256>(b, st, k);
-}
-
-void enq_iter_untimed(wc_batch* b, int st, unsigned long long it) {
-  const dim3 gp = bgrid(b->m_cg.total, b->grid_pt);
-  if (b->march) {
-    if (b->lanes == 512)
-      hipLaunchKernelGGL(kbm_cg1_iter<512>, gp, dim3(1024), 0, b->stream, b->cg[st], it, b->m_cg);
-    else
-      hipLaunchKernelGGL(kbm_cg1_iter<256>, gp, dim3(1024), 0, b->stream, b->cg[st], it, b->m_cg);
-    return;
-  }
-  if (b->lanes == 512)
-    hipLaunchKernelGGL(kb_cg1_iter<512>, gp, dim3(1024), 0, b->stream, b->cg[st], it, b->m_cg);
-  else
-    hipLaunchKernelGGL(kb_cg1_iter<256>, gp, dim3(1024), 0, b->stream, b->cg[st], it, b->m_cg);
-}
-
-void enq_iter(wc_batch* b, int st, unsigned long long it) {
-  timed_iter(b, [&] { enq_iter_untimed(b, st, it); });
-}
-
-void enq_check(wc_batch* b, int st, unsigned long long it, int stats) {
-  const int n = (int)b->active.size();
-  if (b->lanes == 512)
-    hipLaunchKernelGGL(kb_cg1_check<512>, dim3(n), dim3(64), 0, b->stream, b->cg[st], it, stats,
-                       b->d_done, b->d_it);
-  else
-    hipLaunchKernelGGL(kb_cg1_check<256>, dim3(n), dim3(64), 0, b->stream, b->cg[st], it, stats,
-                       b->d_done, b->d_it);
-}
-
-void enq_update(wc_batch* b, int st, int mode) {
-  hipLaunchKernelGGL(kb_stage32, bgrid(b->m_tiles.total, b->grid_st), dim3(NT), 0, b->stream,
-                     b->stg[st], (const double*)b->d_stage, mode, b->m_tiles);
-}
-
-// The rest of stage st's solves after iterations 1..k: first up to the largest count any
-// instance needed last time, then batches of pcg_batch iterations, one look at all flags per
-// batch; then the solves are counted. An instance that has converged leaves the launches after
-// reading its flag, so queueing past its count changes none of its numbers.
-int batch_pcg_tail(wc_batch* b, int st, unsigned long long k) {
-  const int n = (int)b->active.size();
-  unsigned long long it = k + 1;
-  // the largest count of the instances' last solves at this stage, leaving out solves that ran
-  // out of iterations (one instance at max_iterations would otherwise queue that many unchecked
-  // launches for all), and at most pcg_batch past the smallest
-  unsigned long long last = 0, least = ~0ull;
-  for (int q = 0; q < n; ++q) {
-    const unsigned long long v = b->prev_it[st][q];
-    least = std::min(least, v);
-    if (v < b->maxit) last = std::max(last, v);
-  }
-  if (least != ~0ull) last = std::min(last, least + b->pcg_batch);
-  unsigned long long batch = last > k ? last - k : b->pcg_batch;
-  bool look = k > 0 && last <= k;  // nothing to queue before the first look
-  while (true) {
-    if (!look) {
-      const unsigned long long stop = std::min(it + batch, b->maxit + 1);
-      for (; it < stop; ++it) enq_iter(b, st, it);
-      batch = b->pcg_batch;
-    }
-    look = false;
-    enq_check(b, st, it, 0);
-    HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(b->h_done, b->d_done, n * sizeof(int), hipMemcpyDeviceToHost,
-                          b->stream));
-    HIP_OK(hipStreamSynchronize(b->stream));
-    bool all = true;
-    for (int q = 0; q < n; ++q) all = all && b->h_done[q];
-    if (all || it > b->maxit) break;
-  }
-  enq_check(b, st, it, 1);
-  hipLaunchKernelGGL(kb_cg1_flush, bgrid(b->m_tiles.total, b->grid_st), dim3(NT), 0, b->stream,
-                     b->cg[st], b->m_tiles);
-  HIP_OK(hipGetLastError());
-  HIP_OK(hipMemcpyAsync(b->h_it, b->d_it, n * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                        b->stream));
-  HIP_OK(hipStreamSynchronize(b->stream));
-  for (int q = 0; q < n; ++q) {
-    b->prev_it[st][q] = b->h_it[q];
-    b->inst[b->active[q]]->prev_it[st] = b->h_it[q];
-  }
-  return WC_OK;
-}
-
-// The iterations a stage's graph queues: a multiple of 4 below every instance's last count,
-// so that a graph is rebuilt only when the counts drift.
-unsigned long long graph_iterations(const wc_batch* b, int st) {
-  unsigned long long first = ~0ull;
-  for (unsigned long long v : b->prev_it[st]) first = std::min(first, v);
-  if (first == ~0ull || first < 2) return 0;
-  return ((first - 1) / 4) * 4;
-}
-
-// Stretch `seg` of a step (see wc_batch::graph), issued directly.
-void enq_segment(wc_batch* b, int seg, const unsigned long long* k) {
-  if (seg == 0) {
-    enq_fill(b, 0);
-    enq_stage_head(b, 0);
-    if (b->dispersive) enq_pcg_head(b, 0, k[0]);
-  } else if (seg == 1) {
-    enq_update(b, 0, b->rk2 ? MODE_S1 : MODE_EULER);
-    if (!b->rk2) return;
-    enq_fill(b, 1);
-    enq_stage_head(b, 1);
-    if (b->dispersive) enq_pcg_head(b, 1, k[1]);
-  } else {
-    enq_update(b, 1, MODE_FINAL);
-  }
-}
-
-int run_segment(wc_batch* b, int seg, const unsigned long long* k) {
-  if (!b->use_graph || b->prof) {
-    enq_segment(b, seg, k);
-    HIP_OK(hipGetLastError());
-    return WC_OK;
-  }
-  const bool stale = !b->graph[seg] || (seg == 0 && b->graph_k[0] != k[0]) ||
-                     (seg == 1 && b->graph_k[1] != k[1]);
-  if (stale) {
-    if (b->graph[seg]) HIP_OK(hipGraphExecDestroy(b->graph[seg]));
-    b->graph[seg] = nullptr;
-    hipGraph_t g = nullptr;
-    HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    enq_segment(b, seg, k);
-    const hipError_t launched = hipGetLastError();
-    HIP_OK(hipStreamEndCapture(b->stream, &g));
-    HIP_OK(launched);
-    const hipError_t inst = hipGraphInstantiate(&b->graph[seg], g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    HIP_OK(inst);
-    if (seg < 2) b->graph_k[seg] = k[seg];
-    b->graph_builds += 1;
-  }
-  HIP_OK(hipGraphLaunch(b->graph[seg], b->stream));
-  return WC_OK;
-}
-
-// One step of every active instance: three stretches with the PCG tails between them.
-int batch_step(wc_batch* b) {
-  const unsigned long long k[2] = {b->dispersive ? graph_iterations(b, 0) : 0,
-                                   b->dispersive ? graph_iterations(b, 1) : 0};
-  TRY(run_segment(b, 0, k));
-  if (b->dispersive) TRY(batch_pcg_tail(b, 0, k[0]));
-  TRY(run_segment(b, 1, k));
-  if (b->rk2) {
-    if (b->dispersive) TRY(batch_pcg_tail(b, 1, k[1]));
-    TRY(run_segment(b, 2, k));
-  }
-  return WC_OK;
-}
-
-// The autotuner, each choice the fastest of a few candidates timed on launches that change no
-// state: the marching flux's band height, the tile shape of its waves (58 x rows cells; timed
-// on the flux launch, which only writes rates), or with the 64x4 tile flux the persistent width
-// of its launch; and the persistent width of the 1024-thread PCG launches (timed on
-// kb_cg1_start, whose outputs the next solve's start rewrites). No choice changes a bit.
-int batch_tune(wc_batch* b) {
-  hipEvent_t e0, e1;
-  HIP_OK(hipEventCreate(&e0));
-  HIP_OK(hipEventCreate(&e1));
-  auto time = [&](auto launch_fn) -> double {
-    launch_fn();  // warm
-    HIP_OK(hipEventRecord(e0, b->stream));
-    for (int r = 0; r < 3; ++r) launch_fn();
-    HIP_OK(hipEventRecord(e1, b->stream));
-    HIP_OK(hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-    return 1e3 * ms / 3.0;
-  };
-  b->tune_nt.clear();
-  b->tune_pt.clear();
-  int best_nt = b->grid_nt, best_pt = b->grid_pt;
-  double t_nt = 1e30, t_pt = 1e30;
-  if (b->march_flux && !b->flux_rows_fixed) {
-    b->tune_rows.clear();
-    int best_rows = b->flux_rows;
-    double t_rows = 1e30;
-    for (int r : {8, 16, 24, 32, 48, 64, 96, 128}) {
-      b->flux_rows = r;
-      TRY(batch_build(b));  // the map of strips and bands for this height
-      const double us = time([&] { enq_flux(b, 0); });
-      HIP_OK(hipGetLastError());
-      b->tune_rows.push_back({r, us});
-      if (us < t_rows) t_rows = us, best_rows = r;
-    }
-    b->flux_rows = best_rows;
-    TRY(batch_build(b));
-  }
-  for (int w : {512, 1024, 2048, 4096, 8192, 1 << 30}) {
-    if (b->march_flux) break;  // the tile flux's width only
-    const int width = std::min(w, b->m_tiles.total);
-    if (!b->tune_nt.empty() && b->tune_nt.back().first == width) continue;
-    double us = 0.0;
-    const int rc = [&]() -> int {
-      us = time([&] {
-        hipLaunchKernelGGL(b->recon64 ? kb_flux32r : kb_flux32, bgrid(b->m_tiles.total, width),
-                           dim3(NT), 0, b->stream, b->flux[0], b->m_tiles);
-      });
-      return WC_OK;
-    }();
-    if (rc) break;
-    HIP_OK(hipGetLastError());
-    b->tune_nt.push_back({width, us});
-    if (us < t_nt) t_nt = us, best_nt = width;
-  }
-  if (b->dispersive) {
-    for (int w : {256, 512, 1024, 2048, 1 << 30}) {
-      const int width = std::min(w, b->m_cg.total);
-      if (!b->tune_pt.empty() && b->tune_pt.back().first == width) continue;
-      double us = 0.0;
-      const int rc = [&]() -> int {
-        us = time([&] {
-          if (b->march && b->lanes == 512)
-            hipLaunchKernelGGL(kbm_cg1_start<512>, bgrid(b->m_cg.total, width), dim3(1024), 0,
-                               b->stream, b->cg[0], b->m_cg);
-          else if (b->march)
-            hipLaunchKernelGGL(kbm_cg1_start<256>, bgrid(b->m_cg.total, width), dim3(1024), 0,
-                               b->stream, b->cg[0], b->m_cg);
-          else if (b->lanes == 512)
-            hipLaunchKernelGGL(kb_cg1_start<512>, bgrid(b->m_cg.total, width), dim3(1024), 0,
-                               b->stream, b->cg[0], b->m_cg);
-          else
-            hipLaunchKernelGGL(kb_cg1_start<256>, bgrid(b->m_cg.total, width), dim3(1024), 0,
-                               b->stream, b->cg[0], b->m_cg);
-        });
-        return WC_OK;
-      }();
-      if (rc) break;
-      HIP_OK(hipGetLastError());
-      b->tune_pt.push_back({width, us});
-      if (us < t_pt) t_pt = us, best_pt = width;
-    }
-  }
-  HIP_OK(hipEventDestroy(e0));
-  HIP_OK(hipEventDestroy(e1));
-  b->grid_nt = best_nt;
-  b->grid_pt = best_pt;
-  b->tuned = true;
-  return WC_OK;
-}
-
-}  // namespace
-
-extern "C" {
-
-// Whether instance s can step in a batch whose settings are those of s0 (at creation, s0 is the
-// first instance; before every run, each instance against its settings at creation, which the
-// batch's launches were built from).
-int batch_compatible(const wc_solver* s, const wc_solver* s0) {
-  if (s->precision != WC_PRECISION_FP32)
-    return bfail("a batch runs the FP32 form (wc_solver_set_precision)");
-  if (s->precond != WC_PRECOND_JACOBI || s->capture || !(s->pcg_kernels == 0 || s->pcg_kernels == 4))
-    return bfail("a batch runs the one-kernel Jacobi PCG, without capture");
-  if (s->device != s0->device) return bfail("a batch's instances share one device");
-  if (s->pcg_lanes != s0->pcg_lanes || s->pcg_march != s0->pcg_march ||
-      s->recon64 != s0->recon64 || (s->flux_kernel == 3) != (s0->flux_kernel == 3))
-    return bfail("a batch's instances share pcg_lanes, pcg_march, flux_reconstruction and "
-                 "the flux layout (flux_kernel 3 or not)");
-  if ((s->order == WC_ORDER_FIRST) != (s0->order == WC_ORDER_FIRST) ||
-      switch_on(s) != switch_on(s0) || s->dispersive != s0->dispersive)
-    return bfail("a batch's instances share the time stepping, the switch and the dispersion");
-  return WC_OK;
-}
-
-int wc_batch_new(wc_solver* const* solvers, uint64_t n, wc_batch** out) {
-  if (!solvers || !out || n == 0) return fail(WC_EINVAL, "null argument");
-  *out = nullptr;
-  wc_solver* s0 = solvers[0];
-  for (uint64_t q = 0; q < n; ++q) {
-    wc_solver* s = solvers[q];
-    if (!s) return bfail("null solver");
-    TRY(batch_compatible(s, s0));
-    for (uint64_t p = 0; p < q; ++p)
-      if (solvers[p] == s) return bfail("an instance appears twice in the batch");
-  }
-  auto b = std::make_unique<wc_batch>();
-  b->device = s0->device;
-  HIP_OK(hipSetDevice(b->device));
-  HIP_OK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-  struct Undo {  // a failed construction leaves nothing behind
-    std::unique_ptr<wc_batch>& b;
-    ~Undo() {
-      if (!b) return;
-      (void)hipStreamSynchronize(b->stream);
-      for (void* p : b->owned) (void)hipFree(p);
-      for (void* p : b->owned_host) (void)hipHostFree(p);
-      (void)hipStreamDestroy(b->stream);
-    }
-  } undo{b};
-  for (uint64_t q = 0; q < n; ++q) {
-    wc_solver* s = solvers[q];
-    HIP_OK(hipStreamSynchronize(s->stream));
-    b->inst.push_back(s);
-    b->active.push_back((int)q);
-    b->maxit = std::max<unsigned long long>(b->maxit, s->disp.max_iterations);
-    // lanes past an instance's blocks must read zero
-    HIP_OK(hipMemsetAsync(s->lanes, 0, 8 * 512 * sizeof(double), b->stream));
-  }
-  b->rk2 = s0->order != WC_ORDER_FIRST;
-  b->lanes = s0->pcg_lanes;
-  b->march = s0->pcg_march != 0;
-  b->recon64 = s0->recon64 != 0;
-  b->march_flux = s0->flux_kernel == 3;
-  b->sw = switch_on(s0);
-  b->dispersive = s0->dispersive;
-  b->pcg_batch = s0->pcg_batch;
-  TRY(balloc(b.get(), &b->d_red, n));
-  TRY(balloc(b.get(), &b->d_done, n));
-  TRY(balloc(b.get(), &b->d_it, n));
-  TRY(bhost(b.get(), &b->h_red, n));
-  TRY(bhost(b.get(), &b->h_done, n));
-  TRY(bhost(b.get(), &b->h_it, n));
-  TRY(batch_build(b.get()));
-  *out = b.release();
-  return WC_OK;
-}
-
-void wc_batch_free(wc_batch* b) {
-  if (!b) return;
-  (void)hipSetDevice(b->device);
-  (void)hipStreamSynchronize(b->stream);
-  drop_graphs(b);
-  for (auto& pe : b->prof_pending) {
-    (void)hipEventDestroy(pe.first);
-    (void)hipEventDestroy(pe.second);
-  }
-  for (hipEvent_t e : b->prof_pool) (void)hipEventDestroy(e);
-  for (void* p : b->owned) (void)hipFree(p);
-  for (void* p : b->owned_host) (void)hipHostFree(p);
-  (void)hipStreamDestroy(b->stream);
-  delete b;
-}
-
-int wc_batch_set_option(wc_batch* b, const char* key, int64_t value) {
-  if (!b || !key) return fail(WC_EINVAL, "null argument");
-  const std::string k = key;
-  if (k == "grid_nt" || k == "grid_pt") {
-    if (value < 1) return fail(WC_EINVAL, "a launch width must be positive");
-    HIP_OK(hipStreamSynchronize(b->stream));
-    drop_graphs(b);  // the widths are part of the captured launches
-    (k == "grid_nt" ? b->grid_nt : b->grid_pt) = (int)std::min<int64_t>(value, 1 << 30);
-    if (k == "grid_nt") b->grid_st = b->grid_nt;
-    b->tuned = true;  // set by hand: no tuning
-    return WC_OK;
-  }
-  if (k == "flux_rows") {  // the marching flux's band height: 0 = tuned at the next run
-    if (value < 0 || value > (1 << 30)) return fail(WC_EINVAL, "flux_rows must be >= 0");
-    HIP_OK(hipStreamSynchronize(b->stream));
-    b->flux_rows_fixed = value > 0;
-    if (value > 0) b->flux_rows = (int)value;
-    b->tune_rows.clear();
-    if (value == 0) b->tuned = false;
-    return batch_build(b);
-  }
-  if (k == "tune") {
-    b->tuned = value == 0;
-    if (value) b->grid_st = 1 << 30;
-    HIP_OK(hipStreamSynchronize(b->stream));
-    drop_graphs(b);
-    return WC_OK;
-  }
-  if (k == "pcg_batch") {
-    if (value < 1) return fail(WC_EINVAL, "pcg_batch must be >= 1");
-    b->pcg_batch = (unsigned long long)value;
-    return WC_OK;
-  }
-  if (k == "graph") {
-    HIP_OK(hipStreamSynchronize(b->stream));
-    drop_graphs(b);
-    b->use_graph = value != 0;
-    return WC_OK;
-  }
-  if (k == "profile") {  // 1: time every PCG iteration launch (counters reset), 0: stop
-    if (value != 0 && value != 1) return fail(WC_EINVAL, "profile must be 0 or 1");
-    TRY(batch_prof_collect(b));
-    b->prof = value == 1;
-    if (b->prof) {
-      b->prof_ms = 0.0;
-      b->prof_n = 0;
-    }
-    return WC_OK;
-  }
-  return fail(WC_EINVAL, "unknown batch option: " + k);
-}
-
-int wc_batch_profile_read(wc_batch* b, double* total_ms, uint64_t* launches) {
-  if (!b || !total_ms || !launches) return fail(WC_EINVAL, "null argument");
-  HIP_OK(hipSetDevice(b->device));
-  TRY(batch_prof_collect(b));
-  *total_ms = b->prof_ms;
-  *launches = b->prof_n;
-  return WC_OK;
-}
-
-int wc_batch_flux_rows(const wc_batch* b, int32_t* rows, int32_t* tried_rows, double* tried_us,
-                       int32_t capacity, int32_t* tried) {
-  if (!b) return fail(WC_EINVAL, "null batch");
-  if (capacity < 0) return fail(WC_EINVAL, "capacity must be >= 0");
-  if (rows) *rows = b->march_flux ? b->flux_rows : 0;
-  const int n = (int)b->tune_rows.size();
-  for (int q = 0; q < std::min(n, (int)capacity); ++q) {
-    if (tried_rows) tried_rows[q] = b->tune_rows[q].first;
-    if (tried_us) tried_us[q] = b->tune_rows[q].second;
-  }
-  if (tried) *tried = n;
-  return WC_OK;
-}
-
-int wc_batch_launch_widths(const wc_batch* b, int32_t* grid_nt, int32_t* grid_pt) {
-  if (!b) return fail(WC_EINVAL, "null batch");
-  if (grid_nt) *grid_nt = b->grid_nt;
-  if (grid_pt) *grid_pt = b->grid_pt;
-  return WC_OK;
-}
-
-int wc_batch_run_steps(wc_batch* b, uint64_t steps, uint64_t* done_steps, double* last_dt) {
-  if (!b) return fail(WC_EINVAL, "null batch");
-  HIP_OK(hipSetDevice(b->device));
-  const size_t all = b->inst.size();
-  if (done_steps)
-    for (size_t q = 0; q < all; ++q) done_steps[q] = 0;
-  // The instances may have been used alone since the last run: their settings must still be
-  // the batch's, and their PCG lanes past the blocks the batch gives them must read zero again
-  // (a solve alone with other lanes or blocks may have written them).
-  for (size_t q = 0; q < all; ++q) {
-    const wc_solver* s = b->inst[q];
-    if (s->pcg_lanes != b->lanes || (s->pcg_march != 0) != b->march ||
-        (s->recon64 != 0) != b->recon64 || (s->flux_kernel == 3) != b->march_flux ||
-        (s->order != WC_ORDER_FIRST) != b->rk2 ||
-        switch_on(s) != b->sw || s->dispersive != b->dispersive)
-      return bfail("an instance's settings changed since the batch was made");
-    TRY(batch_compatible(s, b->inst[0]));
-    HIP_OK(hipStreamSynchronize(s->stream));
-    HIP_OK(hipMemsetAsync(s->lanes, 0, 8 * 512 * sizeof(double), b->stream));
-  }
-  if (!b->tuned && !b->active.empty()) {
-    TRY(batch_tune(b));
-    drop_graphs(b);
-  }
-  for (uint64_t step = 0; step < steps && !b->active.empty(); ++step) {
-    int n = (int)b->active.size();
-    // stable_dt of every instance (solver.rs:434-450), one read
-    HIP_OK(hipMemsetAsync(b->d_red, 0, n * sizeof(unsigned long long), b->stream));
-    hipLaunchKernelGGL(kb_max_rate32, bgrid(b->m_rate.total, b->grid_st), dim3(NT), 0, b->stream,
-                       b->rate, b->m_rate);
-    HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(b->h_red, b->d_red, n * sizeof(unsigned long long),
-                          hipMemcpyDeviceToHost, b->stream));
-    HIP_OK(hipStreamSynchronize(b->stream));
-    std::vector<double> dts(n);
-    std::vector<int> keep;
-    for (int q = 0; q < n; ++q) {
-      wc_solver* s = b->inst[b->active[q]];
-      double rate;
-      memcpy(&rate, &b->h_red[q], sizeof rate);
-      const double dt = std::isnan(rate) ? NAN : rate > 0.0 ? s->cfl / rate : INFINITY;
-      if (last_dt) last_dt[b->active[q]] = dt;
-      if (std::isfinite(dt) && dt > 0.0) {
-        dts[keep.size()] = dt;
-        keep.push_back(b->active[q]);
-      }
-    }
-    if ((int)keep.size() != n) {  // an instance whose step is not finite stops, as wc_run_steps
-      b->active = keep;
-      n = (int)keep.size();
-      if (n == 0) break;
-      TRY(batch_build(b));
-    }
-    // the step parameters of every instance (glibc, as the reference), one upload
-    for (int q = 0; q < n; ++q) {
-      const wc_solver* s = b->inst[b->active[q]];
-      b->h_stage[q] = dts[q];
-      fill_step_params(s, s->time, dts[q], b->h_stage + n + b->off[q]);
-    }
-    HIP_OK(hipMemcpyAsync(b->d_stage, b->h_stage, (n + b->off[n]) * sizeof(double),
-                          hipMemcpyHostToDevice, b->stream));
-    if (b->off[n] > 0)
-      hipLaunchKernelGGL(kb_scatter, dim3(64), dim3(NT), 0, b->stream,
-                         (const double*)(b->d_stage + n), (const long long*)b->d_off,
-                         (double* const*)b->d_dst, n);
-    TRY(batch_step(b));
-    for (int q = 0; q < n; ++q) {
-      wc_solver* s = b->inst[b->active[q]];
-      s->time += dts[q];
-      if (done_steps) done_steps[b->active[q]] += 1;
-    }
-  }
-  HIP_OK(hipStreamSynchronize(b->stream));
   return WC_OK;
 }
 

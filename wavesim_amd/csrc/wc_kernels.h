@@ -1,5 +1,5 @@
 // Kernel interfaces of the FP64 step: argument blocks and declarations (kernels in
-// wc_kernels.hip, launches in wc_api.hip).
+// wc_kernels.hip, launches in the host sources: wc_api.hip, wc_*_host.hip).
 #pragma once
 
 #include "wc_device.h"

@@ -1,4 +1,4 @@
-// gfx950 kernels of wavecore's FP64 step. Launched by wc_api.hip; declared in wc_kernels.h.
+// gfx950 kernels of wavecore's FP64 step. Launched by the host sources (wc_api.hip, wc_*_host.hip); declared in wc_kernels.h.
 //
 // One thread per cell on 64x4-cell tiles (a wavefront per tile row, so every global access of
 // a row is coalesced along x), blocks mapped to tiles XCD by XCD (wc::tile_of_block). The
