@@ -290,7 +290,10 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
   const float2 *r0 = a.r[cur], *q0 = a.q[cur];
   const float2* s0 = a.s[nxt];  // s_{i-1}: written by K_{n-1} into nxt
   const double na = -alpha;
-  __shared__ double s_u[2][S2Y][S2X], s_f[S1Y][S1X];
+  // With two tiles in flight the tiles alternate between two sets of LDS buffers, so a tile's
+  // staging never waits for the previous tile's last reads: two barriers per tile, not three.
+  constexpr int SETS = AHEAD;
+  __shared__ double s_u[SETS][2][S2Y][S2X], s_f[SETS][S1Y][S1X];
   __shared__ double sums[2][BY][3];
   double acc[3] = {0.0, 0.0, 0.0};
   // u_{i+1} = D^-1 (r_i - alpha (q_i + beta s_{i-1})) on the tile + 2, from q_i, s_{i-1}, r_i
@@ -307,8 +310,8 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
     vx = (double)(float)((double)v[2].x + na * sx) * (double)v[3].x;
     vy = (double)(float)((double)v[2].y + na * sy) * (double)v[3].y;
   };
-  __shared__ float2 s_own[4][BY][BX];  // q_i, s_{i-1}, r_i, D^-1 at the tile's cells
-  __shared__ float s_c[H1];              // c on the tile + 1
+  __shared__ float2 s_own[SETS][4][BY][BX];  // q_i, s_{i-1}, r_i, D^-1 at the tile's cells
+  __shared__ float s_c[SETS][H1];              // c on the tile + 1
   // the iteration reads w at its cells when it updates it (every iteration, or only the odd
   // ones when the step is deferred)
   const bool upd_w = !a.defer || (i & 1);
@@ -340,24 +343,25 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
   for (int q = 0; q < AHEAD; ++q)
     if (blk + a.stride * q < nt) fetch_own(blk + a.stride * q, P[q]);
   auto body = [&](Ahead& Q, int tile, int r) {
-    stage(Q.halo, s_u, s_own, make);
+    const int set = r % SETS;
+    stage(Q.halo, s_u[set], s_own[set], make);
 #pragma unroll
     for (int e = 0; e < 2; ++e)
-      if (threadIdx.x + e * PT < H1) s_c[threadIdx.x + e * PT] = Q.c[e];
+      if (threadIdx.x + e * PT < H1) s_c[set][threadIdx.x + e * PT] = Q.c[e];
     const float2 pp = Q.p, ww = Q.w;
     const float hh = Q.h;
     __syncthreads();
     if (tile + AHEAD * a.stride < nt) fetch_own(tile + AHEAD * a.stride, Q);
-    stencil_f_staged(g, s_u, s_c, s_f);
+    stencil_f_staged(g, s_u[set], s_c[set], s_f[set]);
     bool own;
     int k;
     double ux, uy, ax, ay;
-    stencil_own_h(g, hh, tile, s_u, s_f, own, k, ux, uy, ax, ay);
+    stencil_own_h(g, hh, tile, s_u[set], s_f[set], own, k, ux, uy, ax, ay);
     double v[3] = {0.0, 0.0, 0.0};
     if (own) {
       const int ox = threadIdx.x & 63, oy = threadIdx.x >> 6;
-      const float2 qq = s_own[0][oy][ox], ss = s_own[1][oy][ox], rr = s_own[2][oy][ox];
-      const float2 d = s_own[3][oy][ox];
+      const float2 qq = s_own[set][0][oy][ox], ss = s_own[set][1][oy][ox];
+      const float2 rr = s_own[set][2][oy][ox], d = s_own[set][3][oy][ox];
       const double sx = (double)(float)((double)qq.x + beta * (double)ss.x);
       const double sy = (double)(float)((double)qq.y + beta * (double)ss.y);
       const double px = (double)(float)((double)rr.x * (double)d.x + beta * (double)pp.x);
@@ -380,7 +384,12 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
       v[1] = qx * ux + qy * uy;
       v[2] = nrx * nrx + nry * nry;
     }
-    add_rows<3>(v, sums, r, acc);
+    if constexpr (SETS == 2) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) acc[m] = acc[m] + v[m];
+    } else {
+      add_rows<3>(v, sums, r, acc);
+    }
   };
   for (int r = 0; blk + a.stride * r < nt; r += AHEAD) {
 #pragma unroll
