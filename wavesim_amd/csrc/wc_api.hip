@@ -1255,7 +1255,9 @@ int stage32(wc_solver* s, double dt, int stage, int mode) {
             : launch(s, F_FLUX, k_flux_tile32, tiles(s), dim3(NT), f.f, f.lh, f.lhu, f.lhv));
   }
   if (s->dispersive) {
-    TRY(launch(s, F_SGN_SETUP, k_sgn_setup32, tiles(s), dim3(NT), sgn32_args(s, stage)));
+    const dim3 setup_tiles((unsigned)(s->geo.ntx * ((s->geo.ny + SETUP32_ROWS - 1) / SETUP32_ROWS)));
+    TRY(launch(s, F_SGN_SETUP, k_sgn_setup32, setup_tiles, dim3(TX * SETUP32_ROWS),
+               sgn32_args(s, stage)));
     TRY(pcg32(s, stage));
   }
   TRY(launch(s, F_SGN_FINISH, k_stage32, tiles(s), dim3(NT), stage_args32(s, dt, stage), mode));

@@ -170,17 +170,25 @@ __device__ __forceinline__ void solve_end(const Stage32& a, int k, double& x, do
   }
 }
 
-// SGN setup (dispersion.rs:385-490) on FP32 storage; arithmetic in FP64.
+// SGN setup (dispersion.rs:385-490) on FP32 storage; arithmetic in FP64. A block takes a tile
+// of 64 x ROWS cells, a thread per cell, and stages it with three rings: 16 rows alone (1.5
+// staged points per cell), 4 in the batched kernel, whose virtual blocks are the instances'
+// 64x4 tiles. The staging loads six floats per point and is bound by the rate the texture data
+// unit returns them, so the taller tile's fewer points per cell are what counts.
+template <int ROWS>
 __device__ __forceinline__ void sgn_setup32_body(const Sgn32& a, int blk, int nblk) {
+  constexpr int NTH = TX * ROWS;
+  constexpr int SX3 = TX + 6, SY3 = ROWS + 6, SX2 = TX + 4, SY2 = ROWS + 4;
+  constexpr int SX1 = TX + 2, SY1 = ROWS + 2;
   const Geo& g = a.g;
   const int tile = tile_of_block(blk, nblk);
-  const int i0 = tile_i(g, tile), j0 = tile_j(g, tile);
-  __shared__ double s_h[SY3f][SX3f], s_um[SY3f][SX3f], s_vm[SY3f][SX3f];
-  __shared__ double s_div[SY2f][SX2f];
-  __shared__ double s_c[SY1f][SX1f], s_f[SY1f][SX1f];
+  const int i0 = GH + (tile % g.ntx) * TX, j0 = GH + (tile / g.ntx) * ROWS;
+  __shared__ double s_h[SY3][SX3], s_um[SY3][SX3], s_vm[SY3][SX3];
+  __shared__ double s_div[SY2][SX2];
+  __shared__ double s_c[SY1][SX1], s_f[SY1][SX1];
   const double ix = 0.5 / g.dx, iy = 0.5 / g.dy;
-  for (int t = threadIdx.x; t < SX3f * SY3f; t += NT) {
-    const int lx = t % SX3f, ly = t / SX3f;
+  for (int t = threadIdx.x; t < SX3 * SY3; t += NTH) {
+    const int lx = t % SX3, ly = t / SX3;
     const int i = i0 - 3 + lx, j = j0 - 3 + ly;
     double h = 0.0, um = 0.0, vm = 0.0;
     if (i < g.w && j < g.rows) {
@@ -195,14 +203,14 @@ __device__ __forceinline__ void sgn_setup32_body(const Sgn32& a, int blk, int nb
     s_vm[ly][lx] = vm;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < SX2f * SY2f; t += NT) {
-    const int lx = t % SX2f, ly = t / SX2f;
+  for (int t = threadIdx.x; t < SX2 * SY2; t += NTH) {
+    const int lx = t % SX2, ly = t / SX2;
     const int x = lx + 1, y = ly + 1;
     s_div[ly][lx] = (s_um[y][x + 1] - s_um[y][x - 1]) * ix + (s_vm[y + 1][x] - s_vm[y - 1][x]) * iy;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < SX1f * SY1f; t += NT) {
-    const int lx = t % SX1f, ly = t / SX1f;
+  for (int t = threadIdx.x; t < SX1 * SY1; t += NTH) {
+    const int lx = t % SX1, ly = t / SX1;
     const int i = i0 - 1 + lx, j = j0 - 1 + ly;
     const int x3 = lx + 2, y3 = ly + 2, x2 = lx + 1, y2 = ly + 1;
     double coeff = 0.0, f = 0.0;
@@ -219,7 +227,7 @@ __device__ __forceinline__ void sgn_setup32_body(const Sgn32& a, int blk, int nb
       }
       const bool inside = i >= GH && i < g.nx + GH && j >= GH && j < g.ny + GH;
       const bool ring1 = i >= GH - 1 && j >= GH - 1;
-      const bool mine = (lx >= 1 && lx <= TX && ly >= 1 && ly <= TY);
+      const bool mine = (lx >= 1 && lx <= TX && ly >= 1 && ly <= ROWS);
       if ((mine && inside) || (!inside && ring1)) a.c[j * g.w + i] = (float)coeff;
     }
     s_c[ly][lx] = (double)(float)coeff;  // the matrix product sees c as stored
@@ -296,8 +304,8 @@ __device__ __forceinline__ void matvec32_tail(const Sgn32& a, int i0, int j0,
   ay = mask * (heff * py - (s_f[y1 + 1][x1] - s_f[y1 - 1][x1]) * iy);
 }
 
-__global__ void __launch_bounds__(NT) k_sgn_setup32(Sgn32 a) {
-  sgn_setup32_body(a, blockIdx.x, gridDim.x);
+__global__ void __launch_bounds__(TX * SETUP32_ROWS) k_sgn_setup32(Sgn32 a) {
+  sgn_setup32_body<SETUP32_ROWS>(a, blockIdx.x, gridDim.x);
 }
 
 // r = b - A w, p = z = r / diag, and b.b, r.z, r.r (dispersion.rs:493-514).
@@ -596,7 +604,7 @@ __global__ void __launch_bounds__(NT) kb_switch32(const BSwitch* __restrict__ ar
 __global__ void __launch_bounds__(NT) kb_sgn_setup32(const Sgn32* __restrict__ args, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
-    sgn_setup32_body(args[q], vb - m.first[q], m.first[q + 1] - m.first[q]);
+    sgn_setup32_body<TY>(args[q], vb - m.first[q], m.first[q + 1] - m.first[q]);
     __syncthreads();  // the next virtual block reuses the LDS
   }
 }
