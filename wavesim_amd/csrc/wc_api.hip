@@ -637,7 +637,8 @@ int stage(wc_solver* s, double dt, int stage, int mode) {
   }
   if (!s->dispersive) return WC_OK;
   if (uses_pcg2(s)) TRY(alloc_pcg2(s));
-  TRY(launch(s, F_SGN_SETUP, k_sgn_setup, tiles(s), dim3(NT), sgn_args(s, stage)));
+  const dim3 setup_tiles((unsigned)(s->geo.ntx * ((s->geo.ny + SETUP_ROWS - 1) / SETUP_ROWS)));
+  TRY(launch(s, F_SGN_SETUP, k_sgn_setup, setup_tiles, dim3(TX * SETUP_ROWS), sgn_args(s, stage)));
   TRY(pcg(s, stage));
   TRY(launch(s, F_SGN_FINISH, k_sgn_finish, tiles(s), dim3(NT), a, (const double*)s->wx,
              (const double*)s->wy, mode, ws_args(s)));

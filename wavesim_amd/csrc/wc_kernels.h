@@ -333,7 +333,9 @@ __global__ void k_cg_matvec32(Sgn32 a, int first, const float* psx, const float*
 __global__ void k_cg_update32(Sgn32 a, const float* pcx, const float* pcy);
 __global__ void k_cg_reduce32(Sgn32 a, int kind);
 __global__ void k_stage32(Stage32 a, int mode);
-// Rows of the tiles of the FP32 form's SGN set-up (k_sgn_setup32, 64 x SETUP32_ROWS threads).
+// Rows of the tiles of the SGN set-up (k_sgn_setup, k_sgn_setup32: 64 x ROWS threads a block;
+// the batched kb_sgn_setup32 keeps the instances' 64x4 tiles).
+constexpr int SETUP_ROWS = 16;
 constexpr int SETUP32_ROWS = 16;
 __global__ void k_division_check(double y, const double* x, int n, double* shared,
                                  double* plain);

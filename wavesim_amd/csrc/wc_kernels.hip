@@ -153,23 +153,27 @@ constexpr int SX3 = TX + 6, SY3 = TY + 6;  // tile + 3
 constexpr int SX2 = TX + 4, SY2 = TY + 4;  // tile + 2
 constexpr int SX1 = TX + 2, SY1 = TY + 2;  // tile + 1
 
-__global__ void __launch_bounds__(NT) k_sgn_setup(SgnArgs a) {
+__global__ void __launch_bounds__(TX * SETUP_ROWS) k_sgn_setup(SgnArgs a) {
+  // a tile of 64 x SETUP_ROWS cells, a thread per cell, staged with three rings
+  constexpr int ROWS = SETUP_ROWS, NTH = TX * ROWS;
+  constexpr int SX3 = TX + 6, SY3 = ROWS + 6, SX2 = TX + 4, SY2 = ROWS + 4;
+  constexpr int SX1 = TX + 2, SY1 = ROWS + 2;
   const Geo& g = a.g;
   const int tile = tile_of_block(blockIdx.x, gridDim.x);
   const int tx = tile % g.ntx, ty = tile / g.ntx;
-  const int i0 = GH + tx * TX, j0 = GH + ty * TY;  // padded corner of the tile
+  const int i0 = GH + tx * TX, j0 = GH + ty * ROWS;  // padded corner of the tile
   __shared__ double s_h[SY3][SX3], s_um[SY3][SX3], s_vm[SY3][SX3];
   __shared__ double s_div[SY2][SX2];
   __shared__ double s_c[SY1][SX1], s_f[SY1][SX1];
   const double ix = 0.5 / g.dx, iy = 0.5 / g.dy;
   // Every load of the kernel is issued before the first is used: the state on the tile + 3
-  // (at most three points a thread), phi on the tile + 1 (at most two), the cell's rates.
-  constexpr int N3 = SX3 * SY3, R3 = (N3 + NT - 1) / NT;
-  constexpr int N1 = SX1 * SY1, R1 = (N1 + NT - 1) / NT;
+  // (at most two points a thread), phi on the tile + 1 (at most two), the cell's rates.
+  constexpr int N3 = SX3 * SY3, R3 = (N3 + NTH - 1) / NTH;
+  constexpr int N1 = SX1 * SY1, R1 = (N1 + NTH - 1) / NTH;
   double st[R3][3];
 #pragma unroll
   for (int q = 0; q < R3; ++q) {
-    const int t = threadIdx.x + q * NT;
+    const int t = threadIdx.x + q * NTH;
     const int i = i0 - 3 + t % SX3, j = j0 - 3 + t / SX3;
     st[q][0] = st[q][1] = st[q][2] = 0.0;
     if (t < N3 && i < g.w && j < g.rows) {
@@ -182,7 +186,7 @@ __global__ void __launch_bounds__(NT) k_sgn_setup(SgnArgs a) {
   double ph[R1];
 #pragma unroll
   for (int q = 0; q < R1; ++q) {
-    const int t = threadIdx.x + q * NT;
+    const int t = threadIdx.x + q * NTH;
     const int i = i0 - 1 + t % SX1, j = j0 - 1 + t / SX1;
     ph[q] = 0.0;
     if (t < N1 && i <= g.nx + GH && j <= g.ny + GH) ph[q] = a.phi[j * g.w + i];
@@ -199,7 +203,7 @@ __global__ void __launch_bounds__(NT) k_sgn_setup(SgnArgs a) {
   }
 #pragma unroll
   for (int q = 0; q < R3; ++q) {
-    const int t = threadIdx.x + q * NT;
+    const int t = threadIdx.x + q * NTH;
     if (t < N3) {
       const double h = st[q][0];  // zero outside the array: then um = vm = 0
       const double m = h >= WC_H_DISPERSION_MIN ? 1.0 : 0.0;
@@ -209,7 +213,7 @@ __global__ void __launch_bounds__(NT) k_sgn_setup(SgnArgs a) {
     }
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < SX2 * SY2; t += NT) {
+  for (int t = threadIdx.x; t < SX2 * SY2; t += NTH) {
     const int lx = t % SX2, ly = t / SX2;
     const int x = lx + 1, y = ly + 1;  // in the +3 frame
     s_div[ly][lx] = (s_um[y][x + 1] - s_um[y][x - 1]) * ix + (s_vm[y + 1][x] - s_vm[y - 1][x]) * iy;
@@ -217,7 +221,7 @@ __global__ void __launch_bounds__(NT) k_sgn_setup(SgnArgs a) {
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < R1; ++q) {
-    const int t = threadIdx.x + q * NT;
+    const int t = threadIdx.x + q * NTH;
     if (t >= N1) break;
     const int lx = t % SX1, ly = t / SX1;
     const int i = i0 - 1 + lx, j = j0 - 1 + ly;
@@ -236,7 +240,7 @@ __global__ void __launch_bounds__(NT) k_sgn_setup(SgnArgs a) {
       // c is read by the matrix product on the interior and its first ring.
       const bool inside = i >= GH && i < g.nx + GH && j >= GH && j < g.ny + GH;
       const bool ring1 = i >= GH - 1 && j >= GH - 1;
-      const bool mine = (lx >= 1 && lx <= TX && ly >= 1 && ly <= TY);
+      const bool mine = (lx >= 1 && lx <= TX && ly >= 1 && ly <= ROWS);
       if ((mine && inside) || (!inside && ring1)) a.c[j * g.w + i] = coeff;
     }
     s_c[ly][lx] = coeff;
