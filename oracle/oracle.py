@@ -118,6 +118,7 @@ def lib():
             "wco_face_values": (None, [_P, _D, _P, _P]),
             "wco_smoothstep": (_D, [_D, _D, _D]),
             "wco_hypot": (_D, [_D, _D, _I32]),
+            "wco_switch_off": (_I32, [_P, _P, _P, _P, _U64, _P, _P]),
             "wco_pow43": (_D, [_D, _I32]),
             "wco_matvec": (_I32, [_U64, _U64, _D, _D, _I32, _P, _P, _P, _P, _P, _P, _P]),
             "wco_dot": (_D, [_U64, _U64, _I32, _P, _P, _P, _P]),
@@ -429,6 +430,17 @@ def hypot(a: float, b: float, portable: bool) -> float:
 
 def pow43(h: float, portable: bool) -> float:
     return lib().wco_pow43(h, int(portable))
+
+
+def switch_off(thresholds, ratio, ex, ey):
+    """The GPU switch's shortcut (hypot skipped where its bits cannot matter) and the full
+    expression, cell by cell: (shortcut, full)."""
+    th = np.ascontiguousarray(thresholds, dtype=np.float64)
+    ratio, ex, ey = (np.ascontiguousarray(a, dtype=np.float64) for a in (ratio, ex, ey))
+    fast, full = np.empty_like(ratio), np.empty_like(ratio)
+    lib().wco_switch_off(ptr(th), ptr(ratio), ptr(ex), ptr(ey), ratio.size, ptr(fast),
+                         ptr(full))
+    return fast, full
 
 
 def matvec(grid: Grid, periodic: bool, heff, mask, c, px, py):

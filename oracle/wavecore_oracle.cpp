@@ -1747,6 +1747,20 @@ double wco_hypot(double a, double b, int32_t portable) {
 
 double wco_pow43(double h, int32_t portable) { return portable ? wc_pow43(h) : pow(h, 4.0 / 3.0); }
 
+// The GPU switch's shortcut (wc_math.h: wc_switch_off, which skips the portable hypot where its
+// bits cannot matter) and the full expression, over n cells, for the tests that pin the
+// shortcut against the expression it replaces.
+int wco_switch_off(const double* thresholds, const double* ratio, const double* ex,
+                   const double* ey, uint64_t n, double* shortcut, double* full) {
+  const double er_lo = thresholds[0], er_hi = thresholds[1], sl_lo = thresholds[2],
+               sl_hi = thresholds[3];
+  for (uint64_t k = 0; k < n; ++k) {
+    shortcut[k] = wc_switch_off(er_lo, er_hi, sl_lo, sl_hi, ratio[k], ex[k], ey[k]);
+    full[k] = wc_switch_off_full(er_lo, er_hi, sl_lo, sl_hi, ratio[k], ex[k], ey[k]);
+  }
+  return 0;
+}
+
 int wco_matvec(uint64_t nx, uint64_t ny, double dx, double dy, int32_t periodic,
                const double* heff, const double* mask, const double* c, const double* px,
                const double* py, double* ax, double* ay) {

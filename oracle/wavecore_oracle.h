@@ -156,6 +156,10 @@ void wco_face_values(const double v[5], double blend, double* lo, double* hi);
 double wco_smoothstep(double lo, double hi, double x);
 double wco_hypot(double a, double b, int32_t portable);
 double wco_pow43(double h, int32_t portable);
+/* The GPU switch's shortcut and the full expression it replaces (er_lo, er_hi, sl_lo, sl_hi in
+ * thresholds), cell by cell. */
+int wco_switch_off(const double* thresholds, const double* ratio, const double* ex,
+                   const double* ey, uint64_t n, double* shortcut, double* full);
 /* A p for the SGN operator on a grid, with the given heff, mask and c (padded arrays);
  * p is copied, its ghosts filled, and the result written to ax, ay (interior). */
 int wco_matvec(uint64_t nx, uint64_t ny, double dx, double dy, int32_t periodic,

@@ -371,3 +371,34 @@ def test_canonical_mode_passes_the_same_checks():
     ka.check_lake_at_rest_with_dry_bumps(make_canonical, orders=(Order.Fifth,))
     ka.check_still_water_with_dispersion(make_canonical)
     ka.check_walls_and_periodic_seam_keep_volume(make_canonical, cases=[(True, True)])
+
+
+def test_switch_shortcut_has_the_full_expressions_bits():
+    """The GPU switch skips the portable hypot where the steepness smoothstep is saturated
+    whatever hypot's last bits (wc_math.h: wc_switch_off): against the full expression for the
+    reference's default thresholds and others, on slopes spread over many magnitudes, slopes
+    right at the band's edges (the shortcut's margins), depth ratios on and beside the erosion
+    band, zeros, infinities and NaN; the same bits everywhere, NaN where the full one is NaN."""
+    rng = np.random.default_rng(11)
+    n = 400_000
+    for th in ([0.3, 0.5, 0.1, 0.3], [0.0, 1e-9, 0.6, 0.60000001], [0.2, 0.4, 1e-12, 2e-12],
+               [0.5, 0.3, 0.3, 0.1]):
+        ratio = np.concatenate([rng.uniform(-0.5, 1.5, n), np.full(64, th[1]),
+                                np.full(64, th[0]), np.nextafter(np.full(64, th[1]), 2.0)])
+        m = ratio.size
+        mag = 10.0 ** rng.uniform(-14, 2, m)
+        ang = rng.uniform(0.0, 2 * np.pi, m)
+        ex, ey = mag * np.cos(ang), mag * np.sin(ang)
+        # slopes at the band's edges and at the shortcut's margins
+        edges = np.array([th[2], th[3], th[2] / 1.5, th[3] * (1 + 2.0 ** -40)])
+        k = rng.integers(0, m, 4096)
+        ex[k] = rng.choice(edges, k.size) * rng.choice([-1.0, 1.0], k.size)
+        ey[k] = rng.choice([0.0, 1e-300, 1e-17, 1e-9], k.size)
+        j = rng.integers(0, m, 4096)
+        ex[j] = np.nextafter(ex[j], rng.choice([-np.inf, np.inf], j.size))
+        special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1e300, 2.0 ** 1000, 2.0 ** 1001])
+        ex[:special.size], ey[:special.size] = special, special[::-1]
+        fast, full = orc.switch_off(th, ratio, ex, ey)
+        same = (fast.view(np.uint64) == full.view(np.uint64)) | (np.isnan(fast) & np.isnan(full))
+        assert same.all(), (th, int((~same).sum()))
+        assert (fast == 1.0).any() and (fast == 0.0).any()  # both shortcuts taken

@@ -16,25 +16,11 @@ struct Breaking {
   const double* memory;
 };
 
-// The switch's "off" (dispersion.rs:140-150): the larger of the erosion smoothstep of the
-// depth ratio and the steepness smoothstep of hypot(ex, ey). The portable hypot is the costly
-// part of the switch, and most cells do not need its last bits: where the larger slope component
-// alone already puts hypot past the steepness band (hypot >= max(|ex|, |ey|) and the bits of a
-// correctly rounded hypot stay within 2^-50 of it), the smoothstep is exactly 1; where 1.5 times
-// it stays below the band (hypot <= sqrt 2 max(|ex|, |ey|)), exactly 0; and where the erosion
-// smoothstep is 1 the larger of the two is 1 whatever the other. Those cells skip hypot; the
-// result has the bits of the full expression.
+// The switch's "off" for the thresholds br (wc_math.h: wc_switch_off, the portable hypot
+// skipped where its last bits cannot matter).
 __device__ __forceinline__ double switch_off(const Breaking& br, double ratio, double ex,
                                              double ey) {
-  const double er = wc_smoothstep(br.er_lo, br.er_hi, ratio);
-  const double ax = fabs(ex), ay = fabs(ey);
-  const double mx = ax > ay ? ax : ay;
-  if (br.sl_lo < br.sl_hi && mx <= 0x1p1000) {  // finite slopes (NaN fails the test)
-    if (er == 1.0) return er;
-    if (mx >= br.sl_hi * (1.0 + 0x1p-40)) return wc_max(er, 1.0);
-    if (1.5 * mx <= br.sl_lo) return wc_max(er, 0.0);
-  }
-  return wc_max(er, wc_smoothstep(br.sl_lo, br.sl_hi, wc_hypot(ex, ey)));
+  return wc_switch_off(br.er_lo, br.er_hi, br.sl_lo, br.sl_hi, ratio, ex, ey);
 }
 
 // breaking_into's indicator at interior cell (i, j), padded index k (dispersion.rs:183-201):

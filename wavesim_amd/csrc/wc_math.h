@@ -59,6 +59,31 @@ WC_HD static inline double wc_smoothstep(double lo, double hi, double x) {
   const double t = wc_clamp((x - lo) / (hi - lo), 0.0, 1.0);
   return t * t * (3.0 - 2.0 * t);
 }
+// The breaking switch's "off" (dispersion.rs:140-150): the larger of the erosion smoothstep of
+// the depth ratio and the steepness smoothstep of hypot(ex, ey), hypot being the portable one
+// below. hypot is the costly part, and most cells do not need its last bits: where the larger
+// slope component alone already puts hypot past the steepness band (hypot >= max(|ex|, |ey|),
+// and a correctly rounded hypot stays within 2^-50 of it), the smoothstep is exactly 1; where
+// 1.5 times it stays below the band (hypot <= sqrt 2 max(|ex|, |ey|)), exactly 0; and where the
+// erosion smoothstep is 1 the larger of the two is 1 whatever the other. Those cells skip hypot;
+// the result has the bits of the full expression (wc_switch_off_full).
+WC_HD static inline double wc_hypot(double a, double b);
+WC_HD static inline double wc_switch_off_full(double er_lo, double er_hi, double sl_lo,
+                                              double sl_hi, double ratio, double ex, double ey) {
+  return wc_max(wc_smoothstep(er_lo, er_hi, ratio), wc_smoothstep(sl_lo, sl_hi, wc_hypot(ex, ey)));
+}
+WC_HD static inline double wc_switch_off(double er_lo, double er_hi, double sl_lo, double sl_hi,
+                                         double ratio, double ex, double ey) {
+  const double er = wc_smoothstep(er_lo, er_hi, ratio);
+  const double ax = fabs(ex), ay = fabs(ey);
+  const double mx = ax > ay ? ax : ay;
+  if (sl_lo < sl_hi && mx <= 0x1p1000) {  // finite slopes (NaN fails the test)
+    if (er == 1.0) return er;
+    if (mx >= sl_hi * (1.0 + 0x1p-40)) return wc_max(er, 1.0);
+    if (1.5 * mx <= sl_lo) return wc_max(er, 0.0);
+  }
+  return wc_max(er, wc_smoothstep(sl_lo, sl_hi, wc_hypot(ex, ey)));
+}
 // solver.rs:90-92: velocity of a cell, 0 where it is dry.
 WC_HD static inline double wc_velocity(double h, double hm) { return h > WC_H_DRY ? hm / h : 0.0; }
 
