@@ -437,7 +437,8 @@ def sweep(args, spinup, warmup, steps, cpu_seconds):
                                                                 if args.sweep_size else ""),
                    "instances": len(ws), "cells": cells, "sizes": sizes, "spinup_steps": spinup,
                    "path": "FP32 form, one launch per phase over all instances",
-                   "pcg_tiles_per_block": walk, "launch_widths": list(b.launch_widths())},
+                   "pcg_tiles_per_block": walk, "launch_widths": list(b.launch_widths()),
+                   "flux_band_rows": b.flux_rows()[0], "setup_tile_rows": b.setup_rows()[0]},
         "pcg_iters_per_solve": iters / max(solves, 1),
         "roofline": {"bound": "hbm", "kernel": "kb_cg1_iter", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

@@ -349,7 +349,8 @@ int wc_sampler_read(const wc_sampler* sm, int32_t slot, double* out, double* tim
  * flux_kernel 3 (the default) the flux marches one wave per 58-column strip and band of rows.
  * The first run tunes, by timing candidates, the marching flux's band height (the tile shape of
  * its waves, 58 x rows cells; wc_batch_flux_rows reports it and the timings,
- * wc_batch_set_option("flux_rows", r) fixes it, 0 tunes it again) and the launch widths
+ * wc_batch_set_option("flux_rows", r) fixes it, 0 tunes it again), the SGN set-up's tile height
+ * (wc_batch_setup_rows, option "setup_rows") and the launch widths
  * (persistent grids of the 64x4 tile flux and the PCG launches; the other launches run a block
  * per virtual block); wc_batch_launch_widths reports them, wc_batch_set_option("grid_nt" |
  * "grid_pt", w) fixes them ("grid_nt" then for every 256-thread launch), ("tune", 1) tunes
@@ -364,6 +365,11 @@ int wc_batch_launch_widths(const wc_batch* b, int32_t* grid_nt, int32_t* grid_pt
  * microseconds per launch) measurements, as wc_solver_flux_rows. */
 int wc_batch_flux_rows(const wc_batch* b, int32_t* rows, int32_t* tried_rows, double* tried_us,
                        int32_t capacity, int32_t* tried);
+/* The SGN set-up's tile height in use (64 x rows cells a block: 4, 8 or 16, tuned at the first
+ * run unless wc_batch_set_option("setup_rows", r) fixed it; 0 tunes it again) and the tuner's
+ * (height, microseconds per launch) measurements, as wc_batch_flux_rows. */
+int wc_batch_setup_rows(const wc_batch* b, int32_t* rows, int32_t* tried_rows, double* tried_us,
+                        int32_t capacity, int32_t* tried);
 int wc_batch_run_steps(wc_batch* b, uint64_t steps, uint64_t* done_steps, double* last_dt);
 /* Time spent in the batch's PCG iteration launches since wc_batch_set_option("profile", 1),
  * each launch between its own pair of events (while profiling, the stretches of a step are

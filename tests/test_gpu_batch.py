@@ -114,6 +114,34 @@ def test_the_launch_width_changes_nothing(widths):
     assert_same(alone, batched)
 
 
+@pytest.mark.parametrize("rows", [0, 4, 8, 16])
+def test_the_setup_tile_height_changes_nothing(rows):
+    """The batched SGN set-up's tile height (64 x rows cells a block): the tuner's choice (the
+    fastest of 4, 8 and 16, timed at the first run) and each height fixed step every instance
+    to the bits it reaches alone; other heights are refused."""
+    ws = workloads()[:3]
+    starts = spun_up(ws, 40)
+    alone = run_alone(ws, starts, 6)
+    pairs = [fresh(w, arr, t0) for w, (arr, t0) in zip(ws, starts)]
+    b = Batch([g for g, _ in pairs])
+    if rows:
+        b.set_option("setup_rows", rows)
+    with pytest.raises(ValueError):
+        b.set_option("setup_rows", 12)
+    done, _ = b.run_steps([s for _, s in pairs], 6)
+    assert done == [6] * len(ws)
+    used, tried = b.setup_rows()
+    print(f"setup rows {rows}: in use {used}, timed {tried}", flush=True)
+    if rows == 0:
+        assert [r for r, _ in tried] == [4, 8, 16]
+        assert used == min(tried, key=lambda t: t[1])[0]
+    else:
+        assert used == rows and tried == []
+    batched = [(d, s.time, [a.copy() for a in s.arrays()[:3]], g.dispersion_stats())
+               for d, (g, s) in zip(done, pairs)]
+    assert_same(alone, batched)
+
+
 @pytest.mark.parametrize("rows", [0, 8, 13, 100000, "tiles"])
 def test_the_flux_band_height_changes_nothing(rows):
     """The batched marching flux's band height (each wave's tile is 58 x rows cells): the

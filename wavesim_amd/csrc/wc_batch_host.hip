@@ -23,6 +23,9 @@ struct wc_batch {
   int flux_rows = 32;      // the marching flux's band height (tuned unless fixed by hand)
   bool flux_rows_fixed = false;
   std::vector<std::pair<int, double>> tune_rows;  // (band height, us per launch) as measured
+  int setup_rows = SETUP32_ROWS;  // the SGN set-up's tile height, 4, 8 or 16 (tuned unless fixed)
+  bool setup_rows_fixed = false;
+  std::vector<std::pair<int, double>> tune_setup;  // (tile height, us per launch) as measured
   unsigned long long maxit = 0, pcg_batch = 4;
   int grid_nt = 2048, grid_pt = 512;  // persistent widths of the flux and the PCG launches
   int grid_st = 1 << 30;  // and of the other 256-thread launches: a block per virtual block
@@ -40,7 +43,7 @@ struct wc_batch {
   CgG* cg[2] = {};
   Stage32* stg[2] = {};
   BRate* rate = nullptr;
-  BMap m_fill{}, m_sw{}, m_tiles{}, m_rate{}, m_cg{}, m_march{};
+  BMap m_fill{}, m_sw{}, m_tiles{}, m_setup{}, m_rate{}, m_cg{}, m_march{};
   long long* d_off = nullptr;
   double** d_dst = nullptr;
   double* d_stage = nullptr;  // [dt of each active instance | their step parameters]
@@ -125,8 +128,8 @@ int batch_build(wc_batch* b) {
     return (long long)gx.nx * gx.ny > (long long)gy.nx * gy.ny;
   });
   const int n = (int)b->active.size();
-  std::vector<int> f_fill(n + 1, 0), f_sw(n + 1, 0), f_tiles(n + 1, 0), f_rate(n + 1, 0),
-      f_cg(n + 1, 0), f_march(n + 1, 0);
+  std::vector<int> f_fill(n + 1, 0), f_sw(n + 1, 0), f_tiles(n + 1, 0), f_setup(n + 1, 0),
+      f_rate(n + 1, 0), f_cg(n + 1, 0), f_march(n + 1, 0);
   auto rows_of = [&](const Geo& g) { return std::max(1, std::min(b->flux_rows, g.ny)); };
   b->off.assign(n + 1, 0);
   for (int q = 0; q < n; ++q) {
@@ -135,6 +138,7 @@ int batch_build(wc_batch* b) {
     f_fill[q + 1] = f_fill[q] + (int)((6LL * g.w + 6LL * g.ny + NT - 1) / NT);
     f_sw[q + 1] = f_sw[q] + (int)(((long long)g.w * g.rows + NT - 1) / NT);
     f_tiles[q + 1] = f_tiles[q] + g.ntx * g.nty;
+    f_setup[q + 1] = f_setup[q] + g.ntx * ((g.ny + b->setup_rows - 1) / b->setup_rows);
     f_rate[q + 1] = f_rate[q] + (int)rate_blocks((long long)g.nx * g.ny).x;
     f_cg[q + 1] = f_cg[q] + pcg_blocks(s);
     f_march[q + 1] = f_march[q] + ((g.nx + MARCH_MW - 1) / MARCH_MW) *
@@ -201,6 +205,7 @@ int batch_build(wc_batch* b) {
   const size_t o_ffill = put(f_fill.data(), (n + 1) * sizeof(int));
   const size_t o_fsw = put(f_sw.data(), (n + 1) * sizeof(int));
   const size_t o_ftiles = put(f_tiles.data(), (n + 1) * sizeof(int));
+  const size_t o_fsetup = put(f_setup.data(), (n + 1) * sizeof(int));
   const size_t o_frate = put(f_rate.data(), (n + 1) * sizeof(int));
   const size_t o_fcg = put(f_cg.data(), (n + 1) * sizeof(int));
   const size_t o_fmarch = put(f_march.data(), (n + 1) * sizeof(int));
@@ -231,6 +236,7 @@ int batch_build(wc_batch* b) {
   b->m_fill = BMap{reinterpret_cast<const int*>(d + o_ffill), n, f_fill[n]};
   b->m_sw = BMap{reinterpret_cast<const int*>(d + o_fsw), n, f_sw[n]};
   b->m_tiles = BMap{reinterpret_cast<const int*>(d + o_ftiles), n, f_tiles[n]};
+  b->m_setup = BMap{reinterpret_cast<const int*>(d + o_fsetup), n, f_setup[n]};
   b->m_rate = BMap{reinterpret_cast<const int*>(d + o_frate), n, f_rate[n]};
   b->m_cg = BMap{reinterpret_cast<const int*>(d + o_fcg), n, f_cg[n]};
   b->m_march = BMap{reinterpret_cast<const int*>(d + o_fmarch), n, f_march[n]};
@@ -272,14 +278,26 @@ void enq_flux(wc_batch* b, int st) {
                        dim3(NT), 0, b->stream, b->flux[st], b->m_tiles);
 }
 
+// The SGN set-up of every instance on its 64 x setup_rows tiles, a block per tile.
+void enq_setup(wc_batch* b, int st) {
+  const dim3 grid = bgrid(b->m_setup.total, 1 << 30);
+  if (b->setup_rows == 4)
+    hipLaunchKernelGGL(kb_sgn_setup32<4>, grid, dim3(TX * 4), 0, b->stream, b->sgn[st],
+                       b->m_setup);
+  else if (b->setup_rows == 8)
+    hipLaunchKernelGGL(kb_sgn_setup32<8>, grid, dim3(TX * 8), 0, b->stream, b->sgn[st],
+                       b->m_setup);
+  else
+    hipLaunchKernelGGL(kb_sgn_setup32<16>, grid, dim3(TX * 16), 0, b->stream, b->sgn[st],
+                       b->m_setup);
+}
+
 void enq_stage_head(wc_batch* b, int st) {  // the switch, the rates, the SGN set-up
   if (b->sw)
     hipLaunchKernelGGL(kb_switch32, bgrid(b->m_sw.total, b->grid_st), dim3(NT), 0, b->stream,
                        b->swa[st], b->m_sw);
   enq_flux(b, st);
-  if (b->dispersive)
-    hipLaunchKernelGGL(kb_sgn_setup32, bgrid(b->m_tiles.total, b->grid_st), dim3(NT), 0,
-                       b->stream, b->sgn[st], b->m_tiles);
+  if (b->dispersive) enq_setup(b, st);
 }
 
 // A PCG iteration launch, timed between two events when the batch profiles.
@@ -502,8 +520,10 @@ int batch_step(wc_batch* b) {
 // The autotuner, each choice the fastest of a few candidates timed on launches that change no
 // state: the marching flux's band height, the tile shape of its waves (58 x rows cells; timed
 // on the flux launch, which only writes rates), or with the 64x4 tile flux the persistent width
-// of its launch; and the persistent width of the 1024-thread PCG launches (timed on
-// kb_cg1_start, whose outputs the next solve's start rewrites). No choice changes a bit.
+// of its launch; the SGN set-up's tile height (64 x 4, 8 or 16 cells; timed on the set-up,
+// whose coefficients and right-hand side are a function of the state it reads); and the
+// persistent width of the 1024-thread PCG launches (timed on kb_cg1_start, whose outputs the
+// next solve's start rewrites). No choice changes a bit.
 int batch_tune(wc_batch* b) {
   hipEvent_t e0, e1;
   HIP_OK(hipEventCreate(&e0));
@@ -535,6 +555,21 @@ int batch_tune(wc_batch* b) {
       if (us < t_rows) t_rows = us, best_rows = r;
     }
     b->flux_rows = best_rows;
+    TRY(batch_build(b));
+  }
+  if (b->dispersive && !b->setup_rows_fixed) {
+    b->tune_setup.clear();
+    int best = b->setup_rows;
+    double t_best = 1e30;
+    for (int r : {4, 8, 16}) {
+      b->setup_rows = r;
+      TRY(batch_build(b));  // the map of the instances' tiles for this height
+      const double us = time([&] { enq_setup(b, 0); });
+      HIP_OK(hipGetLastError());
+      b->tune_setup.push_back({r, us});
+      if (us < t_best) t_best = us, best = r;
+    }
+    b->setup_rows = best;
     TRY(batch_build(b));
   }
   for (int w : {512, 1024, 2048, 4096, 8192, 1 << 30}) {
@@ -703,6 +738,16 @@ int wc_batch_set_option(wc_batch* b, const char* key, int64_t value) {
     if (value == 0) b->tuned = false;
     return batch_build(b);
   }
+  if (k == "setup_rows") {  // the SGN set-up's tile height, 4, 8 or 16: 0 = tuned at the next run
+    if (value != 0 && value != 4 && value != 8 && value != 16)
+      return fail(WC_EINVAL, "setup_rows must be 0 (tuned), 4, 8 or 16");
+    HIP_OK(hipStreamSynchronize(b->stream));
+    b->setup_rows_fixed = value > 0;
+    if (value > 0) b->setup_rows = (int)value;
+    b->tune_setup.clear();
+    if (value == 0) b->tuned = false;
+    return batch_build(b);
+  }
   if (k == "tune") {
     b->tuned = value == 0;
     if (value) b->grid_st = 1 << 30;
@@ -740,6 +785,20 @@ int wc_batch_profile_read(wc_batch* b, double* total_ms, uint64_t* launches) {
   TRY(batch_prof_collect(b));
   *total_ms = b->prof_ms;
   *launches = b->prof_n;
+  return WC_OK;
+}
+
+int wc_batch_setup_rows(const wc_batch* b, int32_t* rows, int32_t* tried_rows, double* tried_us,
+                        int32_t capacity, int32_t* tried) {
+  if (!b) return fail(WC_EINVAL, "null batch");
+  if (capacity < 0) return fail(WC_EINVAL, "capacity must be >= 0");
+  if (rows) *rows = b->setup_rows;
+  const int n = (int)b->tune_setup.size();
+  for (int q = 0; q < std::min(n, (int)capacity); ++q) {
+    if (tried_rows) tried_rows[q] = b->tune_setup[q].first;
+    if (tried_us) tried_us[q] = b->tune_setup[q].second;
+  }
+  if (tried) *tried = n;
   return WC_OK;
 }
 

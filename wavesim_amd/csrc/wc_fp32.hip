@@ -171,10 +171,10 @@ __device__ __forceinline__ void solve_end(const Stage32& a, int k, double& x, do
 }
 
 // SGN setup (dispersion.rs:385-490) on FP32 storage; arithmetic in FP64. A block takes a tile
-// of 64 x ROWS cells, a thread per cell, and stages it with three rings: 16 rows alone (1.5
-// staged points per cell), 4 in the batched kernel, whose virtual blocks are the instances'
-// 64x4 tiles. The staging loads six floats per point and is bound by the rate the texture data
-// unit returns them, so the taller tile's fewer points per cell are what counts.
+// of 64 x ROWS cells, a thread per cell, and stages it with three rings (16 rows: 1.5 staged
+// points per cell, against 2.7 on 64x4 tiles). The staging loads six floats per point and is
+// bound by the rate the texture data unit returns them, so the taller tile's fewer points per
+// cell are what counts.
 template <int ROWS>
 __device__ __forceinline__ void sgn_setup32_body(const Sgn32& a, int blk, int nblk) {
   constexpr int NTH = TX * ROWS;
@@ -601,13 +601,20 @@ __global__ void __launch_bounds__(NT) kb_switch32(const BSwitch* __restrict__ ar
   }
 }
 
-__global__ void __launch_bounds__(NT) kb_sgn_setup32(const Sgn32* __restrict__ args, BMap m) {
+// Virtual blocks: each instance's 64 x ROWS tiles, a block of 64 ROWS threads each (the batch
+// tunes ROWS among 4, 8 and 16).
+template <int ROWS>
+__global__ void __launch_bounds__(TX * ROWS) kb_sgn_setup32(const Sgn32* __restrict__ args,
+                                                            BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
-    sgn_setup32_body<TY>(args[q], vb - m.first[q], m.first[q + 1] - m.first[q]);
+    sgn_setup32_body<ROWS>(args[q], vb - m.first[q], m.first[q + 1] - m.first[q]);
     __syncthreads();  // the next virtual block reuses the LDS
   }
 }
+template __global__ void kb_sgn_setup32<4>(const Sgn32*, BMap);
+template __global__ void kb_sgn_setup32<8>(const Sgn32*, BMap);
+template __global__ void kb_sgn_setup32<16>(const Sgn32*, BMap);
 
 __global__ void __launch_bounds__(NT) kb_stage32(const Stage32* __restrict__ args, const double* dt, int mode,
                                                  BMap m) {

@@ -320,7 +320,7 @@ __global__ void k_cg_update32(Sgn32 a, const float* pcx, const float* pcy);
 __global__ void k_cg_reduce32(Sgn32 a, int kind);
 __global__ void k_stage32(Stage32 a, int mode);
 // Rows of the tiles of the SGN set-up (k_sgn_setup, k_sgn_setup32: 64 x ROWS threads a block;
-// the batched kb_sgn_setup32 keeps the instances' 64x4 tiles).
+// the batch's kb_sgn_setup32 takes 4, 8 or 16 rows, tuned or set).
 constexpr int SETUP_ROWS = 16;
 constexpr int SETUP32_ROWS = 16;
 __global__ void k_division_check(double y, const double* x, int n, double* shared,
@@ -470,6 +470,7 @@ __global__ void kb_flux32(const BFlux* args, BMap m);
 __global__ void kb_flux_march32(const BFlux* args, BMap m);
 __global__ void kb_flux_march32r(const BFlux* args, BMap m);
 __global__ void kb_flux32r(const BFlux* args, BMap m);  // reconstruction in FP64
+template <int ROWS>
 __global__ void kb_sgn_setup32(const Sgn32* args, BMap m);
 __global__ void kb_stage32(const Stage32* args, const double* dt, int mode, BMap m);
 __global__ void kb_max_rate32(const BRate* args, BMap m);

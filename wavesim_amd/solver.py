@@ -122,6 +122,8 @@ SIGNATURES = {
     "wc_batch_launch_widths": (_I32, [_VP, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "wc_batch_flux_rows": (_I32, [_VP, ctypes.POINTER(_I32), ctypes.POINTER(_I32), _P, _I32,
                                   ctypes.POINTER(_I32)]),
+    "wc_batch_setup_rows": (_I32, [_VP, ctypes.POINTER(_I32), ctypes.POINTER(_I32), _P, _I32,
+                                   ctypes.POINTER(_I32)]),
     "wc_batch_profile_read": (_I32, [_VP, ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(_U64)]),
     "wc_batch_run_steps": (_I32, [_VP, _U64, ctypes.POINTER(_U64), _P]),
@@ -622,6 +624,17 @@ class Batch:
         tu = (ctypes.c_double * cap)()
         _check(lib().wc_batch_flux_rows(self._b, ctypes.byref(rows), tr, tu, cap,
                                         ctypes.byref(n)))
+        return rows.value, [(tr[q], tu[q]) for q in range(min(n.value, cap))]
+
+    def setup_rows(self) -> tuple[int, list[tuple[int, float]]]:
+        """The SGN set-up's tile height (64 x rows cells a block) and the tuner's (height,
+        microseconds per launch) measurements (wc_batch_setup_rows)."""
+        rows, n = _I32(), _I32()
+        cap = 8
+        tr = (_I32 * cap)()
+        tu = (ctypes.c_double * cap)()
+        _check(lib().wc_batch_setup_rows(self._b, ctypes.byref(rows), tr, tu, cap,
+                                         ctypes.byref(n)))
         return rows.value, [(tr[q], tu[q]) for q in range(min(n.value, cap))]
 
     def launch_widths(self) -> tuple[int, int]:
