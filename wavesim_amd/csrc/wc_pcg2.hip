@@ -83,6 +83,16 @@ __device__ __forceinline__ bool start(const CgV<double>& a, unsigned long long n
   return !done;
 }
 
+// Streamed values of a cell that no other block of the launch reads: loaded and stored with the
+// nontemporal hint.
+__device__ __forceinline__ double2 ld_nt(const double2* p) {
+  return make_double2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
+}
+__device__ __forceinline__ void st_nt(double2* p, double2 v) {
+  __builtin_nontemporal_store(v.x, &p->x);
+  __builtin_nontemporal_store(v.y, &p->y);
+}
+
 // Ring element t of a tile + 2 halo: the mapped interior index m and flags (1: inside the
 // grid's two rings; 2, 4: the x, y component flips at a wall).
 __device__ __forceinline__ int ring2(const Geo& g, int tile, int t, int& m) {
@@ -320,7 +330,7 @@ __global__ void __launch_bounds__(PT) k_cg2_matvec(CgV<double> a, unsigned long 
     if (k >= 0) {
       double px, py, ax, ay;
       apply_own(g, h, s_u, s_f, px, py, ax, ay);
-      pn[k] = make_double2(px, py);
+      st_nt(&pn[k], make_double2(px, py));
       v = px * ax + py * ay;
     }
     acc[0] = acc[0] + v;
@@ -356,14 +366,14 @@ __global__ void __launch_bounds__(PT) k_cg2_update(CgV<double> a, unsigned long 
     v[1] = p.y;
   };
   auto own = [&](int k, double (&o)[NO]) {
-    const double2 r = a.r2[k];
+    const double2 r = ld_nt(&a.r2[k]);
     o[0] = a.h[k];
     o[1] = r.x;
     o[2] = r.y;
     if constexpr (PAIR) {
-      const double2 q = pp[k];  // p_{n-2}
-      o[3] = a.wx[k];
-      o[4] = a.wy[k];
+      const double2 q = ld_nt(&pp[k]);  // p_{n-2}
+      o[3] = __builtin_nontemporal_load(&a.wx[k]);
+      o[4] = __builtin_nontemporal_load(&a.wy[k]);
       o[5] = q.x;
       o[6] = q.y;
     }
@@ -410,14 +420,14 @@ __global__ void __launch_bounds__(PT) k_cg2_update(CgV<double> a, unsigned long 
       double px, py, ax, ay;
       apply_own(g, h, s_u, s_f, px, py, ax, ay);
       if constexpr (PAIR) {
-        a.wx[k] = (wx0 + alpha_prev * qx) + alpha * px;
-        a.wy[k] = (wy0 + alpha_prev * qy) + alpha * py;
+        __builtin_nontemporal_store((wx0 + alpha_prev * qx) + alpha * px, &a.wx[k]);
+        __builtin_nontemporal_store((wy0 + alpha_prev * qy) + alpha * py, &a.wy[k]);
       }
       const double rx = rx0 + na * ax, ry = ry0 + na * ay;
       double dgx, dgy;
       own_diag(g, h, s_c, dgx, dgy);
       const double zx = rx / dgx, zy = ry / dgy;
-      a.r2[k] = make_double2(rx, ry);
+      st_nt(&a.r2[k], make_double2(rx, ry));
       v[0] = rx * rx + ry * ry;
       v[1] = rx * zx + ry * zy;
     }
