@@ -92,6 +92,23 @@ __device__ __forceinline__ double div_q(double x, const Quot& d) {
   return x / d.y;
 }
 
+// ---- stores of streamed outputs ------------------------------------------------------------
+// A pass's outputs at its own cells, which no other block of the launch reads and the next launch
+// reads from HBM anyway (a 4096x2048 grid's arrays are 67 MB each), stored with the
+// nontemporal hint.
+template <typename T>
+__device__ __forceinline__ void store_nt(T* p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+__device__ __forceinline__ void store_nt(double2* p, double2 v) {
+  __builtin_nontemporal_store(v.x, &p->x);
+  __builtin_nontemporal_store(v.y, &p->y);
+}
+__device__ __forceinline__ void store_nt(float2* p, float2 v) {
+  __builtin_nontemporal_store(v.x, &p->x);
+  __builtin_nontemporal_store(v.y, &p->y);
+}
+
 // A scalar field at any padded cell, from its interior.
 __device__ __forceinline__ double ghosted(const double* a, const Geo& g, int i, int j) {
   return a[map_y(j, g.ny, g.periodic) * g.w + map_x(i, g.nx)];

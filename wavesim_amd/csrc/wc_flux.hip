@@ -457,9 +457,9 @@ __device__ __forceinline__ void flux32_body(const FluxTileArgs<float>& f, double
   flux_tile<float, R>(f, MODE_RATES, blk, nblk, rh, rhu, rhv, i, j, valid);
   if (!valid) return;
   const int k = j * f.g.w + i;
-  lh[k] = rh;
-  lhu[k] = rhu;
-  lhv[k] = rhv;
+  store_nt(&lh[k], rh);
+  store_nt(&lhu[k], rhu);
+  store_nt(&lhv[k], rhv);
 }
 
 __global__ void __launch_bounds__(NT) k_flux_tile32(FluxTileArgs<float> f, double* lh,
@@ -477,9 +477,9 @@ template <typename R>
 __device__ __forceinline__ void flux32_march(const FluxTileArgs<float>& f, double* lh,
                                              double* lhu, double* lhv, int rows, int blk) {
   flux_march<float, R>(f, rows, blk, [&](int, int, int k, double rh, double rhu, double rhv) {
-    lh[k] = rh;
-    lhu[k] = rhu;
-    lhv[k] = rhv;
+    store_nt(&lh[k], rh);
+    store_nt(&lhu[k], rhu);
+    store_nt(&lhv[k], rhv);
   });
 }
 

@@ -76,12 +76,6 @@ __device__ __forceinline__ bool start(const CgG& a, unsigned long long n, int bl
 }
 
 __device__ __forceinline__ float2 f2(double x, double y) { return make_float2((float)x, (float)y); }
-// A cell's new values, which no other block of the launch reads: stored with the nontemporal
-// hint (loading the cell's own inputs with it as well measured slower).
-__device__ __forceinline__ void st_nt(float2* p, float2 v) {
-  __builtin_nontemporal_store(v.x, &p->x);
-  __builtin_nontemporal_store(v.y, &p->y);
-}
 
 }  // namespace
 
@@ -372,22 +366,22 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
       const double sy = (double)(float)((double)qq.y + beta * (double)ss.y);
       const double px = (double)(float)((double)rr.x * (double)d.x + beta * (double)pp.x);
       const double py = (double)(float)((double)rr.y * (double)d.y + beta * (double)pp.y);
-      st_nt(&a.p[k], f2(px, py));
+      // the cell's new values stored with the nontemporal hint (loading its own p and w with
+      // the hint as well measured slower)
+      store_nt(&a.p[k], f2(px, py));
       if (!a.defer) {
-        __builtin_nontemporal_store((float)((double)ww.x + alpha * px), &a.wx[k]);
-        __builtin_nontemporal_store((float)((double)ww.y + alpha * py), &a.wy[k]);
+        store_nt(&a.wx[k], (float)((double)ww.x + alpha * px));
+        store_nt(&a.wy[k], (float)((double)ww.y + alpha * py));
       } else if (i & 1) {  // iterations i - 1 and i together (CgG::defer)
-        __builtin_nontemporal_store(
-            (float)((double)ww.x + (alpha_prev * (double)pp.x + alpha * px)), &a.wx[k]);
-        __builtin_nontemporal_store(
-            (float)((double)ww.y + (alpha_prev * (double)pp.y + alpha * py)), &a.wy[k]);
+        store_nt(&a.wx[k], (float)((double)ww.x + (alpha_prev * (double)pp.x + alpha * px)));
+        store_nt(&a.wy[k], (float)((double)ww.y + (alpha_prev * (double)pp.y + alpha * py)));
       }
-      st_nt(&a.s[cur][k], f2(sx, sy));
+      store_nt(&a.s[cur][k], f2(sx, sy));
       const double nrx = (double)(float)((double)rr.x + na * sx);
       const double nry = (double)(float)((double)rr.y + na * sy);
-      st_nt(&a.r[nxt][k], f2(nrx, nry));
+      store_nt(&a.r[nxt][k], f2(nrx, nry));
       const double qx = (double)(float)ax, qy = (double)(float)ay;
-      st_nt(&a.q[nxt][k], f2(qx, qy));
+      store_nt(&a.q[nxt][k], f2(qx, qy));
       v[0] = nrx * ux + nry * uy;
       v[1] = qx * ux + qy * uy;
       v[2] = nrx * nrx + nry * nry;

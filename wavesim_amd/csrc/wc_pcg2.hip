@@ -83,14 +83,10 @@ __device__ __forceinline__ bool start(const CgV<double>& a, unsigned long long n
   return !done;
 }
 
-// Streamed values of a cell that no other block of the launch reads: loaded and stored with the
-// nontemporal hint.
+// Streamed values of a cell that no other block of the launch reads, loaded with the nontemporal
+// hint (stored with it by store_nt).
 __device__ __forceinline__ double2 ld_nt(const double2* p) {
   return make_double2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
-}
-__device__ __forceinline__ void st_nt(double2* p, double2 v) {
-  __builtin_nontemporal_store(v.x, &p->x);
-  __builtin_nontemporal_store(v.y, &p->y);
 }
 
 // Ring element t of a tile + 2 halo: the mapped interior index m and flags (1: inside the
@@ -264,7 +260,7 @@ __global__ void __launch_bounds__(PT) k_cg2_init(CgV<double> a) {
       double dgx, dgy;
       own_diag(g, h, s_c, dgx, dgy);
       const double zx = rx / dgx, zy = ry / dgy;
-      a.r2[k] = make_double2(rx, ry);
+      store_nt(&a.r2[k], make_double2(rx, ry));
       v[0] = bx * bx + by * by;
       v[1] = rx * zx + ry * zy;
       v[2] = rx * rx + ry * ry;
@@ -330,7 +326,7 @@ __global__ void __launch_bounds__(PT) k_cg2_matvec(CgV<double> a, unsigned long 
     if (k >= 0) {
       double px, py, ax, ay;
       apply_own(g, h, s_u, s_f, px, py, ax, ay);
-      st_nt(&pn[k], make_double2(px, py));
+      store_nt(&pn[k], make_double2(px, py));
       v = px * ax + py * ay;
     }
     acc[0] = acc[0] + v;
@@ -420,14 +416,14 @@ __global__ void __launch_bounds__(PT) k_cg2_update(CgV<double> a, unsigned long 
       double px, py, ax, ay;
       apply_own(g, h, s_u, s_f, px, py, ax, ay);
       if constexpr (PAIR) {
-        __builtin_nontemporal_store((wx0 + alpha_prev * qx) + alpha * px, &a.wx[k]);
-        __builtin_nontemporal_store((wy0 + alpha_prev * qy) + alpha * py, &a.wy[k]);
+        store_nt(&a.wx[k], (wx0 + alpha_prev * qx) + alpha * px);
+        store_nt(&a.wy[k], (wy0 + alpha_prev * qy) + alpha * py);
       }
       const double rx = rx0 + na * ax, ry = ry0 + na * ay;
       double dgx, dgy;
       own_diag(g, h, s_c, dgx, dgy);
       const double zx = rx / dgx, zy = ry / dgy;
-      st_nt(&a.r2[k], make_double2(rx, ry));
+      store_nt(&a.r2[k], make_double2(rx, ry));
       v[0] = rx * rx + ry * ry;
       v[1] = rx * zx + ry * zy;
     }

@@ -53,10 +53,10 @@ __device__ __forceinline__ void damp(const StepArgs& a, double dt, int i, int j,
 //   MODE_RATES: store L (the dispersive correction still has to be added)
 __device__ __forceinline__ void stage_out(const StepArgs& a, int mode, int i, int j, int k,
                                           double rh, double rhu, double rhv) {
-  if (mode == MODE_RATES) {
-    a.lh[k] = rh;
-    a.lhu[k] = rhu;
-    a.lhv[k] = rhv;
+  if (mode == MODE_RATES) {  // the rates, read once by the SGN set-up: stored nontemporal
+    store_nt(&a.lh[k], rh);
+    store_nt(&a.lhu[k], rhu);
+    store_nt(&a.lhv[k], rhv);
     return;
   }
   double h = a.in_h[k], hu = a.in_hu[k], hv = a.in_hv[k];
