@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(PT) k_cg1_start(CgG a) {
 // Iteration i = n - 1 (see the top of the file).
 // AHEAD tiles' loads in flight: 2 alone; 1 in the batched kernel, whose argument block and
 // instance walk leave no registers for a second set.
-template <int L, int AHEAD>
+template <int L, int AHEAD, bool NTS>
 __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n, int blk) {
   const Geo& g = a.g;
   const int nt = blk < a.stride ? tiles_all(g) : 0;  // blocks past the stride: zero lanes
@@ -366,22 +366,24 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
       const double sy = (double)(float)((double)qq.y + beta * (double)ss.y);
       const double px = (double)(float)((double)rr.x * (double)d.x + beta * (double)pp.x);
       const double py = (double)(float)((double)rr.y * (double)d.y + beta * (double)pp.y);
-      // the cell's new values stored with the nontemporal hint (loading its own p and w with
-      // the hint as well measured slower)
-      store_nt(&a.p[k], f2(px, py));
+      // the cell's new values: stored with the nontemporal hint in the batched kernel, plainly
+      // alone (where the hint measured slower)
+      store_as<NTS>(&a.p[k], f2(px, py));
       if (!a.defer) {
-        store_nt(&a.wx[k], (float)((double)ww.x + alpha * px));
-        store_nt(&a.wy[k], (float)((double)ww.y + alpha * py));
+        store_as<NTS>(&a.wx[k], (float)((double)ww.x + alpha * px));
+        store_as<NTS>(&a.wy[k], (float)((double)ww.y + alpha * py));
       } else if (i & 1) {  // iterations i - 1 and i together (CgG::defer)
-        store_nt(&a.wx[k], (float)((double)ww.x + (alpha_prev * (double)pp.x + alpha * px)));
-        store_nt(&a.wy[k], (float)((double)ww.y + (alpha_prev * (double)pp.y + alpha * py)));
+        store_as<NTS>(&a.wx[k],
+                      (float)((double)ww.x + (alpha_prev * (double)pp.x + alpha * px)));
+        store_as<NTS>(&a.wy[k],
+                      (float)((double)ww.y + (alpha_prev * (double)pp.y + alpha * py)));
       }
-      store_nt(&a.s[cur][k], f2(sx, sy));
+      store_as<NTS>(&a.s[cur][k], f2(sx, sy));
       const double nrx = (double)(float)((double)rr.x + na * sx);
       const double nry = (double)(float)((double)rr.y + na * sy);
-      store_nt(&a.r[nxt][k], f2(nrx, nry));
+      store_as<NTS>(&a.r[nxt][k], f2(nrx, nry));
       const double qx = (double)(float)ax, qy = (double)(float)ay;
-      store_nt(&a.q[nxt][k], f2(qx, qy));
+      store_as<NTS>(&a.q[nxt][k], f2(qx, qy));
       v[0] = nrx * ux + nry * uy;
       v[1] = qx * ux + qy * uy;
       v[2] = nrx * nrx + nry * nry;
@@ -405,7 +407,7 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
 
 template <int L>
 __global__ void __launch_bounds__(PT) k_cg1_iter(CgG a, unsigned long long n) {
-  cg1_iter_body<L, 2>(a, n, lane_of_block(a.g, a.stride));
+  cg1_iter_body<L, 2, false>(a, n, lane_of_block(a.g, a.stride));
 }
 
 // K_n's prologue alone, one wave: whether the solve is over before iteration n - 1.
@@ -597,7 +599,7 @@ template <int L>
 __global__ void __launch_bounds__(PT) kb_cg1_iter(const CgG* __restrict__ args, unsigned long long n, BMap m) {
   for (int vb = blockIdx.x; vb < m.total; vb += gridDim.x) {
     const int q = batch_instance(m, vb);
-    cg1_iter_body<L, 1>(args[q], n, vb - m.first[q]);
+    cg1_iter_body<L, 1, true>(args[q], n, vb - m.first[q]);
     __syncthreads();
   }
 }
