@@ -108,15 +108,6 @@ __device__ __forceinline__ void store_nt(float2* p, float2 v) {
   __builtin_nontemporal_store(v.x, &p->x);
   __builtin_nontemporal_store(v.y, &p->y);
 }
-// store_nt where NT, a plain store otherwise (for kernels whose instances measured differently).
-template <bool NT, typename T>
-__device__ __forceinline__ void store_as(T* p, T v) {
-  if constexpr (NT) {
-    store_nt(p, v);
-  } else {
-    *p = v;
-  }
-}
 
 // A scalar field at any padded cell, from its interior.
 __device__ __forceinline__ double ghosted(const double* a, const Geo& g, int i, int j) {

@@ -368,22 +368,40 @@ __device__ __forceinline__ void cg1_iter_body(const CgG& a, unsigned long long n
       const double py = (double)(float)((double)rr.y * (double)d.y + beta * (double)pp.y);
       // the cell's new values: stored with the nontemporal hint in the batched kernel, plainly
       // alone (where the hint measured slower)
-      store_as<NTS>(&a.p[k], f2(px, py));
-      if (!a.defer) {
-        store_as<NTS>(&a.wx[k], (float)((double)ww.x + alpha * px));
-        store_as<NTS>(&a.wy[k], (float)((double)ww.y + alpha * py));
-      } else if (i & 1) {  // iterations i - 1 and i together (CgG::defer)
-        store_as<NTS>(&a.wx[k],
-                      (float)((double)ww.x + (alpha_prev * (double)pp.x + alpha * px)));
-        store_as<NTS>(&a.wy[k],
-                      (float)((double)ww.y + (alpha_prev * (double)pp.y + alpha * py)));
+      double nrx, nry, qx, qy;
+      if constexpr (NTS) {
+        store_nt(&a.p[k], f2(px, py));
+        if (!a.defer) {
+          store_nt(&a.wx[k], (float)((double)ww.x + alpha * px));
+          store_nt(&a.wy[k], (float)((double)ww.y + alpha * py));
+        } else if (i & 1) {  // iterations i - 1 and i together (CgG::defer)
+          store_nt(&a.wx[k], (float)((double)ww.x + (alpha_prev * (double)pp.x + alpha * px)));
+          store_nt(&a.wy[k], (float)((double)ww.y + (alpha_prev * (double)pp.y + alpha * py)));
+        }
+        store_nt(&a.s[cur][k], f2(sx, sy));
+        nrx = (double)(float)((double)rr.x + na * sx);
+        nry = (double)(float)((double)rr.y + na * sy);
+        store_nt(&a.r[nxt][k], f2(nrx, nry));
+        qx = (double)(float)ax;
+        qy = (double)(float)ay;
+        store_nt(&a.q[nxt][k], f2(qx, qy));
+      } else {
+        a.p[k] = f2(px, py);
+        if (!a.defer) {
+          a.wx[k] = (float)((double)ww.x + alpha * px);
+          a.wy[k] = (float)((double)ww.y + alpha * py);
+        } else if (i & 1) {
+          a.wx[k] = (float)((double)ww.x + (alpha_prev * (double)pp.x + alpha * px));
+          a.wy[k] = (float)((double)ww.y + (alpha_prev * (double)pp.y + alpha * py));
+        }
+        a.s[cur][k] = f2(sx, sy);
+        nrx = (double)(float)((double)rr.x + na * sx);
+        nry = (double)(float)((double)rr.y + na * sy);
+        a.r[nxt][k] = f2(nrx, nry);
+        qx = (double)(float)ax;
+        qy = (double)(float)ay;
+        a.q[nxt][k] = f2(qx, qy);
       }
-      store_as<NTS>(&a.s[cur][k], f2(sx, sy));
-      const double nrx = (double)(float)((double)rr.x + na * sx);
-      const double nry = (double)(float)((double)rr.y + na * sy);
-      store_as<NTS>(&a.r[nxt][k], f2(nrx, nry));
-      const double qx = (double)(float)ax, qy = (double)(float)ay;
-      store_as<NTS>(&a.q[nxt][k], f2(qx, qy));
       v[0] = nrx * ux + nry * uy;
       v[1] = qx * ux + qy * uy;
       v[2] = nrx * nrx + nry * nry;
