@@ -12,7 +12,9 @@
 // Recomputing A p in U_n saves its store and its load (32 of 208 bytes per cell and
 // iteration), and M_n's z from r and the stored diagonal replaces U_n's store of z by a load;
 // the halo work runs from LDS and L2. Per cell and iteration the passes move r, p_old, diag
-// (read), p (written), c, h in M_n and p, c, h, r, w (read), w, r (written) in U_n: 176 bytes.
+// (read), p (written), c, h in M_n and p, c, h, r (read), r (written) in U_n, and every other
+// iteration w and p_(n-2) (read) and w (written), which U_n updates two steps at a time: 168
+// bytes on average.
 //
 // r, p and the diagonal are stored as interleaved (x, y) pairs (double2): a halo point of each
 // is one 16-byte load, and a halo row's partial 128-byte lines are shared by both components.
