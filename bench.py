@@ -154,6 +154,9 @@ def parse():
                    help="PCG kernels of the FP64 path (3: 64x4 tiles; default 64x16 tiles)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run a device copy of known size (to scale HBM counters) and exit")
+    p.add_argument("--stream-ceilings", action="store_true",
+                   help="only measure what plain HBM streams reach (copy, read, write, three "
+                        "reads and a write; plain and nontemporal; 1 GiB arrays) and exit")
     return p.parse_args()
 
 
@@ -470,6 +473,14 @@ def main():
     torch.cuda.init()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.stream_ceilings:
+        from wavesim_amd.solver import stream_ceilings
+
+        for grid in (2048, 4096, 8192, 16384):
+            print(json.dumps({"stream_tb_per_s": stream_ceilings(1 << 30, grid, 10),
+                              "bytes_per_array": 1 << 30, "grid": grid, "threads": 256,
+                              "reps": 10}))
         return
     if args.workload == "sweep":
         out = sweep(args, args.spinup, args.warmup, args.steps,

@@ -383,6 +383,13 @@ int wc_batch_profile_read(wc_batch* b, double* total_ms, uint64_t* launches);
  * values. The two must agree bit for bit. No counterpart in the reference. */
 int wc_check_shared_division(double y, const double* x, uint64_t n, double* shared,
                              double* plain);
+/* No counterpart in the reference: what plain streams reach on this GPU's HBM, the yardstick
+ * for the stencil kernels' bytes. Four device arrays of `bytes` each (a multiple of 16, at most
+ * 16 GiB) are allocated for the call; each test runs once untimed and `reps` times timed with
+ * `grid` blocks of 256 threads looping over the arrays 16 B a thread a step. tbs[8] receives
+ * TB/s moved: copy, copy with nontemporal loads and stores, read, read nontemporal, write,
+ * write nontemporal, three reads and a write, the same nontemporal. */
+int wc_stream_ceilings(uint64_t bytes, int32_t grid, int32_t reps, double* tbs);
 
 /* Measurement hooks for benchmarks: device time in milliseconds of each kernel family over the
  * steps since the last reset, measured with HIP events on the solver's stream, and the kernel

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libwavecore_hip.so")
 SOURCES = ["wc_api.hip", "wc_stats_host.hip", "wc_nested_host.hip", "wc_batch_host.hip", "wc_kernels.hip", "wc_flux.hip", "wc_fp32.hip", "wc_pcg.hip",
            "wc_pcg1.hip", "wc_pcg2.hip", "wc_relax.hip", "wc_unet.hip", "wc_fpcg.hip",
-           "wc_stats.hip", "wc_run.hip", "wc_eddy.hip", "wc_noalloc.hip"]
+           "wc_stats.hip", "wc_run.hip", "wc_eddy.hip", "wc_noalloc.hip", "wc_stream.hip"]
 HEADERS = ["wc_host.h", "wc_device.h", "wc_json.h", "wc_kernels.h", "wc_math.h", "wc_run.h", "wc_stage.h",
            "wc_tile16.h"]
 

@@ -23,7 +23,6 @@ namespace {
 
 constexpr int SX2f = TX + 4, SY2f = TY + 4;
 constexpr int SX1f = TX + 2, SY1f = TY + 2;
-constexpr int SX3f = TX + 6, SY3f = TY + 6;
 
 __device__ __forceinline__ void split(double v, float& hi, float& lo) {
   hi = (float)v;

@@ -149,7 +149,8 @@ __global__ void __launch_bounds__(NT) k_finish_euler(StepArgs a) {
 // Setup: u, v, the mask, the masked velocities and their divergence, c = phi h^3/3 and c N,
 // the Jacobi diagonal, the right-hand side b, and the warm start zeroed where masked. The tile
 // is staged in LDS with three ghost rings: um, vm on the tile + 3, div on + 2, c and f on + 1.
-constexpr int SX3 = TX + 6, SY3 = TY + 6;  // tile + 3
+// (The 64x4 tile's rings, for the matrix product's kernels below; the set-up's own tiles are
+// 64 x SETUP_ROWS.)
 constexpr int SX2 = TX + 4, SY2 = TY + 4;  // tile + 2
 constexpr int SX1 = TX + 2, SY1 = TY + 2;  // tile + 1
 

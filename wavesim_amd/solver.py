@@ -88,6 +88,7 @@ SIGNATURES = {
     "wc_manning_factor": (_D, [_D, _D, _D, _D]),
     "wc_warm_start_weights": (_I32, [_I32, _I32, _P]),
     "wc_check_shared_division": (_I32, [_D, _P, _U64, _P, _P]),
+    "wc_stream_ceilings": (_I32, [_U64, _I32, _I32, _P]),
     "wc_run_steps": (_I32, [_VP, _U64, ctypes.POINTER(_U64), _P]),
     "wc_solver_set_preconditioner": (_I32, [_VP, _I32, ctypes.POINTER(ctypes.c_float), _U64]),
     "wc_solver_precond_fallbacks": (_I32, [_VP, ctypes.POINTER(_U64)]),
@@ -727,6 +728,17 @@ def check_shared_division(y: float, x: np.ndarray):
     shared, plain = np.empty_like(x), np.empty_like(x)
     _check(lib().wc_check_shared_division(float(y), _ptr(x), x.size, _ptr(shared), _ptr(plain)))
     return shared, plain
+
+
+STREAM_TESTS = ("copy", "copy_nt", "read", "read_nt", "write", "write_nt", "triad", "triad_nt")
+
+
+def stream_ceilings(bytes_per_array: int = 1 << 30, grid: int = 2048, reps: int = 10) -> dict:
+    """What plain streams reach on this GPU's HBM (wc_stream_ceilings): TB/s moved by a copy,
+    reads alone, writes alone and three reads with a write, plain and nontemporal."""
+    out = np.zeros(8)
+    _check(lib().wc_stream_ceilings(int(bytes_per_array), int(grid), int(reps), _ptr(out)))
+    return dict(zip(STREAM_TESTS, out.tolist()))
 
 
 def warm_start_weights(order: int, window: int = 0) -> np.ndarray:
